@@ -1,0 +1,158 @@
+/*
+ * lss.h - C ABI of the MI355X Kilosort4 online template-matching library (liblss.so).
+ *
+ * Replaces the per-batch operator of LiveSpikeSorter's SpikeSorter module:
+ *   OnlineSpikesV2(InputParameters, sockaddr_in, DataSocket*)      src/SpikeSorter/OnlineSpikesV2.h:32
+ *   long kilosortMatchingPursuit(float* d_batch, long W)          src/SpikeSorter/OnlineSpikesV2.h:56
+ *   void computeClosestClusters(long W, long nsp)                 src/SpikeSorter/OnlineSpikesV2.h:57
+ *   void highpassFilter(float*, int C, int W, float fs, float f)  src/SpikeSorter/OnlineSpikesV2.h:58
+ *   meanRemove / medianRemove / projectToPCA / crossCorrelation /
+ *   normalizeConv / reduceToTimeDimByMax / findMatchingIndices /
+ *   updateResidual                                                 src/SpikeSorter/myGPUhelpers.h:20-49
+ * with the semantics of the Kilosort4 Python the module reimplements
+ * (kilosort/io.py:584-637, 757-790; kilosort/template_matching.py:20-26, 50-51, 110-121, 128-202;
+ * kilosort/postprocessing.py:33-46).
+ *
+ * Conventions: plain pointers and sizes, no exceptions and no exit() across the ABI.  Every call
+ * returns an lss_status; lss_last_error() gives the message of the last failure on the calling
+ * thread.  A handle owns all device memory it uses, sized once at creation for max_window; it is
+ * not thread-safe and issues all work on its own HIP stream (lss_set_stream to share one).
+ * Device pointers passed to stage entry points must be device memory of the handle's GPU.
+ */
+#ifndef LSS_H_
+#define LSS_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LSS_ABI_VERSION 1
+
+typedef enum lss_status {
+    LSS_OK = 0,
+    LSS_ERR_ARG = 1,        /* bad argument / shape */
+    LSS_ERR_HIP = 2,        /* HIP runtime error */
+    LSS_ERR_IO = 3,         /* artifact directory / .npy problem */
+    LSS_ERR_CAPACITY = 4,   /* more spikes than the output capacity */
+    LSS_ERR_STATE = 5       /* handle not usable */
+} lss_status;
+
+/* Arithmetic of the two large contractions (whitening and U.Xproj). */
+typedef enum lss_precision {
+    LSS_PREC_FP32 = 0,      /* FP32 MFMA, FP32 accumulation */
+    LSS_PREC_BF16 = 1,      /* BF16 MFMA, FP32 accumulation */
+    LSS_PREC_FP16 = 2,      /* FP16 MFMA, FP32 accumulation */
+    LSS_PREC_BF16X3 = 3     /* split-BF16 (hi*hi + hi*lo + lo*hi) on BF16 MFMA */
+} lss_precision;
+
+/* Sorter configuration (OnlineSpikesV2 members T, K, M, C, nt0min, numNearestChans, Th_learned,
+ * dt: src/SpikeSorter/OnlineSpikesV2.h:84-99; misc.txt keys OnlineSpikesV2.cu:435-457). */
+typedef struct lss_params {
+    int32_t n_chan;        /* C */
+    int32_t n_templates;   /* T */
+    int32_t n_pcs;         /* K (<= 8) */
+    int32_t nt;            /* template length M (odd, <= 127) */
+    int32_t nt0min;        /* trough sample of the template window */
+    int32_t n_nearest;     /* nC channels of the PC features (<= 32) */
+    int32_t max_window;    /* largest padded window W' (samples) a call may pass */
+    int32_t max_passes;    /* pursuit pass cap: 50 (OnlineSpikesV2.cu:903) or 100 (template_matching.py:157) */
+    int32_t max_spikes;    /* spikes one window may produce (device list capacity) */
+    int32_t do_car;        /* median (common average) reference, kilosort/io.py:766-768 */
+    int32_t precision;     /* lss_precision */
+    int32_t drop_negative; /* drop spikes whose sample < 0 (template_matching.py:53-59) */
+    float th_learned;      /* Th: a peak needs relu(B)^2/nm > Th^2 */
+    float fs;              /* sampling rate (Hz) */
+} lss_params;
+
+/* Kilosort4 outputs as host float32 row-major arrays (the oss_input files,
+ * run_online_spikes.py:545-576). */
+typedef struct lss_artifacts {
+    const float* U;          /* [T][K][C]  Wall3.npy */
+    const float* wPCA;       /* [K][nt]    wPCA.npy */
+    const float* Wrot;       /* [C][C]     whiteningMat.npy */
+    const float* drift;      /* [C][C]     drift matrix M applied as (M @ Wrot), or NULL for I */
+    const float* hp_filter;  /* [hp_len]   hp_filter.npy (filtfilt impulse response) */
+    int32_t hp_len;
+    const float* xc;         /* [C] channel x (um) */
+    const float* yc;         /* [C] channel y (um) */
+} lss_artifacts;
+
+/* Caller-owned host output buffers of one window. */
+typedef struct lss_spikes {
+    int64_t capacity;        /* rows available in every non-NULL array */
+    int64_t n;               /* out: spikes written */
+    int32_t passes;          /* out: pursuit passes that found spikes */
+    int64_t* sample;         /* [capacity] recording sample of the trough */
+    int32_t* template_id;    /* [capacity] */
+    float* amplitude;        /* [capacity] B[t,w]/nm[t] */
+    float* features;         /* [capacity][nC][K] PC features, or NULL */
+    float* x;                /* [capacity] position (um), or NULL */
+    float* y;                /* [capacity] */
+    int32_t* column;         /* [capacity] padded-window column of the peak, or NULL */
+} lss_spikes;
+
+const char* lss_last_error(void);
+int32_t lss_abi_version(void);
+
+/* Fills defaults: nt=61, nt0min=20, K=6, nC=10, Th=8, fs=30000, 50 passes, FP32, CAR on,
+ * drop_negative on, max_window=60122. */
+lss_status lss_default_params(lss_params* p);
+
+/* Creates a sorter on the current HIP device: uploads the artifacts and builds the FP64-derived
+ * tables (template norms, template waveforms, ctc) on the device
+ * (replaces OnlineSpikesV2::initializeSorter / initializeStaticMemory, OnlineSpikesV2.cu:321-630). */
+lss_status lss_create(const lss_params* p, const lss_artifacts* a, void** handle);
+
+/* Same, reading an oss_input directory (Wall3.npy, wPCA.npy, whiteningMat.npy or
+ * whitening_mat.npy, drift_matrix.npy, hp_filter.npy, xc.npy, yc.npy, misc.txt);
+ * p supplies everything misc.txt and the array shapes do not. */
+lss_status lss_create_from_dir(const char* dir, const lss_params* p, void** handle);
+
+lss_status lss_destroy(void* handle);
+lss_status lss_get_params(void* handle, lss_params* p);
+lss_status lss_set_stream(void* handle, void* hip_stream);
+
+/* One padded window int16 [W][C] (interleaved, as fetched) through the whole path:
+ * CAR, highpass, whitening, projection, pursuit, features, positions.  window_start is the
+ * recording sample of column 0 (ibatch*NT - nt for Kilosort batching), so a spike's sample is
+ * window_start + column - nt/2 + nt0min (template_matching.py:68).  window may be host or device
+ * memory (on_device).  Replaces kilosortMatchingPursuit + computeClosestClusters. */
+lss_status lss_sort_window(void* handle, const int16_t* window, int64_t W, int32_t on_device,
+                           int64_t window_start, lss_spikes* out);
+
+/* Kilosort batch padding of a host int16 recording [n_samples][C] (kilosort/io.py:584-637):
+ * writes the padded window [NT+2nt][C] and its window_start.  Host only. */
+lss_status lss_pad_batch(const int16_t* rec, int64_t n_samples, int32_t C, int32_t NT, int32_t nt,
+                         int64_t ibatch, int16_t* out, int64_t* window_start);
+
+/* ---- stage entry points (device pointers, handle's stream, synchronous on return) ---------- */
+
+/* int16 window [W][C] (device) -> X after mean, median, highpass: [C][W] (before whitening). */
+lss_status lss_stage_filter(void* handle, const int16_t* d_window, int64_t W, float* d_out_cw);
+/* int16 window [W][C] (device) -> after mean and median removal only: [C][W]. */
+lss_status lss_stage_car(void* handle, const int16_t* d_window, int64_t W, float* d_out_cw);
+/* [C][W] -> highpassed [C][W] (in and out may not alias). */
+lss_status lss_stage_highpass(void* handle, const float* d_in_cw, int64_t W, float* d_out_cw);
+/* [C][W] -> whitened (M @ Wrot) X, time-major [W][C]. */
+lss_status lss_stage_whiten(void* handle, const float* d_in_cw, int64_t W, float* d_out_wc);
+/* whitened X [W][C] -> scores B [W][T] = U . (wPCA-projected X). */
+lss_status lss_stage_scores(void* handle, const float* d_x_wc, int64_t W, float* d_b_wt);
+/* Pursuit on given scores B [W][T] and residual X [W][C] (both updated in place), then features
+ * and positions from the final residual; spikes in pursuit order (pass, then column). */
+lss_status lss_stage_pursuit(void* handle, float* d_b_wt, float* d_x_wc, int64_t W,
+                             int64_t window_start, lss_spikes* out);
+
+/* Copies the device tables to host: nm [T], waves [T][nt][C], ctc [T][2nt+1][T]
+ * (ctc[j][l][i] = Kilosort ctc[i][j][l]), wmat [C][C] = M @ Wrot; any may be NULL. */
+lss_status lss_get_tables(void* handle, float* nm, float* waves, float* ctc, float* wmat);
+
+/* Times of the last lss_sort_window's stages in ms (HIP events): [0] CAR, [1] highpass,
+ * [2] whitening, [3] projection+scores, [4] pursuit, [5] features, [6] total; n <= 8. */
+lss_status lss_last_timings(void* handle, float* ms, int32_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,91 @@
+"""ctypes binding of liblss.so (declarations in include/lss.h).
+
+The library is loaded from this package directory; there is no fallback: if it is missing or
+fails to load, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "liblss.so"
+
+LSS_OK = 0
+STATUS = {0: "LSS_OK", 1: "LSS_ERR_ARG", 2: "LSS_ERR_HIP", 3: "LSS_ERR_IO", 4: "LSS_ERR_CAPACITY",
+          5: "LSS_ERR_STATE"}
+PRECISION = {"fp32": 0, "bf16": 1, "fp16": 2, "bf16x3": 3}
+
+EXPORTS = ["lss_last_error", "lss_abi_version", "lss_default_params", "lss_create",
+           "lss_create_from_dir", "lss_destroy", "lss_get_params", "lss_set_stream",
+           "lss_sort_window", "lss_pad_batch", "lss_stage_filter", "lss_stage_car",
+           "lss_stage_highpass", "lss_stage_whiten", "lss_stage_scores", "lss_stage_pursuit",
+           "lss_get_tables", "lss_last_timings"]
+
+
+class Params(ct.Structure):
+    _fields_ = [("n_chan", ct.c_int32), ("n_templates", ct.c_int32), ("n_pcs", ct.c_int32),
+                ("nt", ct.c_int32), ("nt0min", ct.c_int32), ("n_nearest", ct.c_int32),
+                ("max_window", ct.c_int32), ("max_passes", ct.c_int32), ("max_spikes", ct.c_int32),
+                ("do_car", ct.c_int32), ("precision", ct.c_int32), ("drop_negative", ct.c_int32),
+                ("th_learned", ct.c_float), ("fs", ct.c_float)]
+
+
+class Artifacts(ct.Structure):
+    _fields_ = [("U", ct.c_void_p), ("wPCA", ct.c_void_p), ("Wrot", ct.c_void_p),
+                ("drift", ct.c_void_p), ("hp_filter", ct.c_void_p), ("hp_len", ct.c_int32),
+                ("xc", ct.c_void_p), ("yc", ct.c_void_p)]
+
+
+class Spikes(ct.Structure):
+    _fields_ = [("capacity", ct.c_int64), ("n", ct.c_int64), ("passes", ct.c_int32),
+                ("sample", ct.c_void_p), ("template_id", ct.c_void_p), ("amplitude", ct.c_void_p),
+                ("features", ct.c_void_p), ("x", ct.c_void_p), ("y", ct.c_void_p),
+                ("column", ct.c_void_p)]
+
+
+class LssError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib() -> ct.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise LssError(f"{LIB_PATH} is not built: run `python -m livespikesorter_amd.build`")
+        L = ct.CDLL(str(LIB_PATH))
+        vp, i32, i64 = ct.c_void_p, ct.c_int32, ct.c_int64
+        sig = {
+            "lss_last_error": ([], ct.c_char_p),
+            "lss_abi_version": ([], i32),
+            "lss_default_params": ([ct.POINTER(Params)], i32),
+            "lss_create": ([ct.POINTER(Params), ct.POINTER(Artifacts), ct.POINTER(vp)], i32),
+            "lss_create_from_dir": ([ct.c_char_p, ct.POINTER(Params), ct.POINTER(vp)], i32),
+            "lss_destroy": ([vp], i32),
+            "lss_get_params": ([vp, ct.POINTER(Params)], i32),
+            "lss_set_stream": ([vp, vp], i32),
+            "lss_sort_window": ([vp, vp, i64, i32, i64, ct.POINTER(Spikes)], i32),
+            "lss_pad_batch": ([vp, i64, i32, i32, i32, i64, vp, ct.POINTER(i64)], i32),
+            "lss_stage_filter": ([vp, vp, i64, vp], i32),
+            "lss_stage_car": ([vp, vp, i64, vp], i32),
+            "lss_stage_highpass": ([vp, vp, i64, vp], i32),
+            "lss_stage_whiten": ([vp, vp, i64, vp], i32),
+            "lss_stage_scores": ([vp, vp, i64, vp], i32),
+            "lss_stage_pursuit": ([vp, vp, vp, i64, i64, ct.POINTER(Spikes)], i32),
+            "lss_get_tables": ([vp, vp, vp, vp, vp], i32),
+            "lss_last_timings": ([vp, vp, i32], i32),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def check(status: int, what: str = "") -> None:
+    if status != LSS_OK:
+        msg = lib().lss_last_error().decode(errors="replace")
+        raise LssError(f"{what}: {STATUS.get(status, status)}: {msg}")

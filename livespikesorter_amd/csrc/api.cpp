@@ -1,0 +1,706 @@
+// liblss C ABI: sorter handle, artifact upload, per-window pipeline.
+//
+// The handle plays the role of OnlineSpikesV2 (src/SpikeSorter/OnlineSpikesV2.cu:247-630: load
+// artifacts, allocate every buffer once for the largest window) and of its per-batch operator
+// kilosortMatchingPursuit + computeClosestClusters (OnlineSpikesV2.cu:823-971), with Kilosort4
+// semantics where the two differ.  All device work runs on the handle's stream; the pursuit reads
+// one 4-byte counter back per pass to decide whether to continue.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/lss.h"
+#include "lss_kernels.h"
+#include "npy.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+lss_status fail(lss_status st, const std::string& msg) {
+    g_err = msg;
+    return st;
+}
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(LSS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    hipError_t alloc(size_t count) {
+        n = count;
+        return hipMalloc((void**)&p, sizeof(T) * std::max<size_t>(count, 1));
+    }
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+struct Handle {
+    lss_params prm{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int C = 0, T = 0, K = 0, nt = 0, nC = 0;
+    int64_t Wmax = 0, ldw = 0;
+    int ldq = 0, ldm = 0;   // row strides of U/Xp (K*C) and of the whitening matrix (C), multiples of 4
+
+    // tables
+    DevBuf<float> U, wpca, wmat, nm, waves, ctc, Ucc, xc, yc;
+    DevBuf<int> iCC, iU;
+    DevBuf<float2> spec, tw;
+    std::vector<float> hp_taps;
+    int64_t spec_W = -1;
+    int spec_lo = 0, spec_hi = 0;
+    std::vector<float> h_xc, h_yc;
+
+    // window buffers
+    DevBuf<int16_t> win;
+    DevBuf<long long> sums;
+    DevBuf<float> xcw, xhp, xw, xp, B, cfmax;
+    DevBuf<int> imax, blockcnt;
+    DevBuf<int> sp_col, sp_tmp, counters;
+    DevBuf<float> sp_amp, tF, xs, ys;
+    int* h_counters = nullptr;   // pinned
+
+    hipEvent_t ev[8] = {};
+    float last_ms[8] = {};
+    bool alive = false;
+
+    ~Handle() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (h_counters) (void)hipHostFree(h_counters);
+        if (own_stream && stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+lss_status check_params(const lss_params& p) {
+    if (p.n_chan < 1 || p.n_chan > 1024) return fail(LSS_ERR_ARG, "n_chan must be in [1, 1024]");
+    if (p.n_templates < 1) return fail(LSS_ERR_ARG, "n_templates must be >= 1");
+    if (p.n_pcs < 1 || p.n_pcs > 8) return fail(LSS_ERR_ARG, "n_pcs must be in [1, 8]");
+    if (p.nt < 3 || p.nt > 127 || (p.nt % 2) == 0) return fail(LSS_ERR_ARG, "nt must be odd in [3, 127]");
+    if (p.nt0min < 0 || p.nt0min >= p.nt) return fail(LSS_ERR_ARG, "nt0min must be in [0, nt)");
+    if (p.n_nearest < 1 || p.n_nearest > p.n_chan || p.n_nearest * p.n_pcs > 256)
+        return fail(LSS_ERR_ARG, "n_nearest must be in [1, n_chan] with n_nearest*n_pcs <= 256");
+    if (p.max_window < 2 * p.nt + 2) return fail(LSS_ERR_ARG, "max_window too small");
+    if (p.max_passes < 1) return fail(LSS_ERR_ARG, "max_passes must be >= 1");
+    if (p.max_spikes < 1) return fail(LSS_ERR_ARG, "max_spikes must be >= 1");
+    if (p.precision < LSS_PREC_FP32 || p.precision > LSS_PREC_BF16X3) return fail(LSS_ERR_ARG, "bad precision");
+    return LSS_OK;
+}
+
+// Stable argsort of squared distances (prepare_extract, template_matching.py:20-26; ties by index)
+void nearest_channels(const std::vector<float>& xc, const std::vector<float>& yc, int nC, std::vector<int>& iCC) {
+    const int C = (int)xc.size();
+    iCC.assign((size_t)nC * C, 0);
+    std::vector<int> idx(C);
+    std::vector<double> ds(C);
+    for (int c = 0; c < C; ++c) {
+        for (int j = 0; j < C; ++j) {
+            // ds[j] in the reference's float64 arithmetic: (xc - xc[c])^2 + (yc - yc[c])^2
+            double dx = (double)xc[j] - (double)xc[c], dy = (double)yc[j] - (double)yc[c];
+            ds[j] = dx * dx + dy * dy;
+        }
+        std::iota(idx.begin(), idx.end(), 0);
+        std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return ds[a] < ds[b]; });
+        for (int j = 0; j < nC; ++j) iCC[(size_t)j * C + c] = idx[j];
+    }
+}
+
+// Spectrum of the overlap-save kernel for window length W (fft_highpass pad/crop rule,
+// kilosort/preprocessing.py:138-154, then fftshift): g[j] = f[(W/2 + j) mod W].
+lss_status prepare_spectrum(Handle& h, int64_t W) {
+    if (h.spec_W == W) return LSS_OK;
+    const int64_t ft = (int64_t)h.hp_taps.size();
+    auto f_at = [&](int64_t m) -> double {   // padded/cropped filter value at index m in [0, W)
+        if (ft < W) {
+            int64_t pad = (W - ft) / 2;
+            int64_t q = m - pad;
+            return (q >= 0 && q < ft) ? (double)h.hp_taps[q] : 0.0;
+        } else if (ft > W) {
+            int64_t crop = (ft - W) / 2;
+            return (double)h.hp_taps[crop + m];
+        }
+        return (double)h.hp_taps[m];
+    };
+    const int Llo = (int)std::min<int64_t>(lss::kHpHalf, W / 2);
+    const int Lhi = (int)std::min<int64_t>(lss::kHpHalf, W - 1 - W / 2);
+    const int N = lss::kFftN;
+    std::vector<double> hr(N, 0.0);
+    for (int j = -Llo; j <= Lhi; ++j) {
+        int64_t m = ((W / 2 + j) % W + W) % W;
+        // circular convolution kernel h[p] = g[-p]
+        hr[((-j) % N + N) % N] += f_at(m);
+    }
+    // forward DFT of hr (iterative radix-2, FP64), scaled by 1/N
+    std::vector<double> re(hr), im(N, 0.0);
+    for (int i = 1, j = 0; i < N; ++i) {
+        int bit = N >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) { std::swap(re[i], re[j]); std::swap(im[i], im[j]); }
+    }
+    const double two_pi = 6.283185307179586476925286766559;
+    for (int len = 2; len <= N; len <<= 1) {
+        for (int i = 0; i < N; i += len) {
+            for (int k = 0; k < len / 2; ++k) {
+                const double ang = -two_pi * k / len;
+                const double wr = std::cos(ang), wi = std::sin(ang);
+                const int a = i + k, b = i + k + len / 2;
+                const double xr = re[b] * wr - im[b] * wi, xi = re[b] * wi + im[b] * wr;
+                re[b] = re[a] - xr; im[b] = im[a] - xi;
+                re[a] += xr; im[a] += xi;
+            }
+        }
+    }
+    std::vector<float2> spec(N);
+    for (int n = 0; n < N; ++n) spec[n] = make_float2((float)(re[n] / N), (float)(im[n] / N));
+    // the previous window's highpass may still read the old spectrum on the handle's stream
+    HIP_TRY(hipMemcpyAsync(h.spec.p, spec.data(), sizeof(float2) * N, hipMemcpyHostToDevice, h.stream));
+    HIP_TRY(hipStreamSynchronize(h.stream));
+    h.spec_W = W;
+    h.spec_lo = Llo;
+    h.spec_hi = Lhi;
+    return LSS_OK;
+}
+
+lss_status create_impl(const lss_params* p, const lss_artifacts* a, Handle** out) {
+    if (!p || !a || !out) return fail(LSS_ERR_ARG, "null argument");
+    if (lss_status st = check_params(*p)) return st;
+    if (!a->U || !a->wPCA || !a->Wrot || !a->hp_filter || a->hp_len < 1 || !a->xc || !a->yc)
+        return fail(LSS_ERR_ARG, "missing artifact array");
+    std::unique_ptr<Handle> h(new Handle());
+    h->prm = *p;
+    HIP_TRY(hipGetDevice(&h->device));
+    HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    h->own_stream = true;
+    const int C = p->n_chan, T = p->n_templates, K = p->n_pcs, nt = p->nt, nC = p->n_nearest;
+    h->C = C; h->T = T; h->K = K; h->nt = nt; h->nC = nC;
+    h->Wmax = p->max_window;
+    h->ldw = (h->Wmax + 63) / 64 * 64;
+    h->ldq = (K * C + 3) / 4 * 4;
+    h->ldm = (C + 3) / 4 * 4;
+    const int L = 2 * nt + 1;
+    const int64_t Wm = h->Wmax;
+
+    // ---- tables ----
+    HIP_TRY(h->U.alloc((size_t)T * h->ldq));
+    HIP_TRY(h->wpca.alloc((size_t)K * nt));
+    HIP_TRY(h->wmat.alloc((size_t)C * h->ldm));
+    HIP_TRY(h->nm.alloc(T));
+    HIP_TRY(h->waves.alloc((size_t)T * nt * C));
+    HIP_TRY(h->ctc.alloc((size_t)T * L * T));
+    HIP_TRY(h->Ucc.alloc((size_t)nC * T * K));
+    HIP_TRY(h->xc.alloc(C));
+    HIP_TRY(h->yc.alloc(C));
+    HIP_TRY(h->iCC.alloc((size_t)nC * C));
+    HIP_TRY(h->iU.alloc(T));
+    HIP_TRY(h->spec.alloc(lss::kFftN));
+    HIP_TRY(h->tw.alloc(lss::kFftN));
+    HIP_TRY(hipMemset(h->U.p, 0, sizeof(float) * T * h->ldq));
+    HIP_TRY(hipMemcpy2D(h->U.p, sizeof(float) * h->ldq, a->U, sizeof(float) * K * C, sizeof(float) * K * C, T,
+                        hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->wpca.p, a->wPCA, sizeof(float) * K * nt, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->xc.p, a->xc, sizeof(float) * C, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->yc.p, a->yc, sizeof(float) * C, hipMemcpyHostToDevice));
+    h->h_xc.assign(a->xc, a->xc + C);
+    h->h_yc.assign(a->yc, a->yc + C);
+    h->hp_taps.assign(a->hp_filter, a->hp_filter + a->hp_len);
+
+    // whitening operator (M @ Wrot) in FP64 (kilosort/io.py:787)
+    {
+        std::vector<float> wm((size_t)C * C);
+        if (a->drift) {
+            std::vector<double> acc(C);
+            for (int i = 0; i < C; ++i) {
+                std::fill(acc.begin(), acc.end(), 0.0);
+                for (int k = 0; k < C; ++k) {
+                    const double mik = a->drift[(size_t)i * C + k];
+                    if (mik == 0.0) continue;
+                    const float* wr = a->Wrot + (size_t)k * C;
+                    for (int j = 0; j < C; ++j) acc[j] += mik * (double)wr[j];
+                }
+                for (int j = 0; j < C; ++j) wm[(size_t)i * C + j] = (float)acc[j];
+            }
+        } else {
+            std::copy(a->Wrot, a->Wrot + (size_t)C * C, wm.begin());
+        }
+        HIP_TRY(hipMemset(h->wmat.p, 0, sizeof(float) * C * h->ldm));
+        HIP_TRY(hipMemcpy2D(h->wmat.p, sizeof(float) * h->ldm, wm.data(), sizeof(float) * C, sizeof(float) * C, C,
+                            hipMemcpyHostToDevice));
+    }
+    // twiddles exp(-2 pi i m / N)
+    {
+        std::vector<float2> tw(lss::kFftN);
+        const double two_pi = 6.283185307179586476925286766559;
+        for (int m = 0; m < lss::kFftN; ++m) {
+            double ang = -two_pi * m / lss::kFftN;
+            tw[m] = make_float2((float)std::cos(ang), (float)std::sin(ang));
+        }
+        HIP_TRY(hipMemcpy(h->tw.p, tw.data(), sizeof(float2) * lss::kFftN, hipMemcpyHostToDevice));
+    }
+    // iCC / iU / Ucc (prepare_extract, template_matching.py:20-26)
+    {
+        std::vector<int> iCC;
+        nearest_channels(h->h_xc, h->h_yc, nC, iCC);
+        std::vector<int> iU(T);
+        std::vector<float> Ucc((size_t)nC * T * K);
+        for (int t = 0; t < T; ++t) {
+            int best = 0;
+            float bv = -1.f;
+            for (int c = 0; c < C; ++c) {
+                float e = 0.f;   // (U**2).sum(1) in float32, as the reference
+                for (int k = 0; k < K; ++k) {
+                    float u = a->U[((size_t)t * K + k) * C + c];
+                    e += u * u;
+                }
+                if (e > bv) { bv = e; best = c; }
+            }
+            iU[t] = best;
+        }
+        for (int j = 0; j < nC; ++j)
+            for (int t = 0; t < T; ++t)
+                for (int k = 0; k < K; ++k)
+                    Ucc[((size_t)j * T + t) * K + k] = a->U[((size_t)t * K + k) * C + iCC[(size_t)j * C + iU[t]]];
+        HIP_TRY(hipMemcpy(h->iCC.p, iCC.data(), sizeof(int) * nC * C, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(h->iU.p, iU.data(), sizeof(int) * T, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(h->Ucc.p, Ucc.data(), sizeof(float) * nC * T * K, hipMemcpyHostToDevice));
+    }
+    HIP_TRY(lss::launch_tables(h->U.p, h->ldq, h->wpca.p, T, K, C, nt, h->nm.p, h->waves.p, h->ctc.p, h->stream));
+
+    // ---- window buffers ----
+    HIP_TRY(h->win.alloc((size_t)Wm * C));
+    HIP_TRY(h->sums.alloc(C));
+    HIP_TRY(h->xcw.alloc((size_t)C * h->ldw));
+    HIP_TRY(h->xhp.alloc((size_t)C * h->ldw));
+    HIP_TRY(h->xw.alloc((size_t)Wm * C));
+    HIP_TRY(h->xp.alloc((size_t)Wm * h->ldq));
+    HIP_TRY(h->B.alloc((size_t)Wm * T));
+    HIP_TRY(h->cfmax.alloc(Wm));
+    HIP_TRY(h->imax.alloc(Wm));
+    HIP_TRY(h->blockcnt.alloc((Wm + lss::kPeakCols - 1) / lss::kPeakCols));
+    HIP_TRY(h->sp_col.alloc(p->max_spikes));
+    HIP_TRY(h->sp_tmp.alloc(p->max_spikes));
+    HIP_TRY(h->sp_amp.alloc(p->max_spikes));
+    HIP_TRY(h->counters.alloc(4));
+    HIP_TRY(h->tF.alloc((size_t)p->max_spikes * nC * K));
+    HIP_TRY(h->xs.alloc(p->max_spikes));
+    HIP_TRY(h->ys.alloc(p->max_spikes));
+    HIP_TRY(hipHostMalloc((void**)&h->h_counters, sizeof(int) * 4, hipHostMallocDefault));
+    for (auto& e : h->ev) HIP_TRY(hipEventCreate(&e));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    h->alive = true;
+    *out = h.release();
+    return LSS_OK;
+}
+
+Handle* as_handle(void* p) { return static_cast<Handle*>(p); }
+
+lss_status check_window(Handle& h, int64_t W) {
+    if (W < 2 * (int64_t)h.nt + 2) return fail(LSS_ERR_ARG, "window shorter than 2*nt+2");
+    if (W > h.Wmax) return fail(LSS_ERR_ARG, "window longer than max_window");
+    return LSS_OK;
+}
+
+lss_status run_car(Handle& h, const int16_t* d_win, int64_t W) {
+    HIP_TRY(lss::launch_channel_sums(d_win, W, h.C, h.sums.p, h.stream));
+    HIP_TRY(lss::launch_car_transpose(d_win, W, h.C, h.sums.p, h.prm.do_car, h.xcw.p, h.ldw, h.stream));
+    return LSS_OK;
+}
+
+lss_status run_highpass(Handle& h, const float* in, float* out, int64_t W) {
+    if (lss_status st = prepare_spectrum(h, W)) return st;
+    HIP_TRY(lss::launch_highpass(in, h.ldw, out, h.ldw, h.C, W, h.spec_lo, h.spec_hi, h.spec.p, h.tw.p, h.stream));
+    return LSS_OK;
+}
+
+lss_status run_whiten(Handle& h, const float* in_cw, float* out_wc, int64_t W) {
+    // Xw[w][c'] = sum_c X[c][w] Wm[c'][c]
+    if (h.prm.precision == LSS_PREC_FP32)
+        HIP_TRY(lss::launch_gemm_f32(in_cw, h.ldw, 1, h.wmat.p, h.ldm, out_wc, h.C, (int)W, h.C, h.C, h.stream));
+    else
+        HIP_TRY(lss::launch_gemm_lp(h.prm.precision, in_cw, h.ldw, 1, h.wmat.p, h.ldm, out_wc, h.C, (int)W, h.C, h.C,
+                                    h.stream));
+    return LSS_OK;
+}
+
+lss_status run_scores(Handle& h, const float* x_wc, float* b_wt, int64_t W) {
+    HIP_TRY(lss::launch_project(x_wc, W, h.C, h.wpca.p, h.K, h.nt, h.xp.p, h.ldq, h.stream));
+    const int KC = h.K * h.C;
+    if (h.prm.precision == LSS_PREC_FP32)
+        HIP_TRY(lss::launch_gemm_f32(h.xp.p, h.ldq, 0, h.U.p, h.ldq, b_wt, h.T, (int)W, h.T, KC, h.stream));
+    else
+        HIP_TRY(lss::launch_gemm_lp(h.prm.precision, h.xp.p, h.ldq, 0, h.U.p, h.ldq, b_wt, h.T, (int)W, h.T, KC,
+                                    h.stream));
+    return LSS_OK;
+}
+
+// pursuit over B/X, returns total spikes and passes
+lss_status run_pursuit(Handle& h, float* B, float* X, int64_t W, int* n_total, int* passes) {
+    lss::PursuitArgs a{};
+    a.B = B; a.X = X; a.nm = h.nm.p; a.ctc = h.ctc.p; a.waves = h.waves.p;
+    a.cfmax = h.cfmax.p; a.imax = h.imax.p; a.blockcnt = h.blockcnt.p;
+    a.sp_col = h.sp_col.p; a.sp_tmp = h.sp_tmp.p; a.sp_amp = h.sp_amp.p; a.counters = h.counters.p;
+    a.W = W; a.T = h.T; a.C = h.C; a.nt = h.nt; a.cap = h.prm.max_spikes;
+    a.th2 = h.prm.th_learned * h.prm.th_learned;
+    HIP_TRY(hipMemsetAsync(h.counters.p, 0, sizeof(int) * 4, h.stream));
+    HIP_TRY(lss::launch_score_all(a, h.stream));
+    int p = 0;
+    int total = 0;
+    for (; p < h.prm.max_passes; ++p) {
+        HIP_TRY(lss::launch_peaks(a, h.stream));
+        HIP_TRY(hipMemcpyAsync(h.h_counters, h.counters.p, sizeof(int) * 4, hipMemcpyDeviceToHost, h.stream));
+        HIP_TRY(hipStreamSynchronize(h.stream));
+        const int n_new = h.h_counters[1];
+        total = h.h_counters[0] + n_new;
+        if (h.h_counters[2]) return fail(LSS_ERR_CAPACITY, "spike capacity (max_spikes) exceeded");
+        if (n_new == 0) break;
+        HIP_TRY(lss::launch_update(a, n_new, h.stream));
+    }
+    *n_total = total;
+    *passes = p;
+    return LSS_OK;
+}
+
+lss_status run_features(Handle& h, const float* X, int64_t W, int n) {
+    lss::FeatureArgs f{};
+    f.X = X; f.wpca = h.wpca.p; f.iCC = h.iCC.p; f.iU = h.iU.p; f.Ucc = h.Ucc.p;
+    f.xc = h.xc.p; f.yc = h.yc.p; f.sp_col = h.sp_col.p; f.sp_tmp = h.sp_tmp.p; f.sp_amp = h.sp_amp.p;
+    f.tF = h.tF.p; f.xs = h.xs.p; f.ys = h.ys.p;
+    f.n = n; f.C = h.C; f.T = h.T; f.K = h.K; f.nt = h.nt; f.nC = h.nC; f.W = W;
+    HIP_TRY(lss::launch_features(f, h.stream));
+    return LSS_OK;
+}
+
+lss_status emit_spikes(Handle& h, int n, int passes, int64_t window_start, lss_spikes* out) {
+    out->n = 0;
+    out->passes = passes;
+    if (n == 0) return LSS_OK;
+    const int FK = h.nC * h.K;
+    std::vector<int> col(n), tmp(n);
+    std::vector<float> amp(n), tF((size_t)n * FK), xs(n), ys(n);
+    HIP_TRY(hipMemcpyAsync(col.data(), h.sp_col.p, sizeof(int) * n, hipMemcpyDeviceToHost, h.stream));
+    HIP_TRY(hipMemcpyAsync(tmp.data(), h.sp_tmp.p, sizeof(int) * n, hipMemcpyDeviceToHost, h.stream));
+    HIP_TRY(hipMemcpyAsync(amp.data(), h.sp_amp.p, sizeof(float) * n, hipMemcpyDeviceToHost, h.stream));
+    if (out->features) HIP_TRY(hipMemcpyAsync(tF.data(), h.tF.p, sizeof(float) * n * FK, hipMemcpyDeviceToHost, h.stream));
+    if (out->x) HIP_TRY(hipMemcpyAsync(xs.data(), h.xs.p, sizeof(float) * n, hipMemcpyDeviceToHost, h.stream));
+    if (out->y) HIP_TRY(hipMemcpyAsync(ys.data(), h.ys.p, sizeof(float) * n, hipMemcpyDeviceToHost, h.stream));
+    HIP_TRY(hipStreamSynchronize(h.stream));
+    int64_t k = 0;
+    const int64_t shift = window_start - h.nt / 2 + h.prm.nt0min;   // template_matching.py:68
+    for (int i = 0; i < n; ++i) {
+        const int64_t smp = shift + col[i];
+        if (h.prm.drop_negative && smp < 0) continue;   // template_matching.py:53-59
+        if (k >= out->capacity) return fail(LSS_ERR_CAPACITY, "output capacity exceeded");
+        if (out->sample) out->sample[k] = smp;
+        if (out->template_id) out->template_id[k] = tmp[i];
+        if (out->amplitude) out->amplitude[k] = amp[i];
+        if (out->column) out->column[k] = col[i];
+        if (out->features) std::memcpy(out->features + k * FK, tF.data() + (size_t)i * FK, sizeof(float) * FK);
+        if (out->x) out->x[k] = xs[i];
+        if (out->y) out->y[k] = ys[i];
+        ++k;
+    }
+    out->n = k;
+    return LSS_OK;
+}
+
+std::string trim(const std::string& s) {
+    size_t a = s.find_first_not_of(" \t\r\n"), b = s.find_last_not_of(" \t\r\n");
+    return a == std::string::npos ? "" : s.substr(a, b - a + 1);
+}
+
+}  // namespace
+
+// =================================================================================================
+extern "C" {
+
+const char* lss_last_error(void) { return g_err.c_str(); }
+int32_t lss_abi_version(void) { return LSS_ABI_VERSION; }
+
+lss_status lss_default_params(lss_params* p) {
+    if (!p) return fail(LSS_ERR_ARG, "null params");
+    std::memset(p, 0, sizeof(*p));
+    p->n_chan = 384;
+    p->n_templates = 0;
+    p->n_pcs = 6;
+    p->nt = 61;
+    p->nt0min = 20;
+    p->n_nearest = 10;
+    p->max_window = 60122;
+    p->max_passes = 50;
+    p->max_spikes = 1 << 18;
+    p->do_car = 1;
+    p->precision = LSS_PREC_FP32;
+    p->drop_negative = 1;
+    p->th_learned = 8.f;
+    p->fs = 30000.f;
+    return LSS_OK;
+}
+
+lss_status lss_create(const lss_params* p, const lss_artifacts* a, void** handle) {
+    if (!handle) return fail(LSS_ERR_ARG, "null handle pointer");
+    Handle* h = nullptr;
+    lss_status st = create_impl(p, a, &h);
+    if (st == LSS_OK) *handle = h;
+    return st;
+}
+
+lss_status lss_create_from_dir(const char* dir, const lss_params* p0, void** handle) {
+    if (!dir || !p0 || !handle) return fail(LSS_ERR_ARG, "null argument");
+    std::string d(dir);
+    if (!d.empty() && d.back() != '/') d += '/';
+    lss_params p = *p0;
+    // misc.txt key:value lines (OnlineSpikesV2.cu:435-457)
+    {
+        std::ifstream inf(d + "misc.txt");
+        if (!inf.is_open()) return fail(LSS_ERR_IO, "cannot open " + d + "misc.txt");
+        std::map<std::string, std::string> kv;
+        std::string line;
+        while (std::getline(inf, line)) {
+            size_t pos = line.find(':');
+            if (pos == std::string::npos) continue;
+            kv[trim(line.substr(0, pos))] = trim(line.substr(pos + 1));
+        }
+        try {
+            if (kv.count("nt0min")) p.nt0min = std::stoi(kv["nt0min"]);
+            if (kv.count("numNearestChans")) p.n_nearest = std::stoi(kv["numNearestChans"]);
+            if (kv.count("Th_learned")) p.th_learned = std::stof(kv["Th_learned"]);
+        } catch (...) {
+            return fail(LSS_ERR_IO, "malformed misc.txt");
+        }
+    }
+    lss::NpyArray U, wpca, wrot, drift, hp, xc, yc;
+    std::string err;
+    if (!lss::npy_load_f32(d + "Wall3.npy", U, err)) return fail(LSS_ERR_IO, err);
+    if (!lss::npy_load_f32(d + "wPCA.npy", wpca, err)) return fail(LSS_ERR_IO, err);
+    if (!lss::npy_load_f32(d + "whiteningMat.npy", wrot, err) &&
+        !lss::npy_load_f32(d + "whitening_mat.npy", wrot, err))
+        return fail(LSS_ERR_IO, err);
+    bool have_drift = lss::npy_load_f32(d + "drift_matrix.npy", drift, err);
+    if (!lss::npy_load_f32(d + "hp_filter.npy", hp, err)) return fail(LSS_ERR_IO, err);
+    if (!lss::npy_load_f32(d + "xc.npy", xc, err)) return fail(LSS_ERR_IO, err);
+    if (!lss::npy_load_f32(d + "yc.npy", yc, err)) return fail(LSS_ERR_IO, err);
+    if (U.shape.size() != 3 || wpca.shape.size() != 2) return fail(LSS_ERR_IO, "Wall3/wPCA rank");
+    p.n_templates = (int)U.shape[0];
+    p.n_pcs = (int)U.shape[1];
+    p.n_chan = (int)U.shape[2];
+    p.nt = (int)wpca.shape[1];
+    if ((int)wpca.shape[0] != p.n_pcs) return fail(LSS_ERR_IO, "wPCA rows != Wall3 PCs");
+    const size_t CC = (size_t)p.n_chan * p.n_chan;
+    if (wrot.data.size() != CC) return fail(LSS_ERR_IO, "whitening matrix is not C x C");
+    if (xc.data.size() != (size_t)p.n_chan || yc.data.size() != (size_t)p.n_chan) return fail(LSS_ERR_IO, "xc/yc size");
+    std::vector<float> dm;
+    if (have_drift) {
+        if (drift.data.size() != CC) return fail(LSS_ERR_IO, "drift matrix is not C x C");
+        // the exporter saves M.T (run_online_spikes.py:532); the matrix applied is M
+        dm.resize(CC);
+        for (int i = 0; i < p.n_chan; ++i)
+            for (int j = 0; j < p.n_chan; ++j) dm[(size_t)i * p.n_chan + j] = drift.data[(size_t)j * p.n_chan + i];
+    }
+    lss_artifacts a{};
+    a.U = U.data.data();
+    a.wPCA = wpca.data.data();
+    a.Wrot = wrot.data.data();
+    a.drift = have_drift ? dm.data() : nullptr;
+    a.hp_filter = hp.data.data();
+    a.hp_len = (int32_t)hp.data.size();
+    a.xc = xc.data.data();
+    a.yc = yc.data.data();
+    return lss_create(&p, &a, handle);
+}
+
+lss_status lss_destroy(void* handle) {
+    if (!handle) return fail(LSS_ERR_ARG, "null handle");
+    Handle* h = as_handle(handle);
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    delete h;
+    return LSS_OK;
+}
+
+lss_status lss_get_params(void* handle, lss_params* p) {
+    if (!handle || !p) return fail(LSS_ERR_ARG, "null argument");
+    *p = as_handle(handle)->prm;
+    return LSS_OK;
+}
+
+lss_status lss_set_stream(void* handle, void* hip_stream) {
+    if (!handle) return fail(LSS_ERR_ARG, "null handle");
+    Handle* h = as_handle(handle);
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (h->own_stream) HIP_TRY(hipStreamDestroy(h->stream));
+    h->own_stream = false;
+    h->stream = (hipStream_t)hip_stream;
+    if (!h->stream) {
+        HIP_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+        h->own_stream = true;
+    }
+    return LSS_OK;
+}
+
+lss_status lss_pad_batch(const int16_t* rec, int64_t n_samples, int32_t C, int32_t NT, int32_t nt,
+                         int64_t ibatch, int16_t* out, int64_t* window_start) {
+    // kilosort/io.py:584-637
+    if (!rec || !out || C < 1 || NT < 1 || nt < 0 || n_samples < 1) return fail(LSS_ERR_ARG, "bad argument");
+    const int64_t nb = (n_samples + NT - 1) / NT;
+    if (ibatch < 0 || ibatch >= nb) return fail(LSS_ERR_ARG, "ibatch out of range");
+    const int64_t Wp = (int64_t)NT + 2 * nt;
+    int64_t bstart, bend;
+    if (ibatch == 0) {
+        bstart = 0;
+        bend = std::min<int64_t>(n_samples, (int64_t)NT + nt);
+    } else {
+        bstart = ibatch * NT - nt;
+        bend = std::min<int64_t>(n_samples, bstart + Wp);
+    }
+    const int64_t nsamp = bend - bstart;
+    std::memset(out, 0, sizeof(int16_t) * Wp * C);
+    if (ibatch == 0) {
+        std::memcpy(out + (int64_t)nt * C, rec, sizeof(int16_t) * nsamp * C);
+        for (int64_t w = 0; w < nt; ++w) std::memcpy(out + w * C, out + (int64_t)nt * C, sizeof(int16_t) * C);
+    } else if (ibatch == nb - 1) {
+        std::memcpy(out, rec + bstart * C, sizeof(int16_t) * nsamp * C);
+        for (int64_t w = nsamp; w < Wp; ++w) std::memcpy(out + w * C, out + (nsamp - 1) * C, sizeof(int16_t) * C);
+    } else {
+        std::memcpy(out, rec + bstart * C, sizeof(int16_t) * nsamp * C);
+    }
+    if (window_start) *window_start = ibatch * NT - nt;
+    return LSS_OK;
+}
+
+lss_status lss_sort_window(void* handle, const int16_t* window, int64_t W, int32_t on_device,
+                           int64_t window_start, lss_spikes* out) {
+    if (!handle || !window || !out) return fail(LSS_ERR_ARG, "null argument");
+    Handle& h = *as_handle(handle);
+    if (!h.alive) return fail(LSS_ERR_STATE, "handle not usable");
+    if (lss_status st = check_window(h, W)) return st;
+    const int16_t* d_win = window;
+    HIP_TRY(hipEventRecord(h.ev[0], h.stream));
+    if (!on_device) {
+        HIP_TRY(hipMemcpyAsync(h.win.p, window, sizeof(int16_t) * W * h.C, hipMemcpyHostToDevice, h.stream));
+        d_win = h.win.p;
+    }
+    if (lss_status st = run_car(h, d_win, W)) return st;
+    HIP_TRY(hipEventRecord(h.ev[1], h.stream));
+    if (lss_status st = run_highpass(h, h.xcw.p, h.xhp.p, W)) return st;
+    HIP_TRY(hipEventRecord(h.ev[2], h.stream));
+    if (lss_status st = run_whiten(h, h.xhp.p, h.xw.p, W)) return st;
+    HIP_TRY(hipEventRecord(h.ev[3], h.stream));
+    if (lss_status st = run_scores(h, h.xw.p, h.B.p, W)) return st;
+    HIP_TRY(hipEventRecord(h.ev[4], h.stream));
+    int n = 0, passes = 0;
+    if (lss_status st = run_pursuit(h, h.B.p, h.xw.p, W, &n, &passes)) return st;
+    HIP_TRY(hipEventRecord(h.ev[5], h.stream));
+    if (lss_status st = run_features(h, h.xw.p, W, n)) return st;
+    HIP_TRY(hipEventRecord(h.ev[6], h.stream));
+    HIP_TRY(hipEventSynchronize(h.ev[6]));
+    for (int i = 0; i < 6; ++i) HIP_TRY(hipEventElapsedTime(&h.last_ms[i], h.ev[i], h.ev[i + 1]));
+    HIP_TRY(hipEventElapsedTime(&h.last_ms[6], h.ev[0], h.ev[6]));
+    return emit_spikes(h, n, passes, window_start, out);
+}
+
+lss_status lss_stage_car(void* handle, const int16_t* d_window, int64_t W, float* d_out_cw) {
+    if (!handle || !d_window || !d_out_cw) return fail(LSS_ERR_ARG, "null argument");
+    Handle& h = *as_handle(handle);
+    if (lss_status st = check_window(h, W)) return st;
+    if (lss_status st = run_car(h, d_window, W)) return st;
+    HIP_TRY(hipMemcpy2DAsync(d_out_cw, sizeof(float) * W, h.xcw.p, sizeof(float) * h.ldw, sizeof(float) * W, h.C,
+                             hipMemcpyDeviceToDevice, h.stream));
+    HIP_TRY(hipStreamSynchronize(h.stream));
+    return LSS_OK;
+}
+
+lss_status lss_stage_filter(void* handle, const int16_t* d_window, int64_t W, float* d_out_cw) {
+    if (!handle || !d_window || !d_out_cw) return fail(LSS_ERR_ARG, "null argument");
+    Handle& h = *as_handle(handle);
+    if (lss_status st = check_window(h, W)) return st;
+    if (lss_status st = run_car(h, d_window, W)) return st;
+    if (lss_status st = run_highpass(h, h.xcw.p, h.xhp.p, W)) return st;
+    HIP_TRY(hipMemcpy2DAsync(d_out_cw, sizeof(float) * W, h.xhp.p, sizeof(float) * h.ldw, sizeof(float) * W, h.C,
+                             hipMemcpyDeviceToDevice, h.stream));
+    HIP_TRY(hipStreamSynchronize(h.stream));
+    return LSS_OK;
+}
+
+lss_status lss_stage_highpass(void* handle, const float* d_in_cw, int64_t W, float* d_out_cw) {
+    if (!handle || !d_in_cw || !d_out_cw) return fail(LSS_ERR_ARG, "null argument");
+    Handle& h = *as_handle(handle);
+    if (lss_status st = check_window(h, W)) return st;
+    HIP_TRY(hipMemcpy2DAsync(h.xcw.p, sizeof(float) * h.ldw, d_in_cw, sizeof(float) * W, sizeof(float) * W, h.C,
+                             hipMemcpyDeviceToDevice, h.stream));
+    if (lss_status st = run_highpass(h, h.xcw.p, h.xhp.p, W)) return st;
+    HIP_TRY(hipMemcpy2DAsync(d_out_cw, sizeof(float) * W, h.xhp.p, sizeof(float) * h.ldw, sizeof(float) * W, h.C,
+                             hipMemcpyDeviceToDevice, h.stream));
+    HIP_TRY(hipStreamSynchronize(h.stream));
+    return LSS_OK;
+}
+
+lss_status lss_stage_whiten(void* handle, const float* d_in_cw, int64_t W, float* d_out_wc) {
+    if (!handle || !d_in_cw || !d_out_wc) return fail(LSS_ERR_ARG, "null argument");
+    Handle& h = *as_handle(handle);
+    if (lss_status st = check_window(h, W)) return st;
+    HIP_TRY(hipMemcpy2DAsync(h.xhp.p, sizeof(float) * h.ldw, d_in_cw, sizeof(float) * W, sizeof(float) * W, h.C,
+                             hipMemcpyDeviceToDevice, h.stream));
+    if (lss_status st = run_whiten(h, h.xhp.p, d_out_wc, W)) return st;
+    HIP_TRY(hipStreamSynchronize(h.stream));
+    return LSS_OK;
+}
+
+lss_status lss_stage_scores(void* handle, const float* d_x_wc, int64_t W, float* d_b_wt) {
+    if (!handle || !d_x_wc || !d_b_wt) return fail(LSS_ERR_ARG, "null argument");
+    Handle& h = *as_handle(handle);
+    if (lss_status st = check_window(h, W)) return st;
+    if (lss_status st = run_scores(h, d_x_wc, d_b_wt, W)) return st;
+    HIP_TRY(hipStreamSynchronize(h.stream));
+    return LSS_OK;
+}
+
+lss_status lss_stage_pursuit(void* handle, float* d_b_wt, float* d_x_wc, int64_t W, int64_t window_start,
+                             lss_spikes* out) {
+    if (!handle || !d_b_wt || !d_x_wc || !out) return fail(LSS_ERR_ARG, "null argument");
+    Handle& h = *as_handle(handle);
+    if (lss_status st = check_window(h, W)) return st;
+    int n = 0, passes = 0;
+    if (lss_status st = run_pursuit(h, d_b_wt, d_x_wc, W, &n, &passes)) return st;
+    if (lss_status st = run_features(h, d_x_wc, W, n)) return st;
+    return emit_spikes(h, n, passes, window_start, out);
+}
+
+lss_status lss_get_tables(void* handle, float* nm, float* waves, float* ctc, float* wmat) {
+    if (!handle) return fail(LSS_ERR_ARG, "null handle");
+    Handle& h = *as_handle(handle);
+    const int L = 2 * h.nt + 1;
+    if (nm) HIP_TRY(hipMemcpyAsync(nm, h.nm.p, sizeof(float) * h.T, hipMemcpyDeviceToHost, h.stream));
+    if (waves) HIP_TRY(hipMemcpyAsync(waves, h.waves.p, sizeof(float) * h.T * h.nt * h.C, hipMemcpyDeviceToHost, h.stream));
+    if (ctc) HIP_TRY(hipMemcpyAsync(ctc, h.ctc.p, sizeof(float) * (size_t)h.T * L * h.T, hipMemcpyDeviceToHost, h.stream));
+    if (wmat)
+        HIP_TRY(hipMemcpy2DAsync(wmat, sizeof(float) * h.C, h.wmat.p, sizeof(float) * h.ldm, sizeof(float) * h.C, h.C,
+                                 hipMemcpyDeviceToHost, h.stream));
+    HIP_TRY(hipStreamSynchronize(h.stream));
+    return LSS_OK;
+}
+
+lss_status lss_last_timings(void* handle, float* ms, int32_t n) {
+    if (!handle || !ms || n < 0 || n > 8) return fail(LSS_ERR_ARG, "bad argument");
+    Handle& h = *as_handle(handle);
+    for (int i = 0; i < n; ++i) ms[i] = h.last_ms[i];
+    return LSS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,91 @@
+// Launchers of the gfx950 kernels of liblss.  All take a hipStream_t and return hipError_t of the
+// launch.  Layouts: [a][b] is row-major with the last index fastest; "ld" is the row stride in
+// elements.  X_cw = [C][ldw] channel-major window, X_wc = [W][C] time-major window.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace lss {
+
+// ---- preprocessing (preprocess.hip) ---------------------------------------------------------
+// Exact int64 channel sums of an int16 [W][C] window.
+hipError_t launch_channel_sums(const int16_t* win, int64_t W, int C, long long* sums, hipStream_t s);
+// out[c][w] = (x[w][c] - mean_c) - lower_median_c'(x[w][c'] - mean_c'), transposed to [C][ldo].
+// do_car = 0 skips the median.  C <= 1024.
+hipError_t launch_car_transpose(const int16_t* win, int64_t W, int C, const long long* sums,
+                                int do_car, float* out, int64_t ldo, hipStream_t s);
+
+// Overlap-save highpass: y[c][i] = sum_{j=-Llo..Lhi} g[j] x[c][(i+j) mod W].
+constexpr int kFftN = 8192;          // FFT length of one segment
+constexpr int kFftLog2 = 13;
+constexpr int kHpHalf = 800;         // taps kept on each side of the filter centre
+// spec: [kFftN] complex (float2) spectrum of the time-reversed kernel, scaled by 1/N;
+// tw: [kFftN] float2 twiddles exp(-2 pi i m / N).
+hipError_t launch_highpass(const float* x, int64_t ldx, float* y, int64_t ldy, int C, int64_t W,
+                           int Llo, int Lhi, const float2* spec, const float2* tw, hipStream_t s);
+// Plain batched forward FFT of [nb][kFftN] complex rows (test hook of the LDS FFT).
+hipError_t launch_fft_rows(float2* data, int nb, const float2* tw, hipStream_t s);
+
+// ---- contractions (gemm.hip, project.hip) ---------------------------------------------------
+// C[m][n] = sum_k A(m,k) B(n,k);  A(m,k) = A[m*lda+k] (a_km=0) or A[k*lda+m] (a_km=1);
+// B(n,k) = B[n*ldb+k].  FP32 MFMA, FP32 accumulation.  lda, ldb multiples of 4; 16-byte aligned.
+hipError_t launch_gemm_f32(const float* A, int64_t lda, int a_km, const float* B, int64_t ldb,
+                           float* C, int64_t ldc, int M, int N, int K, hipStream_t s);
+// Precision variants with operands converted on the fly (prec = lss_precision).
+hipError_t launch_gemm_lp(int prec, const float* A, int64_t lda, int a_km, const float* B,
+                          int64_t ldb, float* C, int64_t ldc, int M, int N, int K, hipStream_t s);
+// Xp[w][k*C+c] = sum_m X[w+m-nt/2][c] * wpca[k][m] (zero outside [0,W)), row stride ldp.
+// K <= 8, nt <= 127.
+hipError_t launch_project(const float* x_wc, int64_t W, int C, const float* wpca, int K, int nt,
+                          float* xp, int64_t ldp, hipStream_t s);
+
+// ---- pursuit (pursuit.hip) ------------------------------------------------------------------
+struct PursuitArgs {
+    float* B;            // [W][T] scores (updated)
+    float* X;            // [W][C] residual (updated)
+    const float* nm;     // [T]
+    const float* ctc;    // [T][2nt+1][T]  ctc[j][l][i]
+    const float* waves;  // [T][nt][C]
+    float* cfmax;        // [W]
+    int* imax;           // [W]
+    int* blockcnt;       // [ceil(W/kPeakCols)]
+    int* sp_col;         // [cap] peak column
+    int* sp_tmp;         // [cap] template
+    float* sp_amp;       // [cap]
+    int* counters;       // [0] total spikes so far, [1] spikes of the last pass, [2] overflow flag
+    int64_t W;
+    int T, C, nt;
+    int cap;
+    float th2;
+};
+constexpr int kPeakCols = 1024;      // columns per block of the peak kernels
+hipError_t launch_score_all(const PursuitArgs& a, hipStream_t s);
+hipError_t launch_peaks(const PursuitArgs& a, hipStream_t s);        // two kernels
+hipError_t launch_update(const PursuitArgs& a, int n_new, hipStream_t s);
+
+struct FeatureArgs {
+    const float* X;      // [W][C] final residual
+    const float* wpca;   // [K][nt]
+    const int* iCC;      // [nC][C]
+    const int* iU;       // [T]
+    const float* Ucc;    // [nC][T][K]
+    const float* xc;     // [C]
+    const float* yc;     // [C]
+    const int* sp_col;
+    const int* sp_tmp;
+    const float* sp_amp;
+    float* tF;           // [n][nC][K]
+    float* xs;           // [n]
+    float* ys;           // [n]
+    int n, C, T, K, nt, nC;
+    int64_t W;
+};
+hipError_t launch_features(const FeatureArgs& a, hipStream_t s);
+
+// ---- load-time tables in FP64 (tables.hip) --------------------------------------------------
+// nm[t] = sum U^2; waves[t][m][c] = sum_k U[t][k][c] wpca[k][m]; ctc[j][l][i] (see lss.h).
+// U rows (one per template) have stride ldu >= K*C.
+hipError_t launch_tables(const float* U, int64_t ldu, const float* wpca, int T, int K, int C, int nt,
+                         float* nm, float* waves, float* ctc, hipStream_t s);
+
+}  // namespace lss

@@ -1,0 +1,285 @@
+// Batch ingest, common-average reference and overlap-save FFT highpass for gfx950.
+//
+// Semantics (Kilosort4, kilosort/io.py:757-774): per channel the time mean of the padded window is
+// removed, then per sample the lower median over channels (torch.median), then the circular FFT
+// highpass X <- fftshift(ifft(fft(X) * conj(fft(f)))) with the 300 Hz filtfilt impulse response f
+// padded/cropped to the window (kilosort/preprocessing.py:138-154).  That circular correlation is
+// y[i] = sum_j g[j] x[(i+j) mod W] with g[j] = f[(W/2 + j) mod W]; g decays below 1.3e-13 (of a
+// 0.98 centre tap) beyond |j| = 800, so the taps |j| <= 800 are applied with 8192-point FFTs in
+// LDS (overlap-save), two channels per complex transform.
+#include "lss_kernels.h"
+
+namespace lss {
+
+// ------------------------------------------------------------------------------------------------
+// channel sums: int16 [W][C] -> int64 sums[C] (exact)
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void channel_sums_kernel(const int16_t* __restrict__ win,
+                                                            int64_t W, int C,
+                                                            long long* __restrict__ sums) {
+    const int64_t rows_per_block = 256;
+    const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+    const int64_t r1 = min(W, r0 + rows_per_block);
+    for (int c = threadIdx.x; c < C; c += blockDim.x) {
+        long long acc = 0;
+        for (int64_t r = r0; r < r1; ++r) acc += win[r * C + c];
+        atomicAdd((unsigned long long*)&sums[c], (unsigned long long)acc);
+    }
+}
+
+hipError_t launch_channel_sums(const int16_t* win, int64_t W, int C, long long* sums, hipStream_t s) {
+    hipError_t e = hipMemsetAsync(sums, 0, sizeof(long long) * C, s);
+    if (e != hipSuccess) return e;
+    int nb = (int)((W + 255) / 256);
+    hipLaunchKernelGGL(channel_sums_kernel, dim3(nb), dim3(256), 0, s, win, W, C, sums);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------------
+// CAR: one wavefront per sample; exact lower median via binary search over order-preserving keys
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t f2key(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key2f(uint32_t k) {
+    uint32_t u = (k & 0x80000000u) ? (k & 0x7fffffffu) : ~k;
+    return __uint_as_float(u);
+}
+
+constexpr int kCarSamples = 32;   // samples per block (8 waves x 4 samples)
+
+template <int VPL>
+__global__ __launch_bounds__(512) void car_kernel(const int16_t* __restrict__ win, int64_t W, int C,
+                                                  const long long* __restrict__ sums, int do_car,
+                                                  float* __restrict__ out, int64_t ldo) {
+    // tile[c][s] staged for a coalesced transposed store
+    extern __shared__ float tile[];   // [C][kCarSamples + 1]
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t w0 = (int64_t)blockIdx.x * kCarSamples;
+    const double invW = 1.0 / (double)W;
+    float mean[VPL];
+#pragma unroll
+    for (int i = 0; i < VPL; ++i) {
+        int c = lane + 64 * i;
+        mean[i] = (c < C) ? (float)((double)sums[c] * invW) : 0.f;
+    }
+    const int k = (C - 1) / 2;   // lower median rank (torch.median)
+    for (int si = wave; si < kCarSamples; si += 8) {
+        int64_t w = w0 + si;
+        if (w >= W) break;
+        float v[VPL];
+        uint32_t key[VPL];
+#pragma unroll
+        for (int i = 0; i < VPL; ++i) {
+            int c = lane + 64 * i;
+            v[i] = (c < C) ? ((float)win[w * C + c] - mean[i]) : 0.f;
+            key[i] = (c < C) ? f2key(v[i]) : 0xffffffffu;
+        }
+        float med = 0.f;
+        if (do_car) {
+            // smallest key K with #{key <= K} >= k+1
+            uint32_t lo = 0u, hi = 0xffffffffu;
+            while (lo < hi) {
+                uint32_t mid = lo + ((hi - lo) >> 1);
+                int cnt = 0;
+#pragma unroll
+                for (int i = 0; i < VPL; ++i) cnt += __popcll(__ballot(key[i] <= mid));
+                if (cnt >= k + 1) hi = mid; else lo = mid + 1;
+            }
+            med = key2f(lo);
+        }
+#pragma unroll
+        for (int i = 0; i < VPL; ++i) {
+            int c = lane + 64 * i;
+            if (c < C) tile[c * (kCarSamples + 1) + si] = v[i] - med;
+        }
+    }
+    __syncthreads();
+    // store [C][kCarSamples] rows
+    const int nvalid = (int)min<int64_t>(kCarSamples, W - w0);
+    for (int idx = threadIdx.x; idx < C * kCarSamples; idx += blockDim.x) {
+        int c = idx / kCarSamples, si = idx % kCarSamples;
+        if (si < nvalid) out[(int64_t)c * ldo + w0 + si] = tile[c * (kCarSamples + 1) + si];
+    }
+}
+
+template <int V>
+hipError_t car_launch(dim3 grid, size_t lds, hipStream_t s, const int16_t* win, int64_t W, int C,
+                      const long long* sums, int do_car, float* out, int64_t ldo) {
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)car_kernel<V>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(car_kernel<V>, grid, dim3(512), lds, s, win, W, C, sums, do_car, out, ldo);
+    return hipGetLastError();
+}
+
+hipError_t launch_car_transpose(const int16_t* win, int64_t W, int C, const long long* sums,
+                                int do_car, float* out, int64_t ldo, hipStream_t s) {
+    if (C > 1024 || C < 1) return hipErrorInvalidValue;
+    dim3 grid((unsigned)((W + kCarSamples - 1) / kCarSamples));
+    size_t lds = sizeof(float) * (size_t)C * (kCarSamples + 1);
+    int vpl = (C + 63) / 64;
+    if (vpl <= 1) return car_launch<1>(grid, lds, s, win, W, C, sums, do_car, out, ldo);
+    if (vpl <= 2) return car_launch<2>(grid, lds, s, win, W, C, sums, do_car, out, ldo);
+    if (vpl <= 4) return car_launch<4>(grid, lds, s, win, W, C, sums, do_car, out, ldo);
+    if (vpl <= 6) return car_launch<6>(grid, lds, s, win, W, C, sums, do_car, out, ldo);
+    if (vpl <= 8) return car_launch<8>(grid, lds, s, win, W, C, sums, do_car, out, ldo);
+    if (vpl <= 12) return car_launch<12>(grid, lds, s, win, W, C, sums, do_car, out, ldo);
+    return car_launch<16>(grid, lds, s, win, W, C, sums, do_car, out, ldo);
+}
+
+// ------------------------------------------------------------------------------------------------
+// 8192-point complex FFT in LDS: Stockham autosort, radix 8,8,8,8,2; 512 threads.
+// ------------------------------------------------------------------------------------------------
+constexpr int kFftThreads = 512;
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// multiply by -i
+__device__ __forceinline__ float2 cmi(float2 a) { return make_float2(a.y, -a.x); }
+
+// 4-point DFT (forward, exp(-2 pi i nk/4)), outputs in natural order
+__device__ __forceinline__ void dft4(float2& x0, float2& x1, float2& x2, float2& x3) {
+    float2 a = cadd(x0, x2), b = cadd(x1, x3), c = csub(x0, x2), d = cmi(csub(x1, x3));
+    x0 = cadd(a, b);
+    x2 = csub(a, b);
+    x1 = cadd(c, d);
+    x3 = csub(c, d);
+}
+
+// 8-point DFT (forward), natural order in and out
+__device__ __forceinline__ void dft8(float2* v) {
+    const float r = 0.70710678118654752f;
+    float2 u0 = cadd(v[0], v[4]), u1 = cadd(v[1], v[5]), u2 = cadd(v[2], v[6]), u3 = cadd(v[3], v[7]);
+    float2 d0 = csub(v[0], v[4]), d1 = csub(v[1], v[5]), d2 = csub(v[2], v[6]), d3 = csub(v[3], v[7]);
+    // d_n *= exp(-2 pi i n / 8)
+    d1 = make_float2(r * (d1.x + d1.y), r * (d1.y - d1.x));
+    d2 = cmi(d2);
+    d3 = make_float2(r * (d3.y - d3.x), -r * (d3.x + d3.y));
+    dft4(u0, u1, u2, u3);
+    dft4(d0, d1, d2, d3);
+    v[0] = u0; v[2] = u1; v[4] = u2; v[6] = u3;
+    v[1] = d0; v[3] = d1; v[5] = d2; v[7] = d3;
+}
+
+// One Stockham pass of radix R over the LDS buffer (in place: read all, barrier, write all).
+template <int R>
+__device__ __forceinline__ void stockham_pass(float2* buf, int Ns, const float2* __restrict__ tw) {
+    constexpr int N = kFftN;
+    constexpr int NB = N / R;                     // butterflies
+    constexpr int PER = NB / kFftThreads;         // butterflies per thread
+    float2 v[PER][R];
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        int j = threadIdx.x + p * kFftThreads;
+        int k = j % Ns;
+        int step = (N / (Ns * R)) * k;            // twiddle index multiplier
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            float2 x = buf[j + r * NB];
+            if (r) x = cmul(x, tw[(r * step) & (N - 1)]);
+            v[p][r] = x;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        int j = threadIdx.x + p * kFftThreads;
+        int k = j % Ns;
+        if constexpr (R == 8) {
+            dft8(v[p]);
+        } else {
+            float2 a = v[p][0], b = v[p][1];
+            v[p][0] = cadd(a, b);
+            v[p][1] = csub(a, b);
+        }
+        int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[base + r * Ns] = v[p][r];
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ void fft8192(float2* buf, const float2* __restrict__ tw) {
+    stockham_pass<8>(buf, 1, tw);
+    stockham_pass<8>(buf, 8, tw);
+    stockham_pass<8>(buf, 64, tw);
+    stockham_pass<8>(buf, 512, tw);
+    stockham_pass<2>(buf, 4096, tw);
+}
+
+__global__ __launch_bounds__(kFftThreads) void fft_rows_kernel(float2* data, const float2* __restrict__ tw) {
+    __shared__ float2 buf[kFftN];
+    float2* row = data + (int64_t)blockIdx.x * kFftN;
+    for (int i = threadIdx.x; i < kFftN; i += kFftThreads) buf[i] = row[i];
+    __syncthreads();
+    fft8192(buf, tw);
+    for (int i = threadIdx.x; i < kFftN; i += kFftThreads) row[i] = buf[i];
+}
+
+hipError_t launch_fft_rows(float2* data, int nb, const float2* tw, hipStream_t s) {
+    hipLaunchKernelGGL(fft_rows_kernel, dim3(nb), dim3(kFftThreads), 0, s, data, tw);
+    return hipGetLastError();
+}
+
+// grid: (segments, channel pairs).  Segment b produces outputs [b*V, b*V+V) with V = N-Llo-Lhi.
+__global__ __launch_bounds__(kFftThreads) void highpass_kernel(const float* __restrict__ x, int64_t ldx,
+                                                               float* __restrict__ y, int64_t ldy,
+                                                               int C, int64_t W, int Llo, int Lhi,
+                                                               const float2* __restrict__ spec,
+                                                               const float2* __restrict__ tw) {
+    __shared__ float2 buf[kFftN];
+    const int V = kFftN - Llo - Lhi;
+    const int64_t o0 = (int64_t)blockIdx.x * V;
+    const int ca = 2 * blockIdx.y;
+    const int cb = ca + 1;
+    const float* xa = x + (int64_t)ca * ldx;
+    const float* xb = x + (int64_t)cb * ldx;
+    const bool hasb = cb < C;
+    // segment sample n <- window sample (o0 - Llo + n) mod W
+    int64_t start = o0 - Llo;
+    start %= W;
+    if (start < 0) start += W;
+    for (int n = threadIdx.x; n < kFftN; n += kFftThreads) {
+        int64_t idx = start + n;
+        idx %= W;
+        buf[n] = make_float2(xa[idx], hasb ? xb[idx] : 0.f);
+    }
+    __syncthreads();
+    fft8192(buf, tw);
+    // Z = conj(S * H); the forward FFT of Z conjugated is the (scaled) inverse transform
+    for (int n = threadIdx.x; n < kFftN; n += kFftThreads) {
+        float2 z = cmul(buf[n], spec[n]);
+        buf[n] = make_float2(z.x, -z.y);
+    }
+    __syncthreads();
+    fft8192(buf, tw);
+    float* ya = y + (int64_t)ca * ldy;
+    float* yb = y + (int64_t)cb * ldy;
+    for (int i = threadIdx.x; i < V; i += kFftThreads) {
+        int64_t o = o0 + i;
+        if (o >= W) break;
+        float2 r = buf[Llo + i];
+        ya[o] = r.x;
+        if (hasb) yb[o] = -r.y;
+    }
+}
+
+hipError_t launch_highpass(const float* x, int64_t ldx, float* y, int64_t ldy, int C, int64_t W,
+                           int Llo, int Lhi, const float2* spec, const float2* tw, hipStream_t s) {
+    int V = kFftN - Llo - Lhi;
+    if (V <= 0) return hipErrorInvalidValue;
+    int nseg = (int)((W + V - 1) / V);
+    hipLaunchKernelGGL(highpass_kernel, dim3(nseg, (C + 1) / 2), dim3(kFftThreads), 0, s, x, ldx, y,
+                       ldy, C, W, Llo, Lhi, spec, tw);
+    return hipGetLastError();
+}
+
+}  // namespace lss

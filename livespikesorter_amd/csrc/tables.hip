@@ -1,0 +1,134 @@
+// Load-time tables derived from the templates, accumulated in FP64 and stored as FP32:
+//   nm[t]          = sum_{k,c} U[t,k,c]^2                            (template_matching.py:141)
+//   waves[t][m][c] = sum_k U[t,k,c] wPCA[k,m]                        (the residual update's wave,
+//                                                                     template_matching.py:197)
+//   ctc[j][l][i]   = sum_{k,m} UtU[i,j,k,m] WtW[k,m,l]               (template_matching.py:110-121)
+// with UtU[i,j,k,m] = sum_c U[i,k,c] U[j,m,c] and WtW[k,m,l] = sum_tau W[k][nt-l+tau] W[m][tau]
+// (conv1d(wPCA, wPCA, padding=nt) flipped).  The ctc layout puts the T scores of one (template,
+// lag) pair contiguously, which is the row the pursuit update streams.
+#include "lss_kernels.h"
+
+namespace lss {
+
+__global__ __launch_bounds__(256) void nm_kernel(const float* __restrict__ U, int64_t ldu, int KC,
+                                                 float* __restrict__ nm) {
+    __shared__ double red[256];
+    const float* u = U + (int64_t)blockIdx.x * ldu;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < KC; i += 256) acc += (double)u[i] * (double)u[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) nm[blockIdx.x] = (float)red[0];
+}
+
+__global__ __launch_bounds__(256) void waves_kernel(const float* __restrict__ U, int64_t ldu, const float* __restrict__ wpca,
+                                                    int T, int K, int C, int nt, float* __restrict__ waves) {
+    const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t total = (int64_t)T * nt * C;
+    if (idx >= total) return;
+    const int c = (int)(idx % C);
+    const int m = (int)((idx / C) % nt);
+    const int t = (int)(idx / ((int64_t)C * nt));
+    double acc = 0.0;
+    for (int k = 0; k < K; ++k) acc += (double)U[(int64_t)t * ldu + (int64_t)k * C + c] * (double)wpca[k * nt + m];
+    waves[idx] = (float)acc;
+}
+
+// G[(i,k)][(j,m)] = sum_c U[i,k,c] U[j,m,c]: 16x16 FP64 tiles staged in LDS; row r = i*K+k of U
+// starts at (r/K)*ldu + (r%K)*C.
+__global__ __launch_bounds__(256) void gram_kernel(const float* __restrict__ U, int64_t ldu, int K, int R, int C,
+                                                   double* __restrict__ G) {
+    __shared__ double As[16][17], Bs[16][17];
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    const int r = blockIdx.y * 16 + ty;     // row of G
+    const int q = blockIdx.x * 16 + tx;     // column of G
+    double acc = 0.0;
+    for (int c0 = 0; c0 < C; c0 += 16) {
+        const int ra = blockIdx.y * 16 + ty, rb = blockIdx.x * 16 + ty;
+        As[ty][tx] = (ra < R && c0 + tx < C) ? (double)U[(int64_t)(ra / K) * ldu + (ra % K) * C + c0 + tx] : 0.0;
+        Bs[ty][tx] = (rb < R && c0 + tx < C) ? (double)U[(int64_t)(rb / K) * ldu + (rb % K) * C + c0 + tx] : 0.0;
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < 16; ++e) acc += As[ty][e] * Bs[tx][e];
+        __syncthreads();
+    }
+    if (r < R && q < R) G[(int64_t)r * R + q] = acc;
+}
+
+// ctc[j][l][i]: one thread per (i, j), all lags
+__global__ __launch_bounds__(256) void ctc_kernel(const double* __restrict__ G, const double* __restrict__ wtw,
+                                                  int T, int K, int L, float* __restrict__ ctc) {
+    extern __shared__ double swtw[];   // [K][K][L]
+    for (int e = threadIdx.x; e < K * K * L; e += 256) swtw[e] = wtw[e];
+    __syncthreads();
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= T) return;
+    const int R = T * K;
+    double g[8 * 8];
+    for (int k = 0; k < K; ++k)
+        for (int m = 0; m < K; ++m) g[k * K + m] = G[(int64_t)(i * K + k) * R + (j * K + m)];
+    for (int l = 0; l < L; ++l) {
+        double acc = 0.0;
+        for (int k = 0; k < K; ++k)
+            for (int m = 0; m < K; ++m) acc += g[k * K + m] * swtw[(k * K + m) * L + l];
+        ctc[((int64_t)j * L + l) * T + i] = (float)acc;
+    }
+}
+
+hipError_t launch_tables(const float* U, int64_t ldu, const float* wpca, int T, int K, int C, int nt,
+                         float* nm, float* waves, float* ctc, hipStream_t s) {
+    if (K > 8) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(nm_kernel, dim3(T), dim3(256), 0, s, U, ldu, K * C, nm);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const int64_t nw = (int64_t)T * nt * C;
+    hipLaunchKernelGGL(waves_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, U, ldu, wpca, T, K, C, nt, waves);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // WtW on the host (K*K*(2nt+1) values) from the FP32 wPCA
+    const int L = 2 * nt + 1;
+    float* hw = (float*)malloc(sizeof(float) * K * nt);
+    double* hwtw = (double*)malloc(sizeof(double) * K * K * L);
+    if ((e = hipMemcpyAsync(hw, wpca, sizeof(float) * K * nt, hipMemcpyDeviceToHost, s)) != hipSuccess) { free(hw); free(hwtw); return e; }
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) { free(hw); free(hwtw); return e; }
+    for (int k = 0; k < K; ++k)
+        for (int m = 0; m < K; ++m)
+            for (int l = 0; l < L; ++l) {
+                double acc = 0.0;
+                for (int tau = 0; tau < nt; ++tau) {
+                    const int a = nt - l + tau;
+                    if (a >= 0 && a < nt) acc += (double)hw[k * nt + a] * (double)hw[m * nt + tau];
+                }
+                hwtw[(k * K + m) * L + l] = acc;
+            }
+    const int R = T * K;
+    double* dG = nullptr;
+    double* dW = nullptr;
+    if ((e = hipMalloc(&dG, sizeof(double) * (size_t)R * R)) != hipSuccess) { free(hw); free(hwtw); return e; }
+    if ((e = hipMalloc(&dW, sizeof(double) * K * K * L)) != hipSuccess) { (void)hipFree(dG); free(hw); free(hwtw); return e; }
+    e = hipMemcpyAsync(dW, hwtw, sizeof(double) * K * K * L, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(gram_kernel, dim3((R + 15) / 16, (R + 15) / 16), dim3(256), 0, s, U, ldu, K, R, C, dG);
+        e = hipGetLastError();
+    }
+    const size_t lds = sizeof(double) * K * K * L;
+    if (e == hipSuccess && lds > 65536)
+        e = hipFuncSetAttribute((const void*)ctc_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(ctc_kernel, dim3((T + 255) / 256, T), dim3(256), lds, s,
+                           dG, dW, T, K, L, ctc);
+        e = hipGetLastError();
+    }
+    hipError_t e2 = hipStreamSynchronize(s);
+    (void)hipFree(dG);
+    (void)hipFree(dW);
+    free(hw);
+    free(hwtw);
+    return e != hipSuccess ? e : e2;
+}
+
+}  // namespace lss

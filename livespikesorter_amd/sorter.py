@@ -1,0 +1,258 @@
+"""Host-side interface of the MI355X sorter, mirroring LiveSpikeSorter's SpikeSorter module.
+
+``Sorter`` is the Python face of one ``OnlineSpikesV2`` (src/SpikeSorter/OnlineSpikesV2.h:29-34):
+it is built once from Kilosort4 outputs, owns its device memory, and turns padded int16 windows
+into spikes (sample, template, amplitude, PC features, position) - the work of
+``kilosortMatchingPursuit`` + ``computeClosestClusters`` (OnlineSpikesV2.cu:823-971) with
+Kilosort4's ``run_matching``/``extract`` semantics (kilosort/template_matching.py:28-85, 128-202).
+Every call goes through liblss's C ABI; there is no CPU path.
+
+Device arrays are torch CUDA tensors (PyTorch supplies device memory only); host arrays are numpy.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import Artifacts, Params, Spikes, check, lib
+
+
+@dataclass
+class SpikeList:
+    sample: np.ndarray       # int64 recording sample of each spike
+    template: np.ndarray     # int32
+    amplitude: np.ndarray    # float32
+    features: np.ndarray     # float32 [n][nC][K]
+    x: np.ndarray
+    y: np.ndarray
+    column: np.ndarray       # int32 peak column in the padded window
+    passes: int = 0
+
+    def __len__(self):
+        return len(self.sample)
+
+    @staticmethod
+    def concat(parts):
+        if not parts:
+            return SpikeList(*(np.zeros(0) for _ in range(7)))
+        f = lambda k: np.concatenate([getattr(p, k) for p in parts])
+        return SpikeList(f("sample"), f("template"), f("amplitude"), f("features"), f("x"), f("y"),
+                         f("column"), sum(p.passes for p in parts))
+
+    def sorted(self):
+        o = np.argsort(self.sample, kind="stable")
+        return SpikeList(self.sample[o], self.template[o], self.amplitude[o], self.features[o],
+                         self.x[o], self.y[o], self.column[o], self.passes)
+
+
+def default_params(**over) -> Params:
+    p = Params()
+    check(lib().lss_default_params(ct.byref(p)), "lss_default_params")
+    for k, v in over.items():
+        if k == "precision" and isinstance(v, str):
+            v = _lib.PRECISION[v]
+        setattr(p, k, v)
+    return p
+
+
+def _f32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float32))
+
+
+def _ptr(t) -> int:
+    """Device pointer of a torch tensor (must be contiguous)."""
+    if not t.is_contiguous():
+        raise ValueError("tensor must be contiguous")
+    return t.data_ptr()
+
+
+class Sorter:
+    """One sorter handle on the current HIP device."""
+
+    def __init__(self, art: dict, max_window: int = 60122, max_passes: int = 50,
+                 precision: str = "fp32", max_spikes: int = 1 << 18, do_car: bool = True,
+                 drop_negative: bool = True):
+        U = _f32(art["U"])
+        T, K, C = U.shape
+        wpca = _f32(art["wPCA"])
+        self._keep = dict(U=U, wPCA=wpca, Wrot=_f32(art["Wrot"]),
+                          drift=None if art.get("drift") is None else _f32(art["drift"]),
+                          hp=_f32(art["hp_filter"]), xc=_f32(art["xc"]), yc=_f32(art["yc"]))
+        k = self._keep
+        self.params = default_params(
+            n_chan=C, n_templates=T, n_pcs=K, nt=wpca.shape[1], nt0min=int(art.get("nt0min", 20)),
+            n_nearest=int(art.get("nC", 10)), max_window=int(max_window), max_passes=int(max_passes),
+            max_spikes=int(max_spikes), do_car=int(do_car), precision=precision,
+            drop_negative=int(drop_negative), th_learned=float(art.get("Th", 8.0)),
+            fs=float(art.get("fs", 30000.0)))
+        a = Artifacts()
+        a.U = k["U"].ctypes.data
+        a.wPCA = k["wPCA"].ctypes.data
+        a.Wrot = k["Wrot"].ctypes.data
+        a.drift = None if k["drift"] is None else k["drift"].ctypes.data
+        a.hp_filter = k["hp"].ctypes.data
+        a.hp_len = len(k["hp"])
+        a.xc = k["xc"].ctypes.data
+        a.yc = k["yc"].ctypes.data
+        h = ct.c_void_p()
+        check(lib().lss_create(ct.byref(self.params), ct.byref(a), ct.byref(h)), "lss_create")
+        self._h = h
+        self.C, self.T, self.K, self.nt = C, T, K, wpca.shape[1]
+        self.nC = self.params.n_nearest
+
+    @classmethod
+    def from_dir(cls, path: str, **kw):
+        obj = cls.__new__(cls)
+        obj.params = default_params(**kw)
+        h = ct.c_void_p()
+        check(lib().lss_create_from_dir(str(path).encode(), ct.byref(obj.params), ct.byref(h)),
+              "lss_create_from_dir")
+        obj._h = h
+        check(lib().lss_get_params(h, ct.byref(obj.params)), "lss_get_params")
+        p = obj.params
+        obj.C, obj.T, obj.K, obj.nt, obj.nC = p.n_chan, p.n_templates, p.n_pcs, p.nt, p.n_nearest
+        obj._keep = {}
+        return obj
+
+    # -------------------------------------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            check(lib().lss_destroy(self._h), "lss_destroy")
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -------------------------------------------------------------------------------------------
+    def _out(self, cap):
+        n = int(cap)
+        bufs = dict(sample=np.zeros(n, np.int64), template=np.zeros(n, np.int32),
+                    amplitude=np.zeros(n, np.float32),
+                    features=np.zeros((n, self.nC, self.K), np.float32),
+                    x=np.zeros(n, np.float32), y=np.zeros(n, np.float32), column=np.zeros(n, np.int32))
+        s = Spikes()
+        s.capacity = n
+        s.sample = bufs["sample"].ctypes.data
+        s.template_id = bufs["template"].ctypes.data
+        s.amplitude = bufs["amplitude"].ctypes.data
+        s.features = bufs["features"].ctypes.data
+        s.x = bufs["x"].ctypes.data
+        s.y = bufs["y"].ctypes.data
+        s.column = bufs["column"].ctypes.data
+        return s, bufs
+
+    @staticmethod
+    def _list(s, bufs):
+        n = int(s.n)
+        return SpikeList(bufs["sample"][:n].copy(), bufs["template"][:n].copy(),
+                         bufs["amplitude"][:n].copy(), bufs["features"][:n].copy(),
+                         bufs["x"][:n].copy(), bufs["y"][:n].copy(), bufs["column"][:n].copy(),
+                         int(s.passes))
+
+    def sort_window(self, window, window_start: int, capacity: int | None = None) -> SpikeList:
+        """window: int16 [W][C] numpy (host) or torch CUDA tensor (device)."""
+        cap = capacity or self.params.max_spikes
+        s, bufs = self._out(cap)
+        if isinstance(window, np.ndarray):
+            w = np.ascontiguousarray(window, dtype=np.int16)
+            st = lib().lss_sort_window(self._h, w.ctypes.data, w.shape[0], 0, int(window_start), ct.byref(s))
+        else:
+            st = lib().lss_sort_window(self._h, _ptr(window), window.shape[0], 1, int(window_start), ct.byref(s))
+        check(st, "lss_sort_window")
+        return self._list(s, bufs)
+
+    def sort_recording(self, rec: np.ndarray, NT: int = 60000) -> SpikeList:
+        """All NT batches of an int16 recording [n][C], padded as Kilosort does, time-sorted
+        (kilosort/template_matching.py:44-85)."""
+        rec = np.ascontiguousarray(rec, dtype=np.int16)
+        n = rec.shape[0]
+        nb = (n + NT - 1) // NT
+        win = np.zeros((NT + 2 * self.nt, self.C), np.int16)
+        parts = []
+        for ib in range(nb):
+            ws = ct.c_int64()
+            check(lib().lss_pad_batch(rec.ctypes.data, n, self.C, NT, self.nt, ib, win.ctypes.data,
+                                      ct.byref(ws)), "lss_pad_batch")
+            parts.append(self.sort_window(win, ws.value))
+        return SpikeList.concat(parts).sorted()
+
+    def timings(self):
+        ms = np.zeros(7, np.float32)
+        check(lib().lss_last_timings(self._h, ms.ctypes.data, 7), "lss_last_timings")
+        return dict(zip(["car", "highpass", "whiten", "scores", "pursuit", "features", "total"], ms.tolist()))
+
+    # ---- stage entry points (torch CUDA tensors) ----------------------------------------------
+    def stage_car(self, win_dev, out=None):
+        import torch
+        W = win_dev.shape[0]
+        out = out if out is not None else torch.empty((self.C, W), dtype=torch.float32, device=win_dev.device)
+        check(lib().lss_stage_car(self._h, _ptr(win_dev), W, _ptr(out)), "lss_stage_car")
+        return out
+
+    def stage_filter(self, win_dev, out=None):
+        import torch
+        W = win_dev.shape[0]
+        out = out if out is not None else torch.empty((self.C, W), dtype=torch.float32, device=win_dev.device)
+        check(lib().lss_stage_filter(self._h, _ptr(win_dev), W, _ptr(out)), "lss_stage_filter")
+        return out
+
+    def stage_highpass(self, x_cw, out=None):
+        import torch
+        W = x_cw.shape[1]
+        out = out if out is not None else torch.empty_like(x_cw)
+        check(lib().lss_stage_highpass(self._h, _ptr(x_cw), W, _ptr(out)), "lss_stage_highpass")
+        return out
+
+    def stage_whiten(self, x_cw, out=None):
+        import torch
+        W = x_cw.shape[1]
+        out = out if out is not None else torch.empty((W, self.C), dtype=torch.float32, device=x_cw.device)
+        check(lib().lss_stage_whiten(self._h, _ptr(x_cw), W, _ptr(out)), "lss_stage_whiten")
+        return out
+
+    def stage_scores(self, x_wc, out=None):
+        import torch
+        W = x_wc.shape[0]
+        out = out if out is not None else torch.empty((W, self.T), dtype=torch.float32, device=x_wc.device)
+        check(lib().lss_stage_scores(self._h, _ptr(x_wc), W, _ptr(out)), "lss_stage_scores")
+        return out
+
+    def stage_pursuit(self, b_wt, x_wc, window_start: int, capacity: int | None = None) -> SpikeList:
+        """Pursuit on scores [W][T] and residual [W][C] (both updated in place)."""
+        s, bufs = self._out(capacity or self.params.max_spikes)
+        check(lib().lss_stage_pursuit(self._h, _ptr(b_wt), _ptr(x_wc), b_wt.shape[0], int(window_start),
+                                      ct.byref(s)), "lss_stage_pursuit")
+        return self._list(s, bufs)
+
+    def tables(self) -> dict:
+        T, C, nt = self.T, self.C, self.nt
+        L = 2 * nt + 1
+        nm = np.zeros(T, np.float32)
+        waves = np.zeros((T, nt, C), np.float32)
+        ctc = np.zeros((T, L, T), np.float32)
+        wmat = np.zeros((C, C), np.float32)
+        check(lib().lss_get_tables(self._h, nm.ctypes.data, waves.ctypes.data, ctc.ctypes.data,
+                                   wmat.ctypes.data), "lss_get_tables")
+        return dict(nm=nm, waves=waves, ctc=ctc, wmat=wmat)
+
+
+def pad_batch(rec: np.ndarray, ibatch: int, NT: int, nt: int):
+    """Kilosort padded window (int16 [NT+2nt][C]) and its window_start, via the C ABI."""
+    rec = np.ascontiguousarray(rec, dtype=np.int16)
+    out = np.zeros((NT + 2 * nt, rec.shape[1]), np.int16)
+    ws = ct.c_int64()
+    check(lib().lss_pad_batch(rec.ctypes.data, rec.shape[0], rec.shape[1], NT, nt, ibatch,
+                              out.ctypes.data, ct.byref(ws)), "lss_pad_batch")
+    return out, ws.value

@@ -1,0 +1,183 @@
+"""Device path (liblss through its C ABI) against the CPU oracle, stage by stage and end to end.
+
+Tolerances (written where they are checked):
+* CAR output: max |gpu - fp64| <= 1e-6 of the window's max |x| (exact mean/median up to FP32 rounding);
+* highpass, whitening, scores: max |gpu - fp64| <= 1e-5 of the max |reference|;
+* pursuit given equal scores: bit-exact spike columns, templates, amplitudes, final B and residual;
+* PC features / positions: max |gpu - oracle| <= 1e-4 of the max |oracle feature|;
+* whole window: spike F1 >= 0.999 within +-1 sample and same template, matched amplitudes within
+  1e-3 relative, against both the FP32 and the FP64 oracle.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import ks_oracle as ko
+from tests.conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+
+
+def _window(s, ib):
+    a = s.art
+    X = ko.padded_batch(s.rec, ib, a["NT"], a["nt"], torch.float64)
+    win = np.ascontiguousarray(X.numpy().T.astype(np.int16))   # [W][C] int16
+    return win, X, ko.window_start(ib, a["NT"], a["nt"])
+
+
+@pytest.fixture(scope="module")
+def sorter(synth1):
+    from livespikesorter_amd.sorter import Sorter
+    print(f"[lss] creating sorter C={synth1.art['U'].shape[2]} T={synth1.art['U'].shape[0]}", flush=True)
+    with Sorter(synth1.art, max_window=60122, max_passes=100) as s:
+        yield s
+
+
+def _wm(a, dtype):
+    return torch.as_tensor(a["drift"]).to(dtype) @ torch.as_tensor(a["Wrot"]).to(dtype)
+
+
+def test_car_stage(sorter, synth1):
+    print("[lss] test_car_stage batch 0", flush=True)
+    win, X, _ = _window(synth1, 0)
+    got = sorter.stage_car(torch.from_numpy(win).to(DEV)).cpu().numpy()
+    Xr = X - X.mean(1, keepdim=True)
+    ref = (Xr - torch.median(Xr, 0)[0]).numpy()
+    assert rel_err(got, ref) <= 1e-6
+
+
+@pytest.mark.parametrize("W", [60122, 3122, 2999, 1601, 700])
+def test_highpass_stage(sorter, W):
+    print(f"[lss] test_highpass_stage W={W}", flush=True)
+    rng = np.random.default_rng(W)
+    x = rng.normal(0, 10, (sorter.C, W)).astype(np.float32)
+    got = sorter.stage_highpass(torch.from_numpy(x).to(DEV)).cpu().numpy()
+    hp32 = torch.from_numpy(ko.highpass_taps()).float().double()   # the sorter gets float32 taps
+    ref = torch.real(torch.fft.ifft(torch.fft.fft(torch.from_numpy(x).double()) *
+                                    torch.conj(ko.fft_highpass(hp32, W))))
+    ref = torch.fft.fftshift(ref, dim=-1).numpy()
+    assert rel_err(got, ref) <= 1e-5
+
+
+def test_filter_and_whiten_chain(sorter, synth1):
+    print("[lss] test_filter_and_whiten_chain batch 1", flush=True)
+    a = synth1.art
+    win, X, _ = _window(synth1, 1)
+    filt = sorter.stage_filter(torch.from_numpy(win).to(DEV))
+    xw = sorter.stage_whiten(filt).cpu().numpy()                        # [W][C]
+    hp = torch.from_numpy(a["hp_filter"]).double()
+    ref_f = ko.filter_batch(X, hp, None)
+    assert rel_err(filt.cpu().numpy(), ref_f.numpy()) <= 1e-5
+    ref_w = (_wm(a, torch.float64) @ ref_f).numpy().T
+    assert rel_err(xw, ref_w) <= 1e-5
+
+
+def test_scores_stage(sorter, synth1):
+    print("[lss] test_scores_stage", flush=True)
+    a = synth1.art
+    _, X, _ = _window(synth1, 0)
+    hp = torch.from_numpy(a["hp_filter"]).double()
+    xf = ko.filter_batch(X, hp, _wm(a, torch.float64))
+    x32 = xf.float()
+    B = sorter.stage_scores(x32.T.contiguous().to(DEV)).cpu().numpy()   # [W][T]
+    ref = ko.project(x32.double(), torch.from_numpy(a["U"]).double(),
+                     torch.from_numpy(a["wPCA"]).double()).numpy().T
+    assert rel_err(B, ref) <= 1e-5
+
+
+def _pursuit_case(sorter, synth1, ib, max_passes=100):
+    a = synth1.art
+    _, X, ws = _window(synth1, ib)
+    hp = torch.from_numpy(a["hp_filter"]).double()
+    x32 = ko.filter_batch(X, hp, _wm(a, torch.float64)).float()
+    U32 = torch.from_numpy(a["U"])
+    w32 = torch.from_numpy(a["wPCA"])
+    B32 = ko.project(x32, U32, w32)                                     # [T][W] fp32
+    tb = sorter.tables()
+    ctc_ks = torch.from_numpy(tb["ctc"]).permute(2, 0, 1).contiguous()  # [i][j][l]
+    return x32, B32, tb, ctc_ks, ws, U32, w32
+
+
+def test_pursuit_bit_exact(sorter, synth1):
+    """Given identical FP32 scores, residual and tables, the device pursuit reproduces the
+    reference pursuit (template_matching.py:157-198) bit for bit."""
+    print("[lss] test_pursuit_bit_exact batch 0", flush=True)
+    x32, B32, tb, ctc_ks, ws, U32, w32 = _pursuit_case(sorter, synth1, 0)
+    ref = ko.run_matching(x32, U32, w32, ctc_ks, Th=synth1.art["Th"], max_passes=100,
+                          nm=torch.from_numpy(tb["nm"]), waves=torch.from_numpy(tb["waves"]), B=B32)
+    Bd = B32.T.contiguous().to(DEV)
+    Xd = x32.T.contiguous().to(DEV)
+    got = sorter.stage_pursuit(Bd, Xd, ws)
+    assert got.passes == ref.passes
+    st = ref.st.numpy()
+    # drop_negative removes spikes before sample 0 from the device list
+    samples = ko.spike_samples(ref.st, ws, synth1.art["nt"], synth1.art["nt0min"]).numpy()
+    keep = samples >= 0
+    np.testing.assert_array_equal(got.column, st[keep, 0])
+    np.testing.assert_array_equal(got.template, st[keep, 1])
+    np.testing.assert_array_equal(got.amplitude, ref.amps[keep, 0].numpy())
+    # final score matrix ([T][W] in the oracle, [W][T] on the device) and residual
+    assert np.array_equal(Bd.cpu().numpy().T, ref.B.numpy())
+    assert np.array_equal(Xd.cpu().numpy().T, ref.Xres.numpy())
+
+
+def test_features_stage(sorter, synth1):
+    print("[lss] test_features_stage batch 1", flush=True)
+    a = synth1.art
+    x32, B32, tb, ctc_ks, ws, U32, w32 = _pursuit_case(sorter, synth1, 1)
+    res = ko.run_matching(x32, U32, w32, ctc_ks, Th=a["Th"], max_passes=100,
+                          nm=torch.from_numpy(tb["nm"]), waves=torch.from_numpy(tb["waves"]), B=B32)
+    got = sorter.stage_pursuit(B32.T.contiguous().to(DEV), x32.T.contiguous().to(DEV), ws)
+    iCC, iU, Ucc = ko.prepare_extract(a["xc"].astype(np.float64), a["yc"].astype(np.float64), U32, a["nC"])
+    tF = ko.extract_features(res.Xres, res.st, res.amps, iCC, iU, Ucc, w32).numpy()
+    xs, ys = ko.spike_positions(torch.from_numpy(tF), res.st, iCC, iU, a["xc"], a["yc"])
+    assert got.features.shape == tF.shape
+    assert rel_err(got.features, tF) <= 1e-4
+    assert np.abs(got.x - xs.numpy()).max() <= 1e-3
+    assert np.abs(got.y - ys.numpy()).max() <= 1e-3
+
+
+def _match_amp(t0, y0, a0, t1, y1, a1):
+    """Relative amplitude error over spikes matched exactly (same sample and template)."""
+    d0 = {(int(t), int(y)): float(a) for t, y, a in zip(t0, y0, a0)}
+    errs = [abs(a - d0[(int(t), int(y))]) / abs(d0[(int(t), int(y))])
+            for t, y, a in zip(t1, y1, a1) if (int(t), int(y)) in d0]
+    return max(errs) if errs else 0.0, len(errs)
+
+
+@pytest.mark.parametrize("ib", [0, 1])
+def test_sort_window_vs_oracle(sorter, synth1, ib):
+    print(f"[lss] test_sort_window_vs_oracle batch {ib}", flush=True)
+    a = synth1.art
+    win, X, ws = _window(synth1, ib)
+    got = sorter.sort_window(win, ws)
+    for dtype in (torch.float32, torch.float64):
+        ref = ko.sort_window(X, a, ws, dtype=dtype, max_passes=100)
+        f1 = ko.spike_f1(ref.samples, ref.templates, got.sample, got.template, dt=1)
+        assert f1 >= 0.999, (dtype, f1, len(ref.samples), len(got.sample))
+        err, n = _match_amp(ref.samples, ref.templates, ref.amps, got.sample, got.template, got.amplitude)
+        assert n >= 0.99 * len(ref.samples)
+        assert err <= 1e-3, (dtype, err)
+
+
+def test_sort_recording_vs_ground_truth(sorter, synth1):
+    print("[lss] test_sort_recording_vs_ground_truth", flush=True)
+    got = sorter.sort_recording(synth1.rec, NT=synth1.art["NT"])
+    assert np.all(np.diff(got.sample) >= 0)
+    f1 = ko.spike_f1(synth1.gt_times, synth1.gt_units, got.sample, got.template, dt=2)
+    assert f1 >= 0.9, f1
+
+
+def test_capacity_error(synth1):
+    print("[lss] test_capacity_error", flush=True)
+    from livespikesorter_amd._lib import LssError
+    from livespikesorter_amd.sorter import Sorter
+    win, _, ws = _window(synth1, 0)
+    with Sorter(synth1.art, max_window=60122, max_spikes=16) as s:
+        with pytest.raises(LssError, match="CAPACITY"):
+            s.sort_window(win, ws)
+        # the handle stays usable with an output buffer large enough for nothing: window too long
+        with pytest.raises(LssError, match="ARG"):
+            s.sort_window(np.zeros((60123, win.shape[1]), np.int16), 0)
