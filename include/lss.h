@@ -148,8 +148,9 @@ lss_status lss_stage_pursuit(void* handle, float* d_b_wt, float* d_x_wc, int64_t
  * (ctc[j][l][i] = Kilosort ctc[i][j][l]), wmat [C][C] = M @ Wrot; any may be NULL. */
 lss_status lss_get_tables(void* handle, float* nm, float* waves, float* ctc, float* wmat);
 
-/* Times of the last lss_sort_window's stages in ms (HIP events): [0] CAR, [1] highpass,
- * [2] whitening, [3] projection+scores, [4] pursuit, [5] features, [6] total; n <= 8. */
+/* Times of the last lss_sort_window's stages in ms (HIP events on the handle's stream):
+ * [0] ingest + CAR, [1] highpass, [2] whitening GEMM, [3] wPCA projection, [4] U.Xproj GEMM,
+ * [5] pursuit, [6] features + positions, [7] total; n <= 8. */
 lss_status lss_last_timings(void* handle, float* ms, int32_t n);
 
 #ifdef __cplusplus

@@ -79,7 +79,7 @@ struct Handle {
     int* h_counters = nullptr;   // pinned
 
     hipEvent_t ev[8] = {};
-    float last_ms[8] = {};
+    float last_ms[8] = {};   // car, highpass, whiten, project, scores GEMM, pursuit, features, total
     bool alive = false;
 
     ~Handle() {
@@ -340,8 +340,9 @@ lss_status run_whiten(Handle& h, const float* in_cw, float* out_wc, int64_t W) {
     return LSS_OK;
 }
 
-lss_status run_scores(Handle& h, const float* x_wc, float* b_wt, int64_t W) {
+lss_status run_scores(Handle& h, const float* x_wc, float* b_wt, int64_t W, hipEvent_t mid = nullptr) {
     HIP_TRY(lss::launch_project(x_wc, W, h.C, h.wpca.p, h.K, h.nt, h.xp.p, h.ldq, h.stream));
+    if (mid) HIP_TRY(hipEventRecord(mid, h.stream));
     const int KC = h.K * h.C;
     if (h.prm.precision == LSS_PREC_FP32)
         HIP_TRY(lss::launch_gemm_f32(h.xp.p, h.ldq, 0, h.U.p, h.ldq, b_wt, h.T, (int)W, h.T, KC, h.stream));
@@ -602,16 +603,16 @@ lss_status lss_sort_window(void* handle, const int16_t* window, int64_t W, int32
     HIP_TRY(hipEventRecord(h.ev[2], h.stream));
     if (lss_status st = run_whiten(h, h.xhp.p, h.xw.p, W)) return st;
     HIP_TRY(hipEventRecord(h.ev[3], h.stream));
-    if (lss_status st = run_scores(h, h.xw.p, h.B.p, W)) return st;
-    HIP_TRY(hipEventRecord(h.ev[4], h.stream));
+    if (lss_status st = run_scores(h, h.xw.p, h.B.p, W, h.ev[4])) return st;
+    HIP_TRY(hipEventRecord(h.ev[5], h.stream));
     int n = 0, passes = 0;
     if (lss_status st = run_pursuit(h, h.B.p, h.xw.p, W, &n, &passes)) return st;
-    HIP_TRY(hipEventRecord(h.ev[5], h.stream));
-    if (lss_status st = run_features(h, h.xw.p, W, n)) return st;
     HIP_TRY(hipEventRecord(h.ev[6], h.stream));
-    HIP_TRY(hipEventSynchronize(h.ev[6]));
-    for (int i = 0; i < 6; ++i) HIP_TRY(hipEventElapsedTime(&h.last_ms[i], h.ev[i], h.ev[i + 1]));
-    HIP_TRY(hipEventElapsedTime(&h.last_ms[6], h.ev[0], h.ev[6]));
+    if (lss_status st = run_features(h, h.xw.p, W, n)) return st;
+    HIP_TRY(hipEventRecord(h.ev[7], h.stream));
+    HIP_TRY(hipEventSynchronize(h.ev[7]));
+    for (int i = 0; i < 7; ++i) HIP_TRY(hipEventElapsedTime(&h.last_ms[i], h.ev[i], h.ev[i + 1]));
+    HIP_TRY(hipEventElapsedTime(&h.last_ms[7], h.ev[0], h.ev[7]));
     return emit_spikes(h, n, passes, window_start, out);
 }
 

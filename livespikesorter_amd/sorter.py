@@ -48,6 +48,9 @@ class SpikeList:
                          self.x[o], self.y[o], self.column[o], self.passes)
 
 
+TIMING_KEYS = ["car", "highpass", "whiten", "project", "scores_gemm", "pursuit", "features", "total"]
+
+
 def default_params(**over) -> Params:
     p = Params()
     check(lib().lss_default_params(ct.byref(p)), "lss_default_params")
@@ -189,9 +192,9 @@ class Sorter:
         return SpikeList.concat(parts).sorted()
 
     def timings(self):
-        ms = np.zeros(7, np.float32)
-        check(lib().lss_last_timings(self._h, ms.ctypes.data, 7), "lss_last_timings")
-        return dict(zip(["car", "highpass", "whiten", "scores", "pursuit", "features", "total"], ms.tolist()))
+        ms = np.zeros(8, np.float32)
+        check(lib().lss_last_timings(self._h, ms.ctypes.data, 8), "lss_last_timings")
+        return dict(zip(TIMING_KEYS, ms.tolist()))
 
     # ---- stage entry points (torch CUDA tensors) ----------------------------------------------
     def stage_car(self, win_dev, out=None):

@@ -221,17 +221,16 @@ def build_artifacts(cfg: SynthConfig, rec, shapes, foot, xc, yc) -> dict:
             wPCA[k] = -wPCA[k]
 
     # whitened templates at the median amplitude, projected on wPCA -> U [T][K][C]
+    # the raw template amp0 * shape[m] * foot[c] is separable, so its highpassed, whitened PC
+    # projection is amp0 * (wPCA @ hp(shape))[k] * (Wm @ foot)[c]
     amp0 = np.exp(cfg.amp_mu)
     T, C = shapes.shape[0], len(xc)
     U = np.zeros((T, K, C))
     for u in range(T):
-        raw = amp0 * shapes[u][:, None] * foot[u][None, :]           # [nt][C]
-        z = np.zeros((nt + 1600, C))
-        z[800:800 + nt] = raw
-        from scipy.signal import fftconvolve
-        hp = fftconvolve(z, g[::-1, None], mode="same", axes=0)[800:800 + nt]
-        wf = hp @ Wm.T                                              # [nt][C] whitened
-        U[u] = wPCA @ wf                                            # [K][C]
+        z = np.zeros(nt + 1600)
+        z[800:800 + nt] = shapes[u]
+        hp = np.convolve(z, g[::-1], mode="same")[800:800 + nt]     # [nt]
+        U[u] = amp0 * np.outer(wPCA @ hp, Wm @ foot[u])              # [K][C]
 
     return dict(
         U=U.astype(np.float32), wPCA=wPCA.astype(np.float32), Wrot=Wrot.astype(np.float32),

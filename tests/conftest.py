@@ -22,10 +22,10 @@ def synth1():
 
 @pytest.fixture(scope="session")
 def small_synth():
-    """A reduced workload for fast CPU checks: 16 ch, 6 units, 20000 samples."""
+    """A reduced workload for fast CPU checks: 32 ch, 8 units, 2 x 10000 samples."""
     from livespikesorter_amd import synth
-    return synth.generate(synth.SynthConfig(n_chan=16, n_units=6, n_samples=20000, NT=10000,
-                                            nrange=8, nC=4, seed=3))
+    return synth.generate(synth.SynthConfig(n_chan=32, n_units=8, n_samples=20000, NT=10000,
+                                            nrange=16, nC=4, seed=3))
 
 
 def rel_err(a, b):

@@ -1,0 +1,60 @@
+"""C ABI checks that need no GPU: the library loads, exports every entry point include/lss.h
+declares, and its host-only functions (defaults, batch padding, error reporting) behave."""
+import ctypes as ct
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import ks_oracle as ko
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def _declared():
+    hdr = (ROOT / "include" / "lss.h").read_text()
+    return sorted(set(re.findall(r"\b(lss_[a-z0-9_]+)\s*\(", hdr)))
+
+
+def test_library_exports_every_declared_symbol():
+    from livespikesorter_amd import _lib
+    L = _lib.lib()
+    names = _declared()
+    assert len(names) >= 18
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(_lib.EXPORTS) == set(names)
+
+
+def test_default_params():
+    from livespikesorter_amd.sorter import default_params
+    p = default_params()
+    assert (p.nt, p.nt0min, p.n_pcs, p.n_nearest, p.max_passes) == (61, 20, 6, 10, 50)
+    assert p.th_learned == 8.0 and p.max_window == 60122
+
+
+@pytest.mark.parametrize("n", [130000, 120000, 60000, 70])
+def test_pad_batch_matches_oracle(n):
+    from livespikesorter_amd.sorter import pad_batch
+    rng = np.random.default_rng(n)
+    rec = rng.integers(-3000, 3000, (n, 5)).astype(np.int16)
+    NT, nt = 60000, 61
+    for ib in range(-(-n // NT)):
+        win, ws = pad_batch(rec, ib, NT, nt)
+        ref = ko.padded_batch(rec, ib, NT, nt, torch.float64).numpy().T
+        np.testing.assert_array_equal(win, ref.astype(np.int16))
+        assert ws == ko.window_start(ib, NT, nt)
+
+
+def test_errors_are_reported_not_thrown():
+    from livespikesorter_amd import _lib
+    L = _lib.lib()
+    out = np.zeros((10, 2), np.int16)
+    rec = np.zeros((5, 2), np.int16)
+    ws = ct.c_int64()
+    st = L.lss_pad_batch(rec.ctypes.data, 5, 2, 4, 1, 7, out.ctypes.data, ct.byref(ws))
+    assert st == 1 and b"ibatch" in L.lss_last_error()
+    assert L.lss_destroy(None) == 1
+    assert L.lss_abi_version() == 1
