@@ -58,7 +58,10 @@ struct Handle {
     bool own_stream = false;
     int C = 0, T = 0, K = 0, nt = 0, nC = 0;
     int64_t Wmax = 0, ldw = 0;
-    int ldq = 0, ldm = 0;   // row strides of U/Xp (K*C) and of the whitening matrix (C), multiples of 4
+    // row strides, multiples of 4 (float4 access): U/Xp rows (K*C), whitening matrix, X, waves (C),
+    // B, ctc, nm (T)
+    int ldq = 0, ldm = 0;
+    int64_t ldC = 0, ldT = 0;
 
     // tables
     DevBuf<float> U, wpca, wmat, nm, waves, ctc, Ucc, xc, yc;
@@ -73,7 +76,8 @@ struct Handle {
     DevBuf<int16_t> win;
     DevBuf<long long> sums;
     DevBuf<float> xcw, xhp, xw, xp, B, cfmax;
-    DevBuf<int> imax, blockcnt;
+    DevBuf<int> imax;
+    DevBuf<unsigned char> flags;
     DevBuf<int> sp_col, sp_tmp, counters;
     DevBuf<float> sp_amp, tF, xs, ys;
     int* h_counters = nullptr;   // pinned
@@ -92,7 +96,7 @@ struct Handle {
 
 lss_status check_params(const lss_params& p) {
     if (p.n_chan < 1 || p.n_chan > 1024) return fail(LSS_ERR_ARG, "n_chan must be in [1, 1024]");
-    if (p.n_templates < 1) return fail(LSS_ERR_ARG, "n_templates must be >= 1");
+    if (p.n_templates < 1 || p.n_templates > 1024) return fail(LSS_ERR_ARG, "n_templates must be in [1, 1024]");
     if (p.n_pcs < 1 || p.n_pcs > 8) return fail(LSS_ERR_ARG, "n_pcs must be in [1, 8]");
     if (p.nt < 3 || p.nt > 127 || (p.nt % 2) == 0) return fail(LSS_ERR_ARG, "nt must be odd in [3, 127]");
     if (p.nt0min < 0 || p.nt0min >= p.nt) return fail(LSS_ERR_ARG, "nt0min must be in [0, nt)");
@@ -196,6 +200,8 @@ lss_status create_impl(const lss_params* p, const lss_artifacts* a, Handle** out
     h->ldw = (h->Wmax + 63) / 64 * 64;
     h->ldq = (K * C + 3) / 4 * 4;
     h->ldm = (C + 3) / 4 * 4;
+    h->ldC = h->ldm;
+    h->ldT = (T + 3) / 4 * 4;
     const int L = 2 * nt + 1;
     const int64_t Wm = h->Wmax;
 
@@ -203,9 +209,12 @@ lss_status create_impl(const lss_params* p, const lss_artifacts* a, Handle** out
     HIP_TRY(h->U.alloc((size_t)T * h->ldq));
     HIP_TRY(h->wpca.alloc((size_t)K * nt));
     HIP_TRY(h->wmat.alloc((size_t)C * h->ldm));
-    HIP_TRY(h->nm.alloc(T));
-    HIP_TRY(h->waves.alloc((size_t)T * nt * C));
-    HIP_TRY(h->ctc.alloc((size_t)T * L * T));
+    HIP_TRY(h->nm.alloc(h->ldT));
+    HIP_TRY(h->waves.alloc((size_t)T * nt * h->ldC));
+    HIP_TRY(h->ctc.alloc((size_t)T * L * h->ldT));
+    HIP_TRY(hipMemset(h->nm.p, 0, sizeof(float) * h->ldT));
+    HIP_TRY(hipMemset(h->waves.p, 0, sizeof(float) * T * nt * h->ldC));
+    HIP_TRY(hipMemset(h->ctc.p, 0, sizeof(float) * (size_t)T * L * h->ldT));
     HIP_TRY(h->Ucc.alloc((size_t)nC * T * K));
     HIP_TRY(h->xc.alloc(C));
     HIP_TRY(h->yc.alloc(C));
@@ -282,27 +291,31 @@ lss_status create_impl(const lss_params* p, const lss_artifacts* a, Handle** out
         HIP_TRY(hipMemcpy(h->iU.p, iU.data(), sizeof(int) * T, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(h->Ucc.p, Ucc.data(), sizeof(float) * nC * T * K, hipMemcpyHostToDevice));
     }
-    HIP_TRY(lss::launch_tables(h->U.p, h->ldq, h->wpca.p, T, K, C, nt, h->nm.p, h->waves.p, h->ctc.p, h->stream));
+    HIP_TRY(lss::launch_tables(h->U.p, h->ldq, h->wpca.p, T, K, C, nt, h->nm.p, h->waves.p, h->ldC, h->ctc.p, h->ldT,
+                               h->stream));
 
     // ---- window buffers ----
     HIP_TRY(h->win.alloc((size_t)Wm * C));
     HIP_TRY(h->sums.alloc(C));
     HIP_TRY(h->xcw.alloc((size_t)C * h->ldw));
     HIP_TRY(h->xhp.alloc((size_t)C * h->ldw));
-    HIP_TRY(h->xw.alloc((size_t)Wm * C));
+    HIP_TRY(h->xw.alloc((size_t)Wm * h->ldC));
     HIP_TRY(h->xp.alloc((size_t)Wm * h->ldq));
-    HIP_TRY(h->B.alloc((size_t)Wm * T));
+    HIP_TRY(h->B.alloc((size_t)Wm * h->ldT));
+    HIP_TRY(hipMemset(h->B.p, 0, sizeof(float) * Wm * h->ldT));
+    HIP_TRY(hipMemset(h->xw.p, 0, sizeof(float) * Wm * h->ldC));
     HIP_TRY(h->cfmax.alloc(Wm));
     HIP_TRY(h->imax.alloc(Wm));
-    HIP_TRY(h->blockcnt.alloc((Wm + lss::kPeakCols - 1) / lss::kPeakCols));
+    HIP_TRY(h->flags.alloc((size_t)(Wm + 65535) / 65536 * 65536));
+    HIP_TRY(hipMemset(h->flags.p, 0, (size_t)(Wm + 65535) / 65536 * 65536));
     HIP_TRY(h->sp_col.alloc(p->max_spikes));
     HIP_TRY(h->sp_tmp.alloc(p->max_spikes));
     HIP_TRY(h->sp_amp.alloc(p->max_spikes));
-    HIP_TRY(h->counters.alloc(4));
+    HIP_TRY(h->counters.alloc(8));
     HIP_TRY(h->tF.alloc((size_t)p->max_spikes * nC * K));
     HIP_TRY(h->xs.alloc(p->max_spikes));
     HIP_TRY(h->ys.alloc(p->max_spikes));
-    HIP_TRY(hipHostMalloc((void**)&h->h_counters, sizeof(int) * 4, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&h->h_counters, sizeof(int) * 8, hipHostMallocDefault));
     for (auto& e : h->ev) HIP_TRY(hipEventCreate(&e));
     HIP_TRY(hipStreamSynchronize(h->stream));
     h->alive = true;
@@ -333,59 +346,65 @@ lss_status run_highpass(Handle& h, const float* in, float* out, int64_t W) {
 lss_status run_whiten(Handle& h, const float* in_cw, float* out_wc, int64_t W) {
     // Xw[w][c'] = sum_c X[c][w] Wm[c'][c]
     if (h.prm.precision == LSS_PREC_FP32)
-        HIP_TRY(lss::launch_gemm_f32(in_cw, h.ldw, 1, h.wmat.p, h.ldm, out_wc, h.C, (int)W, h.C, h.C, h.stream));
+        HIP_TRY(lss::launch_gemm_f32(in_cw, h.ldw, 1, h.wmat.p, h.ldm, out_wc, h.ldC, (int)W, h.C, h.C, h.stream));
     else
-        HIP_TRY(lss::launch_gemm_lp(h.prm.precision, in_cw, h.ldw, 1, h.wmat.p, h.ldm, out_wc, h.C, (int)W, h.C, h.C,
+        HIP_TRY(lss::launch_gemm_lp(h.prm.precision, in_cw, h.ldw, 1, h.wmat.p, h.ldm, out_wc, h.ldC, (int)W, h.C, h.C,
                                     h.stream));
     return LSS_OK;
 }
 
 lss_status run_scores(Handle& h, const float* x_wc, float* b_wt, int64_t W, hipEvent_t mid = nullptr) {
-    HIP_TRY(lss::launch_project(x_wc, W, h.C, h.wpca.p, h.K, h.nt, h.xp.p, h.ldq, h.stream));
+    HIP_TRY(lss::launch_project(x_wc, h.ldC, W, h.C, h.wpca.p, h.K, h.nt, h.xp.p, h.ldq, h.stream));
     if (mid) HIP_TRY(hipEventRecord(mid, h.stream));
     const int KC = h.K * h.C;
     if (h.prm.precision == LSS_PREC_FP32)
-        HIP_TRY(lss::launch_gemm_f32(h.xp.p, h.ldq, 0, h.U.p, h.ldq, b_wt, h.T, (int)W, h.T, KC, h.stream));
+        HIP_TRY(lss::launch_gemm_f32(h.xp.p, h.ldq, 0, h.U.p, h.ldq, b_wt, h.ldT, (int)W, h.T, KC, h.stream));
     else
-        HIP_TRY(lss::launch_gemm_lp(h.prm.precision, h.xp.p, h.ldq, 0, h.U.p, h.ldq, b_wt, h.T, (int)W, h.T, KC,
+        HIP_TRY(lss::launch_gemm_lp(h.prm.precision, h.xp.p, h.ldq, 0, h.U.p, h.ldq, b_wt, h.ldT, (int)W, h.T, KC,
                                     h.stream));
     return LSS_OK;
 }
 
-// pursuit over B/X, returns total spikes and passes
-lss_status run_pursuit(Handle& h, float* B, float* X, int64_t W, int* n_total, int* passes) {
+constexpr int kUpdateGrid = 8192;    // workgroups of each update launch (grid-stride over work items)
+constexpr int kFeatureGrid = 1024;
+
+// Pursuit over the handle's B [W][ldT] and residual X [W][ldC]: max_passes passes enqueued with no
+// host round trip (counts stay on the device), then the features of every spike.
+lss_status run_pursuit(Handle& h, float* B, float* X, int64_t W) {
     lss::PursuitArgs a{};
     a.B = B; a.X = X; a.nm = h.nm.p; a.ctc = h.ctc.p; a.waves = h.waves.p;
-    a.cfmax = h.cfmax.p; a.imax = h.imax.p; a.blockcnt = h.blockcnt.p;
+    a.cfmax = h.cfmax.p; a.imax = h.imax.p; a.flags = h.flags.p;
     a.sp_col = h.sp_col.p; a.sp_tmp = h.sp_tmp.p; a.sp_amp = h.sp_amp.p; a.counters = h.counters.p;
-    a.W = W; a.T = h.T; a.C = h.C; a.nt = h.nt; a.cap = h.prm.max_spikes;
+    a.W = W; a.ldT = h.ldT; a.ldC = h.ldC; a.T = h.T; a.C = h.C; a.nt = h.nt; a.cap = h.prm.max_spikes;
     a.th2 = h.prm.th_learned * h.prm.th_learned;
-    HIP_TRY(hipMemsetAsync(h.counters.p, 0, sizeof(int) * 4, h.stream));
+    HIP_TRY(lss::launch_reset_counters(h.counters.p, h.stream));
     HIP_TRY(lss::launch_score_all(a, h.stream));
-    int p = 0;
-    int total = 0;
-    for (; p < h.prm.max_passes; ++p) {
+    for (int p = 0; p < h.prm.max_passes; ++p) {
         HIP_TRY(lss::launch_peaks(a, h.stream));
-        HIP_TRY(hipMemcpyAsync(h.h_counters, h.counters.p, sizeof(int) * 4, hipMemcpyDeviceToHost, h.stream));
-        HIP_TRY(hipStreamSynchronize(h.stream));
-        const int n_new = h.h_counters[1];
-        total = h.h_counters[0] + n_new;
-        if (h.h_counters[2]) return fail(LSS_ERR_CAPACITY, "spike capacity (max_spikes) exceeded");
-        if (n_new == 0) break;
-        HIP_TRY(lss::launch_update(a, n_new, h.stream));
+        HIP_TRY(lss::launch_update(a, kUpdateGrid, h.stream));
     }
-    *n_total = total;
-    *passes = p;
     return LSS_OK;
 }
 
-lss_status run_features(Handle& h, const float* X, int64_t W, int n) {
+lss_status run_features(Handle& h, const float* X, int64_t W) {
     lss::FeatureArgs f{};
     f.X = X; f.wpca = h.wpca.p; f.iCC = h.iCC.p; f.iU = h.iU.p; f.Ucc = h.Ucc.p;
     f.xc = h.xc.p; f.yc = h.yc.p; f.sp_col = h.sp_col.p; f.sp_tmp = h.sp_tmp.p; f.sp_amp = h.sp_amp.p;
+    f.counters = h.counters.p;
     f.tF = h.tF.p; f.xs = h.xs.p; f.ys = h.ys.p;
-    f.n = n; f.C = h.C; f.T = h.T; f.K = h.K; f.nt = h.nt; f.nC = h.nC; f.W = W;
-    HIP_TRY(lss::launch_features(f, h.stream));
+    f.n = 0; f.cap = h.prm.max_spikes; f.C = h.C; f.T = h.T; f.K = h.K; f.nt = h.nt; f.nC = h.nC;
+    f.W = W; f.ldC = h.ldC;
+    HIP_TRY(lss::launch_features(f, kFeatureGrid, h.stream));
+    return LSS_OK;
+}
+
+// Spike count and pass count of the last pursuit (synchronises the stream).
+lss_status read_counts(Handle& h, int* n, int* passes) {
+    HIP_TRY(hipMemcpyAsync(h.h_counters, h.counters.p, sizeof(int) * 8, hipMemcpyDeviceToHost, h.stream));
+    HIP_TRY(hipStreamSynchronize(h.stream));
+    if (h.h_counters[2]) return fail(LSS_ERR_CAPACITY, "spike capacity (max_spikes) exceeded");
+    *n = h.h_counters[0] + h.h_counters[1];
+    *passes = h.h_counters[4];
     return LSS_OK;
 }
 
@@ -605,12 +624,12 @@ lss_status lss_sort_window(void* handle, const int16_t* window, int64_t W, int32
     HIP_TRY(hipEventRecord(h.ev[3], h.stream));
     if (lss_status st = run_scores(h, h.xw.p, h.B.p, W, h.ev[4])) return st;
     HIP_TRY(hipEventRecord(h.ev[5], h.stream));
-    int n = 0, passes = 0;
-    if (lss_status st = run_pursuit(h, h.B.p, h.xw.p, W, &n, &passes)) return st;
+    if (lss_status st = run_pursuit(h, h.B.p, h.xw.p, W)) return st;
     HIP_TRY(hipEventRecord(h.ev[6], h.stream));
-    if (lss_status st = run_features(h, h.xw.p, W, n)) return st;
+    if (lss_status st = run_features(h, h.xw.p, W)) return st;
     HIP_TRY(hipEventRecord(h.ev[7], h.stream));
-    HIP_TRY(hipEventSynchronize(h.ev[7]));
+    int n = 0, passes = 0;
+    if (lss_status st = read_counts(h, &n, &passes)) return st;
     for (int i = 0; i < 7; ++i) HIP_TRY(hipEventElapsedTime(&h.last_ms[i], h.ev[i], h.ev[i + 1]));
     HIP_TRY(hipEventElapsedTime(&h.last_ms[7], h.ev[0], h.ev[7]));
     return emit_spikes(h, n, passes, window_start, out);
@@ -652,13 +671,21 @@ lss_status lss_stage_highpass(void* handle, const float* d_in_cw, int64_t W, flo
     return LSS_OK;
 }
 
+// user [W][cols] <-> internal [W][ld] (device to device)
+lss_status copy_rows(Handle& h, float* dst, int64_t ldd, const float* src, int64_t lds, int64_t cols, int64_t W) {
+    HIP_TRY(hipMemcpy2DAsync(dst, sizeof(float) * ldd, src, sizeof(float) * lds, sizeof(float) * cols, W,
+                             hipMemcpyDeviceToDevice, h.stream));
+    return LSS_OK;
+}
+
 lss_status lss_stage_whiten(void* handle, const float* d_in_cw, int64_t W, float* d_out_wc) {
     if (!handle || !d_in_cw || !d_out_wc) return fail(LSS_ERR_ARG, "null argument");
     Handle& h = *as_handle(handle);
     if (lss_status st = check_window(h, W)) return st;
     HIP_TRY(hipMemcpy2DAsync(h.xhp.p, sizeof(float) * h.ldw, d_in_cw, sizeof(float) * W, sizeof(float) * W, h.C,
                              hipMemcpyDeviceToDevice, h.stream));
-    if (lss_status st = run_whiten(h, h.xhp.p, d_out_wc, W)) return st;
+    if (lss_status st = run_whiten(h, h.xhp.p, h.xw.p, W)) return st;
+    if (lss_status st = copy_rows(h, d_out_wc, h.C, h.xw.p, h.ldC, h.C, W)) return st;
     HIP_TRY(hipStreamSynchronize(h.stream));
     return LSS_OK;
 }
@@ -667,7 +694,9 @@ lss_status lss_stage_scores(void* handle, const float* d_x_wc, int64_t W, float*
     if (!handle || !d_x_wc || !d_b_wt) return fail(LSS_ERR_ARG, "null argument");
     Handle& h = *as_handle(handle);
     if (lss_status st = check_window(h, W)) return st;
-    if (lss_status st = run_scores(h, d_x_wc, d_b_wt, W)) return st;
+    if (lss_status st = copy_rows(h, h.xw.p, h.ldC, d_x_wc, h.C, h.C, W)) return st;
+    if (lss_status st = run_scores(h, h.xw.p, h.B.p, W)) return st;
+    if (lss_status st = copy_rows(h, d_b_wt, h.T, h.B.p, h.ldT, h.T, W)) return st;
     HIP_TRY(hipStreamSynchronize(h.stream));
     return LSS_OK;
 }
@@ -677,9 +706,14 @@ lss_status lss_stage_pursuit(void* handle, float* d_b_wt, float* d_x_wc, int64_t
     if (!handle || !d_b_wt || !d_x_wc || !out) return fail(LSS_ERR_ARG, "null argument");
     Handle& h = *as_handle(handle);
     if (lss_status st = check_window(h, W)) return st;
+    if (lss_status st = copy_rows(h, h.B.p, h.ldT, d_b_wt, h.T, h.T, W)) return st;
+    if (lss_status st = copy_rows(h, h.xw.p, h.ldC, d_x_wc, h.C, h.C, W)) return st;
+    if (lss_status st = run_pursuit(h, h.B.p, h.xw.p, W)) return st;
+    if (lss_status st = run_features(h, h.xw.p, W)) return st;
+    if (lss_status st = copy_rows(h, d_b_wt, h.T, h.B.p, h.ldT, h.T, W)) return st;
+    if (lss_status st = copy_rows(h, d_x_wc, h.C, h.xw.p, h.ldC, h.C, W)) return st;
     int n = 0, passes = 0;
-    if (lss_status st = run_pursuit(h, d_b_wt, d_x_wc, W, &n, &passes)) return st;
-    if (lss_status st = run_features(h, d_x_wc, W, n)) return st;
+    if (lss_status st = read_counts(h, &n, &passes)) return st;
     return emit_spikes(h, n, passes, window_start, out);
 }
 
@@ -688,8 +722,12 @@ lss_status lss_get_tables(void* handle, float* nm, float* waves, float* ctc, flo
     Handle& h = *as_handle(handle);
     const int L = 2 * h.nt + 1;
     if (nm) HIP_TRY(hipMemcpyAsync(nm, h.nm.p, sizeof(float) * h.T, hipMemcpyDeviceToHost, h.stream));
-    if (waves) HIP_TRY(hipMemcpyAsync(waves, h.waves.p, sizeof(float) * h.T * h.nt * h.C, hipMemcpyDeviceToHost, h.stream));
-    if (ctc) HIP_TRY(hipMemcpyAsync(ctc, h.ctc.p, sizeof(float) * (size_t)h.T * L * h.T, hipMemcpyDeviceToHost, h.stream));
+    if (waves)
+        HIP_TRY(hipMemcpy2DAsync(waves, sizeof(float) * h.C, h.waves.p, sizeof(float) * h.ldC, sizeof(float) * h.C,
+                                 (size_t)h.T * h.nt, hipMemcpyDeviceToHost, h.stream));
+    if (ctc)
+        HIP_TRY(hipMemcpy2DAsync(ctc, sizeof(float) * h.T, h.ctc.p, sizeof(float) * h.ldT, sizeof(float) * h.T,
+                                 (size_t)h.T * L, hipMemcpyDeviceToHost, h.stream));
     if (wmat)
         HIP_TRY(hipMemcpy2DAsync(wmat, sizeof(float) * h.C, h.wmat.p, sizeof(float) * h.ldm, sizeof(float) * h.C, h.C,
                                  hipMemcpyDeviceToHost, h.stream));

@@ -34,37 +34,42 @@ hipError_t launch_gemm_f32(const float* A, int64_t lda, int a_km, const float* B
 // Precision variants with operands converted on the fly (prec = lss_precision).
 hipError_t launch_gemm_lp(int prec, const float* A, int64_t lda, int a_km, const float* B,
                           int64_t ldb, float* C, int64_t ldc, int M, int N, int K, hipStream_t s);
-// Xp[w][k*C+c] = sum_m X[w+m-nt/2][c] * wpca[k][m] (zero outside [0,W)), row stride ldp.
-// K <= 8, nt <= 127.
-hipError_t launch_project(const float* x_wc, int64_t W, int C, const float* wpca, int K, int nt,
-                          float* xp, int64_t ldp, hipStream_t s);
+// Xp[w][k*C+c] = sum_m X[w+m-nt/2][c] * wpca[k][m] (zero outside [0,W)); X rows have stride ldx,
+// Xp rows ldp.  K <= 8, nt <= 127.
+hipError_t launch_project(const float* x_wc, int64_t ldx, int64_t W, int C, const float* wpca, int K,
+                          int nt, float* xp, int64_t ldp, hipStream_t s);
 
 // ---- pursuit (pursuit.hip) ------------------------------------------------------------------
+// Row strides ldT (B, ctc, nm) and ldC (X, waves) are multiples of 4 (float4 access); padding
+// columns are never read as scores.
 struct PursuitArgs {
-    float* B;            // [W][T] scores (updated)
-    float* X;            // [W][C] residual (updated)
-    const float* nm;     // [T]
-    const float* ctc;    // [T][2nt+1][T]  ctc[j][l][i]
-    const float* waves;  // [T][nt][C]
+    float* B;            // [W][ldT] scores (updated)
+    float* X;            // [W][ldC] residual (updated)
+    const float* nm;     // [ldT]
+    const float* ctc;    // [T][2nt+1][ldT]  ctc[j][l][i]
+    const float* waves;  // [T][nt][ldC]
     float* cfmax;        // [W]
     int* imax;           // [W]
-    int* blockcnt;       // [ceil(W/kPeakCols)]
+    unsigned char* flags;  // [roundup(W, 65536)] peak flags of the current pass
     int* sp_col;         // [cap] peak column
     int* sp_tmp;         // [cap] template
     float* sp_amp;       // [cap]
-    int* counters;       // [0] total spikes so far, [1] spikes of the last pass, [2] overflow flag
+    // [0] spikes of committed passes, [1] spikes of the last pass, [2] capacity overflow,
+    // [3] converged (a pass found nothing), [4] passes that found spikes
+    int* counters;
     int64_t W;
+    int64_t ldT, ldC;
     int T, C, nt;
     int cap;
     float th2;
 };
-constexpr int kPeakCols = 1024;      // columns per block of the peak kernels
-hipError_t launch_score_all(const PursuitArgs& a, hipStream_t s);
-hipError_t launch_peaks(const PursuitArgs& a, hipStream_t s);        // two kernels
-hipError_t launch_update(const PursuitArgs& a, int n_new, hipStream_t s);
+hipError_t launch_reset_counters(int* counters, hipStream_t s);        // counters[0..7] = 0
+hipError_t launch_score_all(const PursuitArgs& a, hipStream_t s);     // T <= 1024
+hipError_t launch_peaks(const PursuitArgs& a, hipStream_t s);
+hipError_t launch_update(const PursuitArgs& a, int grid, hipStream_t s);
 
 struct FeatureArgs {
-    const float* X;      // [W][C] final residual
+    const float* X;      // [W][ldC] final residual
     const float* wpca;   // [K][nt]
     const int* iCC;      // [nC][C]
     const int* iU;       // [T]
@@ -74,18 +79,19 @@ struct FeatureArgs {
     const int* sp_col;
     const int* sp_tmp;
     const float* sp_amp;
+    const int* counters; // spike count = counters[0] + counters[1] (or n when NULL)
     float* tF;           // [n][nC][K]
     float* xs;           // [n]
     float* ys;           // [n]
-    int n, C, T, K, nt, nC;
-    int64_t W;
+    int n, cap, C, T, K, nt, nC;
+    int64_t W, ldC;
 };
-hipError_t launch_features(const FeatureArgs& a, hipStream_t s);
+hipError_t launch_features(const FeatureArgs& a, int grid, hipStream_t s);
 
 // ---- load-time tables in FP64 (tables.hip) --------------------------------------------------
 // nm[t] = sum U^2; waves[t][m][c] = sum_k U[t][k][c] wpca[k][m]; ctc[j][l][i] (see lss.h).
-// U rows (one per template) have stride ldu >= K*C.
+// U rows (one per template) have stride ldu >= K*C; waves rows ldC, ctc rows ldT.
 hipError_t launch_tables(const float* U, int64_t ldu, const float* wpca, int T, int K, int C, int nt,
-                         float* nm, float* waves, float* ctc, hipStream_t s);
+                         float* nm, float* waves, int64_t ldC, float* ctc, int64_t ldT, hipStream_t s);
 
 }  // namespace lss

@@ -13,7 +13,7 @@ constexpr int kMaxNt = 127;
 constexpr int kMaxK = 8;
 
 template <int KK>
-__global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ x, int64_t W, int C,
+__global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ x, int64_t ldx, int64_t W, int C,
                                                       const float* __restrict__ wpca, int nt,
                                                       float* __restrict__ xp, int64_t ldp) {
     __shared__ float xs[(kProjW + kMaxNt - 1) * kProjC];
@@ -28,7 +28,7 @@ __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ 
         int r = i / kProjC, cc = i % kProjC;
         int64_t w = w0 - h + r;
         int c = c0 + cc;
-        xs[i] = (w >= 0 && w < W && c < C) ? x[w * C + c] : 0.f;
+        xs[i] = (w >= 0 && w < W && c < C) ? x[w * ldx + c] : 0.f;
     }
     __syncthreads();
     const int c = c0 + lane;
@@ -50,12 +50,12 @@ __global__ __launch_bounds__(256) void project_kernel(const float* __restrict__ 
     }
 }
 
-hipError_t launch_project(const float* x_wc, int64_t W, int C, const float* wpca, int K, int nt,
-                          float* xp, int64_t ldp, hipStream_t s) {
+hipError_t launch_project(const float* x_wc, int64_t ldx, int64_t W, int C, const float* wpca, int K,
+                          int nt, float* xp, int64_t ldp, hipStream_t s) {
     if (K < 1 || K > kMaxK || nt < 1 || nt > kMaxNt) return hipErrorInvalidValue;
     dim3 grid((unsigned)((W + kProjW - 1) / kProjW), (C + kProjC - 1) / kProjC);
     switch (K) {
-#define LSS_PK(KV) case KV: hipLaunchKernelGGL(project_kernel<KV>, grid, dim3(256), 0, s, x_wc, W, C, wpca, nt, xp, ldp); break;
+#define LSS_PK(KV) case KV: hipLaunchKernelGGL(project_kernel<KV>, grid, dim3(256), 0, s, x_wc, ldx, W, C, wpca, nt, xp, ldp); break;
         LSS_PK(1) LSS_PK(2) LSS_PK(3) LSS_PK(4) LSS_PK(5) LSS_PK(6) LSS_PK(7) LSS_PK(8)
 #undef LSS_PK
     }

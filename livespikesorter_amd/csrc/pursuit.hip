@@ -1,27 +1,33 @@
-// Greedy matching pursuit over the score matrix B [W][T] (time-major: the T template scores of a
+// Greedy matching pursuit over the score matrix B [W][ldT] (time-major: the T template scores of a
 // sample are contiguous), Kilosort4 semantics (kilosort/template_matching.py:157-198):
 //
 //   Cf = relu(B)^2 / nm, zero in the first and last nt columns             (:159, :165-166)
 //   Cfmax[w], imax[w] = max / first argmax over templates                  (:167)
-//   peaks: maxpool_{2nt+1}(Cfmax)[w] > Th^2 and equal to Cfmax[w]          (:168-174)
+//   peaks: maxpool_{2nt+1}(Cfmax)[w] > Th^2 and |pool - Cfmax[w]| < 1e-9   (:168-174)
 //   amp = B[imax, w] / nm[imax]                                            (:188)
 //   B[:, w-nt..w+nt] -= amp ctc[:, t, :];  Xres[:, w-nt/2..w+nt/2] -= amp wave_t
 //   in two groups (even, then odd spike index of the pass)                 (:195-198)
 //
 // Scores are reduced with one wavefront per sample column (wavefront-shuffle argmax).  Peaks are
-// compacted in ascending column order with a two-kernel block scan.  The update is a gather: the
-// workgroup of spike s owns the columns of its window that no earlier spike's window covers and
-// applies, per column, the last covering spike of each parity (even group first) - exactly the
-// values the reference's gather-subtract-scatter leaves, including its last-write-wins result for
-// windows of one group that overlap (tied peaks).  After updating a column it rescores it, so
-// Cfmax/imax stay current without rescanning the whole matrix in later passes.
+// found and compacted in ascending column order by one workgroup that scans Cfmax in LDS tiles;
+// only columns whose own score can pass the test look at their 2nt+1 neighbourhood.  The update is
+// a gather split into (spike, column chunk) work items: a column belongs to the first spike whose
+// window covers it, and gets, per parity, the last covering spike of that parity (even group
+// first) - the values the reference's gather-subtract-scatter leaves, including its
+// last-write-wins result for windows of one group that overlap (tied peaks).  Every updated score
+// column is rescored in place, so Cfmax/imax stay current without rescanning B.
 //
-// FP contraction is off in this file: the products and differences are rounded separately, as
-// the reference's separate tensor multiply and in-place subtraction are.
+// Spike counts live on the device: the pass loop is enqueued without host round trips; a pass
+// after convergence finds no peaks and its update exits at once.
+//
+// FP contraction is off in this file: products and differences are rounded separately, as the
+// reference's separate tensor multiply and in-place subtraction are.
 #pragma clang fp contract(off)
 #include "lss_kernels.h"
 
 namespace lss {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ void argmax_merge(float& v, int& i, float v2, int i2) {
     if (v2 > v || (v2 == v && i2 < i)) { v = v2; i = i2; }
@@ -41,270 +47,451 @@ __device__ __forceinline__ float score_of(float b, float nmv) {
     return (r * r) / nmv;
 }
 
-// one wavefront per column
-__global__ __launch_bounds__(256) void score_all_kernel(const float* __restrict__ B,
-                                                        const float* __restrict__ nm, int64_t W,
-                                                        int T, int nt, float* __restrict__ cfmax,
-                                                        int* __restrict__ imax) {
+// ------------------------------------------------------------------------------------------------
+// full scan (first pass): one wavefront per column, float4 loads (ldT % 4 == 0)
+// ------------------------------------------------------------------------------------------------
+template <int NV>   // float4 vectors per lane: NV*256 >= T
+__global__ __launch_bounds__(256) void score_all_kernel(PursuitArgs a) {
     const int lane = threadIdx.x & 63;
     const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (w >= W) return;
-    float best = -1.f;
-    int bi = 0x7fffffff;
-    if (w >= nt && w < W - nt) {
-        const float* col = B + w * T;
-        for (int t = lane; t < T; t += 64) argmax_merge(best, bi, score_of(col[t], nm[t]), t);
+    if (w >= a.W) return;
+    float best = 0.f;
+    int bi = 0;
+    if (w >= a.nt && w < a.W - a.nt) {
+        best = -1.f;
+        bi = 0x7fffffff;
+        const float* col = a.B + w * a.ldT;
+        floatx4 b[NV], n[NV];
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            const int t = 4 * lane + 256 * i;
+            if (t < a.T) {
+                b[i] = *(const floatx4*)(col + t);
+                n[i] = *(const floatx4*)(a.nm + t);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            const int t = 4 * lane + 256 * i;
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (t + e < a.T) argmax_merge(best, bi, score_of(b[i][e], n[i][e]), t + e);
+        }
         wave_argmax(best, bi);
-    } else {
-        best = 0.f;
-        bi = 0;
     }
     if (lane == 0) {
-        cfmax[w] = best;
-        imax[w] = bi;
+        a.cfmax[w] = best;
+        a.imax[w] = bi;
     }
 }
 
-hipError_t launch_score_all(const PursuitArgs& a, hipStream_t s) {
-    unsigned nb = (unsigned)((a.W + 3) / 4);
-    hipLaunchKernelGGL(score_all_kernel, dim3(nb), dim3(256), 0, s, a.B, a.nm, a.W, a.T, a.nt,
-                       a.cfmax, a.imax);
-    return hipGetLastError();
-}
-
 // ------------------------------------------------------------------------------------------------
-// peaks: 256 threads x 4 columns per block
+// peaks.  A column is a peak iff pool > Th^2 and |pool - cf| < 1e-9 (pool = window max).  If any
+// window value v has fl(v - cf) >= 1e-9 then fl(pool - cf) >= fl(v - cf) (rounding is monotone),
+// so the column is not a peak: the scan stops at the first such v, nearest neighbours first.  A
+// column with cf <= Th^2 - 2e-9 cannot reach pool > Th^2 with pool - cf < 1e-9 and is skipped.
+// Columns that survive get the literal test on the full window maximum.
+//
+// peaks_flags_kernel: one thread per column over the whole chip -> flag bytes.
+// peaks_emit_kernel: one workgroup compacts the flags in ascending column order.
 // ------------------------------------------------------------------------------------------------
-constexpr int kPeakHalo = 128;   // >= nt
+constexpr int kFlagCols = 256;
+constexpr int kPeakHalo = 128;                       // >= nt
+constexpr int kEmitThreads = 1024;
+constexpr int kEmitPer = 64;                         // columns per thread per emit tile
+constexpr int kEmitTile = kEmitThreads * kEmitPer;   // 65536
 
-__device__ __forceinline__ void peak_flags(const float* __restrict__ cfmax, int64_t W, int nt,
-                                           float th2, float* lds, bool flags[4]) {
-    const int64_t c0 = (int64_t)blockIdx.x * kPeakCols;
-    for (int i = threadIdx.x; i < kPeakCols + 2 * kPeakHalo; i += 256) {
-        int64_t w = c0 - kPeakHalo + i;
-        lds[i] = (w >= 0 && w < W) ? cfmax[w] : -INFINITY;
-    }
-    __syncthreads();
+__global__ __launch_bounds__(kFlagCols) void peaks_flags_kernel(PursuitArgs a) {
+    __shared__ float lds[kFlagCols + 2 * kPeakHalo];
+    if (a.counters[3]) return;   // converged earlier: nothing can change
+    const int tid = threadIdx.x;
+    const int64_t c0 = (int64_t)blockIdx.x * kFlagCols;
+    float v[(kFlagCols + 2 * kPeakHalo) / kFlagCols];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int li = threadIdx.x * 4 + q;
-        const int64_t w = c0 + li;
-        bool f = false;
-        if (w < W) {
-            float cm = -INFINITY;
-            for (int d = -nt; d <= nt; ++d) cm = fmaxf(cm, lds[kPeakHalo + li + d]);
-            const float cf = lds[kPeakHalo + li];
-            f = (cm > th2) && (fabsf(cm - cf) < 1e-9f);
-        }
-        flags[q] = f;
+    for (int r = 0; r < (kFlagCols + 2 * kPeakHalo) / kFlagCols; ++r) {
+        const int64_t w = c0 - kPeakHalo + tid + r * kFlagCols;
+        v[r] = (w >= 0 && w < a.W) ? a.cfmax[w] : -INFINITY;
     }
+#pragma unroll
+    for (int r = 0; r < (kFlagCols + 2 * kPeakHalo) / kFlagCols; ++r) lds[tid + r * kFlagCols] = v[r];
+    __syncthreads();
+    const int64_t w = c0 + tid;
+    if (w >= a.W) return;
+    const float* p = lds + kPeakHalo + tid;
+    const float cf = p[0];
+    const float th2 = a.th2;
+    bool f = false;
+    if ((double)cf > (double)th2 - 2e-9 && !(p[-1] - cf >= 1e-9f) && !(p[1] - cf >= 1e-9f)) {
+        float cm = fmaxf(cf, fmaxf(p[-1], p[1]));
+        bool dominated = false;
+        for (int d = 2; d <= a.nt; ++d) {
+            const float vl = p[-d], vr = p[d];
+            if (vl - cf >= 1e-9f || vr - cf >= 1e-9f) { dominated = true; break; }
+            cm = fmaxf(cm, fmaxf(vl, vr));
+        }
+        f = !dominated && cm > th2 && fabsf(cm - cf) < 1e-9f;
+    }
+    a.flags[w] = f ? 1 : 0;
 }
 
-__global__ __launch_bounds__(256) void peaks_count_kernel(PursuitArgs a) {
-    __shared__ float lds[kPeakCols + 2 * kPeakHalo];
-    __shared__ int wsum[4];
-    // commit the previous pass (its update has completed: stream order)
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.counters[0] += a.counters[1];
+__global__ __launch_bounds__(kEmitThreads) void peaks_emit_kernel(PursuitArgs a) {
+    __shared__ int wsum[kEmitThreads / 64];
+    __shared__ int s_base, s_run;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) {
+        // commit the previous pass (its update has completed: stream order)
+        const int base = a.counters[0] + a.counters[1];
+        a.counters[0] = base;
         a.counters[1] = 0;
+        s_base = base;
+        s_run = 0;
     }
-    bool f[4];
-    peak_flags(a.cfmax, a.W, a.nt, a.th2, lds, f);
-    int cnt = (int)f[0] + (int)f[1] + (int)f[2] + (int)f[3];
-    for (int off = 32; off >= 1; off >>= 1) cnt += __shfl_xor(cnt, off, 64);
-    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = cnt;
     __syncthreads();
-    if (threadIdx.x == 0) a.blockcnt[blockIdx.x] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-}
-
-__global__ __launch_bounds__(256) void peaks_emit_kernel(PursuitArgs a) {
-    __shared__ float lds[kPeakCols + 2 * kPeakHalo];
-    __shared__ int wsum[4];
-    __shared__ int s_off;
-    if (threadIdx.x == 0) {
-        int off = 0;
-        for (int b = 0; b < (int)blockIdx.x; ++b) off += a.blockcnt[b];
-        s_off = off;
-        if (blockIdx.x == gridDim.x - 1) {
-            int tot = off + a.blockcnt[blockIdx.x];
-            a.counters[1] = tot;
-            if (a.counters[0] + tot > a.cap) a.counters[2] = 1;
-        }
-    }
-    bool f[4];
-    peak_flags(a.cfmax, a.W, a.nt, a.th2, lds, f);
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int cnt = (int)f[0] + (int)f[1] + (int)f[2] + (int)f[3];
-    // inclusive wave scan
-    int inc = cnt;
-    for (int off = 1; off < 64; off <<= 1) {
-        int v = __shfl_up(inc, off, 64);
-        if (lane >= off) inc += v;
-    }
-    if (lane == 63) wsum[wave] = inc;
-    __syncthreads();
-    int woff = 0;
-    for (int q = 0; q < wave; ++q) woff += wsum[q];
-    int pos = a.counters[0] + s_off + woff + inc - cnt;
-    const int64_t c0 = (int64_t)blockIdx.x * kPeakCols;
+    if (a.counters[3]) return;
+    for (int64_t c0 = 0; c0 < a.W; c0 += kEmitTile) {
+        // kEmitPer flag bytes per thread as four 16-byte loads (flags has kEmitTile padding)
+        const uint4* fp = (const uint4*)(a.flags + c0 + (int64_t)tid * kEmitPer);
+        uint4 q[kEmitPer / 16];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        if (!f[q]) continue;
-        if (pos < a.cap) {
-            const int64_t w = c0 + threadIdx.x * 4 + q;
-            const int t = a.imax[w];
-            a.sp_col[pos] = (int)w;
-            a.sp_tmp[pos] = t;
-            a.sp_amp[pos] = a.B[w * a.T + t] / a.nm[t];
+        for (int j = 0; j < kEmitPer / 16; ++j) q[j] = fp[j];
+        uint64_t bits = 0;
+#pragma unroll
+        for (int j = 0; j < kEmitPer / 16; ++j) {
+            const uint32_t wv[4] = {q[j].x, q[j].y, q[j].z, q[j].w};
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    if ((wv[k] >> (8 * b)) & 0xffu) bits |= 1ull << (16 * j + 4 * k + b);
         }
-        ++pos;
+        // bytes past W may hold flags of an earlier, longer window
+        const int64_t valid = a.W - (c0 + (int64_t)tid * kEmitPer);
+        if (valid <= 0) bits = 0;
+        else if (valid < 64) bits &= (1ull << valid) - 1;
+        const int cnt = __popcll(bits);
+        int inc = cnt;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int vv = __shfl_up(inc, off, 64);
+            if (lane >= off) inc += vv;
+        }
+        if (lane == 63) wsum[wave] = inc;
+        __syncthreads();
+        int woff = 0, tot = 0;
+        for (int k = 0; k < kEmitThreads / 64; ++k) {
+            const int vv = wsum[k];
+            if (k < wave) woff += vv;
+            tot += vv;
+        }
+        int pos = s_base + s_run + woff + inc - cnt;
+        while (bits) {
+            const int k = __ffsll((unsigned long long)bits) - 1;
+            bits &= bits - 1;
+            const int64_t w = c0 + (int64_t)tid * kEmitPer + k;
+            if (pos < a.cap && w < a.W) {
+                const int t = a.imax[w];
+                a.sp_col[pos] = (int)w;
+                a.sp_tmp[pos] = t;
+                a.sp_amp[pos] = a.B[w * a.ldT + t] / a.nm[t];
+            }
+            ++pos;
+        }
+        __syncthreads();
+        if (tid == 0) s_run += tot;
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const int n = s_run;
+        a.counters[1] = n;
+        if (s_base + n > a.cap) a.counters[2] = 1;
+        if (n == 0) a.counters[3] = 1;
+        a.counters[4] += (n > 0);
     }
 }
 
-hipError_t launch_peaks(const PursuitArgs& a, hipStream_t s) {
-    unsigned nb = (unsigned)((a.W + kPeakCols - 1) / kPeakCols);
-    hipLaunchKernelGGL(peaks_count_kernel, dim3(nb), dim3(256), 0, s, a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(peaks_emit_kernel, dim3(nb), dim3(256), 0, s, a);
-    return hipGetLastError();
+// ------------------------------------------------------------------------------------------------
+// update: work items (spike s, chunk); B chunks of kColsB columns, residual chunks of kColsX
+// ------------------------------------------------------------------------------------------------
+constexpr int kColsB = 8;     // score columns per work item (2 per wavefront)
+constexpr int kColsX = 16;    // residual columns per work item (4 per wavefront)
+
+// Spikes of the pass covering column w within +-reach, starting at spike s (whose window covers
+// w): the last even-indexed (e) and odd-indexed (o) ones.  Spikes are in ascending column order,
+// so the covering ones are s, s+1, ...; c1, c2 are col[s+1], col[s+2] (INT_MAX past the end).
+__device__ __forceinline__ void covering(const int* col, int s, int n, int c1, int c2, int w, int reach,
+                                         int& e, int& o) {
+    int last = s;
+    if (c1 - reach <= w) {
+        last = s + 1;
+        if (c2 - reach <= w) {
+            last = s + 2;
+            while (last + 1 < n && col[last + 1] - reach <= w) ++last;   // tied peaks only
+        }
+    }
+    if (last == s) {
+        e = (s & 1) ? -1 : s;
+        o = (s & 1) ? s : -1;
+    } else {
+        // the last two covering indices have different parities
+        e = (last & 1) ? last - 1 : last;
+        o = (last & 1) ? last : last - 1;
+    }
 }
 
-// ------------------------------------------------------------------------------------------------
-// update: one workgroup per spike of the pass (grid-stride), 4 wavefronts walk its columns
-// ------------------------------------------------------------------------------------------------
+template <int NV>   // float4 vectors per lane over templates: NV*256 >= T
+__device__ __forceinline__ void update_score_columns(const PursuitArgs& a, const int* col, const int* tmp,
+                                                     const float* amp, int s, int n, int c1, int c2, int c0,
+                                                     int lo, int w1, const floatx4* nmv) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nt = a.nt, L = 2 * nt + 1;
+    // two columns per wavefront, loads of both issued before any use
+    const int wa = c0 + 2 * wave, wb = wa + 1;
+    const bool va = wa >= lo && wa <= w1, vb = wb >= lo && wb <= w1;
+    int ea, oa, eb, ob;
+    covering(col, s, n, c1, c2, wa, nt, ea, oa);
+    covering(col, s, n, c1, c2, wb, nt, eb, ob);
+    floatx4 ba[NV], bb[NV], cea[NV], coa[NV], ceb[NV], cob[NV];
+    float* pa = a.B + (int64_t)wa * a.ldT;
+    float* pb = a.B + (int64_t)wb * a.ldT;
+    const float* qea = (ea >= 0) ? a.ctc + ((int64_t)tmp[ea] * L + (wa - col[ea] + nt)) * a.ldT : nullptr;
+    const float* qoa = (oa >= 0) ? a.ctc + ((int64_t)tmp[oa] * L + (wa - col[oa] + nt)) * a.ldT : nullptr;
+    const float* qeb = (eb >= 0) ? a.ctc + ((int64_t)tmp[eb] * L + (wb - col[eb] + nt)) * a.ldT : nullptr;
+    const float* qob = (ob >= 0) ? a.ctc + ((int64_t)tmp[ob] * L + (wb - col[ob] + nt)) * a.ldT : nullptr;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        const int t = 4 * lane + 256 * i;
+        if (t < a.T) {
+            if (va) {
+                ba[i] = *(const floatx4*)(pa + t);
+                if (qea) cea[i] = *(const floatx4*)(qea + t);
+                if (qoa) coa[i] = *(const floatx4*)(qoa + t);
+            }
+            if (vb) {
+                bb[i] = *(const floatx4*)(pb + t);
+                if (qeb) ceb[i] = *(const floatx4*)(qeb + t);
+                if (qob) cob[i] = *(const floatx4*)(qob + t);
+            }
+        }
+    }
+    const float aea = (ea >= 0) ? amp[ea] : 0.f, aoa = (oa >= 0) ? amp[oa] : 0.f;
+    const float aeb = (eb >= 0) ? amp[eb] : 0.f, aob = (ob >= 0) ? amp[ob] : 0.f;
+    const bool ra = va && wa >= nt && wa < a.W - nt;
+    const bool rb = vb && wb >= nt && wb < a.W - nt;
+    float besta = -1.f, bestb = -1.f;
+    int ia = 0x7fffffff, ib = 0x7fffffff;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        const int t = 4 * lane + 256 * i;
+        if (t < a.T) {
+            if (va) {
+                floatx4 x = ba[i];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    float v = x[e];
+                    if (qea) v = v - aea * cea[i][e];
+                    if (qoa) v = v - aoa * coa[i][e];
+                    x[e] = v;
+                    if (ra && t + e < a.T) argmax_merge(besta, ia, score_of(v, nmv[i][e]), t + e);
+                }
+                *(floatx4*)(pa + t) = x;
+            }
+            if (vb) {
+                floatx4 x = bb[i];
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    float v = x[e];
+                    if (qeb) v = v - aeb * ceb[i][e];
+                    if (qob) v = v - aob * cob[i][e];
+                    x[e] = v;
+                    if (rb && t + e < a.T) argmax_merge(bestb, ib, score_of(v, nmv[i][e]), t + e);
+                }
+                *(floatx4*)(pb + t) = x;
+            }
+        }
+    }
+    if (ra) {
+        wave_argmax(besta, ia);
+        if (lane == 0) { a.cfmax[wa] = besta; a.imax[wa] = ia; }
+    }
+    if (rb) {
+        wave_argmax(bestb, ib);
+        if (lane == 0) { a.cfmax[wb] = bestb; a.imax[wb] = ib; }
+    }
+}
+
+template <int NV>
 __global__ __launch_bounds__(256) void update_kernel(PursuitArgs a) {
     const int base = a.counters[0];
     int n = a.counters[1];
+    if (n == 0) return;
     if (base + n > a.cap) n = a.cap - base;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int nt = a.nt, h = nt / 2, T = a.T, C = a.C;
-    const int L = 2 * nt + 1;
+    const int nt = a.nt, h = nt / 2;
     const int* col = a.sp_col + base;
     const int* tmp = a.sp_tmp + base;
     const float* amp = a.sp_amp + base;
-    for (int s = blockIdx.x; s < n; s += gridDim.x) {
+    const int nB = (2 * nt + 1 + kColsB - 1) / kColsB;
+    const int nX = (2 * h + 1 + kColsX - 1) / kColsX;
+    const int per = nB + nX;
+    floatx4 nmv[NV];
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        const int t = 4 * lane + 256 * i;
+        if (t < a.T) nmv[i] = *(const floatx4*)(a.nm + t);
+    }
+    for (int64_t q = blockIdx.x; q < (int64_t)n * per; q += gridDim.x) {
+        const int s = (int)(q / per), sub = (int)(q % per);
+        // the spike's column and its neighbours' (independent loads, issued together)
         const int ws = col[s];
-        // ---- score columns [ws-nt, ws+nt] not covered by spike s-1 ----
-        int lo = ws - nt;
-        if (s > 0) lo = max(lo, col[s - 1] + nt + 1);
-        lo = max(lo, 0);
-        const int hi = min((int64_t)ws + nt, a.W - 1);
-        for (int w = lo + wave; w <= hi; w += 4) {
-            int e = -1, o = -1;
-            for (int j = s; j < n && col[j] - nt <= w; ++j) {
-                if (j & 1) o = j; else e = j;
-            }
-            const bool rescore = (w >= nt) && (w < a.W - nt);
-            float best = -1.f;
-            int bi = 0x7fffffff;
-            float* bcol = a.B + (int64_t)w * T;
-            const float* ce = (e >= 0) ? a.ctc + ((int64_t)tmp[e] * L + (w - col[e] + nt)) * T : nullptr;
-            const float* co = (o >= 0) ? a.ctc + ((int64_t)tmp[o] * L + (w - col[o] + nt)) * T : nullptr;
-            const float ae = (e >= 0) ? amp[e] : 0.f;
-            const float ao = (o >= 0) ? amp[o] : 0.f;
-            for (int t = lane; t < T; t += 64) {
-                float b = bcol[t];
-                if (e >= 0) b = b - ae * ce[t];
-                if (o >= 0) b = b - ao * co[t];
-                bcol[t] = b;
-                if (rescore) argmax_merge(best, bi, score_of(b, a.nm[t]), t);
-            }
-            if (rescore) {
-                wave_argmax(best, bi);
-                if (lane == 0) {
-                    a.cfmax[w] = best;
-                    a.imax[w] = bi;
+        const int cp = (s > 0) ? col[s - 1] : INT_MIN / 2;
+        const int c1 = (s + 1 < n) ? col[s + 1] : INT_MAX / 2;
+        const int c2 = (s + 2 < n) ? col[s + 2] : INT_MAX / 2;
+        if (sub < nB) {
+            int lo = ws - nt;
+            if (s > 0) lo = max(lo, cp + nt + 1);
+            lo = max(lo, 0);
+            const int hi = (int)min((int64_t)ws + nt, a.W - 1);
+            const int c0 = ws - nt + sub * kColsB;
+            const int w1 = min(c0 + kColsB - 1, hi);
+            if (max(c0, lo) <= w1) update_score_columns<NV>(a, col, tmp, amp, s, n, c1, c2, c0, lo, w1, nmv);
+        } else {
+            const int xs = sub - nB;
+            int lo = ws - h;
+            if (s > 0) lo = max(lo, cp + h + 1);
+            lo = max(lo, 0);
+            const int hi = (int)min((int64_t)ws + h, a.W - 1);
+            const int c0 = ws - h + xs * kColsX;
+            for (int k = 0; k < kColsX / 4; ++k) {
+                const int w = c0 + 4 * k + wave;
+                if (w < lo || w > hi || w > c0 + kColsX - 1) continue;
+                int e, o;
+                covering(col, s, n, c1, c2, w, h, e, o);
+                float* xc = a.X + (int64_t)w * a.ldC;
+                const float* ve = (e >= 0) ? a.waves + ((int64_t)tmp[e] * nt + (w - col[e] + h)) * a.ldC : nullptr;
+                const float* vo = (o >= 0) ? a.waves + ((int64_t)tmp[o] * nt + (w - col[o] + h)) * a.ldC : nullptr;
+                const float ae = (e >= 0) ? amp[e] : 0.f, ao = (o >= 0) ? amp[o] : 0.f;
+                for (int c = 4 * lane; c < a.C; c += 256) {
+                    floatx4 x = *(const floatx4*)(xc + c);
+                    floatx4 pe, po;
+                    if (ve) pe = *(const floatx4*)(ve + c);
+                    if (vo) po = *(const floatx4*)(vo + c);
+#pragma unroll
+                    for (int e4 = 0; e4 < 4; ++e4) {
+                        float v = x[e4];
+                        if (ve) v = v - ae * pe[e4];
+                        if (vo) v = v - ao * po[e4];
+                        x[e4] = v;
+                    }
+                    *(floatx4*)(xc + c) = x;
                 }
-            }
-        }
-        // ---- residual columns [ws-h, ws+h] not covered by spike s-1 ----
-        int xlo = ws - h;
-        if (s > 0) xlo = max(xlo, col[s - 1] + h + 1);
-        xlo = max(xlo, 0);
-        const int xhi = min((int64_t)ws + h, a.W - 1);
-        for (int w = xlo + wave; w <= xhi; w += 4) {
-            int e = -1, o = -1;
-            for (int j = s; j < n && col[j] - h <= w; ++j) {
-                if (j & 1) o = j; else e = j;
-            }
-            float* xcol = a.X + (int64_t)w * C;
-            const float* ve = (e >= 0) ? a.waves + ((int64_t)tmp[e] * nt + (w - col[e] + h)) * C : nullptr;
-            const float* vo = (o >= 0) ? a.waves + ((int64_t)tmp[o] * nt + (w - col[o] + h)) * C : nullptr;
-            const float ae = (e >= 0) ? amp[e] : 0.f;
-            const float ao = (o >= 0) ? amp[o] : 0.f;
-            for (int c = lane; c < C; c += 64) {
-                float x = xcol[c];
-                if (e >= 0) x = x - ae * ve[c];
-                if (o >= 0) x = x - ao * vo[c];
-                xcol[c] = x;
             }
         }
     }
 }
 
-hipError_t launch_update(const PursuitArgs& a, int n_new, hipStream_t s) {
-    if (n_new <= 0) return hipSuccess;
-    hipLaunchKernelGGL(update_kernel, dim3(n_new), dim3(256), 0, s, a);
+// ------------------------------------------------------------------------------------------------
+hipError_t launch_score_all(const PursuitArgs& a, hipStream_t s) {
+    unsigned nb = (unsigned)((a.W + 3) / 4);
+    const int nv = (a.T + 255) / 256;
+    switch (nv) {
+#define LSS_SA(V) case V: hipLaunchKernelGGL(score_all_kernel<V>, dim3(nb), dim3(256), 0, s, a); break;
+        LSS_SA(1) LSS_SA(2) LSS_SA(3) LSS_SA(4)
+#undef LSS_SA
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_peaks(const PursuitArgs& a, hipStream_t s) {
+    if (a.nt > kPeakHalo) return hipErrorInvalidValue;
+    const unsigned nb = (unsigned)((a.W + kFlagCols - 1) / kFlagCols);
+    hipLaunchKernelGGL(peaks_flags_kernel, dim3(nb), dim3(kFlagCols), 0, s, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(peaks_emit_kernel, dim3(1), dim3(kEmitThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_update(const PursuitArgs& a, int grid, hipStream_t s) {
+    const int nv = (a.T + 255) / 256;
+    switch (nv) {
+#define LSS_UP(V) case V: hipLaunchKernelGGL(update_kernel<V>, dim3(grid), dim3(256), 0, s, a); break;
+        LSS_UP(1) LSS_UP(2) LSS_UP(3) LSS_UP(4)
+#undef LSS_UP
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
 // ------------------------------------------------------------------------------------------------
-// PC features and positions: one wavefront per spike
+// PC features and positions: one wavefront per spike, count read from the device counters
 // ------------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void features_kernel(FeatureArgs a) {
     __shared__ float sf[4][32 * 8];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int s = blockIdx.x * 4 + wave;
-    const bool valid = s < a.n;
+    int n = a.counters ? a.counters[0] + a.counters[1] : a.n;
+    if (n > a.cap) n = a.cap;
     const int h = a.nt / 2, K = a.K, nC = a.nC;
-    int w = 0, t = 0, iu = 0;
-    float am = 0.f;
-    if (valid) {
-        w = a.sp_col[s];
-        t = a.sp_tmp[s];
-        am = a.sp_amp[s];
-        iu = a.iU[t];
-        for (int p = lane; p < nC * K; p += 64) {
-            const int j = p / K, k = p % K;
-            const int ch = a.iCC[j * a.C + iu];
-            float acc = 0.f;
-            for (int m = 0; m < a.nt; ++m) {
-                const int64_t ww = (int64_t)w - h + m;
-                const float xv = (ww >= 0 && ww < a.W) ? a.X[ww * a.C + ch] : 0.f;
-                acc = acc + xv * a.wpca[k * a.nt + m];
+    for (int s0 = blockIdx.x * 4; s0 < n; s0 += gridDim.x * 4) {
+        const int s = s0 + wave;
+        const bool valid = s < n;
+        int t = 0, iu = 0;
+        if (valid) {
+            const int w = a.sp_col[s];
+            t = a.sp_tmp[s];
+            const float am = a.sp_amp[s];
+            iu = a.iU[t];
+            for (int p = lane; p < nC * K; p += 64) {
+                const int j = p / K, k = p % K;
+                const int ch = a.iCC[j * a.C + iu];
+                float acc = 0.f;
+                for (int m = 0; m < a.nt; ++m) {
+                    const int64_t ww = (int64_t)w - h + m;
+                    const float xv = (ww >= 0 && ww < a.W) ? a.X[ww * a.ldC + ch] : 0.f;
+                    acc = acc + xv * a.wpca[k * a.nt + m];
+                }
+                const float f = acc + am * a.Ucc[((int64_t)j * a.T + t) * K + k];
+                sf[wave][p] = f;
+                a.tF[(int64_t)s * nC * K + p] = f;
             }
-            const float f = acc + am * a.Ucc[((int64_t)j * a.T + t) * K + k];
-            sf[wave][p] = f;
-            a.tF[(int64_t)s * nC * K + p] = f;
         }
-    }
-    __syncthreads();
-    if (valid && lane == 0) {
-        float tot = 0.f;
-        for (int j = 0; j < nC; ++j) {
-            float m = 0.f;
-            for (int k = 0; k < K; ++k) m += sf[wave][j * K + k] * sf[wave][j * K + k];
-            tot += m;
+        __syncthreads();
+        if (valid && lane == 0) {
+            float tot = 0.f;
+            for (int j = 0; j < nC; ++j) {
+                float m = 0.f;
+                for (int k = 0; k < K; ++k) m += sf[wave][j * K + k] * sf[wave][j * K + k];
+                tot += m;
+            }
+            float xs = 0.f, ys = 0.f;
+            for (int j = 0; j < nC; ++j) {
+                float m = 0.f;
+                for (int k = 0; k < K; ++k) m += sf[wave][j * K + k] * sf[wave][j * K + k];
+                const int ch = a.iCC[j * a.C + iu];
+                const float wgt = m / tot;
+                xs += a.xc[ch] * wgt;
+                ys += a.yc[ch] * wgt;
+            }
+            a.xs[s] = xs;
+            a.ys[s] = ys;
         }
-        float xs = 0.f, ys = 0.f;
-        for (int j = 0; j < nC; ++j) {
-            float m = 0.f;
-            for (int k = 0; k < K; ++k) m += sf[wave][j * K + k] * sf[wave][j * K + k];
-            const int ch = a.iCC[j * a.C + iu];
-            const float wgt = m / tot;
-            xs += a.xc[ch] * wgt;
-            ys += a.yc[ch] * wgt;
-        }
-        a.xs[s] = xs;
-        a.ys[s] = ys;
+        __syncthreads();
     }
 }
 
-hipError_t launch_features(const FeatureArgs& a, hipStream_t s) {
-    if (a.n <= 0) return hipSuccess;
+hipError_t launch_features(const FeatureArgs& a, int grid, hipStream_t s) {
     if (a.nC * a.K > 32 * 8) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(features_kernel, dim3((a.n + 3) / 4), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(features_kernel, dim3(grid), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+// counters[0..7] = 0 (a fill kernel, so that the sequence can be captured in a graph)
+__global__ void reset_counters_kernel(int* c) {
+    if (threadIdx.x < 8) c[threadIdx.x] = 0;
+}
+
+hipError_t launch_reset_counters(int* counters, hipStream_t s) {
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, counters);
     return hipGetLastError();
 }
 

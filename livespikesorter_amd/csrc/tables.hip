@@ -26,7 +26,8 @@ __global__ __launch_bounds__(256) void nm_kernel(const float* __restrict__ U, in
 }
 
 __global__ __launch_bounds__(256) void waves_kernel(const float* __restrict__ U, int64_t ldu, const float* __restrict__ wpca,
-                                                    int T, int K, int C, int nt, float* __restrict__ waves) {
+                                                    int T, int K, int C, int nt, float* __restrict__ waves,
+                                                    int64_t ldC) {
     const int64_t idx = (int64_t)blockIdx.x * 256 + threadIdx.x;
     const int64_t total = (int64_t)T * nt * C;
     if (idx >= total) return;
@@ -35,7 +36,7 @@ __global__ __launch_bounds__(256) void waves_kernel(const float* __restrict__ U,
     const int t = (int)(idx / ((int64_t)C * nt));
     double acc = 0.0;
     for (int k = 0; k < K; ++k) acc += (double)U[(int64_t)t * ldu + (int64_t)k * C + c] * (double)wpca[k * nt + m];
-    waves[idx] = (float)acc;
+    waves[((int64_t)t * nt + m) * ldC + c] = (float)acc;
 }
 
 // G[(i,k)][(j,m)] = sum_c U[i,k,c] U[j,m,c]: 16x16 FP64 tiles staged in LDS; row r = i*K+k of U
@@ -61,7 +62,7 @@ __global__ __launch_bounds__(256) void gram_kernel(const float* __restrict__ U, 
 
 // ctc[j][l][i]: one thread per (i, j), all lags
 __global__ __launch_bounds__(256) void ctc_kernel(const double* __restrict__ G, const double* __restrict__ wtw,
-                                                  int T, int K, int L, float* __restrict__ ctc) {
+                                                  int T, int K, int L, float* __restrict__ ctc, int64_t ldT) {
     extern __shared__ double swtw[];   // [K][K][L]
     for (int e = threadIdx.x; e < K * K * L; e += 256) swtw[e] = wtw[e];
     __syncthreads();
@@ -76,18 +77,18 @@ __global__ __launch_bounds__(256) void ctc_kernel(const double* __restrict__ G, 
         double acc = 0.0;
         for (int k = 0; k < K; ++k)
             for (int m = 0; m < K; ++m) acc += g[k * K + m] * swtw[(k * K + m) * L + l];
-        ctc[((int64_t)j * L + l) * T + i] = (float)acc;
+        ctc[((int64_t)j * L + l) * ldT + i] = (float)acc;
     }
 }
 
 hipError_t launch_tables(const float* U, int64_t ldu, const float* wpca, int T, int K, int C, int nt,
-                         float* nm, float* waves, float* ctc, hipStream_t s) {
+                         float* nm, float* waves, int64_t ldC, float* ctc, int64_t ldT, hipStream_t s) {
     if (K > 8) return hipErrorInvalidValue;
     hipLaunchKernelGGL(nm_kernel, dim3(T), dim3(256), 0, s, U, ldu, K * C, nm);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const int64_t nw = (int64_t)T * nt * C;
-    hipLaunchKernelGGL(waves_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, U, ldu, wpca, T, K, C, nt, waves);
+    hipLaunchKernelGGL(waves_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, U, ldu, wpca, T, K, C, nt, waves, ldC);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     // WtW on the host (K*K*(2nt+1) values) from the FP32 wPCA
     const int L = 2 * nt + 1;
@@ -120,7 +121,7 @@ hipError_t launch_tables(const float* U, int64_t ldu, const float* wpca, int T, 
         e = hipFuncSetAttribute((const void*)ctc_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(ctc_kernel, dim3((T + 255) / 256, T), dim3(256), lds, s,
-                           dG, dW, T, K, L, ctc);
+                           dG, dW, T, K, L, ctc, ldT);
         e = hipGetLastError();
     }
     hipError_t e2 = hipStreamSynchronize(s);
