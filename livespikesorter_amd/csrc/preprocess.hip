@@ -27,11 +27,71 @@ __global__ __launch_bounds__(256) void channel_sums_kernel(const int16_t* __rest
     }
 }
 
+// C % 8 == 0: each thread owns one 16-byte column group (8 channels) and walks rows with stride
+// G = 256 / (C/8); the G partial sums of a group are combined in LDS, then one 64-bit atomic per
+// channel and block.
+constexpr int kSumRows = 512;    // rows per block
+__global__ __launch_bounds__(256) void channel_sums_vec_kernel(const int16_t* __restrict__ win,
+                                                               int64_t W, int C,
+                                                               long long* __restrict__ sums) {
+    __shared__ int part[256][8];
+    const int nv = C / 8;                 // vectors per row (<= 128)
+    const int G = 256 / nv;               // row lanes
+    const int tid = threadIdx.x;
+    const int v = tid % nv, g = tid / nv;
+    const int64_t r0 = (int64_t)blockIdx.x * kSumRows;
+    const int64_t r1 = min(W, r0 + kSumRows);
+    int acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (g < G) {
+        const uint4* base = (const uint4*)win;
+        int64_t r = r0 + g;
+        for (; r + 3 * G < r1; r += 4 * G) {
+            uint4 q[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) q[u] = base[(r + u * G) * nv + v];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const uint32_t w4[4] = {q[u].x, q[u].y, q[u].z, q[u].w};
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    acc[2 * j] += (int16_t)(w4[j] & 0xffffu);
+                    acc[2 * j + 1] += (int16_t)(w4[j] >> 16);
+                }
+            }
+        }
+        for (; r < r1; r += G) {
+            const uint4 q = base[r * nv + v];
+            const uint32_t w4[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                acc[2 * j] += (int16_t)(w4[j] & 0xffffu);
+                acc[2 * j + 1] += (int16_t)(w4[j] >> 16);
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) part[tid][j] = acc[j];
+    __syncthreads();
+    if (tid < nv) {
+        long long tot[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int gg = 0; gg < G; ++gg)
+#pragma unroll
+            for (int j = 0; j < 8; ++j) tot[j] += part[gg * nv + tid][j];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) atomicAdd((unsigned long long*)&sums[8 * tid + j], (unsigned long long)tot[j]);
+    }
+}
+
 hipError_t launch_channel_sums(const int16_t* win, int64_t W, int C, long long* sums, hipStream_t s) {
     hipError_t e = hipMemsetAsync(sums, 0, sizeof(long long) * C, s);
     if (e != hipSuccess) return e;
-    int nb = (int)((W + 255) / 256);
-    hipLaunchKernelGGL(channel_sums_kernel, dim3(nb), dim3(256), 0, s, win, W, C, sums);
+    if (C % 8 == 0 && (((uintptr_t)win) & 15) == 0) {
+        const int nb = (int)((W + kSumRows - 1) / kSumRows);
+        hipLaunchKernelGGL(channel_sums_vec_kernel, dim3(nb), dim3(256), 0, s, win, W, C, sums);
+    } else {
+        const int nb = (int)((W + 255) / 256);
+        hipLaunchKernelGGL(channel_sums_kernel, dim3(nb), dim3(256), 0, s, win, W, C, sums);
+    }
     return hipGetLastError();
 }
 

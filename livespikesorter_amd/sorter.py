@@ -140,7 +140,11 @@ class Sorter:
 
     # -------------------------------------------------------------------------------------------
     def _out(self, cap):
+        """Host output buffers, allocated once per capacity and reused across calls."""
         n = int(cap)
+        cached = getattr(self, "_out_cache", None)
+        if cached is not None and cached[0] == n:
+            return cached[1], cached[2]
         bufs = dict(sample=np.zeros(n, np.int64), template=np.zeros(n, np.int32),
                     amplitude=np.zeros(n, np.float32),
                     features=np.zeros((n, self.nC, self.K), np.float32),
@@ -154,6 +158,7 @@ class Sorter:
         s.x = bufs["x"].ctypes.data
         s.y = bufs["y"].ctypes.data
         s.column = bufs["column"].ctypes.data
+        self._out_cache = (n, s, bufs)
         return s, bufs
 
     @staticmethod
