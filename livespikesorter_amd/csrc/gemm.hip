@@ -4,18 +4,27 @@
 //   scores    : B[w][t]   = sum_q Xp[w][q] * U[t][q], q = k*C+c   (M = W, N = T, K = K*C)
 //
 // both as C[m][n] = sum_k A(m,k) B(n,k).  A 128x128 output tile per 256-thread workgroup; each of
-// the 4 wavefronts owns a 64x64 quarter as 2x2 blocks of 32x32 MFMA accumulators.  Operand tiles
-// are staged through LDS k-major so that every MFMA operand read is 32 consecutive floats.
-// Workgroups are renumbered so that each XCD (blockIdx % 8) walks a contiguous run of tiles with
-// the N tiles of one M row-block adjacent: the A rows are then fetched into that XCD's L2 once.
+// the 4 wavefronts owns a 64x64 quarter as 2x2 blocks of 32x32 MFMA accumulators.  The K loop
+// advances 32 at a time with register prefetch: the global loads of step k+1 are issued before
+// the MFMAs of step k, so load latency hides behind matrix work.  Workgroups are renumbered so
+// that each XCD (blockIdx % 8) walks a contiguous run of tiles with the N tiles of one M
+// row-block adjacent: the A rows are fetched into that XCD's L2 once.
+//
+// FP32 uses v_mfma_f32_32x32x2_f32 on k-major LDS tiles.  The reduced-precision variants convert
+// the FP32 operands while staging them into LDS (m/n-major, k contiguous, 8 values per lane read)
+// and use v_mfma_f32_32x32x16_{bf16,f16}; bf16x3 splits x = hi + lo (hi = bf16(x),
+// lo = bf16(x - hi)) and accumulates hi*hi + hi*lo + lo*hi in FP32.
+#include "../../include/lss.h"
 #include "lss_kernels.h"
 
 namespace lss {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
 
-constexpr int BM = 128, BN = 128, BK = 16;
+constexpr int BM = 128, BN = 128, BK = 32;
 
 __device__ __forceinline__ void xcd_tile(int nM, int nN, int& mt, int& nt) {
     const int total = nM * nN;
@@ -26,6 +35,68 @@ __device__ __forceinline__ void xcd_tile(int nM, int nN, int& mt, int& nt) {
     const int v = xcd * per + idx;
     mt = (v < total) ? v / nN : nM;
     nt = v % nN;
+}
+
+// ---- global tile fetch: 128 rows x 32 k of an MK (row = m, k contiguous) or KM operand -------
+// MK: thread t loads rows t/2 (+64 for the second half), k quarter (t&1)*16 .. +16 as 4 float4.
+// KM: thread t loads k row t/8, m quarter (t&7)*16 .. +16 as 4 float4.
+struct Frag {
+    floatx4 v[4];
+};
+
+template <bool KM>
+__device__ __forceinline__ void fetch(const float* __restrict__ P, int64_t ld, int r0, int k0, int R, int K,
+                                      Frag& f) {
+    const int tid = threadIdx.x;
+    if (!KM) {
+        const int row = tid >> 1, kq = (tid & 1) * 16;
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const int r = r0 + row;
+            const int k = k0 + kq + 4 * h;
+            floatx4 v = {0.f, 0.f, 0.f, 0.f};
+            if (r < R) {
+                const float* p = P + (int64_t)r * ld + k;
+                if (k + 3 < K) v = *(const floatx4*)p;
+                else
+                    for (int e = 0; e < 4; ++e) v[e] = (k + e < K) ? p[e] : 0.f;
+            }
+            f.v[h] = v;
+        }
+        (void)R;
+    } else {
+        const int kr = tid >> 3, mq = (tid & 7) * 16;
+        const int k = k0 + kr;
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const int r = r0 + mq + 4 * h;
+            floatx4 v = {0.f, 0.f, 0.f, 0.f};
+            if (k < K) {
+                const float* p = P + (int64_t)k * ld + r;
+                if (r + 3 < R) v = *(const floatx4*)p;
+                else
+                    for (int e = 0; e < 4; ++e) v[e] = (r + e < R) ? p[e] : 0.f;
+            }
+            f.v[h] = v;
+        }
+    }
+}
+
+// FP32: LDS tile [BK][128 + 4] (k-major)
+template <bool KM>
+__device__ __forceinline__ void stash_f32(float (*S)[BM + 4], const Frag& f) {
+    const int tid = threadIdx.x;
+    if (!KM) {
+        const int row = tid >> 1, kq = (tid & 1) * 16;
+#pragma unroll
+        for (int h = 0; h < 4; ++h)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) S[kq + 4 * h + e][row] = f.v[h][e];
+    } else {
+        const int kr = tid >> 3, mq = (tid & 7) * 16;
+#pragma unroll
+        for (int h = 0; h < 4; ++h) *(floatx4*)&S[kr][mq + 4 * h] = f.v[h];
+    }
 }
 
 template <bool A_KM>
@@ -39,7 +110,7 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(const float* __restrict__
     xcd_tile(nM, nN, mt, ntile);
     if (mt >= nM) return;
     const int m0 = mt * BM, n0 = ntile * BN;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave >> 1, wn = wave & 1;
 
     floatx16 acc[2][2];
@@ -50,56 +121,17 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(const float* __restrict__
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
 
+    Frag fa, fb;
+    fetch<A_KM>(A, lda, m0, 0, M, K, fa);
+    fetch<false>(B, ldb, n0, 0, N, K, fb);
     for (int k0 = 0; k0 < K; k0 += BK) {
-        // ---- stage A ----
-        if (!A_KM) {
-            const int row = tid >> 1, kq = (tid & 1) * 8;
-            const int m = m0 + row;
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int k = k0 + kq + 4 * h;
-                floatx4 v = {0.f, 0.f, 0.f, 0.f};
-                if (m < M) {
-                    if (k + 3 < K) v = *(const floatx4*)(A + (int64_t)m * lda + k);
-                    else
-                        for (int e = 0; e < 4; ++e) v[e] = (k + e < K) ? A[(int64_t)m * lda + k + e] : 0.f;
-                }
-#pragma unroll
-                for (int e = 0; e < 4; ++e) As[kq + 4 * h + e][row] = v[e];
-            }
-        } else {
-            const int kr = tid >> 4, mq = (tid & 15) * 8;
-            const int k = k0 + kr;
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int m = m0 + mq + 4 * h;
-                floatx4 v = {0.f, 0.f, 0.f, 0.f};
-                if (k < K) {
-                    if (m + 3 < M) v = *(const floatx4*)(A + (int64_t)k * lda + m);
-                    else
-                        for (int e = 0; e < 4; ++e) v[e] = (m + e < M) ? A[(int64_t)k * lda + m + e] : 0.f;
-                }
-                *(floatx4*)&As[kr][mq + 4 * h] = v;
-            }
-        }
-        // ---- stage B (N x K, k contiguous) ----
-        {
-            const int row = tid >> 1, kq = (tid & 1) * 8;
-            const int n = n0 + row;
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int k = k0 + kq + 4 * h;
-                floatx4 v = {0.f, 0.f, 0.f, 0.f};
-                if (n < N) {
-                    if (k + 3 < K) v = *(const floatx4*)(B + (int64_t)n * ldb + k);
-                    else
-                        for (int e = 0; e < 4; ++e) v[e] = (k + e < K) ? B[(int64_t)n * ldb + k + e] : 0.f;
-                }
-#pragma unroll
-                for (int e = 0; e < 4; ++e) Bs[kq + 4 * h + e][row] = v[e];
-            }
-        }
+        stash_f32<A_KM>(As, fa);
+        stash_f32<false>(Bs, fb);
         __syncthreads();
+        if (k0 + BK < K) {
+            fetch<A_KM>(A, lda, m0, k0 + BK, M, K, fa);
+            fetch<false>(B, ldb, n0, k0 + BK, N, K, fb);
+        }
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 2) {
             const int kr = kk + (lane >> 5);
@@ -116,7 +148,136 @@ __global__ __launch_bounds__(256) void gemm_f32_kernel(const float* __restrict__
         }
         __syncthreads();
     }
-    // ---- store: acc[r] of lane l is row 8*(r/4) + 4*(l/32) + r%4, column l%32 ----
+    // acc[r] of lane l is row 8*(r/4) + 4*(l/32) + r%4, column l%32 of its 32x32 block
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int n = n0 + wn * 64 + j * 32 + (lane & 31);
+            if (n >= N) continue;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int m = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (m < M) C[(int64_t)m * ldc + n] = acc[i][j][r];
+            }
+        }
+}
+
+// ---- reduced precision ------------------------------------------------------------------------
+constexpr int kLdsK = BK + 8;   // k stride of the m/n-major LDS tiles (16-bit elements)
+
+template <int PREC>
+struct LpType;
+template <>
+struct LpType<LSS_PREC_BF16> { typedef __bf16 T; typedef bf16x8 V; };
+template <>
+struct LpType<LSS_PREC_FP16> { typedef _Float16 T; typedef halfx8 V; };
+template <>
+struct LpType<LSS_PREC_BF16X3> { typedef __bf16 T; typedef bf16x8 V; };
+
+// element (row, k) of the tile held by this thread's Frag
+template <bool KM>
+__device__ __forceinline__ void frag_coord(int h, int e, int& row, int& k) {
+    const int tid = threadIdx.x;
+    if (!KM) {
+        row = tid >> 1;
+        k = (tid & 1) * 16 + 4 * h + e;
+    } else {
+        row = (tid & 7) * 16 + 4 * h + e;
+        k = tid >> 3;
+    }
+}
+
+template <int PREC, bool KM>
+__device__ __forceinline__ void stash_lp(typename LpType<PREC>::T (*Hi)[kLdsK], typename LpType<PREC>::T (*Lo)[kLdsK],
+                                         const Frag& f) {
+    typedef typename LpType<PREC>::T T;
+#pragma unroll
+    for (int h = 0; h < 4; ++h)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            int row, k;
+            frag_coord<KM>(h, e, row, k);
+            const float x = f.v[h][e];
+            const T hi = (T)x;
+            Hi[row][k] = hi;
+            if (PREC == LSS_PREC_BF16X3) Lo[row][k] = (T)(x - (float)hi);
+        }
+}
+
+template <int PREC>
+__device__ __forceinline__ floatx16 mfma_lp(typename LpType<PREC>::V a, typename LpType<PREC>::V b, floatx16 c) {
+    if constexpr (PREC == LSS_PREC_FP16) return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    else return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+template <int PREC, bool A_KM>
+__global__ __launch_bounds__(256) void gemm_lp_kernel(const float* __restrict__ A, int64_t lda,
+                                                      const float* __restrict__ B, int64_t ldb,
+                                                      float* __restrict__ C, int64_t ldc,
+                                                      int M, int N, int K, int nM, int nN) {
+    typedef typename LpType<PREC>::T T;
+    typedef typename LpType<PREC>::V V;
+    constexpr bool X3 = PREC == LSS_PREC_BF16X3;
+    __shared__ __attribute__((aligned(16))) T Ah[BM][kLdsK];
+    __shared__ __attribute__((aligned(16))) T Bh[BN][kLdsK];
+    __shared__ __attribute__((aligned(16))) T Al[X3 ? BM : 1][kLdsK];
+    __shared__ __attribute__((aligned(16))) T Bl[X3 ? BN : 1][kLdsK];
+    int mt, ntile;
+    xcd_tile(nM, nN, mt, ntile);
+    if (mt >= nM) return;
+    const int m0 = mt * BM, n0 = ntile * BN;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+
+    floatx16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+
+    Frag fa, fb;
+    fetch<A_KM>(A, lda, m0, 0, M, K, fa);
+    fetch<false>(B, ldb, n0, 0, N, K, fb);
+    for (int k0 = 0; k0 < K; k0 += BK) {
+        stash_lp<PREC, A_KM>(Ah, Al, fa);
+        stash_lp<PREC, false>(Bh, Bl, fb);
+        __syncthreads();
+        if (k0 + BK < K) {
+            fetch<A_KM>(A, lda, m0, k0 + BK, M, K, fa);
+            fetch<false>(B, ldb, n0, k0 + BK, N, K, fb);
+        }
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 16) {
+            const int kq = kk + 8 * (lane >> 5);
+            V ah[2], bh[2], al[2], bl[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int r = wm * 64 + i * 32 + (lane & 31);
+                ah[i] = *(const V*)&Ah[r][kq];
+                if constexpr (X3) al[i] = *(const V*)&Al[r][kq];
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int r = wn * 64 + j * 32 + (lane & 31);
+                bh[j] = *(const V*)&Bh[r][kq];
+                if constexpr (X3) bl[j] = *(const V*)&Bl[r][kq];
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    if constexpr (X3) {
+                        acc[i][j] = mfma_lp<PREC>(al[i], bh[j], acc[i][j]);
+                        acc[i][j] = mfma_lp<PREC>(ah[i], bl[j], acc[i][j]);
+                    }
+                    acc[i][j] = mfma_lp<PREC>(ah[i], bh[j], acc[i][j]);
+                }
+        }
+        __syncthreads();
+    }
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -141,6 +302,29 @@ hipError_t launch_gemm_f32(const float* A, int64_t lda, int a_km, const float* B
     else
         hipLaunchKernelGGL(gemm_f32_kernel<false>, grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K, nM, nN);
     return hipGetLastError();
+}
+
+template <int PREC>
+hipError_t lp_launch(const float* A, int64_t lda, int a_km, const float* B, int64_t ldb, float* C, int64_t ldc,
+                     int M, int N, int K, hipStream_t s) {
+    const int nM = (M + BM - 1) / BM, nN = (N + BN - 1) / BN;
+    dim3 grid(8 * ((nM * nN + 7) / 8));
+    if (a_km)
+        hipLaunchKernelGGL((gemm_lp_kernel<PREC, true>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K, nM, nN);
+    else
+        hipLaunchKernelGGL((gemm_lp_kernel<PREC, false>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K, nM, nN);
+    return hipGetLastError();
+}
+
+hipError_t launch_gemm_lp(int prec, const float* A, int64_t lda, int a_km, const float* B, int64_t ldb,
+                          float* C, int64_t ldc, int M, int N, int K, hipStream_t s) {
+    if ((lda & 3) || (ldb & 3)) return hipErrorInvalidValue;
+    switch (prec) {
+        case LSS_PREC_BF16: return lp_launch<LSS_PREC_BF16>(A, lda, a_km, B, ldb, C, ldc, M, N, K, s);
+        case LSS_PREC_FP16: return lp_launch<LSS_PREC_FP16>(A, lda, a_km, B, ldb, C, ldc, M, N, K, s);
+        case LSS_PREC_BF16X3: return lp_launch<LSS_PREC_BF16X3>(A, lda, a_km, B, ldb, C, ldc, M, N, K, s);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace lss

@@ -181,3 +181,55 @@ def test_capacity_error(synth1):
         # the handle stays usable with an output buffer large enough for nothing: window too long
         with pytest.raises(LssError, match="ARG"):
             s.sort_window(np.zeros((60123, win.shape[1]), np.int16), 0)
+
+
+# ---- reduced-precision contractions (FP32 accumulation) -----------------------------------------
+# Error bounds, max |gpu - fp64| / max |fp64| over the window: BF16 operands carry 8 significant
+# bits (unit roundoff 2^-9), FP16 11 bits (2^-12), bf16x3 ~16 bits (2^-17 per split term).
+PREC_BOUNDS = {"bf16": 2e-2, "fp16": 3e-3, "bf16x3": 1e-4}
+
+
+@pytest.fixture(scope="module")
+def lp_sorters(synth1):
+    from livespikesorter_amd.sorter import Sorter
+    out = {p: Sorter(synth1.art, max_window=60122, max_passes=100, precision=p) for p in PREC_BOUNDS}
+    yield out
+    for s in out.values():
+        s.close()
+
+
+@pytest.mark.parametrize("prec", list(PREC_BOUNDS))
+def test_scores_and_whitening_precision(lp_sorters, synth1, prec):
+    print(f"[lss] test_scores_and_whitening_precision {prec}", flush=True)
+    a = synth1.art
+    so = lp_sorters[prec]
+    _, X, _ = _window(synth1, 0)
+    hp = torch.from_numpy(a["hp_filter"]).double()
+    xf = ko.filter_batch(X, hp, None)                      # before whitening, [C][W]
+    xw = so.stage_whiten(xf.float().contiguous().to(DEV)).cpu().numpy()
+    ref_w = (_wm(a, torch.float64) @ xf).numpy().T
+    ew = rel_err(xw, ref_w)
+    x32 = torch.from_numpy(ref_w.T.copy()).float()
+    B = so.stage_scores(x32.T.contiguous().to(DEV)).cpu().numpy()
+    ref = ko.project(x32.double(), torch.from_numpy(a["U"]).double(),
+                     torch.from_numpy(a["wPCA"]).double()).numpy().T
+    eb = rel_err(B, ref)
+    print(f"[lss]   {prec}: whitening rel err {ew:.2e}, scores rel err {eb:.2e}", flush=True)
+    assert ew <= PREC_BOUNDS[prec] and eb <= PREC_BOUNDS[prec]
+
+
+@pytest.mark.parametrize("prec", list(PREC_BOUNDS))
+def test_sort_window_precision_vs_fp32(sorter, lp_sorters, synth1, prec):
+    """Spike list of a reduced-precision path against the FP32 device path (+-1 sample, same
+    template); bf16x3 must reach F1 >= 0.999 with amplitudes within 1e-3."""
+    print(f"[lss] test_sort_window_precision_vs_fp32 {prec}", flush=True)
+    win, _, ws = _window(synth1, 1)
+    ref = sorter.sort_window(win, ws)
+    got = lp_sorters[prec].sort_window(win, ws)
+    f1 = ko.spike_f1(ref.sample, ref.template, got.sample, got.template, dt=1)
+    err, n = _match_amp(ref.sample, ref.template, ref.amplitude, got.sample, got.template, got.amplitude)
+    print(f"[lss]   {prec}: F1 vs fp32 {f1:.5f}, amp rel err {err:.2e} over {n}", flush=True)
+    if prec == "bf16x3":
+        assert f1 >= 0.999 and err <= 1e-3
+    else:
+        assert f1 >= 0.95
