@@ -78,7 +78,7 @@ struct Handle {
     DevBuf<float> xcw, xhp, xw, xp, B, cfmax;
     DevBuf<int> imax;
     DevBuf<unsigned char> flags;
-    DevBuf<int> sp_col, sp_tmp, counters;
+    DevBuf<int> sp_col, sp_tmp, counters, pass_off;
     DevBuf<float> sp_amp, tF, xs, ys;
     int* h_counters = nullptr;   // pinned
 
@@ -103,7 +103,7 @@ lss_status check_params(const lss_params& p) {
     if (p.n_nearest < 1 || p.n_nearest > p.n_chan || p.n_nearest * p.n_pcs > 256)
         return fail(LSS_ERR_ARG, "n_nearest must be in [1, n_chan] with n_nearest*n_pcs <= 256");
     if (p.max_window < 2 * p.nt + 2) return fail(LSS_ERR_ARG, "max_window too small");
-    if (p.max_passes < 1) return fail(LSS_ERR_ARG, "max_passes must be >= 1");
+    if (p.max_passes < 1 || p.max_passes > 256) return fail(LSS_ERR_ARG, "max_passes must be in [1, 256]");
     if (p.max_spikes < 1) return fail(LSS_ERR_ARG, "max_spikes must be >= 1");
     if (p.precision < LSS_PREC_FP32 || p.precision > LSS_PREC_BF16X3) return fail(LSS_ERR_ARG, "bad precision");
     return LSS_OK;
@@ -212,9 +212,9 @@ lss_status create_impl(const lss_params* p, const lss_artifacts* a, Handle** out
     HIP_TRY(h->nm.alloc(h->ldT));
     HIP_TRY(h->waves.alloc((size_t)T * nt * h->ldC));
     HIP_TRY(h->ctc.alloc((size_t)T * L * h->ldT));
+    HIP_TRY(hipMemset(h->ctc.p, 0, sizeof(float) * (size_t)T * L * h->ldT));
     HIP_TRY(hipMemset(h->nm.p, 0, sizeof(float) * h->ldT));
     HIP_TRY(hipMemset(h->waves.p, 0, sizeof(float) * T * nt * h->ldC));
-    HIP_TRY(hipMemset(h->ctc.p, 0, sizeof(float) * (size_t)T * L * h->ldT));
     HIP_TRY(h->Ucc.alloc((size_t)nC * T * K));
     HIP_TRY(h->xc.alloc(C));
     HIP_TRY(h->yc.alloc(C));
@@ -312,6 +312,7 @@ lss_status create_impl(const lss_params* p, const lss_artifacts* a, Handle** out
     HIP_TRY(h->sp_tmp.alloc(p->max_spikes));
     HIP_TRY(h->sp_amp.alloc(p->max_spikes));
     HIP_TRY(h->counters.alloc(8));
+    HIP_TRY(h->pass_off.alloc(p->max_passes + 2));
     HIP_TRY(h->tF.alloc((size_t)p->max_spikes * nC * K));
     HIP_TRY(h->xs.alloc(p->max_spikes));
     HIP_TRY(h->ys.alloc(p->max_spikes));
@@ -365,7 +366,6 @@ lss_status run_scores(Handle& h, const float* x_wc, float* b_wt, int64_t W, hipE
     return LSS_OK;
 }
 
-constexpr int kUpdateGrid = 8192;    // workgroups of each update launch (grid-stride over work items)
 constexpr int kFeatureGrid = 1024;
 
 // Pursuit over the handle's B [W][ldT] and residual X [W][ldC]: max_passes passes enqueued with no
@@ -373,16 +373,18 @@ constexpr int kFeatureGrid = 1024;
 lss_status run_pursuit(Handle& h, float* B, float* X, int64_t W) {
     lss::PursuitArgs a{};
     a.B = B; a.X = X; a.nm = h.nm.p; a.ctc = h.ctc.p; a.waves = h.waves.p;
+    a.pass_off = h.pass_off.p;
     a.cfmax = h.cfmax.p; a.imax = h.imax.p; a.flags = h.flags.p;
     a.sp_col = h.sp_col.p; a.sp_tmp = h.sp_tmp.p; a.sp_amp = h.sp_amp.p; a.counters = h.counters.p;
-    a.W = W; a.ldT = h.ldT; a.ldC = h.ldC; a.T = h.T; a.C = h.C; a.nt = h.nt; a.cap = h.prm.max_spikes;
+    a.W = W; a.ldT = h.ldT; a.ldC = h.ldC; a.T = h.T; a.C = h.C; a.K = h.K; a.nt = h.nt; a.cap = h.prm.max_spikes;
     a.th2 = h.prm.th_learned * h.prm.th_learned;
     HIP_TRY(lss::launch_reset_counters(h.counters.p, h.stream));
     HIP_TRY(lss::launch_score_all(a, h.stream));
     for (int p = 0; p < h.prm.max_passes; ++p) {
         HIP_TRY(lss::launch_peaks(a, h.stream));
-        HIP_TRY(lss::launch_update(a, kUpdateGrid, h.stream));
+        HIP_TRY(lss::launch_update(a, h.stream));
     }
+    HIP_TRY(lss::launch_residual(a, h.stream));
     return LSS_OK;
 }
 
@@ -403,6 +405,9 @@ lss_status read_counts(Handle& h, int* n, int* passes) {
     HIP_TRY(hipMemcpyAsync(h.h_counters, h.counters.p, sizeof(int) * 8, hipMemcpyDeviceToHost, h.stream));
     HIP_TRY(hipStreamSynchronize(h.stream));
     if (h.h_counters[2]) return fail(LSS_ERR_CAPACITY, "spike capacity (max_spikes) exceeded");
+    if (h.h_counters[5])
+        return fail(LSS_ERR_CAPACITY, "more than 1024 spikes of all passes reach one 64-sample block "
+                                      "(residual list); the residual is incomplete");
     *n = h.h_counters[0] + h.h_counters[1];
     *passes = h.h_counters[4];
     return LSS_OK;

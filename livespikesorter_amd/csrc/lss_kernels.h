@@ -43,30 +43,33 @@ hipError_t launch_project(const float* x_wc, int64_t ldx, int64_t W, int C, cons
 // Row strides ldT (B, ctc, nm) and ldC (X, waves) are multiples of 4 (float4 access); padding
 // columns are never read as scores.
 struct PursuitArgs {
-    float* B;            // [W][ldT] scores (updated)
-    float* X;            // [W][ldC] residual (updated)
-    const float* nm;     // [ldT]
-    const float* ctc;    // [T][2nt+1][ldT]  ctc[j][l][i]
-    const float* waves;  // [T][nt][ldC]
-    float* cfmax;        // [W]
-    int* imax;           // [W]
+    float* B;              // [W][ldT] scores (updated)
+    float* X;              // [W][ldC] residual (updated by the final residual kernel)
+    const float* nm;       // [ldT]
+    const float* ctc;      // [T][2nt+1][ldT]  ctc[j][l][i]
+    const float* waves;    // [T][nt][ldC]
+    float* cfmax;          // [W]
+    int* imax;             // [W]
     unsigned char* flags;  // [roundup(W, 65536)] peak flags of the current pass
-    int* sp_col;         // [cap] peak column
-    int* sp_tmp;         // [cap] template
-    float* sp_amp;       // [cap]
+    int* sp_col;           // [cap] peak column
+    int* sp_tmp;           // [cap] template
+    float* sp_amp;         // [cap]
+    int* pass_off;         // [max_passes + 1] first spike of each pass that found spikes
     // [0] spikes of committed passes, [1] spikes of the last pass, [2] capacity overflow,
-    // [3] converged (a pass found nothing), [4] passes that found spikes
+    // [3] converged (a pass found nothing), [4] passes that found spikes, [5] residual list overflow
     int* counters;
     int64_t W;
     int64_t ldT, ldC;
-    int T, C, nt;
+    int T, C, K, nt;
     int cap;
     float th2;
 };
 hipError_t launch_reset_counters(int* counters, hipStream_t s);        // counters[0..7] = 0
 hipError_t launch_score_all(const PursuitArgs& a, hipStream_t s);     // T <= 1024
 hipError_t launch_peaks(const PursuitArgs& a, hipStream_t s);
-hipError_t launch_update(const PursuitArgs& a, int grid, hipStream_t s);
+hipError_t launch_update(const PursuitArgs& a, hipStream_t s);
+// residual of all passes in pass order (even, then odd spikes of a pass), after the last pass
+hipError_t launch_residual(const PursuitArgs& a, hipStream_t s);
 
 struct FeatureArgs {
     const float* X;      // [W][ldC] final residual

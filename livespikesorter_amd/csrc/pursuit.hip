@@ -8,14 +8,12 @@
 //   B[:, w-nt..w+nt] -= amp ctc[:, t, :];  Xres[:, w-nt/2..w+nt/2] -= amp wave_t
 //   in two groups (even, then odd spike index of the pass)                 (:195-198)
 //
-// Scores are reduced with one wavefront per sample column (wavefront-shuffle argmax).  Peaks are
-// found and compacted in ascending column order by one workgroup that scans Cfmax in LDS tiles;
-// only columns whose own score can pass the test look at their 2nt+1 neighbourhood.  The update is
-// a gather split into (spike, column chunk) work items: a column belongs to the first spike whose
-// window covers it, and gets, per parity, the last covering spike of that parity (even group
-// first) - the values the reference's gather-subtract-scatter leaves, including its
-// last-write-wins result for windows of one group that overlap (tied peaks).  Every updated score
-// column is rescored in place, so Cfmax/imax stay current without rescanning B.
+// Scores are reduced with one wavefront per sample column (wavefront-shuffle argmax) in the first
+// pass; afterwards every updated score column is rescored in place, so Cfmax/imax stay current
+// without rescanning B.  Peak flags are computed over the whole chip and compacted in ascending
+// column order by one workgroup.  The score update is a gather over (spike, column chunk) work
+// items that streams ctc rows; the residual update is deferred to one kernel after the last pass
+// (see the sections below for how both reproduce the reference's update order exactly).
 //
 // Spike counts live on the device: the pass loop is enqueued without host round trips; a pass
 // after convergence finds no peaks and its update exits at once.
@@ -204,16 +202,22 @@ __global__ __launch_bounds__(kEmitThreads) void peaks_emit_kernel(PursuitArgs a)
         const int n = s_run;
         a.counters[1] = n;
         if (s_base + n > a.cap) a.counters[2] = 1;
-        if (n == 0) a.counters[3] = 1;
-        a.counters[4] += (n > 0);
+        if (n == 0) {
+            a.counters[3] = 1;
+        } else {
+            const int p = a.counters[4];
+            a.pass_off[p] = s_base;
+            a.pass_off[p + 1] = min(s_base + n, a.cap);
+            a.counters[4] = p + 1;
+        }
     }
 }
 
 // ------------------------------------------------------------------------------------------------
-// update: work items (spike s, chunk); B chunks of kColsB columns, residual chunks of kColsX
+// update: work items (spike s, chunk of kColsB of its window's score columns)
 // ------------------------------------------------------------------------------------------------
 constexpr int kColsB = 8;     // score columns per work item (2 per wavefront)
-constexpr int kColsX = 16;    // residual columns per work item (4 per wavefront)
+constexpr int kUpdateGrid = 8192;   // workgroups per update launch (grid-stride over work items)
 
 // Spikes of the pass covering column w within +-reach, starting at spike s (whose window covers
 // w): the last even-indexed (e) and odd-indexed (o) ones.  Spikes are in ascending column order,
@@ -331,8 +335,7 @@ __global__ __launch_bounds__(256) void update_kernel(PursuitArgs a) {
     const int* tmp = a.sp_tmp + base;
     const float* amp = a.sp_amp + base;
     const int nB = (2 * nt + 1 + kColsB - 1) / kColsB;
-    const int nX = (2 * h + 1 + kColsX - 1) / kColsX;
-    const int per = nB + nX;
+    const int per = nB;
     floatx4 nmv[NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
@@ -354,37 +357,130 @@ __global__ __launch_bounds__(256) void update_kernel(PursuitArgs a) {
             const int c0 = ws - nt + sub * kColsB;
             const int w1 = min(c0 + kColsB - 1, hi);
             if (max(c0, lo) <= w1) update_score_columns<NV>(a, col, tmp, amp, s, n, c1, c2, c0, lo, w1, nmv);
-        } else {
-            const int xs = sub - nB;
-            int lo = ws - h;
-            if (s > 0) lo = max(lo, cp + h + 1);
-            lo = max(lo, 0);
-            const int hi = (int)min((int64_t)ws + h, a.W - 1);
-            const int c0 = ws - h + xs * kColsX;
-            for (int k = 0; k < kColsX / 4; ++k) {
-                const int w = c0 + 4 * k + wave;
-                if (w < lo || w > hi || w > c0 + kColsX - 1) continue;
-                int e, o;
-                covering(col, s, n, c1, c2, w, h, e, o);
-                float* xc = a.X + (int64_t)w * a.ldC;
-                const float* ve = (e >= 0) ? a.waves + ((int64_t)tmp[e] * nt + (w - col[e] + h)) * a.ldC : nullptr;
-                const float* vo = (o >= 0) ? a.waves + ((int64_t)tmp[o] * nt + (w - col[o] + h)) * a.ldC : nullptr;
-                const float ae = (e >= 0) ? amp[e] : 0.f, ao = (o >= 0) ? amp[o] : 0.f;
-                for (int c = 4 * lane; c < a.C; c += 256) {
-                    floatx4 x = *(const floatx4*)(xc + c);
-                    floatx4 pe, po;
-                    if (ve) pe = *(const floatx4*)(ve + c);
-                    if (vo) po = *(const floatx4*)(vo + c);
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// residual, once after the last pass.  A workgroup owns 64 columns; it finds, per pass, the spikes
+// whose +-nt/2 window reaches its columns (binary search in the pass's column-sorted list), lists
+// them in pursuit order in LDS, and each wavefront walks that list for its columns applying, pass
+// by pass, the last even then the last odd covering spike:
+//   X[w][c] -= amp * waves[t][w - col + nt/2][c]
+// which is the residual the reference's per-pass updates leave (template_matching.py:197).
+// ------------------------------------------------------------------------------------------------
+constexpr int kResCols = 64;
+constexpr int kResCap = 1024;       // listed spikes per workgroup
+constexpr int kResMaxPasses = 256;
+constexpr int kResBatch = 8;        // wave-row loads in flight per column
+
+template <int NVC>   // float4 channel chunks per lane: NVC*256 >= C
+__global__ __launch_bounds__(256) void residual_kernel(PursuitArgs a) {
+    __shared__ int s_first[kResMaxPasses], s_cnt[kResMaxPasses], s_off[kResMaxPasses + 1];
+    __shared__ int l_col[kResCap], l_tmp[kResCap], l_pp[kResCap];   // l_pp = pass * 2 + parity
+    __shared__ float l_amp[kResCap];
+    __shared__ int l_apply[4][2 * kResMaxPasses];                    // per wavefront
+    if (a.counters[2]) return;                  // spike list overflowed: the call reports an error
+    const int P = a.counters[4];                // passes that found spikes
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int h = a.nt / 2;
+    const int64_t w0 = (int64_t)blockIdx.x * kResCols;
+    const int lo = (int)(w0 - h), hi = (int)(w0 + kResCols - 1 + h);
+    for (int p = tid; p < P; p += 256) {
+        const int b0 = a.pass_off[p], b1 = a.pass_off[p + 1];
+        int l = b0, r = b1;                     // first index with col >= lo
+        while (l < r) {
+            const int m = (l + r) >> 1;
+            if (a.sp_col[m] < lo) l = m + 1; else r = m;
+        }
+        int l2 = l, r2 = b1;                    // first index with col > hi
+        while (l2 < r2) {
+            const int m = (l2 + r2) >> 1;
+            if (a.sp_col[m] <= hi) l2 = m + 1; else r2 = m;
+        }
+        s_first[p] = l;
+        s_cnt[p] = l2 - l;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        int acc = 0;
+        for (int p = 0; p < P; ++p) { s_off[p] = acc; acc += s_cnt[p]; }
+        s_off[P] = acc;
+        if (acc > kResCap) a.counters[5] = 1;
+    }
+    __syncthreads();
+    const int total = min(s_off[P], kResCap);
+    for (int p = tid; p < P; p += 256) {
+        for (int k = 0; k < s_cnt[p]; ++k) {
+            const int d = s_off[p] + k;
+            if (d >= kResCap) break;
+            const int j = s_first[p] + k;
+            l_col[d] = a.sp_col[j];
+            l_tmp[d] = a.sp_tmp[j];
+            l_amp[d] = a.sp_amp[j];
+            l_pp[d] = p * 2 + ((j - a.pass_off[p]) & 1);
+        }
+    }
+    __syncthreads();
+    if (s_off[P] > kResCap) return;
+    constexpr int nvc = NVC;
+    int* apply = l_apply[wave];
+    for (int cw = wave; cw < kResCols; cw += 4) {
+        const int64_t w = w0 + cw;
+        if (w >= a.W) break;
+        // 1) the ordered list of subtractions for this column: per pass, last even then last odd
+        int na = 0, cur = -1, e = -1, o = -1;
+        for (int d = 0; d <= total; ++d) {
+            const bool end = d == total;
+            const int pp = end ? -2 : l_pp[d];
+            const bool cov = !end && (int)(w - l_col[d]) <= h && (int)(l_col[d] - w) <= h;
+            if (end || (cov && (pp >> 1) != cur)) {
+                if (e >= 0) apply[na++] = e;
+                if (o >= 0) apply[na++] = o;
+                e = o = -1;
+                if (!end) cur = pp >> 1;
+            }
+            if (cov) {
+                if (pp & 1) o = d; else e = d;
+            }
+        }
+        // 2) apply them in order, with the wave rows of 8 subtractions loaded ahead
+        float* xr = a.X + w * a.ldC;
+        floatx4 x[NVC];
 #pragma unroll
-                    for (int e4 = 0; e4 < 4; ++e4) {
-                        float v = x[e4];
-                        if (ve) v = v - ae * pe[e4];
-                        if (vo) v = v - ao * po[e4];
-                        x[e4] = v;
+        for (int v = 0; v < nvc; ++v) {
+            const int c = 4 * lane + 256 * v;
+            if (c < a.C) x[v] = *(const floatx4*)(xr + c);
+        }
+        for (int b0 = 0; b0 < na; b0 += kResBatch) {
+            floatx4 wv[kResBatch][NVC];
+#pragma unroll
+            for (int q = 0; q < kResBatch; ++q) {
+                if (b0 + q < na) {
+                    const int d = apply[b0 + q];
+                    const float* wr = a.waves + ((int64_t)l_tmp[d] * a.nt + (w - l_col[d] + h)) * a.ldC;
+#pragma unroll
+                    for (int v = 0; v < nvc; ++v) {
+                        const int c = 4 * lane + 256 * v;
+                        if (c < a.C) wv[q][v] = *(const floatx4*)(wr + c);
                     }
-                    *(floatx4*)(xc + c) = x;
                 }
             }
+#pragma unroll
+            for (int q = 0; q < kResBatch; ++q) {
+                if (b0 + q < na) {
+                    const float am = l_amp[apply[b0 + q]];
+#pragma unroll
+                    for (int v = 0; v < nvc; ++v)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) x[v][r] = x[v][r] - am * wv[q][v][r];
+                }
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < nvc; ++v) {
+            const int c = 4 * lane + 256 * v;
+            if (c < a.C) *(floatx4*)(xr + c) = x[v];
         }
     }
 }
@@ -412,13 +508,24 @@ hipError_t launch_peaks(const PursuitArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_update(const PursuitArgs& a, int grid, hipStream_t s) {
+hipError_t launch_update(const PursuitArgs& a, hipStream_t s) {
     const int nv = (a.T + 255) / 256;
     switch (nv) {
-#define LSS_UP(V) case V: hipLaunchKernelGGL(update_kernel<V>, dim3(grid), dim3(256), 0, s, a); break;
+#define LSS_UP(V) case V: hipLaunchKernelGGL(update_kernel<V>, dim3(kUpdateGrid), dim3(256), 0, s, a); break;
         LSS_UP(1) LSS_UP(2) LSS_UP(3) LSS_UP(4)
 #undef LSS_UP
         default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_residual(const PursuitArgs& a, hipStream_t s) {
+    if (a.C > 1024) return hipErrorInvalidValue;
+    const unsigned nb = (unsigned)((a.W + kResCols - 1) / kResCols);
+    switch ((a.C + 255) / 256) {
+        case 1: hipLaunchKernelGGL(residual_kernel<1>, dim3(nb), dim3(256), 0, s, a); break;
+        case 2: hipLaunchKernelGGL(residual_kernel<2>, dim3(nb), dim3(256), 0, s, a); break;
+        default: hipLaunchKernelGGL(residual_kernel<4>, dim3(nb), dim3(256), 0, s, a); break;
     }
     return hipGetLastError();
 }
