@@ -2,19 +2,23 @@
 
 Workload (the drop-in configuration at full probe size): 384 channels, 600 templates, 6 PCs,
 nt = 61, 10 s of seeded synthetic int16 at 30 kHz cut into Kilosort batches of NT = 60000 samples
-(padded windows of 60122 samples), at most 50 pursuit passes per window, FP32.  One step = one
-padded window through the whole device path (CAR, highpass, whitening, projection, U.Xproj,
-pursuit, PC features, positions) with the int16 windows already resident in HBM.
+(padded windows of 60122 samples), at most 50 pursuit passes per window.  One step = one padded
+window through the whole device path (CAR, highpass, whitening, projection, U.Xproj, pursuit,
+residual, PC features, positions, spike-list readback) with the int16 windows already resident in
+HBM.  `value` is the FP32 path; the bf16x3 path runs the same windows and is reported beside it
+with its spike F1 against FP32.
 
 Prints one JSON line: x-realtime (seconds of recording per wall second), p50/p99 per-window
-latency, the roofline figure of the dominant kernel (measured with HIP events on the library's
-stream over the timed steps) and the CPU oracle timed on a bounded sample on the host.
+latency, the roofline figure of the dominant kernel (the pursuit update kernel, timed per launch
+with HIP events on the library's stream during the timed steps) and the CPU oracle timed on a
+bounded sample on the host.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -26,6 +30,47 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP32_MFMA_PEAK_TFLOPS = 157.3
 BF16_MFMA_PEAK_TFLOPS = 2500.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "update_kernel_traffic_384ch_600t.md")
+
+
+def run_path(sorter, dwin, starts, steps, warmup, profile=False):
+    import torch
+    nb = len(starts)
+    for i in range(warmup):
+        sorter.sort_window(dwin[i % nb], starts[i % nb])
+    torch.cuda.synchronize()
+    if profile:
+        sorter.profile_update(True)
+    lat, stage, outs = [], [], []
+    upd_ms, upd_n = 0.0, 0
+    t_start = time.perf_counter()
+    for i in range(steps):
+        t1 = time.perf_counter()
+        out = sorter.sort_window(dwin[i % nb], starts[i % nb])
+        lat.append(time.perf_counter() - t1)
+        stage.append(sorter.timings())
+        outs.append(out)
+        if profile:
+            ms, n = sorter.update_stats()
+            upd_ms += ms
+            upd_n += n
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t_start
+    if profile:
+        sorter.profile_update(False)
+    return dict(elapsed=elapsed, lat_ms=np.array(lat) * 1e3,
+                stage_ms={k: float(np.mean([d[k] for d in stage])) for k in stage[0]},
+                outs=outs, upd_ms=upd_ms, upd_n=upd_n)
+
+
+def read_traffic():
+    """HBM bytes per update launch measured with rocprofv3 --pmc (committed summary), or None."""
+    try:
+        txt = open(TRAFFIC_FILE).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt)
+    return float(m.group(1)) if m else None
 
 
 def main():
@@ -34,9 +79,12 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16", "bf16x3"])
+    ap.add_argument("--compare", default="bf16x3", help="second precision reported beside the first ('' = none)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="instead of the bench, copy a tensor of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each probe stream is "
@@ -44,8 +92,19 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 4}))
+        return
+
     from livespikesorter_amd import synth
     from livespikesorter_amd.sorter import Sorter, pad_batch
+    from livespikesorter_amd.metrics import f1_same_template
 
     t0 = time.time()
     s = synth.make(2, seed=args.seed)
@@ -64,39 +123,51 @@ def main():
     print(f"[bench] workload ready in {time.time() - t0:.1f}s: C={C} T={T} windows={nb} W'={Wp}",
           file=sys.stderr, flush=True)
 
-    sorter = Sorter(a, max_window=Wp, max_passes=50, precision=args.precision)
-    for i in range(args.warmup):
-        sorter.sort_window(dwin[i % nb], starts[i % nb])
-    torch.cuda.synchronize()
-    lat, stage = [], []
-    nsp = 0
-    passes = []
-    t_start = time.perf_counter()
-    for i in range(args.steps):
-        t1 = time.perf_counter()
-        out = sorter.sort_window(dwin[i % nb], starts[i % nb])
-        lat.append(time.perf_counter() - t1)
-        stage.append(sorter.timings())
-        nsp += len(out)
-        passes.append(out.passes)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t_start
-    print(f"[bench] {args.steps} windows in {elapsed:.3f}s", file=sys.stderr, flush=True)
-
+    sorter = Sorter(a, max_window=Wp, max_passes=50, precision=args.precision, max_spikes=1 << 16)
+    r = run_path(sorter, dwin, starts, args.steps, args.warmup, profile=True)
+    print(f"[bench] {args.precision}: {args.steps} windows in {r['elapsed']:.3f}s", file=sys.stderr, flush=True)
     fs = a["fs"]
-    xrt = args.steps * NT / fs / elapsed
-    lat_ms = np.array(lat) * 1e3
-    st_ms = {k: float(np.mean([d[k] for d in stage])) for k in stage[0]}
+    xrt = args.steps * NT / fs / r["elapsed"]
+    spikes = sum(len(o) for o in r["outs"])
+    passes = float(np.mean([o.passes for o in r["outs"]]))
 
-    # dominant kernel: the larger of the U.Xproj GEMM and the pursuit's update traffic
+    # dominant kernel: the pursuit update (per accepted spike it reads and writes the B rows of the
+    # spike's 2nt+1 columns and reads as many ctc rows: 3 * 4 * T * (2nt+1) bytes)
+    bytes_per_spike = 3 * 4 * T * (2 * nt + 1)
+    upd_launches = r["upd_n"]
+    per_launch_bytes = spikes * bytes_per_spike / max(upd_launches, 1)
+    avg_launch_ms = r["upd_ms"] / max(upd_launches, 1)
+    achieved = per_launch_bytes / (avg_launch_ms * 1e-3) / 1e9
+    traffic = read_traffic()
+    roofline = {"kernel": "update_kernel (pursuit score update)", "bound": "hbm",
+                "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic,
+                "per_launch": f"{bytes_per_spike / 1e3:.1f} KB per accepted spike x "
+                              f"{spikes / max(upd_launches, 1):.1f} spikes per launch (avg over "
+                              f"{upd_launches // args.steps} launches per window incl. converged ones), "
+                              f"avg {avg_launch_ms * 1e3:.1f} us per launch"}
     gemm_flops = 2.0 * T * K * C * Wp
-    gemm_ms = st_ms["scores_gemm"]
-    peak = FP32_MFMA_PEAK_TFLOPS if args.precision == "fp32" else BF16_MFMA_PEAK_TFLOPS
-    achieved = gemm_flops / (gemm_ms * 1e-3) / 1e12
-    roofline = {"kernel": "gemm_f32_kernel (U.Xproj scores)", "bound": "mfma",
-                "achieved": round(achieved, 2), "peak": peak, "unit": "TFLOP/s",
-                "frac": round(achieved / peak, 4), "traffic": None,
-                "per_launch": f"2*T*K*C*W' = {gemm_flops / 1e9:.1f} GFLOP, avg {gemm_ms:.3f} ms"}
+    gemm_ms = r["stage_ms"]["scores_gemm"]
+    peak_gemm = FP32_MFMA_PEAK_TFLOPS if args.precision == "fp32" else BF16_MFMA_PEAK_TFLOPS
+    gemm = {"kernel": "U.Xproj GEMM", "bound": "mfma",
+            "achieved": round(gemm_flops / (gemm_ms * 1e-3) / 1e12, 2), "peak": peak_gemm, "unit": "TFLOP/s",
+            "frac": round(gemm_flops / (gemm_ms * 1e-3) / 1e12 / peak_gemm, 4)}
+
+    compare = None
+    if args.compare and args.compare != args.precision:
+        sorter2 = Sorter(a, max_window=Wp, max_passes=50, precision=args.compare, max_spikes=1 << 16)
+        r2 = run_path(sorter2, dwin, starts, args.steps, args.warmup)
+        f1s = [f1_same_template(o1.sample, o1.template, o2.sample, o2.template, dt=1)
+               for o1, o2 in zip(r["outs"][:nb], r2["outs"][:nb])]
+        compare = {"precision": args.compare,
+                   "value": round(args.steps * NT / fs / r2["elapsed"], 2),
+                   "p50_ms": round(float(np.percentile(r2["lat_ms"], 50)), 3),
+                   "p99_ms": round(float(np.percentile(r2["lat_ms"], 99)), 3),
+                   "scores_gemm_ms": round(r2["stage_ms"]["scores_gemm"], 4),
+                   "whiten_ms": round(r2["stage_ms"]["whiten"], 4),
+                   "spike_f1_vs_fp32_min": round(float(min(f1s)), 6)}
+        sorter2.close()
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -117,19 +188,21 @@ def main():
     res = {
         "metric": "x-realtime & p99 batch latency, 384ch@30kHz; bf16x3 vs fp32 at spike F1>=0.999",
         "value": round(xrt, 2), "unit": "x realtime", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "warmup": args.warmup, "ms_per_step": round(r["elapsed"] / args.steps * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": args.precision,
         "data": "synthetic (seeded probe, templates and spikes; no recordings)",
         "config": {"workload": "384 ch, 600 templates, 6 PCs, nt=61, NT=60000-sample Kilosort windows "
                                "(W'=60122), 10 s recording cycled, <=50 pursuit passes",
                    "n_chan": C, "n_templates": T, "window": int(Wp), "NT": NT, "max_passes": 50,
                    "precision": args.precision},
-        "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
-        "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
-        "spikes_per_window": round(nsp / args.steps, 1),
-        "passes_per_window": round(float(np.mean(passes)), 2),
-        "stage_ms": {k: round(v, 4) for k, v in st_ms.items()},
+        "p50_ms": round(float(np.percentile(r["lat_ms"], 50)), 3),
+        "p99_ms": round(float(np.percentile(r["lat_ms"], 99)), 3),
+        "spikes_per_window": round(spikes / args.steps, 1),
+        "passes_per_window": round(passes, 2),
+        "stage_ms": {k: round(v, 4) for k, v in r["stage_ms"].items()},
         "roofline": roofline,
+        "gemm_roofline": gemm,
+        "compare": compare,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res))

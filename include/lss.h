@@ -153,6 +153,12 @@ lss_status lss_get_tables(void* handle, float* nm, float* waves, float* ctc, flo
  * [5] pursuit, [6] features + positions, [7] total; n <= 8. */
 lss_status lss_last_timings(void* handle, float* ms, int32_t n);
 
+/* Measurement mode: time every launch of the pursuit update kernel with HIP events on the
+ * handle's stream; lss_last_update_stats gives the last window's summed update time and the number
+ * of update launches (max_passes per window). */
+lss_status lss_profile_update(void* handle, int32_t enable);
+lss_status lss_last_update_stats(void* handle, float* total_ms, int32_t* launches);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,7 +19,7 @@ EXPORTS = ["lss_last_error", "lss_abi_version", "lss_default_params", "lss_creat
            "lss_create_from_dir", "lss_destroy", "lss_get_params", "lss_set_stream",
            "lss_sort_window", "lss_pad_batch", "lss_stage_filter", "lss_stage_car",
            "lss_stage_highpass", "lss_stage_whiten", "lss_stage_scores", "lss_stage_pursuit",
-           "lss_get_tables", "lss_last_timings"]
+           "lss_get_tables", "lss_last_timings", "lss_profile_update", "lss_last_update_stats"]
 
 
 class Params(ct.Structure):
@@ -76,6 +76,8 @@ def lib() -> ct.CDLL:
             "lss_stage_pursuit": ([vp, vp, vp, i64, i64, ct.POINTER(Spikes)], i32),
             "lss_get_tables": ([vp, vp, vp, vp, vp], i32),
             "lss_last_timings": ([vp, vp, i32], i32),
+            "lss_profile_update": ([vp, i32], i32),
+            "lss_last_update_stats": ([vp, ct.POINTER(ct.c_float), ct.POINTER(i32)], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

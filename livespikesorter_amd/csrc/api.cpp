@@ -83,11 +83,18 @@ struct Handle {
     int* h_counters = nullptr;   // pinned
 
     hipEvent_t ev[8] = {};
+    // per-launch timing of the pursuit update kernel (measurement mode)
+    bool prof_update = false;
+    std::vector<hipEvent_t> upd_ev;
+    float upd_ms = 0.f;
+    int upd_launches = 0;
     float last_ms[8] = {};   // car, highpass, whiten, project, scores GEMM, pursuit, features, total
     bool alive = false;
 
     ~Handle() {
         for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto& e : upd_ev)
             if (e) (void)hipEventDestroy(e);
         if (h_counters) (void)hipHostFree(h_counters);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
@@ -382,7 +389,9 @@ lss_status run_pursuit(Handle& h, float* B, float* X, int64_t W) {
     HIP_TRY(lss::launch_score_all(a, h.stream));
     for (int p = 0; p < h.prm.max_passes; ++p) {
         HIP_TRY(lss::launch_peaks(a, h.stream));
+        if (h.prof_update) HIP_TRY(hipEventRecord(h.upd_ev[2 * p], h.stream));
         HIP_TRY(lss::launch_update(a, h.stream));
+        if (h.prof_update) HIP_TRY(hipEventRecord(h.upd_ev[2 * p + 1], h.stream));
     }
     HIP_TRY(lss::launch_residual(a, h.stream));
     return LSS_OK;
@@ -410,6 +419,15 @@ lss_status read_counts(Handle& h, int* n, int* passes) {
                                       "(residual list); the residual is incomplete");
     *n = h.h_counters[0] + h.h_counters[1];
     *passes = h.h_counters[4];
+    if (h.prof_update) {
+        h.upd_ms = 0.f;
+        h.upd_launches = h.prm.max_passes;
+        for (int p = 0; p < h.prm.max_passes; ++p) {
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, h.upd_ev[2 * p], h.upd_ev[2 * p + 1]));
+            h.upd_ms += ms;
+        }
+    }
     return LSS_OK;
 }
 
@@ -737,6 +755,26 @@ lss_status lss_get_tables(void* handle, float* nm, float* waves, float* ctc, flo
         HIP_TRY(hipMemcpy2DAsync(wmat, sizeof(float) * h.C, h.wmat.p, sizeof(float) * h.ldm, sizeof(float) * h.C, h.C,
                                  hipMemcpyDeviceToHost, h.stream));
     HIP_TRY(hipStreamSynchronize(h.stream));
+    return LSS_OK;
+}
+
+lss_status lss_profile_update(void* handle, int32_t enable) {
+    if (!handle) return fail(LSS_ERR_ARG, "null handle");
+    Handle& h = *as_handle(handle);
+    if (enable && h.upd_ev.empty()) {
+        h.upd_ev.assign(2 * (size_t)h.prm.max_passes, nullptr);
+        for (auto& e : h.upd_ev) HIP_TRY(hipEventCreate(&e));
+    }
+    h.prof_update = enable != 0;
+    return LSS_OK;
+}
+
+lss_status lss_last_update_stats(void* handle, float* total_ms, int32_t* launches) {
+    if (!handle || !total_ms || !launches) return fail(LSS_ERR_ARG, "null argument");
+    Handle& h = *as_handle(handle);
+    if (!h.prof_update) return fail(LSS_ERR_STATE, "update profiling is off (lss_profile_update)");
+    *total_ms = h.upd_ms;
+    *launches = h.upd_launches;
     return LSS_OK;
 }
 

@@ -201,6 +201,16 @@ class Sorter:
         check(lib().lss_last_timings(self._h, ms.ctypes.data, 8), "lss_last_timings")
         return dict(zip(TIMING_KEYS, ms.tolist()))
 
+    def profile_update(self, enable: bool = True):
+        """Time each pursuit update launch with HIP events (measurement mode)."""
+        check(lib().lss_profile_update(self._h, int(enable)), "lss_profile_update")
+
+    def update_stats(self):
+        """(summed update-kernel ms, update launches) of the last window in measurement mode."""
+        ms, n = ct.c_float(), ct.c_int32()
+        check(lib().lss_last_update_stats(self._h, ct.byref(ms), ct.byref(n)), "lss_last_update_stats")
+        return ms.value, n.value
+
     # ---- stage entry points (torch CUDA tensors) ----------------------------------------------
     def stage_car(self, win_dev, out=None):
         import torch
