@@ -153,6 +153,28 @@ lss_status lss_get_tables(void* handle, float* nm, float* waves, float* ctc, flo
  * [5] pursuit, [6] features + positions, [7] total; n <= 8. */
 lss_status lss_last_timings(void* handle, float* ms, int32_t n);
 
+/* ---- output step (host only; OnlineSpikesV2::closestCluster / saveSpikes / writeSpikesToFile,
+ * src/SpikeSorter/OnlineSpikesV2.cu:1028-1043, 1117-1164; Kilosort remove_duplicates,
+ * kilosort/postprocessing.py:10-30) ------------------------------------------------------------- */
+
+/* label[i] = index of the cluster centroid (cx, cy) nearest to spike position (x[i], y[i])
+ * (Euclidean, first index on ties). */
+lss_status lss_label_spikes(const float* x, const float* y, int64_t n, const float* cx, const float* cy,
+                            int32_t n_clusters, int32_t* label);
+
+/* In-place removal of same-label spikes closer than dt samples to the previous kept spike of that
+ * label.  Spikes must be in ascending sample order.  last_kept [n_labels] carries the last kept
+ * sample of each label across calls (initialise to INT64_MIN/2 before the first window);
+ * amplitude may be NULL.  *n_out = spikes kept. */
+lss_status lss_dedup_spikes(int64_t* sample, int32_t* label, float* amplitude, int64_t n, int32_t dt,
+                            int64_t* last_kept, int32_t n_labels, int64_t* n_out);
+
+/* Appends "sample,label,amplitude,y" lines (sample + recording_offset) to a text file, as the
+ * reference's spikeOutput.txt.  append = 0 truncates first. */
+lss_status lss_write_spikes_txt(const char* path, const int64_t* sample, const int32_t* label,
+                                const float* amplitude, const float* y, int64_t n,
+                                int64_t recording_offset, int32_t append);
+
 /* Measurement mode: time every launch of the pursuit update kernel with HIP events on the
  * handle's stream; lss_last_update_stats gives the last window's summed update time and the number
  * of update launches (max_passes per window). */

@@ -19,7 +19,8 @@ EXPORTS = ["lss_last_error", "lss_abi_version", "lss_default_params", "lss_creat
            "lss_create_from_dir", "lss_destroy", "lss_get_params", "lss_set_stream",
            "lss_sort_window", "lss_pad_batch", "lss_stage_filter", "lss_stage_car",
            "lss_stage_highpass", "lss_stage_whiten", "lss_stage_scores", "lss_stage_pursuit",
-           "lss_get_tables", "lss_last_timings", "lss_profile_update", "lss_last_update_stats"]
+           "lss_get_tables", "lss_last_timings", "lss_profile_update", "lss_last_update_stats",
+           "lss_label_spikes", "lss_dedup_spikes", "lss_write_spikes_txt"]
 
 
 class Params(ct.Structure):
@@ -78,6 +79,9 @@ def lib() -> ct.CDLL:
             "lss_last_timings": ([vp, vp, i32], i32),
             "lss_profile_update": ([vp, i32], i32),
             "lss_last_update_stats": ([vp, ct.POINTER(ct.c_float), ct.POINTER(i32)], i32),
+            "lss_label_spikes": ([vp, vp, i64, vp, vp, i32, vp], i32),
+            "lss_dedup_spikes": ([vp, vp, vp, i64, i32, vp, i32, ct.POINTER(i64)], i32),
+            "lss_write_spikes_txt": ([ct.c_char_p, vp, vp, vp, vp, i64, i64, i32], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

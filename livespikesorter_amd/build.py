@@ -18,7 +18,7 @@ CSRC = PKG / "csrc"
 LIB = PKG / "liblss.so"
 ARCH = os.environ.get("LSS_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["api.cpp", "npy.cpp", "preprocess.hip", "gemm.hip", "project.hip",
+SOURCES = ["api.cpp", "npy.cpp", "output.cpp", "preprocess.hip", "gemm.hip", "project.hip",
            "pursuit.hip", "tables.hip"]
 
 # Packed FP32 instructions are disabled for all device code.

@@ -333,6 +333,14 @@ lss_status create_impl(const lss_params* p, const lss_artifacts* a, Handle** out
 
 Handle* as_handle(void* p) { return static_cast<Handle*>(p); }
 
+}  // namespace
+
+namespace lss {
+lss_status set_error(lss_status st, const char* msg) { return fail(st, msg); }
+}  // namespace lss
+
+namespace {
+
 lss_status check_window(Handle& h, int64_t W) {
     if (W < 2 * (int64_t)h.nt + 2) return fail(LSS_ERR_ARG, "window shorter than 2*nt+2");
     if (W > h.Wmax) return fail(LSS_ERR_ARG, "window longer than max_window");

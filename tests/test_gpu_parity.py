@@ -233,3 +233,18 @@ def test_sort_window_precision_vs_fp32(sorter, lp_sorters, synth1, prec):
         assert f1 >= 0.999 and err <= 1e-3
     else:
         assert f1 >= 0.95
+
+
+def test_from_dir_matches_in_memory_artifacts(sorter, synth1, tmp_path):
+    """oss_input export -> lss_create_from_dir gives the same spikes as the in-memory artifacts."""
+    print("[lss] test_from_dir_matches_in_memory_artifacts", flush=True)
+    from livespikesorter_amd.export import write_oss_input
+    from livespikesorter_amd.sorter import Sorter
+    d = write_oss_input(synth1.art, tmp_path / "oss_input")
+    win, _, ws = _window(synth1, 1)
+    ref = sorter.sort_window(win, ws)
+    with Sorter.from_dir(d, max_window=60122, max_passes=100) as s2:
+        got = s2.sort_window(win, ws)
+    np.testing.assert_array_equal(got.sample, ref.sample)
+    np.testing.assert_array_equal(got.template, ref.template)
+    np.testing.assert_array_equal(got.amplitude, ref.amplitude)

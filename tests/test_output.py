@@ -1,0 +1,61 @@
+"""Output step (host-only C ABI functions) and artifact export: checked against brute force and
+against the oracle's restatement of Kilosort's duplicate removal (postprocessing.py:10-30)."""
+import numpy as np
+
+from oracle import ks_oracle as ko
+
+
+def test_label_spikes_bruteforce():
+    from livespikesorter_amd.output import label_spikes
+    rng = np.random.default_rng(0)
+    x, y = rng.uniform(0, 40, 500), rng.uniform(0, 400, 500)
+    cx, cy = rng.uniform(0, 40, 37), rng.uniform(0, 400, 37)
+    got = label_spikes(x, y, cx, cy)
+    d = np.sqrt((x[:, None].astype(np.float32) - cx.astype(np.float32)) ** 2 +
+                (y[:, None].astype(np.float32) - cy.astype(np.float32)) ** 2)
+    np.testing.assert_array_equal(got, np.argmin(d, 1))
+    # ties: the first centroid wins
+    assert label_spikes([1.0], [0.0], [0.0, 2.0], [0.0, 0.0])[0] == 0
+
+
+def test_dedup_matches_kilosort_rule_and_carries_state():
+    from livespikesorter_amd.output import Deduplicator
+    rng = np.random.default_rng(1)
+    t = np.sort(rng.integers(0, 5000, 800))
+    c = rng.integers(0, 6, 800)
+    dd = Deduplicator(6, dt=7)
+    s, l, _ = dd(t, c)
+    rt, rc, _ = ko.remove_duplicates(t, c, dt=7)
+    np.testing.assert_array_equal(s, rt)
+    np.testing.assert_array_equal(l, rc)
+    # split into two windows with an overlap: the state removes the re-found spikes
+    dd2 = Deduplicator(6, dt=7)
+    s1, l1, _ = dd2(t[t < 2600], c[t < 2600])
+    s2, l2, _ = dd2(t[t >= 2500], c[t >= 2500])
+    both = np.concatenate([s1, s2])
+    np.testing.assert_array_equal(np.sort(both), rt)
+
+
+def test_write_spikes_txt(tmp_path):
+    from livespikesorter_amd.output import write_spikes_txt
+    p = tmp_path / "spikeOutput.txt"
+    write_spikes_txt(p, [10, 20], [1, 2], [0.5, 1.25], [40.0, 60.0], recording_offset=1000, append=False)
+    write_spikes_txt(p, [30], [3], [2.0], [80.0], recording_offset=1000)
+    assert p.read_text().splitlines() == ["1010,1,0.5,40", "1020,2,1.25,60", "1030,3,2,80"]
+
+
+def test_export_oss_input(tmp_path, small_synth):
+    from livespikesorter_amd.export import write_oss_input
+    a = small_synth.art
+    d = write_oss_input(a, tmp_path / "oss_input")
+    U = np.load(d / "Wall3.npy")
+    np.testing.assert_array_equal(U, a["U"])
+    np.testing.assert_array_equal(np.load(d / "drift_matrix.npy").T, a["drift"])
+    misc = dict(l.split(":") for l in (d / "misc.txt").read_text().split())
+    assert misc["nt0min"] == "20" and misc["Th_learned"] == "8"
+    iCC = np.load(d / "iCC.npy")
+    import torch
+    iCC_o, iU_o, Ucc_o = ko.prepare_extract(a["xc"].astype(np.float64), a["yc"].astype(np.float64),
+                                            torch.from_numpy(a["U"]).double(), int(a["nC"]))
+    np.testing.assert_array_equal(iCC, iCC_o.numpy())
+    np.testing.assert_array_equal(np.load(d / "iU.npy"), iU_o.numpy())
