@@ -175,6 +175,38 @@ lss_status lss_write_spikes_txt(const char* path, const int64_t* sample, const i
                                 const float* amplitude, const float* y, int64_t n,
                                 int64_t recording_offset, int32_t append);
 
+/* ---- multi-probe streaming (OnlineSpikesV2::runSpikeSorting's batching loop, V2:632-820, for
+ * several probes on one sample clock) ------------------------------------------------------------ */
+
+/* Creates a batcher over n_probes sorter handles (one per probe, distinct).  Each step sorts the
+ * next `batch` samples of every probe: probe p's window is samples [t0 - nt, t0 + batch + nt)
+ * (samples before 0 repeat sample 0), so consecutive steps report disjoint sample ranges.  With
+ * use_graph the device work of a step is two replayed HIP graphs whose branches run the probes
+ * concurrently: (upload, preprocessing, scores, first P1 pursuit passes) and (residual, features,
+ * readback); probes that have not converged after P1 passes (lss_autotune / lss_set_tuning) get
+ * their remaining passes in between.  Each handle needs max_window >= batch + 2*nt. */
+lss_status lss_stream_create(void* const* handles, int32_t n_probes, int32_t batch, int32_t use_graph,
+                             void** stream);
+lss_status lss_stream_destroy(void* stream);
+/* Appends n samples (int16 [n][C] interleaved, host memory) of one probe. */
+lss_status lss_stream_push(void* stream, int32_t probe, const int16_t* samples, int64_t n);
+/* Runs one batch when every probe holds enough samples (*ready = 1) and fills outs[0..n_probes);
+ * otherwise *ready = 0 and nothing runs. */
+lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready);
+/* Wall time of the last step in ms (upload to spike lists) and steps done. */
+lss_status lss_stream_stats(void* stream, float* last_ms, int64_t* batches);
+
+/* ---- tuning ------------------------------------------------------------------------------------- */
+
+/* Times the window (device memory) with each candidate update grid (256 .. 8192 workgroups) and
+ * keeps the fastest; sets the batcher's first-segment pass count to the passes seen + 4 (capped at
+ * max_passes).  grid_ms (6 floats, may be NULL) receives the best time per candidate.  Results do
+ * not depend on the tuning. */
+lss_status lss_autotune(void* handle, const int16_t* d_window, int64_t W, int32_t reps, float* grid_ms,
+                        int32_t* best_grid, int32_t* first_passes);
+lss_status lss_set_tuning(void* handle, int32_t update_grid, int32_t first_passes);
+lss_status lss_get_tuning(void* handle, int32_t* update_grid, int32_t* first_passes);
+
 /* Measurement mode: time every launch of the pursuit update kernel with HIP events on the
  * handle's stream; lss_last_update_stats gives the last window's summed update time and the number
  * of update launches (max_passes per window). */

@@ -20,7 +20,9 @@ EXPORTS = ["lss_last_error", "lss_abi_version", "lss_default_params", "lss_creat
            "lss_sort_window", "lss_pad_batch", "lss_stage_filter", "lss_stage_car",
            "lss_stage_highpass", "lss_stage_whiten", "lss_stage_scores", "lss_stage_pursuit",
            "lss_get_tables", "lss_last_timings", "lss_profile_update", "lss_last_update_stats",
-           "lss_label_spikes", "lss_dedup_spikes", "lss_write_spikes_txt"]
+           "lss_label_spikes", "lss_dedup_spikes", "lss_write_spikes_txt",
+           "lss_stream_create", "lss_stream_destroy", "lss_stream_push", "lss_stream_step",
+           "lss_stream_stats", "lss_autotune", "lss_set_tuning", "lss_get_tuning"]
 
 
 class Params(ct.Structure):
@@ -82,6 +84,14 @@ def lib() -> ct.CDLL:
             "lss_label_spikes": ([vp, vp, i64, vp, vp, i32, vp], i32),
             "lss_dedup_spikes": ([vp, vp, vp, i64, i32, vp, i32, ct.POINTER(i64)], i32),
             "lss_write_spikes_txt": ([ct.c_char_p, vp, vp, vp, vp, i64, i64, i32], i32),
+            "lss_stream_create": ([ct.POINTER(vp), i32, i32, i32, ct.POINTER(vp)], i32),
+            "lss_stream_destroy": ([vp], i32),
+            "lss_stream_push": ([vp, i32, vp, i64], i32),
+            "lss_stream_step": ([vp, ct.POINTER(Spikes), ct.POINTER(i32)], i32),
+            "lss_stream_stats": ([vp, ct.POINTER(ct.c_float), ct.POINTER(i64)], i32),
+            "lss_autotune": ([vp, vp, i64, i32, vp, ct.POINTER(i32), ct.POINTER(i32)], i32),
+            "lss_set_tuning": ([vp, i32, i32], i32),
+            "lss_get_tuning": ([vp, ct.POINTER(i32), ct.POINTER(i32)], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -81,10 +82,17 @@ struct Handle {
     DevBuf<int> sp_col, sp_tmp, counters, pass_off;
     DevBuf<float> sp_amp, tF, xs, ys;
     int* h_counters = nullptr;   // pinned
+    // pinned readback staging of the batched/streaming path (first stage_cap spikes)
+    int stage_cap = 0;
+    int* hs_counters = nullptr;
+    int *hs_col = nullptr, *hs_tmp = nullptr;
+    float *hs_amp = nullptr, *hs_tF = nullptr, *hs_xs = nullptr, *hs_ys = nullptr;
 
     hipEvent_t ev[8] = {};
     // per-launch timing of the pursuit update kernel (measurement mode)
     bool prof_update = false;
+    int upd_grid = 8192;        // workgroups of the update kernel (autotuned)
+    int tuned_passes = 0;       // passes in the first graph segment of the batcher (0 = all)
     std::vector<hipEvent_t> upd_ev;
     float upd_ms = 0.f;
     int upd_launches = 0;
@@ -97,6 +105,9 @@ struct Handle {
         for (auto& e : upd_ev)
             if (e) (void)hipEventDestroy(e);
         if (h_counters) (void)hipHostFree(h_counters);
+        for (void* q : {(void*)hs_counters, (void*)hs_col, (void*)hs_tmp, (void*)hs_amp, (void*)hs_tF,
+                        (void*)hs_xs, (void*)hs_ys})
+            if (q) (void)hipHostFree(q);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -383,9 +394,7 @@ lss_status run_scores(Handle& h, const float* x_wc, float* b_wt, int64_t W, hipE
 
 constexpr int kFeatureGrid = 1024;
 
-// Pursuit over the handle's B [W][ldT] and residual X [W][ldC]: max_passes passes enqueued with no
-// host round trip (counts stay on the device), then the features of every spike.
-lss_status run_pursuit(Handle& h, float* B, float* X, int64_t W) {
+lss::PursuitArgs pursuit_args(Handle& h, float* B, float* X, int64_t W) {
     lss::PursuitArgs a{};
     a.B = B; a.X = X; a.nm = h.nm.p; a.ctc = h.ctc.p; a.waves = h.waves.p;
     a.pass_off = h.pass_off.p;
@@ -393,14 +402,29 @@ lss_status run_pursuit(Handle& h, float* B, float* X, int64_t W) {
     a.sp_col = h.sp_col.p; a.sp_tmp = h.sp_tmp.p; a.sp_amp = h.sp_amp.p; a.counters = h.counters.p;
     a.W = W; a.ldT = h.ldT; a.ldC = h.ldC; a.T = h.T; a.C = h.C; a.K = h.K; a.nt = h.nt; a.cap = h.prm.max_spikes;
     a.th2 = h.prm.th_learned * h.prm.th_learned;
-    HIP_TRY(lss::launch_reset_counters(h.counters.p, h.stream));
-    HIP_TRY(lss::launch_score_all(a, h.stream));
-    for (int p = 0; p < h.prm.max_passes; ++p) {
+    return a;
+}
+
+// Pursuit passes [p0, p1) on the handle's stream with no host round trip (counts stay on the
+// device); pass 0 also resets the counters and scores every column.
+lss_status enqueue_passes(Handle& h, const lss::PursuitArgs& a, int p0, int p1) {
+    if (p0 == 0) {
+        HIP_TRY(lss::launch_reset_counters(h.counters.p, h.stream));
+        HIP_TRY(lss::launch_score_all(a, h.stream));
+    }
+    for (int p = p0; p < p1; ++p) {
         HIP_TRY(lss::launch_peaks(a, h.stream));
         if (h.prof_update) HIP_TRY(hipEventRecord(h.upd_ev[2 * p], h.stream));
-        HIP_TRY(lss::launch_update(a, h.stream));
+        HIP_TRY(lss::launch_update(a, h.upd_grid, h.stream));
         if (h.prof_update) HIP_TRY(hipEventRecord(h.upd_ev[2 * p + 1], h.stream));
     }
+    return LSS_OK;
+}
+
+// All max_passes passes, then the residual of every pass.
+lss_status run_pursuit(Handle& h, float* B, float* X, int64_t W) {
+    const lss::PursuitArgs a = pursuit_args(h, B, X, W);
+    if (lss_status st = enqueue_passes(h, a, 0, h.prm.max_passes)) return st;
     HIP_TRY(lss::launch_residual(a, h.stream));
     return LSS_OK;
 }
@@ -414,6 +438,44 @@ lss_status run_features(Handle& h, const float* X, int64_t W) {
     f.n = 0; f.cap = h.prm.max_spikes; f.C = h.C; f.T = h.T; f.K = h.K; f.nt = h.nt; f.nC = h.nC;
     f.W = W; f.ldC = h.ldC;
     HIP_TRY(lss::launch_features(f, kFeatureGrid, h.stream));
+    return LSS_OK;
+}
+
+// Device part of one window up to the scores (no events, no host synchronisation: capturable).
+lss_status enqueue_pre(Handle& h, const int16_t* d_win, int64_t W) {
+    if (lss_status st = run_car(h, d_win, W)) return st;
+    HIP_TRY(lss::launch_highpass(h.xcw.p, h.ldw, h.xhp.p, h.ldw, h.C, W, h.spec_lo, h.spec_hi, h.spec.p, h.tw.p,
+                                 h.stream));
+    if (lss_status st = run_whiten(h, h.xhp.p, h.xw.p, W)) return st;
+    return run_scores(h, h.xw.p, h.B.p, W);
+}
+
+lss_status ensure_stage(Handle& h) {
+    if (h.stage_cap) return LSS_OK;
+    const int cap = std::min(h.prm.max_spikes, 16384);
+    const int FK = h.nC * h.K;
+    HIP_TRY(hipHostMalloc((void**)&h.hs_counters, sizeof(int) * 8, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&h.hs_col, sizeof(int) * cap, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&h.hs_tmp, sizeof(int) * cap, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&h.hs_amp, sizeof(float) * cap, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&h.hs_tF, sizeof(float) * cap * FK, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&h.hs_xs, sizeof(float) * cap, hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&h.hs_ys, sizeof(float) * cap, hipHostMallocDefault));
+    h.stage_cap = cap;
+    return LSS_OK;
+}
+
+// Counters and the first stage_cap spikes to pinned host memory (capturable).
+lss_status enqueue_readback(Handle& h) {
+    const int cap = h.stage_cap;
+    const int FK = h.nC * h.K;
+    HIP_TRY(hipMemcpyAsync(h.hs_counters, h.counters.p, sizeof(int) * 8, hipMemcpyDeviceToHost, h.stream));
+    HIP_TRY(hipMemcpyAsync(h.hs_col, h.sp_col.p, sizeof(int) * cap, hipMemcpyDeviceToHost, h.stream));
+    HIP_TRY(hipMemcpyAsync(h.hs_tmp, h.sp_tmp.p, sizeof(int) * cap, hipMemcpyDeviceToHost, h.stream));
+    HIP_TRY(hipMemcpyAsync(h.hs_amp, h.sp_amp.p, sizeof(float) * cap, hipMemcpyDeviceToHost, h.stream));
+    HIP_TRY(hipMemcpyAsync(h.hs_tF, h.tF.p, sizeof(float) * cap * FK, hipMemcpyDeviceToHost, h.stream));
+    HIP_TRY(hipMemcpyAsync(h.hs_xs, h.xs.p, sizeof(float) * cap, hipMemcpyDeviceToHost, h.stream));
+    HIP_TRY(hipMemcpyAsync(h.hs_ys, h.ys.p, sizeof(float) * cap, hipMemcpyDeviceToHost, h.stream));
     return LSS_OK;
 }
 
@@ -439,6 +501,30 @@ lss_status read_counts(Handle& h, int* n, int* passes) {
     return LSS_OK;
 }
 
+// Host-side emission of n device-order spikes from host copies of the spike arrays.
+lss_status emit_from(Handle& h, int n, int passes, int64_t window_start, lss_spikes* out, const int* col,
+                     const int* tmp, const float* amp, const float* tF, const float* xs, const float* ys) {
+    const int FK = h.nC * h.K;
+    int64_t k = 0;
+    const int64_t shift = window_start - h.nt / 2 + h.prm.nt0min;   // template_matching.py:68
+    for (int i = 0; i < n; ++i) {
+        const int64_t smp = shift + col[i];
+        if (h.prm.drop_negative && smp < 0) continue;   // template_matching.py:53-59
+        if (k >= out->capacity) return fail(LSS_ERR_CAPACITY, "output capacity exceeded");
+        if (out->sample) out->sample[k] = smp;
+        if (out->template_id) out->template_id[k] = tmp[i];
+        if (out->amplitude) out->amplitude[k] = amp[i];
+        if (out->column) out->column[k] = col[i];
+        if (out->features) std::memcpy(out->features + k * FK, tF + (size_t)i * FK, sizeof(float) * FK);
+        if (out->x) out->x[k] = xs[i];
+        if (out->y) out->y[k] = ys[i];
+        ++k;
+    }
+    out->n = k;
+    out->passes = passes;
+    return LSS_OK;
+}
+
 lss_status emit_spikes(Handle& h, int n, int passes, int64_t window_start, lss_spikes* out) {
     out->n = 0;
     out->passes = passes;
@@ -453,23 +539,110 @@ lss_status emit_spikes(Handle& h, int n, int passes, int64_t window_start, lss_s
     if (out->x) HIP_TRY(hipMemcpyAsync(xs.data(), h.xs.p, sizeof(float) * n, hipMemcpyDeviceToHost, h.stream));
     if (out->y) HIP_TRY(hipMemcpyAsync(ys.data(), h.ys.p, sizeof(float) * n, hipMemcpyDeviceToHost, h.stream));
     HIP_TRY(hipStreamSynchronize(h.stream));
-    int64_t k = 0;
-    const int64_t shift = window_start - h.nt / 2 + h.prm.nt0min;   // template_matching.py:68
-    for (int i = 0; i < n; ++i) {
-        const int64_t smp = shift + col[i];
-        if (h.prm.drop_negative && smp < 0) continue;   // template_matching.py:53-59
-        if (k >= out->capacity) return fail(LSS_ERR_CAPACITY, "output capacity exceeded");
-        if (out->sample) out->sample[k] = smp;
-        if (out->template_id) out->template_id[k] = tmp[i];
-        if (out->amplitude) out->amplitude[k] = amp[i];
-        if (out->column) out->column[k] = col[i];
-        if (out->features) std::memcpy(out->features + k * FK, tF.data() + (size_t)i * FK, sizeof(float) * FK);
-        if (out->x) out->x[k] = xs[i];
-        if (out->y) out->y[k] = ys[i];
-        ++k;
+    return emit_from(h, n, passes, window_start, out, col.data(), tmp.data(), amp.data(), tF.data(), xs.data(),
+                     ys.data());
+}
+
+// ------------------------------------------------------------------------------------------------
+// Multi-probe streaming batcher.  Probes share one sample clock; every step sorts the next `batch`
+// samples of every probe.  A probe's window is samples [t0 - nt, t0 + batch + nt) (Kilosort
+// padding: samples before 0 repeat sample 0), so consecutive windows report disjoint sample
+// ranges.  Device work per step: segment A = window upload, preprocessing, scores and the first P1
+// pursuit passes of every probe as parallel graph branches; one host check of the converged flags;
+// extra passes (non-graph) for probes that need them; segment C = residual, features and the
+// readback of every probe.  With use_graph, A and C are captured once and replayed.
+// ------------------------------------------------------------------------------------------------
+struct Probe {
+    Handle* h = nullptr;
+    std::vector<int16_t> buf;     // received samples [base, base + rows) as [row][C]
+    int64_t base = 0;
+    int64_t recv = 0;
+    int16_t* pin = nullptr;       // pinned window [W][C]
+    int64_t W = 0;
+    int P1 = 0;
+    hipEvent_t done = nullptr;
+};
+
+struct Batcher {
+    std::vector<Probe> pr;
+    int batch = 0;
+    bool use_graph = false;
+    hipStream_t main = nullptr;
+    hipEvent_t fork = nullptr;
+    hipGraphExec_t gA = nullptr, gC = nullptr;
+    hipGraph_t graphA = nullptr, graphC = nullptr;
+    int64_t t0 = 0;
+    int64_t batches = 0;
+    float last_ms = 0.f;
+    ~Batcher() {
+        if (gA) (void)hipGraphExecDestroy(gA);
+        if (gC) (void)hipGraphExecDestroy(gC);
+        if (graphA) (void)hipGraphDestroy(graphA);
+        if (graphC) (void)hipGraphDestroy(graphC);
+        for (auto& p : pr) {
+            if (p.pin) (void)hipHostFree(p.pin);
+            if (p.done) (void)hipEventDestroy(p.done);
+        }
+        if (fork) (void)hipEventDestroy(fork);
+        if (main) (void)hipStreamDestroy(main);
     }
-    out->n = k;
+};
+
+lss_status segment_A(Batcher& b) {
+    HIP_TRY(hipEventRecord(b.fork, b.main));
+    for (auto& p : b.pr) {
+        Handle& h = *p.h;
+        HIP_TRY(hipStreamWaitEvent(h.stream, b.fork, 0));
+        HIP_TRY(hipMemcpyAsync(h.win.p, p.pin, sizeof(int16_t) * p.W * h.C, hipMemcpyHostToDevice, h.stream));
+        if (lss_status st = enqueue_pre(h, h.win.p, p.W)) return st;
+        const lss::PursuitArgs a = pursuit_args(h, h.B.p, h.xw.p, p.W);
+        if (lss_status st = enqueue_passes(h, a, 0, p.P1)) return st;
+        HIP_TRY(hipMemcpyAsync(h.hs_counters, h.counters.p, sizeof(int) * 8, hipMemcpyDeviceToHost, h.stream));
+        HIP_TRY(hipEventRecord(p.done, h.stream));
+        HIP_TRY(hipStreamWaitEvent(b.main, p.done, 0));
+    }
     return LSS_OK;
+}
+
+lss_status segment_C(Batcher& b) {
+    HIP_TRY(hipEventRecord(b.fork, b.main));
+    for (auto& p : b.pr) {
+        Handle& h = *p.h;
+        HIP_TRY(hipStreamWaitEvent(h.stream, b.fork, 0));
+        const lss::PursuitArgs a = pursuit_args(h, h.B.p, h.xw.p, p.W);
+        HIP_TRY(lss::launch_residual(a, h.stream));
+        if (lss_status st = run_features(h, h.xw.p, p.W)) return st;
+        if (lss_status st = enqueue_readback(h)) return st;
+        HIP_TRY(hipEventRecord(p.done, h.stream));
+        HIP_TRY(hipStreamWaitEvent(b.main, p.done, 0));
+    }
+    return LSS_OK;
+}
+
+lss_status capture(Batcher& b, lss_status (*seg)(Batcher&), hipGraph_t* g, hipGraphExec_t* ge) {
+    HIP_TRY(hipStreamBeginCapture(b.main, hipStreamCaptureModeRelaxed));
+    lss_status st = seg(b);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(b.main, &graph);
+    if (st != LSS_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    if (e != hipSuccess) return fail(LSS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    *g = graph;
+    HIP_TRY(hipGraphInstantiate(ge, graph, nullptr, nullptr, 0));
+    return LSS_OK;
+}
+
+// Assembles probe p's window for batch start t0 into its pinned staging buffer.
+void assemble(Probe& p, int64_t t0) {
+    Handle& h = *p.h;
+    const int C = h.C, nt = h.nt;
+    for (int64_t c = 0; c < p.W; ++c) {
+        int64_t smp = t0 - nt + c;
+        if (smp < 0) smp = 0;
+        std::memcpy(p.pin + c * C, p.buf.data() + (smp - p.base) * C, sizeof(int16_t) * C);
+    }
 }
 
 std::string trim(const std::string& s) {
@@ -790,6 +963,181 @@ lss_status lss_last_timings(void* handle, float* ms, int32_t n) {
     if (!handle || !ms || n < 0 || n > 8) return fail(LSS_ERR_ARG, "bad argument");
     Handle& h = *as_handle(handle);
     for (int i = 0; i < n; ++i) ms[i] = h.last_ms[i];
+    return LSS_OK;
+}
+
+}  // extern "C"
+
+// =================================================================================================
+// streaming batcher and autotuner
+// =================================================================================================
+extern "C" {
+
+lss_status lss_stream_create(void* const* handles, int32_t n_probes, int32_t batch, int32_t use_graph,
+                             void** stream_out) {
+    if (!handles || n_probes < 1 || batch < 1 || !stream_out) return fail(LSS_ERR_ARG, "bad argument");
+    std::unique_ptr<Batcher> b(new Batcher());
+    b->batch = batch;
+    b->use_graph = use_graph != 0;
+    HIP_TRY(hipStreamCreateWithFlags(&b->main, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+    b->pr.resize(n_probes);
+    for (int i = 0; i < n_probes; ++i) {
+        Probe& p = b->pr[i];
+        if (!handles[i]) return fail(LSS_ERR_ARG, "null handle");
+        p.h = as_handle(handles[i]);
+        Handle& h = *p.h;
+        for (int j = 0; j < i; ++j)
+            if (b->pr[j].h == p.h) return fail(LSS_ERR_ARG, "a handle appears twice");
+        p.W = (int64_t)batch + 2 * h.nt;
+        if (p.W > h.Wmax) return fail(LSS_ERR_ARG, "batch + 2*nt exceeds a handle's max_window");
+        if (h.prof_update) return fail(LSS_ERR_STATE, "turn update profiling off before streaming");
+        if (lss_status st = ensure_stage(h)) return st;
+        if (lss_status st = prepare_spectrum(h, p.W)) return st;
+        HIP_TRY(hipHostMalloc((void**)&p.pin, sizeof(int16_t) * p.W * h.C, hipHostMallocDefault));
+        HIP_TRY(hipEventCreateWithFlags(&p.done, hipEventDisableTiming));
+        p.P1 = (h.tuned_passes > 0) ? std::min(h.tuned_passes, h.prm.max_passes) : h.prm.max_passes;
+    }
+    *stream_out = b.release();
+    return LSS_OK;
+}
+
+lss_status lss_stream_destroy(void* stream) {
+    if (!stream) return fail(LSS_ERR_ARG, "null stream");
+    Batcher* b = static_cast<Batcher*>(stream);
+    (void)hipStreamSynchronize(b->main);
+    delete b;
+    return LSS_OK;
+}
+
+lss_status lss_stream_push(void* stream, int32_t probe, const int16_t* samples, int64_t n) {
+    if (!stream || (n > 0 && !samples) || n < 0) return fail(LSS_ERR_ARG, "bad argument");
+    Batcher& b = *static_cast<Batcher*>(stream);
+    if (probe < 0 || probe >= (int32_t)b.pr.size()) return fail(LSS_ERR_ARG, "probe out of range");
+    Probe& p = b.pr[probe];
+    const int C = p.h->C;
+    p.buf.insert(p.buf.end(), samples, samples + n * C);
+    p.recv += n;
+    return LSS_OK;
+}
+
+lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready) {
+    if (!stream || !outs || !ready) return fail(LSS_ERR_ARG, "null argument");
+    Batcher& b = *static_cast<Batcher*>(stream);
+    *ready = 0;
+    for (auto& p : b.pr)
+        if (p.recv < b.t0 + b.batch + p.h->nt) return LSS_OK;   // not enough samples yet
+    for (auto& p : b.pr) assemble(p, b.t0);
+    auto t_start = std::chrono::steady_clock::now();
+    // segment A
+    if (b.use_graph) {
+        if (!b.gA)
+            if (lss_status st = capture(b, segment_A, &b.graphA, &b.gA)) return st;
+        HIP_TRY(hipGraphLaunch(b.gA, b.main));
+    } else if (lss_status st = segment_A(b)) {
+        return st;
+    }
+    HIP_TRY(hipStreamSynchronize(b.main));
+    // passes past P1 for probes that have not converged
+    for (auto& p : b.pr) {
+        Handle& h = *p.h;
+        if (p.P1 < h.prm.max_passes && h.hs_counters[3] == 0) {
+            const lss::PursuitArgs a = pursuit_args(h, h.B.p, h.xw.p, p.W);
+            if (lss_status st = enqueue_passes(h, a, p.P1, h.prm.max_passes)) return st;
+            HIP_TRY(hipEventRecord(p.done, h.stream));
+            HIP_TRY(hipStreamWaitEvent(b.main, p.done, 0));
+        }
+    }
+    // segment C
+    if (b.use_graph) {
+        if (!b.gC)
+            if (lss_status st = capture(b, segment_C, &b.graphC, &b.gC)) return st;
+        HIP_TRY(hipGraphLaunch(b.gC, b.main));
+    } else if (lss_status st = segment_C(b)) {
+        return st;
+    }
+    HIP_TRY(hipStreamSynchronize(b.main));
+    for (size_t i = 0; i < b.pr.size(); ++i) {
+        Probe& p = b.pr[i];
+        Handle& h = *p.h;
+        const int* c = h.hs_counters;
+        if (c[2]) return fail(LSS_ERR_CAPACITY, "spike capacity (max_spikes) exceeded");
+        if (c[5]) return fail(LSS_ERR_CAPACITY, "residual list overflow");
+        const int n = c[0] + c[1];
+        const int64_t ws = b.t0 - h.nt;
+        lss_status st = (n <= h.stage_cap)
+                            ? emit_from(h, n, c[4], ws, &outs[i], h.hs_col, h.hs_tmp, h.hs_amp, h.hs_tF, h.hs_xs, h.hs_ys)
+                            : emit_spikes(h, n, c[4], ws, &outs[i]);
+        if (st != LSS_OK) return st;
+    }
+    b.last_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    ++b.batches;
+    b.t0 += b.batch;
+    for (auto& p : b.pr) {
+        // keep samples from t0 - nt on
+        const int64_t keep_from = std::max<int64_t>(0, b.t0 - p.h->nt);
+        const int64_t drop = keep_from - p.base;
+        if (drop > 0) {
+            p.buf.erase(p.buf.begin(), p.buf.begin() + drop * p.h->C);
+            p.base = keep_from;
+        }
+    }
+    *ready = 1;
+    return LSS_OK;
+}
+
+lss_status lss_stream_stats(void* stream, float* last_ms, int64_t* batches) {
+    if (!stream || !last_ms || !batches) return fail(LSS_ERR_ARG, "null argument");
+    Batcher& b = *static_cast<Batcher*>(stream);
+    *last_ms = b.last_ms;
+    *batches = b.batches;
+    return LSS_OK;
+}
+
+lss_status lss_set_tuning(void* handle, int32_t update_grid, int32_t first_passes) {
+    if (!handle || update_grid < 1 || first_passes < 0) return fail(LSS_ERR_ARG, "bad argument");
+    Handle& h = *as_handle(handle);
+    h.upd_grid = update_grid;
+    h.tuned_passes = first_passes;
+    return LSS_OK;
+}
+
+lss_status lss_get_tuning(void* handle, int32_t* update_grid, int32_t* first_passes) {
+    if (!handle || !update_grid || !first_passes) return fail(LSS_ERR_ARG, "null argument");
+    Handle& h = *as_handle(handle);
+    *update_grid = h.upd_grid;
+    *first_passes = h.tuned_passes;
+    return LSS_OK;
+}
+
+lss_status lss_autotune(void* handle, const int16_t* d_window, int64_t W, int32_t reps, float* grid_ms,
+                        int32_t* best_grid, int32_t* first_passes) {
+    if (!handle || !d_window || reps < 1 || !best_grid || !first_passes) return fail(LSS_ERR_ARG, "bad argument");
+    Handle& h = *as_handle(handle);
+    static const int kGrids[] = {256, 512, 1024, 2048, 4096, 8192};
+    lss_spikes out{};
+    out.capacity = h.prm.max_spikes;   // counts only: every output array is NULL
+    float best = 1e30f;
+    int bg = h.upd_grid, max_passes_seen = 0;
+    for (int gi = 0; gi < 6; ++gi) {
+        h.upd_grid = kGrids[gi];
+        float t = 1e30f;
+        for (int r = 0; r <= reps; ++r) {   // first run warms up
+            if (lss_status st = lss_sort_window(handle, d_window, W, 1, 0, &out)) return st;
+            if (r > 0) t = std::min(t, h.last_ms[7]);
+            max_passes_seen = std::max(max_passes_seen, out.passes);
+        }
+        if (grid_ms) grid_ms[gi] = t;
+        if (t < best) {
+            best = t;
+            bg = kGrids[gi];
+        }
+    }
+    h.upd_grid = bg;
+    // the first graph segment covers the passes seen plus a margin; the rest run only when needed
+    h.tuned_passes = std::min(h.prm.max_passes, max_passes_seen + 4);
+    *best_grid = bg;
+    *first_passes = h.tuned_passes;
     return LSS_OK;
 }
 

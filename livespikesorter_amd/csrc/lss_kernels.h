@@ -67,7 +67,8 @@ struct PursuitArgs {
 hipError_t launch_reset_counters(int* counters, hipStream_t s);        // counters[0..7] = 0
 hipError_t launch_score_all(const PursuitArgs& a, hipStream_t s);     // T <= 1024
 hipError_t launch_peaks(const PursuitArgs& a, hipStream_t s);
-hipError_t launch_update(const PursuitArgs& a, hipStream_t s);
+// grid = workgroups (grid-stride over the pass's work items; any grid gives the same result)
+hipError_t launch_update(const PursuitArgs& a, int grid, hipStream_t s);
 // residual of all passes in pass order (even, then odd spikes of a pass), after the last pass
 hipError_t launch_residual(const PursuitArgs& a, hipStream_t s);
 

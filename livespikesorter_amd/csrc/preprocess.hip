@@ -82,8 +82,15 @@ __global__ __launch_bounds__(256) void channel_sums_vec_kernel(const int16_t* __
     }
 }
 
+__global__ void zero_i64_kernel(long long* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
 hipError_t launch_channel_sums(const int16_t* win, int64_t W, int C, long long* sums, hipStream_t s) {
-    hipError_t e = hipMemsetAsync(sums, 0, sizeof(long long) * C, s);
+    // a fill kernel rather than a memset node, so that the sequence can be captured in a graph
+    hipLaunchKernelGGL(zero_i64_kernel, dim3((C + 255) / 256), dim3(256), 0, s, sums, C);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (C % 8 == 0 && (((uintptr_t)win) & 15) == 0) {
         const int nb = (int)((W + kSumRows - 1) / kSumRows);

@@ -508,10 +508,11 @@ hipError_t launch_peaks(const PursuitArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_update(const PursuitArgs& a, hipStream_t s) {
+hipError_t launch_update(const PursuitArgs& a, int grid, hipStream_t s) {
     const int nv = (a.T + 255) / 256;
+    if (grid < 1) grid = kUpdateGrid;
     switch (nv) {
-#define LSS_UP(V) case V: hipLaunchKernelGGL(update_kernel<V>, dim3(kUpdateGrid), dim3(256), 0, s, a); break;
+#define LSS_UP(V) case V: hipLaunchKernelGGL(update_kernel<V>, dim3(grid), dim3(256), 0, s, a); break;
         LSS_UP(1) LSS_UP(2) LSS_UP(3) LSS_UP(4)
 #undef LSS_UP
         default: return hipErrorInvalidValue;

@@ -48,6 +48,23 @@ class SpikeList:
                          self.x[o], self.y[o], self.column[o], self.passes)
 
 
+def alloc_spikes(n: int, nC: int, K: int):
+    """An lss_spikes struct over freshly allocated host arrays of capacity n."""
+    bufs = dict(sample=np.zeros(n, np.int64), template=np.zeros(n, np.int32),
+                amplitude=np.zeros(n, np.float32), features=np.zeros((n, nC, K), np.float32),
+                x=np.zeros(n, np.float32), y=np.zeros(n, np.float32), column=np.zeros(n, np.int32))
+    s = Spikes()
+    s.capacity = n
+    s.sample = bufs["sample"].ctypes.data
+    s.template_id = bufs["template"].ctypes.data
+    s.amplitude = bufs["amplitude"].ctypes.data
+    s.features = bufs["features"].ctypes.data
+    s.x = bufs["x"].ctypes.data
+    s.y = bufs["y"].ctypes.data
+    s.column = bufs["column"].ctypes.data
+    return s, bufs
+
+
 TIMING_KEYS = ["car", "highpass", "whiten", "project", "scores_gemm", "pursuit", "features", "total"]
 
 
@@ -145,19 +162,7 @@ class Sorter:
         cached = getattr(self, "_out_cache", None)
         if cached is not None and cached[0] == n:
             return cached[1], cached[2]
-        bufs = dict(sample=np.zeros(n, np.int64), template=np.zeros(n, np.int32),
-                    amplitude=np.zeros(n, np.float32),
-                    features=np.zeros((n, self.nC, self.K), np.float32),
-                    x=np.zeros(n, np.float32), y=np.zeros(n, np.float32), column=np.zeros(n, np.int32))
-        s = Spikes()
-        s.capacity = n
-        s.sample = bufs["sample"].ctypes.data
-        s.template_id = bufs["template"].ctypes.data
-        s.amplitude = bufs["amplitude"].ctypes.data
-        s.features = bufs["features"].ctypes.data
-        s.x = bufs["x"].ctypes.data
-        s.y = bufs["y"].ctypes.data
-        s.column = bufs["column"].ctypes.data
+        s, bufs = alloc_spikes(n, self.nC, self.K)
         self._out_cache = (n, s, bufs)
         return s, bufs
 

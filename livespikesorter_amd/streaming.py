@@ -1,0 +1,74 @@
+"""Multi-probe streaming sorter (the batching loop of OnlineSpikesV2::runSpikeSorting,
+src/SpikeSorter/OnlineSpikesV2.cu:632-820, for several probes on one sample clock), over liblss's
+batcher: samples are pushed per probe as they arrive; every step sorts the next `batch` samples of
+every probe in one replayed HIP graph (probes as concurrent branches)."""
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+
+from ._lib import Spikes, check, lib
+from .sorter import Sorter, alloc_spikes
+
+
+class StreamingSorter:
+    def __init__(self, sorters: list[Sorter], batch: int = 3000, use_graph: bool = True):
+        self.sorters = list(sorters)
+        arr = (ct.c_void_p * len(self.sorters))(*[s._h.value for s in self.sorters])
+        h = ct.c_void_p()
+        check(lib().lss_stream_create(arr, len(self.sorters), int(batch), int(use_graph), ct.byref(h)),
+              "lss_stream_create")
+        self._h = h
+        self.batch = int(batch)
+        arr_out = (Spikes * len(self.sorters))()
+        self._bufs = []
+        for i, s in enumerate(self.sorters):
+            sp, bufs = alloc_spikes(min(s.params.max_spikes, 1 << 16), s.nC, s.K)
+            arr_out[i] = sp
+            self._bufs.append(bufs)
+        self._arr_out = arr_out
+
+    def push(self, probe: int, samples: np.ndarray):
+        x = np.ascontiguousarray(samples, np.int16)
+        check(lib().lss_stream_push(self._h, int(probe), x.ctypes.data, x.shape[0]), "lss_stream_push")
+
+    def step(self):
+        """Spike lists of the next batch for every probe, or None if samples are missing."""
+        ready = ct.c_int32()
+        check(lib().lss_stream_step(self._h, self._arr_out, ct.byref(ready)), "lss_stream_step")
+        if not ready.value:
+            return None
+        return [Sorter._list(self._arr_out[i], self._bufs[i]) for i in range(len(self.sorters))]
+
+    def last_ms(self) -> float:
+        ms, n = ct.c_float(), ct.c_int64()
+        check(lib().lss_stream_stats(self._h, ct.byref(ms), ct.byref(n)), "lss_stream_stats")
+        return ms.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            check(lib().lss_stream_destroy(self._h), "lss_stream_destroy")
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def autotune(sorter: Sorter, window_dev, reps: int = 2):
+    """Picks the update grid (fastest of 256..8192 workgroups) and the first-segment pass count
+    for `sorter` on a device window; returns (grid, first_passes, {grid: ms})."""
+    ms = np.zeros(6, np.float32)
+    g, p = ct.c_int32(), ct.c_int32()
+    check(lib().lss_autotune(sorter._h, window_dev.data_ptr(), window_dev.shape[0], int(reps), ms.ctypes.data,
+                             ct.byref(g), ct.byref(p)), "lss_autotune")
+    return g.value, p.value, dict(zip([256, 512, 1024, 2048, 4096, 8192], ms.tolist()))

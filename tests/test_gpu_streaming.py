@@ -1,0 +1,109 @@
+"""Multi-probe streaming batcher (graph-captured) against the single-window device path and the
+CPU oracle.  The batcher must return exactly the spikes that sorting each window alone returns:
+same kernels, and the tuning (update grid, first-segment passes) does not change results."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import ks_oracle as ko
+
+pytestmark = pytest.mark.gpu
+
+BATCH = 3000
+
+
+@pytest.fixture(scope="module")
+def probes():
+    from livespikesorter_amd import synth
+    a = synth.generate(synth.SynthConfig(n_chan=64, n_units=24, n_samples=15000, NT=3000, seed=11))
+    b = synth.generate(synth.SynthConfig(n_chan=32, n_units=8, n_samples=15000, NT=3000, nrange=16, nC=4, seed=12))
+    return [a, b]
+
+
+def _window(rec, t0, nt):
+    idx = np.clip(np.arange(t0 - nt, t0 + BATCH + nt), 0, None)
+    return np.ascontiguousarray(rec[idx])
+
+
+def _stream(probes, sorters, use_graph, chunk_seed=0):
+    from livespikesorter_amd.streaming import StreamingSorter
+    rng = np.random.default_rng(chunk_seed)
+    out = [[] for _ in probes]
+    with StreamingSorter(sorters, batch=BATCH, use_graph=use_graph) as st:
+        n = probes[0].rec.shape[0]
+        pos = 0
+        while pos < n:
+            k = int(rng.integers(500, 2500))
+            for i, p in enumerate(probes):
+                st.push(i, p.rec[pos:pos + k])
+            pos += k
+            while True:
+                r = st.step()
+                if r is None:
+                    break
+                for i in range(len(probes)):
+                    out[i].append(r[i])
+    return out
+
+
+def _make_sorters(probes, **kw):
+    from livespikesorter_amd.sorter import Sorter
+    return [Sorter(p.art, max_window=BATCH + 2 * p.art["nt"], max_passes=100, **kw) for p in probes]
+
+
+def test_stream_matches_single_windows(probes):
+    print("[lss] test_stream_matches_single_windows", flush=True)
+    sorters = _make_sorters(probes)
+    got = _stream(probes, sorters, use_graph=True)
+    nb = len(got[0])
+    assert nb == (probes[0].rec.shape[0] - 61) // BATCH
+    for i, p in enumerate(probes):
+        nt = p.art["nt"]
+        for k in range(nb):
+            t0 = k * BATCH
+            ref = sorters[i].sort_window(_window(p.rec, t0, nt), t0 - nt)
+            g = got[i][k]
+            np.testing.assert_array_equal(g.sample, ref.sample)
+            np.testing.assert_array_equal(g.template, ref.template)
+            np.testing.assert_array_equal(g.amplitude, ref.amplitude)
+            np.testing.assert_array_equal(g.features, ref.features)
+    for s in sorters:
+        s.close()
+
+
+def test_graph_and_direct_launches_agree_and_tuning_is_result_neutral(probes):
+    print("[lss] test_graph_and_direct_launches_agree_and_tuning_is_result_neutral", flush=True)
+    from livespikesorter_amd.streaming import autotune
+    s1 = _make_sorters(probes)
+    g = _stream(probes, s1, use_graph=True, chunk_seed=1)
+    s2 = _make_sorters(probes)
+    # tuned: the fastest update grid, and a deliberately short first segment so that the extra
+    # passes between the two graphs run
+    w = torch.from_numpy(_window(probes[0].rec, BATCH, probes[0].art["nt"])).to("cuda:0")
+    grid, p1, table = autotune(s2[0], w)
+    print(f"[lss]   autotune probe 0: grid {grid}, first passes {p1}, ms {table}", flush=True)
+    from livespikesorter_amd._lib import lib
+    assert lib().lss_set_tuning(s2[1]._h, 512, 2) == 0
+    d = _stream(probes, s2, use_graph=False, chunk_seed=2)
+    for i in range(len(probes)):
+        for a, b in zip(g[i], d[i]):
+            np.testing.assert_array_equal(a.sample, b.sample)
+            np.testing.assert_array_equal(a.amplitude, b.amplitude)
+    for s in s1 + s2:
+        s.close()
+
+
+def test_stream_vs_oracle(probes):
+    print("[lss] test_stream_vs_oracle", flush=True)
+    sorters = _make_sorters(probes)
+    got = _stream(probes, sorters, use_graph=True)
+    for i, p in enumerate(probes):
+        nt = p.art["nt"]
+        k = 2
+        t0 = k * BATCH
+        X = torch.from_numpy(_window(p.rec, t0, nt).T.astype(np.float64))
+        ref = ko.sort_window(X, p.art, t0 - nt, dtype=torch.float64, max_passes=100)
+        f1 = ko.spike_f1(ref.samples, ref.templates, got[i][k].sample, got[i][k].template, dt=1)
+        assert f1 >= 0.999, (i, f1)
+    for s in sorters:
+        s.close()
