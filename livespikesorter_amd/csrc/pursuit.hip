@@ -369,7 +369,7 @@ __global__ __launch_bounds__(256) void update_kernel(PursuitArgs a) {
 //   X[w][c] -= amp * waves[t][w - col + nt/2][c]
 // which is the residual the reference's per-pass updates leave (template_matching.py:197).
 // ------------------------------------------------------------------------------------------------
-constexpr int kResCols = 64;
+constexpr int kResCols = 16;
 constexpr int kResCap = 1024;       // listed spikes per workgroup
 constexpr int kResMaxPasses = 256;
 constexpr int kResBatch = 8;        // wave-row loads in flight per column
@@ -532,64 +532,83 @@ hipError_t launch_residual(const PursuitArgs& a, hipStream_t s) {
 }
 
 // ------------------------------------------------------------------------------------------------
-// PC features and positions: one wavefront per spike, count read from the device counters
+// PC features and positions (template_matching.py:50-51, postprocessing.py:33-46): one wavefront
+// per spike.  The spike's nt x nC residual window is gathered into LDS with all loads in flight at
+// once, then lane (j, k) forms sum_m X[w - nt/2 + m][iCC[j, iU[t]]] wPCA[k][m] + amp Ucc[j][t][k].
 // ------------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void features_kernel(FeatureArgs a) {
-    __shared__ float sf[4][32 * 8];
+    extern __shared__ float fsm[];   // wpca [K*nt] | per wavefront: window [nt][nC], features [nC*K]
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int n = a.counters ? a.counters[0] + a.counters[1] : a.n;
     if (n > a.cap) n = a.cap;
-    const int h = a.nt / 2, K = a.K, nC = a.nC;
+    const int h = a.nt / 2, K = a.K, nC = a.nC, nt = a.nt;
+    float* wp = fsm;
+    float* xwin = fsm + K * nt + wave * (nt * nC + nC * K);
+    float* sf = xwin + nt * nC;
+    for (int e = threadIdx.x; e < K * nt; e += 256) wp[e] = a.wpca[e];
+    __syncthreads();
     for (int s0 = blockIdx.x * 4; s0 < n; s0 += gridDim.x * 4) {
         const int s = s0 + wave;
-        const bool valid = s < n;
-        int t = 0, iu = 0;
-        if (valid) {
+        if (s < n) {
             const int w = a.sp_col[s];
-            t = a.sp_tmp[s];
+            const int t = a.sp_tmp[s];
             const float am = a.sp_amp[s];
-            iu = a.iU[t];
+            const int iu = a.iU[t];
+            for (int e = lane; e < nt * nC; e += 64) {
+                const int m = e / nC, j = e % nC;
+                const int64_t ww = (int64_t)w - h + m;
+                const int ch = a.iCC[j * a.C + iu];
+                xwin[e] = (ww >= 0 && ww < a.W) ? a.X[ww * a.ldC + ch] : 0.f;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             for (int p = lane; p < nC * K; p += 64) {
                 const int j = p / K, k = p % K;
-                const int ch = a.iCC[j * a.C + iu];
                 float acc = 0.f;
-                for (int m = 0; m < a.nt; ++m) {
-                    const int64_t ww = (int64_t)w - h + m;
-                    const float xv = (ww >= 0 && ww < a.W) ? a.X[ww * a.ldC + ch] : 0.f;
-                    acc = acc + xv * a.wpca[k * a.nt + m];
-                }
+                for (int m = 0; m < nt; ++m) acc = acc + xwin[m * nC + j] * wp[k * nt + m];
                 const float f = acc + am * a.Ucc[((int64_t)j * a.T + t) * K + k];
-                sf[wave][p] = f;
+                sf[p] = f;
                 a.tF[(int64_t)s * nC * K + p] = f;
             }
-        }
-        __syncthreads();
-        if (valid && lane == 0) {
-            float tot = 0.f;
-            for (int j = 0; j < nC; ++j) {
-                float m = 0.f;
-                for (int k = 0; k < K; ++k) m += sf[wave][j * K + k] * sf[wave][j * K + k];
-                tot += m;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (lane == 0) {
+                float tot = 0.f;
+                for (int j = 0; j < nC; ++j) {
+                    float m = 0.f;
+                    for (int k = 0; k < K; ++k) m += sf[j * K + k] * sf[j * K + k];
+                    tot += m;
+                }
+                float xs = 0.f, ys = 0.f;
+                for (int j = 0; j < nC; ++j) {
+                    float m = 0.f;
+                    for (int k = 0; k < K; ++k) m += sf[j * K + k] * sf[j * K + k];
+                    const int ch = a.iCC[j * a.C + iu];
+                    const float wgt = m / tot;
+                    xs += a.xc[ch] * wgt;
+                    ys += a.yc[ch] * wgt;
+                }
+                a.xs[s] = xs;
+                a.ys[s] = ys;
             }
-            float xs = 0.f, ys = 0.f;
-            for (int j = 0; j < nC; ++j) {
-                float m = 0.f;
-                for (int k = 0; k < K; ++k) m += sf[wave][j * K + k] * sf[wave][j * K + k];
-                const int ch = a.iCC[j * a.C + iu];
-                const float wgt = m / tot;
-                xs += a.xc[ch] * wgt;
-                ys += a.yc[ch] * wgt;
-            }
-            a.xs[s] = xs;
-            a.ys[s] = ys;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-        __syncthreads();
     }
 }
 
 hipError_t launch_features(const FeatureArgs& a, int grid, hipStream_t s) {
-    if (a.nC * a.K > 32 * 8) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(features_kernel, dim3(grid), dim3(256), 0, s, a);
+    if (a.nC * a.K > 32 * 8 || a.nC > 32) return hipErrorInvalidValue;
+    const size_t lds = sizeof(float) * ((size_t)a.K * a.nt + 4 * ((size_t)a.nt * a.nC + a.nC * a.K));
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)features_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(features_kernel, dim3(grid), dim3(256), lds, s, a);
     return hipGetLastError();
 }
 
