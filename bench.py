@@ -84,7 +84,7 @@ def bench_stream(args):
     from livespikesorter_amd import synth
     from livespikesorter_amd.metrics import f1_same_template
     from livespikesorter_amd.sorter import Sorter
-    from livespikesorter_amd.streaming import StreamingSorter, autotune
+    from livespikesorter_amd.streaming import StreamingSorter, autotune, autotune_persistent
 
     batch, fs = 3000, 30000.0
     n = int(args.seconds * fs) + 2 * batch
@@ -104,7 +104,10 @@ def bench_stream(args):
     for s, p in zip(sorters, probes):
         w = torch.from_numpy(window(p.rec, 3, p.art["nt"])).to("cuda:0")
         g, p1, _ = autotune(s, w)
-        tuning.append({"update_grid": g, "first_passes": p1})
+        cap = s.persistent(w.shape[0])[2] // len(probes)   # the batcher's share per probe
+        pg, pf, pms = autotune_persistent(s, w, max_grid=cap)
+        tuning.append({"update_grid": g, "first_passes": p1, "persistent_grid": pg, "persistent_after": pf,
+                       "persistent_ms": {f"{k[0]}/{k[1]}": round(v, 3) for k, v in pms.items()}})
     nbat = (n - 2 * batch) // batch
     # plain per-probe path: the same windows one probe at a time, host windows
     plain = []
@@ -114,7 +117,7 @@ def bench_stream(args):
                       for s, p in zip(sorters, probes)])
     t_plain = time.perf_counter() - t_plain
     # streaming batcher: samples pushed in 100 ms chunks, one step per batch
-    lat, outs = [], []
+    lat, outs, parts = [], [], []
     with StreamingSorter(sorters, batch=batch, use_graph=True) as st:
         for i, p in enumerate(probes):
             st.push(i, p.rec[:batch + 64])
@@ -131,6 +134,7 @@ def bench_stream(args):
                 continue
             lat.append(time.perf_counter() - t1)
             outs.append(r)
+            parts.append(st.breakdown())
             k += 1
         t_s = time.perf_counter() - t_s
     lat_ms = np.array(lat) * 1e3
@@ -152,6 +156,7 @@ def bench_stream(args):
         "speedup_vs_plain": round(t_plain / t_s, 3),
         "spike_f1_vs_plain_min": round(float(f1), 6),
         "tuning": tuning,
+        "step_breakdown_ms_p50": {k: round(float(np.percentile([p[k] for p in parts], 50)), 3) for k in parts[0]},
     }
     print(json.dumps(res))
 

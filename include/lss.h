@@ -62,7 +62,8 @@ typedef struct lss_params {
     int32_t do_car;        /* median (common average) reference, kilosort/io.py:766-768 */
     int32_t precision;     /* lss_precision */
     int32_t drop_negative; /* drop spikes whose sample < 0 (template_matching.py:53-59) */
-    float th_learned;      /* Th: a peak needs relu(B)^2/nm > Th^2 */
+    double th_learned;     /* Th (Kilosort's Th_learned, a double): a peak needs relu(B)^2/nm >
+                              (float)(Th*Th), the reference's FP32 comparison with Th**2 */
     float fs;              /* sampling rate (Hz) */
 } lss_params;
 
@@ -181,10 +182,14 @@ lss_status lss_write_spikes_txt(const char* path, const int64_t* sample, const i
 /* Creates a batcher over n_probes sorter handles (one per probe, distinct).  Each step sorts the
  * next `batch` samples of every probe: probe p's window is samples [t0 - nt, t0 + batch + nt)
  * (samples before 0 repeat sample 0), so consecutive steps report disjoint sample ranges.  With
- * use_graph the device work of a step is two replayed HIP graphs whose branches run the probes
- * concurrently: (upload, preprocessing, scores, first P1 pursuit passes) and (residual, features,
- * readback); probes that have not converged after P1 passes (lss_autotune / lss_set_tuning) get
- * their remaining passes in between.  Each handle needs max_window >= batch + 2*nt. */
+ * use_graph the device work of a step is replayed HIP graphs whose branches run the probes
+ * concurrently.  When every probe runs the persistent pursuit (lss_set_persistent, windows up to
+ * LSS_PERSIST_MAX_WINDOW) a step is ONE graph: upload, preprocessing, scores, all pursuit passes,
+ * residual, features and a readback of only the spikes found; the probes share the device's
+ * co-resident workgroups equally.  Otherwise a step is two graphs, (upload, preprocessing,
+ * scores, first P1 passes) and (residual, features, readback), and probes that have not converged
+ * after P1 passes (lss_autotune / lss_set_tuning) get their remaining passes in between.  Each
+ * handle needs max_window >= batch + 2*nt. */
 lss_status lss_stream_create(void* const* handles, int32_t n_probes, int32_t batch, int32_t use_graph,
                              void** stream);
 lss_status lss_stream_destroy(void* stream);
@@ -195,17 +200,40 @@ lss_status lss_stream_push(void* stream, int32_t probe, const int16_t* samples, 
 lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready);
 /* Wall time of the last step in ms (upload to spike lists) and steps done. */
 lss_status lss_stream_stats(void* stream, float* last_ms, int64_t* batches);
+/* Host wall-time split of the last step in ms: [0] window assembly, [1] segment A (launch to
+ * completion), [2] extra passes enqueue, [3] segment C (launch to completion), [4] spike emission,
+ * [5] total. */
+lss_status lss_stream_breakdown(void* stream, float* ms);
 
 /* ---- tuning ------------------------------------------------------------------------------------- */
 
-/* Times the window (device memory) with each candidate update grid (256 .. 8192 workgroups) and
- * keeps the fastest; sets the batcher's first-segment pass count to the passes seen + 4 (capped at
- * max_passes).  grid_ms (6 floats, may be NULL) receives the best time per candidate.  Results do
- * not depend on the tuning. */
+/* Per-pass kernels: times the window (device memory) with each candidate update grid (256 .. 8192
+ * workgroups) and keeps the fastest; sets the batcher's first-segment pass count to the passes
+ * seen + 4 (capped at max_passes).  grid_ms (6 floats, may be NULL) receives the best time per
+ * candidate.  Results do not depend on the tuning. */
 lss_status lss_autotune(void* handle, const int16_t* d_window, int64_t W, int32_t reps, float* grid_ms,
                         int32_t* best_grid, int32_t* first_passes);
 lss_status lss_set_tuning(void* handle, int32_t update_grid, int32_t first_passes);
 lss_status lss_get_tuning(void* handle, int32_t* update_grid, int32_t* first_passes);
+
+/* Persistent pursuit: windows of at most LSS_PERSIST_MAX_WINDOW samples (streaming batches) run
+ * their first `first_passes` passes (the ones with many spikes, whose wide updates want the whole
+ * chip) as per-pass kernels and the rest in one launch of `grid` co-resident workgroups with two
+ * grid barriers per pass, instead of three kernels per pass.  Results are identical.  grid = 0
+ * selects the per-pass kernels throughout; grid is clamped at the number of workgroups the device
+ * holds at once (max_grid from lss_get_persistent; the streaming batcher splits it between its
+ * probes).  A barrier that waits past ~1 s (workgroups not co-resident) makes the call fail with
+ * LSS_ERR_STATE instead of hanging.  Defaults: grid 128, first_passes 0. */
+#define LSS_PERSIST_MAX_WINDOW 4096
+lss_status lss_set_persistent(void* handle, int32_t grid, int32_t first_passes);
+/* Current setting and the device's co-resident limit for windows of W samples (0 when W exceeds
+ * LSS_PERSIST_MAX_WINDOW). */
+lss_status lss_get_persistent(void* handle, int64_t W, int32_t* grid, int32_t* first_passes, int32_t* max_grid);
+/* Times the window for grid in {0, 32, 64, 128, 256} x first_passes in {0, 1, 2, 4}: ms[4*gi + fi]
+ * (20 floats, may be NULL; -1 = not run: above max_grid or the device limit, or grid 0 with
+ * first_passes > 0; max_grid 0 = device limit) and keeps the fastest as the handle's setting. */
+lss_status lss_autotune_persistent(void* handle, const int16_t* d_window, int64_t W, int32_t reps,
+                                   int32_t max_grid, float* ms, int32_t* best_grid, int32_t* best_first);
 
 /* Measurement mode: time every launch of the pursuit update kernel with HIP events on the
  * handle's stream; lss_last_update_stats gives the last window's summed update time and the number

@@ -14,6 +14,7 @@ LSS_OK = 0
 STATUS = {0: "LSS_OK", 1: "LSS_ERR_ARG", 2: "LSS_ERR_HIP", 3: "LSS_ERR_IO", 4: "LSS_ERR_CAPACITY",
           5: "LSS_ERR_STATE"}
 PRECISION = {"fp32": 0, "bf16": 1, "fp16": 2, "bf16x3": 3}
+PERSIST_MAX_WINDOW = 4096
 
 EXPORTS = ["lss_last_error", "lss_abi_version", "lss_default_params", "lss_create",
            "lss_create_from_dir", "lss_destroy", "lss_get_params", "lss_set_stream",
@@ -22,7 +23,8 @@ EXPORTS = ["lss_last_error", "lss_abi_version", "lss_default_params", "lss_creat
            "lss_get_tables", "lss_last_timings", "lss_profile_update", "lss_last_update_stats",
            "lss_label_spikes", "lss_dedup_spikes", "lss_write_spikes_txt",
            "lss_stream_create", "lss_stream_destroy", "lss_stream_push", "lss_stream_step",
-           "lss_stream_stats", "lss_autotune", "lss_set_tuning", "lss_get_tuning"]
+           "lss_stream_stats", "lss_stream_breakdown", "lss_autotune", "lss_set_tuning", "lss_get_tuning",
+           "lss_set_persistent", "lss_get_persistent", "lss_autotune_persistent"]
 
 
 class Params(ct.Structure):
@@ -30,7 +32,7 @@ class Params(ct.Structure):
                 ("nt", ct.c_int32), ("nt0min", ct.c_int32), ("n_nearest", ct.c_int32),
                 ("max_window", ct.c_int32), ("max_passes", ct.c_int32), ("max_spikes", ct.c_int32),
                 ("do_car", ct.c_int32), ("precision", ct.c_int32), ("drop_negative", ct.c_int32),
-                ("th_learned", ct.c_float), ("fs", ct.c_float)]
+                ("th_learned", ct.c_double), ("fs", ct.c_float)]
 
 
 class Artifacts(ct.Structure):
@@ -89,9 +91,13 @@ def lib() -> ct.CDLL:
             "lss_stream_push": ([vp, i32, vp, i64], i32),
             "lss_stream_step": ([vp, ct.POINTER(Spikes), ct.POINTER(i32)], i32),
             "lss_stream_stats": ([vp, ct.POINTER(ct.c_float), ct.POINTER(i64)], i32),
+            "lss_stream_breakdown": ([vp, vp], i32),
             "lss_autotune": ([vp, vp, i64, i32, vp, ct.POINTER(i32), ct.POINTER(i32)], i32),
             "lss_set_tuning": ([vp, i32, i32], i32),
             "lss_get_tuning": ([vp, ct.POINTER(i32), ct.POINTER(i32)], i32),
+            "lss_set_persistent": ([vp, i32, i32], i32),
+            "lss_get_persistent": ([vp, i64, ct.POINTER(i32), ct.POINTER(i32), ct.POINTER(i32)], i32),
+            "lss_autotune_persistent": ([vp, vp, i64, i32, i32, vp, ct.POINTER(i32), ct.POINTER(i32)], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

@@ -87,12 +87,15 @@ struct Handle {
     int* hs_counters = nullptr;
     int *hs_col = nullptr, *hs_tmp = nullptr;
     float *hs_amp = nullptr, *hs_tF = nullptr, *hs_xs = nullptr, *hs_ys = nullptr;
+    lss::StageArgs stage{};      // device-side view of the staging buffers
 
     hipEvent_t ev[8] = {};
     // per-launch timing of the pursuit update kernel (measurement mode)
     bool prof_update = false;
     int upd_grid = 8192;        // workgroups of the update kernel (autotuned)
     int tuned_passes = 0;       // passes in the first graph segment of the batcher (0 = all)
+    int pers_grid = 128;        // workgroups of the persistent pursuit (windows <= kPersistMaxW; 0 = off)
+    int pers_first = 0;         // leading passes on the per-pass kernels before the persistent kernel
     std::vector<hipEvent_t> upd_ev;
     float upd_ms = 0.f;
     int upd_launches = 0;
@@ -401,7 +404,8 @@ lss::PursuitArgs pursuit_args(Handle& h, float* B, float* X, int64_t W) {
     a.cfmax = h.cfmax.p; a.imax = h.imax.p; a.flags = h.flags.p;
     a.sp_col = h.sp_col.p; a.sp_tmp = h.sp_tmp.p; a.sp_amp = h.sp_amp.p; a.counters = h.counters.p;
     a.W = W; a.ldT = h.ldT; a.ldC = h.ldC; a.T = h.T; a.C = h.C; a.K = h.K; a.nt = h.nt; a.cap = h.prm.max_spikes;
-    a.th2 = h.prm.th_learned * h.prm.th_learned;
+    // Th**2 in double, rounded to FP32 as the reference's float32 comparison rounds it
+    a.th2 = (float)(h.prm.th_learned * h.prm.th_learned);
     return a;
 }
 
@@ -421,10 +425,32 @@ lss_status enqueue_passes(Handle& h, const lss::PursuitArgs& a, int p0, int p1) 
     return LSS_OK;
 }
 
+bool use_persistent(const Handle& h, int64_t W) {
+    return h.pers_grid > 0 && W <= lss::kPersistMaxW && !h.prof_update;
+}
+
+// The pursuit of one window: short windows run their first pers_first passes (the wide ones) as
+// per-pass kernels and the rest in the persistent kernel (grid capped at max_grid when > 0);
+// others get passes [0, p1) as per-pass kernels.
+lss_status enqueue_pursuit(Handle& h, const lss::PursuitArgs& a, int p1, int max_grid) {
+    if (use_persistent(h, a.W)) {
+        const int g = max_grid > 0 ? std::min(h.pers_grid, max_grid) : h.pers_grid;
+        const int p0 = std::min(h.pers_first, h.prm.max_passes - 1);
+        if (p0 > 0) {
+            if (lss_status st = enqueue_passes(h, a, 0, p0)) return st;
+        } else {
+            HIP_TRY(lss::launch_reset_counters(h.counters.p, h.stream));
+        }
+        HIP_TRY(lss::launch_pursuit_persistent(a, g, p0, h.prm.max_passes, h.stream));
+        return LSS_OK;
+    }
+    return enqueue_passes(h, a, 0, p1);
+}
+
 // All max_passes passes, then the residual of every pass.
 lss_status run_pursuit(Handle& h, float* B, float* X, int64_t W) {
     const lss::PursuitArgs a = pursuit_args(h, B, X, W);
-    if (lss_status st = enqueue_passes(h, a, 0, h.prm.max_passes)) return st;
+    if (lss_status st = enqueue_pursuit(h, a, h.prm.max_passes, 0)) return st;
     HIP_TRY(lss::launch_residual(a, h.stream));
     return LSS_OK;
 }
@@ -450,39 +476,50 @@ lss_status enqueue_pre(Handle& h, const int16_t* d_win, int64_t W) {
     return run_scores(h, h.xw.p, h.B.p, W);
 }
 
+// Pinned, device-mapped, coherent host buffers that the staging kernel writes directly.
 lss_status ensure_stage(Handle& h) {
     if (h.stage_cap) return LSS_OK;
     const int cap = std::min(h.prm.max_spikes, 16384);
     const int FK = h.nC * h.K;
-    HIP_TRY(hipHostMalloc((void**)&h.hs_counters, sizeof(int) * 8, hipHostMallocDefault));
-    HIP_TRY(hipHostMalloc((void**)&h.hs_col, sizeof(int) * cap, hipHostMallocDefault));
-    HIP_TRY(hipHostMalloc((void**)&h.hs_tmp, sizeof(int) * cap, hipHostMallocDefault));
-    HIP_TRY(hipHostMalloc((void**)&h.hs_amp, sizeof(float) * cap, hipHostMallocDefault));
-    HIP_TRY(hipHostMalloc((void**)&h.hs_tF, sizeof(float) * cap * FK, hipHostMallocDefault));
-    HIP_TRY(hipHostMalloc((void**)&h.hs_xs, sizeof(float) * cap, hipHostMallocDefault));
-    HIP_TRY(hipHostMalloc((void**)&h.hs_ys, sizeof(float) * cap, hipHostMallocDefault));
+    const unsigned fl = hipHostMallocMapped | hipHostMallocCoherent;
+    HIP_TRY(hipHostMalloc((void**)&h.hs_counters, sizeof(int) * 8, fl));
+    HIP_TRY(hipHostMalloc((void**)&h.hs_col, sizeof(int) * cap, fl));
+    HIP_TRY(hipHostMalloc((void**)&h.hs_tmp, sizeof(int) * cap, fl));
+    HIP_TRY(hipHostMalloc((void**)&h.hs_amp, sizeof(float) * cap, fl));
+    HIP_TRY(hipHostMalloc((void**)&h.hs_tF, sizeof(float) * cap * FK, fl));
+    HIP_TRY(hipHostMalloc((void**)&h.hs_xs, sizeof(float) * cap, fl));
+    HIP_TRY(hipHostMalloc((void**)&h.hs_ys, sizeof(float) * cap, fl));
+    lss::StageArgs& g = h.stage;
+    g.counters = h.counters.p;
+    g.col = h.sp_col.p; g.tmp = h.sp_tmp.p; g.amp = h.sp_amp.p; g.tF = h.tF.p; g.xs = h.xs.p; g.ys = h.ys.p;
+    HIP_TRY(hipHostGetDevicePointer((void**)&g.h_counters, h.hs_counters, 0));
+    HIP_TRY(hipHostGetDevicePointer((void**)&g.h_col, h.hs_col, 0));
+    HIP_TRY(hipHostGetDevicePointer((void**)&g.h_tmp, h.hs_tmp, 0));
+    HIP_TRY(hipHostGetDevicePointer((void**)&g.h_amp, h.hs_amp, 0));
+    HIP_TRY(hipHostGetDevicePointer((void**)&g.h_tF, h.hs_tF, 0));
+    HIP_TRY(hipHostGetDevicePointer((void**)&g.h_xs, h.hs_xs, 0));
+    HIP_TRY(hipHostGetDevicePointer((void**)&g.h_ys, h.hs_ys, 0));
+    g.cap = cap;
+    g.FK = FK;
     h.stage_cap = cap;
     return LSS_OK;
 }
 
-// Counters and the first stage_cap spikes to pinned host memory (capturable).
+// Counters and the first stage_cap spikes to pinned host memory by one kernel that moves only the
+// spikes found (capturable).
 lss_status enqueue_readback(Handle& h) {
-    const int cap = h.stage_cap;
-    const int FK = h.nC * h.K;
-    HIP_TRY(hipMemcpyAsync(h.hs_counters, h.counters.p, sizeof(int) * 8, hipMemcpyDeviceToHost, h.stream));
-    HIP_TRY(hipMemcpyAsync(h.hs_col, h.sp_col.p, sizeof(int) * cap, hipMemcpyDeviceToHost, h.stream));
-    HIP_TRY(hipMemcpyAsync(h.hs_tmp, h.sp_tmp.p, sizeof(int) * cap, hipMemcpyDeviceToHost, h.stream));
-    HIP_TRY(hipMemcpyAsync(h.hs_amp, h.sp_amp.p, sizeof(float) * cap, hipMemcpyDeviceToHost, h.stream));
-    HIP_TRY(hipMemcpyAsync(h.hs_tF, h.tF.p, sizeof(float) * cap * FK, hipMemcpyDeviceToHost, h.stream));
-    HIP_TRY(hipMemcpyAsync(h.hs_xs, h.xs.p, sizeof(float) * cap, hipMemcpyDeviceToHost, h.stream));
-    HIP_TRY(hipMemcpyAsync(h.hs_ys, h.ys.p, sizeof(float) * cap, hipMemcpyDeviceToHost, h.stream));
+    HIP_TRY(lss::launch_stage_spikes(h.stage, h.stream));
     return LSS_OK;
 }
+
+const char* const kBarrierTimeout =
+    "persistent pursuit: grid barrier timed out (workgroups not co-resident); results are invalid";
 
 // Spike count and pass count of the last pursuit (synchronises the stream).
 lss_status read_counts(Handle& h, int* n, int* passes) {
     HIP_TRY(hipMemcpyAsync(h.h_counters, h.counters.p, sizeof(int) * 8, hipMemcpyDeviceToHost, h.stream));
     HIP_TRY(hipStreamSynchronize(h.stream));
+    if (h.h_counters[7]) return fail(LSS_ERR_STATE, kBarrierTimeout);
     if (h.h_counters[2]) return fail(LSS_ERR_CAPACITY, "spike capacity (max_spikes) exceeded");
     if (h.h_counters[5])
         return fail(LSS_ERR_CAPACITY, "more than 1024 spikes of all passes reach one 64-sample block "
@@ -550,7 +587,8 @@ lss_status emit_spikes(Handle& h, int n, int passes, int64_t window_start, lss_s
 // ranges.  Device work per step: segment A = window upload, preprocessing, scores and the first P1
 // pursuit passes of every probe as parallel graph branches; one host check of the converged flags;
 // extra passes (non-graph) for probes that need them; segment C = residual, features and the
-// readback of every probe.  With use_graph, A and C are captured once and replayed.
+// readback of every probe.  With use_graph, A and C are captured once and replayed.  When every
+// probe runs the persistent pursuit (all passes in A), C joins A and a step is one graph.
 // ------------------------------------------------------------------------------------------------
 struct Probe {
     Handle* h = nullptr;
@@ -560,6 +598,7 @@ struct Probe {
     int16_t* pin = nullptr;       // pinned window [W][C]
     int64_t W = 0;
     int P1 = 0;
+    int max_grid = 0;             // persistent-pursuit workgroups, so that all probes' fit at once
     hipEvent_t done = nullptr;
 };
 
@@ -574,6 +613,8 @@ struct Batcher {
     int64_t t0 = 0;
     int64_t batches = 0;
     float last_ms = 0.f;
+    bool single = false;     // every probe runs all its passes in segment A: one graph per step
+    float part_ms[6] = {};   // assemble, segment A, extra passes, segment C, emit, total
     ~Batcher() {
         if (gA) (void)hipGraphExecDestroy(gA);
         if (gC) (void)hipGraphExecDestroy(gC);
@@ -588,16 +629,38 @@ struct Batcher {
     }
 };
 
+// Probe part of segment A: upload, preprocessing, scores and the first P1 passes (all passes for
+// a persistent pursuit); the counters go to the host only when extra passes may follow.
+lss_status probe_A(Probe& p) {
+    Handle& h = *p.h;
+    HIP_TRY(hipMemcpyAsync(h.win.p, p.pin, sizeof(int16_t) * p.W * h.C, hipMemcpyHostToDevice, h.stream));
+    if (lss_status st = enqueue_pre(h, h.win.p, p.W)) return st;
+    const lss::PursuitArgs a = pursuit_args(h, h.B.p, h.xw.p, p.W);
+    if (lss_status st = enqueue_pursuit(h, a, p.P1, p.max_grid)) return st;
+    if (p.P1 < h.prm.max_passes)
+        HIP_TRY(hipMemcpyAsync(h.hs_counters, h.counters.p, sizeof(int) * 8, hipMemcpyDeviceToHost, h.stream));
+    return LSS_OK;
+}
+
+// Probe part of segment C: residual, features, readback.
+lss_status probe_C(Probe& p) {
+    Handle& h = *p.h;
+    const lss::PursuitArgs a = pursuit_args(h, h.B.p, h.xw.p, p.W);
+    HIP_TRY(lss::launch_residual(a, h.stream));
+    if (lss_status st = run_features(h, h.xw.p, p.W)) return st;
+    return enqueue_readback(h);
+}
+
+// Segment A; when no probe can need extra passes (b.single) it also holds segment C, so that a
+// step is one graph.
 lss_status segment_A(Batcher& b) {
     HIP_TRY(hipEventRecord(b.fork, b.main));
     for (auto& p : b.pr) {
         Handle& h = *p.h;
         HIP_TRY(hipStreamWaitEvent(h.stream, b.fork, 0));
-        HIP_TRY(hipMemcpyAsync(h.win.p, p.pin, sizeof(int16_t) * p.W * h.C, hipMemcpyHostToDevice, h.stream));
-        if (lss_status st = enqueue_pre(h, h.win.p, p.W)) return st;
-        const lss::PursuitArgs a = pursuit_args(h, h.B.p, h.xw.p, p.W);
-        if (lss_status st = enqueue_passes(h, a, 0, p.P1)) return st;
-        HIP_TRY(hipMemcpyAsync(h.hs_counters, h.counters.p, sizeof(int) * 8, hipMemcpyDeviceToHost, h.stream));
+        if (lss_status st = probe_A(p)) return st;
+        if (b.single)
+            if (lss_status st = probe_C(p)) return st;
         HIP_TRY(hipEventRecord(p.done, h.stream));
         HIP_TRY(hipStreamWaitEvent(b.main, p.done, 0));
     }
@@ -609,10 +672,7 @@ lss_status segment_C(Batcher& b) {
     for (auto& p : b.pr) {
         Handle& h = *p.h;
         HIP_TRY(hipStreamWaitEvent(h.stream, b.fork, 0));
-        const lss::PursuitArgs a = pursuit_args(h, h.B.p, h.xw.p, p.W);
-        HIP_TRY(lss::launch_residual(a, h.stream));
-        if (lss_status st = run_features(h, h.xw.p, p.W)) return st;
-        if (lss_status st = enqueue_readback(h)) return st;
+        if (lss_status st = probe_C(p)) return st;
         HIP_TRY(hipEventRecord(p.done, h.stream));
         HIP_TRY(hipStreamWaitEvent(b.main, p.done, 0));
     }
@@ -634,15 +694,15 @@ lss_status capture(Batcher& b, lss_status (*seg)(Batcher&), hipGraph_t* g, hipGr
     return LSS_OK;
 }
 
-// Assembles probe p's window for batch start t0 into its pinned staging buffer.
+// Assembles probe p's window for batch start t0 into its pinned staging buffer: one contiguous
+// copy, after the rows that repeat sample 0 when the window starts before the recording.
 void assemble(Probe& p, int64_t t0) {
     Handle& h = *p.h;
     const int C = h.C, nt = h.nt;
-    for (int64_t c = 0; c < p.W; ++c) {
-        int64_t smp = t0 - nt + c;
-        if (smp < 0) smp = 0;
-        std::memcpy(p.pin + c * C, p.buf.data() + (smp - p.base) * C, sizeof(int16_t) * C);
-    }
+    int64_t c = 0;
+    for (; c < p.W && t0 - nt + c < 0; ++c) std::memcpy(p.pin + c * C, p.buf.data() + (0 - p.base) * C, sizeof(int16_t) * C);
+    const int64_t first = t0 - nt + c;
+    std::memcpy(p.pin + c * C, p.buf.data() + (first - p.base) * C, sizeof(int16_t) * (p.W - c) * C);
 }
 
 std::string trim(const std::string& s) {
@@ -673,7 +733,7 @@ lss_status lss_default_params(lss_params* p) {
     p->do_car = 1;
     p->precision = LSS_PREC_FP32;
     p->drop_negative = 1;
-    p->th_learned = 8.f;
+    p->th_learned = 8.0;
     p->fs = 30000.f;
     return LSS_OK;
 }
@@ -705,7 +765,7 @@ lss_status lss_create_from_dir(const char* dir, const lss_params* p0, void** han
         try {
             if (kv.count("nt0min")) p.nt0min = std::stoi(kv["nt0min"]);
             if (kv.count("numNearestChans")) p.n_nearest = std::stoi(kv["numNearestChans"]);
-            if (kv.count("Th_learned")) p.th_learned = std::stof(kv["Th_learned"]);
+            if (kv.count("Th_learned")) p.th_learned = std::stod(kv["Th_learned"]);
         } catch (...) {
             return fail(LSS_ERR_IO, "malformed misc.txt");
         }
@@ -997,6 +1057,19 @@ lss_status lss_stream_create(void* const* handles, int32_t n_probes, int32_t bat
         HIP_TRY(hipHostMalloc((void**)&p.pin, sizeof(int16_t) * p.W * h.C, hipHostMallocDefault));
         HIP_TRY(hipEventCreateWithFlags(&p.done, hipEventDisableTiming));
         p.P1 = (h.tuned_passes > 0) ? std::min(h.tuned_passes, h.prm.max_passes) : h.prm.max_passes;
+        if (use_persistent(h, p.W)) {
+            p.P1 = h.prm.max_passes;
+            HIP_TRY(lss::persistent_max_grid(h.T, p.W, &p.max_grid));
+        }
+    }
+    // the probes' persistent kernels run concurrently: share the co-resident workgroups
+    int cap = 0;
+    for (auto& p : b->pr)
+        if (p.max_grid > 0) cap = cap > 0 ? std::min(cap, p.max_grid) : p.max_grid;
+    b->single = true;
+    for (auto& p : b->pr) {
+        if (p.max_grid > 0) p.max_grid = std::max(1, cap / n_probes);
+        if (p.P1 < p.h->prm.max_passes) b->single = false;
     }
     *stream_out = b.release();
     return LSS_OK;
@@ -1027,8 +1100,12 @@ lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready) {
     *ready = 0;
     for (auto& p : b.pr)
         if (p.recv < b.t0 + b.batch + p.h->nt) return LSS_OK;   // not enough samples yet
+    using clk = std::chrono::steady_clock;
+    auto ms_since = [](clk::time_point t) { return std::chrono::duration<float, std::milli>(clk::now() - t).count(); };
+    auto t_start = clk::now();
     for (auto& p : b.pr) assemble(p, b.t0);
-    auto t_start = std::chrono::steady_clock::now();
+    b.part_ms[0] = ms_since(t_start);
+    auto t1 = clk::now();
     // segment A
     if (b.use_graph) {
         if (!b.gA)
@@ -1038,8 +1115,11 @@ lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready) {
         return st;
     }
     HIP_TRY(hipStreamSynchronize(b.main));
+    b.part_ms[1] = ms_since(t1);
+    t1 = clk::now();
     // passes past P1 for probes that have not converged
     for (auto& p : b.pr) {
+        if (b.single) break;
         Handle& h = *p.h;
         if (p.P1 < h.prm.max_passes && h.hs_counters[3] == 0) {
             const lss::PursuitArgs a = pursuit_args(h, h.B.p, h.xw.p, p.W);
@@ -1048,19 +1128,26 @@ lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready) {
             HIP_TRY(hipStreamWaitEvent(b.main, p.done, 0));
         }
     }
-    // segment C
-    if (b.use_graph) {
-        if (!b.gC)
-            if (lss_status st = capture(b, segment_C, &b.graphC, &b.gC)) return st;
-        HIP_TRY(hipGraphLaunch(b.gC, b.main));
-    } else if (lss_status st = segment_C(b)) {
-        return st;
+    b.part_ms[2] = ms_since(t1);
+    t1 = clk::now();
+    // segment C (already in segment A when b.single)
+    if (!b.single) {
+        if (b.use_graph) {
+            if (!b.gC)
+                if (lss_status st = capture(b, segment_C, &b.graphC, &b.gC)) return st;
+            HIP_TRY(hipGraphLaunch(b.gC, b.main));
+        } else if (lss_status st = segment_C(b)) {
+            return st;
+        }
+        HIP_TRY(hipStreamSynchronize(b.main));
     }
-    HIP_TRY(hipStreamSynchronize(b.main));
+    b.part_ms[3] = ms_since(t1);
+    t1 = clk::now();
     for (size_t i = 0; i < b.pr.size(); ++i) {
         Probe& p = b.pr[i];
         Handle& h = *p.h;
         const int* c = h.hs_counters;
+        if (c[7]) return fail(LSS_ERR_STATE, kBarrierTimeout);
         if (c[2]) return fail(LSS_ERR_CAPACITY, "spike capacity (max_spikes) exceeded");
         if (c[5]) return fail(LSS_ERR_CAPACITY, "residual list overflow");
         const int n = c[0] + c[1];
@@ -1070,6 +1157,8 @@ lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready) {
                             : emit_spikes(h, n, c[4], ws, &outs[i]);
         if (st != LSS_OK) return st;
     }
+    b.part_ms[4] = ms_since(t1);
+    b.part_ms[5] = ms_since(t_start);
     b.last_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     ++b.batches;
     b.t0 += b.batch;
@@ -1094,6 +1183,13 @@ lss_status lss_stream_stats(void* stream, float* last_ms, int64_t* batches) {
     return LSS_OK;
 }
 
+lss_status lss_stream_breakdown(void* stream, float* ms) {
+    if (!stream || !ms) return fail(LSS_ERR_ARG, "null argument");
+    Batcher& b = *static_cast<Batcher*>(stream);
+    for (int i = 0; i < 6; ++i) ms[i] = b.part_ms[i];
+    return LSS_OK;
+}
+
 lss_status lss_set_tuning(void* handle, int32_t update_grid, int32_t first_passes) {
     if (!handle || update_grid < 1 || first_passes < 0) return fail(LSS_ERR_ARG, "bad argument");
     Handle& h = *as_handle(handle);
@@ -1110,6 +1206,69 @@ lss_status lss_get_tuning(void* handle, int32_t* update_grid, int32_t* first_pas
     return LSS_OK;
 }
 
+lss_status lss_set_persistent(void* handle, int32_t grid, int32_t first_passes) {
+    if (!handle || grid < 0 || first_passes < 0) return fail(LSS_ERR_ARG, "bad argument");
+    Handle& h = *as_handle(handle);
+    h.pers_grid = grid;
+    h.pers_first = first_passes;
+    return LSS_OK;
+}
+
+lss_status lss_get_persistent(void* handle, int64_t W, int32_t* grid, int32_t* first_passes, int32_t* max_grid) {
+    if (!handle || !grid || !first_passes || !max_grid) return fail(LSS_ERR_ARG, "null argument");
+    Handle& h = *as_handle(handle);
+    *grid = h.pers_grid;
+    *first_passes = h.pers_first;
+    *max_grid = 0;
+    if (W >= 1 && W <= lss::kPersistMaxW) HIP_TRY(lss::persistent_max_grid(h.T, W, max_grid));
+    return LSS_OK;
+}
+
+lss_status lss_autotune_persistent(void* handle, const int16_t* d_window, int64_t W, int32_t reps, int32_t max_grid,
+                                   float* ms, int32_t* best_grid, int32_t* best_first) {
+    if (!handle || !d_window || reps < 1 || max_grid < 0 || !best_grid || !best_first)
+        return fail(LSS_ERR_ARG, "bad argument");
+    if (W > lss::kPersistMaxW) return fail(LSS_ERR_ARG, "window longer than LSS_PERSIST_MAX_WINDOW");
+    Handle& h = *as_handle(handle);
+    int cap = 0;
+    HIP_TRY(lss::persistent_max_grid(h.T, W, &cap));
+    if (max_grid > 0) cap = std::min(cap, max_grid);
+    static const int kGrid[] = {0, 32, 64, 128, 256};
+    static const int kFirst[] = {0, 1, 2, 4};
+    lss_spikes out{};
+    out.capacity = h.prm.max_spikes;
+    float best = 1e30f;
+    const int g0 = h.pers_grid, f0 = h.pers_first;
+    int bg = g0, bf = f0;
+    for (int gi = 0; gi < 5; ++gi)
+        for (int fi = 0; fi < 4; ++fi) {
+            if (ms) ms[gi * 4 + fi] = -1.f;
+            if (kGrid[gi] > cap || (kGrid[gi] == 0 && fi > 0) || kFirst[fi] >= h.prm.max_passes) continue;
+            h.pers_grid = kGrid[gi];
+            h.pers_first = kFirst[fi];
+            float t = 1e30f;
+            for (int r = 0; r <= reps; ++r) {
+                if (lss_status st = lss_sort_window(handle, d_window, W, 1, 0, &out)) {
+                    h.pers_grid = g0;
+                    h.pers_first = f0;
+                    return st;
+                }
+                if (r > 0) t = std::min(t, h.last_ms[7]);
+            }
+            if (ms) ms[gi * 4 + fi] = t;
+            if (t < best) {
+                best = t;
+                bg = kGrid[gi];
+                bf = kFirst[fi];
+            }
+        }
+    h.pers_grid = bg;
+    h.pers_first = bf;
+    *best_grid = bg;
+    *best_first = bf;
+    return LSS_OK;
+}
+
 lss_status lss_autotune(void* handle, const int16_t* d_window, int64_t W, int32_t reps, float* grid_ms,
                         int32_t* best_grid, int32_t* first_passes) {
     if (!handle || !d_window || reps < 1 || !best_grid || !first_passes) return fail(LSS_ERR_ARG, "bad argument");
@@ -1119,11 +1278,16 @@ lss_status lss_autotune(void* handle, const int16_t* d_window, int64_t W, int32_
     out.capacity = h.prm.max_spikes;   // counts only: every output array is NULL
     float best = 1e30f;
     int bg = h.upd_grid, max_passes_seen = 0;
+    const int pers = h.pers_grid;   // the sweep times the per-pass kernels
+    h.pers_grid = 0;
     for (int gi = 0; gi < 6; ++gi) {
         h.upd_grid = kGrids[gi];
         float t = 1e30f;
         for (int r = 0; r <= reps; ++r) {   // first run warms up
-            if (lss_status st = lss_sort_window(handle, d_window, W, 1, 0, &out)) return st;
+            if (lss_status st = lss_sort_window(handle, d_window, W, 1, 0, &out)) {
+                h.pers_grid = pers;
+                return st;
+            }
             if (r > 0) t = std::min(t, h.last_ms[7]);
             max_passes_seen = std::max(max_passes_seen, out.passes);
         }
@@ -1133,6 +1297,7 @@ lss_status lss_autotune(void* handle, const int16_t* d_window, int64_t W, int32_
             bg = kGrids[gi];
         }
     }
+    h.pers_grid = pers;
     h.upd_grid = bg;
     // the first graph segment covers the passes seen plus a margin; the rest run only when needed
     h.tuned_passes = std::min(h.prm.max_passes, max_passes_seen + 4);

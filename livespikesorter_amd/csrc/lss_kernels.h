@@ -56,7 +56,8 @@ struct PursuitArgs {
     float* sp_amp;         // [cap]
     int* pass_off;         // [max_passes + 1] first spike of each pass that found spikes
     // [0] spikes of committed passes, [1] spikes of the last pass, [2] capacity overflow,
-    // [3] converged (a pass found nothing), [4] passes that found spikes, [5] residual list overflow
+    // [3] converged (a pass found nothing), [4] passes that found spikes, [5] residual list overflow,
+    // [6] grid-barrier arrivals and [7] grid-barrier timeout of the persistent pursuit
     int* counters;
     int64_t W;
     int64_t ldT, ldC;
@@ -69,6 +70,16 @@ hipError_t launch_score_all(const PursuitArgs& a, hipStream_t s);     // T <= 10
 hipError_t launch_peaks(const PursuitArgs& a, hipStream_t s);
 // grid = workgroups (grid-stride over the pass's work items; any grid gives the same result)
 hipError_t launch_update(const PursuitArgs& a, int grid, hipStream_t s);
+// Persistent pursuit of windows W <= kPersistMaxW: passes [first_pass, max_passes) in one launch of
+// `grid` co-resident workgroups (clamped at persistent_max_grid); with first_pass = 0 it also does
+// the first scoring, otherwise it continues from the state that first_pass per-pass (peaks,
+// update) rounds left.  Results equal those of score_all + max_passes x (peaks, update).
+// counters[7] = 1 reports a barrier timeout.
+constexpr int kPersistMaxW = 4096;
+hipError_t launch_pursuit_persistent(const PursuitArgs& a, int grid, int first_pass, int max_passes,
+                                     hipStream_t s);
+// Workgroups of the persistent pursuit that the device holds at once for T templates, window W.
+hipError_t persistent_max_grid(int T, int64_t W, int* grid);
 // residual of all passes in pass order (even, then odd spikes of a pass), after the last pass
 hipError_t launch_residual(const PursuitArgs& a, hipStream_t s);
 
@@ -91,6 +102,27 @@ struct FeatureArgs {
     int64_t W, ldC;
 };
 hipError_t launch_features(const FeatureArgs& a, int grid, hipStream_t s);
+
+// Counters and the first min(counters[0] + counters[1], cap) spikes into host-mapped (pinned)
+// buffers: h_counters[8], h_col/h_tmp/h_amp/h_xs/h_ys [cap], h_tF [cap][FK].
+struct StageArgs {
+    const int* counters;
+    const int* col;
+    const int* tmp;
+    const float* amp;
+    const float* tF;
+    const float* xs;
+    const float* ys;
+    int* h_counters;
+    int* h_col;
+    int* h_tmp;
+    float* h_amp;
+    float* h_tF;
+    float* h_xs;
+    float* h_ys;
+    int cap, FK;
+};
+hipError_t launch_stage_spikes(const StageArgs& a, hipStream_t s);
 
 // ---- load-time tables in FP64 (tables.hip) --------------------------------------------------
 // nm[t] = sum U^2; waves[t][m][c] = sum_k U[t][k][c] wpca[k][m]; ctc[j][l][i] (see lss.h).

@@ -49,10 +49,7 @@ __device__ __forceinline__ float score_of(float b, float nmv) {
 // full scan (first pass): one wavefront per column, float4 loads (ldT % 4 == 0)
 // ------------------------------------------------------------------------------------------------
 template <int NV>   // float4 vectors per lane: NV*256 >= T
-__global__ __launch_bounds__(256) void score_all_kernel(PursuitArgs a) {
-    const int lane = threadIdx.x & 63;
-    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (w >= a.W) return;
+__device__ __forceinline__ void score_column(const PursuitArgs& a, int64_t w, int lane) {
     float best = 0.f;
     int bi = 0;
     if (w >= a.nt && w < a.W - a.nt) {
@@ -83,6 +80,12 @@ __global__ __launch_bounds__(256) void score_all_kernel(PursuitArgs a) {
     }
 }
 
+template <int NV>
+__global__ __launch_bounds__(256) void score_all_kernel(PursuitArgs a) {
+    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (w < a.W) score_column<NV>(a, w, threadIdx.x & 63);
+}
+
 // ------------------------------------------------------------------------------------------------
 // peaks.  A column is a peak iff pool > Th^2 and |pool - cf| < 1e-9 (pool = window max).  If any
 // window value v has fl(v - cf) >= 1e-9 then fl(pool - cf) >= fl(v - cf) (rounding is monotone),
@@ -98,6 +101,52 @@ constexpr int kPeakHalo = 128;                       // >= nt
 constexpr int kEmitThreads = 1024;
 constexpr int kEmitPer = 64;                         // columns per thread per emit tile
 constexpr int kEmitTile = kEmitThreads * kEmitPer;   // 65536
+
+// p points at the column's Cfmax in a buffer holding its +-nt neighbours (-inf outside the window)
+__device__ __forceinline__ bool peak_test(const float* p, float th2, int nt) {
+    const float cf = p[0];
+    if (!((double)cf > (double)th2 - 2e-9) || p[-1] - cf >= 1e-9f || p[1] - cf >= 1e-9f) return false;
+    float cm = fmaxf(cf, fmaxf(p[-1], p[1]));
+    for (int d = 2; d <= nt; ++d) {
+        const float vl = p[-d], vr = p[d];
+        if (vl - cf >= 1e-9f || vr - cf >= 1e-9f) return false;
+        cm = fmaxf(cm, fmaxf(vl, vr));
+    }
+    return cm > th2 && fabsf(cm - cf) < 1e-9f;
+}
+
+// Branch-free form of the same test for a whole row held in LDS: mx[i] = max(src[i .. i+S-1]) by
+// log2(S) doubling steps (S = the largest power of two <= 2nt+1, so two such windows cover
+// [w-nt, w+nt]); pool is then exact and the reference's literal test applies.  NT threads, each
+// owning MAXR entries; mx holds NT*MAXR + S/2 values whose tail past the row is -inf, so every
+// read is in bounds and none is conditional.
+template <int NT, int MAXR>
+__device__ __forceinline__ void window_max_pow2(float* mx, int S) {
+    for (int s = 1; s < S; s <<= 1) {
+        float t[MAXR];
+#pragma unroll
+        for (int r = 0; r < MAXR; ++r) {
+            const int i = threadIdx.x + NT * r;
+            t[r] = fmaxf(mx[i], mx[i + s]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < MAXR; ++r) mx[threadIdx.x + NT * r] = t[r];
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ int pow2_window(int nt) {
+    int S = 1;
+    while (2 * S <= 2 * nt + 1) S *= 2;
+    return S;
+}
+
+// j indexes the column in cf/mx (window max of S-runs of cf)
+__device__ __forceinline__ bool peak_pooled(const float* cf, const float* mx, int j, int nt, int S, float th2) {
+    const float pool = fmaxf(mx[j - nt], mx[j + nt - S + 1]);
+    return pool > th2 && fabsf(pool - cf[j]) < 1e-9f;
+}
 
 __global__ __launch_bounds__(kFlagCols) void peaks_flags_kernel(PursuitArgs a) {
     __shared__ float lds[kFlagCols + 2 * kPeakHalo];
@@ -115,21 +164,7 @@ __global__ __launch_bounds__(kFlagCols) void peaks_flags_kernel(PursuitArgs a) {
     __syncthreads();
     const int64_t w = c0 + tid;
     if (w >= a.W) return;
-    const float* p = lds + kPeakHalo + tid;
-    const float cf = p[0];
-    const float th2 = a.th2;
-    bool f = false;
-    if ((double)cf > (double)th2 - 2e-9 && !(p[-1] - cf >= 1e-9f) && !(p[1] - cf >= 1e-9f)) {
-        float cm = fmaxf(cf, fmaxf(p[-1], p[1]));
-        bool dominated = false;
-        for (int d = 2; d <= a.nt; ++d) {
-            const float vl = p[-d], vr = p[d];
-            if (vl - cf >= 1e-9f || vr - cf >= 1e-9f) { dominated = true; break; }
-            cm = fmaxf(cm, fmaxf(vl, vr));
-        }
-        f = !dominated && cm > th2 && fabsf(cm - cf) < 1e-9f;
-    }
-    a.flags[w] = f ? 1 : 0;
+    a.flags[w] = peak_test(lds + kPeakHalo + tid, a.th2, a.nt) ? 1 : 0;
 }
 
 __global__ __launch_bounds__(kEmitThreads) void peaks_emit_kernel(PursuitArgs a) {
@@ -324,40 +359,216 @@ __device__ __forceinline__ void update_score_columns(const PursuitArgs& a, const
 }
 
 template <int NV>
-__global__ __launch_bounds__(256) void update_kernel(PursuitArgs a) {
-    const int base = a.counters[0];
-    int n = a.counters[1];
-    if (n == 0) return;
-    if (base + n > a.cap) n = a.cap - base;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int nt = a.nt, h = nt / 2;
-    const int* col = a.sp_col + base;
-    const int* tmp = a.sp_tmp + base;
-    const float* amp = a.sp_amp + base;
-    const int nB = (2 * nt + 1 + kColsB - 1) / kColsB;
-    const int per = nB;
-    floatx4 nmv[NV];
+__device__ __forceinline__ void load_norms(const PursuitArgs& a, floatx4* nmv) {
+    const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
         const int t = 4 * lane + 256 * i;
         if (t < a.T) nmv[i] = *(const floatx4*)(a.nm + t);
     }
-    for (int64_t q = blockIdx.x; q < (int64_t)n * per; q += gridDim.x) {
-        const int s = (int)(q / per), sub = (int)(q % per);
+}
+
+// Work items q = q0, q0 + stride, ... of a pass's n spikes (col/tmp/amp, ascending columns; global
+// or LDS memory).  A column is written only by the item of the first spike whose window covers it.
+template <int NV>
+__device__ __forceinline__ void update_items(const PursuitArgs& a, const int* col, const int* tmp, const float* amp,
+                                             int n, int64_t q0, int64_t stride, const floatx4* nmv) {
+    const int nt = a.nt;
+    const int nB = (2 * nt + 1 + kColsB - 1) / kColsB;
+    for (int64_t q = q0; q < (int64_t)n * nB; q += stride) {
+        const int s = (int)(q / nB), sub = (int)(q % nB);
         // the spike's column and its neighbours' (independent loads, issued together)
         const int ws = col[s];
         const int cp = (s > 0) ? col[s - 1] : INT_MIN / 2;
         const int c1 = (s + 1 < n) ? col[s + 1] : INT_MAX / 2;
         const int c2 = (s + 2 < n) ? col[s + 2] : INT_MAX / 2;
-        if (sub < nB) {
-            int lo = ws - nt;
-            if (s > 0) lo = max(lo, cp + nt + 1);
-            lo = max(lo, 0);
-            const int hi = (int)min((int64_t)ws + nt, a.W - 1);
-            const int c0 = ws - nt + sub * kColsB;
-            const int w1 = min(c0 + kColsB - 1, hi);
-            if (max(c0, lo) <= w1) update_score_columns<NV>(a, col, tmp, amp, s, n, c1, c2, c0, lo, w1, nmv);
+        int lo = ws - nt;
+        if (s > 0) lo = max(lo, cp + nt + 1);
+        lo = max(lo, 0);
+        const int hi = (int)min((int64_t)ws + nt, a.W - 1);
+        const int c0 = ws - nt + sub * kColsB;
+        const int w1 = min(c0 + kColsB - 1, hi);
+        if (max(c0, lo) <= w1) update_score_columns<NV>(a, col, tmp, amp, s, n, c1, c2, c0, lo, w1, nmv);
+    }
+}
+
+template <int NV>
+__global__ __launch_bounds__(256) void update_kernel(PursuitArgs a) {
+    const int base = a.counters[0];
+    int n = a.counters[1];
+    if (n == 0) return;
+    if (base + n > a.cap) n = a.cap - base;
+    floatx4 nmv[NV];
+    load_norms<NV>(a, nmv);
+    update_items<NV>(a, a.sp_col + base, a.sp_tmp + base, a.sp_amp + base, n, blockIdx.x, gridDim.x, nmv);
+}
+
+// ------------------------------------------------------------------------------------------------
+// persistent pursuit for short windows (W <= kPersistMaxW, the streaming batches): the first
+// scoring and every pass in one launch of G co-resident workgroups, two grid barriers per pass.
+//
+// After the first barrier every workgroup loads the whole Cfmax/imax row into LDS, forms the exact
+// window max by doubling (window_max_pow2), applies the reference's test and compacts in ascending
+// order as peaks_flags/peaks_emit do, so each holds the pass's spike list without exchanging it; workgroup 0 also stores it (sp_*, pass_off, counters) for the residual and
+// feature kernels.  The second barrier keeps every read of Cfmax/imax/B of the pass ahead of the
+// update, which rewrites them (the update of a spike zeroes its own score).  The update is
+// update_kernel's, over the LDS list (or over the stored list for a pass of more than
+// kPersistList spikes).  Every workgroup sees the same scores, so all take the same branches and
+// leave the loop on the same pass.
+//
+// The barrier waits a bounded time: on a timeout (workgroups not co-resident) the kernel sets
+// counters[7], every workgroup leaves, and the host reports the error; the launcher caps G at the
+// occupancy limit so that this does not happen in normal operation.
+// ------------------------------------------------------------------------------------------------
+constexpr int kPersistList = 512;
+constexpr int kBarrierSpin = 1 << 20;   // ~1 s of polling
+constexpr int kPersistLoads = (kPersistMaxW + 2 * kPeakHalo + 255) / 256;   // row values per thread
+constexpr int kPersistMx = 256 * kPersistLoads + kPeakHalo;                   // window-max buffer
+
+// Grid barrier number `gen` (1, 2, ...) over counters[6] (zero at launch).  Every wavefront waits
+// for its own memory operations, so that one device-scope release by thread 0 (after the
+// workgroup barrier) publishes the whole workgroup's stores; every wavefront then acquires.
+// Returns false after a timeout or when another workgroup has timed out.
+__device__ __forceinline__ bool grid_barrier(int* counters, int gen, int* s_ok) {
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        atomicAdd(counters + 6, 1);
+        const int target = gen * (int)gridDim.x;
+        int ok = 1;
+        for (int it = 0; __hip_atomic_load(counters + 6, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; ++it) {
+            if (it >= kBarrierSpin || __hip_atomic_load(counters + 7, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                atomicExch(counters + 7, 1);
+                ok = 0;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(4);
         }
+        *s_ok = ok;
+    }
+    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return *s_ok != 0;
+}
+
+template <int NV>
+__global__ __launch_bounds__(256) void pursuit_persistent_kernel(PursuitArgs a, int first_pass, int max_passes) {
+    extern __shared__ float psm[];   // cf [W + 2H] | mx [kPersistMx] | im [W] | list col, tmp, amp [kPersistList]
+    __shared__ int wsum[4];
+    __shared__ int s_ok;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int W = (int)a.W, H = kPeakHalo;
+    const int N = W + 2 * H;
+    float* cf = psm;
+    float* mx = cf + N;
+    int* im = (int*)(mx + kPersistMx);
+    int* lcol = im + W;
+    int* ltmp = lcol + kPersistList;
+    float* lamp = (float*)(ltmp + kPersistList);
+    const bool lead = blockIdx.x == 0 && tid == 0;
+    floatx4 nmv[NV];
+    load_norms<NV>(a, nmv);
+    int base = 0, np = 0;
+    bool done = false;
+    if (first_pass == 0) {
+        // first pass: every column scored, one wavefront per column
+        for (int w = blockIdx.x * 4 + wave; w < W; w += gridDim.x * 4) score_column<NV>(a, w, lane);
+    } else {
+        // after first_pass passes of the per-pass kernels: commit the last one as peaks_emit does
+        base = a.counters[0] + a.counters[1];
+        np = a.counters[4];
+        done = a.counters[3] != 0;
+    }
+    int gen = 1;
+    if (!grid_barrier(a.counters, gen++, &s_ok)) return;
+    const int R = ((W + 255) / 256) | 1;   // columns per thread, odd: conflict-free LDS strides
+    const int S = pow2_window(a.nt);
+    for (int pass = first_pass; pass < max_passes && !done; ++pass) {
+        {
+            // the Cfmax/imax row (with -inf halos) into LDS, every load in flight at once
+            float v[kPersistLoads];
+            int m[kPersistLoads];
+#pragma unroll
+            for (int r = 0; r < kPersistLoads; ++r) {
+                const int w = tid + 256 * r - H;
+                const bool in = w >= 0 && w < W;
+                v[r] = in ? a.cfmax[w] : -INFINITY;
+                m[r] = in ? a.imax[w] : 0;
+            }
+#pragma unroll
+            for (int r = 0; r < kPersistLoads; ++r) {
+                const int i = tid + 256 * r, w = i - H;
+                if (i < N) cf[i] = v[r];
+                mx[i] = v[r];   // -inf past the row
+                if (w >= 0 && w < W) im[w] = m[r];
+            }
+            if (tid < kPeakHalo) mx[256 * kPersistLoads + tid] = -INFINITY;
+        }
+        __syncthreads();
+        window_max_pow2<256, kPersistLoads>(mx, S);
+        uint32_t bits = 0;
+        for (int q = 0; q < R; ++q) {
+            const int w = tid * R + q;
+            const bool pk = peak_pooled(cf, mx, H + min(w, W - 1), a.nt, S, a.th2);
+            if (pk && w < W) bits |= 1u << q;
+        }
+        const int cnt = __popc(bits);
+        int inc = cnt;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const int vv = __shfl_up(inc, off, 64);
+            if (lane >= off) inc += vv;
+        }
+        if (lane == 63) wsum[wave] = inc;
+        __syncthreads();
+        int woff = 0, n = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int vv = wsum[k];
+            if (k < wave) woff += vv;
+            n += vv;
+        }
+        int pos = woff + inc - cnt;
+        while (bits) {
+            const int q = __ffs(bits) - 1;
+            bits &= bits - 1;
+            const int w = tid * R + q;
+            const int t = im[w];
+            const float am = a.B[(int64_t)w * a.ldT + t] / a.nm[t];
+            if (pos < kPersistList) { lcol[pos] = w; ltmp[pos] = t; lamp[pos] = am; }
+            if (blockIdx.x == 0 && base + pos < a.cap) {
+                a.sp_col[base + pos] = w;
+                a.sp_tmp[base + pos] = t;
+                a.sp_amp[base + pos] = am;
+            }
+            ++pos;
+        }
+        if (n == 0) {
+            if (lead) a.counters[3] = 1;
+            break;
+        }
+        if (lead) {
+            if (base + n > a.cap) a.counters[2] = 1;
+            a.pass_off[np] = base;
+            a.pass_off[np + 1] = min(base + n, a.cap);
+        }
+        ++np;
+        // every workgroup has read this pass's Cfmax/imax/B before any update writes them (and
+        // workgroup 0's stored list is complete)
+        if (!grid_barrier(a.counters, gen++, &s_ok)) return;
+        const int nu = min(n, a.cap - base);   // spikes stored, as update_kernel applies them
+        if (nu > kPersistList)
+            update_items<NV>(a, a.sp_col + base, a.sp_tmp + base, a.sp_amp + base, nu, blockIdx.x, gridDim.x, nmv);
+        else
+            update_items<NV>(a, lcol, ltmp, lamp, nu, blockIdx.x, gridDim.x, nmv);
+        base += n;
+        if (pass + 1 < max_passes && !grid_barrier(a.counters, gen++, &s_ok)) return;
+    }
+    if (lead) {
+        a.counters[0] = base;
+        a.counters[1] = 0;
+        a.counters[4] = np;
     }
 }
 
@@ -520,6 +731,46 @@ hipError_t launch_update(const PursuitArgs& a, int grid, hipStream_t s) {
     return hipGetLastError();
 }
 
+size_t persistent_lds(int64_t W) {
+    return sizeof(float) * (W + 2 * kPeakHalo + kPersistMx) + sizeof(int) * W +
+           (2 * sizeof(int) + sizeof(float)) * kPersistList;
+}
+
+typedef void (*PersistentFn)(PursuitArgs, int, int);
+PersistentFn persistent_fn(int T) {
+    switch ((T + 255) / 256) {
+        case 1: return pursuit_persistent_kernel<1>;
+        case 2: return pursuit_persistent_kernel<2>;
+        case 3: return pursuit_persistent_kernel<3>;
+        case 4: return pursuit_persistent_kernel<4>;
+        default: return nullptr;
+    }
+}
+
+hipError_t persistent_max_grid(int T, int64_t W, int* grid) {
+    const PersistentFn fn = persistent_fn(T);
+    if (!fn || W < 1 || W > kPersistMaxW) return hipErrorInvalidValue;
+    int dev = 0, cus = 0, per = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)fn, 256, persistent_lds(W));
+    if (e != hipSuccess) return e;
+    *grid = per * cus;
+    return hipSuccess;
+}
+
+hipError_t launch_pursuit_persistent(const PursuitArgs& a, int grid, int first_pass, int max_passes, hipStream_t s) {
+    const PersistentFn fn = persistent_fn(a.T);
+    if (!fn || a.W > kPersistMaxW || a.nt > kPeakHalo || grid < 1 || first_pass < 0 || first_pass >= max_passes)
+        return hipErrorInvalidValue;
+    int cap = 0;
+    hipError_t e = persistent_max_grid(a.T, a.W, &cap);
+    if (e != hipSuccess) return e;
+    if (cap < 1) return hipErrorInvalidConfiguration;
+    hipLaunchKernelGGL(fn, dim3(std::min(grid, cap)), dim3(256), persistent_lds(a.W), s, a, first_pass, max_passes);
+    return hipGetLastError();
+}
+
 hipError_t launch_residual(const PursuitArgs& a, hipStream_t s) {
     if (a.C > 1024) return hipErrorInvalidValue;
     const unsigned nb = (unsigned)((a.W + kResCols - 1) / kResCols);
@@ -609,6 +860,31 @@ hipError_t launch_features(const FeatureArgs& a, int grid, hipStream_t s) {
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(features_kernel, dim3(grid), dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------------
+// Readback staging: the counters and the first min(n, cap) spikes (n = counters[0] + counters[1])
+// written straight into pinned host buffers, so that a replayed graph moves only the spikes found.
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void stage_spikes_kernel(StageArgs a) {
+    int n = a.counters[0] + a.counters[1];
+    n = min(n, a.cap);
+    const int gid = blockIdx.x * 256 + threadIdx.x, stride = gridDim.x * 256;
+    if (gid < 8) a.h_counters[gid] = a.counters[gid];
+    for (int i = gid; i < n; i += stride) {
+        a.h_col[i] = a.col[i];
+        a.h_tmp[i] = a.tmp[i];
+        a.h_amp[i] = a.amp[i];
+        a.h_xs[i] = a.xs[i];
+        a.h_ys[i] = a.ys[i];
+    }
+    const int64_t nf = (int64_t)n * a.FK;
+    for (int64_t e = gid; e < nf; e += stride) a.h_tF[e] = a.tF[e];
+}
+
+hipError_t launch_stage_spikes(const StageArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(stage_spikes_kernel, dim3(64), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 

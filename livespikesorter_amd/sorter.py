@@ -206,6 +206,17 @@ class Sorter:
         check(lib().lss_last_timings(self._h, ms.ctypes.data, 8), "lss_last_timings")
         return dict(zip(TIMING_KEYS, ms.tolist()))
 
+    def set_persistent(self, grid: int, first_passes: int = 0):
+        """Pursuit of windows <= 4096 samples: the first `first_passes` passes on the per-pass kernels,
+        the rest in the persistent kernel of `grid` workgroups (grid 0 = per-pass kernels only)."""
+        check(lib().lss_set_persistent(self._h, int(grid), int(first_passes)), "lss_set_persistent")
+
+    def persistent(self, W: int):
+        """(grid, first_passes, device co-resident limit for windows of W samples; 0 if W is too long)."""
+        g, f, m = ct.c_int32(), ct.c_int32(), ct.c_int32()
+        check(lib().lss_get_persistent(self._h, int(W), ct.byref(g), ct.byref(f), ct.byref(m)), "lss_get_persistent")
+        return g.value, f.value, m.value
+
     def profile_update(self, enable: bool = True):
         """Time each pursuit update launch with HIP events (measurement mode)."""
         check(lib().lss_profile_update(self._h, int(enable)), "lss_profile_update")

@@ -46,6 +46,12 @@ class StreamingSorter:
         check(lib().lss_stream_stats(self._h, ct.byref(ms), ct.byref(n)), "lss_stream_stats")
         return ms.value
 
+    def breakdown(self) -> dict:
+        """Host wall-time split of the last step (ms)."""
+        ms = np.zeros(6, np.float32)
+        check(lib().lss_stream_breakdown(self._h, ms.ctypes.data), "lss_stream_breakdown")
+        return dict(zip(["assemble", "segment_A", "extra_passes", "segment_C", "emit", "total"], ms.tolist()))
+
     def close(self):
         if getattr(self, "_h", None):
             check(lib().lss_stream_destroy(self._h), "lss_stream_destroy")
@@ -72,3 +78,22 @@ def autotune(sorter: Sorter, window_dev, reps: int = 2):
     check(lib().lss_autotune(sorter._h, window_dev.data_ptr(), window_dev.shape[0], int(reps), ms.ctypes.data,
                              ct.byref(g), ct.byref(p)), "lss_autotune")
     return g.value, p.value, dict(zip([256, 512, 1024, 2048, 4096, 8192], ms.tolist()))
+
+
+PERSIST_GRIDS = [0, 32, 64, 128, 256]
+PERSIST_FIRST = [0, 1, 2, 4]
+
+
+def autotune_persistent(sorter: Sorter, window_dev, reps: int = 2, max_grid: int = 0):
+    """Picks the pursuit schedule of a short window (<= 4096 samples): per-pass kernels (grid 0) or
+    the persistent kernel at 32..256 workgroups (at most max_grid; 0 = the device limit) after 0, 1,
+    2 or 4 leading per-pass rounds.  Returns (grid, first_passes, {(grid, first): ms}) over the
+    candidates that ran."""
+    ms = np.zeros(20, np.float32)
+    g, f = ct.c_int32(), ct.c_int32()
+    check(lib().lss_autotune_persistent(sorter._h, window_dev.data_ptr(), window_dev.shape[0], int(reps),
+                                        int(max_grid), ms.ctypes.data, ct.byref(g), ct.byref(f)),
+          "lss_autotune_persistent")
+    table = {(gg, ff): float(ms[4 * i + j]) for i, gg in enumerate(PERSIST_GRIDS)
+             for j, ff in enumerate(PERSIST_FIRST) if ms[4 * i + j] >= 0}
+    return g.value, f.value, table

@@ -100,11 +100,27 @@ def _pursuit_case(sorter, synth1, ib, max_passes=100):
     return x32, B32, tb, ctc_ks, ws, U32, w32
 
 
-def test_pursuit_bit_exact(sorter, synth1):
+@pytest.mark.parametrize("W,grid,first", [(60122, 128, 0), (3122, 0, 0), (3122, 32, 0), (3122, 128, 0),
+                                          (3122, 64, 2)])
+def test_pursuit_bit_exact(sorter, synth1, W, grid, first):
     """Given identical FP32 scores, residual and tables, the device pursuit reproduces the
-    reference pursuit (template_matching.py:157-198) bit for bit."""
-    print("[lss] test_pursuit_bit_exact batch 0", flush=True)
+    reference pursuit (template_matching.py:157-198) bit for bit: long windows and grid 0 on the
+    per-pass kernels, short windows on the persistent kernel at 32 and 128 workgroups, and two
+    per-pass rounds followed by the persistent kernel."""
+    print(f"[lss] test_pursuit_bit_exact batch 0 W={W} persistent grid {grid} after {first}", flush=True)
+    sorter.set_persistent(grid, first)
+    try:
+        _pursuit_bit_exact(sorter, synth1, W)
+    finally:
+        sorter.set_persistent(128)
+
+
+def _pursuit_bit_exact(sorter, synth1, W):
     x32, B32, tb, ctc_ks, ws, U32, w32 = _pursuit_case(sorter, synth1, 0)
+    if W < x32.shape[1]:
+        x32 = x32[:, 5000:5000 + W].contiguous()
+        B32 = ko.project(x32, U32, w32)
+        ws = ws + 5000
     ref = ko.run_matching(x32, U32, w32, ctc_ks, Th=synth1.art["Th"], max_passes=100,
                           nm=torch.from_numpy(tb["nm"]), waves=torch.from_numpy(tb["waves"]), B=B32)
     Bd = B32.T.contiguous().to(DEV)
@@ -121,6 +137,46 @@ def test_pursuit_bit_exact(sorter, synth1):
     # final score matrix ([T][W] in the oracle, [W][T] on the device) and residual
     assert np.array_equal(Bd.cpu().numpy().T, ref.B.numpy())
     assert np.array_equal(Xd.cpu().numpy().T, ref.Xres.numpy())
+
+
+@pytest.mark.parametrize("grid,first", [(0, 0), (16, 0), (256, 0), (64, 1)])
+def test_pursuit_crowded_passes(grid, first):
+    """nt = 3, T = 16, W = 4096, random scores and a low threshold: the first pass finds ~600
+    peaks, more than the persistent kernel keeps in LDS (512), so it takes its stored-list branch.
+    Bit-exact against the reference pursuit, for the per-pass kernels, two persistent grids and one
+    per-pass round before the persistent kernel."""
+    from livespikesorter_amd.sorter import Sorter
+    print(f"[lss] test_pursuit_crowded_passes grid {grid} after {first}", flush=True)
+    rng = np.random.default_rng(7)
+    C, T, K, nt, W = 32, 16, 2, 3, 4096
+    art = dict(U=rng.standard_normal((T, K, C)).astype(np.float32) * 0.3,
+               wPCA=rng.standard_normal((K, nt)).astype(np.float32) * 0.5,
+               Wrot=np.eye(C, dtype=np.float32), hp_filter=np.zeros(101, np.float32),
+               xc=np.arange(C, dtype=np.float32), yc=np.zeros(C, np.float32), nt0min=1, nC=4, Th=0.1)
+    s = Sorter(art, max_window=W, max_passes=60, max_spikes=1 << 15)
+    try:
+        s.set_persistent(grid, first)
+        tb = s.tables()
+        B32 = torch.from_numpy(rng.standard_normal((T, W)).astype(np.float32))
+        x32 = torch.from_numpy(rng.standard_normal((C, W)).astype(np.float32))
+        ctc_ks = torch.from_numpy(tb["ctc"]).permute(2, 0, 1).contiguous()
+        ref = ko.run_matching(x32, torch.from_numpy(art["U"]), torch.from_numpy(art["wPCA"]), ctc_ks, Th=0.1,
+                              max_passes=60, nm=torch.from_numpy(tb["nm"]), waves=torch.from_numpy(tb["waves"]),
+                              B=B32.clone())
+        assert ref.per_pass[0] > 512, ref.per_pass[:4]
+        Bd = B32.T.contiguous().to(DEV)
+        Xd = x32.T.contiguous().to(DEV)
+        got = s.stage_pursuit(Bd, Xd, 1 << 20)
+        print(f"[lss]   passes {got.passes} spikes {len(got.column)} (oracle per pass {ref.per_pass[:4]})",
+              flush=True)
+        assert got.passes == ref.passes
+        np.testing.assert_array_equal(got.column, ref.st[:, 0].numpy())
+        np.testing.assert_array_equal(got.template, ref.st[:, 1].numpy())
+        np.testing.assert_array_equal(got.amplitude, ref.amps[:, 0].numpy())
+        assert np.array_equal(Bd.cpu().numpy().T, ref.B.numpy())
+        assert np.array_equal(Xd.cpu().numpy().T, ref.Xres.numpy())
+    finally:
+        s.close()
 
 
 def test_features_stage(sorter, synth1):

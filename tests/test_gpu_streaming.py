@@ -1,6 +1,7 @@
 """Multi-probe streaming batcher (graph-captured) against the single-window device path and the
 CPU oracle.  The batcher must return exactly the spikes that sorting each window alone returns:
-same kernels, and the tuning (update grid, first-segment passes) does not change results."""
+same kernels, and the tuning (persistent or per-pass pursuit, grids, first-segment passes) does
+not change results."""
 import numpy as np
 import pytest
 import torch
@@ -77,13 +78,19 @@ def test_graph_and_direct_launches_agree_and_tuning_is_result_neutral(probes):
     s1 = _make_sorters(probes)
     g = _stream(probes, s1, use_graph=True, chunk_seed=1)
     s2 = _make_sorters(probes)
-    # tuned: the fastest update grid, and a deliberately short first segment so that the extra
-    # passes between the two graphs run
+    # tuned: probe 0 with the fastest persistent grid; probe 1 on the per-pass kernels with the
+    # fastest update grid and a deliberately short first segment, so that the extra passes between
+    # the two graphs run (s1 runs both probes persistent, one graph per step)
+    from livespikesorter_amd.streaming import autotune_persistent
     w = torch.from_numpy(_window(probes[0].rec, BATCH, probes[0].art["nt"])).to("cuda:0")
-    grid, p1, table = autotune(s2[0], w)
-    print(f"[lss]   autotune probe 0: grid {grid}, first passes {p1}, ms {table}", flush=True)
+    pg, pf, ptable = autotune_persistent(s2[0], w, max_grid=s2[0].persistent(w.shape[0])[2] // 2)
+    print(f"[lss]   persistent autotune probe 0: grid {pg} after {pf}, ms {ptable}", flush=True)
+    s2[1].set_persistent(0)
+    w1 = torch.from_numpy(_window(probes[1].rec, BATCH, probes[1].art["nt"])).to("cuda:0")
+    grid, p1, table = autotune(s2[1], w1)
+    print(f"[lss]   autotune probe 1: grid {grid}, first passes {p1}, ms {table}", flush=True)
     from livespikesorter_amd._lib import lib
-    assert lib().lss_set_tuning(s2[1]._h, 512, 2) == 0
+    assert lib().lss_set_tuning(s2[1]._h, grid, 2) == 0
     d = _stream(probes, s2, use_graph=False, chunk_seed=2)
     for i in range(len(probes)):
         for a, b in zip(g[i], d[i]):
