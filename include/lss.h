@@ -205,6 +205,30 @@ lss_status lss_stream_stats(void* stream, float* last_ms, int64_t* batches);
  * [5] total. */
 lss_status lss_stream_breakdown(void* stream, float* ms);
 
+/* ---- template refinement (no reference counterpart) ---------------------------------------------- */
+
+/* Residual energy of a fixed spike set, the loss of template refinement against drift:
+ *   R = X - sum_s amp[s] * wave_{tmp[s]} placed at columns col[s] - nt/2 .. col[s] + nt/2,
+ *   wave_t[m][c] = sum_k U[t][k][c] wPCA[k][m]  (the residual update of template_matching.py:197;
+ *   contributions of overlapping spikes add; columns outside [0, W) are dropped),
+ *   loss = sum R^2 (FP64 sum of FP32 residuals).
+ * d_x: [W][C] whitened window (device); d_col/d_tmp/d_amp: n spikes sorted by column (device;
+ * col = peak column in the window, as lss_spikes.column); d_U [T][K][C] and d_wpca [K][nt]
+ * (device) may differ from the handle's templates.  The residual is recomputed on every call
+ * (nothing of a forward pass is kept).  Results are deterministic.  Synchronises the stream. */
+lss_status lss_residual_energy(void* handle, const float* d_x, int64_t W, const int32_t* d_col,
+                               const int32_t* d_tmp, const float* d_amp, int64_t n, const float* d_U,
+                               const float* d_wpca, double* loss);
+/* The same loss and its gradient (adjoint): d_grad_U [T][K][C] = dL/dU and d_grad_wpca [K][nt] =
+ * dL/dwPCA (device, FP32), from the spike-aligned residual windows
+ * G_t[m][c] = sum_{s: tmp=t} amp_s R[col_s - nt/2 + m][c]:
+ * dL/dU[t] = -2 wPCA G_t, dL/dwPCA = -2 sum_t U_t G_t. */
+lss_status lss_residual_energy_grad(void* handle, const float* d_x, int64_t W, const int32_t* d_col,
+                                    const int32_t* d_tmp, const float* d_amp, int64_t n, const float* d_U,
+                                    const float* d_wpca, double* loss, float* d_grad_U, float* d_grad_wpca);
+/* Device time of the last refinement call in ms: [0] waves + residual + loss, [1] adjoint, [2] total. */
+lss_status lss_last_refine_timings(void* handle, float* ms);
+
 /* ---- tuning ------------------------------------------------------------------------------------- */
 
 /* Per-pass kernels: times the window (device memory) with each candidate update grid (256 .. 8192

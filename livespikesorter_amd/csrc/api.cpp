@@ -89,6 +89,13 @@ struct Handle {
     float *hs_amp = nullptr, *hs_tF = nullptr, *hs_xs = nullptr, *hs_ys = nullptr;
     lss::StageArgs stage{};      // device-side view of the staging buffers
 
+    // template refinement (allocated on first use): residual, waves of the given U/wPCA, partials
+    DevBuf<float> rf_R, rf_waves, rf_dwpart, rf_dupart;
+    DevBuf<double> rf_part, rf_loss;
+    DevBuf<int> rf_flag;
+    hipEvent_t rf_ev[3] = {};
+    float rf_ms[3] = {};         // residual (with waves), adjoint, total of the last call
+
     hipEvent_t ev[8] = {};
     // per-launch timing of the pursuit update kernel (measurement mode)
     bool prof_update = false;
@@ -106,6 +113,8 @@ struct Handle {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         for (auto& e : upd_ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto& e : rf_ev)
             if (e) (void)hipEventDestroy(e);
         if (h_counters) (void)hipHostFree(h_counters);
         for (void* q : {(void*)hs_counters, (void*)hs_col, (void*)hs_tmp, (void*)hs_amp, (void*)hs_tF,
@@ -979,6 +988,70 @@ lss_status lss_stage_pursuit(void* handle, float* d_b_wt, float* d_x_wc, int64_t
     int n = 0, passes = 0;
     if (lss_status st = read_counts(h, &n, &passes)) return st;
     return emit_spikes(h, n, passes, window_start, out);
+}
+
+// ---- template refinement ------------------------------------------------------------------------
+lss_status refine_run(Handle& h, const float* d_x, int64_t W, const int32_t* d_col, const int32_t* d_tmp,
+                      const float* d_amp, int64_t n, const float* d_U, const float* d_wpca, double* loss, float* d_gU,
+                      float* d_gW) {
+    if (!d_x || !d_U || !d_wpca || !loss || (n > 0 && (!d_col || !d_tmp || !d_amp)))
+        return fail(LSS_ERR_ARG, "null argument");
+    if (W < 1 || W > h.Wmax) return fail(LSS_ERR_ARG, "W must be in [1, max_window]");
+    if (n < 0 || n > INT32_MAX) return fail(LSS_ERR_ARG, "bad spike count");
+    if (!h.rf_R.p) {
+        HIP_TRY(h.rf_R.alloc((size_t)h.Wmax * h.ldC));
+        HIP_TRY(h.rf_waves.alloc((size_t)h.T * h.nt * h.ldC));
+        HIP_TRY(h.rf_dwpart.alloc((size_t)h.T * h.K * h.nt));
+        HIP_TRY(h.rf_dupart.alloc((size_t)h.T * lss::refine_chunks(h.nt) * h.K * h.C));
+        HIP_TRY(h.rf_part.alloc((size_t)lss::refine_blocks(h.Wmax)));
+        HIP_TRY(h.rf_loss.alloc(1));
+        HIP_TRY(h.rf_flag.alloc(1));
+        for (auto& e : h.rf_ev) HIP_TRY(hipEventCreate(&e));
+    }
+    lss::RefineArgs a{};
+    a.X = d_x; a.ldx = h.C;
+    a.vec_x = (h.C % 4 == 0) && ((reinterpret_cast<uintptr_t>(d_x) & 15) == 0);
+    a.col = d_col; a.tmp = d_tmp; a.amp = d_amp; a.n = (int)n;
+    a.U = d_U; a.wpca = d_wpca; a.waves = h.rf_waves.p; a.R = h.rf_R.p;
+    a.part = h.rf_part.p; a.loss = h.rf_loss.p; a.gU = d_gU; a.dwpart = h.rf_dwpart.p; a.dupart = h.rf_dupart.p; a.gW = d_gW;
+    a.flag = h.rf_flag.p;
+    a.W = W; a.ldC = h.ldC; a.T = h.T; a.C = h.C; a.K = h.K; a.nt = h.nt;
+    HIP_TRY(hipEventRecord(h.rf_ev[0], h.stream));
+    HIP_TRY(lss::launch_fill_i32(h.rf_flag.p, 1, 0, h.stream));
+    HIP_TRY(lss::launch_waves(d_U, (int64_t)h.K * h.C, d_wpca, h.T, h.K, h.C, h.nt, h.rf_waves.p, h.ldC, h.stream));
+    HIP_TRY(lss::launch_refine_residual(a, h.stream));
+    HIP_TRY(hipEventRecord(h.rf_ev[1], h.stream));
+    if (d_gU && d_gW) HIP_TRY(lss::launch_refine_adjoint(a, h.stream));
+    HIP_TRY(hipEventRecord(h.rf_ev[2], h.stream));
+    int flag = 0;
+    HIP_TRY(hipMemcpyAsync(loss, h.rf_loss.p, sizeof(double), hipMemcpyDeviceToHost, h.stream));
+    HIP_TRY(hipMemcpyAsync(&flag, h.rf_flag.p, sizeof(int), hipMemcpyDeviceToHost, h.stream));
+    HIP_TRY(hipStreamSynchronize(h.stream));
+    HIP_TRY(hipEventElapsedTime(&h.rf_ms[0], h.rf_ev[0], h.rf_ev[1]));
+    HIP_TRY(hipEventElapsedTime(&h.rf_ms[1], h.rf_ev[1], h.rf_ev[2]));
+    h.rf_ms[2] = h.rf_ms[0] + h.rf_ms[1];
+    if (flag) return fail(LSS_ERR_ARG, "template id out of range in the spike list");
+    return LSS_OK;
+}
+
+lss_status lss_residual_energy(void* handle, const float* d_x, int64_t W, const int32_t* d_col, const int32_t* d_tmp,
+                               const float* d_amp, int64_t n, const float* d_U, const float* d_wpca, double* loss) {
+    if (!handle) return fail(LSS_ERR_ARG, "null handle");
+    return refine_run(*as_handle(handle), d_x, W, d_col, d_tmp, d_amp, n, d_U, d_wpca, loss, nullptr, nullptr);
+}
+
+lss_status lss_residual_energy_grad(void* handle, const float* d_x, int64_t W, const int32_t* d_col,
+                                    const int32_t* d_tmp, const float* d_amp, int64_t n, const float* d_U,
+                                    const float* d_wpca, double* loss, float* d_grad_U, float* d_grad_wpca) {
+    if (!handle || !d_grad_U || !d_grad_wpca) return fail(LSS_ERR_ARG, "null argument");
+    return refine_run(*as_handle(handle), d_x, W, d_col, d_tmp, d_amp, n, d_U, d_wpca, loss, d_grad_U, d_grad_wpca);
+}
+
+lss_status lss_last_refine_timings(void* handle, float* ms) {
+    if (!handle || !ms) return fail(LSS_ERR_ARG, "null argument");
+    Handle& h = *as_handle(handle);
+    for (int i = 0; i < 3; ++i) ms[i] = h.rf_ms[i];
+    return LSS_OK;
 }
 
 lss_status lss_get_tables(void* handle, float* nm, float* waves, float* ctc, float* wmat) {

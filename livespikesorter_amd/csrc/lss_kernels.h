@@ -124,9 +124,44 @@ struct StageArgs {
 };
 hipError_t launch_stage_spikes(const StageArgs& a, hipStream_t s);
 
+// ---- template refinement (adjoint.hip) --------------------------------------------------------
+// Residual energy of a fixed spike set (sorted by column) and its gradient in U and wPCA.
+struct RefineArgs {
+    const float* X;       // [W][ldx] whitened window
+    int64_t ldx;
+    int vec_x;            // X rows may be read as float4 (ldx % 4 == 0, 16-byte aligned)
+    const int* col;       // [n] spike columns, ascending
+    const int* tmp;       // [n] templates
+    const float* amp;     // [n]
+    int n;
+    const float* U;       // [T][K][C]
+    const float* wpca;    // [K][nt]
+    const float* waves;   // [T][nt][ldC] from U, wpca (launch_waves)
+    float* R;             // [W][ldC] residual (written by launch_refine_residual)
+    double* part;         // [refine_blocks(W)] partial sums of R^2
+    double* loss;         // [1]
+    float* gU;            // [T][K][C]
+    float* dwpart;        // [T][K][nt]
+    float* dupart;        // [T][refine_chunks(nt)][K][C] per-row-chunk shares of dU
+    float* gW;            // [K][nt]
+    int* flag;            // [1] set when a template id is out of range
+    int64_t W, ldC;
+    int T, C, K, nt;
+};
+int refine_blocks(int64_t W);
+int refine_chunks(int nt);
+hipError_t launch_fill_i32(int* p, int n, int v, hipStream_t s);   // p[0..n) = v (capturable)
+// R and loss = sum R^2 (C <= 1024, nt <= 127)
+hipError_t launch_refine_residual(const RefineArgs& a, hipStream_t s);
+// gU, gW from R (after launch_refine_residual; K <= 8)
+hipError_t launch_refine_adjoint(const RefineArgs& a, hipStream_t s);
+
 // ---- load-time tables in FP64 (tables.hip) --------------------------------------------------
 // nm[t] = sum U^2; waves[t][m][c] = sum_k U[t][k][c] wpca[k][m]; ctc[j][l][i] (see lss.h).
 // U rows (one per template) have stride ldu >= K*C; waves rows ldC, ctc rows ldT.
+// waves[t][m][c] = sum_k U[t][k][c] wpca[k][m] (FP64 accumulation), rows of stride ldC
+hipError_t launch_waves(const float* U, int64_t ldu, const float* wpca, int T, int K, int C, int nt,
+                        float* waves, int64_t ldC, hipStream_t s);
 hipError_t launch_tables(const float* U, int64_t ldu, const float* wpca, int T, int K, int C, int nt,
                          float* nm, float* waves, int64_t ldC, float* ctc, int64_t ldT, hipStream_t s);
 

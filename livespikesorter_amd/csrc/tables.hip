@@ -81,6 +81,14 @@ __global__ __launch_bounds__(256) void ctc_kernel(const double* __restrict__ G, 
     }
 }
 
+hipError_t launch_waves(const float* U, int64_t ldu, const float* wpca, int T, int K, int C, int nt,
+                        float* waves, int64_t ldC, hipStream_t s) {
+    const int64_t nw = (int64_t)T * nt * C;
+    hipLaunchKernelGGL(waves_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, U, ldu, wpca, T, K, C, nt,
+                       waves, ldC);
+    return hipGetLastError();
+}
+
 hipError_t launch_tables(const float* U, int64_t ldu, const float* wpca, int T, int K, int C, int nt,
                          float* nm, float* waves, int64_t ldC, float* ctc, int64_t ldT, hipStream_t s) {
     if (K > 8) return hipErrorInvalidValue;

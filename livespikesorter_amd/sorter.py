@@ -217,6 +217,32 @@ class Sorter:
         check(lib().lss_get_persistent(self._h, int(W), ct.byref(g), ct.byref(f), ct.byref(m)), "lss_get_persistent")
         return g.value, f.value, m.value
 
+    # ---- template refinement -------------------------------------------------------------------
+    def residual_energy(self, x_wc, col, tmp, amp, U, wpca) -> float:
+        """sum R^2 for the fixed spike set (col sorted ascending); all arguments device tensors:
+        x_wc float32 [W][C], col/tmp int32 [n], amp float32 [n], U float32 [T][K][C], wpca [K][nt]."""
+        loss = ct.c_double()
+        check(lib().lss_residual_energy(self._h, _ptr(x_wc), x_wc.shape[0], _ptr(col), _ptr(tmp), _ptr(amp),
+                                        col.numel(), _ptr(U), _ptr(wpca), ct.byref(loss)), "lss_residual_energy")
+        return loss.value
+
+    def residual_energy_grad(self, x_wc, col, tmp, amp, U, wpca, grad_U=None, grad_wpca=None):
+        """(loss, dL/dU, dL/dwPCA) from the adjoint kernel; the residual is recomputed here."""
+        import torch
+        gU = torch.empty_like(U) if grad_U is None else grad_U
+        gW = torch.empty_like(wpca) if grad_wpca is None else grad_wpca
+        loss = ct.c_double()
+        check(lib().lss_residual_energy_grad(self._h, _ptr(x_wc), x_wc.shape[0], _ptr(col), _ptr(tmp), _ptr(amp),
+                                             col.numel(), _ptr(U), _ptr(wpca), ct.byref(loss), _ptr(gU), _ptr(gW)),
+              "lss_residual_energy_grad")
+        return loss.value, gU, gW
+
+    def refine_timings(self):
+        """Device ms of the last refinement call: residual (with waves and loss), adjoint, total."""
+        ms = np.zeros(3, np.float32)
+        check(lib().lss_last_refine_timings(self._h, ms.ctypes.data), "lss_last_refine_timings")
+        return dict(zip(["residual", "adjoint", "total"], ms.tolist()))
+
     def profile_update(self, enable: bool = True):
         """Time each pursuit update launch with HIP events (measurement mode)."""
         check(lib().lss_profile_update(self._h, int(enable)), "lss_profile_update")

@@ -49,6 +49,7 @@ class SynthConfig:
     sig_interp: float = 20.0
     drift_um: float = 0.0          # vertical probe shift applied to the drift matrix (um)
     drift_rate_um_per_s: float = 0.0  # drift baked into the recording (um per second)
+    t_offset_s: float = 0.0        # recording time of sample 0 (the drift clock)
     dup_bins: int = 7              # int(0.25 ms * fs / 1000), ks/run_kilosort.py:306
     seed: int = 0
 
@@ -58,6 +59,10 @@ CONFIGS = {
     1: dict(n_chan=64, n_units=32, n_samples=120000, NT=60000),
     2: dict(n_chan=384, n_units=600, n_samples=300000, NT=60000),
     3: dict(n_chan=384, n_units=1000, n_samples=1800000, NT=60000),
+    # 2 s at 58-60 s of a probe that drifts linearly 0-20 um over 60 s; the templates are those
+    # of drift 0, so the window's units sit ~19-20 um away from them
+    4: dict(n_chan=384, n_units=600, n_samples=60000, NT=60000, drift_rate_um_per_s=20.0 / 60.0,
+            t_offset_s=58.0),
 }
 
 
@@ -174,7 +179,7 @@ def generate(cfg: SynthConfig) -> Synth:
         tu, au = times[sel], amps[sel]
         if cfg.drift_rate_um_per_s != 0.0:
             for s, a in zip(tu, au):
-                y = uy[u] + cfg.drift_rate_um_per_s * s / cfg.fs
+                y = uy[u] + cfg.drift_rate_um_per_s * (cfg.t_offset_s + s / cfg.fs)
                 f = np.exp(-np.sqrt((ux[u] - xc) ** 2 + (y - yc) ** 2) / cfg.footprint_um)
                 f[f < 1e-3] = 0.0
                 ch = np.nonzero(f)[0]

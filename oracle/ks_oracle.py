@@ -25,6 +25,10 @@ are under ``tests/test_oracle_*.py``):
 * ``extract_features``    - brute force on tiny inputs.
 * ``spike_positions``     - closed form (all mass on one channel gives that channel's position).
 * ``nmatch``              - hand-checked lists.
+* ``residual_energy``     - equals |Xres|^2 of ``run_matching`` for the pursuit's own spikes and
+                            amplitudes; its autograd gradient against FP64 central differences
+                            (the loss is quadratic in U and in wPCA, so central differences are
+                            exact up to rounding); brute-force loops on a tiny case.
 """
 from __future__ import annotations
 
@@ -269,6 +273,34 @@ def run_matching(X: torch.Tensor, U: torch.Tensor, wPCA: torch.Tensor, ctc: torc
         st = torch.zeros((0, 2), dtype=torch.int64)
         amps = torch.zeros((0, 1), dtype=X.dtype)
     return MatchResult(st, amps, Xres, len(per_pass), per_pass, B)
+
+
+# ----------------------------------------------------------------------------------------------
+# template refinement: residual energy of a fixed spike set (no reference counterpart)
+# ----------------------------------------------------------------------------------------------
+def residual_energy(X: torch.Tensor, col: torch.Tensor, tmp: torch.Tensor, amp: torch.Tensor,
+                    U: torch.Tensor, wPCA: torch.Tensor, chunk: int = 512):
+    """Loss of template refinement for a fixed spike set, differentiable in U and wPCA.
+
+    R = X - sum_s amp_s * wave_{tmp_s}[:, m] placed at column col_s - nt//2 + m, with
+    wave = template_waves(U, wPCA) (the update of ks/template_matching.py:197); contributions of
+    different spikes add (the reference's successive passes subtract them one after another) and
+    columns outside [0, W') are dropped.  Returns (sum R^2, R).  X [C][W'], U [T][K][C],
+    wPCA [K][nt], col/tmp int64 [n], amp [n].  For the pursuit's own spikes and amplitudes, R is
+    the pursuit's residual Xres.
+    """
+    C, W = X.shape
+    nt = wPCA.shape[1]
+    tiw = torch.arange(-(nt // 2), nt // 2 + 1)
+    waves = template_waves(U, wPCA)                       # [T][nt][C]
+    R = X
+    for i0 in range(0, col.numel(), chunk):
+        c, t, a = col[i0:i0 + chunk], tmp[i0:i0 + chunk], amp[i0:i0 + chunk]
+        idx = c[:, None] + tiw                            # [n][nt]
+        v = a[:, None, None] * waves[t]                   # [n][nt][C]
+        ok = (idx >= 0) & (idx < W)
+        R = R.index_add(1, idx[ok], -v[ok].T)
+    return (R * R).sum(), R
 
 
 # ----------------------------------------------------------------------------------------------

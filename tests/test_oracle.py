@@ -310,3 +310,71 @@ def test_product_f1_matches_oracle_f1():
     y1 = np.where(rng.random(300) < 0.05, (y0 + 1) % 5, y0)
     for dt in (0, 1, 2):
         assert f1_same_template(t0, y0, t1, y1, dt) == ko.spike_f1(t0, y0, t1, y1, dt)
+
+
+# ---- template refinement loss (residual energy of a fixed spike set) ------------------------
+def test_residual_energy_equals_pursuit_residual(small_synth):
+    """For the pursuit's own spikes and amplitudes the loss is |Xres|^2 of run_matching."""
+    a = small_synth.art
+    X = ko.padded_batch(small_synth.rec, 0, a["NT"], a["nt"], D)
+    x = ko.filter_batch(X, torch.from_numpy(a["hp_filter"]), torch.from_numpy(a["drift"]).double()
+                        @ torch.from_numpy(a["Wrot"]).double())
+    U, W = torch.from_numpy(a["U"]).double(), torch.from_numpy(a["wPCA"]).double()
+    r = ko.run_matching(x, U, W, ko.prepare_matching(U, W), Th=a["Th"], max_passes=100)
+    assert r.st.shape[0] > 20
+    loss, R = ko.residual_energy(x, r.st[:, 0], r.st[:, 1], r.amps[:, 0], U, W)
+    np.testing.assert_allclose(R.numpy(), r.Xres.numpy(), rtol=0, atol=1e-9)
+    assert abs(float(loss) - float((r.Xres ** 2).sum())) <= 1e-9 * float(loss)
+
+
+def _tiny_refinement_case(seed=5, C=6, T=4, K=3, nt=7, W=90, n=14):
+    g = torch.Generator().manual_seed(seed)
+    X = torch.randn(C, W, generator=g, dtype=D)
+    U = torch.randn(T, K, C, generator=g, dtype=D)
+    Wp = torch.randn(K, nt, generator=g, dtype=D)
+    col = torch.randint(-2, W + 2, (n,), generator=g)          # includes windows clipped at both ends
+    col[:3] = torch.tensor([10, 12, 12])                         # overlaps and a repeated column
+    tmp = torch.randint(0, T, (n,), generator=g)
+    amp = torch.rand(n, generator=g, dtype=D) + 0.5
+    return X, col, tmp, amp, U, Wp
+
+
+def test_residual_energy_bruteforce():
+    X, col, tmp, amp, U, Wp = _tiny_refinement_case()
+    C, W = X.shape
+    K, nt = Wp.shape
+    R = X.numpy().copy()
+    for s in range(len(col)):
+        for m in range(nt):
+            w = int(col[s]) - nt // 2 + m
+            if 0 <= w < W:
+                for c in range(C):
+                    R[c, w] -= float(amp[s]) * sum(float(U[tmp[s], k, c]) * float(Wp[k, m]) for k in range(K))
+    loss, Rk = ko.residual_energy(X, col, tmp, amp, U, Wp, chunk=5)
+    np.testing.assert_allclose(Rk.numpy(), R, rtol=0, atol=1e-12)
+    assert abs(float(loss) - float((R ** 2).sum())) <= 1e-12 * float(loss)
+
+
+def test_residual_energy_gradient_vs_central_differences():
+    """Autograd gradient of the oracle loss against FP64 central differences (h = 1e-4) of the
+    full loss.  The loss is quadratic in U (and in wPCA) coordinate-wise, so the differences are
+    exact up to rounding."""
+    X, col, tmp, amp, U, Wp = _tiny_refinement_case()
+    Ug, Wg = U.clone().requires_grad_(True), Wp.clone().requires_grad_(True)
+    loss, _ = ko.residual_energy(X, col, tmp, amp, Ug, Wg)
+    gU, gW = torch.autograd.grad(loss, (Ug, Wg))
+    h = 1e-4
+    for flat in range(U.numel()):
+        e = torch.zeros(U.numel(), dtype=D)
+        e[flat] = h
+        e = e.view_as(U)
+        fd = (ko.residual_energy(X, col, tmp, amp, U + e, Wp)[0] -
+              ko.residual_energy(X, col, tmp, amp, U - e, Wp)[0]) / (2 * h)
+        assert abs(float(fd) - float(gU.reshape(-1)[flat])) <= 1e-6 * (1 + abs(float(fd)))
+    for flat in range(Wp.numel()):
+        e = torch.zeros(Wp.numel(), dtype=D)
+        e[flat] = h
+        e = e.view_as(Wp)
+        fd = (ko.residual_energy(X, col, tmp, amp, U, Wp + e)[0] -
+              ko.residual_energy(X, col, tmp, amp, U, Wp - e)[0]) / (2 * h)
+        assert abs(float(fd) - float(gW.reshape(-1)[flat])) <= 1e-6 * (1 + abs(float(fd)))
