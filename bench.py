@@ -243,19 +243,106 @@ def bench_adjoint(args):
     print(json.dumps(res))
 
 
+def bench_precision(args):
+    """Config-3 precision report: 384 ch, 1000 templates, 60 s.  For FP32 and the BF16 / FP16 MFMA
+    paths with and without the split (x3) correction: throughput over the whole recording, and
+    whitening / score / amplitude / spike-list error against the FP64 CPU oracle on one window."""
+    import torch
+    from livespikesorter_amd import synth
+    from livespikesorter_amd.sorter import Sorter, pad_batch
+    from oracle import ks_oracle as ko
+
+    t0 = time.time()
+    s = synth.make(3, seed=args.seed)
+    a = s.art
+    NT, nt, C, T = a["NT"], a["nt"], s.rec.shape[1], a["U"].shape[0]
+    nb = (s.rec.shape[0] + NT - 1) // NT
+    wins, starts = [], []
+    for ib in range(nb):
+        w, ws = pad_batch(s.rec, ib, NT, nt)
+        wins.append(w)
+        starts.append(ws)
+    dev = torch.device("cuda:0")
+    dwin = torch.from_numpy(np.stack(wins)).to(dev)
+    Wp = dwin.shape[1]
+    print(f"[bench] precision workload ready in {time.time() - t0:.1f}s: C={C} T={T} windows={nb}",
+          file=sys.stderr, flush=True)
+    # FP64 oracle on one window: filtered/whitened data, scores and the spike list
+    torch.set_num_threads(args.cpu_threads)
+    ib = 1 % nb
+    X = ko.padded_batch(s.rec, ib, NT, nt, torch.float64)
+    c0 = time.perf_counter()
+    hp = torch.from_numpy(a["hp_filter"]).double()
+    wm = torch.from_numpy(a["drift"]).double() @ torch.from_numpy(a["Wrot"]).double()
+    xf = ko.filter_batch(X, hp, None)
+    xw64 = wm @ xf
+    x32 = xw64.float()
+    U64, W64 = torch.from_numpy(a["U"]).double(), torch.from_numpy(a["wPCA"]).double()
+    B64 = ko.project(x32.double(), U64, W64).numpy().T
+    ref = ko.sort_window(X, a, ko.window_start(ib, NT, nt), torch.float64, max_passes=50)
+    print(f"[bench] FP64 oracle window in {time.perf_counter() - c0:.1f}s: {len(ref.samples)} spikes",
+          file=sys.stderr, flush=True)
+
+    def rel(a_, b_):
+        return float(np.abs(np.asarray(a_, np.float64) - b_).max() / np.abs(b_).max())
+
+    variants = {}
+    for prec in ["fp32", "bf16", "bf16x3", "fp16", "fp16x3"]:
+        so = Sorter(a, max_window=Wp, max_passes=50, precision=prec, max_spikes=1 << 16)
+        r = run_path(so, dwin, starts, nb, min(args.warmup, nb))
+        ew = rel(so.stage_whiten(xf.float().contiguous().to(dev)).cpu().numpy(), xw64.numpy().T)
+        eb = rel(so.stage_scores(x32.T.contiguous().to(dev)).cpu().numpy(), B64)
+        got = r["outs"][ib]
+        f1 = ko.spike_f1(ref.samples, ref.templates, got.sample, got.template, dt=1)
+        d0 = {(int(t), int(y)): float(v) for t, y, v in zip(ref.samples, ref.templates, ref.amps)}
+        errs = [abs(v - d0[(int(t), int(y))]) / abs(d0[(int(t), int(y))])
+                for t, y, v in zip(got.sample, got.template, got.amplitude) if (int(t), int(y)) in d0]
+        variants[prec] = {
+            "x_realtime": round(nb * NT / a["fs"] / r["elapsed"], 2),
+            "p50_ms": round(float(np.percentile(r["lat_ms"], 50)), 3),
+            "p99_ms": round(float(np.percentile(r["lat_ms"], 99)), 3),
+            "whiten_ms": round(r["stage_ms"]["whiten"], 4), "scores_gemm_ms": round(r["stage_ms"]["scores_gemm"], 4),
+            "whitening_rel_err_vs_fp64": float(f"{ew:.3e}"), "scores_rel_err_vs_fp64": float(f"{eb:.3e}"),
+            "spike_f1_vs_fp64": round(float(f1), 6),
+            "amp_rel_err_vs_fp64": {"median": float(f"{np.median(errs):.3e}") if errs else None,
+                                    "max": float(f"{max(errs):.3e}") if errs else None, "matched": len(errs)},
+        }
+        print(f"[bench] {prec}: {json.dumps(variants[prec])}", file=sys.stderr, flush=True)
+        so.close()
+    for prec in variants:
+        if prec != "fp32":
+            variants[prec]["speedup_vs_fp32"] = round(variants[prec]["x_realtime"] / variants["fp32"]["x_realtime"], 3)
+    res = {
+        "metric": "precision report: score / amplitude / spike-list error vs FP64 oracle and x-realtime, "
+                  "384 ch, 1000 templates, 60 s",
+        "value": variants["bf16x3"]["x_realtime"], "unit": "x realtime (bf16x3 path)", "n_gpus": 1,
+        "steps": nb, "warmup": min(args.warmup, nb),
+        "ms_per_step": round(NT / a["fs"] * 1e3 / variants["bf16x3"]["x_realtime"], 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16x3",
+        "data": "synthetic (seeded probe, templates and spikes; no recordings)",
+        "config": {"workload": "384 ch, 1000 templates, 6 PCs, nt=61, 60 s in NT=60000 Kilosort windows, "
+                               "<=50 pursuit passes; errors on window 1 against the FP64 CPU oracle",
+                   "n_chan": C, "n_templates": T, "windows": nb},
+        "fp64_oracle_spikes": int(len(ref.samples)),
+        "variants": variants,
+    }
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16", "bf16x3"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16", "bf16x3", "fp16x3"])
     ap.add_argument("--compare", default="bf16x3", help="second precision reported beside the first ('none' = none)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--workload", default="window", choices=["window", "stream", "adjoint"],
+    ap.add_argument("--workload", default="window", choices=["window", "stream", "adjoint", "precision"],
                     help="window: 384-ch drop-in windows (default); stream: 4 uneven probes, 100 ms batches; "
-                         "adjoint: template-refinement backward on a 2 s drift window")
+                         "adjoint: template-refinement backward on a 2 s drift window; "
+                         "precision: config-3 precision report (1000 templates, 60 s)")
     ap.add_argument("--seconds", type=float, default=20.0, help="stream workload: seconds per probe")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="instead of the bench, copy a tensor of this many MiB (counter calibration)")
@@ -280,6 +367,8 @@ def main():
         return bench_stream(args)
     if args.workload == "adjoint":
         return bench_adjoint(args)
+    if args.workload == "precision":
+        return bench_precision(args)
 
     from livespikesorter_amd import synth
     from livespikesorter_amd.sorter import Sorter, pad_batch

@@ -44,7 +44,8 @@ typedef enum lss_precision {
     LSS_PREC_FP32 = 0,      /* FP32 MFMA, FP32 accumulation */
     LSS_PREC_BF16 = 1,      /* BF16 MFMA, FP32 accumulation */
     LSS_PREC_FP16 = 2,      /* FP16 MFMA, FP32 accumulation */
-    LSS_PREC_BF16X3 = 3     /* split-BF16 (hi*hi + hi*lo + lo*hi) on BF16 MFMA */
+    LSS_PREC_BF16X3 = 3,    /* split-BF16 (hi*hi + hi*lo + lo*hi) on BF16 MFMA */
+    LSS_PREC_FP16X3 = 4     /* split-FP16 (hi*hi + hi*lo + lo*hi) on FP16 MFMA */
 } lss_precision;
 
 /* Sorter configuration (OnlineSpikesV2 members T, K, M, C, nt0min, numNearestChans, Th_learned,

@@ -13,7 +13,7 @@ LIB_PATH = Path(__file__).resolve().parent / "liblss.so"
 LSS_OK = 0
 STATUS = {0: "LSS_OK", 1: "LSS_ERR_ARG", 2: "LSS_ERR_HIP", 3: "LSS_ERR_IO", 4: "LSS_ERR_CAPACITY",
           5: "LSS_ERR_STATE"}
-PRECISION = {"fp32": 0, "bf16": 1, "fp16": 2, "bf16x3": 3}
+PRECISION = {"fp32": 0, "bf16": 1, "fp16": 2, "bf16x3": 3, "fp16x3": 4}
 PERSIST_MAX_WINDOW = 4096
 
 EXPORTS = ["lss_last_error", "lss_abi_version", "lss_default_params", "lss_create",

@@ -135,7 +135,7 @@ lss_status check_params(const lss_params& p) {
     if (p.max_window < 2 * p.nt + 2) return fail(LSS_ERR_ARG, "max_window too small");
     if (p.max_passes < 1 || p.max_passes > 256) return fail(LSS_ERR_ARG, "max_passes must be in [1, 256]");
     if (p.max_spikes < 1) return fail(LSS_ERR_ARG, "max_spikes must be >= 1");
-    if (p.precision < LSS_PREC_FP32 || p.precision > LSS_PREC_BF16X3) return fail(LSS_ERR_ARG, "bad precision");
+    if (p.precision < LSS_PREC_FP32 || p.precision > LSS_PREC_FP16X3) return fail(LSS_ERR_ARG, "bad precision");
     return LSS_OK;
 }
 

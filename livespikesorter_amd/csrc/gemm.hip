@@ -13,7 +13,8 @@
 // FP32 uses v_mfma_f32_32x32x2_f32 on k-major LDS tiles.  The reduced-precision variants convert
 // the FP32 operands while staging them into LDS (m/n-major, k contiguous, 8 values per lane read)
 // and use v_mfma_f32_32x32x16_{bf16,f16}; bf16x3 splits x = hi + lo (hi = bf16(x),
-// lo = bf16(x - hi)) and accumulates hi*hi + hi*lo + lo*hi in FP32.
+// lo = bf16(x - hi)) and accumulates hi*hi + hi*lo + lo*hi in FP32; fp16x3 is the same split with
+// FP16 halves (lo below FP16's normal range loses bits: its error is reported, not assumed).
 #include "../../include/lss.h"
 #include "lss_kernels.h"
 
@@ -174,6 +175,11 @@ template <>
 struct LpType<LSS_PREC_FP16> { typedef _Float16 T; typedef halfx8 V; };
 template <>
 struct LpType<LSS_PREC_BF16X3> { typedef __bf16 T; typedef bf16x8 V; };
+template <>
+struct LpType<LSS_PREC_FP16X3> { typedef _Float16 T; typedef halfx8 V; };
+
+template <int PREC>
+constexpr bool is_split() { return PREC == LSS_PREC_BF16X3 || PREC == LSS_PREC_FP16X3; }
 
 // element (row, k) of the tile held by this thread's Frag
 template <bool KM>
@@ -201,13 +207,14 @@ __device__ __forceinline__ void stash_lp(typename LpType<PREC>::T (*Hi)[kLdsK], 
             const float x = f.v[h][e];
             const T hi = (T)x;
             Hi[row][k] = hi;
-            if (PREC == LSS_PREC_BF16X3) Lo[row][k] = (T)(x - (float)hi);
+            if (is_split<PREC>()) Lo[row][k] = (T)(x - (float)hi);
         }
 }
 
 template <int PREC>
 __device__ __forceinline__ floatx16 mfma_lp(typename LpType<PREC>::V a, typename LpType<PREC>::V b, floatx16 c) {
-    if constexpr (PREC == LSS_PREC_FP16) return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    if constexpr (PREC == LSS_PREC_FP16 || PREC == LSS_PREC_FP16X3)
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
     else return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
 
@@ -218,7 +225,7 @@ __global__ __launch_bounds__(256) void gemm_lp_kernel(const float* __restrict__ 
                                                       int M, int N, int K, int nM, int nN) {
     typedef typename LpType<PREC>::T T;
     typedef typename LpType<PREC>::V V;
-    constexpr bool X3 = PREC == LSS_PREC_BF16X3;
+    constexpr bool X3 = is_split<PREC>();
     __shared__ __attribute__((aligned(16))) T Ah[BM][kLdsK];
     __shared__ __attribute__((aligned(16))) T Bh[BN][kLdsK];
     __shared__ __attribute__((aligned(16))) T Al[X3 ? BM : 1][kLdsK];
@@ -323,6 +330,7 @@ hipError_t launch_gemm_lp(int prec, const float* A, int64_t lda, int a_km, const
         case LSS_PREC_BF16: return lp_launch<LSS_PREC_BF16>(A, lda, a_km, B, ldb, C, ldc, M, N, K, s);
         case LSS_PREC_FP16: return lp_launch<LSS_PREC_FP16>(A, lda, a_km, B, ldb, C, ldc, M, N, K, s);
         case LSS_PREC_BF16X3: return lp_launch<LSS_PREC_BF16X3>(A, lda, a_km, B, ldb, C, ldc, M, N, K, s);
+        case LSS_PREC_FP16X3: return lp_launch<LSS_PREC_FP16X3>(A, lda, a_km, B, ldb, C, ldc, M, N, K, s);
         default: return hipErrorInvalidValue;
     }
 }
