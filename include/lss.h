@@ -205,6 +205,12 @@ lss_status lss_stream_stats(void* stream, float* last_ms, int64_t* batches);
  * completion), [2] extra passes enqueue, [3] segment C (launch to completion), [4] spike emission,
  * [5] total. */
 lss_status lss_stream_breakdown(void* stream, float* ms);
+/* Template refinement while streaming: every `every_batches` steps (0 = never) a refinement round
+ * starts in which probe i's handle takes one lss_refine_templates step at the end of the round's
+ * i-th step, on that step's window (drift tracking); the refining step's wall time includes it.
+ * lss_stream_last_refine gives a probe's last info (4 doubles). */
+lss_status lss_stream_set_refine(void* stream, int32_t every_batches);
+lss_status lss_stream_last_refine(void* stream, int32_t probe, double* info);
 
 /* ---- template refinement (no reference counterpart) ---------------------------------------------- */
 
@@ -229,6 +235,13 @@ lss_status lss_residual_energy_grad(void* handle, const float* d_x, int64_t W, c
                                     const float* d_wpca, double* loss, float* d_grad_U, float* d_grad_wpca);
 /* Device time of the last refinement call in ms: [0] waves + residual + loss, [1] adjoint, [2] total. */
 lss_status lss_last_refine_timings(void* handle, float* ms);
+/* One refinement step of the handle's own templates on the last window it sorted: the gradient of
+ * that window's residual energy (its spikes, the residual the pursuit left) by the adjoint
+ * kernel, a step along the negative gradient whose length minimises a quadratic fit of the loss
+ * through one trial step, and every derived table (nm, waves, ctc in FP64, iU, Ucc) rebuilt.  The
+ * step is adopted only if it lowers the loss.  info (4 doubles, may be NULL): loss before, loss
+ * after, step length, gradient norm (zeros when nothing changed).  Synchronises the stream. */
+lss_status lss_refine_templates(void* handle, double* info);
 
 /* ---- tuning ------------------------------------------------------------------------------------- */
 

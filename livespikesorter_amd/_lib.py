@@ -25,7 +25,8 @@ EXPORTS = ["lss_last_error", "lss_abi_version", "lss_default_params", "lss_creat
            "lss_stream_create", "lss_stream_destroy", "lss_stream_push", "lss_stream_step",
            "lss_stream_stats", "lss_stream_breakdown", "lss_autotune", "lss_set_tuning", "lss_get_tuning",
            "lss_set_persistent", "lss_get_persistent", "lss_autotune_persistent",
-           "lss_residual_energy", "lss_residual_energy_grad", "lss_last_refine_timings"]
+           "lss_residual_energy", "lss_residual_energy_grad", "lss_last_refine_timings",
+           "lss_refine_templates", "lss_stream_set_refine", "lss_stream_last_refine"]
 
 
 class Params(ct.Structure):
@@ -102,6 +103,9 @@ def lib() -> ct.CDLL:
             "lss_residual_energy": ([vp, vp, i64, vp, vp, vp, i64, vp, vp, ct.POINTER(ct.c_double)], i32),
             "lss_residual_energy_grad": ([vp, vp, i64, vp, vp, vp, i64, vp, vp, ct.POINTER(ct.c_double), vp, vp], i32),
             "lss_last_refine_timings": ([vp, vp], i32),
+            "lss_refine_templates": ([vp, vp], i32),
+            "lss_stream_set_refine": ([vp, i32], i32),
+            "lss_stream_last_refine": ([vp, i32, vp], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

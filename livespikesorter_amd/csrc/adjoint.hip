@@ -286,6 +286,43 @@ __global__ __launch_bounds__(256) void refine_dw_kernel(const float* dwpart, int
     if (threadIdx.x == 0) gW[e] = (float)(-2.0 * s_red[0]);
 }
 
+// out[i] = (float)(x[i] - alpha * y[i]) (alpha in FP64)
+__global__ __launch_bounds__(256) void step_kernel(float* out, const float* x, const float* y, double alpha, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+        out[i] = (float)((double)x[i] - alpha * (double)y[i]);
+}
+
+// part[b] = sum of squares of x over block b's grid-stride share (FP64), then one workgroup sums
+// the partials in order
+constexpr int kSumBlocks = 256;
+__global__ __launch_bounds__(256) void sumsq_kernel(const float* x, int64_t n, double* part) {
+    __shared__ double s_red[256];
+    double acc = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)kSumBlocks * 256)
+        acc += (double)x[i] * (double)x[i];
+    s_red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st) s_red[threadIdx.x] += s_red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = s_red[0];
+}
+
+hipError_t launch_step(float* out, const float* x, const float* y, double alpha, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(step_kernel, dim3(1024), dim3(256), 0, s, out, x, y, alpha, n);
+    return hipGetLastError();
+}
+
+// out[0] += sum of squares of x (out must be initialised); part: kSumBlocks doubles of scratch
+hipError_t launch_sumsq(const float* x, int64_t n, double* part, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(sumsq_kernel, dim3(kSumBlocks), dim3(256), 0, s, x, n, part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(refine_loss_kernel, dim3(1), dim3(256), 0, s, part, kSumBlocks, out);
+    return hipGetLastError();
+}
+
 __global__ void fill_i32_kernel(int* p, int n, int v) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) p[i] = v;

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -66,12 +67,15 @@ struct Handle {
 
     // tables
     DevBuf<float> U, wpca, wmat, nm, waves, ctc, Ucc, xc, yc;
+    DevBuf<double> tables_ws;   // Gram matrix + WtW of the FP64 table build (kept for refinement)
     DevBuf<int> iCC, iU;
     DevBuf<float2> spec, tw;
     std::vector<float> hp_taps;
     int64_t spec_W = -1;
     int spec_lo = 0, spec_hi = 0;
     std::vector<float> h_xc, h_yc;
+    std::vector<int> h_iCC;
+    int64_t last_W = 0;          // window length of the last pursuit (template refinement reads it)
 
     // window buffers
     DevBuf<int16_t> win;
@@ -90,7 +94,10 @@ struct Handle {
     lss::StageArgs stage{};      // device-side view of the staging buffers
 
     // template refinement (allocated on first use): residual, waves of the given U/wPCA, partials
-    DevBuf<float> rf_R, rf_waves, rf_dwpart, rf_dupart;
+    DevBuf<float> rf_R, rf_R2, rf_waves, rf_dwpart, rf_dupart, rf_U, rf_U1, rf_gU, rf_gW, rf_W1, rf_amp;
+    DevBuf<double> rf_sums;
+    float* hs_U = nullptr;       // pinned [T][K][C] (refined templates for iU/Ucc)
+    DevBuf<int> rf_col, rf_tmp;
     DevBuf<double> rf_part, rf_loss;
     DevBuf<int> rf_flag;
     hipEvent_t rf_ev[3] = {};
@@ -118,7 +125,7 @@ struct Handle {
             if (e) (void)hipEventDestroy(e);
         if (h_counters) (void)hipHostFree(h_counters);
         for (void* q : {(void*)hs_counters, (void*)hs_col, (void*)hs_tmp, (void*)hs_amp, (void*)hs_tF,
-                        (void*)hs_xs, (void*)hs_ys})
+                        (void*)hs_xs, (void*)hs_ys, (void*)hs_U})
             if (q) (void)hipHostFree(q);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
     }
@@ -214,6 +221,35 @@ lss_status prepare_spectrum(Handle& h, int64_t W) {
     return LSS_OK;
 }
 
+// iU (channel of largest template energy) and Ucc = U[:, :, iCC[:, iU]] for host U [T][K][C]
+// (prepare_extract, template_matching.py:20-26), uploaded to the handle.
+lss_status upload_extract(Handle& h, const float* U) {
+    const int C = h.C, T = h.T, K = h.K, nC = h.nC;
+    std::vector<int> iU(T);
+    std::vector<float> Ucc((size_t)nC * T * K);
+    for (int t = 0; t < T; ++t) {
+        int best = 0;
+        float bv = -1.f;
+        for (int c = 0; c < C; ++c) {
+            float e = 0.f;   // (U**2).sum(1) in float32, as the reference
+            for (int k = 0; k < K; ++k) {
+                const float u = U[((size_t)t * K + k) * C + c];
+                e += u * u;
+            }
+            if (e > bv) { bv = e; best = c; }
+        }
+        iU[t] = best;
+    }
+    for (int j = 0; j < nC; ++j)
+        for (int t = 0; t < T; ++t)
+            for (int k = 0; k < K; ++k)
+                Ucc[((size_t)j * T + t) * K + k] = U[((size_t)t * K + k) * C + h.h_iCC[(size_t)j * C + iU[t]]];
+    HIP_TRY(hipMemcpyAsync(h.iU.p, iU.data(), sizeof(int) * T, hipMemcpyHostToDevice, h.stream));
+    HIP_TRY(hipMemcpyAsync(h.Ucc.p, Ucc.data(), sizeof(float) * nC * T * K, hipMemcpyHostToDevice, h.stream));
+    HIP_TRY(hipStreamSynchronize(h.stream));
+    return LSS_OK;
+}
+
 lss_status create_impl(const lss_params* p, const lss_artifacts* a, Handle** out) {
     if (!p || !a || !out) return fail(LSS_ERR_ARG, "null argument");
     if (lss_status st = check_params(*p)) return st;
@@ -298,31 +334,13 @@ lss_status create_impl(const lss_params* p, const lss_artifacts* a, Handle** out
     {
         std::vector<int> iCC;
         nearest_channels(h->h_xc, h->h_yc, nC, iCC);
-        std::vector<int> iU(T);
-        std::vector<float> Ucc((size_t)nC * T * K);
-        for (int t = 0; t < T; ++t) {
-            int best = 0;
-            float bv = -1.f;
-            for (int c = 0; c < C; ++c) {
-                float e = 0.f;   // (U**2).sum(1) in float32, as the reference
-                for (int k = 0; k < K; ++k) {
-                    float u = a->U[((size_t)t * K + k) * C + c];
-                    e += u * u;
-                }
-                if (e > bv) { bv = e; best = c; }
-            }
-            iU[t] = best;
-        }
-        for (int j = 0; j < nC; ++j)
-            for (int t = 0; t < T; ++t)
-                for (int k = 0; k < K; ++k)
-                    Ucc[((size_t)j * T + t) * K + k] = a->U[((size_t)t * K + k) * C + iCC[(size_t)j * C + iU[t]]];
         HIP_TRY(hipMemcpy(h->iCC.p, iCC.data(), sizeof(int) * nC * C, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(h->iU.p, iU.data(), sizeof(int) * T, hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(h->Ucc.p, Ucc.data(), sizeof(float) * nC * T * K, hipMemcpyHostToDevice));
+        h->h_iCC = iCC;
+        if (lss_status st = upload_extract(*h, a->U)) return st;
     }
+    HIP_TRY(h->tables_ws.alloc(lss::tables_workspace_doubles(T, K, nt)));
     HIP_TRY(lss::launch_tables(h->U.p, h->ldq, h->wpca.p, T, K, C, nt, h->nm.p, h->waves.p, h->ldC, h->ctc.p, h->ldT,
-                               h->stream));
+                               h->tables_ws.p, h->stream));
 
     // ---- window buffers ----
     HIP_TRY(h->win.alloc((size_t)Wm * C));
@@ -407,6 +425,7 @@ lss_status run_scores(Handle& h, const float* x_wc, float* b_wt, int64_t W, hipE
 constexpr int kFeatureGrid = 1024;
 
 lss::PursuitArgs pursuit_args(Handle& h, float* B, float* X, int64_t W) {
+    h.last_W = W;
     lss::PursuitArgs a{};
     a.B = B; a.X = X; a.nm = h.nm.p; a.ctc = h.ctc.p; a.waves = h.waves.p;
     a.pass_off = h.pass_off.p;
@@ -623,6 +642,8 @@ struct Batcher {
     int64_t batches = 0;
     float last_ms = 0.f;
     bool single = false;     // every probe runs all its passes in segment A: one graph per step
+    int refine_every = 0;    // template refinement step every this many batches (0 = never)
+    std::vector<std::array<double, 4>> refine_info;   // per probe, last refinement (lss_refine_templates)
     float part_ms[6] = {};   // assemble, segment A, extra passes, segment C, emit, total
     ~Batcher() {
         if (gA) (void)hipGraphExecDestroy(gA);
@@ -991,6 +1012,19 @@ lss_status lss_stage_pursuit(void* handle, float* d_b_wt, float* d_x_wc, int64_t
 }
 
 // ---- template refinement ------------------------------------------------------------------------
+lss_status refine_alloc(Handle& h) {
+    if (h.rf_R.p) return LSS_OK;
+    HIP_TRY(h.rf_R.alloc((size_t)h.Wmax * h.ldC));
+    HIP_TRY(h.rf_waves.alloc((size_t)h.T * h.nt * h.ldC));
+    HIP_TRY(h.rf_dwpart.alloc((size_t)h.T * h.K * h.nt));
+    HIP_TRY(h.rf_dupart.alloc((size_t)h.T * lss::refine_chunks(h.nt) * h.K * h.C));
+    HIP_TRY(h.rf_part.alloc((size_t)lss::refine_blocks(h.Wmax)));
+    HIP_TRY(h.rf_loss.alloc(1));
+    HIP_TRY(h.rf_flag.alloc(1));
+    for (auto& e : h.rf_ev) HIP_TRY(hipEventCreate(&e));
+    return LSS_OK;
+}
+
 lss_status refine_run(Handle& h, const float* d_x, int64_t W, const int32_t* d_col, const int32_t* d_tmp,
                       const float* d_amp, int64_t n, const float* d_U, const float* d_wpca, double* loss, float* d_gU,
                       float* d_gW) {
@@ -998,16 +1032,7 @@ lss_status refine_run(Handle& h, const float* d_x, int64_t W, const int32_t* d_c
         return fail(LSS_ERR_ARG, "null argument");
     if (W < 1 || W > h.Wmax) return fail(LSS_ERR_ARG, "W must be in [1, max_window]");
     if (n < 0 || n > INT32_MAX) return fail(LSS_ERR_ARG, "bad spike count");
-    if (!h.rf_R.p) {
-        HIP_TRY(h.rf_R.alloc((size_t)h.Wmax * h.ldC));
-        HIP_TRY(h.rf_waves.alloc((size_t)h.T * h.nt * h.ldC));
-        HIP_TRY(h.rf_dwpart.alloc((size_t)h.T * h.K * h.nt));
-        HIP_TRY(h.rf_dupart.alloc((size_t)h.T * lss::refine_chunks(h.nt) * h.K * h.C));
-        HIP_TRY(h.rf_part.alloc((size_t)lss::refine_blocks(h.Wmax)));
-        HIP_TRY(h.rf_loss.alloc(1));
-        HIP_TRY(h.rf_flag.alloc(1));
-        for (auto& e : h.rf_ev) HIP_TRY(hipEventCreate(&e));
-    }
+    if (lss_status st = refine_alloc(h)) return st;
     lss::RefineArgs a{};
     a.X = d_x; a.ldx = h.C;
     a.vec_x = (h.C % 4 == 0) && ((reinterpret_cast<uintptr_t>(d_x) & 15) == 0);
@@ -1045,6 +1070,136 @@ lss_status lss_residual_energy_grad(void* handle, const float* d_x, int64_t W, c
                                     const float* d_wpca, double* loss, float* d_grad_U, float* d_grad_wpca) {
     if (!handle || !d_grad_U || !d_grad_wpca) return fail(LSS_ERR_ARG, "null argument");
     return refine_run(*as_handle(handle), d_x, W, d_col, d_tmp, d_amp, n, d_U, d_wpca, loss, d_grad_U, d_grad_wpca);
+}
+
+// Loss of the last window's spike set for templates (U1 [T][K][C], wpca1) given the window's
+// input x (rf_R holds it): sum (x - sum_s amp_s wave1_s)^2.  Spikes must be in rf_col/tmp/amp.
+lss_status refine_loss_at(Handle& h, const float* U1, const float* W1, const float* x, int n, double* loss) {
+    lss::RefineArgs a{};
+    a.X = x; a.ldx = h.ldC; a.vec_x = 1;
+    a.col = h.rf_col.p; a.tmp = h.rf_tmp.p; a.amp = h.rf_amp.p; a.n = n;
+    a.U = U1; a.wpca = W1; a.waves = h.rf_waves.p; a.R = h.rf_R2.p;
+    a.part = h.rf_part.p; a.loss = h.rf_loss.p; a.flag = h.rf_flag.p;
+    a.W = h.last_W; a.ldC = h.ldC; a.T = h.T; a.C = h.C; a.K = h.K; a.nt = h.nt;
+    HIP_TRY(lss::launch_waves(U1, (int64_t)h.K * h.C, W1, h.T, h.K, h.C, h.nt, h.rf_waves.p, h.ldC, h.stream));
+    HIP_TRY(lss::launch_refine_residual(a, h.stream));
+    HIP_TRY(hipMemcpyAsync(loss, h.rf_loss.p, sizeof(double), hipMemcpyDeviceToHost, h.stream));
+    HIP_TRY(hipStreamSynchronize(h.stream));
+    return LSS_OK;
+}
+
+lss_status lss_refine_templates(void* handle, double* info) {
+    if (!handle) return fail(LSS_ERR_ARG, "null handle");
+    Handle& h = *as_handle(handle);
+    if (info)
+        for (int i = 0; i < 4; ++i) info[i] = 0.0;
+    if (h.last_W < 1) return fail(LSS_ERR_STATE, "no window has been sorted yet");
+    HIP_TRY(hipMemcpyAsync(h.h_counters, h.counters.p, sizeof(int) * 8, hipMemcpyDeviceToHost, h.stream));
+    HIP_TRY(hipStreamSynchronize(h.stream));
+    const int n = std::min(h.h_counters[0] + h.h_counters[1], h.prm.max_spikes);
+    if (n < 1) return LSS_OK;
+    if (lss_status st = refine_alloc(h)) return st;
+    const int T = h.T, K = h.K, C = h.C, nt = h.nt;
+    const size_t nU = (size_t)T * K * C, nW = (size_t)K * nt;
+    if (!h.rf_U.p) {
+        HIP_TRY(h.rf_R2.alloc((size_t)h.Wmax * h.ldC));
+        HIP_TRY(h.rf_U.alloc(nU));
+        HIP_TRY(h.rf_U1.alloc(nU));
+        HIP_TRY(h.rf_gU.alloc(nU));
+        HIP_TRY(h.rf_gW.alloc(nW));
+        HIP_TRY(h.rf_W1.alloc(nW));
+        HIP_TRY(h.rf_col.alloc(h.prm.max_spikes));
+        HIP_TRY(h.rf_tmp.alloc(h.prm.max_spikes));
+        HIP_TRY(h.rf_amp.alloc(h.prm.max_spikes));
+        HIP_TRY(h.rf_sums.alloc(256 + 4));
+        HIP_TRY(hipHostMalloc((void**)&h.hs_U, sizeof(float) * nU, hipHostMallocDefault));
+    }
+    // the window's spikes sorted by column (the residual kernel's order)
+    std::vector<int> col(n), tmp(n), ord(n);
+    std::vector<float> amp(n), namp(n);
+    HIP_TRY(hipMemcpyAsync(col.data(), h.sp_col.p, sizeof(int) * n, hipMemcpyDeviceToHost, h.stream));
+    HIP_TRY(hipMemcpyAsync(tmp.data(), h.sp_tmp.p, sizeof(int) * n, hipMemcpyDeviceToHost, h.stream));
+    HIP_TRY(hipMemcpyAsync(amp.data(), h.sp_amp.p, sizeof(float) * n, hipMemcpyDeviceToHost, h.stream));
+    HIP_TRY(hipMemcpy2DAsync(h.rf_U.p, sizeof(float) * K * C, h.U.p, sizeof(float) * h.ldq, sizeof(float) * K * C, T,
+                             hipMemcpyDeviceToDevice, h.stream));
+    HIP_TRY(hipStreamSynchronize(h.stream));
+    for (int i = 0; i < n; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return col[x] < col[y]; });
+    {
+        std::vector<int> c2(n), t2(n);
+        std::vector<float> a2(n);
+        for (int i = 0; i < n; ++i) {
+            c2[i] = col[ord[i]];
+            t2[i] = tmp[ord[i]];
+            a2[i] = amp[ord[i]];
+            namp[i] = -a2[i];
+        }
+        col.swap(c2);
+        tmp.swap(t2);
+        amp.swap(a2);
+    }
+    HIP_TRY(hipMemcpyAsync(h.rf_col.p, col.data(), sizeof(int) * n, hipMemcpyHostToDevice, h.stream));
+    HIP_TRY(hipMemcpyAsync(h.rf_tmp.p, tmp.data(), sizeof(int) * n, hipMemcpyHostToDevice, h.stream));
+    // the window's input x = residual + sum_s amp_s wave_s (residual kernel with negated amplitudes)
+    HIP_TRY(hipMemcpyAsync(h.rf_amp.p, namp.data(), sizeof(float) * n, hipMemcpyHostToDevice, h.stream));
+    double* sums = h.rf_sums.p;   // [0..255] partials, [256] |g|^2
+    {
+        lss::RefineArgs a{};
+        a.X = h.xw.p; a.ldx = h.ldC; a.vec_x = 1;
+        a.col = h.rf_col.p; a.tmp = h.rf_tmp.p; a.amp = h.rf_amp.p; a.n = n;
+        a.U = h.rf_U.p; a.wpca = h.wpca.p; a.waves = h.rf_waves.p; a.R = h.rf_R.p;
+        a.part = h.rf_part.p; a.loss = h.rf_loss.p; a.flag = h.rf_flag.p;
+        a.W = h.last_W; a.ldC = h.ldC; a.T = T; a.C = C; a.K = K; a.nt = nt;
+        HIP_TRY(lss::launch_waves(h.rf_U.p, (int64_t)K * C, h.wpca.p, T, K, C, nt, h.rf_waves.p, h.ldC, h.stream));
+        HIP_TRY(lss::launch_refine_residual(a, h.stream));
+        // gradient at (U0, W0) from the residual the pursuit left (R = xw); any spike order works here
+        a.R = h.xw.p;
+        a.amp = h.sp_amp.p; a.col = h.sp_col.p; a.tmp = h.sp_tmp.p;
+        a.gU = h.rf_gU.p; a.gW = h.rf_gW.p; a.dwpart = h.rf_dwpart.p; a.dupart = h.rf_dupart.p;
+        HIP_TRY(lss::launch_refine_adjoint(a, h.stream));
+    }
+    HIP_TRY(hipMemcpyAsync(h.rf_amp.p, amp.data(), sizeof(float) * n, hipMemcpyHostToDevice, h.stream));
+    double g2u = 0.0, g2w = 0.0;
+    HIP_TRY(lss::launch_sumsq(h.rf_gU.p, (int64_t)nU, sums, sums + 256, h.stream));
+    HIP_TRY(lss::launch_sumsq(h.rf_gW.p, (int64_t)nW, sums, sums + 257, h.stream));
+    HIP_TRY(hipMemcpyAsync(&g2u, sums + 256, sizeof(double), hipMemcpyDeviceToHost, h.stream));
+    HIP_TRY(hipMemcpyAsync(&g2w, sums + 257, sizeof(double), hipMemcpyDeviceToHost, h.stream));
+    double L0 = 0.0;
+    if (lss_status st = refine_loss_at(h, h.rf_U.p, h.wpca.p, h.rf_R.p, n, &L0)) return st;   // synchronises
+    const double g2 = g2u + g2w;
+    if (!(g2 > 0.0) || !(L0 > 0.0)) return LSS_OK;
+    // quadratic fit of L(alpha) = L(U0 - alpha gU, W0 - alpha gW) through one trial step
+    auto step = [&](double al) -> lss_status {
+        HIP_TRY(lss::launch_step(h.rf_U1.p, h.rf_U.p, h.rf_gU.p, al, (int64_t)nU, h.stream));
+        HIP_TRY(lss::launch_step(h.rf_W1.p, h.wpca.p, h.rf_gW.p, al, (int64_t)nW, h.stream));
+        return LSS_OK;
+    };
+    const double a1 = 1e-4 * L0 / g2;
+    if (lss_status st = step(a1)) return st;
+    double L1 = 0.0;
+    if (lss_status st = refine_loss_at(h, h.rf_U1.p, h.rf_W1.p, h.rf_R.p, n, &L1)) return st;
+    const double q = (L1 - L0 + a1 * g2) / (a1 * a1);
+    const double al = (q > 0.0) ? std::min(g2 / (2.0 * q), 100.0 * a1) : a1;
+    if (lss_status st = step(al)) return st;
+    double L2 = 0.0;
+    if (lss_status st = refine_loss_at(h, h.rf_U1.p, h.rf_W1.p, h.rf_R.p, n, &L2)) return st;
+    if (!(L2 < L0)) return LSS_OK;   // no descent: templates unchanged
+    // adopt the step: U, wPCA, then every table derived from them
+    HIP_TRY(hipMemcpy2DAsync(h.U.p, sizeof(float) * h.ldq, h.rf_U1.p, sizeof(float) * K * C, sizeof(float) * K * C, T,
+                             hipMemcpyDeviceToDevice, h.stream));
+    HIP_TRY(hipMemcpyAsync(h.wpca.p, h.rf_W1.p, sizeof(float) * nW, hipMemcpyDeviceToDevice, h.stream));
+    HIP_TRY(hipMemcpyAsync(h.hs_U, h.rf_U1.p, sizeof(float) * nU, hipMemcpyDeviceToHost, h.stream));
+    HIP_TRY(hipStreamSynchronize(h.stream));
+    if (lss_status st = upload_extract(h, h.hs_U)) return st;
+    HIP_TRY(lss::launch_tables(h.U.p, h.ldq, h.wpca.p, T, K, C, nt, h.nm.p, h.waves.p, h.ldC, h.ctc.p, h.ldT,
+                               h.tables_ws.p, h.stream));
+    if (info) {
+        info[0] = L0;
+        info[1] = L2;
+        info[2] = al;
+        info[3] = std::sqrt(g2);
+    }
+    return LSS_OK;
 }
 
 lss_status lss_last_refine_timings(void* handle, float* ms) {
@@ -1231,9 +1386,18 @@ lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready) {
         if (st != LSS_OK) return st;
     }
     b.part_ms[4] = ms_since(t1);
+    ++b.batches;
+    if (b.refine_every > 0) {
+        // one probe per step, round-robin, starting every refine_every steps: the step that refines
+        // pays for one probe's refinement only
+        const int64_t k = b.batches % b.refine_every;
+        if (b.batches >= b.refine_every && k < (int64_t)b.pr.size()) {
+            b.refine_info.resize(b.pr.size());
+            if (lss_status st = lss_refine_templates(b.pr[k].h, b.refine_info[k].data())) return st;
+        }
+    }
     b.part_ms[5] = ms_since(t_start);
     b.last_ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t_start).count();
-    ++b.batches;
     b.t0 += b.batch;
     for (auto& p : b.pr) {
         // keep samples from t0 - nt on
@@ -1253,6 +1417,20 @@ lss_status lss_stream_stats(void* stream, float* last_ms, int64_t* batches) {
     Batcher& b = *static_cast<Batcher*>(stream);
     *last_ms = b.last_ms;
     *batches = b.batches;
+    return LSS_OK;
+}
+
+lss_status lss_stream_set_refine(void* stream, int32_t every_batches) {
+    if (!stream || every_batches < 0) return fail(LSS_ERR_ARG, "bad argument");
+    static_cast<Batcher*>(stream)->refine_every = every_batches;
+    return LSS_OK;
+}
+
+lss_status lss_stream_last_refine(void* stream, int32_t probe, double* info) {
+    if (!stream || !info) return fail(LSS_ERR_ARG, "null argument");
+    Batcher& b = *static_cast<Batcher*>(stream);
+    if (probe < 0 || probe >= (int32_t)b.pr.size()) return fail(LSS_ERR_ARG, "probe out of range");
+    for (int i = 0; i < 4; ++i) info[i] = (probe < (int32_t)b.refine_info.size()) ? b.refine_info[probe][i] : 0.0;
     return LSS_OK;
 }
 

@@ -151,6 +151,10 @@ struct RefineArgs {
 int refine_blocks(int64_t W);
 int refine_chunks(int nt);
 hipError_t launch_fill_i32(int* p, int n, int v, hipStream_t s);   // p[0..n) = v (capturable)
+// out = x - alpha * y elementwise (FP64 arithmetic, FP32 result)
+hipError_t launch_step(float* out, const float* x, const float* y, double alpha, int64_t n, hipStream_t s);
+// *out = sum of squares of x in FP64 (fixed order); part: 256 doubles of scratch
+hipError_t launch_sumsq(const float* x, int64_t n, double* part, double* out, hipStream_t s);
 // R and loss = sum R^2 (C <= 1024, nt <= 127)
 hipError_t launch_refine_residual(const RefineArgs& a, hipStream_t s);
 // gU, gW from R (after launch_refine_residual; K <= 8)
@@ -162,7 +166,9 @@ hipError_t launch_refine_adjoint(const RefineArgs& a, hipStream_t s);
 // waves[t][m][c] = sum_k U[t][k][c] wpca[k][m] (FP64 accumulation), rows of stride ldC
 hipError_t launch_waves(const float* U, int64_t ldu, const float* wpca, int T, int K, int C, int nt,
                         float* waves, int64_t ldC, hipStream_t s);
+// ws: device workspace of tables_workspace_doubles(T, K, nt) doubles (the Gram matrix and WtW).
+size_t tables_workspace_doubles(int T, int K, int nt);
 hipError_t launch_tables(const float* U, int64_t ldu, const float* wpca, int T, int K, int C, int nt,
-                         float* nm, float* waves, int64_t ldC, float* ctc, int64_t ldT, hipStream_t s);
+                         float* nm, float* waves, int64_t ldC, float* ctc, int64_t ldT, double* ws, hipStream_t s);
 
 }  // namespace lss

@@ -60,7 +60,8 @@ __global__ __launch_bounds__(256) void gram_kernel(const float* __restrict__ U, 
     if (r < R && q < R) G[(int64_t)r * R + q] = acc;
 }
 
-// ctc[j][l][i]: one thread per (i, j), all lags
+// ctc[j][l][i]: one thread per (i, j), all lags.  G is symmetric, so G[(i,k)][(j,m)] is read as
+// G[(j,m)][(i,k)]: neighbouring threads (consecutive i) then read neighbouring K-runs of one row.
 __global__ __launch_bounds__(256) void ctc_kernel(const double* __restrict__ G, const double* __restrict__ wtw,
                                                   int T, int K, int L, float* __restrict__ ctc, int64_t ldT) {
     extern __shared__ double swtw[];   // [K][K][L]
@@ -72,7 +73,7 @@ __global__ __launch_bounds__(256) void ctc_kernel(const double* __restrict__ G, 
     const int R = T * K;
     double g[8 * 8];
     for (int k = 0; k < K; ++k)
-        for (int m = 0; m < K; ++m) g[k * K + m] = G[(int64_t)(i * K + k) * R + (j * K + m)];
+        for (int m = 0; m < K; ++m) g[k * K + m] = G[(int64_t)(j * K + m) * R + (i * K + k)];
     for (int l = 0; l < L; ++l) {
         double acc = 0.0;
         for (int k = 0; k < K; ++k)
@@ -89,8 +90,12 @@ hipError_t launch_waves(const float* U, int64_t ldu, const float* wpca, int T, i
     return hipGetLastError();
 }
 
+size_t tables_workspace_doubles(int T, int K, int nt) {
+    return (size_t)T * K * T * K + (size_t)K * K * (2 * nt + 1);
+}
+
 hipError_t launch_tables(const float* U, int64_t ldu, const float* wpca, int T, int K, int C, int nt,
-                         float* nm, float* waves, int64_t ldC, float* ctc, int64_t ldT, hipStream_t s) {
+                         float* nm, float* waves, int64_t ldC, float* ctc, int64_t ldT, double* ws, hipStream_t s) {
     if (K > 8) return hipErrorInvalidValue;
     hipLaunchKernelGGL(nm_kernel, dim3(T), dim3(256), 0, s, U, ldu, K * C, nm);
     hipError_t e = hipGetLastError();
@@ -115,10 +120,8 @@ hipError_t launch_tables(const float* U, int64_t ldu, const float* wpca, int T, 
                 hwtw[(k * K + m) * L + l] = acc;
             }
     const int R = T * K;
-    double* dG = nullptr;
-    double* dW = nullptr;
-    if ((e = hipMalloc(&dG, sizeof(double) * (size_t)R * R)) != hipSuccess) { free(hw); free(hwtw); return e; }
-    if ((e = hipMalloc(&dW, sizeof(double) * K * K * L)) != hipSuccess) { (void)hipFree(dG); free(hw); free(hwtw); return e; }
+    double* dG = ws;
+    double* dW = ws + (size_t)R * R;
     e = hipMemcpyAsync(dW, hwtw, sizeof(double) * K * K * L, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) {
         hipLaunchKernelGGL(gram_kernel, dim3((R + 15) / 16, (R + 15) / 16), dim3(256), 0, s, U, ldu, K, R, C, dG);
@@ -133,8 +136,6 @@ hipError_t launch_tables(const float* U, int64_t ldu, const float* wpca, int T, 
         e = hipGetLastError();
     }
     hipError_t e2 = hipStreamSynchronize(s);
-    (void)hipFree(dG);
-    (void)hipFree(dW);
     free(hw);
     free(hwtw);
     return e != hipSuccess ? e : e2;

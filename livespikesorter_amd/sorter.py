@@ -237,6 +237,13 @@ class Sorter:
               "lss_residual_energy_grad")
         return loss.value, gU, gW
 
+    def refine_templates(self) -> dict:
+        """One refinement step of this sorter's templates on the last window it sorted (adopted only
+        if it lowers that window's residual energy); returns loss before/after, step, |gradient|."""
+        info = np.zeros(4, np.float64)
+        check(lib().lss_refine_templates(self._h, info.ctypes.data), "lss_refine_templates")
+        return dict(zip(["loss_before", "loss_after", "step", "grad_norm"], info.tolist()))
+
     def refine_timings(self):
         """Device ms of the last refinement call: residual (with waves and loss), adjoint, total."""
         ms = np.zeros(3, np.float32)

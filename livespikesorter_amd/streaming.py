@@ -13,7 +13,9 @@ from .sorter import Sorter, alloc_spikes
 
 
 class StreamingSorter:
-    def __init__(self, sorters: list[Sorter], batch: int = 3000, use_graph: bool = True):
+    def __init__(self, sorters: list[Sorter], batch: int = 3000, use_graph: bool = True, refine_every: int = 0):
+        """refine_every: one template-refinement step per probe every this many batches (drift
+        tracking, lss_refine_templates on that batch's window); 0 = never."""
         self.sorters = list(sorters)
         arr = (ct.c_void_p * len(self.sorters))(*[s._h.value for s in self.sorters])
         h = ct.c_void_p()
@@ -21,6 +23,8 @@ class StreamingSorter:
               "lss_stream_create")
         self._h = h
         self.batch = int(batch)
+        if refine_every:
+            check(lib().lss_stream_set_refine(h, int(refine_every)), "lss_stream_set_refine")
         arr_out = (Spikes * len(self.sorters))()
         self._bufs = []
         for i, s in enumerate(self.sorters):
@@ -51,6 +55,12 @@ class StreamingSorter:
         ms = np.zeros(6, np.float32)
         check(lib().lss_stream_breakdown(self._h, ms.ctypes.data), "lss_stream_breakdown")
         return dict(zip(["assemble", "segment_A", "extra_passes", "segment_C", "emit", "total"], ms.tolist()))
+
+    def last_refine(self, probe: int) -> dict:
+        """The probe's last refinement step: loss before/after, step length, gradient norm."""
+        info = np.zeros(4, np.float64)
+        check(lib().lss_stream_last_refine(self._h, int(probe), info.ctypes.data), "lss_stream_last_refine")
+        return dict(zip(["loss_before", "loss_after", "step", "grad_norm"], info.tolist()))
 
     def close(self):
         if getattr(self, "_h", None):

@@ -87,6 +87,20 @@ def test_scores_stage(sorter, synth1):
     assert rel_err(B, ref) <= 1e-5
 
 
+def test_tables_vs_fp64_oracle(sorter, synth1):
+    """Load-time tables built on the device in FP64 and stored as FP32: nm, template waves and ctc
+    against the oracle's FP64 template_norms / template_waves / prepare_matching
+    (template_matching.py:110-121, 141, 197); max |gpu - fp64| <= 1e-6 of max |fp64|."""
+    print("[lss] test_tables_vs_fp64_oracle", flush=True)
+    a = synth1.art
+    tb = sorter.tables()
+    U, W = torch.from_numpy(a["U"]).double(), torch.from_numpy(a["wPCA"]).double()
+    assert rel_err(tb["nm"], ko.template_norms(U).numpy()) <= 1e-6
+    assert rel_err(tb["waves"], ko.template_waves(U, W).numpy()) <= 1e-6
+    ctc = ko.prepare_matching(U, W).numpy()                               # [i][j][l]
+    assert rel_err(np.transpose(tb["ctc"], (2, 0, 1)), ctc) <= 1e-6
+
+
 def _pursuit_case(sorter, synth1, ib, max_passes=100):
     a = synth1.art
     _, X, ws = _window(synth1, ib)

@@ -24,15 +24,21 @@ DEV = "cuda:0"
 
 
 @pytest.fixture(scope="module")
-def case4():
+def drift_window():
     from livespikesorter_amd import synth
-    from livespikesorter_amd.refine import SpikeSet
-    from livespikesorter_amd.sorter import Sorter
     print("[lss] refine: generating the drift window (384 ch, 600 templates, 2 s)", flush=True)
     s = synth.make(4, seed=4)
     a = s.art
     X = ko.padded_batch(s.rec, 0, a["NT"], a["nt"], torch.float32)
     win = torch.from_numpy(np.ascontiguousarray(X.numpy().T.astype(np.int16))).to(DEV)
+    return a, win
+
+
+@pytest.fixture(scope="module")
+def case4(drift_window):
+    from livespikesorter_amd.refine import SpikeSet
+    from livespikesorter_amd.sorter import Sorter
+    a, win = drift_window
     sorter = Sorter(a, max_window=win.shape[0], max_passes=100, drop_negative=False)
     x_wc = sorter.stage_whiten(sorter.stage_filter(win)).contiguous()          # [W][C] float32
     sp = sorter.sort_window(win, ko.window_start(0, a["NT"], a["nt"]))
@@ -170,3 +176,34 @@ def test_refine_step_lowers_the_loss(case4):
     print(f"[lss]   loss {loss0:.6e} -> {loss1:.6e}; decrease {loss0 - loss1:.3f}, first order {pred:.3f}", flush=True)
     assert loss1 < loss0
     assert abs((loss0 - loss1) - pred) <= 0.1 * pred
+
+
+def test_refine_templates_step(drift_window, case4):
+    """lss_refine_templates on the sorter's own last window: the loss drops, the loss before equals
+    the refinement loss of that window's spikes, and the rebuilt tables are those of
+    U0 - step * dL/dU, wPCA0 - step * dL/dwPCA."""
+    print("[lss] test_refine_templates_step", flush=True)
+    from livespikesorter_amd.sorter import Sorter
+    a, win = drift_window
+    _, x_wc, spikes, U, wp = case4
+    s2 = Sorter(a, max_window=win.shape[0], max_passes=100, drop_negative=False)
+    try:
+        s2.sort_window(win, ko.window_start(0, a["NT"], a["nt"]))
+        info = s2.refine_templates()
+        print(f"[lss]   {info}", flush=True)
+        loss0, gU, gW = s2.residual_energy_grad(x_wc, spikes.col, spikes.tmp, spikes.amp, U, wp)
+        assert info["loss_after"] < info["loss_before"]
+        assert abs(info["loss_before"] - loss0) <= 1e-5 * loss0
+        U1 = (U.double() - info["step"] * gU.double()).float()
+        W1 = (wp.double() - info["step"] * gW.double()).float()
+        tb = s2.tables()
+        np.testing.assert_allclose(tb["nm"], ko.template_norms(U1.cpu().double()).numpy(), rtol=1e-4)
+        ctc1 = ko.prepare_matching(U1.cpu().double(), W1.cpu().double()).numpy()
+        assert np.abs(np.transpose(tb["ctc"], (2, 0, 1)) - ctc1).max() <= 1e-4 * np.abs(ctc1).max()
+        l1 = s2.residual_energy(x_wc, spikes.col, spikes.tmp, spikes.amp, U1.contiguous(), W1.contiguous())
+        assert abs(l1 - info["loss_after"]) <= 1e-4 * l1
+        # the refined sorter still sorts the window
+        sp2 = s2.sort_window(win, ko.window_start(0, a["NT"], a["nt"]))
+        assert len(sp2) > 0.9 * len(spikes)
+    finally:
+        s2.close()

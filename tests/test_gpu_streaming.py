@@ -114,3 +114,32 @@ def test_stream_vs_oracle(probes):
         assert f1 >= 0.999, (i, f1)
     for s in sorters:
         s.close()
+
+
+def test_stream_refinement_every_two_batches(probes):
+    """With refine_every=2 the first two batches are those of the plain stream, every probe's
+    refinement lowers its window's residual energy, and the stream keeps running."""
+    print("[lss] test_stream_refinement_every_two_batches", flush=True)
+    from livespikesorter_amd.streaming import StreamingSorter
+    plain = _stream(probes, _make_sorters(probes), use_graph=True)
+    sorters = _make_sorters(probes)
+    out = [[] for _ in probes]
+    with StreamingSorter(sorters, batch=BATCH, use_graph=True, refine_every=2) as st:
+        for i, p in enumerate(probes):
+            st.push(i, p.rec)
+        while True:
+            r = st.step()
+            if r is None:
+                break
+            for i in range(len(probes)):
+                out[i].append(r[i])
+        infos = [st.last_refine(i) for i in range(len(probes))]
+    print(f"[lss]   {infos}", flush=True)
+    for i in range(len(probes)):
+        assert len(out[i]) == len(plain[i])
+        for k in range(2):
+            np.testing.assert_array_equal(out[i][k].sample, plain[i][k].sample)
+            np.testing.assert_array_equal(out[i][k].amplitude, plain[i][k].amplitude)
+        assert infos[i]["loss_after"] < infos[i]["loss_before"]
+    for s in sorters:
+        s.close()
