@@ -84,7 +84,7 @@ def bench_stream(args):
     from livespikesorter_amd import synth
     from livespikesorter_amd.metrics import f1_same_template
     from livespikesorter_amd.sorter import Sorter
-    from livespikesorter_amd.streaming import StreamingSorter, autotune, autotune_persistent
+    from livespikesorter_amd.streaming import StreamingSorter, autotune, autotune_persistent, batcher_grid_shares
 
     batch, fs = 3000, 30000.0
     n = int(args.seconds * fs) + 2 * batch
@@ -105,12 +105,12 @@ def bench_stream(args):
                             max_spikes=1 << 15) for p in probes]
     refine_every = int(round(args.refine_seconds * fs / batch)) if args.refine_seconds > 0 else 0
     tuning = []
-    for s, p in zip(sorters, probes):
+    shares = batcher_grid_shares(sorters, batch + 2 * probes[0].art["nt"])
+    for s, p, cap in zip(sorters, probes, shares):
         w = torch.from_numpy(window(p.rec, 3, p.art["nt"])).to("cuda:0")
         g, p1, _ = autotune(s, w)
-        cap = s.persistent(w.shape[0])[2] // len(probes)   # the batcher's share per probe
         pg, pf, pms = autotune_persistent(s, w, max_grid=cap)
-        tuning.append({"update_grid": g, "first_passes": p1, "persistent_grid": pg, "persistent_after": pf,
+        tuning.append({"update_grid": g, "first_passes": p1, "grid_share": cap, "persistent_grid": pg, "persistent_after": pf,
                        "persistent_ms": {f"{k[0]}/{k[1]}": round(v, 3) for k, v in pms.items()}})
     nbat = (n - 2 * batch) // batch
     # plain per-probe path: the same windows one probe at a time, host windows

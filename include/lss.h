@@ -259,17 +259,19 @@ lss_status lss_get_tuning(void* handle, int32_t* update_grid, int32_t* first_pas
  * chip) as per-pass kernels and the rest in one launch of `grid` co-resident workgroups with two
  * grid barriers per pass, instead of three kernels per pass.  Results are identical.  grid = 0
  * selects the per-pass kernels throughout; grid is clamped at the number of workgroups the device
- * holds at once (max_grid from lss_get_persistent; the streaming batcher splits it between its
- * probes).  A barrier that waits past ~1 s (workgroups not co-resident) makes the call fail with
+ * holds at once (max_grid from lss_get_persistent; the streaming batcher gives probe i the
+ * fraction sqrt(T_i C_i) / sum_j sqrt(T_j C_j) of its limit, so that all probes fit at once).  A
+ * barrier that waits past ~1 s (workgroups not co-resident) makes the call fail with
  * LSS_ERR_STATE instead of hanging.  Defaults: grid 128, first_passes 0. */
 #define LSS_PERSIST_MAX_WINDOW 4096
 lss_status lss_set_persistent(void* handle, int32_t grid, int32_t first_passes);
 /* Current setting and the device's co-resident limit for windows of W samples (0 when W exceeds
  * LSS_PERSIST_MAX_WINDOW). */
 lss_status lss_get_persistent(void* handle, int64_t W, int32_t* grid, int32_t* first_passes, int32_t* max_grid);
-/* Times the window for grid in {0, 32, 64, 128, 256} x first_passes in {0, 1, 2, 4}: ms[4*gi + fi]
- * (20 floats, may be NULL; -1 = not run: above max_grid or the device limit, or grid 0 with
- * first_passes > 0; max_grid 0 = device limit) and keeps the fastest as the handle's setting. */
+/* Times the window for grid in {0, 64, 128, 256, 384} x first_passes in {0, 2, 4, P1} (P1 = the
+ * first-segment pass count lss_autotune chose, skipped when unset): ms[4*gi + fi]
+ * (20 floats, may be NULL; a grid above max_grid or the device limit runs once at that limit;
+ * -1 = not run; max_grid 0 = device limit) and keeps the fastest as the handle's setting. */
 lss_status lss_autotune_persistent(void* handle, const int16_t* d_window, int64_t W, int32_t reps,
                                    int32_t max_grid, float* ms, int32_t* best_grid, int32_t* best_first);
 

@@ -1290,13 +1290,15 @@ lss_status lss_stream_create(void* const* handles, int32_t n_probes, int32_t bat
             HIP_TRY(lss::persistent_max_grid(h.T, p.W, &p.max_grid));
         }
     }
-    // the probes' persistent kernels run concurrently: share the co-resident workgroups
-    int cap = 0;
-    for (auto& p : b->pr)
-        if (p.max_grid > 0) cap = cap > 0 ? std::min(cap, p.max_grid) : p.max_grid;
+    // the probes' persistent kernels run concurrently: probe i may use the fraction
+    // f_i = sqrt(T_i C_i) / sum_j sqrt(T_j C_j) of the workgroups the device holds of its kernel, so
+    // that the fractions of all probes add up to at most one device
+    double wsum = 0.0;
+    for (auto& p : b->pr) wsum += std::sqrt((double)p.h->T * p.h->C);
     b->single = true;
     for (auto& p : b->pr) {
-        if (p.max_grid > 0) p.max_grid = std::max(1, cap / n_probes);
+        if (p.max_grid > 0)
+            p.max_grid = std::max(1, (int)std::floor(p.max_grid * std::sqrt((double)p.h->T * p.h->C) / wsum));
         if (p.P1 < p.h->prm.max_passes) b->single = false;
     }
     *stream_out = b.release();
@@ -1484,8 +1486,10 @@ lss_status lss_autotune_persistent(void* handle, const int16_t* d_window, int64_
     int cap = 0;
     HIP_TRY(lss::persistent_max_grid(h.T, W, &cap));
     if (max_grid > 0) cap = std::min(cap, max_grid);
-    static const int kGrid[] = {0, 32, 64, 128, 256};
-    static const int kFirst[] = {0, 1, 2, 4};
+    static const int kGrid[] = {0, 64, 128, 256, 384};
+    // leading per-pass rounds: none, 2, 4, or the tuned first-segment pass count (lss_autotune),
+    // after which the persistent kernel usually finds the pursuit converged and exits at once
+    const int kFirst[] = {0, 2, 4, (h.tuned_passes > 0 && h.tuned_passes < h.prm.max_passes) ? h.tuned_passes : -1};
     lss_spikes out{};
     out.capacity = h.prm.max_spikes;
     float best = 1e30f;
@@ -1494,8 +1498,12 @@ lss_status lss_autotune_persistent(void* handle, const int16_t* d_window, int64_
     for (int gi = 0; gi < 5; ++gi)
         for (int fi = 0; fi < 4; ++fi) {
             if (ms) ms[gi * 4 + fi] = -1.f;
-            if (kGrid[gi] > cap || (kGrid[gi] == 0 && fi > 0) || kFirst[fi] >= h.prm.max_passes) continue;
-            h.pers_grid = kGrid[gi];
+            // candidates above the limit run at the limit once
+            const int g = std::min(kGrid[gi], cap);
+            if ((gi > 0 && g <= std::min(kGrid[gi - 1], cap)) || (g == 0 && fi > 0) || kFirst[fi] < 0 ||
+                kFirst[fi] >= h.prm.max_passes)
+                continue;
+            h.pers_grid = g;
             h.pers_first = kFirst[fi];
             float t = 1e30f;
             for (int r = 0; r <= reps; ++r) {
@@ -1509,7 +1517,7 @@ lss_status lss_autotune_persistent(void* handle, const int16_t* d_window, int64_
             if (ms) ms[gi * 4 + fi] = t;
             if (t < best) {
                 best = t;
-                bg = kGrid[gi];
+                bg = g;
                 bf = kFirst[fi];
             }
         }

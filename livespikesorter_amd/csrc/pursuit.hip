@@ -45,10 +45,30 @@ __device__ __forceinline__ float score_of(float b, float nmv) {
     return (r * r) / nmv;
 }
 
+// Device-scope (all XCDs) relaxed loads and stores: the only way workgroups of one launch of the
+// persistent pursuit exchange data, so that its grid barrier needs no cache write-back/invalidate.
+template <typename V>
+__device__ __forceinline__ V ld_dev(const V* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+template <typename V>
+__device__ __forceinline__ void st_dev(V* p, V v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+// Cfmax/imax of column w: plain stores between kernels, device-scope stores inside the persistent
+// pursuit (DEV)
+template <bool DEV>
+__device__ __forceinline__ void store_score(const PursuitArgs& a, int64_t w, float v, int i) {
+    if constexpr (DEV) {
+        st_dev(a.cfmax + w, v);
+        st_dev(a.imax + w, i);
+    } else {
+        a.cfmax[w] = v;
+        a.imax[w] = i;
+    }
+}
+
 // ------------------------------------------------------------------------------------------------
 // full scan (first pass): one wavefront per column, float4 loads (ldT % 4 == 0)
 // ------------------------------------------------------------------------------------------------
-template <int NV>   // float4 vectors per lane: NV*256 >= T
+template <int NV, bool DEV = false>   // float4 vectors per lane: NV*256 >= T
 __device__ __forceinline__ void score_column(const PursuitArgs& a, int64_t w, int lane) {
     float best = 0.f;
     int bi = 0;
@@ -74,10 +94,7 @@ __device__ __forceinline__ void score_column(const PursuitArgs& a, int64_t w, in
         }
         wave_argmax(best, bi);
     }
-    if (lane == 0) {
-        a.cfmax[w] = best;
-        a.imax[w] = bi;
-    }
+    if (lane == 0) store_score<DEV>(a, w, best, bi);
 }
 
 template <int NV>
@@ -254,17 +271,37 @@ __global__ __launch_bounds__(kEmitThreads) void peaks_emit_kernel(PursuitArgs a)
 constexpr int kColsB = 8;     // score columns per work item (2 per wavefront)
 constexpr int kUpdateGrid = 8192;   // workgroups per update launch (grid-stride over work items)
 
+// A pass's spike list (ascending columns): plain loads (a list stored by an earlier kernel, or in
+// LDS) or device-scope loads (a list another workgroup of the same launch stored).
+struct ListPlain {
+    const int* col;
+    const int* tmp;
+    const float* amp;
+    __device__ int c(int i) const { return col[i]; }
+    __device__ int t(int i) const { return tmp[i]; }
+    __device__ float a(int i) const { return amp[i]; }
+};
+struct ListDev {
+    const int* col;
+    const int* tmp;
+    const float* amp;
+    __device__ int c(int i) const { return ld_dev(col + i); }
+    __device__ int t(int i) const { return ld_dev(tmp + i); }
+    __device__ float a(int i) const { return ld_dev(amp + i); }
+};
+
 // Spikes of the pass covering column w within +-reach, starting at spike s (whose window covers
 // w): the last even-indexed (e) and odd-indexed (o) ones.  Spikes are in ascending column order,
 // so the covering ones are s, s+1, ...; c1, c2 are col[s+1], col[s+2] (INT_MAX past the end).
-__device__ __forceinline__ void covering(const int* col, int s, int n, int c1, int c2, int w, int reach,
+template <typename L>
+__device__ __forceinline__ void covering(const L& l, int s, int n, int c1, int c2, int w, int reach,
                                          int& e, int& o) {
     int last = s;
     if (c1 - reach <= w) {
         last = s + 1;
         if (c2 - reach <= w) {
             last = s + 2;
-            while (last + 1 < n && col[last + 1] - reach <= w) ++last;   // tied peaks only
+            while (last + 1 < n && l.c(last + 1) - reach <= w) ++last;   // tied peaks only
         }
     }
     if (last == s) {
@@ -277,25 +314,24 @@ __device__ __forceinline__ void covering(const int* col, int s, int n, int c1, i
     }
 }
 
-template <int NV>   // float4 vectors per lane over templates: NV*256 >= T
-__device__ __forceinline__ void update_score_columns(const PursuitArgs& a, const int* col, const int* tmp,
-                                                     const float* amp, int s, int n, int c1, int c2, int c0,
-                                                     int lo, int w1, const floatx4* nmv) {
+template <int NV, typename List, bool DEV>   // float4 vectors per lane over templates: NV*256 >= T
+__device__ __forceinline__ void update_score_columns(const PursuitArgs& a, const List& l, int s, int n, int c1, int c2,
+                                                     int c0, int lo, int w1, const floatx4* nmv) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nt = a.nt, L = 2 * nt + 1;
     // two columns per wavefront, loads of both issued before any use
     const int wa = c0 + 2 * wave, wb = wa + 1;
     const bool va = wa >= lo && wa <= w1, vb = wb >= lo && wb <= w1;
     int ea, oa, eb, ob;
-    covering(col, s, n, c1, c2, wa, nt, ea, oa);
-    covering(col, s, n, c1, c2, wb, nt, eb, ob);
+    covering(l, s, n, c1, c2, wa, nt, ea, oa);
+    covering(l, s, n, c1, c2, wb, nt, eb, ob);
     floatx4 ba[NV], bb[NV], cea[NV], coa[NV], ceb[NV], cob[NV];
     float* pa = a.B + (int64_t)wa * a.ldT;
     float* pb = a.B + (int64_t)wb * a.ldT;
-    const float* qea = (ea >= 0) ? a.ctc + ((int64_t)tmp[ea] * L + (wa - col[ea] + nt)) * a.ldT : nullptr;
-    const float* qoa = (oa >= 0) ? a.ctc + ((int64_t)tmp[oa] * L + (wa - col[oa] + nt)) * a.ldT : nullptr;
-    const float* qeb = (eb >= 0) ? a.ctc + ((int64_t)tmp[eb] * L + (wb - col[eb] + nt)) * a.ldT : nullptr;
-    const float* qob = (ob >= 0) ? a.ctc + ((int64_t)tmp[ob] * L + (wb - col[ob] + nt)) * a.ldT : nullptr;
+    const float* qea = (ea >= 0) ? a.ctc + ((int64_t)l.t(ea) * L + (wa - l.c(ea) + nt)) * a.ldT : nullptr;
+    const float* qoa = (oa >= 0) ? a.ctc + ((int64_t)l.t(oa) * L + (wa - l.c(oa) + nt)) * a.ldT : nullptr;
+    const float* qeb = (eb >= 0) ? a.ctc + ((int64_t)l.t(eb) * L + (wb - l.c(eb) + nt)) * a.ldT : nullptr;
+    const float* qob = (ob >= 0) ? a.ctc + ((int64_t)l.t(ob) * L + (wb - l.c(ob) + nt)) * a.ldT : nullptr;
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
         const int t = 4 * lane + 256 * i;
@@ -312,8 +348,8 @@ __device__ __forceinline__ void update_score_columns(const PursuitArgs& a, const
             }
         }
     }
-    const float aea = (ea >= 0) ? amp[ea] : 0.f, aoa = (oa >= 0) ? amp[oa] : 0.f;
-    const float aeb = (eb >= 0) ? amp[eb] : 0.f, aob = (ob >= 0) ? amp[ob] : 0.f;
+    const float aea = (ea >= 0) ? l.a(ea) : 0.f, aoa = (oa >= 0) ? l.a(oa) : 0.f;
+    const float aeb = (eb >= 0) ? l.a(eb) : 0.f, aob = (ob >= 0) ? l.a(ob) : 0.f;
     const bool ra = va && wa >= nt && wa < a.W - nt;
     const bool rb = vb && wb >= nt && wb < a.W - nt;
     float besta = -1.f, bestb = -1.f;
@@ -350,11 +386,11 @@ __device__ __forceinline__ void update_score_columns(const PursuitArgs& a, const
     }
     if (ra) {
         wave_argmax(besta, ia);
-        if (lane == 0) { a.cfmax[wa] = besta; a.imax[wa] = ia; }
+        if (lane == 0) store_score<DEV>(a, wa, besta, ia);
     }
     if (rb) {
         wave_argmax(bestb, ib);
-        if (lane == 0) { a.cfmax[wb] = bestb; a.imax[wb] = ib; }
+        if (lane == 0) store_score<DEV>(a, wb, bestb, ib);
     }
 }
 
@@ -371,24 +407,52 @@ __device__ __forceinline__ void load_norms(const PursuitArgs& a, floatx4* nmv) {
 // Work items q = q0, q0 + stride, ... of a pass's n spikes (col/tmp/amp, ascending columns; global
 // or LDS memory).  A column is written only by the item of the first spike whose window covers it.
 template <int NV>
-__device__ __forceinline__ void update_items(const PursuitArgs& a, const int* col, const int* tmp, const float* amp,
-                                             int n, int64_t q0, int64_t stride, const floatx4* nmv) {
+__device__ __forceinline__ void update_items(const PursuitArgs& a, const ListPlain& l, int n, int64_t q0,
+                                             int64_t stride, const floatx4* nmv) {
     const int nt = a.nt;
     const int nB = (2 * nt + 1 + kColsB - 1) / kColsB;
     for (int64_t q = q0; q < (int64_t)n * nB; q += stride) {
         const int s = (int)(q / nB), sub = (int)(q % nB);
         // the spike's column and its neighbours' (independent loads, issued together)
-        const int ws = col[s];
-        const int cp = (s > 0) ? col[s - 1] : INT_MIN / 2;
-        const int c1 = (s + 1 < n) ? col[s + 1] : INT_MAX / 2;
-        const int c2 = (s + 2 < n) ? col[s + 2] : INT_MAX / 2;
+        const int ws = l.c(s);
+        const int cp = (s > 0) ? l.c(s - 1) : INT_MIN / 2;
+        const int c1 = (s + 1 < n) ? l.c(s + 1) : INT_MAX / 2;
+        const int c2 = (s + 2 < n) ? l.c(s + 2) : INT_MAX / 2;
         int lo = ws - nt;
         if (s > 0) lo = max(lo, cp + nt + 1);
         lo = max(lo, 0);
         const int hi = (int)min((int64_t)ws + nt, a.W - 1);
         const int c0 = ws - nt + sub * kColsB;
         const int w1 = min(c0 + kColsB - 1, hi);
-        if (max(c0, lo) <= w1) update_score_columns<NV>(a, col, tmp, amp, s, n, c1, c2, c0, lo, w1, nmv);
+        if (max(c0, lo) <= w1) update_score_columns<NV, ListPlain, false>(a, l, s, n, c1, c2, c0, lo, w1, nmv);
+    }
+}
+
+// The columns [a0, a1) a workgroup owns: for each spike whose columns (those its window covers and
+// the previous spike's does not) meet them, that intersection in chunks of kColsB.  The same
+// columns are written as by update_items, by the same arithmetic.
+template <int NV, typename List>
+__device__ __forceinline__ void update_owned(const PursuitArgs& a, const List& l, int n, int a0, int a1,
+                                             const floatx4* nmv) {
+    const int nt = a.nt;
+    int lo_s = 0, hi_s = n;   // first spike with col + nt >= a0
+    while (lo_s < hi_s) {
+        const int mid = (lo_s + hi_s) >> 1;
+        if (l.c(mid) + nt < a0) lo_s = mid + 1;
+        else hi_s = mid;
+    }
+    for (int s = lo_s; s < n; ++s) {
+        const int ws = l.c(s);
+        if (ws - nt >= a1) break;
+        const int cp = (s > 0) ? l.c(s - 1) : INT_MIN / 2;
+        const int c1 = (s + 1 < n) ? l.c(s + 1) : INT_MAX / 2;
+        const int c2 = (s + 2 < n) ? l.c(s + 2) : INT_MAX / 2;
+        int lo = ws - nt;
+        if (s > 0) lo = max(lo, cp + nt + 1);
+        lo = max(max(lo, 0), a0);
+        const int hi = min(min(ws + nt, (int)a.W - 1), a1 - 1);
+        for (int c0 = lo; c0 <= hi; c0 += kColsB)
+            update_score_columns<NV, List, true>(a, l, s, n, c1, c2, c0, c0, min(c0 + kColsB - 1, hi), nmv);
     }
 }
 
@@ -400,21 +464,26 @@ __global__ __launch_bounds__(256) void update_kernel(PursuitArgs a) {
     if (base + n > a.cap) n = a.cap - base;
     floatx4 nmv[NV];
     load_norms<NV>(a, nmv);
-    update_items<NV>(a, a.sp_col + base, a.sp_tmp + base, a.sp_amp + base, n, blockIdx.x, gridDim.x, nmv);
+    update_items<NV>(a, ListPlain{a.sp_col + base, a.sp_tmp + base, a.sp_amp + base}, n, blockIdx.x, gridDim.x, nmv);
 }
 
 // ------------------------------------------------------------------------------------------------
 // persistent pursuit for short windows (W <= kPersistMaxW, the streaming batches): the first
 // scoring and every pass in one launch of G co-resident workgroups, two grid barriers per pass.
 //
-// After the first barrier every workgroup loads the whole Cfmax/imax row into LDS, forms the exact
-// window max by doubling (window_max_pow2), applies the reference's test and compacts in ascending
-// order as peaks_flags/peaks_emit do, so each holds the pass's spike list without exchanging it; workgroup 0 also stores it (sp_*, pass_off, counters) for the residual and
-// feature kernels.  The second barrier keeps every read of Cfmax/imax/B of the pass ahead of the
-// update, which rewrites them (the update of a spike zeroes its own score).  The update is
-// update_kernel's, over the LDS list (or over the stored list for a pass of more than
-// kPersistList spikes).  Every workgroup sees the same scores, so all take the same branches and
-// leave the loop on the same pass.
+// Workgroup g owns the columns [a0, a1) (a fixed partition): it alone scores them, reads their B
+// rows and updates them, so B stays in the caches of one XCD and needs no coherence traffic.  What
+// workgroups exchange - Cfmax/imax, the pass's amplitudes and stored spike list - goes through
+// device-scope loads and stores only, so the grid barrier is a plain arrival count with no cache
+// write-back or invalidate.
+//
+// After barrier A every workgroup loads the whole Cfmax/imax row into LDS, forms the exact window
+// max by doubling (window_max_pow2), applies the reference's test and compacts in ascending order
+// as peaks_flags/peaks_emit do, so each holds the pass's columns and templates without exchanging
+// them; the owner of a peak's column publishes its amplitude, workgroup 0 stores the list.  Barrier
+// B keeps every read of the pass ahead of the update (the update of a spike zeroes its own score)
+// and makes the amplitudes visible.  Every workgroup sees the same scores, so all take the same
+// branches and leave the loop on the same pass.
 //
 // The barrier waits a bounded time: on a timeout (workgroups not co-resident) the kernel sets
 // counters[7], every workgroup leaves, and the host reports the error; the launcher caps G at the
@@ -425,30 +494,30 @@ constexpr int kBarrierSpin = 1 << 20;   // ~1 s of polling
 constexpr int kPersistLoads = (kPersistMaxW + 2 * kPeakHalo + 255) / 256;   // row values per thread
 constexpr int kPersistMx = 256 * kPersistLoads + kPeakHalo;                   // window-max buffer
 
-// Grid barrier number `gen` (1, 2, ...) over counters[6] (zero at launch).  Every wavefront waits
-// for its own memory operations, so that one device-scope release by thread 0 (after the
-// workgroup barrier) publishes the whole workgroup's stores; every wavefront then acquires.
+// Grid barrier number `gen` (1, 2, ...) over counters[6] (zero at launch).  Data shared between
+// workgroups is only accessed with device-scope atomics, so waiting for this wavefront's memory
+// operations before the workgroup meets orders them before the arrival; no cache maintenance.
 // Returns false after a timeout or when another workgroup has timed out.
 __device__ __forceinline__ bool grid_barrier(int* counters, int gen, int* s_ok) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         atomicAdd(counters + 6, 1);
         const int target = gen * (int)gridDim.x;
         int ok = 1;
-        for (int it = 0; __hip_atomic_load(counters + 6, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target; ++it) {
-            if (it >= kBarrierSpin || __hip_atomic_load(counters + 7, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        for (int it = 0; ld_dev(counters + 6) < target; ++it) {
+            if (it >= kBarrierSpin || ld_dev(counters + 7)) {
                 atomicExch(counters + 7, 1);
                 ok = 0;
                 break;
             }
-            __builtin_amdgcn_s_sleep(4);
+            __builtin_amdgcn_s_sleep(2);
         }
         *s_ok = ok;
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     return *s_ok != 0;
 }
 
@@ -467,13 +536,16 @@ __global__ __launch_bounds__(256) void pursuit_persistent_kernel(PursuitArgs a, 
     int* ltmp = lcol + kPersistList;
     float* lamp = (float*)(ltmp + kPersistList);
     const bool lead = blockIdx.x == 0 && tid == 0;
+    // this workgroup's columns: kColsB-aligned runs, G of them cover [0, W)
+    const int Wc = (W + (int)gridDim.x * kColsB - 1) / ((int)gridDim.x * kColsB) * kColsB;
+    const int a0 = min(W, (int)blockIdx.x * Wc), a1 = min(W, a0 + Wc);
     floatx4 nmv[NV];
     load_norms<NV>(a, nmv);
     int base = 0, np = 0;
     bool done = false;
     if (first_pass == 0) {
-        // first pass: every column scored, one wavefront per column
-        for (int w = blockIdx.x * 4 + wave; w < W; w += gridDim.x * 4) score_column<NV>(a, w, lane);
+        // first pass: own columns scored, one wavefront per column
+        for (int w = a0 + wave; w < a1; w += 4) score_column<NV, true>(a, w, lane);
     } else {
         // after first_pass passes of the per-pass kernels: commit the last one as peaks_emit does
         base = a.counters[0] + a.counters[1];
@@ -493,8 +565,8 @@ __global__ __launch_bounds__(256) void pursuit_persistent_kernel(PursuitArgs a, 
             for (int r = 0; r < kPersistLoads; ++r) {
                 const int w = tid + 256 * r - H;
                 const bool in = w >= 0 && w < W;
-                v[r] = in ? a.cfmax[w] : -INFINITY;
-                m[r] = in ? a.imax[w] : 0;
+                v[r] = in ? ld_dev(a.cfmax + w) : -INFINITY;
+                m[r] = in ? ld_dev(a.imax + w) : 0;
             }
 #pragma unroll
             for (int r = 0; r < kPersistLoads; ++r) {
@@ -535,12 +607,13 @@ __global__ __launch_bounds__(256) void pursuit_persistent_kernel(PursuitArgs a, 
             bits &= bits - 1;
             const int w = tid * R + q;
             const int t = im[w];
-            const float am = a.B[(int64_t)w * a.ldT + t] / a.nm[t];
-            if (pos < kPersistList) { lcol[pos] = w; ltmp[pos] = t; lamp[pos] = am; }
-            if (blockIdx.x == 0 && base + pos < a.cap) {
-                a.sp_col[base + pos] = w;
-                a.sp_tmp[base + pos] = t;
-                a.sp_amp[base + pos] = am;
+            if (pos < kPersistList) { lcol[pos] = w; ltmp[pos] = t; }
+            if (base + pos < a.cap) {
+                if (w >= a0 && w < a1) st_dev(a.sp_amp + base + pos, a.B[(int64_t)w * a.ldT + t] / a.nm[t]);
+                if (blockIdx.x == 0) {
+                    st_dev(a.sp_col + base + pos, w);
+                    st_dev(a.sp_tmp + base + pos, t);
+                }
             }
             ++pos;
         }
@@ -554,14 +627,17 @@ __global__ __launch_bounds__(256) void pursuit_persistent_kernel(PursuitArgs a, 
             a.pass_off[np + 1] = min(base + n, a.cap);
         }
         ++np;
-        // every workgroup has read this pass's Cfmax/imax/B before any update writes them (and
-        // workgroup 0's stored list is complete)
+        // every workgroup has read this pass's Cfmax/imax before any update writes them, and the
+        // amplitudes and the stored list are complete
         if (!grid_barrier(a.counters, gen++, &s_ok)) return;
         const int nu = min(n, a.cap - base);   // spikes stored, as update_kernel applies them
-        if (nu > kPersistList)
-            update_items<NV>(a, a.sp_col + base, a.sp_tmp + base, a.sp_amp + base, nu, blockIdx.x, gridDim.x, nmv);
-        else
-            update_items<NV>(a, lcol, ltmp, lamp, nu, blockIdx.x, gridDim.x, nmv);
+        if (nu > kPersistList) {
+            update_owned<NV>(a, ListDev{a.sp_col + base, a.sp_tmp + base, a.sp_amp + base}, nu, a0, a1, nmv);
+        } else {
+            for (int i = tid; i < nu; i += 256) lamp[i] = ld_dev(a.sp_amp + base + i);
+            __syncthreads();
+            update_owned<NV>(a, ListPlain{lcol, ltmp, lamp}, nu, a0, a1, nmv);
+        }
         base += n;
         if (pass + 1 < max_passes && !grid_barrier(a.counters, gen++, &s_ok)) return;
     }

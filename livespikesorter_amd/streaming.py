@@ -90,14 +90,15 @@ def autotune(sorter: Sorter, window_dev, reps: int = 2):
     return g.value, p.value, dict(zip([256, 512, 1024, 2048, 4096, 8192], ms.tolist()))
 
 
-PERSIST_GRIDS = [0, 32, 64, 128, 256]
-PERSIST_FIRST = [0, 1, 2, 4]
+PERSIST_GRIDS = [0, 64, 128, 256, 384]
+PERSIST_FIRST = [0, 2, 4, "P1"]   # P1: the first-segment pass count autotune() chose
 
 
 def autotune_persistent(sorter: Sorter, window_dev, reps: int = 2, max_grid: int = 0):
     """Picks the pursuit schedule of a short window (<= 4096 samples): per-pass kernels (grid 0) or
-    the persistent kernel at 32..256 workgroups (at most max_grid; 0 = the device limit) after 0, 1,
-    2 or 4 leading per-pass rounds.  Returns (grid, first_passes, {(grid, first): ms}) over the
+    the persistent kernel at 64..384 workgroups (at most max_grid; 0 = the device limit) after 0, 2,
+    4 or P1 leading per-pass rounds (P1 from autotune(); after it the persistent kernel usually finds
+    the pursuit converged, which makes a streaming step one graph).  Returns (grid, first_passes, {(grid, first): ms}) over the
     candidates that ran."""
     ms = np.zeros(20, np.float32)
     g, f = ct.c_int32(), ct.c_int32()
@@ -107,3 +108,11 @@ def autotune_persistent(sorter: Sorter, window_dev, reps: int = 2, max_grid: int
     table = {(gg, ff): float(ms[4 * i + j]) for i, gg in enumerate(PERSIST_GRIDS)
              for j, ff in enumerate(PERSIST_FIRST) if ms[4 * i + j] >= 0}
     return g.value, f.value, table
+
+
+def batcher_grid_shares(sorters: list[Sorter], W: int) -> list[int]:
+    """The persistent-pursuit workgroups the batcher allows each probe (lss_stream_create's split:
+    probe i gets sqrt(T_i C_i) / sum_j sqrt(T_j C_j) of the device's limit for its kernel)."""
+    w = [float(np.sqrt(s.T * s.C)) for s in sorters]
+    tot = sum(w)
+    return [max(1, int(np.floor(s.persistent(W)[2] * wi / tot))) for s, wi in zip(sorters, w)]
