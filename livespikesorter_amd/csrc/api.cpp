@@ -359,7 +359,7 @@ lss_status create_impl(const lss_params* p, const lss_artifacts* a, Handle** out
     HIP_TRY(h->sp_col.alloc(p->max_spikes));
     HIP_TRY(h->sp_tmp.alloc(p->max_spikes));
     HIP_TRY(h->sp_amp.alloc(p->max_spikes));
-    HIP_TRY(h->counters.alloc(8));
+    HIP_TRY(h->counters.alloc(lss::kCounters));
     HIP_TRY(h->pass_off.alloc(p->max_passes + 2));
     HIP_TRY(h->tF.alloc((size_t)p->max_spikes * nC * K));
     HIP_TRY(h->xs.alloc(p->max_spikes));

@@ -50,14 +50,15 @@ struct PursuitArgs {
     const float* waves;    // [T][nt][ldC]
     float* cfmax;          // [W]
     int* imax;             // [W]
-    unsigned char* flags;  // [roundup(W, 65536)] peak flags of the current pass
+    unsigned char* flags;  // peak flag words of the current pass (uint32 [ceil(W/32)])
     int* sp_col;           // [cap] peak column
     int* sp_tmp;           // [cap] template
     float* sp_amp;         // [cap]
     int* pass_off;         // [max_passes + 1] first spike of each pass that found spikes
     // [0] spikes of committed passes, [1] spikes of the last pass, [2] capacity overflow,
     // [3] converged (a pass found nothing), [4] passes that found spikes, [5] residual list overflow,
-    // [6] grid-barrier arrivals and [7] grid-barrier timeout of the persistent pursuit
+    // [6] grid-barrier arrivals and [7] grid-barrier timeout of the persistent pursuit, [8] the
+    // peak kernel's workgroup ticket (kCounters ints)
     int* counters;
     int64_t W;
     int64_t ldT, ldC;
@@ -65,7 +66,8 @@ struct PursuitArgs {
     int cap;
     float th2;
 };
-hipError_t launch_reset_counters(int* counters, hipStream_t s);        // counters[0..7] = 0
+constexpr int kCounters = 16;
+hipError_t launch_reset_counters(int* counters, hipStream_t s);        // counters[0..kCounters) = 0
 hipError_t launch_score_all(const PursuitArgs& a, hipStream_t s);     // T <= 1024
 hipError_t launch_peaks(const PursuitArgs& a, hipStream_t s);
 // grid = workgroups (grid-stride over the pass's work items; any grid gives the same result)

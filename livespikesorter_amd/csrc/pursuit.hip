@@ -110,14 +110,10 @@ __global__ __launch_bounds__(256) void score_all_kernel(PursuitArgs a) {
 // column with cf <= Th^2 - 2e-9 cannot reach pool > Th^2 with pool - cf < 1e-9 and is skipped.
 // Columns that survive get the literal test on the full window maximum.
 //
-// peaks_flags_kernel: one thread per column over the whole chip -> flag bytes.
-// peaks_emit_kernel: one workgroup compacts the flags in ascending column order.
+// peaks_kernel: flags over the whole chip, compaction by the last workgroup (below).
 // ------------------------------------------------------------------------------------------------
-constexpr int kFlagCols = 256;
+constexpr int kFlagCols = 1024;
 constexpr int kPeakHalo = 128;                       // >= nt
-constexpr int kEmitThreads = 1024;
-constexpr int kEmitPer = 64;                         // columns per thread per emit tile
-constexpr int kEmitTile = kEmitThreads * kEmitPer;   // 65536
 
 // p points at the column's Cfmax in a buffer holding its +-nt neighbours (-inf outside the window)
 __device__ __forceinline__ bool peak_test(const float* p, float th2, int nt) {
@@ -165,27 +161,14 @@ __device__ __forceinline__ bool peak_pooled(const float* cf, const float* mx, in
     return pool > th2 && fabsf(pool - cf[j]) < 1e-9f;
 }
 
-__global__ __launch_bounds__(kFlagCols) void peaks_flags_kernel(PursuitArgs a) {
-    __shared__ float lds[kFlagCols + 2 * kPeakHalo];
-    if (a.counters[3]) return;   // converged earlier: nothing can change
-    const int tid = threadIdx.x;
-    const int64_t c0 = (int64_t)blockIdx.x * kFlagCols;
-    float v[(kFlagCols + 2 * kPeakHalo) / kFlagCols];
-#pragma unroll
-    for (int r = 0; r < (kFlagCols + 2 * kPeakHalo) / kFlagCols; ++r) {
-        const int64_t w = c0 - kPeakHalo + tid + r * kFlagCols;
-        v[r] = (w >= 0 && w < a.W) ? a.cfmax[w] : -INFINITY;
-    }
-#pragma unroll
-    for (int r = 0; r < (kFlagCols + 2 * kPeakHalo) / kFlagCols; ++r) lds[tid + r * kFlagCols] = v[r];
-    __syncthreads();
-    const int64_t w = c0 + tid;
-    if (w >= a.W) return;
-    a.flags[w] = peak_test(lds + kPeakHalo + tid, a.th2, a.nt) ? 1 : 0;
-}
+// peaks_kernel: one thread per column over the whole chip; each wavefront publishes its 64 peak
+// flags as two 32-bit words (device-scope stores), takes a ticket, and the last workgroup to finish
+// compacts the flag words in ascending column order (one launch per pass, no waiting).
+constexpr int kEmitPer = 64;                          // columns per thread per compaction tile
+constexpr int kEmitTile = kFlagCols * kEmitPer;       // 65536
 
-__global__ __launch_bounds__(kEmitThreads) void peaks_emit_kernel(PursuitArgs a) {
-    __shared__ int wsum[kEmitThreads / 64];
+__device__ __forceinline__ void emit_pass(const PursuitArgs& a, const uint32_t* flagw) {
+    __shared__ int wsum[kFlagCols / 64];
     __shared__ int s_base, s_run;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (tid == 0) {
@@ -197,27 +180,12 @@ __global__ __launch_bounds__(kEmitThreads) void peaks_emit_kernel(PursuitArgs a)
         s_run = 0;
     }
     __syncthreads();
-    if (a.counters[3]) return;
+    const int64_t nwords = (a.W + 31) / 32;
     for (int64_t c0 = 0; c0 < a.W; c0 += kEmitTile) {
-        // kEmitPer flag bytes per thread as four 16-byte loads (flags has kEmitTile padding)
-        const uint4* fp = (const uint4*)(a.flags + c0 + (int64_t)tid * kEmitPer);
-        uint4 q[kEmitPer / 16];
-#pragma unroll
-        for (int j = 0; j < kEmitPer / 16; ++j) q[j] = fp[j];
-        uint64_t bits = 0;
-#pragma unroll
-        for (int j = 0; j < kEmitPer / 16; ++j) {
-            const uint32_t wv[4] = {q[j].x, q[j].y, q[j].z, q[j].w};
-#pragma unroll
-            for (int k = 0; k < 4; ++k)
-#pragma unroll
-                for (int b = 0; b < 4; ++b)
-                    if ((wv[k] >> (8 * b)) & 0xffu) bits |= 1ull << (16 * j + 4 * k + b);
-        }
-        // bytes past W may hold flags of an earlier, longer window
-        const int64_t valid = a.W - (c0 + (int64_t)tid * kEmitPer);
-        if (valid <= 0) bits = 0;
-        else if (valid < 64) bits &= (1ull << valid) - 1;
+        const int64_t wd = (c0 >> 5) + 2 * tid;
+        const uint32_t lo = (wd < nwords) ? ld_dev(flagw + wd) : 0u;
+        const uint32_t hi = (wd + 1 < nwords) ? ld_dev(flagw + wd + 1) : 0u;
+        uint64_t bits = ((uint64_t)hi << 32) | lo;
         const int cnt = __popcll(bits);
         int inc = cnt;
 #pragma unroll
@@ -228,7 +196,8 @@ __global__ __launch_bounds__(kEmitThreads) void peaks_emit_kernel(PursuitArgs a)
         if (lane == 63) wsum[wave] = inc;
         __syncthreads();
         int woff = 0, tot = 0;
-        for (int k = 0; k < kEmitThreads / 64; ++k) {
+#pragma unroll
+        for (int k = 0; k < kFlagCols / 64; ++k) {
             const int vv = wsum[k];
             if (k < wave) woff += vv;
             tot += vv;
@@ -263,6 +232,43 @@ __global__ __launch_bounds__(kEmitThreads) void peaks_emit_kernel(PursuitArgs a)
             a.counters[4] = p + 1;
         }
     }
+}
+
+__global__ __launch_bounds__(kFlagCols) void peaks_kernel(PursuitArgs a) {
+    __shared__ float lds[kFlagCols + 2 * kPeakHalo];
+    __shared__ int s_last;
+    if (a.counters[3]) return;   // converged earlier: nothing can change (no workgroup has ticketed)
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    uint32_t* flagw = (uint32_t*)a.flags;
+    const int64_t c0 = (int64_t)blockIdx.x * kFlagCols;
+    constexpr int kRows = (kFlagCols + 2 * kPeakHalo + kFlagCols - 1) / kFlagCols;
+    float v[kRows];
+#pragma unroll
+    for (int r = 0; r < kRows; ++r) {
+        const int64_t w = c0 - kPeakHalo + tid + r * kFlagCols;
+        v[r] = (w >= 0 && w < a.W) ? a.cfmax[w] : -INFINITY;
+    }
+#pragma unroll
+    for (int r = 0; r < kRows; ++r)
+        if (tid + r * kFlagCols < kFlagCols + 2 * kPeakHalo) lds[tid + r * kFlagCols] = v[r];
+    __syncthreads();
+    const int64_t w = c0 + tid;
+    const bool pk = w < a.W && peak_test(lds + kPeakHalo + tid, a.th2, a.nt);
+    const uint64_t bal = __ballot(pk);
+    if (lane == 0) {
+        st_dev(flagw + (c0 >> 5) + 2 * wave, (uint32_t)bal);
+        st_dev(flagw + (c0 >> 5) + 2 * wave + 1, (uint32_t)(bal >> 32));
+    }
+    // ticket after this workgroup's words are stored; the last workgroup sees every word
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (tid == 0) s_last = atomicAdd(a.counters + 8, 1) == (int)gridDim.x - 1;
+    __syncthreads();
+    if (!s_last) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    if (tid == 0) a.counters[8] = 0;   // ticket of the next pass
+    emit_pass(a, flagw);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -788,10 +794,7 @@ hipError_t launch_score_all(const PursuitArgs& a, hipStream_t s) {
 hipError_t launch_peaks(const PursuitArgs& a, hipStream_t s) {
     if (a.nt > kPeakHalo) return hipErrorInvalidValue;
     const unsigned nb = (unsigned)((a.W + kFlagCols - 1) / kFlagCols);
-    hipLaunchKernelGGL(peaks_flags_kernel, dim3(nb), dim3(kFlagCols), 0, s, a);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(peaks_emit_kernel, dim3(1), dim3(kEmitThreads), 0, s, a);
+    hipLaunchKernelGGL(peaks_kernel, dim3(nb), dim3(kFlagCols), 0, s, a);
     return hipGetLastError();
 }
 
@@ -966,7 +969,7 @@ hipError_t launch_stage_spikes(const StageArgs& a, hipStream_t s) {
 
 // counters[0..7] = 0 (a fill kernel, so that the sequence can be captured in a graph)
 __global__ void reset_counters_kernel(int* c) {
-    if (threadIdx.x < 8) c[threadIdx.x] = 0;
+    if (threadIdx.x < kCounters) c[threadIdx.x] = 0;
 }
 
 hipError_t launch_reset_counters(int* counters, hipStream_t s) {
