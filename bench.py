@@ -113,37 +113,61 @@ def bench_stream(args):
         tuning.append({"update_grid": g, "first_passes": p1, "grid_share": cap, "persistent_grid": pg, "persistent_after": pf,
                        "persistent_ms": {f"{k[0]}/{k[1]}": round(v, 3) for k, v in pms.items()}})
     nbat = (n - 2 * batch) // batch
-    # plain per-probe path: the same windows one probe at a time, host windows
+    dev = torch.device("cuda:0")
+    drecs = [torch.from_numpy(p.rec).to(dev) for p in probes]     # recordings resident in HBM
+    w0 = [torch.from_numpy(window(p.rec, 0, p.art["nt"])).to(dev) for p in probes]
+
+    def dwindow(i, k):
+        nt = probes[i].art["nt"]
+        return w0[i] if k == 0 else drecs[i][k * batch - nt:(k + 1) * batch + nt]
+
+    # plain per-probe path: the same windows one probe at a time (device-resident windows)
     plain = []
+    torch.cuda.synchronize()
     t_plain = time.perf_counter()
     for k in range(nbat):
-        plain.append([s.sort_window(window(p.rec, k, p.art["nt"]), k * batch - p.art["nt"])
-                      for s, p in zip(plain_sorters, probes)])
+        plain.append([s.sort_window(dwindow(i, k), k * batch - p.art["nt"])
+                      for i, (s, p) in enumerate(zip(plain_sorters, probes))])
     t_plain = time.perf_counter() - t_plain
-    # streaming batcher: samples pushed in 100 ms chunks, one step per batch
-    lat, outs, parts = [], [], []
-    refines = []
-    with StreamingSorter(sorters, batch=batch, use_graph=True, refine_every=refine_every) as st:
-        for i, p in enumerate(probes):
-            st.push(i, p.rec[:batch + 64])
-        pos = batch + 64
-        t_s = time.perf_counter()
-        k = 0
-        while k < nbat:
-            for i, p in enumerate(probes):
-                st.push(i, p.rec[pos:pos + batch])
-            pos += batch
-            t1 = time.perf_counter()
-            r = st.step()
-            if r is None:
-                continue
-            lat.append(time.perf_counter() - t1)
-            outs.append(r)
-            parts.append(st.breakdown())
-            k += 1
-            if refine_every and k % refine_every == 0:
-                refines.append([st.last_refine(i) for i in range(len(probes))])
-        t_s = time.perf_counter() - t_s
+
+    def run_stream(srt, device_input):
+        """One pass over the recordings: samples pushed 100 ms at a time, one step per batch."""
+        lat, outs, parts, refines = [], [], [], []
+        src = drecs if device_input else [p.rec for p in probes]
+        with StreamingSorter(srt, batch=batch, use_graph=True, refine_every=refine_every) as st:
+            for i in range(len(probes)):
+                st.push(i, src[i][:batch + 64])
+            pos = batch + 64
+            torch.cuda.synchronize()
+            t_s = time.perf_counter()
+            k = 0
+            while k < nbat:
+                for i in range(len(probes)):
+                    st.push(i, src[i][pos:pos + batch])
+                pos += batch
+                t1 = time.perf_counter()
+                r = st.step()
+                if r is None:
+                    continue
+                lat.append(time.perf_counter() - t1)
+                outs.append(r)
+                parts.append(st.breakdown())
+                k += 1
+                if refine_every and k % refine_every == 0:
+                    refines.append([st.last_refine(i) for i in range(len(probes))])
+            t_s = time.perf_counter() - t_s
+        return lat, outs, parts, refines, t_s
+
+    # the measured run: recordings resident in HBM, windows assembled device to device
+    lat, outs, parts, refines, t_s = run_stream(sorters, True)
+    # the same stream with host samples (each batch crosses PCIe): fresh sorters, same tuning
+    host_sorters = [Sorter(p.art, max_window=batch + 2 * p.art["nt"], max_passes=100, precision=args.precision,
+                           max_spikes=1 << 15) for p in probes]
+    for s2, tu in zip(host_sorters, tuning):
+        s2.set_persistent(tu["persistent_grid"], tu["persistent_after"])
+        from livespikesorter_amd._lib import lib as _l
+        _l().lss_set_tuning(s2._h, tu["update_grid"], tu["first_passes"])
+    lat_h, _, _, _, t_h = run_stream(host_sorters, False)
     lat_ms = np.array(lat) * 1e3
     # spike F1 per probe, pooled over batches (samples are recording samples, disjoint per batch)
     def pooled_f1(k0, k1):
@@ -164,13 +188,17 @@ def bench_stream(args):
         "scaling": "weak", "vs_baseline": None, "dtype": args.precision,
         "data": "synthetic (seeded probes, templates and spikes; no recordings)",
         "config": {"workload": "streaming, 4 probes, batch 3000 + 61-sample halos, one graph-replayed step per batch, "
-                               "template refinement every refine_seconds",
+                               "template refinement every refine_seconds; recordings resident in HBM (device samples)",
                    "probes": STREAM_PROBES, "batch": batch, "seconds_per_probe": round(nbat * batch / fs, 2)},
         "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
         "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
         "max_ms": round(float(lat_ms.max()), 3),
         "plain_per_probe_x_realtime": round(data_s / t_plain, 2),
         "speedup_vs_plain": round(t_plain / t_s, 3),
+        "host_input": {"x_realtime": round(data_s / t_h, 2),
+                       "p50_ms": round(float(np.percentile(np.array(lat_h) * 1e3, 50)), 3),
+                       "p99_ms": round(float(np.percentile(np.array(lat_h) * 1e3, 99)), 3),
+                       "note": "samples pushed from host memory, each window uploaded over PCIe in the step"},
         "spike_f1_vs_plain_fp32_before_refinement": round(float(f1), 6),
         "spike_f1_vs_plain_fp32_all_batches": round(float(f1_all), 6),
         "f1_note": "per probe, pooled over batches, +-1 sample and same template; the minimum over probes",

@@ -196,6 +196,11 @@ lss_status lss_stream_create(void* const* handles, int32_t n_probes, int32_t bat
 lss_status lss_stream_destroy(void* stream);
 /* Appends n samples (int16 [n][C] interleaved, host memory) of one probe. */
 lss_status lss_stream_push(void* stream, int32_t probe, const int16_t* samples, int64_t n);
+/* Appends n samples of one probe that are already in device memory (int16 [n][C], this GPU): the
+ * step copies its window rows device to device and its graph has no upload.  The caller keeps the
+ * buffer valid until every step that reads its samples has returned.  A probe takes either host
+ * or device samples. */
+lss_status lss_stream_push_device(void* stream, int32_t probe, const int16_t* d_samples, int64_t n);
 /* Runs one batch when every probe holds enough samples (*ready = 1) and fills outs[0..n_probes);
  * otherwise *ready = 0 and nothing runs. */
 lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready);

@@ -26,7 +26,7 @@ EXPORTS = ["lss_last_error", "lss_abi_version", "lss_default_params", "lss_creat
            "lss_stream_stats", "lss_stream_breakdown", "lss_autotune", "lss_set_tuning", "lss_get_tuning",
            "lss_set_persistent", "lss_get_persistent", "lss_autotune_persistent",
            "lss_residual_energy", "lss_residual_energy_grad", "lss_last_refine_timings",
-           "lss_refine_templates", "lss_stream_set_refine", "lss_stream_last_refine"]
+           "lss_refine_templates", "lss_stream_set_refine", "lss_stream_last_refine", "lss_stream_push_device"]
 
 
 class Params(ct.Structure):
@@ -91,6 +91,7 @@ def lib() -> ct.CDLL:
             "lss_stream_create": ([ct.POINTER(vp), i32, i32, i32, ct.POINTER(vp)], i32),
             "lss_stream_destroy": ([vp], i32),
             "lss_stream_push": ([vp, i32, vp, i64], i32),
+            "lss_stream_push_device": ([vp, i32, vp, i64], i32),
             "lss_stream_step": ([vp, ct.POINTER(Spikes), ct.POINTER(i32)], i32),
             "lss_stream_stats": ([vp, ct.POINTER(ct.c_float), ct.POINTER(i64)], i32),
             "lss_stream_breakdown": ([vp, vp], i32),

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
 #include <fstream>
 #include <map>
 #include <memory>
@@ -620,7 +621,10 @@ lss_status emit_spikes(Handle& h, int n, int passes, int64_t window_start, lss_s
 // ------------------------------------------------------------------------------------------------
 struct Probe {
     Handle* h = nullptr;
-    std::vector<int16_t> buf;     // received samples [base, base + rows) as [row][C]
+    int mode = 0;                 // 0 nothing pushed yet, 1 host samples, 2 device samples
+    std::vector<int16_t> buf;     // host mode: received samples [base, base + rows) as [row][C]
+    std::deque<std::pair<const int16_t*, int64_t>> dchunks;   // device mode: (pointer, first sample)
+    std::deque<int64_t> dlen;     // rows of each device chunk
     int64_t base = 0;
     int64_t recv = 0;
     int16_t* pin = nullptr;       // pinned window [W][C]
@@ -663,7 +667,8 @@ struct Batcher {
 // a persistent pursuit); the counters go to the host only when extra passes may follow.
 lss_status probe_A(Probe& p) {
     Handle& h = *p.h;
-    HIP_TRY(hipMemcpyAsync(h.win.p, p.pin, sizeof(int16_t) * p.W * h.C, hipMemcpyHostToDevice, h.stream));
+    if (p.mode == 1)
+        HIP_TRY(hipMemcpyAsync(h.win.p, p.pin, sizeof(int16_t) * p.W * h.C, hipMemcpyHostToDevice, h.stream));
     if (lss_status st = enqueue_pre(h, h.win.p, p.W)) return st;
     const lss::PursuitArgs a = pursuit_args(h, h.B.p, h.xw.p, p.W);
     if (lss_status st = enqueue_pursuit(h, a, p.P1, p.max_grid)) return st;
@@ -726,6 +731,30 @@ lss_status capture(Batcher& b, lss_status (*seg)(Batcher&), hipGraph_t* g, hipGr
 
 // Assembles probe p's window for batch start t0 into its pinned staging buffer: one contiguous
 // copy, after the rows that repeat sample 0 when the window starts before the recording.
+// Device mode: the window's rows (samples t0 - nt .. t0 + batch + nt - 1, sample 0 repeated
+// before the recording starts) copied device to device into the handle's window buffer, on the
+// stream the step's graph is launched on.
+lss_status assemble_dev(Probe& p, int64_t t0, hipStream_t s) {
+    Handle& h = *p.h;
+    const int C = h.C, nt = h.nt;
+    const size_t row = sizeof(int16_t) * C;
+    int64_t r = 0;
+    for (; r < p.W && t0 - nt + r < 0; ++r)
+        HIP_TRY(hipMemcpyAsync(h.win.p + r * C, p.dchunks.front().first, row, hipMemcpyDeviceToDevice, s));
+    for (size_t k = 0; k < p.dchunks.size() && r < p.W; ++k) {
+        const int64_t c0 = p.dchunks[k].second, c1 = c0 + p.dlen[k];
+        const int64_t need = t0 - nt + r;   // next sample to place
+        if (c1 <= need) continue;
+        const int64_t from = std::max(need, c0);
+        const int64_t rows = std::min(c1 - from, p.W - r);
+        HIP_TRY(hipMemcpyAsync(h.win.p + r * C, p.dchunks[k].first + (from - c0) * C, row * rows,
+                               hipMemcpyDeviceToDevice, s));
+        r += rows;
+    }
+    if (r != p.W) return fail(LSS_ERR_STATE, "device window incomplete");
+    return LSS_OK;
+}
+
 void assemble(Probe& p, int64_t t0) {
     Handle& h = *p.h;
     const int C = h.C, nt = h.nt;
@@ -1318,8 +1347,25 @@ lss_status lss_stream_push(void* stream, int32_t probe, const int16_t* samples, 
     Batcher& b = *static_cast<Batcher*>(stream);
     if (probe < 0 || probe >= (int32_t)b.pr.size()) return fail(LSS_ERR_ARG, "probe out of range");
     Probe& p = b.pr[probe];
+    if (p.mode == 2) return fail(LSS_ERR_STATE, "this probe receives device samples");
+    p.mode = 1;
     const int C = p.h->C;
     p.buf.insert(p.buf.end(), samples, samples + n * C);
+    p.recv += n;
+    return LSS_OK;
+}
+
+lss_status lss_stream_push_device(void* stream, int32_t probe, const int16_t* d_samples, int64_t n) {
+    if (!stream || (n > 0 && !d_samples) || n < 0) return fail(LSS_ERR_ARG, "bad argument");
+    Batcher& b = *static_cast<Batcher*>(stream);
+    if (probe < 0 || probe >= (int32_t)b.pr.size()) return fail(LSS_ERR_ARG, "probe out of range");
+    Probe& p = b.pr[probe];
+    if (p.mode == 1) return fail(LSS_ERR_STATE, "this probe receives host samples");
+    if (b.gA && p.mode == 0) return fail(LSS_ERR_STATE, "the step graph was captured for host samples");
+    p.mode = 2;
+    if (n == 0) return LSS_OK;
+    p.dchunks.emplace_back(d_samples, p.recv);
+    p.dlen.push_back(n);
     p.recv += n;
     return LSS_OK;
 }
@@ -1333,7 +1379,13 @@ lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready) {
     using clk = std::chrono::steady_clock;
     auto ms_since = [](clk::time_point t) { return std::chrono::duration<float, std::milli>(clk::now() - t).count(); };
     auto t_start = clk::now();
-    for (auto& p : b.pr) assemble(p, b.t0);
+    for (auto& p : b.pr) {
+        if (p.mode == 1) {
+            assemble(p, b.t0);
+        } else if (lss_status st = assemble_dev(p, b.t0, b.main)) {
+            return st;
+        }
+    }
     b.part_ms[0] = ms_since(t_start);
     auto t1 = clk::now();
     // segment A
@@ -1405,7 +1457,13 @@ lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready) {
         // keep samples from t0 - nt on
         const int64_t keep_from = std::max<int64_t>(0, b.t0 - p.h->nt);
         const int64_t drop = keep_from - p.base;
-        if (drop > 0) {
+        if (p.mode == 2) {
+            while (!p.dchunks.empty() && p.dchunks.front().second + p.dlen.front() <= keep_from) {
+                p.dchunks.pop_front();
+                p.dlen.pop_front();
+            }
+            p.base = keep_from;
+        } else if (drop > 0) {
             p.buf.erase(p.buf.begin(), p.buf.begin() + drop * p.h->C);
             p.base = keep_from;
         }

@@ -23,6 +23,7 @@ class StreamingSorter:
               "lss_stream_create")
         self._h = h
         self.batch = int(batch)
+        self._held = {}
         if refine_every:
             check(lib().lss_stream_set_refine(h, int(refine_every)), "lss_stream_set_refine")
         arr_out = (Spikes * len(self.sorters))()
@@ -33,7 +34,17 @@ class StreamingSorter:
             self._bufs.append(bufs)
         self._arr_out = arr_out
 
-    def push(self, probe: int, samples: np.ndarray):
+    def push(self, probe: int, samples):
+        """Host samples (numpy int16 [n][C]) are copied in; device samples (a contiguous torch int16
+        tensor on this GPU) are referenced: this object keeps the tensor alive, and the stream must
+        keep receiving device samples for that probe."""
+        if hasattr(samples, "data_ptr"):
+            if not samples.is_contiguous():
+                raise ValueError("device samples must be contiguous")
+            self._held.setdefault(probe, []).append(samples)
+            check(lib().lss_stream_push_device(self._h, int(probe), samples.data_ptr(), samples.shape[0]),
+                  "lss_stream_push_device")
+            return
         x = np.ascontiguousarray(samples, np.int16)
         check(lib().lss_stream_push(self._h, int(probe), x.ctypes.data, x.shape[0]), "lss_stream_push")
 

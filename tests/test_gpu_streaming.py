@@ -143,3 +143,38 @@ def test_stream_refinement_every_two_batches(probes):
         assert infos[i]["loss_after"] < infos[i]["loss_before"]
     for s in sorters:
         s.close()
+
+
+def test_stream_device_samples_match_host_samples(probes):
+    """Samples pushed as device buffers (windows assembled device to device, no upload in the
+    graph) give exactly the spikes of the same stream pushed from host memory."""
+    print("[lss] test_stream_device_samples_match_host_samples", flush=True)
+    from livespikesorter_amd.streaming import StreamingSorter
+    host = _stream(probes, _make_sorters(probes), use_graph=True, chunk_seed=5)
+    sorters = _make_sorters(probes)
+    drec = [torch.from_numpy(p.rec).to("cuda:0") for p in probes]
+    rng = np.random.default_rng(5)
+    out = [[] for _ in probes]
+    with StreamingSorter(sorters, batch=BATCH, use_graph=True) as st:
+        n = probes[0].rec.shape[0]
+        pos = 0
+        while pos < n:
+            k = int(rng.integers(500, 2500))
+            for i in range(len(probes)):
+                st.push(i, drec[i][pos:pos + k])
+            pos += k
+            while True:
+                r = st.step()
+                if r is None:
+                    break
+                for i in range(len(probes)):
+                    out[i].append(r[i])
+    for i in range(len(probes)):
+        assert len(out[i]) == len(host[i])
+        for a, b in zip(out[i], host[i]):
+            np.testing.assert_array_equal(a.sample, b.sample)
+            np.testing.assert_array_equal(a.template, b.template)
+            np.testing.assert_array_equal(a.amplitude, b.amplitude)
+            np.testing.assert_array_equal(a.features, b.features)
+    for s in sorters:
+        s.close()
