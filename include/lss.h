@@ -286,6 +286,67 @@ lss_status lss_autotune_persistent(void* handle, const int16_t* d_window, int64_
 lss_status lss_profile_update(void* handle, int32_t enable);
 lss_status lss_last_update_stats(void* handle, float* total_ms, int32_t* launches);
 
+/* ---- per-batch drift correction ----------------------------------------------------------------
+ * Replaces the handle's whitening operator with (M @ Wrot), M a device [C][C] FP64 matrix (row
+ * stride C, from lss_drift_matrix), the product taken in FP64 on the handle's stream and stored as
+ * FP32; NULL restores Wrot alone.  Kilosort applies the batch's drift this way
+ * (kilosort/io.py:781-787: M = get_drift_matrix(ops, dshift[ibatch]); X = (M @ Wrot) @ X); LSS
+ * loads one fixed M (drift_matrix.npy, src/SpikeSorter/OnlineSpikesV2.cu:575-582).  Ordered with
+ * later windows of the handle's stream; in streaming mode call it between lss_stream_step calls. */
+lss_status lss_set_drift(void* handle, const double* d_m);
+
+/* ---- training-side artifacts in FP64 -------------------------------------------------------------
+ * Builders of the Kilosort4 artifacts the sorter loads, on the GPU in FP64.  Synchronous: each
+ * call allocates its scratch, runs on hip_stream (NULL: the null stream), waits and frees.
+ * d_* arguments are device memory of the current device; xc, yc, yblk, dshift are host arrays.
+ *
+ * Covariance of one filtered batch, accumulated: cc[i][j] += sum_t x[i][t] x[j][t] / n over the
+ * n columns of the [C][ldx] rows x (the loop body of get_whitening_matrix,
+ * kilosort/preprocessing.py:96-113, which then divides by the number of batches).  Products and
+ * sums in FP64, summation order fixed; cc comes out exactly symmetric. */
+lss_status lss_covariance_add(const float* d_x, int64_t ldx, int32_t C, int64_t n, double* d_cc, int64_t ldcc,
+                              void* hip_stream);
+/* ZCA whitening W = E diag(1/sqrt(d + eps)) E^T of an n x n covariance, n <= 96
+ * (whitening_from_covariance, kilosort/preprocessing.py:8-17; eps = 1e-6 there).  The
+ * eigendecomposition is a parallel-order Jacobi in FP64; d enters as |d|, the singular values the
+ * reference's SVD gives. */
+lss_status lss_whitening_from_covariance(const double* d_cc, int64_t ldcc, int32_t n, double eps, double* d_w,
+                                         int64_t ldw, void* hip_stream);
+/* Local whitening (whitening_local, kilosort/preprocessing.py:19-35): for channel j the
+ * min(nrange, C) <= 96 channels nearest to it (squared distance, ties by channel index), the ZCA
+ * of cc over them, and row j of W = that ZCA's first row on those channels, zero elsewhere. */
+lss_status lss_whitening_local(const double* d_cc, int64_t ldcc, int32_t C, const double* xc, const double* yc,
+                               int32_t nrange, double eps, double* d_w, int64_t ldw, void* hip_stream);
+/* iKxx = inv(Kxx + lam I), Kxx[i][j] = exp(-|p_i - p_j|^2 / (2 sig^2)) over the sites
+ * p = (xc, yc) (kilosort/datashift.py:176-184, 215-219; lam = 0.01 there), by Newton-Schulz
+ * iteration X <- X + X (I - A X) from X = I / ||A||_inf on the FP64 GEMM, stopped when the
+ * residual ||I - A X||_F stops shrinking; iterations (may be NULL) gets the count. */
+lss_status lss_drift_inverse(const double* xc, const double* yc, int32_t C, double sig_interp, double lam,
+                             double* d_ikxx, int64_t ldk, int32_t* iterations, void* hip_stream);
+/* Drift interpolation matrix M = Kyx @ iKxx (get_drift_matrix, kilosort/preprocessing.py:44-69):
+ * the sites shifted down by the per-channel shift, which is dshift[0] for nblocks == 1 and
+ * otherwise the linear interpolation (extrapolated at both ends) of dshift[b] over yblk[b] at yc. */
+lss_status lss_drift_matrix(const double* xc, const double* yc, int32_t C, const double* yblk, const double* dshift,
+                            int32_t nblocks, double sig_interp, const double* d_ikxx, int64_t ldk, double* d_m,
+                            int64_t ldm, void* hip_stream);
+/* Isolated single-channel peaks and their snippets (extract_snippets,
+ * kilosort/spikedetect.py:32-47) of a whitened time-major window x [W][ldx]: a peak is a sample
+ * whose |x| is the maximum over channels c +- loc_range[0] and samples t +- loc_range[1] and
+ * exceeds th_single_ch; it is isolated when it is the only peak over c +- long_range[0],
+ * t +- long_range[1]; samples t < nt and t >= W - nt are excluded.  Snippet i (of peak (c_i, t_i),
+ * in channel-then-sample order) is clips[i][m] = x[t_i - twav_min + m][c_i], m < nt; the first
+ * max_clips are written (chan, time may be NULL) and n_found gets the total.  loc_range <= 8. */
+lss_status lss_extract_snippets(const float* d_x_wc, int64_t ldx, int32_t C, int64_t W, int32_t nt, int32_t twav_min,
+                                float th_single_ch, const int32_t* loc_range, const int32_t* long_range,
+                                float* d_clips, int64_t max_clips, int32_t* d_chan, int32_t* d_time, int64_t* n_found,
+                                void* hip_stream);
+/* wPCA [n_pcs][nt] (FP64) and singular values [n_pcs]: the leading right singular vectors of the
+ * row-normalised snippets [n][nt] (TruncatedSVD(n_pcs).fit(clips / ||clips||) of
+ * extract_wPCA_wTEMP, kilosort/spikedetect.py:50-72), each signed so that its largest-magnitude
+ * entry is positive (the sign rule TruncatedSVD applies); nt <= 96. */
+lss_status lss_wpca(const float* d_clips, int64_t n, int32_t nt, int32_t n_pcs, double* d_wpca, double* d_sv,
+                    void* hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

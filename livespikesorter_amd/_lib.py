@@ -26,7 +26,9 @@ EXPORTS = ["lss_last_error", "lss_abi_version", "lss_default_params", "lss_creat
            "lss_stream_stats", "lss_stream_breakdown", "lss_autotune", "lss_set_tuning", "lss_get_tuning",
            "lss_set_persistent", "lss_get_persistent", "lss_autotune_persistent",
            "lss_residual_energy", "lss_residual_energy_grad", "lss_last_refine_timings",
-           "lss_refine_templates", "lss_stream_set_refine", "lss_stream_last_refine", "lss_stream_push_device"]
+           "lss_refine_templates", "lss_stream_set_refine", "lss_stream_last_refine", "lss_stream_push_device",
+           "lss_set_drift", "lss_covariance_add", "lss_whitening_from_covariance", "lss_whitening_local",
+           "lss_drift_inverse", "lss_drift_matrix", "lss_extract_snippets", "lss_wpca"]
 
 
 class Params(ct.Structure):
@@ -63,7 +65,7 @@ def lib() -> ct.CDLL:
         if not LIB_PATH.exists():
             raise LssError(f"{LIB_PATH} is not built: run `python -m livespikesorter_amd.build`")
         L = ct.CDLL(str(LIB_PATH))
-        vp, i32, i64 = ct.c_void_p, ct.c_int32, ct.c_int64
+        vp, i32, i64, f64 = ct.c_void_p, ct.c_int32, ct.c_int64, ct.c_double
         sig = {
             "lss_last_error": ([], ct.c_char_p),
             "lss_abi_version": ([], i32),
@@ -107,6 +109,15 @@ def lib() -> ct.CDLL:
             "lss_refine_templates": ([vp, vp], i32),
             "lss_stream_set_refine": ([vp, i32], i32),
             "lss_stream_last_refine": ([vp, i32, vp], i32),
+            "lss_set_drift": ([vp, vp], i32),
+            "lss_covariance_add": ([vp, i64, i32, i64, vp, i64, vp], i32),
+            "lss_whitening_from_covariance": ([vp, i64, i32, f64, vp, i64, vp], i32),
+            "lss_whitening_local": ([vp, i64, i32, vp, vp, i32, f64, vp, i64, vp], i32),
+            "lss_drift_inverse": ([vp, vp, i32, f64, f64, vp, i64, ct.POINTER(i32), vp], i32),
+            "lss_drift_matrix": ([vp, vp, i32, vp, vp, i32, f64, vp, i64, vp, i64, vp], i32),
+            "lss_extract_snippets": ([vp, i64, i32, i64, i32, i32, ct.c_float, vp, vp, vp, i64, vp, vp,
+                                      ct.POINTER(i64), vp], i32),
+            "lss_wpca": ([vp, i64, i32, i32, vp, vp, vp], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

@@ -69,6 +69,8 @@ struct Handle {
     // tables
     DevBuf<float> U, wpca, wmat, nm, waves, ctc, Ucc, xc, yc;
     DevBuf<double> tables_ws;   // Gram matrix + WtW of the FP64 table build (kept for refinement)
+    DevBuf<double> wrot64;      // [C][C] Wrot in FP64 (per-batch drift: wmat = M @ Wrot)
+    DevBuf<double> drift_ws;    // split-K partials and the FP64 product of lss_set_drift
     DevBuf<int> iCC, iU;
     DevBuf<float2> spec, tw;
     std::vector<float> hp_taps;
@@ -320,6 +322,9 @@ lss_status create_impl(const lss_params* p, const lss_artifacts* a, Handle** out
         HIP_TRY(hipMemset(h->wmat.p, 0, sizeof(float) * C * h->ldm));
         HIP_TRY(hipMemcpy2D(h->wmat.p, sizeof(float) * h->ldm, wm.data(), sizeof(float) * C, sizeof(float) * C, C,
                             hipMemcpyHostToDevice));
+        std::vector<double> w64(a->Wrot, a->Wrot + (size_t)C * C);
+        HIP_TRY(h->wrot64.alloc((size_t)C * C));
+        HIP_TRY(hipMemcpy(h->wrot64.p, w64.data(), sizeof(double) * C * C, hipMemcpyHostToDevice));
     }
     // twiddles exp(-2 pi i m / N)
     {
@@ -1253,6 +1258,32 @@ lss_status lss_get_tables(void* handle, float* nm, float* waves, float* ctc, flo
         HIP_TRY(hipMemcpy2DAsync(wmat, sizeof(float) * h.C, h.wmat.p, sizeof(float) * h.ldm, sizeof(float) * h.C, h.C,
                                  hipMemcpyDeviceToHost, h.stream));
     HIP_TRY(hipStreamSynchronize(h.stream));
+    return LSS_OK;
+}
+
+lss_status lss_set_drift(void* handle, const double* d_m) {
+    if (!handle) return fail(LSS_ERR_ARG, "null handle");
+    Handle& h = *as_handle(handle);
+    const int C = h.C;
+    if (!d_m) {
+        HIP_TRY(lss::launch_f64_to_f32(h.wrot64.p, C, C, C, h.wmat.p, h.ldm, h.stream));
+        return LSS_OK;
+    }
+    int S;
+    int64_t kchunk;
+    lss::gemm64_split(C, C, C, 0, &S, &kchunk);
+    const size_t need = (size_t)(S + 1) * C * C;
+    if (h.drift_ws.n < need) {
+        HIP_TRY(hipStreamSynchronize(h.stream));
+        if (h.drift_ws.p) HIP_TRY(hipFree(h.drift_ws.p));
+        h.drift_ws.p = nullptr;
+        HIP_TRY(h.drift_ws.alloc(need));
+    }
+    double* P = h.drift_ws.p;
+    double* wm = P + (size_t)S * C * C;
+    HIP_TRY(lss::launch_gemm64_kn(d_m, C, h.wrot64.p, C, P, C, C, C, S, kchunk, h.stream));
+    HIP_TRY(lss::launch_reduce64(P, S, C, C, 0, 1.0, 0.0, 0.0, wm, C, h.stream));
+    HIP_TRY(lss::launch_f64_to_f32(wm, C, C, C, h.wmat.p, h.ldm, h.stream));
     return LSS_OK;
 }
 

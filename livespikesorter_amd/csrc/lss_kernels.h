@@ -173,4 +173,43 @@ size_t tables_workspace_doubles(int T, int K, int nt);
 hipError_t launch_tables(const float* U, int64_t ldu, const float* wpca, int T, int K, int C, int nt,
                          float* nm, float* waves, int64_t ldC, float* ctc, int64_t ldT, double* ws, hipStream_t s);
 
+// ---- training-side artifacts in FP64 (train.hip) -------------------------------------------------
+// P[z][m][n] = sum over K-chunk z of A(m,k) B(k,n), A(m,k) = A[m*lda+k]; "_nk": B(k,n) = B[n*ldb+k],
+// "_kn": B(k,n) = B[k*ldb+n].  sym (A = B, M = N) skips tiles below the diagonal.  gemm64_split
+// picks S chunks of kchunk (a multiple of 32) for about 1024 workgroups; P holds S*M*N doubles.
+void gemm64_split(int M, int N, int64_t K, int sym, int* S, int64_t* kchunk);
+hipError_t launch_gemm64_f32_nk(const float* A, int64_t lda, const float* B, int64_t ldb, double* P, int M, int N,
+                                int64_t K, int S, int64_t kchunk, int sym, hipStream_t s);
+hipError_t launch_gemm64_nk(const double* A, int64_t lda, const double* B, int64_t ldb, double* P, int M, int N,
+                            int64_t K, int S, int64_t kchunk, int sym, hipStream_t s);
+hipError_t launch_gemm64_kn(const double* A, int64_t lda, const double* B, int64_t ldb, double* P, int M, int N,
+                            int64_t K, int S, int64_t kchunk, hipStream_t s);
+// out = beta*out + alpha*sum_z P[z] (+ diag on the diagonal), chunks summed in order
+hipError_t launch_reduce64(const double* P, int S, int M, int N, int sym, double alpha, double beta, double diag,
+                           double* out, int64_t ldo, hipStream_t s);
+hipError_t launch_sumsq64(const double* x, int M, int N, int64_t ldx, double* out, hipStream_t s);
+hipError_t launch_f64_to_f32(const double* x, int M, int N, int64_t ldx, float* y, int64_t ldy, hipStream_t s);
+// K[i][j] = exp(-((xc_i-xc_j)^2 + (yc_i-shift_i-yc_j)^2) / (2 sig^2)) + lam [i==j]; shift may be NULL
+hipError_t launch_interp_kernel(const double* xc, const double* yc, const double* shift, int C, double sig, double lam,
+                                double* Kout, hipStream_t s);
+// Jacobi eigensolver in LDS, matrices of order n <= kEigMax
+constexpr int kEigMax = 96;
+// row j of W = first row of the ZCA of cc restricted to idx[j][0..nr), scattered to columns idx[j]
+hipError_t launch_whiten_local(const double* cc, int64_t ldcc, const int* idx, int nr, int C, double eps, double* W,
+                               int64_t ldw, hipStream_t s);
+hipError_t launch_whiten_full(const double* cc, int64_t ldcc, int n, double eps, double* W, int64_t ldw, hipStream_t s);
+// out [npc][n]: leading eigenvectors of G (sign: largest-magnitude entry positive); sv [npc]
+hipError_t launch_pca(const double* G, int64_t ldg, int n, int npc, double* out, double* sv, hipStream_t s);
+// Isolated peaks of a [W][ldx] window: bitmaps pk, iso of C rows x nw words (nw >= ceil(W/32)).
+hipError_t launch_isolated_peaks(const float* X, int64_t ldx, int C, int64_t W, int lc, int lt, float th, int gc,
+                                 int gt, int edge, unsigned* pk, unsigned* iso, int64_t nw, hipStream_t s);
+int snippet_scan_blocks(int64_t nwords);
+// clips[i][m] = X[t_i - twav_min + m][c_i] for the iso bits in (channel, time) order, i < max_clips;
+// bsum: snippet_scan_blocks(nwords) ints; *total = number of iso bits
+hipError_t launch_emit_snippets(const unsigned* iso, int64_t nwords, int64_t nw, int* bsum, long long* total,
+                                const float* X, int64_t ldx, int nt, int twav_min, float* clips, int64_t max_clips,
+                                int* chan, int* tim, hipStream_t s);
+// Q[m][i] = clips[i][m] / ||clips[i]|| (FP64)
+hipError_t launch_normalize_clips(const float* clips, int64_t n, int nt, double* Q, hipStream_t s);
+
 }  // namespace lss

@@ -1,0 +1,280 @@
+// C ABI of the training-side FP64 builders (whitening, drift interpolation, wPCA).
+//
+// These run once per recording (or once per batch for the drift matrix), so every entry point is
+// synchronous: it allocates its scratch, enqueues its kernels on the given stream, waits for them
+// and frees the scratch.  Inputs and outputs named d_* are device memory of the current device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "../../include/lss.h"
+#include "lss_kernels.h"
+
+namespace lss {
+lss_status set_error(lss_status st, const char* msg);
+}
+
+namespace {
+
+lss_status err(lss_status st, const std::string& m) { return lss::set_error(st, m.c_str()); }
+
+#define TRY_HIP(expr)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess) return err(LSS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Device scratch freed on scope exit, after the stream has drained.
+struct Scratch {
+    hipStream_t s;
+    std::vector<void*> ptrs;
+    explicit Scratch(hipStream_t st) : s(st) {}
+    template <typename T>
+    hipError_t get(T** p, size_t count) {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, sizeof(T) * std::max<size_t>(count, 1));
+        if (e == hipSuccess) ptrs.push_back(q);
+        *p = (T*)q;
+        return e;
+    }
+    ~Scratch() {
+        (void)hipStreamSynchronize(s);
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+};
+
+// out = A . B for [C][C] row-major FP64 matrices (out may alias neither input)
+lss_status gemm_square(const double* A, int64_t lda, const double* B, int64_t ldb, double* out, int64_t ldo, int n,
+                       double* P, int S, int64_t kchunk, hipStream_t s) {
+    TRY_HIP(lss::launch_gemm64_kn(A, lda, B, ldb, P, n, n, n, S, kchunk, s));
+    TRY_HIP(lss::launch_reduce64(P, S, n, n, 0, 1.0, 0.0, 0.0, out, ldo, s));
+    return LSS_OK;
+}
+
+// Linear interpolation with linear extrapolation (scipy.interpolate.interp1d(kind='linear',
+// fill_value='extrapolate')): x sorted stably, interval from searchsorted(side='left') clipped to
+// [1, n-1], y = slope * (x_new - x_lo) + y_lo.
+void interp_linear(const double* x, const double* y, int n, const double* xn, int m, double* out) {
+    std::vector<int> ord(n);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return x[a] < x[b]; });
+    std::vector<double> xs(n), ys(n);
+    for (int i = 0; i < n; ++i) { xs[i] = x[ord[i]]; ys[i] = y[ord[i]]; }
+    for (int j = 0; j < m; ++j) {
+        int idx = (int)(std::lower_bound(xs.begin(), xs.end(), xn[j]) - xs.begin());
+        idx = std::min(std::max(idx, 1), n - 1);
+        const int lo = idx - 1, hi = idx;
+        const double slope = (ys[hi] - ys[lo]) / (xs[hi] - xs[lo]);
+        out[j] = slope * (xn[j] - xs[lo]) + ys[lo];
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+lss_status lss_covariance_add(const float* d_x, int64_t ldx, int32_t C, int64_t n, double* d_cc, int64_t ldcc,
+                              void* hip_stream) {
+    if (!d_x || !d_cc || C < 1 || n < 1 || ldx < n || ldcc < C)
+        return err(LSS_ERR_ARG, "lss_covariance_add: bad argument");
+    hipStream_t s = (hipStream_t)hip_stream;
+    int S;
+    int64_t kchunk;
+    lss::gemm64_split(C, C, n, 1, &S, &kchunk);
+    Scratch sc(s);
+    double* P;
+    TRY_HIP(sc.get(&P, (size_t)S * C * C));
+    TRY_HIP(lss::launch_gemm64_f32_nk(d_x, ldx, d_x, ldx, P, C, C, n, S, kchunk, 1, s));
+    TRY_HIP(lss::launch_reduce64(P, S, C, C, 1, 1.0 / (double)n, 1.0, 0.0, d_cc, ldcc, s));
+    TRY_HIP(hipStreamSynchronize(s));
+    return LSS_OK;
+}
+
+lss_status lss_whitening_from_covariance(const double* d_cc, int64_t ldcc, int32_t n, double eps, double* d_w,
+                                         int64_t ldw, void* hip_stream) {
+    if (!d_cc || !d_w || n < 1 || n > lss::kEigMax || ldcc < n || ldw < n)
+        return err(LSS_ERR_ARG, "lss_whitening_from_covariance: bad argument (n <= 96)");
+    hipStream_t s = (hipStream_t)hip_stream;
+    TRY_HIP(lss::launch_whiten_full(d_cc, ldcc, n, eps, d_w, ldw, s));
+    TRY_HIP(hipStreamSynchronize(s));
+    return LSS_OK;
+}
+
+lss_status lss_whitening_local(const double* d_cc, int64_t ldcc, int32_t C, const double* xc, const double* yc,
+                               int32_t nrange, double eps, double* d_w, int64_t ldw, void* hip_stream) {
+    if (!d_cc || !d_w || !xc || !yc || C < 1 || C > 1024 || nrange < 1 || ldcc < C || ldw < C)
+        return err(LSS_ERR_ARG, "lss_whitening_local: bad argument");
+    const int nr = std::min<int>(nrange, C);
+    if (nr > lss::kEigMax) return err(LSS_ERR_ARG, "lss_whitening_local: nrange must be <= 96");
+    hipStream_t s = (hipStream_t)hip_stream;
+    std::vector<int> idx((size_t)C * nr);
+    std::vector<int> ord(C);
+    std::vector<double> ds(C);
+    for (int j = 0; j < C; ++j) {
+        for (int i = 0; i < C; ++i) {
+            const double dx = xc[j] - xc[i], dy = yc[j] - yc[i];
+            ds[i] = dx * dx + dy * dy;
+        }
+        std::iota(ord.begin(), ord.end(), 0);
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return ds[a] < ds[b]; });
+        std::copy(ord.begin(), ord.begin() + nr, idx.begin() + (size_t)j * nr);
+    }
+    Scratch sc(s);
+    int* d_idx;
+    TRY_HIP(sc.get(&d_idx, idx.size()));
+    TRY_HIP(hipMemcpyAsync(d_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice, s));
+    TRY_HIP(lss::launch_whiten_local(d_cc, ldcc, d_idx, nr, C, eps, d_w, ldw, s));
+    TRY_HIP(hipStreamSynchronize(s));
+    return LSS_OK;
+}
+
+lss_status lss_drift_inverse(const double* xc, const double* yc, int32_t C, double sig_interp, double lam,
+                             double* d_ikxx, int64_t ldk, int32_t* iterations, void* hip_stream) {
+    if (!xc || !yc || !d_ikxx || C < 1 || ldk < C || !(sig_interp > 0.0) || !(lam >= 0.0))
+        return err(LSS_ERR_ARG, "lss_drift_inverse: bad argument");
+    hipStream_t s = (hipStream_t)hip_stream;
+    int S;
+    int64_t kchunk;
+    lss::gemm64_split(C, C, C, 0, &S, &kchunk);
+    const size_t CC = (size_t)C * C;
+    Scratch sc(s);
+    double *dxc, *dyc, *A, *X, *R, *P, *nrm;
+    TRY_HIP(sc.get(&dxc, C));
+    TRY_HIP(sc.get(&dyc, C));
+    TRY_HIP(sc.get(&A, CC));
+    TRY_HIP(sc.get(&X, CC));
+    TRY_HIP(sc.get(&R, CC));
+    TRY_HIP(sc.get(&P, (size_t)S * CC));
+    TRY_HIP(sc.get(&nrm, 1));
+    TRY_HIP(hipMemcpyAsync(dxc, xc, sizeof(double) * C, hipMemcpyHostToDevice, s));
+    TRY_HIP(hipMemcpyAsync(dyc, yc, sizeof(double) * C, hipMemcpyHostToDevice, s));
+    TRY_HIP(lss::launch_interp_kernel(dxc, dyc, nullptr, C, sig_interp, lam, A, s));
+    // X0 = I / ||A||_inf: A is symmetric positive definite with positive entries, so the
+    // eigenvalues of I - A X0 lie in [0, 1) and Newton-Schulz X <- X + X (I - A X) converges.
+    std::vector<double> hA(CC);
+    TRY_HIP(hipMemcpyAsync(hA.data(), A, sizeof(double) * CC, hipMemcpyDeviceToHost, s));
+    TRY_HIP(hipStreamSynchronize(s));
+    double ninf = 0.0;
+    for (int i = 0; i < C; ++i) {
+        double r = 0.0;
+        for (int j = 0; j < C; ++j) r += std::fabs(hA[(size_t)i * C + j]);
+        ninf = std::max(ninf, r);
+    }
+    TRY_HIP(lss::launch_reduce64(nullptr, 0, C, C, 0, 0.0, 0.0, 1.0 / ninf, X, C, s));
+    double prev = INFINITY, r = INFINITY;
+    int it = 0;
+    for (; it < 200; ++it) {
+        TRY_HIP(lss::launch_gemm64_kn(A, C, X, C, P, C, C, C, S, kchunk, s));
+        TRY_HIP(lss::launch_reduce64(P, S, C, C, 0, -1.0, 0.0, 1.0, R, C, s));     // R = I - A X
+        TRY_HIP(lss::launch_sumsq64(R, C, C, C, nrm, s));
+        double r2 = 0.0;
+        TRY_HIP(hipMemcpyAsync(&r2, nrm, sizeof(double), hipMemcpyDeviceToHost, s));
+        TRY_HIP(hipStreamSynchronize(s));
+        r = std::sqrt(r2);
+        // stop at FP64 resolution: the residual no longer shrinks quadratically
+        if (r < 1e-300 || (r < 1e-6 && r > 0.5 * prev)) break;
+        prev = r;
+        TRY_HIP(lss::launch_gemm64_kn(X, C, R, C, P, C, C, C, S, kchunk, s));
+        TRY_HIP(lss::launch_reduce64(P, S, C, C, 0, 1.0, 1.0, 0.0, X, C, s));     // X += X R
+    }
+    if (!(r < 1e-6))
+        return err(LSS_ERR_ARG, "lss_drift_inverse: Newton-Schulz did not converge (||I - A X|| = " +
+                                    std::to_string(r) + ")");
+    TRY_HIP(hipMemcpy2DAsync(d_ikxx, sizeof(double) * ldk, X, sizeof(double) * C, sizeof(double) * C, C,
+                             hipMemcpyDeviceToDevice, s));
+    TRY_HIP(hipStreamSynchronize(s));
+    if (iterations) *iterations = it;
+    return LSS_OK;
+}
+
+lss_status lss_drift_matrix(const double* xc, const double* yc, int32_t C, const double* yblk, const double* dshift,
+                            int32_t nblocks, double sig_interp, const double* d_ikxx, int64_t ldk, double* d_m,
+                            int64_t ldm, void* hip_stream) {
+    if (!xc || !yc || !dshift || !d_ikxx || !d_m || C < 1 || nblocks < 1 || ldk < C || ldm < C ||
+        !(sig_interp > 0.0) || (nblocks > 1 && !yblk))
+        return err(LSS_ERR_ARG, "lss_drift_matrix: bad argument");
+    hipStream_t s = (hipStream_t)hip_stream;
+    std::vector<double> shift(C);
+    if (nblocks == 1)
+        std::fill(shift.begin(), shift.end(), dshift[0]);
+    else
+        interp_linear(yblk, dshift, nblocks, yc, C, shift.data());
+    int S;
+    int64_t kchunk;
+    lss::gemm64_split(C, C, C, 0, &S, &kchunk);
+    const size_t CC = (size_t)C * C;
+    Scratch sc(s);
+    double *dxc, *dyc, *dsh, *K, *P;
+    TRY_HIP(sc.get(&dxc, C));
+    TRY_HIP(sc.get(&dyc, C));
+    TRY_HIP(sc.get(&dsh, C));
+    TRY_HIP(sc.get(&K, CC));
+    TRY_HIP(sc.get(&P, (size_t)S * CC));
+    TRY_HIP(hipMemcpyAsync(dxc, xc, sizeof(double) * C, hipMemcpyHostToDevice, s));
+    TRY_HIP(hipMemcpyAsync(dyc, yc, sizeof(double) * C, hipMemcpyHostToDevice, s));
+    TRY_HIP(hipMemcpyAsync(dsh, shift.data(), sizeof(double) * C, hipMemcpyHostToDevice, s));
+    TRY_HIP(lss::launch_interp_kernel(dxc, dyc, dsh, C, sig_interp, 0.0, K, s));   // Kyx
+    if (lss_status st = gemm_square(K, C, d_ikxx, ldk, d_m, ldm, C, P, S, kchunk, s)) return st;
+    TRY_HIP(hipStreamSynchronize(s));
+    return LSS_OK;
+}
+
+lss_status lss_extract_snippets(const float* d_x_wc, int64_t ldx, int32_t C, int64_t W, int32_t nt, int32_t twav_min,
+                                float th_single_ch, const int32_t* loc_range, const int32_t* long_range,
+                                float* d_clips, int64_t max_clips, int32_t* d_chan, int32_t* d_time, int64_t* n_found,
+                                void* hip_stream) {
+    if (!d_x_wc || !loc_range || !long_range || !n_found || C < 1 || ldx < C || W < 1 || nt < 1 ||
+        twav_min < 0 || twav_min > nt || max_clips < 0 || (max_clips > 0 && !d_clips))
+        return err(LSS_ERR_ARG, "lss_extract_snippets: bad argument");
+    if (loc_range[0] < 0 || loc_range[1] < 0 || loc_range[0] > 8 || loc_range[1] > 8 || long_range[0] < 0 ||
+        long_range[1] < 0)
+        return err(LSS_ERR_ARG, "lss_extract_snippets: loc_range entries must be in [0, 8], long_range >= 0");
+    hipStream_t s = (hipStream_t)hip_stream;
+    const int64_t nw = (W + 63) / 64 * 2;
+    const int64_t nwords = (int64_t)C * nw;
+    const int nb = lss::snippet_scan_blocks(nwords);
+    Scratch sc(s);
+    unsigned *pk, *iso;
+    int* bsum;
+    long long* total;
+    TRY_HIP(sc.get(&pk, nwords));
+    TRY_HIP(sc.get(&iso, nwords));
+    TRY_HIP(sc.get(&bsum, nb));
+    TRY_HIP(sc.get(&total, 1));
+    TRY_HIP(lss::launch_isolated_peaks(d_x_wc, ldx, C, W, loc_range[0], loc_range[1], th_single_ch, long_range[0],
+                                       long_range[1], nt, pk, iso, nw, s));
+    TRY_HIP(lss::launch_emit_snippets(iso, nwords, nw, bsum, total, d_x_wc, ldx, nt, twav_min, d_clips, max_clips,
+                                      d_chan, d_time, s));
+    long long n = 0;
+    TRY_HIP(hipMemcpyAsync(&n, total, sizeof(long long), hipMemcpyDeviceToHost, s));
+    TRY_HIP(hipStreamSynchronize(s));
+    *n_found = n;
+    return LSS_OK;
+}
+
+lss_status lss_wpca(const float* d_clips, int64_t n, int32_t nt, int32_t n_pcs, double* d_wpca, double* d_sv,
+                    void* hip_stream) {
+    if (!d_clips || !d_wpca || !d_sv || n < 1 || nt < 1 || nt > lss::kEigMax || n_pcs < 1 || n_pcs > nt)
+        return err(LSS_ERR_ARG, "lss_wpca: bad argument (nt <= 96, 1 <= n_pcs <= nt)");
+    hipStream_t s = (hipStream_t)hip_stream;
+    int S;
+    int64_t kchunk;
+    lss::gemm64_split(nt, nt, n, 1, &S, &kchunk);
+    Scratch sc(s);
+    double *Q, *P, *G;
+    TRY_HIP(sc.get(&Q, (size_t)n * nt));
+    TRY_HIP(sc.get(&P, (size_t)S * nt * nt));
+    TRY_HIP(sc.get(&G, (size_t)nt * nt));
+    TRY_HIP(lss::launch_normalize_clips(d_clips, n, nt, Q, s));
+    TRY_HIP(lss::launch_gemm64_nk(Q, n, Q, n, P, nt, nt, n, S, kchunk, 1, s));
+    TRY_HIP(lss::launch_reduce64(P, S, nt, nt, 1, 1.0, 0.0, 0.0, G, nt, s));
+    TRY_HIP(lss::launch_pca(G, nt, nt, n_pcs, d_wpca, d_sv, s));
+    TRY_HIP(hipStreamSynchronize(s));
+    return LSS_OK;
+}
+
+}  // extern "C"

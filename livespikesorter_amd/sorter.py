@@ -303,6 +303,19 @@ class Sorter:
                                       ct.byref(s)), "lss_stage_pursuit")
         return self._list(s, bufs)
 
+    def set_drift(self, M=None):
+        """Whitening operator := (M @ Wrot) for the next windows (M: [C][C] FP64 device tensor, e.g.
+        from train.get_drift_matrix); None restores Wrot (kilosort/io.py:781-787)."""
+        if M is None:
+            check(lib().lss_set_drift(self._h, None), "lss_set_drift")
+            return
+        import torch
+        M = M.to(torch.float64).contiguous()
+        if tuple(M.shape) != (self.C, self.C) or not M.is_cuda:
+            raise ValueError("M must be a [C][C] device tensor")
+        self._drift_keep = M          # read by the handle's stream after this call returns
+        check(lib().lss_set_drift(self._h, M.data_ptr()), "lss_set_drift")
+
     def tables(self) -> dict:
         T, C, nt = self.T, self.C, self.nt
         L = 2 * nt + 1
