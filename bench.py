@@ -379,6 +379,139 @@ def bench_precision(args):
     print(json.dumps(res))
 
 
+FP64_PEAK_TFLOPS = 78.6
+
+
+def _event_ms(fn, reps):
+    """Per-call ms of fn() between HIP events on torch's current stream (the builders' stream)."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    out = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        e1.synchronize()
+        out.append(e0.elapsed_time(e1))
+    return out
+
+
+def bench_train(args):
+    """Training-side builders in FP64 on the 384-channel drop-in workload (10 s, 5 batches of
+    NT = 60000): covariance of a filtered batch (the get_whitening_matrix inner loop), local ZCA
+    whitening, iKxx, the per-batch drift matrix and its application to the whitening operator,
+    isolated-peak snippets of a whitened batch and the wPCA eigen-solve."""
+    import torch
+    from livespikesorter_amd import synth, train
+    from livespikesorter_amd.sorter import Sorter, pad_batch
+
+    t0 = time.time()
+    s = synth.make(2, seed=args.seed)
+    a = s.art
+    NT, nt = a["NT"], a["nt"]
+    C = s.rec.shape[1]
+    nb = (s.rec.shape[0] + NT - 1) // NT
+    dev = torch.device("cuda:0")
+    filt = train.FilterStage(C, a["hp_filter"], a["xc"], a["yc"], NT + 2 * nt, nt=nt)
+    xs = [filt(torch.from_numpy(pad_batch(s.rec, j, NT, nt)[0]).to(dev)) for j in range(nb)]
+    filt.close()
+    sorter = Sorter(a)
+    xw = [sorter.stage_whiten(x) for x in xs]
+    xc = a["xc"].astype(np.float64)
+    yc = a["yc"].astype(np.float64)
+    print(f"[bench] train workload ready in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    reps = max(args.steps, 3)
+    CC = torch.zeros((C, C), dtype=torch.float64, device=dev)
+    it = iter(range(1 << 30))
+    cov_ms = _event_ms(lambda: train.covariance_add(xs[next(it) % nb][:, nt:-nt], CC), reps)
+    CC.zero_()
+    for x in xs:
+        train.covariance_add(x[:, nt:-nt], CC)
+    CC /= nb
+    zca_ms = _event_ms(lambda: train.whitening_local(CC, xc, yc, 32), max(reps // 5, 3))
+    inv = {}
+
+    def _inv():
+        inv["iK"], inv["it"] = train.drift_inverse(xc, yc, 20.0, 0.01)
+    inv_ms = _event_ms(_inv, max(reps // 10, 3))
+    yblk = np.linspace(float(yc.min()) + 200.0, float(yc.max()) - 200.0, 5)
+    dsh = np.array([3.0, -1.5, 4.0, 0.5, -2.0])
+    dm_ms = _event_ms(lambda: train.get_drift_matrix(xc, yc, dsh, inv["iK"], 20.0, yblk=yblk), reps)
+    M = train.get_drift_matrix(xc, yc, dsh, inv["iK"], 20.0, yblk=yblk)
+
+    def _set():
+        sorter.set_drift(M)
+        torch.cuda.synchronize()
+    set_ms = _event_ms(_set, reps)
+    it2 = iter(range(1 << 30))
+    snip_ms = _event_ms(lambda: train.extract_snippets(xw[next(it2) % nb]), reps)
+    parts = [train.extract_snippets(x)[0] for x in xw]
+    clips = torch.cat(parts)
+    ncl = clips.shape[0]
+    pca_ms = _event_ms(lambda: train.wpca(clips, 6), max(reps // 5, 3))
+    rep = (500000 + ncl - 1) // max(ncl, 1)
+    big = clips.repeat(rep, 1)[:500000].contiguous()
+    pca_big_ms = _event_ms(lambda: train.wpca(big, 6), 3)
+    flops = float(C) * (C + 1) * NT
+    cov_med = float(np.median(cov_ms))
+    ach = flops / (cov_med * 1e-3) / 1e12
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import train_oracle as to
+        X = xs[0][:, nt:-nt].cpu().numpy()
+        c1 = time.perf_counter()
+        to.covariance([X])
+        cov_cpu = time.perf_counter() - c1
+        CCn = CC.cpu().numpy()
+        c1 = time.perf_counter()
+        to.whitening_local(CCn, xc, yc, 32)
+        zca_cpu = time.perf_counter() - c1
+        c1 = time.perf_counter()
+        iKn = to.drift_inverse(xc, yc, 20.0, 0.01)
+        inv_cpu = time.perf_counter() - c1
+        c1 = time.perf_counter()
+        to.drift_matrix(xc, yc, dsh, iKn, 20.0, yblk=yblk)
+        dm_cpu = time.perf_counter() - c1
+        c1 = time.perf_counter()
+        to.extract_snippets(xw[0].cpu().numpy().T)
+        snip_cpu = time.perf_counter() - c1
+        c1 = time.perf_counter()
+        to.wpca(clips.cpu().numpy(), 6)
+        pca_cpu = time.perf_counter() - c1
+        cpu = {"value": round(2.0 / cov_cpu, 2), "unit": "x realtime (covariance of one 2 s batch)",
+               "cores": int(os.environ.get("OMP_NUM_THREADS", "16")), "kind": "port",
+               "sample": f"FP64 numpy oracle on the host: covariance of batch 0 {cov_cpu * 1e3:.1f} ms, "
+                         f"whitening_local {zca_cpu * 1e3:.0f} ms, iKxx {inv_cpu * 1e3:.0f} ms, drift matrix "
+                         f"{dm_cpu * 1e3:.1f} ms, snippets of batch 0 {snip_cpu * 1e3:.0f} ms, wPCA of {ncl} "
+                         f"snippets {pca_cpu * 1e3:.1f} ms"}
+    res = {
+        "metric": "FP64 covariance of filtered 2 s batches, 384 ch (the get_whitening_matrix inner loop)",
+        "value": round(2.0 / (cov_med * 1e-3), 1), "unit": "x realtime (2 s batch per call)",
+        "n_gpus": 1, "steps": reps, "warmup": 1, "ms_per_step": round(cov_med, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded 384-ch probe, 600 units, 10 s)",
+        "config": {"workload": "training-side builders: 384 ch, NT=60000 (5 filtered batches), nrange=32, "
+                               "sig_interp=20 um, 5 drift blocks, Th_single_ch=6, 6 PCs"},
+        "builders_ms_p50": {
+            "covariance_add": round(cov_med, 4),
+            "whitening_local_384x32": round(float(np.median(zca_ms)), 3),
+            "drift_inverse": round(float(np.median(inv_ms)), 3), "newton_schulz_steps": inv["it"],
+            "drift_matrix_5_blocks": round(float(np.median(dm_ms)), 4),
+            "set_drift": round(float(np.median(set_ms)), 4),
+            "extract_snippets_per_batch": round(float(np.median(snip_ms)), 4),
+            f"wpca_{ncl}_snippets": round(float(np.median(pca_ms)), 3),
+            "wpca_500000_snippets": round(float(np.median(pca_big_ms)), 3)},
+        "roofline": {"kernel": "gemm64_kernel<float,float,true> + reduce64 (symmetric FP64 rank-NT update)",
+                     "bound": "fp64", "achieved": round(ach, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(ach / FP64_PEAK_TFLOPS, 4), "traffic": None,
+                     "per_launch": f"{flops / 1e9:.2f} GFLOP algorithmic (C(C+1) NT), p50 {cov_med:.3f} ms"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -389,10 +522,11 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--workload", default="window", choices=["window", "stream", "adjoint", "precision"],
+    ap.add_argument("--workload", default="window", choices=["window", "stream", "adjoint", "precision", "train"],
                     help="window: 384-ch drop-in windows (default); stream: 4 uneven probes, 100 ms batches; "
                          "adjoint: template-refinement backward on a 2 s drift window; "
-                         "precision: config-3 precision report (1000 templates, 60 s)")
+                         "precision: config-3 precision report (1000 templates, 60 s); "
+                         "train: training-side FP64 builders (whitening, drift, wPCA)")
     ap.add_argument("--seconds", type=float, default=20.0, help="stream workload: seconds per probe")
     ap.add_argument("--refine-seconds", type=float, default=10.0,
                     help="stream workload: one template-refinement step per probe every this many seconds (0 = off)")
@@ -421,6 +555,8 @@ def main():
         return bench_adjoint(args)
     if args.workload == "precision":
         return bench_precision(args)
+    if args.workload == "train":
+        return bench_train(args)
 
     from livespikesorter_amd import synth
     from livespikesorter_amd.sorter import Sorter, pad_batch

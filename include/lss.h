@@ -297,10 +297,13 @@ lss_status lss_set_drift(void* handle, const double* d_m);
 
 /* ---- training-side artifacts in FP64 -------------------------------------------------------------
  * Builders of the Kilosort4 artifacts the sorter loads, on the GPU in FP64.  Synchronous: each
- * call allocates its scratch, runs on hip_stream (NULL: the null stream), waits and frees.
- * d_* arguments are device memory of the current device; xc, yc, yblk, dshift are host arrays.
- *
- * Covariance of one filtered batch, accumulated: cc[i][j] += sum_t x[i][t] x[j][t] / n over the
+ * call runs on hip_stream (NULL: the null stream) and waits for its work.  Scratch comes from a
+ * per-device arena that grows to the largest request and is kept until lss_train_release (calls on
+ * one device take turns on it).  d_* arguments are device memory of the current device; xc, yc,
+ * yblk, dshift are host arrays. */
+lss_status lss_train_release(void);
+
+/* Covariance of one filtered batch, accumulated: cc[i][j] += sum_t x[i][t] x[j][t] / n over the
  * n columns of the [C][ldx] rows x (the loop body of get_whitening_matrix,
  * kilosort/preprocessing.py:96-113, which then divides by the number of batches).  Products and
  * sums in FP64, summation order fixed; cc comes out exactly symmetric. */

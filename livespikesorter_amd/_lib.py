@@ -28,7 +28,7 @@ EXPORTS = ["lss_last_error", "lss_abi_version", "lss_default_params", "lss_creat
            "lss_residual_energy", "lss_residual_energy_grad", "lss_last_refine_timings",
            "lss_refine_templates", "lss_stream_set_refine", "lss_stream_last_refine", "lss_stream_push_device",
            "lss_set_drift", "lss_covariance_add", "lss_whitening_from_covariance", "lss_whitening_local",
-           "lss_drift_inverse", "lss_drift_matrix", "lss_extract_snippets", "lss_wpca"]
+           "lss_drift_inverse", "lss_drift_matrix", "lss_extract_snippets", "lss_wpca", "lss_train_release"]
 
 
 class Params(ct.Structure):
@@ -118,6 +118,7 @@ def lib() -> ct.CDLL:
             "lss_extract_snippets": ([vp, i64, i32, i64, i32, i32, ct.c_float, vp, vp, vp, i64, vp, vp,
                                       ct.POINTER(i64), vp], i32),
             "lss_wpca": ([vp, i64, i32, i32, vp, vp, vp], i32),
+            "lss_train_release": ([], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

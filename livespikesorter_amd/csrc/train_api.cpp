@@ -1,12 +1,15 @@
 // C ABI of the training-side FP64 builders (whitening, drift interpolation, wPCA).
 //
 // These run once per recording (or once per batch for the drift matrix), so every entry point is
-// synchronous: it allocates its scratch, enqueues its kernels on the given stream, waits for them
-// and frees the scratch.  Inputs and outputs named d_* are device memory of the current device.
+// synchronous: it takes its scratch from a per-device arena kept between calls, enqueues its
+// kernels on the given stream and waits for them.  Inputs and outputs named d_* are device memory
+// of the current device.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -28,22 +31,52 @@ lss_status err(lss_status st, const std::string& m) { return lss::set_error(st, 
         if (e_ != hipSuccess) return err(LSS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-// Device scratch freed on scope exit, after the stream has drained.
+// Device scratch of one call, carved from a per-device arena that grows to the largest request
+// and is kept for later calls (lss_train_release frees it).  A call holds the arena's lock from
+// commit() to its end, so concurrent calls on one device run one after the other.
+struct Arena {
+    void* p = nullptr;
+    size_t cap = 0;
+};
+std::mutex g_arena_mu;
+std::map<int, Arena> g_arena;
+
 struct Scratch {
     hipStream_t s;
-    std::vector<void*> ptrs;
+    std::vector<std::pair<void**, size_t>> req;
+    std::unique_lock<std::mutex> lock;
     explicit Scratch(hipStream_t st) : s(st) {}
     template <typename T>
-    hipError_t get(T** p, size_t count) {
-        void* q = nullptr;
-        hipError_t e = hipMalloc(&q, sizeof(T) * std::max<size_t>(count, 1));
-        if (e == hipSuccess) ptrs.push_back(q);
-        *p = (T*)q;
-        return e;
+    void add(T** p, size_t count) {
+        req.emplace_back((void**)p, (sizeof(T) * std::max<size_t>(count, 1) + 255) / 256 * 256);
+    }
+    hipError_t commit() {
+        size_t total = 0;
+        for (auto& r : req) total += r.second;
+        int dev = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e != hipSuccess) return e;
+        lock = std::unique_lock<std::mutex>(g_arena_mu);
+        Arena& a = g_arena[dev];
+        if (a.cap < total) {
+            if (a.p) {
+                if ((e = hipDeviceSynchronize()) != hipSuccess) return e;
+                (void)hipFree(a.p);
+                a.p = nullptr;
+                a.cap = 0;
+            }
+            if ((e = hipMalloc(&a.p, total)) != hipSuccess) return e;
+            a.cap = total;
+        }
+        size_t off = 0;
+        for (auto& r : req) {
+            *r.first = (char*)a.p + off;
+            off += r.second;
+        }
+        return hipSuccess;
     }
     ~Scratch() {
-        (void)hipStreamSynchronize(s);
-        for (void* p : ptrs) (void)hipFree(p);
+        if (lock.owns_lock()) (void)hipStreamSynchronize(s);
     }
 };
 
@@ -87,7 +120,8 @@ lss_status lss_covariance_add(const float* d_x, int64_t ldx, int32_t C, int64_t 
     lss::gemm64_split(C, C, n, 1, &S, &kchunk);
     Scratch sc(s);
     double* P;
-    TRY_HIP(sc.get(&P, (size_t)S * C * C));
+    sc.add(&P, (size_t)S * C * C);
+    TRY_HIP(sc.commit());
     TRY_HIP(lss::launch_gemm64_f32_nk(d_x, ldx, d_x, ldx, P, C, C, n, S, kchunk, 1, s));
     TRY_HIP(lss::launch_reduce64(P, S, C, C, 1, 1.0 / (double)n, 1.0, 0.0, d_cc, ldcc, s));
     TRY_HIP(hipStreamSynchronize(s));
@@ -125,7 +159,8 @@ lss_status lss_whitening_local(const double* d_cc, int64_t ldcc, int32_t C, cons
     }
     Scratch sc(s);
     int* d_idx;
-    TRY_HIP(sc.get(&d_idx, idx.size()));
+    sc.add(&d_idx, idx.size());
+    TRY_HIP(sc.commit());
     TRY_HIP(hipMemcpyAsync(d_idx, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice, s));
     TRY_HIP(lss::launch_whiten_local(d_cc, ldcc, d_idx, nr, C, eps, d_w, ldw, s));
     TRY_HIP(hipStreamSynchronize(s));
@@ -143,13 +178,14 @@ lss_status lss_drift_inverse(const double* xc, const double* yc, int32_t C, doub
     const size_t CC = (size_t)C * C;
     Scratch sc(s);
     double *dxc, *dyc, *A, *X, *R, *P, *nrm;
-    TRY_HIP(sc.get(&dxc, C));
-    TRY_HIP(sc.get(&dyc, C));
-    TRY_HIP(sc.get(&A, CC));
-    TRY_HIP(sc.get(&X, CC));
-    TRY_HIP(sc.get(&R, CC));
-    TRY_HIP(sc.get(&P, (size_t)S * CC));
-    TRY_HIP(sc.get(&nrm, 1));
+    sc.add(&dxc, C);
+    sc.add(&dyc, C);
+    sc.add(&A, CC);
+    sc.add(&X, CC);
+    sc.add(&R, CC);
+    sc.add(&P, (size_t)S * CC);
+    sc.add(&nrm, 1);
+    TRY_HIP(sc.commit());
     TRY_HIP(hipMemcpyAsync(dxc, xc, sizeof(double) * C, hipMemcpyHostToDevice, s));
     TRY_HIP(hipMemcpyAsync(dyc, yc, sizeof(double) * C, hipMemcpyHostToDevice, s));
     TRY_HIP(lss::launch_interp_kernel(dxc, dyc, nullptr, C, sig_interp, lam, A, s));
@@ -209,11 +245,12 @@ lss_status lss_drift_matrix(const double* xc, const double* yc, int32_t C, const
     const size_t CC = (size_t)C * C;
     Scratch sc(s);
     double *dxc, *dyc, *dsh, *K, *P;
-    TRY_HIP(sc.get(&dxc, C));
-    TRY_HIP(sc.get(&dyc, C));
-    TRY_HIP(sc.get(&dsh, C));
-    TRY_HIP(sc.get(&K, CC));
-    TRY_HIP(sc.get(&P, (size_t)S * CC));
+    sc.add(&dxc, C);
+    sc.add(&dyc, C);
+    sc.add(&dsh, C);
+    sc.add(&K, CC);
+    sc.add(&P, (size_t)S * CC);
+    TRY_HIP(sc.commit());
     TRY_HIP(hipMemcpyAsync(dxc, xc, sizeof(double) * C, hipMemcpyHostToDevice, s));
     TRY_HIP(hipMemcpyAsync(dyc, yc, sizeof(double) * C, hipMemcpyHostToDevice, s));
     TRY_HIP(hipMemcpyAsync(dsh, shift.data(), sizeof(double) * C, hipMemcpyHostToDevice, s));
@@ -241,10 +278,11 @@ lss_status lss_extract_snippets(const float* d_x_wc, int64_t ldx, int32_t C, int
     unsigned *pk, *iso;
     int* bsum;
     long long* total;
-    TRY_HIP(sc.get(&pk, nwords));
-    TRY_HIP(sc.get(&iso, nwords));
-    TRY_HIP(sc.get(&bsum, nb));
-    TRY_HIP(sc.get(&total, 1));
+    sc.add(&pk, nwords);
+    sc.add(&iso, nwords);
+    sc.add(&bsum, nb);
+    sc.add(&total, 1);
+    TRY_HIP(sc.commit());
     TRY_HIP(lss::launch_isolated_peaks(d_x_wc, ldx, C, W, loc_range[0], loc_range[1], th_single_ch, long_range[0],
                                        long_range[1], nt, pk, iso, nw, s));
     TRY_HIP(lss::launch_emit_snippets(iso, nwords, nw, bsum, total, d_x_wc, ldx, nt, twav_min, d_clips, max_clips,
@@ -266,14 +304,30 @@ lss_status lss_wpca(const float* d_clips, int64_t n, int32_t nt, int32_t n_pcs, 
     lss::gemm64_split(nt, nt, n, 1, &S, &kchunk);
     Scratch sc(s);
     double *Q, *P, *G;
-    TRY_HIP(sc.get(&Q, (size_t)n * nt));
-    TRY_HIP(sc.get(&P, (size_t)S * nt * nt));
-    TRY_HIP(sc.get(&G, (size_t)nt * nt));
+    sc.add(&Q, (size_t)n * nt);
+    sc.add(&P, (size_t)S * nt * nt);
+    sc.add(&G, (size_t)nt * nt);
+    TRY_HIP(sc.commit());
     TRY_HIP(lss::launch_normalize_clips(d_clips, n, nt, Q, s));
     TRY_HIP(lss::launch_gemm64_nk(Q, n, Q, n, P, nt, nt, n, S, kchunk, 1, s));
     TRY_HIP(lss::launch_reduce64(P, S, nt, nt, 1, 1.0, 0.0, 0.0, G, nt, s));
     TRY_HIP(lss::launch_pca(G, nt, nt, n_pcs, d_wpca, d_sv, s));
     TRY_HIP(hipStreamSynchronize(s));
+    return LSS_OK;
+}
+
+lss_status lss_train_release(void) {
+    std::lock_guard<std::mutex> g(g_arena_mu);
+    for (auto& kv : g_arena) {
+        if (!kv.second.p) continue;
+        int cur = 0;
+        TRY_HIP(hipGetDevice(&cur));
+        TRY_HIP(hipSetDevice(kv.first));
+        TRY_HIP(hipDeviceSynchronize());
+        TRY_HIP(hipFree(kv.second.p));
+        TRY_HIP(hipSetDevice(cur));
+        kv.second = Arena{};
+    }
     return LSS_OK;
 }
 
