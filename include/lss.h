@@ -113,6 +113,9 @@ lss_status lss_create(const lss_params* p, const lss_artifacts* a, void** handle
 lss_status lss_create_from_dir(const char* dir, const lss_params* p, void** handle);
 
 lss_status lss_destroy(void* handle);
+/* Makes `device` the calling thread's current HIP device (OnlineSpikesV2::initializeSorter's
+ * cudaSetDevice, OnlineSpikesV2.cu:326-327); handles are created on the current device. */
+lss_status lss_select_device(int32_t device);
 lss_status lss_get_params(void* handle, lss_params* p);
 lss_status lss_set_stream(void* handle, void* hip_stream);
 

@@ -28,7 +28,8 @@ EXPORTS = ["lss_last_error", "lss_abi_version", "lss_default_params", "lss_creat
            "lss_residual_energy", "lss_residual_energy_grad", "lss_last_refine_timings",
            "lss_refine_templates", "lss_stream_set_refine", "lss_stream_last_refine", "lss_stream_push_device",
            "lss_set_drift", "lss_covariance_add", "lss_whitening_from_covariance", "lss_whitening_local",
-           "lss_drift_inverse", "lss_drift_matrix", "lss_extract_snippets", "lss_wpca", "lss_train_release"]
+           "lss_drift_inverse", "lss_drift_matrix", "lss_extract_snippets", "lss_wpca", "lss_train_release",
+           "lss_select_device"]
 
 
 class Params(ct.Structure):
@@ -119,6 +120,7 @@ def lib() -> ct.CDLL:
                                       ct.POINTER(i64), vp], i32),
             "lss_wpca": ([vp, i64, i32, i32, vp, vp, vp], i32),
             "lss_train_release": ([], i32),
+            "lss_select_device": ([i32], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

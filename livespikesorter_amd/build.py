@@ -58,7 +58,26 @@ def build(verbose: bool = False, jobs: int = 8) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}{r.stderr}")
     os.replace(tmp, LIB)
+    build_facade_check()
     return LIB
+
+
+def build_facade_check() -> Path | None:
+    """The C++ program that drives include/lss_online_spikes.hpp in the tests (tests/ next to its
+    source; it finds liblss.so through its rpath)."""
+    src = ROOT / "tests" / "online_facade_check.cpp"
+    if not src.exists():
+        return None
+    exe = src.with_suffix("")
+    cxx = shutil.which("g++") or shutil.which("c++")
+    if cxx is None:
+        raise RuntimeError("no host C++ compiler for the facade check program")
+    cmd = [cxx, "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", str(ROOT / "include"), str(src), "-o", str(exe),
+           "-L", str(PKG), "-llss", "-Wl,-rpath,$ORIGIN/../livespikesorter_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"facade check build failed:\n{r.stdout}{r.stderr}")
+    return exe
 
 
 if __name__ == "__main__":

@@ -780,6 +780,14 @@ std::string trim(const std::string& s) {
 extern "C" {
 
 const char* lss_last_error(void) { return g_err.c_str(); }
+
+lss_status lss_select_device(int32_t device) {
+    int n = 0;
+    HIP_TRY(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(LSS_ERR_ARG, "lss_select_device: no such device");
+    HIP_TRY(hipSetDevice(device));
+    return LSS_OK;
+}
 int32_t lss_abi_version(void) { return LSS_ABI_VERSION; }
 
 lss_status lss_default_params(lss_params* p) {
