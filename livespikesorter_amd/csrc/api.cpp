@@ -3,8 +3,9 @@
 // The handle plays the role of OnlineSpikesV2 (src/SpikeSorter/OnlineSpikesV2.cu:247-630: load
 // artifacts, allocate every buffer once for the largest window) and of its per-batch operator
 // kilosortMatchingPursuit + computeClosestClusters (OnlineSpikesV2.cu:823-971), with Kilosort4
-// semantics where the two differ.  All device work runs on the handle's stream; the pursuit reads
-// one 4-byte counter back per pass to decide whether to continue.
+// semantics where the two differ.  All device work runs on the handle's stream; the pursuit runs
+// its max_passes passes without host round trips (a pass that finds no peak ends the work of the
+// later ones on the device) and the counters are read back once per window.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
