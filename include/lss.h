@@ -127,6 +127,12 @@ lss_status lss_set_stream(void* handle, void* hip_stream);
 lss_status lss_sort_window(void* handle, const int16_t* window, int64_t W, int32_t on_device,
                            int64_t window_start, lss_spikes* out);
 
+/* Several windows with up to n_handles of them in flight: window i is sorted by handles[i %
+ * n_handles] (distinct handles of one device, one host thread each, windows of a handle in
+ * order), so one window's GEMMs overlap another's pursuit.  outs[i] receives window i's spikes as
+ * lss_sort_window gives them.  Returns the first failure (its message names the window). */
+lss_status lss_sort_windows(void* const* handles, int32_t n_handles, const int16_t* const* windows, const int64_t* W,
+                            const int64_t* window_start, int32_t on_device, int32_t n_windows, lss_spikes* outs);
 /* Kilosort batch padding of a host int16 recording [n_samples][C] (kilosort/io.py:584-637):
  * writes the padded window [NT+2nt][C] and its window_start.  Host only. */
 lss_status lss_pad_batch(const int16_t* rec, int64_t n_samples, int32_t C, int32_t NT, int32_t nt,

@@ -20,6 +20,7 @@
 #include <memory>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/lss.h"
@@ -970,6 +971,49 @@ lss_status lss_sort_window(void* handle, const int16_t* window, int64_t W, int32
     for (int i = 0; i < 7; ++i) HIP_TRY(hipEventElapsedTime(&h.last_ms[i], h.ev[i], h.ev[i + 1]));
     HIP_TRY(hipEventElapsedTime(&h.last_ms[7], h.ev[0], h.ev[7]));
     return emit_spikes(h, n, passes, window_start, out);
+}
+
+lss_status lss_sort_windows(void* const* handles, int32_t n_handles, const int16_t* const* windows, const int64_t* W,
+                            const int64_t* window_start, int32_t on_device, int32_t n_windows, lss_spikes* outs) {
+    if (!handles || n_handles < 1 || n_windows < 0 || (n_windows > 0 && (!windows || !W || !window_start || !outs)))
+        return fail(LSS_ERR_ARG, "lss_sort_windows: bad argument");
+    for (int j = 0; j < n_handles; ++j) {
+        if (!handles[j]) return fail(LSS_ERR_ARG, "lss_sort_windows: null handle");
+        for (int i = 0; i < j; ++i)
+            if (handles[i] == handles[j]) return fail(LSS_ERR_ARG, "lss_sort_windows: handles must be distinct");
+    }
+    if (n_handles == 1 || n_windows <= 1) {
+        for (int i = 0; i < n_windows; ++i)
+            if (lss_status st = lss_sort_window(handles[0], windows[i], W[i], on_device, window_start[i], &outs[i]))
+                return st;
+        return LSS_OK;
+    }
+    // one host thread per handle: window i goes to handle i % n_handles, in order on that handle
+    std::vector<lss_status> st(n_handles, LSS_OK);
+    std::vector<std::string> msg(n_handles);
+    std::vector<std::thread> th;
+    th.reserve(n_handles);
+    for (int j = 0; j < n_handles; ++j)
+        th.emplace_back([&, j] {
+            Handle& h = *as_handle(handles[j]);
+            if (hipSetDevice(h.device) != hipSuccess) {
+                st[j] = LSS_ERR_HIP;
+                msg[j] = "hipSetDevice failed";
+                return;
+            }
+            for (int i = j; i < n_windows; i += n_handles) {
+                lss_status s = lss_sort_window(handles[j], windows[i], W[i], on_device, window_start[i], &outs[i]);
+                if (s != LSS_OK) {
+                    st[j] = s;
+                    msg[j] = "window " + std::to_string(i) + ": " + g_err;
+                    return;
+                }
+            }
+        });
+    for (auto& t : th) t.join();
+    for (int j = 0; j < n_handles; ++j)
+        if (st[j] != LSS_OK) return fail(st[j], msg[j]);
+    return LSS_OK;
 }
 
 lss_status lss_stage_car(void* handle, const int16_t* d_window, int64_t W, float* d_out_cw) {

@@ -328,6 +328,71 @@ class Sorter:
         return dict(nm=nm, waves=waves, ctc=ctc, wmat=wmat)
 
 
+def sort_windows(sorters, windows, starts, capacity: int | None = None) -> list:
+    """Windows with up to len(sorters) in flight: window i is sorted by sorters[i % len(sorters)]
+    on that sorter's own host thread (ctypes releases the GIL during the library call), so one
+    window's GEMMs overlap another's pursuit and each thread copies its spike lists out while the
+    GPU runs the others.  Results in window order, equal to sorting each window alone."""
+    import threading
+    k = len(sorters)
+    if k == 0:
+        raise ValueError("no sorters")
+    if len({id(so) for so in sorters}) != k:
+        raise ValueError("sorters must be distinct")
+    res = [None] * len(windows)
+    errs = []
+
+    def work(j):
+        try:
+            for i in range(j, len(windows), k):
+                res[i] = sorters[j].sort_window(windows[i], starts[i], capacity)
+        except Exception as e:          # re-raised in the caller
+            errs.append(e)
+
+    th = [threading.Thread(target=work, args=(j,)) for j in range(k)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    if errs:
+        raise errs[0]
+    return res
+
+
+def sort_windows_native(sorters, windows, starts, capacity: int = 1 << 15, chunk: int = 16) -> list:
+    """The same through lss_sort_windows (the C ABI's own threads, one per handle), for C-side
+    callers' semantics: output buffers of `capacity` spikes per window, reused over groups of
+    `chunk` windows and kept on the first sorter."""
+    if not sorters:
+        raise ValueError("no sorters")
+    on_device = not isinstance(windows[0], np.ndarray)
+    hs = (ct.c_void_p * len(sorters))(*[so._h for so in sorters])
+    ref = sorters[0]
+    group = max(int(chunk), len(sorters))
+    need = min(group, len(windows))
+    cached = getattr(ref, "_windows_pool", None)
+    if cached is None or cached[0] != int(capacity) or len(cached[1]) < need:
+        ref._windows_pool = (int(capacity), [alloc_spikes(int(capacity), ref.nC, ref.K) for _ in range(need)])
+    pool = ref._windows_pool[1]
+    res = []
+    for g0 in range(0, len(windows), group):
+        ws = windows[g0:g0 + group]
+        if on_device:
+            ptrs = [_ptr(w) for w in ws]
+        else:
+            ws = [np.ascontiguousarray(w, dtype=np.int16) for w in ws]
+            ptrs = [w.ctypes.data for w in ws]
+        n = len(ws)
+        wp = (ct.c_void_p * n)(*ptrs)
+        Wn = (ct.c_int64 * n)(*[int(w.shape[0]) for w in ws])
+        st = (ct.c_int64 * n)(*[int(x) for x in starts[g0:g0 + n]])
+        outs = (Spikes * n)(*[pool[i][0] for i in range(n)])
+        check(lib().lss_sort_windows(hs, len(sorters), wp, Wn, st, int(on_device), n, outs), "lss_sort_windows")
+        for i in range(n):
+            res.append(Sorter._list(outs[i], pool[i][1]))
+    return res
+
+
 def pad_batch(rec: np.ndarray, ibatch: int, NT: int, nt: int):
     """Kilosort padded window (int16 [NT+2nt][C]) and its window_start, via the C ABI."""
     rec = np.ascontiguousarray(rec, dtype=np.int16)

@@ -319,3 +319,30 @@ def test_from_dir_matches_in_memory_artifacts(sorter, synth1, tmp_path):
     np.testing.assert_array_equal(got.sample, ref.sample)
     np.testing.assert_array_equal(got.template, ref.template)
     np.testing.assert_array_equal(got.amplitude, ref.amplitude)
+
+
+def test_sort_windows_in_flight_equals_one_at_a_time(sorter, synth1):
+    """Two windows in flight over two handles (lss_sort_windows, and Python threads over
+    lss_sort_window) give each window's spikes exactly as one at a time; repeated handles are
+    rejected."""
+    print("[lss] test_sort_windows_in_flight_equals_one_at_a_time", flush=True)
+    from livespikesorter_amd._lib import LssError
+    from livespikesorter_amd.sorter import Sorter, sort_windows, sort_windows_native
+    a = synth1.art
+    wins, starts = [], []
+    for ib in range(2):
+        win, _, ws = _window(synth1, ib)
+        wins.append(win)
+        starts.append(ws)
+    order = [0, 1, 1, 0, 1]
+    ref = [sorter.sort_window(wins[i], starts[i]) for i in order]
+    with Sorter(a, max_window=60122, max_passes=100) as s2:
+        for fn, kw in ((sort_windows_native, dict(chunk=3)), (sort_windows, {})):
+            got = fn([sorter, s2], [wins[i] for i in order], [starts[i] for i in order], **kw)
+            for r, g in zip(ref, got):
+                np.testing.assert_array_equal(g.sample, r.sample)
+                np.testing.assert_array_equal(g.template, r.template)
+                np.testing.assert_array_equal(g.amplitude, r.amplitude)
+                np.testing.assert_array_equal(g.features, r.features)
+        with pytest.raises(LssError):
+            sort_windows_native([sorter, sorter], wins, starts)
