@@ -1112,6 +1112,25 @@ lss_status refine_alloc(Handle& h) {
     return LSS_OK;
 }
 
+// everything lss_refine_templates needs beyond refine_alloc (done once per handle)
+lss_status refine_templates_alloc(Handle& h) {
+    if (lss_status st = refine_alloc(h)) return st;
+    if (h.rf_U.p) return LSS_OK;
+    const size_t nU = (size_t)h.T * h.K * h.C, nW = (size_t)h.K * h.nt;
+    HIP_TRY(h.rf_R2.alloc((size_t)h.Wmax * h.ldC));
+    HIP_TRY(h.rf_U.alloc(nU));
+    HIP_TRY(h.rf_U1.alloc(nU));
+    HIP_TRY(h.rf_gU.alloc(nU));
+    HIP_TRY(h.rf_gW.alloc(nW));
+    HIP_TRY(h.rf_W1.alloc(nW));
+    HIP_TRY(h.rf_col.alloc(h.prm.max_spikes));
+    HIP_TRY(h.rf_tmp.alloc(h.prm.max_spikes));
+    HIP_TRY(h.rf_amp.alloc(h.prm.max_spikes));
+    HIP_TRY(h.rf_sums.alloc(256 + 4));
+    HIP_TRY(hipHostMalloc((void**)&h.hs_U, sizeof(float) * nU, hipHostMallocDefault));
+    return LSS_OK;
+}
+
 lss_status refine_run(Handle& h, const float* d_x, int64_t W, const int32_t* d_col, const int32_t* d_tmp,
                       const float* d_amp, int64_t n, const float* d_U, const float* d_wpca, double* loss, float* d_gU,
                       float* d_gW) {
@@ -1185,22 +1204,9 @@ lss_status lss_refine_templates(void* handle, double* info) {
     HIP_TRY(hipStreamSynchronize(h.stream));
     const int n = std::min(h.h_counters[0] + h.h_counters[1], h.prm.max_spikes);
     if (n < 1) return LSS_OK;
-    if (lss_status st = refine_alloc(h)) return st;
+    if (lss_status st = refine_templates_alloc(h)) return st;
     const int T = h.T, K = h.K, C = h.C, nt = h.nt;
     const size_t nU = (size_t)T * K * C, nW = (size_t)K * nt;
-    if (!h.rf_U.p) {
-        HIP_TRY(h.rf_R2.alloc((size_t)h.Wmax * h.ldC));
-        HIP_TRY(h.rf_U.alloc(nU));
-        HIP_TRY(h.rf_U1.alloc(nU));
-        HIP_TRY(h.rf_gU.alloc(nU));
-        HIP_TRY(h.rf_gW.alloc(nW));
-        HIP_TRY(h.rf_W1.alloc(nW));
-        HIP_TRY(h.rf_col.alloc(h.prm.max_spikes));
-        HIP_TRY(h.rf_tmp.alloc(h.prm.max_spikes));
-        HIP_TRY(h.rf_amp.alloc(h.prm.max_spikes));
-        HIP_TRY(h.rf_sums.alloc(256 + 4));
-        HIP_TRY(hipHostMalloc((void**)&h.hs_U, sizeof(float) * nU, hipHostMallocDefault));
-    }
     // the window's spikes sorted by column (the residual kernel's order)
     std::vector<int> col(n), tmp(n), ord(n);
     std::vector<float> amp(n), namp(n);
@@ -1566,7 +1572,12 @@ lss_status lss_stream_stats(void* stream, float* last_ms, int64_t* batches) {
 
 lss_status lss_stream_set_refine(void* stream, int32_t every_batches) {
     if (!stream || every_batches < 0) return fail(LSS_ERR_ARG, "bad argument");
-    static_cast<Batcher*>(stream)->refine_every = every_batches;
+    Batcher& b = *static_cast<Batcher*>(stream);
+    // the refinement buffers are allocated now, not in the step that first refines
+    if (every_batches > 0)
+        for (auto& p : b.pr)
+            if (lss_status st = refine_templates_alloc(*p.h)) return st;
+    b.refine_every = every_batches;
     return LSS_OK;
 }
 

@@ -62,24 +62,40 @@ __global__ __launch_bounds__(256) void gram_kernel(const float* __restrict__ U, 
 
 // ctc[j][l][i]: one thread per (i, j), all lags.  G is symmetric, so G[(i,k)][(j,m)] is read as
 // G[(j,m)][(i,k)]: neighbouring threads (consecutive i) then read neighbouring K-runs of one row.
+// K is a template parameter so that the K*K Gram entries of a pair stay in registers.
+template <int KK>
 __global__ __launch_bounds__(256) void ctc_kernel(const double* __restrict__ G, const double* __restrict__ wtw,
-                                                  int T, int K, int L, float* __restrict__ ctc, int64_t ldT) {
+                                                  int T, int L, float* __restrict__ ctc, int64_t ldT) {
     extern __shared__ double swtw[];   // [K][K][L]
-    for (int e = threadIdx.x; e < K * K * L; e += 256) swtw[e] = wtw[e];
+    for (int e = threadIdx.x; e < KK * KK * L; e += 256) swtw[e] = wtw[e];
     __syncthreads();
     const int i = blockIdx.x * 256 + threadIdx.x;
     const int j = blockIdx.y;
     if (i >= T) return;
-    const int R = T * K;
-    double g[8 * 8];
-    for (int k = 0; k < K; ++k)
-        for (int m = 0; m < K; ++m) g[k * K + m] = G[(int64_t)(j * K + m) * R + (i * K + k)];
+    const int R = T * KK;
+    double g[KK * KK];
+#pragma unroll
+    for (int k = 0; k < KK; ++k)
+#pragma unroll
+        for (int m = 0; m < KK; ++m) g[k * KK + m] = G[(int64_t)(j * KK + m) * R + (i * KK + k)];
     for (int l = 0; l < L; ++l) {
         double acc = 0.0;
-        for (int k = 0; k < K; ++k)
-            for (int m = 0; m < K; ++m) acc += g[k * K + m] * swtw[(k * K + m) * L + l];
+#pragma unroll
+        for (int q = 0; q < KK * KK; ++q) acc += g[q] * swtw[q * L + l];
         ctc[((int64_t)j * L + l) * ldT + i] = (float)acc;
     }
+}
+
+template <int KK>
+static hipError_t ctc_launch(const double* G, const double* wtw, int T, int L, float* ctc, int64_t ldT, hipStream_t s) {
+    const size_t lds = sizeof(double) * KK * KK * L;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)ctc_kernel<KK>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((ctc_kernel<KK>), dim3((T + 255) / 256, T), dim3(256), lds, s, G, wtw, T, L, ctc, ldT);
+    return hipGetLastError();
 }
 
 hipError_t launch_waves(const float* U, int64_t ldu, const float* wpca, int T, int K, int C, int nt,
@@ -127,13 +143,17 @@ hipError_t launch_tables(const float* U, int64_t ldu, const float* wpca, int T, 
         hipLaunchKernelGGL(gram_kernel, dim3((R + 15) / 16, (R + 15) / 16), dim3(256), 0, s, U, ldu, K, R, C, dG);
         e = hipGetLastError();
     }
-    const size_t lds = sizeof(double) * K * K * L;
-    if (e == hipSuccess && lds > 65536)
-        e = hipFuncSetAttribute((const void*)ctc_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(ctc_kernel, dim3((T + 255) / 256, T), dim3(256), lds, s,
-                           dG, dW, T, K, L, ctc, ldT);
-        e = hipGetLastError();
+        switch (K) {
+            case 1: e = ctc_launch<1>(dG, dW, T, L, ctc, ldT, s); break;
+            case 2: e = ctc_launch<2>(dG, dW, T, L, ctc, ldT, s); break;
+            case 3: e = ctc_launch<3>(dG, dW, T, L, ctc, ldT, s); break;
+            case 4: e = ctc_launch<4>(dG, dW, T, L, ctc, ldT, s); break;
+            case 5: e = ctc_launch<5>(dG, dW, T, L, ctc, ldT, s); break;
+            case 6: e = ctc_launch<6>(dG, dW, T, L, ctc, ldT, s); break;
+            case 7: e = ctc_launch<7>(dG, dW, T, L, ctc, ldT, s); break;
+            default: e = ctc_launch<8>(dG, dW, T, L, ctc, ldT, s); break;
+        }
     }
     hipError_t e2 = hipStreamSynchronize(s);
     free(hw);
