@@ -310,13 +310,18 @@ __global__ __launch_bounds__(kFftThreads) void highpass_kernel(const float* __re
     const float* xa = x + (int64_t)ca * ldx;
     const float* xb = x + (int64_t)cb * ldx;
     const bool hasb = cb < C;
-    // segment sample n <- window sample (o0 - Llo + n) mod W
+    // segment sample n <- window sample (o0 - Llo + n) mod W; the reduction is a subtraction
+    // except where the segment wraps more than once (windows shorter than the FFT)
     int64_t start = o0 - Llo;
     start %= W;
     if (start < 0) start += W;
+    const int Wi = (int)W, s0 = (int)start;
     for (int n = threadIdx.x; n < kFftN; n += kFftThreads) {
-        int64_t idx = start + n;
-        idx %= W;
+        int idx = s0 + n;
+        if (idx >= Wi) {
+            idx -= Wi;
+            if (idx >= Wi) idx %= Wi;
+        }
         buf[n] = make_float2(xa[idx], hasb ? xb[idx] : 0.f);
     }
     __syncthreads();
