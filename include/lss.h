@@ -300,8 +300,9 @@ lss_status lss_last_update_stats(void* handle, float* total_ms, int32_t* launche
  * stride C, from lss_drift_matrix), the product taken in FP64 on the handle's stream and stored as
  * FP32; NULL restores Wrot alone.  Kilosort applies the batch's drift this way
  * (kilosort/io.py:781-787: M = get_drift_matrix(ops, dshift[ibatch]); X = (M @ Wrot) @ X); LSS
- * loads one fixed M (drift_matrix.npy, src/SpikeSorter/OnlineSpikesV2.cu:575-582).  Ordered with
- * later windows of the handle's stream; in streaming mode call it between lss_stream_step calls. */
+ * loads one fixed M (drift_matrix.npy, src/SpikeSorter/OnlineSpikesV2.cu:575-582).  Synchronous
+ * (the operator is in place when it returns); in streaming mode call it between lss_stream_step
+ * calls. */
 lss_status lss_set_drift(void* handle, const double* d_m);
 
 /* ---- training-side artifacts in FP64 -------------------------------------------------------------

@@ -1326,6 +1326,7 @@ lss_status lss_set_drift(void* handle, const double* d_m) {
     const int C = h.C;
     if (!d_m) {
         HIP_TRY(lss::launch_f64_to_f32(h.wrot64.p, C, C, C, h.wmat.p, h.ldm, h.stream));
+        HIP_TRY(hipStreamSynchronize(h.stream));
         return LSS_OK;
     }
     int S;
@@ -1343,6 +1344,8 @@ lss_status lss_set_drift(void* handle, const double* d_m) {
     HIP_TRY(lss::launch_gemm64_kn(d_m, C, h.wrot64.p, C, P, C, C, C, S, kchunk, h.stream));
     HIP_TRY(lss::launch_reduce64(P, S, C, C, 0, 1.0, 0.0, 0.0, wm, C, h.stream));
     HIP_TRY(lss::launch_f64_to_f32(wm, C, C, C, h.wmat.p, h.ldm, h.stream));
+    // done before returning: streaming steps launch on the batcher's own streams
+    HIP_TRY(hipStreamSynchronize(h.stream));
     return LSS_OK;
 }
 

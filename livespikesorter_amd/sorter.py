@@ -313,7 +313,7 @@ class Sorter:
         M = M.to(torch.float64).contiguous()
         if tuple(M.shape) != (self.C, self.C) or not M.is_cuda:
             raise ValueError("M must be a [C][C] device tensor")
-        self._drift_keep = M          # read by the handle's stream after this call returns
+        torch.cuda.current_stream().synchronize()   # M is complete before the handle's stream reads it
         check(lib().lss_set_drift(self._h, M.data_ptr()), "lss_set_drift")
 
     def tables(self) -> dict:
