@@ -89,6 +89,13 @@ def _ptr(t) -> int:
     return t.data_ptr()
 
 
+def _ready():
+    """Waits for torch's pending work on the current stream, so that device tensors handed to the
+    library (which works on its own streams) are complete."""
+    import torch
+    torch.cuda.current_stream().synchronize()
+
+
 class Sorter:
     """One sorter handle on the current HIP device."""
 
@@ -182,6 +189,7 @@ class Sorter:
             w = np.ascontiguousarray(window, dtype=np.int16)
             st = lib().lss_sort_window(self._h, w.ctypes.data, w.shape[0], 0, int(window_start), ct.byref(s))
         else:
+            _ready()
             st = lib().lss_sort_window(self._h, _ptr(window), window.shape[0], 1, int(window_start), ct.byref(s))
         check(st, "lss_sort_window")
         return self._list(s, bufs)
@@ -222,6 +230,7 @@ class Sorter:
         """sum R^2 for the fixed spike set (col sorted ascending); all arguments device tensors:
         x_wc float32 [W][C], col/tmp int32 [n], amp float32 [n], U float32 [T][K][C], wpca [K][nt]."""
         loss = ct.c_double()
+        _ready()
         check(lib().lss_residual_energy(self._h, _ptr(x_wc), x_wc.shape[0], _ptr(col), _ptr(tmp), _ptr(amp),
                                         col.numel(), _ptr(U), _ptr(wpca), ct.byref(loss)), "lss_residual_energy")
         return loss.value
@@ -232,6 +241,7 @@ class Sorter:
         gU = torch.empty_like(U) if grad_U is None else grad_U
         gW = torch.empty_like(wpca) if grad_wpca is None else grad_wpca
         loss = ct.c_double()
+        _ready()
         check(lib().lss_residual_energy_grad(self._h, _ptr(x_wc), x_wc.shape[0], _ptr(col), _ptr(tmp), _ptr(amp),
                                              col.numel(), _ptr(U), _ptr(wpca), ct.byref(loss), _ptr(gU), _ptr(gW)),
               "lss_residual_energy_grad")
@@ -265,6 +275,7 @@ class Sorter:
         import torch
         W = win_dev.shape[0]
         out = out if out is not None else torch.empty((self.C, W), dtype=torch.float32, device=win_dev.device)
+        _ready()
         check(lib().lss_stage_car(self._h, _ptr(win_dev), W, _ptr(out)), "lss_stage_car")
         return out
 
@@ -272,6 +283,7 @@ class Sorter:
         import torch
         W = win_dev.shape[0]
         out = out if out is not None else torch.empty((self.C, W), dtype=torch.float32, device=win_dev.device)
+        _ready()
         check(lib().lss_stage_filter(self._h, _ptr(win_dev), W, _ptr(out)), "lss_stage_filter")
         return out
 
@@ -279,6 +291,7 @@ class Sorter:
         import torch
         W = x_cw.shape[1]
         out = out if out is not None else torch.empty_like(x_cw)
+        _ready()
         check(lib().lss_stage_highpass(self._h, _ptr(x_cw), W, _ptr(out)), "lss_stage_highpass")
         return out
 
@@ -286,6 +299,7 @@ class Sorter:
         import torch
         W = x_cw.shape[1]
         out = out if out is not None else torch.empty((W, self.C), dtype=torch.float32, device=x_cw.device)
+        _ready()
         check(lib().lss_stage_whiten(self._h, _ptr(x_cw), W, _ptr(out)), "lss_stage_whiten")
         return out
 
@@ -293,12 +307,14 @@ class Sorter:
         import torch
         W = x_wc.shape[0]
         out = out if out is not None else torch.empty((W, self.T), dtype=torch.float32, device=x_wc.device)
+        _ready()
         check(lib().lss_stage_scores(self._h, _ptr(x_wc), W, _ptr(out)), "lss_stage_scores")
         return out
 
     def stage_pursuit(self, b_wt, x_wc, window_start: int, capacity: int | None = None) -> SpikeList:
         """Pursuit on scores [W][T] and residual [W][C] (both updated in place)."""
         s, bufs = self._out(capacity or self.params.max_spikes)
+        _ready()
         check(lib().lss_stage_pursuit(self._h, _ptr(b_wt), _ptr(x_wc), b_wt.shape[0], int(window_start),
                                       ct.byref(s)), "lss_stage_pursuit")
         return self._list(s, bufs)

@@ -42,6 +42,8 @@ class StreamingSorter:
             if not samples.is_contiguous():
                 raise ValueError("device samples must be contiguous")
             self._held.setdefault(probe, []).append(samples)
+            import torch
+            torch.cuda.current_stream().synchronize()   # the samples are complete before a step copies them
             check(lib().lss_stream_push_device(self._h, int(probe), samples.data_ptr(), samples.shape[0]),
                   "lss_stream_push_device")
             return
