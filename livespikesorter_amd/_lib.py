@@ -65,6 +65,12 @@ def lib() -> ct.CDLL:
     if _lib is None:
         if not LIB_PATH.exists():
             raise LssError(f"{LIB_PATH} is not built: run `python -m livespikesorter_amd.build`")
+        # torch first: its HIP runtime is then the one liblss binds to, so that torch tensors and
+        # the library share one runtime (loading liblss first leaves torch a second one)
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ct.CDLL(str(LIB_PATH))
         vp, i32, i64, f64 = ct.c_void_p, ct.c_int32, ct.c_int64, ct.c_double
         sig = {
