@@ -577,6 +577,9 @@ def main():
     ap.add_argument("--seconds", type=float, default=20.0, help="stream workload: seconds per probe")
     ap.add_argument("--refine-seconds", type=float, default=10.0,
                     help="stream workload: one template-refinement step per probe every this many seconds (0 = off)")
+    ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3],
+                    help="window workload: synthetic configuration (1: 64 ch / 32 units, the reference's CPU-runnable "
+                         "case; 2: 384 ch / 600 units, default; 3: 384 ch / 1000 units)")
     ap.add_argument("--in-flight", type=int, default=3,
                     help="window workload: windows in flight on separate handles for the value (1 = one at a time)")
     ap.add_argument("--copy-calibration", type=int, default=0,
@@ -612,7 +615,7 @@ def main():
     from livespikesorter_amd.metrics import f1_same_template
 
     t0 = time.time()
-    s = synth.make(2, seed=args.seed)
+    s = synth.make(args.config, seed=args.seed)
     a = s.art
     NT, nt, C, T = a["NT"], a["nt"], s.rec.shape[1], a["U"].shape[0]
     K = a["U"].shape[1]
@@ -706,8 +709,9 @@ def main():
         "warmup": args.warmup, "ms_per_step": round((fly or r)["elapsed"] / args.steps * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": args.precision,
         "data": "synthetic (seeded probe, templates and spikes; no recordings)",
-        "config": {"workload": "384 ch, 600 templates, 6 PCs, nt=61, NT=60000-sample Kilosort windows "
-                               "(W'=60122), 10 s recording cycled, <=50 pursuit passes",
+        "config": {"workload": f"{C} ch, {T} templates, {K} PCs, nt={nt}, NT={NT}-sample Kilosort windows "
+                               f"(W'={int(Wp)}), {s.rec.shape[0] / a['fs']:.0f} s recording cycled, <=50 pursuit passes",
+                   "synthetic_config": args.config,
                    "n_chan": C, "n_templates": T, "window": int(Wp), "NT": NT, "max_passes": 50,
                    "precision": args.precision},
         "p50_ms": round(float(np.percentile(r["lat_ms"], 50)), 3),
