@@ -346,7 +346,7 @@ lss_status create_impl(const lss_params* p, const lss_artifacts* a, Handle** out
         h->h_iCC = iCC;
         if (lss_status st = upload_extract(*h, a->U)) return st;
     }
-    HIP_TRY(h->tables_ws.alloc(lss::tables_workspace_doubles(T, K, nt)));
+    HIP_TRY(h->tables_ws.alloc(lss::tables_workspace_doubles(T, K, C, nt)));
     HIP_TRY(lss::launch_tables(h->U.p, h->ldq, h->wpca.p, T, K, C, nt, h->nm.p, h->waves.p, h->ldC, h->ctc.p, h->ldT,
                                h->tables_ws.p, h->stream));
 

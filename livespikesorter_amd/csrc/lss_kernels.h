@@ -168,8 +168,9 @@ hipError_t launch_refine_adjoint(const RefineArgs& a, hipStream_t s);
 // waves[t][m][c] = sum_k U[t][k][c] wpca[k][m] (FP64 accumulation), rows of stride ldC
 hipError_t launch_waves(const float* U, int64_t ldu, const float* wpca, int T, int K, int C, int nt,
                         float* waves, int64_t ldC, hipStream_t s);
-// ws: device workspace of tables_workspace_doubles(T, K, nt) doubles (the Gram matrix and WtW).
-size_t tables_workspace_doubles(int T, int K, int nt);
+// ws: device workspace of tables_workspace_doubles(T, K, C, nt) doubles (the Gram matrix, WtW
+// and the Gram GEMM's split-K partials).
+size_t tables_workspace_doubles(int T, int K, int C, int nt);
 hipError_t launch_tables(const float* U, int64_t ldu, const float* wpca, int T, int K, int C, int nt,
                          float* nm, float* waves, int64_t ldC, float* ctc, int64_t ldT, double* ws, hipStream_t s);
 

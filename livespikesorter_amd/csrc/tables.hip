@@ -106,8 +106,13 @@ hipError_t launch_waves(const float* U, int64_t ldu, const float* wpca, int T, i
     return hipGetLastError();
 }
 
-size_t tables_workspace_doubles(int T, int K, int nt) {
-    return (size_t)T * K * T * K + (size_t)K * K * (2 * nt + 1);
+// Gram matrix, WtW, and the split-K partials of the Gram GEMM
+size_t tables_workspace_doubles(int T, int K, int C, int nt) {
+    const int R = T * K;
+    int S;
+    int64_t kchunk;
+    gemm64_split(R, R, C, 1, &S, &kchunk);
+    return (size_t)(1 + S) * R * R + (size_t)K * K * (2 * nt + 1);
 }
 
 hipError_t launch_tables(const float* U, int64_t ldu, const float* wpca, int T, int K, int C, int nt,
@@ -138,10 +143,20 @@ hipError_t launch_tables(const float* U, int64_t ldu, const float* wpca, int T, 
     const int R = T * K;
     double* dG = ws;
     double* dW = ws + (size_t)R * R;
+    double* dP = dW + (size_t)K * K * L;
     e = hipMemcpyAsync(dW, hwtw, sizeof(double) * K * K * L, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) {
-        hipLaunchKernelGGL(gram_kernel, dim3((R + 15) / 16, (R + 15) / 16), dim3(256), 0, s, U, ldu, K, R, C, dG);
-        e = hipGetLastError();
+        int S;
+        int64_t kchunk;
+        gemm64_split(R, R, C, 1, &S, &kchunk);
+        if (ldu == (int64_t)K * C) {
+            // U rows are a plain [R][C] matrix: the symmetric FP64 GEMM (FP32 inputs widened exactly)
+            e = launch_gemm64_f32_nk(U, C, U, C, dP, R, R, C, S, kchunk, 1, s);
+            if (e == hipSuccess) e = launch_reduce64(dP, S, R, R, 1, 1.0, 0.0, 0.0, dG, R, s);
+        } else {
+            hipLaunchKernelGGL(gram_kernel, dim3((R + 15) / 16, (R + 15) / 16), dim3(256), 0, s, U, ldu, K, R, C, dG);
+            e = hipGetLastError();
+        }
     }
     if (e == hipSuccess) {
         switch (K) {
