@@ -419,7 +419,12 @@ lss_status run_whiten(Handle& h, const float* in_cw, float* out_wc, int64_t W) {
 }
 
 lss_status run_scores(Handle& h, const float* x_wc, float* b_wt, int64_t W, hipEvent_t mid = nullptr) {
-    HIP_TRY(lss::launch_project(x_wc, h.ldC, W, h.C, h.wpca.p, h.K, h.nt, h.xp.p, h.ldq, h.stream));
+    // the split-precision paths project on the matrix cores with the same split as their score GEMM
+    if (h.prm.precision == LSS_PREC_BF16X3 || h.prm.precision == LSS_PREC_FP16X3)
+        HIP_TRY(lss::launch_project_x3(h.prm.precision, x_wc, h.ldC, W, h.C, h.wpca.p, h.K, h.nt, h.xp.p, h.ldq,
+                                       h.stream));
+    else
+        HIP_TRY(lss::launch_project(x_wc, h.ldC, W, h.C, h.wpca.p, h.K, h.nt, h.xp.p, h.ldq, h.stream));
     if (mid) HIP_TRY(hipEventRecord(mid, h.stream));
     const int KC = h.K * h.C;
     if (h.prm.precision == LSS_PREC_FP32)

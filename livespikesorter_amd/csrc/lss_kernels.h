@@ -38,6 +38,9 @@ hipError_t launch_gemm_lp(int prec, const float* A, int64_t lda, int a_km, const
 // Xp rows ldp.  K <= 8, nt <= 127.
 hipError_t launch_project(const float* x_wc, int64_t ldx, int64_t W, int C, const float* wpca, int K,
                           int nt, float* xp, int64_t ldp, hipStream_t s);
+// The same product on the matrix cores with split operands (prec = LSS_PREC_BF16X3 / FP16X3).
+hipError_t launch_project_x3(int prec, const float* x_wc, int64_t ldx, int64_t W, int C, const float* wpca, int K,
+                             int nt, float* xp, int64_t ldp, hipStream_t s);
 
 // ---- pursuit (pursuit.hip) ------------------------------------------------------------------
 // Row strides ldT (B, ctc, nm) and ldC (X, waves) are multiples of 4 (float4 access); padding

@@ -6,6 +6,7 @@
 // inputs of its channel from LDS and does 8 x 8 x K FMAs against the (wavefront-uniform) wPCA
 // coefficients, so the kernel is FMA-bound rather than LDS-bound.  PW = 128 for long windows;
 // short windows (the streaming batches) use PW = 32 so that the grid still covers the device.
+#include "../../include/lss.h"
 #include "lss_kernels.h"
 
 namespace lss {
@@ -98,6 +99,129 @@ hipError_t launch_project(const float* x_wc, int64_t ldx, int64_t W, int C, cons
         LSS_PK(1) LSS_PK(2) LSS_PK(3) LSS_PK(4) LSS_PK(5) LSS_PK(6) LSS_PK(7) LSS_PK(8)
 #undef LSS_PK
     }
+    return hipErrorInvalidValue;
+}
+
+// ---- split-precision projection on the matrix cores ---------------------------------------------
+// For the bf16x3 / fp16x3 paths.  A workgroup covers 32 output samples x 128 channels with one
+// wavefront per PC k (64*K threads).  Per k the 32 x 128 output block is a banded product
+//   Xp[w0+s][k*C+c] = sum_j T_k[s][j] Xb[j][c],   T_k[s][j] = wPCA[k][j-s] (0 <= j-s < nt),
+//   Xb[j][c] = X[w0 - nt/2 + j][c] (zero outside [0, W)),   j < J = roundup(32 + nt - 1, 16),
+// on 32x32x16 MFMA with the split operands x = hi + lo and hi*hi + hi*lo + lo*hi accumulated in FP32
+// (the split of the score GEMM).  Each lane forms its A fragment (8 band entries) from wPCA in LDS
+// once per K step and uses it for the four 32-channel blocks; X is staged transposed (channel rows,
+// j contiguous) as hi/lo halves.
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef __bf16 pbf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 phalfx8 __attribute__((ext_vector_type(8)));
+
+constexpr int kPxS = 32;      // output samples per workgroup
+constexpr int kPxC = 128;     // channels per workgroup (4 blocks of 32)
+
+template <int PREC>
+struct PxType;
+template <>
+struct PxType<LSS_PREC_BF16X3> { typedef __bf16 T; typedef pbf16x8 V; };
+template <>
+struct PxType<LSS_PREC_FP16X3> { typedef _Float16 T; typedef phalfx8 V; };
+
+template <int PREC>
+__device__ __forceinline__ floatx16 px_mfma(typename PxType<PREC>::V a, typename PxType<PREC>::V b, floatx16 c) {
+    if constexpr (PREC == LSS_PREC_FP16X3) return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    else return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+}
+
+template <int PREC>
+__global__ __launch_bounds__(512) void project_x3_kernel(const float* __restrict__ x, int64_t ldx, int64_t W, int C,
+                                                         const float* __restrict__ wpca, int K, int nt,
+                                                         float* __restrict__ xp, int64_t ldp, int J) {
+    typedef typename PxType<PREC>::T T;
+    typedef typename PxType<PREC>::V V;
+    extern __shared__ __attribute__((aligned(16))) unsigned char px_smem[];
+    const int ldj = J + 8;                             // 16-byte aligned rows
+    T* Xh = (T*)px_smem;                               // [kPxC][ldj]
+    T* Xl = Xh + kPxC * ldj;
+    float* wp = (float*)(Xl + kPxC * ldj);             // [K][nt]
+    const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63, k = tid >> 6;
+    const int64_t w0 = (int64_t)blockIdx.x * kPxS;
+    const int c0 = blockIdx.y * kPxC;
+    const int h = nt / 2;
+    for (int e = tid; e < K * nt; e += nth) wp[e] = wpca[e];
+    for (int e = tid; e < kPxC * J; e += nth) {          // consecutive lanes read consecutive channels
+        const int cc = e % kPxC, jj = e / kPxC;
+        const int c = c0 + cc;
+        {
+            const int64_t w = w0 - h + jj;
+            const float v = (w >= 0 && w < W && c < C) ? x[w * ldx + c] : 0.f;
+            const T hi = (T)v;
+            Xh[cc * ldj + jj] = hi;
+            Xl[cc * ldj + jj] = (T)(v - (float)hi);
+        }
+    }
+    __syncthreads();
+    const int m = lane & 31, g = lane >> 5;
+    floatx16 acc[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[b][r] = 0.f;
+    const float* wk = wp + k * nt;
+    for (int kk = 0; kk < J; kk += 16) {
+        V ah, al;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int d = kk + 8 * g + e - m;                // band entry T_k[m][j], j = kk + 8g + e
+            const float a = (d >= 0 && d < nt) ? wk[d] : 0.f;
+            const T hi = (T)a;
+            ah[e] = hi;
+            al[e] = (T)(a - (float)hi);
+        }
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int row = (b * 32 + m) * ldj + kk + 8 * g;
+            const V bh = *(const V*)&Xh[row];
+            const V bl = *(const V*)&Xl[row];
+            acc[b] = px_mfma<PREC>(al, bh, acc[b]);
+            acc[b] = px_mfma<PREC>(ah, bl, acc[b]);
+            acc[b] = px_mfma<PREC>(ah, bh, acc[b]);
+        }
+    }
+    // acc[b][r] of lane l: sample 8*(r/4) + 4*(l/32) + r%4, channel c0 + 32b + l%32
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        const int c = c0 + b * 32 + m;
+        if (c >= C) continue;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int64_t w = w0 + 8 * (r >> 2) + 4 * g + (r & 3);
+            if (w < W) xp[w * ldp + (int64_t)k * C + c] = acc[b][r];
+        }
+    }
+}
+
+template <int PREC>
+static hipError_t project_x3_launch(const float* x_wc, int64_t ldx, int64_t W, int C, const float* wpca, int K, int nt,
+                                    float* xp, int64_t ldp, hipStream_t s) {
+    const int J = (kPxS + nt - 1 + 15) / 16 * 16;
+    const size_t lds = 2 * sizeof(typename PxType<PREC>::T) * (size_t)kPxC * (J + 8) + sizeof(float) * K * nt;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void*)project_x3_kernel<PREC>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    const dim3 grid((unsigned)((W + kPxS - 1) / kPxS), (C + kPxC - 1) / kPxC);
+    hipLaunchKernelGGL((project_x3_kernel<PREC>), grid, dim3(64 * K), lds, s, x_wc, ldx, W, C, wpca, K, nt, xp, ldp,
+                       J);
+    return hipGetLastError();
+}
+
+hipError_t launch_project_x3(int prec, const float* x_wc, int64_t ldx, int64_t W, int C, const float* wpca, int K,
+                             int nt, float* xp, int64_t ldp, hipStream_t s) {
+    if (K < 1 || K > kMaxK || nt < 1 || nt > kMaxNt) return hipErrorInvalidValue;
+    if (prec == LSS_PREC_BF16X3)
+        return project_x3_launch<LSS_PREC_BF16X3>(x_wc, ldx, W, C, wpca, K, nt, xp, ldp, s);
+    if (prec == LSS_PREC_FP16X3)
+        return project_x3_launch<LSS_PREC_FP16X3>(x_wc, ldx, W, C, wpca, K, nt, xp, ldp, s);
     return hipErrorInvalidValue;
 }
 
