@@ -517,10 +517,13 @@ lss_status enqueue_pre(Handle& h, const int16_t* d_win, int64_t W) {
     return run_scores(h, h.xw.p, h.B.p, W);
 }
 
-// Pinned, device-mapped, coherent host buffers that the staging kernel writes directly.
+// Pinned, device-mapped, coherent host buffers that the staging kernel writes directly.  Windows
+// with more spikes than this are read back by copies (emit_spikes).
+constexpr int kStageCap = 1 << 16;
+
 lss_status ensure_stage(Handle& h) {
     if (h.stage_cap) return LSS_OK;
-    const int cap = std::min(h.prm.max_spikes, 16384);
+    const int cap = std::min(h.prm.max_spikes, kStageCap);
     const int FK = h.nC * h.K;
     const unsigned fl = hipHostMallocMapped | hipHostMallocCoherent;
     HIP_TRY(hipHostMalloc((void**)&h.hs_counters, sizeof(int) * 8, fl));
@@ -557,16 +560,15 @@ const char* const kBarrierTimeout =
     "persistent pursuit: grid barrier timed out (workgroups not co-resident); results are invalid";
 
 // Spike count and pass count of the last pursuit (synchronises the stream).
-lss_status read_counts(Handle& h, int* n, int* passes) {
-    HIP_TRY(hipMemcpyAsync(h.h_counters, h.counters.p, sizeof(int) * 8, hipMemcpyDeviceToHost, h.stream));
-    HIP_TRY(hipStreamSynchronize(h.stream));
-    if (h.h_counters[7]) return fail(LSS_ERR_STATE, kBarrierTimeout);
-    if (h.h_counters[2]) return fail(LSS_ERR_CAPACITY, "spike capacity (max_spikes) exceeded");
-    if (h.h_counters[5])
+// The error flags and counts of a finished window, from a host copy c of the device counters.
+lss_status check_counts(Handle& h, const int* c, int* n, int* passes) {
+    if (c[7]) return fail(LSS_ERR_STATE, kBarrierTimeout);
+    if (c[2]) return fail(LSS_ERR_CAPACITY, "spike capacity (max_spikes) exceeded");
+    if (c[5])
         return fail(LSS_ERR_CAPACITY, "more than 1024 spikes of all passes reach one 64-sample block "
                                       "(residual list); the residual is incomplete");
-    *n = h.h_counters[0] + h.h_counters[1];
-    *passes = h.h_counters[4];
+    *n = c[0] + c[1];
+    *passes = c[4];
     if (h.prof_update) {
         h.upd_ms = 0.f;
         h.upd_launches = h.prm.max_passes;
@@ -577,6 +579,12 @@ lss_status read_counts(Handle& h, int* n, int* passes) {
         }
     }
     return LSS_OK;
+}
+
+lss_status read_counts(Handle& h, int* n, int* passes) {
+    HIP_TRY(hipMemcpyAsync(h.h_counters, h.counters.p, sizeof(int) * 8, hipMemcpyDeviceToHost, h.stream));
+    HIP_TRY(hipStreamSynchronize(h.stream));
+    return check_counts(h, h.h_counters, n, passes);
 }
 
 // Host-side emission of n device-order spikes from host copies of the spike arrays.
@@ -971,10 +979,16 @@ lss_status lss_sort_window(void* handle, const int16_t* window, int64_t W, int32
     HIP_TRY(hipEventRecord(h.ev[6], h.stream));
     if (lss_status st = run_features(h, h.xw.p, W)) return st;
     HIP_TRY(hipEventRecord(h.ev[7], h.stream));
+    // counters and spikes straight into pinned host memory, one synchronisation
+    if (lss_status st = ensure_stage(h)) return st;
+    if (lss_status st = enqueue_readback(h)) return st;
+    HIP_TRY(hipStreamSynchronize(h.stream));
     int n = 0, passes = 0;
-    if (lss_status st = read_counts(h, &n, &passes)) return st;
+    if (lss_status st = check_counts(h, h.hs_counters, &n, &passes)) return st;
     for (int i = 0; i < 7; ++i) HIP_TRY(hipEventElapsedTime(&h.last_ms[i], h.ev[i], h.ev[i + 1]));
     HIP_TRY(hipEventElapsedTime(&h.last_ms[7], h.ev[0], h.ev[7]));
+    if (n <= h.stage_cap)
+        return emit_from(h, n, passes, window_start, out, h.hs_col, h.hs_tmp, h.hs_amp, h.hs_tF, h.hs_xs, h.hs_ys);
     return emit_spikes(h, n, passes, window_start, out);
 }
 

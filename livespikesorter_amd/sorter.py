@@ -344,11 +344,13 @@ class Sorter:
         return dict(nm=nm, waves=waves, ctc=ctc, wmat=wmat)
 
 
-def sort_windows(sorters, windows, starts, capacity: int | None = None) -> list:
+def sort_windows(sorters, windows, starts, capacity: int | None = None, consume=None) -> list:
     """Windows with up to len(sorters) in flight: window i is sorted by sorters[i % len(sorters)]
     on that sorter's own host thread (ctypes releases the GIL during the library call), so one
     window's GEMMs overlap another's pursuit and each thread copies its spike lists out while the
-    GPU runs the others.  Results in window order, equal to sorting each window alone."""
+    GPU runs the others.  Results in window order, equal to sorting each window alone.
+    consume(i, spikes), if given, is called on the worker thread as window i finishes and its
+    return value is stored in place of the spike list (a writer that keeps nothing returns None)."""
     import threading
     k = len(sorters)
     if k == 0:
@@ -361,7 +363,8 @@ def sort_windows(sorters, windows, starts, capacity: int | None = None) -> list:
     def work(j):
         try:
             for i in range(j, len(windows), k):
-                res[i] = sorters[j].sort_window(windows[i], starts[i], capacity)
+                out = sorters[j].sort_window(windows[i], starts[i], capacity)
+                res[i] = consume(i, out) if consume is not None else out
         except Exception as e:          # re-raised in the caller
             errs.append(e)
 

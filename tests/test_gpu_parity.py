@@ -344,5 +344,11 @@ def test_sort_windows_in_flight_equals_one_at_a_time(sorter, synth1):
                 np.testing.assert_array_equal(g.template, r.template)
                 np.testing.assert_array_equal(g.amplitude, r.amplitude)
                 np.testing.assert_array_equal(g.features, r.features)
+        # a consumer: called once per window with that window's spikes, its results kept in order
+        seen = []
+        got = sort_windows([sorter, s2], [wins[i] for i in order], [starts[i] for i in order],
+                           consume=lambda i, o: (seen.append(i), len(o))[1])
+        assert sorted(seen) == list(range(len(order)))
+        assert got == [len(r) for r in ref]
         with pytest.raises(LssError):
             sort_windows_native([sorter, sorter], wins, starts)
