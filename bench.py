@@ -666,7 +666,8 @@ def main():
     per_launch_bytes = spikes_p * bytes_per_spike / max(upd_launches, 1)
     avg_launch_ms = rp["upd_ms"] / max(upd_launches, 1)
     achieved = per_launch_bytes / (avg_launch_ms * 1e-3) / 1e9
-    traffic = read_traffic()
+    # the committed counter measurement covers the 384-ch / 600-template workload only
+    traffic = read_traffic() if (C, T) == (384, 600) else None
     roofline = {"kernel": "update_kernel (pursuit score update)", "bound": "hbm",
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
