@@ -1,0 +1,95 @@
+"""The drop-in configuration at full size: 384 channels, 600 templates, 6 PCs, nt = 61, one
+interior NT = 60000 Kilosort window, at most 50 pursuit passes.
+
+* FP32 device path against the FP32 CPU oracle on the same window: spike F1 >= 0.999 (+-1 sample,
+  same template), amplitudes within 1e-3 relative over the matched spikes, PC features within
+  1e-3 of their scale, positions within 0.05 um.
+* bf16x3 (the split-precision GEMMs) against the FP32 device path: spike F1 >= 0.999 and
+  amplitudes within 1e-3 relative.
+* Size-independent properties: samples inside the padded window, peak columns off the nt zeroed
+  score columns at each edge, templates in [0, T), at most 50 passes, and the same spike list
+  when the window is sorted twice (the handle carries no state from one window to the next)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import ks_oracle as ko
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+IB = 1
+
+
+@pytest.fixture(scope="module")
+def full():
+    from livespikesorter_amd import synth
+    print("[lss] generating the 384-ch / 600-template recording", flush=True)
+    s = synth.make(2, seed=3)
+    a = s.art
+    X = ko.padded_batch(s.rec, IB, a["NT"], a["nt"], torch.float32)
+    win = np.ascontiguousarray(X.numpy().T.astype(np.int16))
+    return s, X, win, ko.window_start(IB, a["NT"], a["nt"])
+
+
+@pytest.fixture(scope="module")
+def fp32_result(full):
+    from livespikesorter_amd.sorter import Sorter
+    s, _, win, ws = full
+    with Sorter(s.art, max_window=win.shape[0], max_passes=50, max_spikes=1 << 16) as so:
+        got = so.sort_window(torch.from_numpy(win).to(DEV), ws)
+        again = so.sort_window(torch.from_numpy(win).to(DEV), ws)
+    return got, again
+
+
+def _index(samples, templates):
+    return {(int(t), int(y)): i for i, (t, y) in enumerate(zip(samples, templates))}
+
+
+def test_fullsize_fp32_vs_cpu_oracle(full, fp32_result):
+    print("[lss] test_fullsize_fp32_vs_cpu_oracle", flush=True)
+    s, X, _, ws = full
+    got, _ = fp32_result
+    ref = ko.sort_window(X, s.art, ws, dtype=torch.float32, max_passes=50)
+    assert len(ref.samples) > 5000
+    f1 = ko.spike_f1(ref.samples, ref.templates, got.sample, got.template, dt=1)
+    assert f1 >= 0.999, (f1, len(ref.samples), len(got))
+    ri = _index(ref.samples, ref.templates)
+    pairs = [(ri[(int(t), int(y))], j) for j, (t, y) in enumerate(zip(got.sample, got.template))
+             if (int(t), int(y)) in ri]
+    assert len(pairs) >= 0.99 * len(ref.samples)
+    r, g = np.array([p[0] for p in pairs]), np.array([p[1] for p in pairs])
+    amp_err = np.abs(got.amplitude[g] - ref.amps[r]) / np.abs(ref.amps[r])
+    assert amp_err.max() <= 1e-3, amp_err.max()
+    F_ref = ref.tF[r]
+    assert np.abs(got.features[g] - F_ref).max() <= 1e-3 * np.abs(F_ref).max()
+    assert np.abs(got.y[g] - ref.ys[r]).max() <= 0.05
+
+
+def test_fullsize_bf16x3_vs_fp32(full, fp32_result):
+    from livespikesorter_amd.sorter import Sorter
+    print("[lss] test_fullsize_bf16x3_vs_fp32", flush=True)
+    s, _, win, ws = full
+    ref, _ = fp32_result
+    with Sorter(s.art, max_window=win.shape[0], max_passes=50, max_spikes=1 << 16, precision="bf16x3") as so:
+        got = so.sort_window(torch.from_numpy(win).to(DEV), ws)
+    f1 = ko.spike_f1(ref.sample, ref.template, got.sample, got.template, dt=1)
+    assert f1 >= 0.999, f1
+    ri = _index(ref.sample, ref.template)
+    errs = [abs(a - ref.amplitude[ri[(int(t), int(y))]]) / abs(ref.amplitude[ri[(int(t), int(y))]])
+            for t, y, a in zip(got.sample, got.template, got.amplitude) if (int(t), int(y)) in ri]
+    assert len(errs) >= 0.99 * len(ref) and max(errs) <= 1e-3
+
+
+def test_fullsize_properties(full, fp32_result):
+    print("[lss] test_fullsize_properties", flush=True)
+    s, _, _, _ = full
+    got, again = fp32_result
+    NT, T = s.art["NT"], s.art["U"].shape[0]
+    assert len(got) > 5000 and 0 < got.passes <= 50
+    nt = s.art["nt"]
+    assert got.sample.min() >= IB * NT - nt and got.sample.max() < (IB + 1) * NT + nt
+    assert np.all(got.column >= nt) and np.all(got.column < NT + nt)   # score edges are zeroed
+    assert got.template.min() >= 0 and got.template.max() < T
+    for k in ("sample", "template", "amplitude", "features", "x", "y"):
+        np.testing.assert_array_equal(getattr(got, k), getattr(again, k))
