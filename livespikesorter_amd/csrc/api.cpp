@@ -526,13 +526,15 @@ lss_status ensure_stage(Handle& h) {
     const int cap = std::min(h.prm.max_spikes, kStageCap);
     const int FK = h.nC * h.K;
     const unsigned fl = hipHostMallocMapped | hipHostMallocCoherent;
-    HIP_TRY(hipHostMalloc((void**)&h.hs_counters, sizeof(int) * 8, fl));
-    HIP_TRY(hipHostMalloc((void**)&h.hs_col, sizeof(int) * cap, fl));
-    HIP_TRY(hipHostMalloc((void**)&h.hs_tmp, sizeof(int) * cap, fl));
-    HIP_TRY(hipHostMalloc((void**)&h.hs_amp, sizeof(float) * cap, fl));
-    HIP_TRY(hipHostMalloc((void**)&h.hs_tF, sizeof(float) * cap * FK, fl));
-    HIP_TRY(hipHostMalloc((void**)&h.hs_xs, sizeof(float) * cap, fl));
-    HIP_TRY(hipHostMalloc((void**)&h.hs_ys, sizeof(float) * cap, fl));
+    // buffers left by an earlier attempt that failed part way are kept (the destructor frees them)
+    auto pin = [&](void** p, size_t bytes) { return *p ? hipSuccess : hipHostMalloc(p, bytes, fl); };
+    HIP_TRY(pin((void**)&h.hs_counters, sizeof(int) * 8));
+    HIP_TRY(pin((void**)&h.hs_col, sizeof(int) * cap));
+    HIP_TRY(pin((void**)&h.hs_tmp, sizeof(int) * cap));
+    HIP_TRY(pin((void**)&h.hs_amp, sizeof(float) * cap));
+    HIP_TRY(pin((void**)&h.hs_tF, sizeof(float) * (size_t)cap * FK));
+    HIP_TRY(pin((void**)&h.hs_xs, sizeof(float) * cap));
+    HIP_TRY(pin((void**)&h.hs_ys, sizeof(float) * cap));
     lss::StageArgs& g = h.stage;
     g.counters = h.counters.p;
     g.col = h.sp_col.p; g.tmp = h.sp_tmp.p; g.amp = h.sp_amp.p; g.tF = h.tF.p; g.xs = h.xs.p; g.ys = h.ys.p;
