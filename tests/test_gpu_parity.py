@@ -2,7 +2,7 @@
 
 Tolerances (written where they are checked):
 * CAR output: max |gpu - fp64| <= 1e-6 of the window's max |x| (exact mean/median up to FP32 rounding);
-* highpass, whitening, scores: max |gpu - fp64| <= 1e-5 of the max |reference| (north-star bound);
+* highpass, whitening, scores: max |gpu - fp64| <= 1e-5 of the max |reference|;
 * pursuit given equal scores: bit-exact spike columns, templates, amplitudes, final B and residual;
 * PC features / positions: max |gpu - oracle| <= 1e-4 of the max |oracle feature|;
 * whole window: spike F1 >= 0.999 within +-1 sample and same template, matched amplitudes within
