@@ -186,6 +186,23 @@ lss_status lss_write_spikes_txt(const char* path, const int64_t* sample, const i
                                 const float* amplitude, const float* y, int64_t n,
                                 int64_t recording_offset, int32_t append);
 
+/* ---- feature-space cluster assignment of the reference's Python online sorter
+ * (src/Python/OnlineKilosort.py:211-284); device pointers, synchronous ---------------------------- */
+
+/* d_out [T][n_nearest * n_pcs]: each template's PC features on its n_nearest channels (row t =
+ * Ucc[:, t, :] flattened channel-major), the cluster centroids of cluster_centroids_pca_compute
+ * (OnlineKilosort.py:211-231) with pc_feature_ind[t] = the template's nearest channels. */
+lss_status lss_cluster_centroids_pca(void* handle, float* d_out);
+
+/* assign_to_clusters (OnlineKilosort.py:233-284): d_label[i] = the cluster k maximising
+ * 2 <tF[i], mu[k]> - |mu[k]|^2 among those with (x[i] - cx[k])^2 + (y[i] - cy[k])^2 <= R^2 (NaN
+ * centroids never qualify), the first such k on ties, -1 when none is in range.  tF [n][D]
+ * (a spike's [n_nearest][n_pcs] features flattened), mu [n_clusters][D], D <= 256; runs on
+ * `stream` (a hipStream_t, NULL = the null stream) and waits for it. */
+lss_status lss_assign_clusters(const float* d_tF, int64_t n, int32_t D, const float* d_mu, int32_t n_clusters,
+                               const float* d_x, const float* d_y, const float* d_cx, const float* d_cy, double R,
+                               int32_t* d_label, void* stream);
+
 /* ---- multi-probe streaming (OnlineSpikesV2::runSpikeSorting's batching loop, V2:632-820, for
  * several probes on one sample clock) ------------------------------------------------------------ */
 

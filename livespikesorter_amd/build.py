@@ -19,7 +19,7 @@ LIB = PKG / "liblss.so"
 ARCH = os.environ.get("LSS_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["api.cpp", "npy.cpp", "output.cpp", "preprocess.hip", "gemm.hip", "project.hip",
-           "pursuit.hip", "tables.hip", "adjoint.hip", "train.hip", "train_api.cpp"]
+           "pursuit.hip", "tables.hip", "adjoint.hip", "train.hip", "train_api.cpp", "assign.hip"]
 
 # Packed FP32 instructions are disabled for all device code.
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]

@@ -907,6 +907,14 @@ lss_status lss_destroy(void* handle) {
     return LSS_OK;
 }
 
+lss_status lss_cluster_centroids_pca(void* handle, float* d_out) {
+    if (!handle || !d_out) return fail(LSS_ERR_ARG, "null argument");
+    Handle& h = *as_handle(handle);
+    HIP_TRY(lss::launch_cluster_centroids(h.Ucc.p, h.T, h.nC, h.K, d_out, h.stream));
+    HIP_TRY(hipStreamSynchronize(h.stream));
+    return LSS_OK;
+}
+
 lss_status lss_get_params(void* handle, lss_params* p) {
     if (!handle || !p) return fail(LSS_ERR_ARG, "null argument");
     *p = as_handle(handle)->prm;

@@ -216,4 +216,10 @@ hipError_t launch_emit_snippets(const unsigned* iso, int64_t nwords, int64_t nw,
 // Q[m][i] = clips[i][m] / ||clips[i]|| (FP64)
 hipError_t launch_normalize_clips(const float* clips, int64_t n, int nt, double* Q, hipStream_t s);
 
+// ---- feature-space cluster assignment (assign.hip) ----
+hipError_t launch_assign_clusters(const float* tF, int64_t n, int D, const float* mu, int K, const float* xs,
+                                  const float* ys, const float* cx, const float* cy, float R2, int* label,
+                                  hipStream_t s);
+hipError_t launch_cluster_centroids(const float* Ucc, int T, int nC, int K, float* out, hipStream_t s);
+
 }  // namespace lss

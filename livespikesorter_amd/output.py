@@ -1,6 +1,8 @@
-"""Output step of the sorter (host side, through liblss): nearest-centroid labels, duplicate
-removal with state carried across windows, and the reference's spikeOutput.txt writer
-(src/SpikeSorter/OnlineSpikesV2.cu:1028-1043, 1117-1164; kilosort/postprocessing.py:10-30)."""
+"""Output step of the sorter (through liblss): nearest-centroid labels, duplicate removal with
+state carried across windows, and the reference's spikeOutput.txt writer
+(src/SpikeSorter/OnlineSpikesV2.cu:1028-1043, 1117-1164; kilosort/postprocessing.py:10-30); and
+the feature-space cluster assignment of the reference's Python online sorter on the device
+(src/Python/OnlineKilosort.py:211-284)."""
 from __future__ import annotations
 
 import ctypes as ct
@@ -51,3 +53,50 @@ def write_spikes_txt(path, sample, label, amplitude, y, recording_offset: int = 
     check(lib().lss_write_spikes_txt(str(path).encode(), s.ctypes.data, l.ctypes.data, a.ctypes.data,
                                      yy.ctypes.data, len(s), int(recording_offset), int(append)),
           "lss_write_spikes_txt")
+
+
+def cluster_centroids_pca(sorter):
+    """[T][n_nearest * n_pcs] float32 device tensor: each template's PC features on its nearest
+    channels, the cluster centroids of cluster_centroids_pca_compute
+    (src/Python/OnlineKilosort.py:211-231) with pc_feature_ind[t] = the template's nearest
+    channels (Kilosort's iCC[:, iU[t]])."""
+    import torch
+    out = torch.empty((sorter.T, sorter.nC * sorter.K), dtype=torch.float32, device="cuda")
+    check(lib().lss_cluster_centroids_pca(sorter._h, out.data_ptr()), "lss_cluster_centroids_pca")
+    return out
+
+
+def assign_to_clusters(tF, centroids_pca, spike_xs, spike_ys, cluster_xy, R, device="cuda"):
+    """Cluster of each spike in PC-feature space (src/Python/OnlineKilosort.py:233-284), same
+    arguments: tF [N][nC][D] features, centroids_pca [K][nC*D], spike positions, cluster_xy
+    {cluster index: (x, y)} (clusters absent from it never qualify), R the distance limit in um.
+    The best of 2 <tF, mu_k> - |mu_k|^2 over clusters within R of the spike, first on ties; -1
+    where none is within R.  Runs on the device (lss_assign_clusters); returns int64 labels."""
+    import torch
+    if len(tF) == 0:
+        return np.zeros(0, np.int64)
+    dev = torch.device(device)
+    X = torch.as_tensor(np.asarray(tF, np.float32) if not torch.is_tensor(tF) else tF).to(dev, torch.float32)
+    X = X.reshape(X.shape[0], -1).contiguous()
+    mu = torch.as_tensor(np.asarray(centroids_pca, np.float32) if not torch.is_tensor(centroids_pca)
+                         else centroids_pca).to(dev, torch.float32).contiguous()
+    n, D = X.shape
+    K = mu.shape[0]
+    if mu.ndim != 2 or mu.shape[1] != D:
+        raise ValueError("centroids_pca must be [K][nC*D] with the features' width")
+    cx = torch.full((K,), float("nan"), dtype=torch.float32)
+    cy = torch.full((K,), float("nan"), dtype=torch.float32)
+    for idx, (x, y) in cluster_xy.items():
+        cx[idx] = float(x)
+        cy[idx] = float(y)
+    cx, cy = cx.to(dev), cy.to(dev)
+    xs = torch.as_tensor(np.asarray(spike_xs, np.float32)).to(dev).contiguous()
+    ys = torch.as_tensor(np.asarray(spike_ys, np.float32)).to(dev).contiguous()
+    if xs.numel() != n or ys.numel() != n:
+        raise ValueError("one position per spike")
+    label = torch.empty(n, dtype=torch.int32, device=dev)
+    stream = torch.cuda.current_stream(dev).cuda_stream    # after the copies above, then waited for
+    check(lib().lss_assign_clusters(X.data_ptr(), n, D, mu.data_ptr(), K, xs.data_ptr(), ys.data_ptr(),
+                                    cx.data_ptr(), cy.data_ptr(), float(R), label.data_ptr(), stream),
+          "lss_assign_clusters")
+    return label.cpu().numpy().astype(np.int64)

@@ -7,7 +7,7 @@ FP64, chosen per call), the algorithm of the Kilosort 4.0.13 copy vendored by Li
 function cites the file:line it follows.  Nothing here imports or runs the reference.
 
 Pinning (what checks each function independently of the restatement itself; the tests named
-are under ``tests/test_oracle_*.py``):
+are in ``tests/test_oracle.py``):
 
 * ``padded_batch``        - brute-force edge replication on tiny recordings (first/middle/last
                             batch, single batch).
@@ -25,6 +25,11 @@ are under ``tests/test_oracle_*.py``):
 * ``extract_features``    - brute force on tiny inputs.
 * ``spike_positions``     - closed form (all mass on one channel gives that channel's position).
 * ``nmatch``              - hand-checked lists.
+* ``cluster_centroids_pca`` - the reference layout (Wall [T][C][P], pc_feature_ind [T][nC]) gives
+                            the same rows as Ucc of ``prepare_extract``.
+* ``assign_to_clusters``  - 2<x,mu> - |mu|^2 = |x|^2 - |x-mu|^2: the label is the nearest centroid
+                            in feature space among clusters within R (brute force with
+                            ``scipy.spatial.distance.cdist``); no cluster in range gives -1.
 * ``residual_energy``     - equals |Xres|^2 of ``run_matching`` for the pursuit's own spikes and
                             amplitudes; its autograd gradient against FP64 central differences
                             (the loss is quadratic in U and in wPCA, so central differences are
@@ -424,3 +429,37 @@ def sort_window(Xraw: torch.Tensor, art: dict, wstart: int, dtype=torch.float64,
         torch.zeros(0), torch.zeros(0))
     return WindowResult(samples.numpy(), st[:, 1].numpy(), amps[:, 0].numpy(), tF.numpy(),
                         xs.numpy(), ys.numpy(), X, m)
+
+
+# ----------------------------------------------------------------------------------------------
+# feature-space cluster assignment of LSS's Python online sorter
+# (src/Python/OnlineKilosort.py:211-284)
+# ----------------------------------------------------------------------------------------------
+def cluster_centroids_pca(Wall: np.ndarray, pc_feature_ind: np.ndarray) -> np.ndarray:
+    """centroids[t] = Wall[t, pc_feature_ind[t], :] flattened (OnlineKilosort.py:211-231);
+    Wall [T][C][P], pc_feature_ind [T][nC] -> [T][nC*P]."""
+    T = Wall.shape[0]
+    return np.stack([Wall[t, pc_feature_ind[t], :].reshape(-1) for t in range(T)])
+
+
+def assign_to_clusters(tF: np.ndarray, centroids_pca: np.ndarray, xs, ys, cluster_xy: dict, R: float,
+                       dtype=np.float64) -> np.ndarray:
+    """OnlineKilosort.py:233-284 in ``dtype``: scores 2 X mu^T - |mu|^2 (X = tF flattened per
+    spike), clusters farther than R from the spike (or without a position) masked to -inf, first
+    argmax, -1 where no cluster is in range."""
+    X = np.asarray(tF, dtype).reshape(len(tF), -1)
+    mu = np.asarray(centroids_pca, dtype)
+    scores = 2 * (X @ mu.T) - (mu ** 2).sum(1)
+    K = mu.shape[0]
+    cx = np.full(K, np.nan, dtype)
+    cy = np.full(K, np.nan, dtype)
+    for k, (x, y) in cluster_xy.items():
+        cx[k], cy[k] = x, y
+    xs = np.asarray(xs, dtype)
+    ys = np.asarray(ys, dtype)
+    with np.errstate(invalid="ignore"):
+        mask = (xs[:, None] - cx[None, :]) ** 2 + (ys[:, None] - cy[None, :]) ** 2 <= dtype(R) ** 2
+    scores = np.where(mask, scores, -np.inf)
+    best = np.argmax(scores, axis=1)
+    best[~mask.any(axis=1)] = -1
+    return best.astype(np.int64)

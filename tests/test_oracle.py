@@ -378,3 +378,46 @@ def test_residual_energy_gradient_vs_central_differences():
         fd = (ko.residual_energy(X, col, tmp, amp, U, Wp + e)[0] -
               ko.residual_energy(X, col, tmp, amp, U, Wp - e)[0]) / (2 * h)
         assert abs(float(fd) - float(gW.reshape(-1)[flat])) <= 1e-6 * (1 + abs(float(fd)))
+
+
+# ---- feature-space cluster assignment (src/Python/OnlineKilosort.py:211-284) -----------------
+def test_cluster_centroids_are_ucc_rows():
+    """The reference's centroid gather on its file layout (Wall [T][C][P], pc_feature_ind [T][nC]
+    = each template's nearest channels) gives the rows of Ucc."""
+    rng = np.random.default_rng(11)
+    C, T, K, nC = 12, 5, 3, 4
+    xc = np.tile([0.0, 32.0], C // 2)
+    yc = np.repeat(np.arange(C // 2) * 20.0, 2)
+    U = torch.from_numpy(rng.normal(size=(T, K, C)))
+    iCC, iU, Ucc = ko.prepare_extract(xc, yc, U, nC)
+    Wall = U.numpy().transpose(0, 2, 1)                   # [T][C][P]
+    pcfi = np.stack([iCC[:, int(iU[t])].numpy() for t in range(T)])
+    cent = ko.cluster_centroids_pca(Wall, pcfi)
+    for t in range(T):
+        np.testing.assert_array_equal(cent[t], Ucc[:, t, :].numpy().reshape(-1))
+
+
+def test_assign_to_clusters_is_nearest_centroid_within_R():
+    """2<x, mu> - |mu|^2 = |x|^2 - |x - mu|^2, so the label is the nearest centroid in feature
+    space among the clusters within R of the spike (brute force with cdist), -1 when none is."""
+    from scipy.spatial.distance import cdist
+    rng = np.random.default_rng(12)
+    N, K, nC, P = 300, 40, 4, 3
+    tF = rng.normal(size=(N, nC, P))
+    mu = rng.normal(size=(K, nC * P))
+    cxy = rng.uniform(0, 400, size=(K, 2))
+    cluster_xy = {k: (cxy[k, 0], cxy[k, 1]) for k in range(K) if k % 7 != 3}   # some without a position
+    xs, ys = rng.uniform(-100, 500, N), rng.uniform(-100, 500, N)
+    R = 80.0
+    got = ko.assign_to_clusters(tF, mu, xs, ys, cluster_xy, R)
+    d_feat = cdist(tF.reshape(N, -1), mu, "sqeuclidean")
+    n_none = 0
+    for i in range(N):
+        ok = [k for k in cluster_xy if (xs[i] - cxy[k, 0]) ** 2 + (ys[i] - cxy[k, 1]) ** 2 <= R * R]
+        if not ok:
+            assert got[i] == -1
+            n_none += 1
+        else:
+            assert got[i] == min(ok, key=lambda k: (d_feat[i, k], k))
+    assert 0 < n_none < N                                  # both branches exercised
+    assert not np.isin(got, [k for k in range(K) if k % 7 == 3]).any()
