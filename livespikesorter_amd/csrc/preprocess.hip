@@ -204,6 +204,12 @@ hipError_t launch_car_transpose(const int16_t* win, int64_t W, int C, const long
 // ------------------------------------------------------------------------------------------------
 constexpr int kFftThreads = 512;
 
+// LDS index of FFT element i: 3 pad slots per 32 elements, so that the strided stores of the
+// first radix-8 passes (stride 8 and 64 elements) spread over the banks (bank-conflict count of
+// the five passes' loads and stores 40 -> 22, the minimum being 20).  71.7 KB per transform.
+__device__ __forceinline__ int fpad(int i) { return i + 3 * (i >> 5); }
+constexpr int kFftLds = 8192 + 3 * (8192 >> 5);
+
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
     return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -237,29 +243,42 @@ __device__ __forceinline__ void dft8(float2* v) {
 }
 
 // One Stockham pass of radix R over the LDS buffer (in place: read all, barrier, write all).
-template <int R>
-__device__ __forceinline__ void stockham_pass(float2* buf, int Ns, const float2* __restrict__ tw) {
+// Ns (the size of the sub-transforms already done) is a compile-time constant: the first pass
+// (Ns = 1) has no twiddles, and a radix-8 pass loads only w, w^2 and w^4 from the table and forms
+// w^3, w^5, w^6 and w^7 by complex products (at most two roundings from the table values).
+template <int R, int Ns>
+__device__ __forceinline__ void stockham_pass(float2* buf, const float2* __restrict__ tw) {
     constexpr int N = kFftN;
     constexpr int NB = N / R;                     // butterflies
     constexpr int PER = NB / kFftThreads;         // butterflies per thread
     float2 v[PER][R];
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
-        int j = threadIdx.x + p * kFftThreads;
-        int k = j % Ns;
-        int step = (N / (Ns * R)) * k;            // twiddle index multiplier
+        const int j = threadIdx.x + p * kFftThreads;
 #pragma unroll
-        for (int r = 0; r < R; ++r) {
-            float2 x = buf[j + r * NB];
-            if (r) x = cmul(x, tw[(r * step) & (N - 1)]);
-            v[p][r] = x;
+        for (int r = 0; r < R; ++r) v[p][r] = buf[fpad(j) + r * NB + 3 * ((r * NB) >> 5)];   // NB % 32 == 0
+        if constexpr (Ns > 1) {
+            const int step = (N / (Ns * R)) * (j % Ns);   // twiddle index multiplier
+            if constexpr (R == 8) {
+                const float2 w1 = tw[step], w2 = tw[2 * step], w4 = tw[4 * step];
+                const float2 w3 = cmul(w1, w2);
+                v[p][1] = cmul(v[p][1], w1);
+                v[p][2] = cmul(v[p][2], w2);
+                v[p][3] = cmul(v[p][3], w3);
+                v[p][4] = cmul(v[p][4], w4);
+                v[p][5] = cmul(v[p][5], cmul(w4, w1));
+                v[p][6] = cmul(v[p][6], cmul(w4, w2));
+                v[p][7] = cmul(v[p][7], cmul(w4, w3));
+            } else {
+                v[p][1] = cmul(v[p][1], tw[step]);
+            }
         }
     }
     __syncthreads();
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
-        int j = threadIdx.x + p * kFftThreads;
-        int k = j % Ns;
+        const int j = threadIdx.x + p * kFftThreads;
+        const int k = j % Ns;
         if constexpr (R == 8) {
             dft8(v[p]);
         } else {
@@ -267,32 +286,44 @@ __device__ __forceinline__ void stockham_pass(float2* buf, int Ns, const float2*
             v[p][0] = cadd(a, b);
             v[p][1] = csub(a, b);
         }
-        int base = (j / Ns) * Ns * R + k;
+        const int base = (j / Ns) * Ns * R + k;
+        // fpad(base + r*Ns) with a constant offset per r: base is a multiple of 8 for Ns = 1, has
+        // base % 32 < 8 for Ns = 8, and Ns is a multiple of 32 otherwise
+        const int pb = fpad(base);
 #pragma unroll
-        for (int r = 0; r < R; ++r) buf[base + r * Ns] = v[p][r];
+        for (int r = 0; r < R; ++r) {
+            const int off = (Ns == 1) ? r : (Ns == 8) ? 8 * r + 3 * (r >> 2) : r * Ns + 3 * ((r * Ns) >> 5);
+            buf[pb + off] = v[p][r];
+        }
     }
     __syncthreads();
 }
 
 __device__ __forceinline__ void fft8192(float2* buf, const float2* __restrict__ tw) {
-    stockham_pass<8>(buf, 1, tw);
-    stockham_pass<8>(buf, 8, tw);
-    stockham_pass<8>(buf, 64, tw);
-    stockham_pass<8>(buf, 512, tw);
-    stockham_pass<2>(buf, 4096, tw);
+    stockham_pass<8, 1>(buf, tw);
+    stockham_pass<8, 8>(buf, tw);
+    stockham_pass<8, 64>(buf, tw);
+    stockham_pass<8, 512>(buf, tw);
+    stockham_pass<2, 4096>(buf, tw);
 }
 
 __global__ __launch_bounds__(kFftThreads) void fft_rows_kernel(float2* data, const float2* __restrict__ tw) {
-    __shared__ float2 buf[kFftN];
+    extern __shared__ float2 fft_buf[];   // [kFftLds]
+    float2* buf = fft_buf;
     float2* row = data + (int64_t)blockIdx.x * kFftN;
-    for (int i = threadIdx.x; i < kFftN; i += kFftThreads) buf[i] = row[i];
+    for (int i = threadIdx.x; i < kFftN; i += kFftThreads) buf[fpad(i)] = row[i];
     __syncthreads();
     fft8192(buf, tw);
-    for (int i = threadIdx.x; i < kFftN; i += kFftThreads) row[i] = buf[i];
+    for (int i = threadIdx.x; i < kFftN; i += kFftThreads) row[i] = buf[fpad(i)];
+}
+
+static hipError_t fft_lds_attr(const void* kernel) {
+    return hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(sizeof(float2) * kFftLds));
 }
 
 hipError_t launch_fft_rows(float2* data, int nb, const float2* tw, hipStream_t s) {
-    hipLaunchKernelGGL(fft_rows_kernel, dim3(nb), dim3(kFftThreads), 0, s, data, tw);
+    if (hipError_t e = fft_lds_attr((const void*)fft_rows_kernel)) return e;
+    hipLaunchKernelGGL(fft_rows_kernel, dim3(nb), dim3(kFftThreads), sizeof(float2) * kFftLds, s, data, tw);
     return hipGetLastError();
 }
 
@@ -302,7 +333,8 @@ __global__ __launch_bounds__(kFftThreads) void highpass_kernel(const float* __re
                                                                int C, int64_t W, int Llo, int Lhi,
                                                                const float2* __restrict__ spec,
                                                                const float2* __restrict__ tw) {
-    __shared__ float2 buf[kFftN];
+    extern __shared__ float2 fft_buf[];   // [kFftLds]
+    float2* buf = fft_buf;
     const int V = kFftN - Llo - Lhi;
     const int64_t o0 = (int64_t)blockIdx.x * V;
     const int ca = 2 * blockIdx.y;
@@ -322,14 +354,14 @@ __global__ __launch_bounds__(kFftThreads) void highpass_kernel(const float* __re
             idx -= Wi;
             if (idx >= Wi) idx %= Wi;
         }
-        buf[n] = make_float2(xa[idx], hasb ? xb[idx] : 0.f);
+        buf[fpad(n)] = make_float2(xa[idx], hasb ? xb[idx] : 0.f);
     }
     __syncthreads();
     fft8192(buf, tw);
     // Z = conj(S * H); the forward FFT of Z conjugated is the (scaled) inverse transform
     for (int n = threadIdx.x; n < kFftN; n += kFftThreads) {
-        float2 z = cmul(buf[n], spec[n]);
-        buf[n] = make_float2(z.x, -z.y);
+        float2 z = cmul(buf[fpad(n)], spec[n]);
+        buf[fpad(n)] = make_float2(z.x, -z.y);
     }
     __syncthreads();
     fft8192(buf, tw);
@@ -338,7 +370,7 @@ __global__ __launch_bounds__(kFftThreads) void highpass_kernel(const float* __re
     for (int i = threadIdx.x; i < V; i += kFftThreads) {
         int64_t o = o0 + i;
         if (o >= W) break;
-        float2 r = buf[Llo + i];
+        float2 r = buf[fpad(Llo + i)];
         ya[o] = r.x;
         if (hasb) yb[o] = -r.y;
     }
@@ -349,8 +381,9 @@ hipError_t launch_highpass(const float* x, int64_t ldx, float* y, int64_t ldy, i
     int V = kFftN - Llo - Lhi;
     if (V <= 0) return hipErrorInvalidValue;
     int nseg = (int)((W + V - 1) / V);
-    hipLaunchKernelGGL(highpass_kernel, dim3(nseg, (C + 1) / 2), dim3(kFftThreads), 0, s, x, ldx, y,
-                       ldy, C, W, Llo, Lhi, spec, tw);
+    if (hipError_t e = fft_lds_attr((const void*)highpass_kernel)) return e;
+    hipLaunchKernelGGL(highpass_kernel, dim3(nseg, (C + 1) / 2), dim3(kFftThreads), sizeof(float2) * kFftLds, s, x,
+                       ldx, y, ldy, C, W, Llo, Lhi, spec, tw);
     return hipGetLastError();
 }
 
