@@ -58,3 +58,20 @@ def test_errors_are_reported_not_thrown():
     assert st == 1 and b"ibatch" in L.lss_last_error()
     assert L.lss_destroy(None) == 1
     assert L.lss_abi_version() == 1
+
+
+def test_argument_checks_before_any_device_work():
+    """Entry points taking device pointers or handles reject bad arguments with LSS_ERR_ARG and a
+    message, before any HIP call (so these run without a GPU)."""
+    from livespikesorter_amd import _lib
+    L = _lib.lib()
+    p = ct.c_void_p(16)                     # never dereferenced: the checks fail first
+    # lss_assign_clusters: negative n, D out of range, no clusters, non-finite R, null labels
+    cases = [(-1, 6, 4, 80.0, p), (10, 0, 4, 80.0, p), (10, 257, 4, 80.0, p), (10, 6, 0, 80.0, p),
+             (10, 6, 4, float("nan"), p), (10, 6, 4, 80.0, None)]
+    for n, D, K, R, lab in cases:
+        st = L.lss_assign_clusters(p, n, D, p, K, p, p, p, p, R, lab, None)
+        assert st == 1 and b"lss_assign_clusters" in L.lss_last_error(), (n, D, K, R)
+    assert L.lss_cluster_centroids_pca(None, p) == 1
+    assert L.lss_sort_windows(None, 0, None, None, None, 1, 1, None) == 1
+    assert L.lss_sort_window(None, p, 100, 1, 0, None) == 1
