@@ -171,6 +171,10 @@ __device__ __forceinline__ void emit_pass(const PursuitArgs& a, const uint32_t* 
     __shared__ int wsum[kFlagCols / 64];
     __shared__ int s_base, s_run;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t nwords = (a.W + 31) / 32;
+    // the first tile's flag words, loaded while thread 0 reads the counters
+    const uint32_t lo0 = (2 * tid < nwords) ? ld_dev(flagw + 2 * tid) : 0u;
+    const uint32_t hi0 = (2 * tid + 1 < nwords) ? ld_dev(flagw + 2 * tid + 1) : 0u;
     if (tid == 0) {
         // commit the previous pass (its update has completed: stream order)
         const int base = a.counters[0] + a.counters[1];
@@ -180,11 +184,10 @@ __device__ __forceinline__ void emit_pass(const PursuitArgs& a, const uint32_t* 
         s_run = 0;
     }
     __syncthreads();
-    const int64_t nwords = (a.W + 31) / 32;
     for (int64_t c0 = 0; c0 < a.W; c0 += kEmitTile) {
         const int64_t wd = (c0 >> 5) + 2 * tid;
-        const uint32_t lo = (wd < nwords) ? ld_dev(flagw + wd) : 0u;
-        const uint32_t hi = (wd + 1 < nwords) ? ld_dev(flagw + wd + 1) : 0u;
+        const uint32_t lo = (c0 == 0) ? lo0 : (wd < nwords) ? ld_dev(flagw + wd) : 0u;
+        const uint32_t hi = (c0 == 0) ? hi0 : (wd + 1 < nwords) ? ld_dev(flagw + wd + 1) : 0u;
         uint64_t bits = ((uint64_t)hi << 32) | lo;
         const int cnt = __popcll(bits);
         int inc = cnt;
@@ -237,17 +240,18 @@ __device__ __forceinline__ void emit_pass(const PursuitArgs& a, const uint32_t* 
 __global__ __launch_bounds__(kFlagCols) void peaks_kernel(PursuitArgs a) {
     __shared__ float lds[kFlagCols + 2 * kPeakHalo];
     __shared__ int s_last;
-    if (a.counters[3]) return;   // converged earlier: nothing can change (no workgroup has ticketed)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     uint32_t* flagw = (uint32_t*)a.flags;
     const int64_t c0 = (int64_t)blockIdx.x * kFlagCols;
     constexpr int kRows = (kFlagCols + 2 * kPeakHalo + kFlagCols - 1) / kFlagCols;
+    // the row's loads go out together with the converged-flag load
     float v[kRows];
 #pragma unroll
     for (int r = 0; r < kRows; ++r) {
         const int64_t w = c0 - kPeakHalo + tid + r * kFlagCols;
         v[r] = (w >= 0 && w < a.W) ? a.cfmax[w] : -INFINITY;
     }
+    if (a.counters[3]) return;   // converged earlier: nothing can change (no workgroup has ticketed)
 #pragma unroll
     for (int r = 0; r < kRows; ++r)
         if (tid + r * kFlagCols < kFlagCols + 2 * kPeakHalo) lds[tid + r * kFlagCols] = v[r];
