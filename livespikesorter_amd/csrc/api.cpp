@@ -79,7 +79,6 @@ struct Handle {
     int64_t spec_W = -1;
     int spec_lo = 0, spec_hi = 0;
     std::vector<float> h_xc, h_yc;
-    std::vector<int> h_iCC;
     int64_t last_W = 0;          // window length of the last pursuit (template refinement reads it)
 
     // window buffers
@@ -101,7 +100,6 @@ struct Handle {
     // template refinement (allocated on first use): residual, waves of the given U/wPCA, partials
     DevBuf<float> rf_R, rf_R2, rf_waves, rf_dwpart, rf_dupart, rf_U, rf_U1, rf_gU, rf_gW, rf_W1, rf_amp;
     DevBuf<double> rf_sums;
-    float* hs_U = nullptr;       // pinned [T][K][C] (refined templates for iU/Ucc)
     DevBuf<int> rf_col, rf_tmp;
     DevBuf<double> rf_part, rf_loss;
     DevBuf<int> rf_flag;
@@ -130,7 +128,7 @@ struct Handle {
             if (e) (void)hipEventDestroy(e);
         if (h_counters) (void)hipHostFree(h_counters);
         for (void* q : {(void*)hs_counters, (void*)hs_col, (void*)hs_tmp, (void*)hs_amp, (void*)hs_tF,
-                        (void*)hs_xs, (void*)hs_ys, (void*)hs_U})
+                        (void*)hs_xs, (void*)hs_ys})
             if (q) (void)hipHostFree(q);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
     }
@@ -228,30 +226,10 @@ lss_status prepare_spectrum(Handle& h, int64_t W) {
 
 // iU (channel of largest template energy) and Ucc = U[:, :, iCC[:, iU]] for host U [T][K][C]
 // (prepare_extract, template_matching.py:20-26), uploaded to the handle.
-lss_status upload_extract(Handle& h, const float* U) {
-    const int C = h.C, T = h.T, K = h.K, nC = h.nC;
-    std::vector<int> iU(T);
-    std::vector<float> Ucc((size_t)nC * T * K);
-    for (int t = 0; t < T; ++t) {
-        int best = 0;
-        float bv = -1.f;
-        for (int c = 0; c < C; ++c) {
-            float e = 0.f;   // (U**2).sum(1) in float32, as the reference
-            for (int k = 0; k < K; ++k) {
-                const float u = U[((size_t)t * K + k) * C + c];
-                e += u * u;
-            }
-            if (e > bv) { bv = e; best = c; }
-        }
-        iU[t] = best;
-    }
-    for (int j = 0; j < nC; ++j)
-        for (int t = 0; t < T; ++t)
-            for (int k = 0; k < K; ++k)
-                Ucc[((size_t)j * T + t) * K + k] = U[((size_t)t * K + k) * C + h.h_iCC[(size_t)j * C + iU[t]]];
-    HIP_TRY(hipMemcpyAsync(h.iU.p, iU.data(), sizeof(int) * T, hipMemcpyHostToDevice, h.stream));
-    HIP_TRY(hipMemcpyAsync(h.Ucc.p, Ucc.data(), sizeof(float) * nC * T * K, hipMemcpyHostToDevice, h.stream));
-    HIP_TRY(hipStreamSynchronize(h.stream));
+// iU / Ucc (prepare_extract, template_matching.py:20-26) from the device templates; iCC is on the
+// device already.  Enqueued on the handle's stream.
+lss_status refresh_extract(Handle& h) {
+    HIP_TRY(lss::launch_extract(h.U.p, h.ldq, h.T, h.K, h.C, h.iCC.p, h.nC, h.iU.p, h.Ucc.p, h.stream));
     return LSS_OK;
 }
 
@@ -343,8 +321,8 @@ lss_status create_impl(const lss_params* p, const lss_artifacts* a, Handle** out
         std::vector<int> iCC;
         nearest_channels(h->h_xc, h->h_yc, nC, iCC);
         HIP_TRY(hipMemcpy(h->iCC.p, iCC.data(), sizeof(int) * nC * C, hipMemcpyHostToDevice));
-        h->h_iCC = iCC;
-        if (lss_status st = upload_extract(*h, a->U)) return st;
+        if (lss_status st = refresh_extract(*h)) return st;
+        HIP_TRY(hipStreamSynchronize(h->stream));
     }
     HIP_TRY(h->tables_ws.alloc(lss::tables_workspace_doubles(T, K, C, nt)));
     HIP_TRY(lss::launch_tables(h->U.p, h->ldq, h->wpca.p, T, K, C, nt, h->nm.p, h->waves.p, h->ldC, h->ctc.p, h->ldT,
@@ -1156,7 +1134,7 @@ lss_status refine_templates_alloc(Handle& h) {
     HIP_TRY(h.rf_tmp.alloc(h.prm.max_spikes));
     HIP_TRY(h.rf_amp.alloc(h.prm.max_spikes));
     HIP_TRY(h.rf_sums.alloc(256 + 4));
-    HIP_TRY(hipHostMalloc((void**)&h.hs_U, sizeof(float) * nU, hipHostMallocDefault));
+
     return LSS_OK;
 }
 
@@ -1310,11 +1288,12 @@ lss_status lss_refine_templates(void* handle, double* info) {
     HIP_TRY(hipMemcpy2DAsync(h.U.p, sizeof(float) * h.ldq, h.rf_U1.p, sizeof(float) * K * C, sizeof(float) * K * C, T,
                              hipMemcpyDeviceToDevice, h.stream));
     HIP_TRY(hipMemcpyAsync(h.wpca.p, h.rf_W1.p, sizeof(float) * nW, hipMemcpyDeviceToDevice, h.stream));
-    HIP_TRY(hipMemcpyAsync(h.hs_U, h.rf_U1.p, sizeof(float) * nU, hipMemcpyDeviceToHost, h.stream));
-    HIP_TRY(hipStreamSynchronize(h.stream));
-    if (lss_status st = upload_extract(h, h.hs_U)) return st;
+    if (lss_status st = refresh_extract(h)) return st;
     HIP_TRY(lss::launch_tables(h.U.p, h.ldq, h.wpca.p, T, K, C, nt, h.nm.p, h.waves.p, h.ldC, h.ctc.p, h.ldT,
                                h.tables_ws.p, h.stream));
+    // complete before returning: the streaming batcher replays its graphs on another stream, which
+    // must see the new tables
+    HIP_TRY(hipStreamSynchronize(h.stream));
     if (info) {
         info[0] = L0;
         info[1] = L2;

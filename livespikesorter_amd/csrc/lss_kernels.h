@@ -174,6 +174,9 @@ hipError_t launch_waves(const float* U, int64_t ldu, const float* wpca, int T, i
 // ws: device workspace of tables_workspace_doubles(T, K, C, nt) doubles (the Gram matrix, WtW
 // and the Gram GEMM's split-K partials).
 size_t tables_workspace_doubles(int T, int K, int C, int nt);
+// iU and Ucc of prepare_extract from the device templates (one workgroup per template).
+hipError_t launch_extract(const float* U, int64_t ldu, int T, int K, int C, const int* iCC, int nC, int* iU,
+                          float* Ucc, hipStream_t s);
 hipError_t launch_tables(const float* U, int64_t ldu, const float* wpca, int T, int K, int C, int nt,
                          float* nm, float* waves, int64_t ldC, float* ctc, int64_t ldT, double* ws, hipStream_t s);
 

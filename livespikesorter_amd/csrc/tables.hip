@@ -106,6 +106,57 @@ hipError_t launch_waves(const float* U, int64_t ldu, const float* wpca, int T, i
     return hipGetLastError();
 }
 
+// Features tables of prepare_extract (template_matching.py:20-26), one workgroup per template:
+// iU[t] = argmax_c sum_k U[t][k][c]^2 (squares rounded, then summed in k order; first channel on
+// ties) and Ucc[j][t][k] = U[t][k][iCC[j][iU[t]]] (iCC [nC][C]: channel iU's nearest channels).
+__global__ __launch_bounds__(256) void extract_kernel(const float* __restrict__ U, int64_t ldu, int T, int K, int C,
+                                                      const int* __restrict__ iCC, int nC, int* __restrict__ iU,
+                                                      float* __restrict__ Ucc) {
+    __shared__ float sv[256];
+    __shared__ int si[256];
+    const int t = blockIdx.x, tid = threadIdx.x;
+    const float* u = U + (int64_t)t * ldu;
+    float best = -1.f;
+    int bi = 0x7fffffff;
+    for (int c = tid; c < C; c += 256) {
+        float e = 0.f;
+        for (int k = 0; k < K; ++k) {
+            const float x = u[(int64_t)k * C + c];
+            e = __fadd_rn(e, __fmul_rn(x, x));
+        }
+        if (e > best) {   // ascending c within the thread: the first maximum stays
+            best = e;
+            bi = c;
+        }
+    }
+    sv[tid] = best;
+    si[tid] = bi;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (tid < off) {
+            const float v2 = sv[tid + off];
+            const int i2 = si[tid + off];
+            if (v2 > sv[tid] || (v2 == sv[tid] && i2 < si[tid])) {
+                sv[tid] = v2;
+                si[tid] = i2;
+            }
+        }
+        __syncthreads();
+    }
+    const int b = (si[0] < C) ? si[0] : 0;   // no finite energy at all: channel 0
+    if (tid == 0) iU[t] = b;
+    for (int e2 = tid; e2 < nC * K; e2 += 256) {
+        const int j = e2 / K, k = e2 - j * K;
+        Ucc[((int64_t)j * T + t) * K + k] = u[(int64_t)k * C + iCC[(int64_t)j * C + b]];
+    }
+}
+
+hipError_t launch_extract(const float* U, int64_t ldu, int T, int K, int C, const int* iCC, int nC, int* iU,
+                          float* Ucc, hipStream_t s) {
+    hipLaunchKernelGGL(extract_kernel, dim3(T), dim3(256), 0, s, U, ldu, T, K, C, iCC, nC, iU, Ucc);
+    return hipGetLastError();
+}
+
 // Gram matrix, WtW, and the split-K partials of the Gram GEMM
 size_t tables_workspace_doubles(int T, int K, int C, int nt) {
     const int R = T * K;
