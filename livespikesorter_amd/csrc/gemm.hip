@@ -181,13 +181,42 @@ struct LpType<LSS_PREC_FP16X3> { typedef _Float16 T; typedef halfx8 V; };
 template <int PREC>
 constexpr bool is_split() { return PREC == LSS_PREC_BF16X3 || PREC == LSS_PREC_FP16X3; }
 
-// element (row, k) of the tile held by this thread's Frag
+// Global fetch of the reduced-precision kernel: a k-major (KM) operand as fetch<true>; an m/n-major
+// one with eight lanes per row, so that one load instruction covers 8 rows' 128-byte k runs (8
+// cache lines) instead of 32 rows' halves: thread t loads rows t/8 + 32h, k (t&7)*4 .. +4
+// (bf16x3 scores 0.68 -> 0.61 ms, plain BF16 0.56 -> 0.42 ms).  The k-major operand kept its
+// layout (32 lanes per k row made the whitening GEMM slower, 0.11 -> 0.17 ms), and so did the
+// FP32 kernel (eight lanes per row: 1.71 vs 1.67 ms; its MFMA work is 16x slower per element).
+template <bool KM>
+__device__ __forceinline__ void fetch_lp(const float* __restrict__ P, int64_t ld, int r0, int k0, int R, int K,
+                                         Frag& f) {
+    if (KM) {
+        fetch<true>(P, ld, r0, k0, R, K, f);
+        return;
+    }
+    const int tid = threadIdx.x;
+    const int row = tid >> 3, k = k0 + (tid & 7) * 4;
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+        const int r = r0 + row + 32 * h;
+        floatx4 v = {0.f, 0.f, 0.f, 0.f};
+        if (r < R) {
+            const float* p = P + (int64_t)r * ld + k;
+            if (k + 3 < K) v = *(const floatx4*)p;
+            else
+                for (int e = 0; e < 4; ++e) v[e] = (k + e < K) ? p[e] : 0.f;
+        }
+        f.v[h] = v;
+    }
+}
+
+// element (row, k) of the tile held by this thread's Frag (as fetch_lp loads it)
 template <bool KM>
 __device__ __forceinline__ void frag_coord(int h, int e, int& row, int& k) {
     const int tid = threadIdx.x;
     if (!KM) {
-        row = tid >> 1;
-        k = (tid & 1) * 16 + 4 * h + e;
+        row = (tid >> 3) + 32 * h;
+        k = (tid & 7) * 4 + e;
     } else {
         row = (tid & 7) * 16 + 4 * h + e;
         k = tid >> 3;
@@ -248,15 +277,15 @@ __global__ __launch_bounds__(256, 3) void gemm_lp_kernel(const float* __restrict
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
 
     Frag fa, fb;
-    fetch<A_KM>(A, lda, m0, 0, M, K, fa);
-    fetch<false>(B, ldb, n0, 0, N, K, fb);
+    fetch_lp<A_KM>(A, lda, m0, 0, M, K, fa);
+    fetch_lp<false>(B, ldb, n0, 0, N, K, fb);
     for (int k0 = 0; k0 < K; k0 += BK) {
         stash_lp<PREC, A_KM>(Ah, Al, fa);
         stash_lp<PREC, false>(Bh, Bl, fb);
         __syncthreads();
         if (k0 + BK < K) {
-            fetch<A_KM>(A, lda, m0, k0 + BK, M, K, fa);
-            fetch<false>(B, ldb, n0, k0 + BK, N, K, fb);
+            fetch_lp<A_KM>(A, lda, m0, k0 + BK, M, K, fa);
+            fetch_lp<false>(B, ldb, n0, k0 + BK, N, K, fb);
         }
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 16) {
