@@ -83,7 +83,7 @@ struct Handle {
 
     // window buffers
     DevBuf<int16_t> win;
-    DevBuf<long long> sums;
+    DevBuf<long long> sums, sum_part;   // channel sums and their per-block partials
     DevBuf<float> xcw, xhp, xw, xp, B, cfmax;
     DevBuf<int> imax;
     DevBuf<unsigned char> flags;
@@ -331,6 +331,7 @@ lss_status create_impl(const lss_params* p, const lss_artifacts* a, Handle** out
     // ---- window buffers ----
     HIP_TRY(h->win.alloc((size_t)Wm * C));
     HIP_TRY(h->sums.alloc(C));
+    HIP_TRY(h->sum_part.alloc((size_t)lss::channel_sum_blocks(Wm) * C));
     HIP_TRY(h->xcw.alloc((size_t)C * h->ldw));
     HIP_TRY(h->xhp.alloc((size_t)C * h->ldw));
     HIP_TRY(h->xw.alloc((size_t)Wm * h->ldC));
@@ -375,7 +376,7 @@ lss_status check_window(Handle& h, int64_t W) {
 }
 
 lss_status run_car(Handle& h, const int16_t* d_win, int64_t W) {
-    HIP_TRY(lss::launch_channel_sums(d_win, W, h.C, h.sums.p, h.stream));
+    HIP_TRY(lss::launch_channel_sums(d_win, W, h.C, h.sum_part.p, h.sums.p, h.stream));
     HIP_TRY(lss::launch_car_transpose(d_win, W, h.C, h.sums.p, h.prm.do_car, h.xcw.p, h.ldw, h.stream));
     return LSS_OK;
 }

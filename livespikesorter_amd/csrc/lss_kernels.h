@@ -9,7 +9,10 @@ namespace lss {
 
 // ---- preprocessing (preprocess.hip) ---------------------------------------------------------
 // Exact int64 channel sums of an int16 [W][C] window.
-hipError_t launch_channel_sums(const int16_t* win, int64_t W, int C, long long* sums, hipStream_t s);
+// Exact int64 channel sums: per-block partials part [channel_sum_blocks(W)][C], then sums[C].
+int64_t channel_sum_blocks(int64_t W);
+hipError_t launch_channel_sums(const int16_t* win, int64_t W, int C, long long* part, long long* sums,
+                               hipStream_t s);
 // out[c][w] = (x[w][c] - mean_c) - lower_median_c'(x[w][c'] - mean_c'), transposed to [C][ldo].
 // do_car = 0 skips the median.  C <= 1024.
 hipError_t launch_car_transpose(const int16_t* win, int64_t W, int C, const long long* sums,
