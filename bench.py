@@ -102,6 +102,20 @@ def run_in_flight(a, sorter, dwin, starts, steps, warmup, k, precision, Wp, seq_
     return dict(elapsed=el, equal=bool(equal))
 
 
+def host_cpu() -> dict:
+    """The host the CPU baseline ran on: logical CPUs and model name."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "host_cpus": os.cpu_count()}
+
+
 def read_traffic():
     """HBM bytes per update launch measured with rocprofv3 --pmc (committed summary), or None."""
     try:
@@ -327,7 +341,7 @@ def bench_adjoint(args):
         lc, _ = ko.residual_energy(X32, spikes.col.cpu().long(), spikes.tmp.cpu().long(), spikes.amp.cpu(), U32, W32)
         torch.autograd.grad(lc, (U32, W32))
         csec = time.perf_counter() - c1
-        cpu = {"value": round(csec * 1e3, 1), "unit": "ms per backward", "cores": torch.get_num_threads(),
+        cpu = {"value": round(csec * 1e3, 1), "unit": "ms per backward", "cores": torch.get_num_threads(), **host_cpu(),
                "kind": "port",
                "sample": f"one loss + torch-autograd gradient of the FP32 oracle on the same 2 s window "
                          f"and {n} spikes ({csec:.1f} s)"}
@@ -541,7 +555,7 @@ def bench_train(args):
         to.wpca(clips.cpu().numpy(), 6)
         pca_cpu = time.perf_counter() - c1
         cpu = {"value": round(2.0 / cov_cpu, 2), "unit": "x realtime (covariance of one 2 s batch)",
-               "cores": int(os.environ.get("OMP_NUM_THREADS", "16")), "kind": "port",
+               "cores": int(os.environ.get("OMP_NUM_THREADS", "16")), **host_cpu(), "kind": "port",
                "sample": f"FP64 numpy oracle on the host: covariance of batch 0 {cov_cpu * 1e3:.1f} ms, "
                          f"whitening_local {zca_cpu * 1e3:.0f} ms, iKxx {inv_cpu * 1e3:.0f} ms, drift matrix "
                          f"{dm_cpu * 1e3:.1f} ms, snippets of batch 0 {snip_cpu * 1e3:.0f} ms, wPCA of {ncl} "
@@ -719,7 +733,7 @@ def main():
         csec = time.perf_counter() - c0
         got = sorter.sort_window(dwin[ib], starts[ib])
         f1 = ko.spike_f1(ref.samples, ref.templates, got.sample, got.template, dt=1)
-        cpu = {"value": round(NT / fs / csec, 4), "unit": "x realtime", "cores": torch.get_num_threads(),
+        cpu = {"value": round(NT / fs / csec, 4), "unit": "x realtime", "cores": torch.get_num_threads(), **host_cpu(),
                "kind": "port",
                "sample": f"one padded window (batch {ib}, {NT} samples = {NT / fs:.1f} s) through the "
                          f"FP32 torch CPU oracle, {csec:.1f} s; GPU-vs-CPU spike F1 {f1:.5f}"}
