@@ -6,6 +6,8 @@ interior NT = 60000 Kilosort window, at most 50 pursuit passes.
   1e-3 of their scale, positions within 0.05 um.
 * bf16x3 (the split-precision GEMMs) against the FP32 device path: spike F1 >= 0.999 and
   amplitudes within 1e-3 relative.
+* 1000 templates: FP32, bf16x3 and fp16x3 against the FP64 oracle (F1 >= 0.999, amplitudes within
+  1e-3 relative).
 * Size-independent properties: samples inside the padded window, peak columns off the nt zeroed
   score columns at each edge, templates in [0, T), at most 50 passes, and the same spike list
   when the window is sorted twice (the handle carries no state from one window to the next)."""
@@ -93,3 +95,36 @@ def test_fullsize_properties(full, fp32_result):
     assert got.template.min() >= 0 and got.template.max() < T
     for k in ("sample", "template", "amplitude", "features", "x", "y"):
         np.testing.assert_array_equal(getattr(got, k), getattr(again, k))
+
+
+# ---- 1000 templates: the precision variants against the FP64 oracle -------------------------------
+@pytest.fixture(scope="module")
+def full1000():
+    from livespikesorter_amd import synth
+    print("[lss] generating the 384-ch / 1000-template recording (2 windows)", flush=True)
+    s = synth.generate(synth.SynthConfig(n_chan=384, n_units=1000, n_samples=120000, NT=60000, seed=9))
+    a = s.art
+    X = ko.padded_batch(s.rec, IB, a["NT"], a["nt"], torch.float64)
+    win = np.ascontiguousarray(X.numpy().T.astype(np.int16))
+    ws = ko.window_start(IB, a["NT"], a["nt"])
+    print("[lss] FP64 oracle on one window", flush=True)
+    ref = ko.sort_window(X, a, ws, dtype=torch.float64, max_passes=50)
+    return s, win, ws, ref
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3", "fp16x3"])
+def test_t1000_precisions_vs_fp64_oracle(full1000, precision):
+    """1000 templates (the split-precision GEMMs at their largest size): spike F1 >= 0.999 against
+    the FP64 oracle and amplitudes within 1e-3 relative over the matched spikes."""
+    from livespikesorter_amd.sorter import Sorter
+    print(f"[lss] test_t1000_precisions_vs_fp64_oracle {precision}", flush=True)
+    s, win, ws, ref = full1000
+    with Sorter(s.art, max_window=win.shape[0], max_passes=50, max_spikes=1 << 16, precision=precision) as so:
+        got = so.sort_window(torch.from_numpy(win).to(DEV), ws)
+    assert len(ref.samples) > 10000
+    f1 = ko.spike_f1(ref.samples, ref.templates, got.sample, got.template, dt=1)
+    assert f1 >= 0.999, (precision, f1)
+    ri = _index(ref.samples, ref.templates)
+    errs = [abs(a - ref.amps[ri[(int(t), int(y))]]) / abs(ref.amps[ri[(int(t), int(y))]])
+            for t, y, a in zip(got.sample, got.template, got.amplitude) if (int(t), int(y)) in ri]
+    assert len(errs) >= 0.99 * len(ref.samples) and max(errs) <= 1e-3, (precision, len(errs), max(errs))
