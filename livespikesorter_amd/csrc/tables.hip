@@ -166,6 +166,23 @@ size_t tables_workspace_doubles(int T, int K, int C, int nt) {
     return (size_t)(1 + S) * R * R + (size_t)K * K * (2 * nt + 1);
 }
 
+// WtW[k][m][l] = sum_tau wPCA[k][nt - l + tau] wPCA[m][tau] (in range), the temporal part of ctc
+// (prepare_matching, template_matching.py:110-121), in FP64 from the FP32 wPCA: each product of two
+// widened FP32 values is exact, so the sum in tau order rounds as the host loop it replaces.
+__global__ __launch_bounds__(256) void wtw_kernel(const float* __restrict__ wpca, int K, int nt,
+                                                  double* __restrict__ wtw) {
+    const int L = 2 * nt + 1;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= K * K * L) return;
+    const int l = i % L, km = i / L, m = km % K, k = km / K;
+    double acc = 0.0;
+    for (int tau = 0; tau < nt; ++tau) {
+        const int a = nt - l + tau;
+        if (a >= 0 && a < nt) acc += (double)wpca[k * nt + a] * (double)wpca[m * nt + tau];
+    }
+    wtw[i] = acc;
+}
+
 hipError_t launch_tables(const float* U, int64_t ldu, const float* wpca, int T, int K, int C, int nt,
                          float* nm, float* waves, int64_t ldC, float* ctc, int64_t ldT, double* ws, hipStream_t s) {
     if (K > 8) return hipErrorInvalidValue;
@@ -175,27 +192,14 @@ hipError_t launch_tables(const float* U, int64_t ldu, const float* wpca, int T, 
     const int64_t nw = (int64_t)T * nt * C;
     hipLaunchKernelGGL(waves_kernel, dim3((unsigned)((nw + 255) / 256)), dim3(256), 0, s, U, ldu, wpca, T, K, C, nt, waves, ldC);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    // WtW on the host (K*K*(2nt+1) values) from the FP32 wPCA
+    // WtW on the device (K*K*(2nt+1) values) from the FP32 wPCA
     const int L = 2 * nt + 1;
-    float* hw = (float*)malloc(sizeof(float) * K * nt);
-    double* hwtw = (double*)malloc(sizeof(double) * K * K * L);
-    if ((e = hipMemcpyAsync(hw, wpca, sizeof(float) * K * nt, hipMemcpyDeviceToHost, s)) != hipSuccess) { free(hw); free(hwtw); return e; }
-    if ((e = hipStreamSynchronize(s)) != hipSuccess) { free(hw); free(hwtw); return e; }
-    for (int k = 0; k < K; ++k)
-        for (int m = 0; m < K; ++m)
-            for (int l = 0; l < L; ++l) {
-                double acc = 0.0;
-                for (int tau = 0; tau < nt; ++tau) {
-                    const int a = nt - l + tau;
-                    if (a >= 0 && a < nt) acc += (double)hw[k * nt + a] * (double)hw[m * nt + tau];
-                }
-                hwtw[(k * K + m) * L + l] = acc;
-            }
     const int R = T * K;
     double* dG = ws;
     double* dW = ws + (size_t)R * R;
     double* dP = dW + (size_t)K * K * L;
-    e = hipMemcpyAsync(dW, hwtw, sizeof(double) * K * K * L, hipMemcpyHostToDevice, s);
+    hipLaunchKernelGGL(wtw_kernel, dim3((K * K * L + 255) / 256), dim3(256), 0, s, wpca, K, nt, dW);
+    e = hipGetLastError();
     if (e == hipSuccess) {
         int S;
         int64_t kchunk;
@@ -222,8 +226,6 @@ hipError_t launch_tables(const float* U, int64_t ldu, const float* wpca, int T, 
         }
     }
     hipError_t e2 = hipStreamSynchronize(s);
-    free(hw);
-    free(hwtw);
     return e != hipSuccess ? e : e2;
 }
 
