@@ -1,6 +1,6 @@
 """Edge cases of the device path: ragged sizes (channel, template and PC counts that are not
-multiples of the vector width, a short template), mean-only preprocessing (no median reference),
-a window without spikes, and the window-length limits.  The oracle is the FP64 CPU restatement;
+multiples of the vector width, a short template) in FP32 and in the split precisions, mean-only
+preprocessing (no median reference), a window without spikes, and the window-length limits.  The oracle is the FP64 CPU restatement;
 spike lists must agree at F1 >= 0.999 (+-1 sample, same template), amplitudes within 1e-3."""
 import numpy as np
 import pytest
@@ -88,3 +88,29 @@ def test_window_length_limits(ragged):
             s.sort_window(np.ascontiguousarray(rec[:63]), 0)
         with pytest.raises(LssError, match="ARG"):
             s.sort_window(np.zeros((1001, 37), np.int16), 0)
+
+
+@pytest.mark.parametrize("precision", ["bf16x3", "fp16x3"])
+def test_ragged_sizes_split_precision(ragged, precision):
+    """The split-precision paths (MFMA projection and score GEMM) at ragged sizes: C = 37 and
+    T = 13 not multiples of the 32/64-wide tiles, K = 3 wavefronts, nt = 31: the same spikes as
+    the FP32 path (F1 >= 0.999, amplitudes within 1e-3 relative)."""
+    from livespikesorter_amd.sorter import Sorter
+    print(f"case: ragged C=37 T=13 K=3 nt=31 precision={precision}", flush=True)
+    NT, nt = 4000, 31
+    win = np.ascontiguousarray(ko.padded_batch(ragged.rec, 1, NT, nt, torch.float64).numpy().T.astype(np.int16))
+    ws = ko.window_start(1, NT, nt)
+    with Sorter(ragged.art, max_window=NT + 2 * nt, max_passes=100) as s32:
+        ref = s32.sort_window(win, ws)
+    with Sorter(ragged.art, max_window=NT + 2 * nt, max_passes=100, precision=precision) as s:
+        got = s.sort_window(win, ws)
+    assert len(ref) > 20
+    f1 = ko.spike_f1(ref.sample, ref.template, got.sample, got.template, dt=1)
+    assert f1 >= 0.999, f1
+
+    class R:
+        pass
+    r = R()
+    r.samples, r.templates, r.amps = ref.sample, ref.template, ref.amplitude
+    err, n = _match_amp(r, got)
+    assert n >= 0.99 * len(ref) and err <= 1e-3, (n, err)
