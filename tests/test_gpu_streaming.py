@@ -52,9 +52,10 @@ def _make_sorters(probes, **kw):
     return [Sorter(p.art, max_window=BATCH + 2 * p.art["nt"], max_passes=100, **kw) for p in probes]
 
 
-def test_stream_matches_single_windows(probes):
-    print("[lss] test_stream_matches_single_windows", flush=True)
-    sorters = _make_sorters(probes)
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_stream_matches_single_windows(probes, precision):
+    print(f"[lss] test_stream_matches_single_windows {precision}", flush=True)
+    sorters = _make_sorters(probes, precision=precision)
     got = _stream(probes, sorters, use_graph=True)
     nb = len(got[0])
     assert nb == (probes[0].rec.shape[0] - 61) // BATCH
