@@ -186,6 +186,13 @@ lss_status lss_write_spikes_txt(const char* path, const int64_t* sample, const i
                                 const float* amplitude, const float* y, int64_t n,
                                 int64_t recording_offset, int32_t append);
 
+/* The label centroids of an oss_input directory, cluster_centroids.npy stored flattened as
+ * [x1, y1, x2, y2, ...] (OnlineSpikesV2::loadKilosortClusteringData, OnlineSpikesV2.cu:612-628;
+ * any numeric dtype, C or Fortran order).  *n = the number of centroids; with x or y NULL only
+ * *n is set.  LSS_ERR_IO when the file is missing or malformed, LSS_ERR_CAPACITY when capacity
+ * is below *n.  Host only. */
+lss_status lss_load_cluster_centroids(const char* dir, float* x, float* y, int32_t capacity, int32_t* n);
+
 /* ---- feature-space cluster assignment of the reference's Python online sorter
  * (src/Python/OnlineKilosort.py:211-284); device pointers, synchronous ---------------------------- */
 

@@ -87,7 +87,8 @@ struct InputParameters {
     int iPrecision = LSS_PREC_FP32;
     int iMaxSpikes = 1 << 16;            // spikes one batch may produce
     int iDt = 7;                         // duplicate window in samples
-    std::vector<float> vCentroidX, vCentroidY;   // cluster centroids (um); empty: template ids
+    std::vector<float> vCentroidX, vCentroidY;   // cluster centroids (um); empty: those of
+                                                 // sInputFolder/cluster_centroids.npy, or template ids
 };
 
 struct OnlineSpikesPayload {
@@ -152,6 +153,17 @@ private:
         os_.lNChans = C_;
         os_.vImecChannels = prm_.vChannelSubset;
         os_.substream = prm_.iSubstream;
+        if (prm_.vCentroidX.empty() && !prm_.sInputFolder.empty()) {
+            // the artifact directory's label centroids (cluster_centroids.npy, as
+            // OnlineSpikesV2::loadKilosortClusteringData reads them); template ids without it
+            int32_t n = 0;
+            if (lss_load_cluster_centroids(prm_.sInputFolder.c_str(), nullptr, nullptr, 0, &n) == LSS_OK && n > 0) {
+                prm_.vCentroidX.resize(n);
+                prm_.vCentroidY.resize(n);
+                call(lss_load_cluster_centroids(prm_.sInputFolder.c_str(), prm_.vCentroidX.data(),
+                                                prm_.vCentroidY.data(), n, &n));
+            }
+        }
         if (prm_.vCentroidX.size() != prm_.vCentroidY.size())
             throw std::runtime_error("OnlineSpikesV2: centroid x and y differ in length");
         nlabels_ = prm_.vCentroidX.empty() ? T_ : (int)prm_.vCentroidX.size();

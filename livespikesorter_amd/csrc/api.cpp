@@ -886,6 +886,26 @@ lss_status lss_destroy(void* handle) {
     return LSS_OK;
 }
 
+lss_status lss_load_cluster_centroids(const char* dir, float* x, float* y, int32_t capacity, int32_t* n) {
+    if (!dir || !n) return fail(LSS_ERR_ARG, "null argument");
+    std::string d = dir;
+    if (!d.empty() && d.back() != '/') d += '/';
+    lss::NpyArray a;
+    std::string err;
+    if (!lss::npy_load_f32(d + "cluster_centroids.npy", a, err)) return fail(LSS_ERR_IO, err);
+    if (a.data.size() % 2) return fail(LSS_ERR_IO, "cluster_centroids.npy: odd number of values (x, y pairs)");
+    const int64_t m = (int64_t)a.data.size() / 2;
+    if (m > INT32_MAX) return fail(LSS_ERR_IO, "cluster_centroids.npy: too many centroids");
+    *n = (int32_t)m;
+    if (!x || !y) return LSS_OK;   // count only
+    if (capacity < m) return fail(LSS_ERR_CAPACITY, "lss_load_cluster_centroids: capacity below the centroid count");
+    for (int64_t i = 0; i < m; ++i) {
+        x[i] = a.data[2 * i];
+        y[i] = a.data[2 * i + 1];
+    }
+    return LSS_OK;
+}
+
 lss_status lss_cluster_centroids_pca(void* handle, float* d_out) {
     if (!handle || !d_out) return fail(LSS_ERR_ARG, "null argument");
     Handle& h = *as_handle(handle);

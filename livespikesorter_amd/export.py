@@ -13,7 +13,7 @@ from pathlib import Path
 import numpy as np
 
 
-def write_oss_input(art: dict, out_dir, with_ctc: bool = False) -> Path:
+def write_oss_input(art: dict, out_dir, with_ctc: bool = False, cluster_xy=None) -> Path:
     d = Path(out_dir)
     d.mkdir(parents=True, exist_ok=True)
     U = np.asarray(art["U"], np.float32)                 # [T][K][C]
@@ -55,6 +55,9 @@ def write_oss_input(art: dict, out_dir, with_ctc: bool = False) -> Path:
         ctc = np.einsum("ijkm,kml->ijl", utu, wtw).astype(np.float32)
         np.save(d / "ctc.npy", ctc)
         np.save(d / "ctc_permuted.npy", np.ascontiguousarray(ctc.transpose(1, 0, 2)))
+    if cluster_xy is not None:
+        # label centroids, flattened [x1, y1, x2, y2, ...] as OnlineSpikesV2 reads them
+        np.save(d / "cluster_centroids.npy", np.asarray(cluster_xy, np.float32).reshape(-1, 2))
     with open(d / "misc.txt", "w") as f:
         f.write(f"nt0min:{int(art.get('nt0min', 20))}\n")
         f.write(f"numNearestChans:{nC}\n")

@@ -55,6 +55,19 @@ def write_spikes_txt(path, sample, label, amplitude, y, recording_offset: int = 
           "lss_write_spikes_txt")
 
 
+def load_cluster_centroids(directory):
+    """(cx, cy) float32 arrays from an oss_input directory's cluster_centroids.npy
+    ([x1, y1, x2, y2, ...], src/SpikeSorter/OnlineSpikesV2.cu:612-628)."""
+    d = str(directory).encode()
+    n = ct.c_int32()
+    check(lib().lss_load_cluster_centroids(d, None, None, 0, ct.byref(n)), "lss_load_cluster_centroids")
+    cx = np.zeros(n.value, np.float32)
+    cy = np.zeros(n.value, np.float32)
+    check(lib().lss_load_cluster_centroids(d, cx.ctypes.data, cy.ctypes.data, n.value, ct.byref(n)),
+          "lss_load_cluster_centroids")
+    return cx, cy
+
+
 def cluster_centroids_pca(sorter):
     """[T][n_nearest * n_pcs] float32 device tensor: each template's PC features on its nearest
     channels, the cluster centroids of cluster_centroids_pca_compute

@@ -1,8 +1,8 @@
 """The C++ facade with the shape of OnlineSpikesV2 (include/lss_online_spikes.hpp), driven by
 tests/online_facade_check over a recorded stream on a simulated clock, against the same batches
 sorted through the Python binding: window [start - nt, start + n + nt) per batch (sample 0
-repeated before the stream's start), spikes time-ordered, labelled by template, deduplicated per
-label with dt = 7 across batches.  The batch schedule (minimum/maximum window, lag tolerance,
+repeated before the stream's start), spikes time-ordered, labelled by template (or by the nearest
+of the artifact directory's cluster centroids), deduplicated per label with dt = 7 across batches.  The batch schedule (minimum/maximum window, lag tolerance,
 small/big skip) is recomputed here from its definition.  Exact equality: same device path, same
 windows."""
 import subprocess
@@ -45,8 +45,8 @@ def _schedule(n_total, first, mn, mx, nt, tb_ms, lag, small):
     return out
 
 
-def _expected(sorter, rec, batches, nt, n_labels, dt=7):
-    from livespikesorter_amd.output import Deduplicator
+def _expected(sorter, rec, batches, nt, n_labels, dt=7, centroids=None):
+    from livespikesorter_amd.output import Deduplicator, label_spikes
     dd = Deduplicator(n_labels, dt)
     rows = []
     for start, n, _ in batches:
@@ -54,7 +54,8 @@ def _expected(sorter, rec, batches, nt, n_labels, dt=7):
         idx = np.clip(np.arange(first, first + n + 2 * nt), 0, None)
         sp = sorter.sort_window(np.ascontiguousarray(rec[idx]), first)
         pos = np.arange(len(sp), dtype=np.float32)
-        s, l, kept = dd(sp.sample, sp.template, pos)
+        lab = sp.template if centroids is None else label_spikes(sp.x, sp.y, *centroids)
+        s, l, kept = dd(sp.sample, lab, pos)
         k = kept.astype(np.int64)
         rows.append(np.stack([s.astype(np.float64), l.astype(np.float64),
                               sp.amplitude[k].astype(np.float64), sp.y[k].astype(np.float64)], 1))
@@ -114,3 +115,35 @@ def test_facade_matches_python_path(facade_inputs, mn, mx, tb, first, lag, small
         # batches tile the stream: no gaps, no overlaps
         ends = [a + b for a, b, _ in sched]
         assert all(sched[i + 1][0] == ends[i] for i in range(len(sched) - 1))
+
+
+def test_facade_labels_from_directory_centroids(synth1, tmp_path_factory):
+    """With cluster_centroids.npy in the artifact directory the facade labels spikes by the nearest
+    centroid (OnlineSpikesV2::loadKilosortClusteringData / closestCluster) and deduplicates per
+    label."""
+    from livespikesorter_amd.export import write_oss_input
+    from livespikesorter_amd.output import load_cluster_centroids
+    from livespikesorter_amd.sorter import Sorter
+    if not EXE.exists():
+        pytest.fail(f"{EXE} is not built (livespikesorter_amd.build)")
+    print("case: facade with directory centroids", flush=True)
+    d = tmp_path_factory.mktemp("facade_c")
+    rng = np.random.default_rng(21)
+    xc, yc = np.asarray(synth1.art["xc"]), np.asarray(synth1.art["yc"])
+    xy = np.stack([rng.uniform(xc.min(), xc.max(), 12), rng.uniform(yc.min(), yc.max(), 12)], 1)
+    art_dir = write_oss_input(synth1.art, d / "oss_input", cluster_xy=xy)
+    rec = np.ascontiguousarray(synth1.rec[:30000], np.int16)
+    rec.tofile(d / "rec.i16")
+    out = d / "out"
+    out.mkdir()
+    r = subprocess.run([str(EXE), str(art_dir), str(d / "rec.i16"), str(rec.shape[1]), "3000", "3000", str(out)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    sched = _schedule(rec.shape[0], 0, 3000, 3000, 61, 100000, 0, 0)
+    cents = load_cluster_centroids(art_dir)
+    with Sorter.from_dir(art_dir, max_window=3000 + 2 * 127, max_passes=100, max_spikes=1 << 16) as s:
+        exp = _expected(s, rec, sched, 61, len(xy), centroids=cents)
+    got = _read(out / "spikeOutput.txt")
+    assert len(got) == len(exp) > 0
+    np.testing.assert_array_equal(got, exp)
+    assert got[:, 1].max() < len(xy)

@@ -59,3 +59,33 @@ def test_export_oss_input(tmp_path, small_synth):
                                             torch.from_numpy(a["U"]).double(), int(a["nC"]))
     np.testing.assert_array_equal(iCC, iCC_o.numpy())
     np.testing.assert_array_equal(np.load(d / "iU.npy"), iU_o.numpy())
+
+
+def test_load_cluster_centroids(tmp_path, small_synth):
+    """cluster_centroids.npy as OnlineSpikesV2 reads it ([x1, y1, x2, y2, ...]): written by the
+    exporter, read back in C or Fortran order and any numeric dtype; missing, odd-length and
+    too-small outputs are reported."""
+    import ctypes as ct
+    import pytest
+    from livespikesorter_amd._lib import LssError, lib
+    from livespikesorter_amd.export import write_oss_input
+    from livespikesorter_amd.output import load_cluster_centroids
+    xy = np.array([[0.0, 10.0], [32.0, 55.5], [16.0, 300.25]], np.float32)
+    d = write_oss_input(small_synth.art, tmp_path / "a", cluster_xy=xy)
+    cx, cy = load_cluster_centroids(d)
+    np.testing.assert_array_equal(cx, xy[:, 0])
+    np.testing.assert_array_equal(cy, xy[:, 1])
+    np.save(d / "cluster_centroids.npy", np.asfortranarray(xy.astype(np.float64)))
+    cx, cy = load_cluster_centroids(d)
+    np.testing.assert_array_equal(cx, xy[:, 0])
+    np.testing.assert_array_equal(cy, xy[:, 1])
+    np.save(d / "cluster_centroids.npy", xy.reshape(-1)[:5])
+    with pytest.raises(LssError, match="IO"):
+        load_cluster_centroids(d)
+    with pytest.raises(LssError, match="IO"):
+        load_cluster_centroids(tmp_path / "missing")
+    np.save(d / "cluster_centroids.npy", xy)
+    n = ct.c_int32()
+    buf = np.zeros(2, np.float32)
+    st = lib().lss_load_cluster_centroids(str(d).encode(), buf.ctypes.data, buf.ctypes.data, 2, ct.byref(n))
+    assert st == 4 and n.value == 3
