@@ -193,6 +193,13 @@ lss_status lss_write_spikes_txt(const char* path, const int64_t* sample, const i
  * is below *n.  Host only. */
 lss_status lss_load_cluster_centroids(const char* dir, float* x, float* y, int32_t capacity, int32_t* n);
 
+/* The acquisition channels of an oss_input directory, channelMap.npy (the channel list
+ * OnlineSpikesV2 hands its DataSocket, OnlineSpikesV2.cu:474, 657-661; any integer or float
+ * dtype holding non-negative integers).  *n = the number of entries; with out NULL only *n is set.
+ * LSS_ERR_IO when the file is missing or malformed, LSS_ERR_CAPACITY when capacity is below *n.
+ * Host only. */
+lss_status lss_load_channel_map(const char* dir, int32_t* out, int32_t capacity, int32_t* n);
+
 /* ---- feature-space cluster assignment of the reference's Python online sorter
  * (src/Python/OnlineKilosort.py:211-284); device pointers, synchronous ---------------------------- */
 

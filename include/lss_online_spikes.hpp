@@ -151,6 +151,18 @@ private:
         nt_ = got.nt;
         T_ = got.n_templates;
         os_.lNChans = C_;
+        if (prm_.vChannelSubset.empty() && !prm_.sInputFolder.empty()) {
+            // the artifact directory's channel list (channelMap.npy), when there is one
+            int32_t n = 0;
+            if (lss_load_channel_map(prm_.sInputFolder.c_str(), nullptr, 0, &n) == LSS_OK && n > 0) {
+                if (n != C_)   // OnlineSpikesV2::loadChannelMap requires one entry per channel
+                    throw std::runtime_error("OnlineSpikesV2: channelMap.npy has " + std::to_string(n) +
+                                             " entries, the templates " + std::to_string(C_) + " channels");
+                std::vector<int32_t> m(n);
+                call(lss_load_channel_map(prm_.sInputFolder.c_str(), m.data(), n, &n));
+                prm_.vChannelSubset.assign(m.begin(), m.end());
+            }
+        }
         os_.vImecChannels = prm_.vChannelSubset;
         os_.substream = prm_.iSubstream;
         if (prm_.vCentroidX.empty() && !prm_.sInputFolder.empty()) {

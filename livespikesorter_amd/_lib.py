@@ -29,7 +29,7 @@ EXPORTS = ["lss_last_error", "lss_abi_version", "lss_default_params", "lss_creat
            "lss_refine_templates", "lss_stream_set_refine", "lss_stream_last_refine", "lss_stream_push_device",
            "lss_set_drift", "lss_covariance_add", "lss_whitening_from_covariance", "lss_whitening_local",
            "lss_drift_inverse", "lss_drift_matrix", "lss_extract_snippets", "lss_wpca", "lss_train_release",
-           "lss_select_device", "lss_sort_windows", "lss_cluster_centroids_pca", "lss_assign_clusters", "lss_load_cluster_centroids"]
+           "lss_select_device", "lss_sort_windows", "lss_cluster_centroids_pca", "lss_assign_clusters", "lss_load_cluster_centroids", "lss_load_channel_map"]
 
 
 class Params(ct.Structure):
@@ -132,6 +132,7 @@ def lib() -> ct.CDLL:
             "lss_cluster_centroids_pca": ([vp, vp], i32),
             "lss_assign_clusters": ([vp, i64, i32, vp, i32, vp, vp, vp, vp, f64, vp, vp], i32),
             "lss_load_cluster_centroids": ([ct.c_char_p, vp, vp, i32, ct.POINTER(i32)], i32),
+            "lss_load_channel_map": ([ct.c_char_p, vp, i32, ct.POINTER(i32)], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

@@ -906,6 +906,26 @@ lss_status lss_load_cluster_centroids(const char* dir, float* x, float* y, int32
     return LSS_OK;
 }
 
+lss_status lss_load_channel_map(const char* dir, int32_t* out, int32_t capacity, int32_t* n) {
+    if (!dir || !n) return fail(LSS_ERR_ARG, "null argument");
+    std::string d = dir;
+    if (!d.empty() && d.back() != '/') d += '/';
+    lss::NpyArray a;
+    std::string err;
+    if (!lss::npy_load_f32(d + "channelMap.npy", a, err)) return fail(LSS_ERR_IO, err);
+    if (a.data.size() > (size_t)INT32_MAX) return fail(LSS_ERR_IO, "channelMap.npy: too many entries");
+    *n = (int32_t)a.data.size();
+    if (!out) return LSS_OK;   // count only
+    if (capacity < *n) return fail(LSS_ERR_CAPACITY, "lss_load_channel_map: capacity below the entry count");
+    for (size_t i = 0; i < a.data.size(); ++i) {
+        const float v = a.data[i];
+        if (!(v >= 0.f) || v != std::floor(v) || v > 16777216.f)
+            return fail(LSS_ERR_IO, "channelMap.npy: entries must be non-negative integers");
+        out[i] = (int32_t)v;
+    }
+    return LSS_OK;
+}
+
 lss_status lss_cluster_centroids_pca(void* handle, float* d_out) {
     if (!handle || !d_out) return fail(LSS_ERR_ARG, "null argument");
     Handle& h = *as_handle(handle);

@@ -13,7 +13,7 @@ from pathlib import Path
 import numpy as np
 
 
-def write_oss_input(art: dict, out_dir, with_ctc: bool = False, cluster_xy=None) -> Path:
+def write_oss_input(art: dict, out_dir, with_ctc: bool = False, cluster_xy=None, channel_map=None) -> Path:
     d = Path(out_dir)
     d.mkdir(parents=True, exist_ok=True)
     U = np.asarray(art["U"], np.float32)                 # [T][K][C]
@@ -30,6 +30,14 @@ def write_oss_input(art: dict, out_dir, with_ctc: bool = False, cluster_xy=None)
     if art.get("drift") is not None:
         np.save(d / "drift_matrix.npy", np.asarray(art["drift"], np.float32).T)
     np.save(d / "hp_filter.npy", np.asarray(art["hp_filter"], np.float32))
+    # acquisition channels, template and channel masks (crop_methods.py:171-175); channel_map
+    # defaults to the channels 0..C-1
+    cmap = np.arange(C) if channel_map is None else np.asarray(channel_map)
+    if cmap.shape != (C,):
+        raise ValueError(f"channel_map must hold {C} entries")
+    np.save(d / "channelMap.npy", cmap.astype(np.int32))
+    np.save(d / "channelMask.npy", np.arange(C, dtype=np.int64))
+    np.save(d / "templateMap.npy", np.arange(T, dtype=np.int64))
     np.save(d / "xc.npy", xc)
     np.save(d / "yc.npy", yc)
     # prepare_extract (kilosort/template_matching.py:20-26), ties broken by channel index

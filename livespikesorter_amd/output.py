@@ -68,6 +68,17 @@ def load_cluster_centroids(directory):
     return cx, cy
 
 
+def load_channel_map(directory) -> np.ndarray:
+    """int32 channel list from an oss_input directory's channelMap.npy (the channels OnlineSpikesV2
+    fetches, src/SpikeSorter/OnlineSpikesV2.cu:474, 657-661)."""
+    d = str(directory).encode()
+    n = ct.c_int32()
+    check(lib().lss_load_channel_map(d, None, 0, ct.byref(n)), "lss_load_channel_map")
+    out = np.zeros(n.value, np.int32)
+    check(lib().lss_load_channel_map(d, out.ctypes.data, n.value, ct.byref(n)), "lss_load_channel_map")
+    return out
+
+
 def cluster_centroids_pca(sorter):
     """[T][n_nearest * n_pcs] float32 device tensor: each template's PC features on its nearest
     channels, the cluster centroids of cluster_centroids_pca_compute

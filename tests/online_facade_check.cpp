@@ -6,7 +6,8 @@
 //                            <out dir> [time_behind_ms first_count lag small_skip]
 // The clock starts at first_count; each wait moves it to the awaited count and then lag samples
 // further (a stream that runs ahead of the sorter).  Writes <out dir>/spikeOutput.txt through the
-// facade and prints one line "batches B skips S spikes N processed P".
+// facade and prints one line "batches B skips S spikes N processed P channels M first F last L"
+// (M channels handed to the socket, F and L the first and last of them).
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -24,7 +25,8 @@ public:
         now_ = std::min(first, n_);
     }
     lss::t_ull getStreamSampleCt(int, lss::OSSSpecificParams) override { return (lss::t_ull)now_; }
-    lss::t_ull fetchFromPlace(int16_t* out, lss::OSSSpecificParams, lss::t_ull start, long maxs) override {
+    lss::t_ull fetchFromPlace(int16_t* out, lss::OSSSpecificParams os, lss::t_ull start, long maxs) override {
+        chans_ = os.vImecChannels;
         const long s = (long)start;
         const long k = std::max(0L, std::min(maxs, now_ - s));
         std::copy(d_.begin() + s * C_, d_.begin() + (s + k) * C_, out);
@@ -36,8 +38,10 @@ public:
         now_ = std::min(now_ + lag_, n_);
     }
     bool isRunning() override { return !ended_; }
+    const std::vector<int>& channels() const { return chans_; }
 
 private:
+    std::vector<int> chans_;   // the channel list of the last fetch
     std::vector<int16_t> d_;
     long C_, lag_, n_, now_;
     bool ended_ = false;
@@ -80,8 +84,10 @@ int main(int argc, char** argv) {
         lss::OnlineSpikesV2 oss(p, [&](const lss::OnlineSpikesPayload& pl) { spikes += (long)pl.Times.size(); },
                                 &sock);
         oss.runSpikeSorting();
-        std::printf("batches %ld skips %ld spikes %ld processed %ld\n", oss.batches(), oss.skips(), spikes,
-                    oss.processedCt());
+        const std::vector<int>& ch = sock.channels();
+        std::printf("batches %ld skips %ld spikes %ld processed %ld channels %zu first %d last %d\n", oss.batches(),
+                    oss.skips(), spikes, oss.processedCt(), ch.size(), ch.empty() ? -1 : ch.front(),
+                    ch.empty() ? -1 : ch.back());
     } catch (const std::exception& e) {
         std::fprintf(stderr, "error: %s\n", e.what());
         return 1;

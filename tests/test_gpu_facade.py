@@ -106,6 +106,7 @@ def test_facade_matches_python_path(facade_inputs, mn, mx, tb, first, lag, small
     sched = _schedule(rec.shape[0], first, mn, mx, nt, tb, lag, small)
     assert got_batches == len(sched) and got_skips == sum(s for _, _, s in sched)
     assert int(f[7]) == sched[-1][0] + sched[-1][1]
+    assert (int(f[9]), int(f[11]), int(f[13])) == (rec.shape[1], 0, rec.shape[1] - 1)   # channelMap.npy
     with Sorter.from_dir(art_dir, max_window=mx + 2 * 127, max_passes=100, max_spikes=1 << 16) as s:
         exp = _expected(s, rec, sched, nt, s.T)
     got = _read(out / "spikeOutput.txt")
@@ -120,7 +121,8 @@ def test_facade_matches_python_path(facade_inputs, mn, mx, tb, first, lag, small
 def test_facade_labels_from_directory_centroids(synth1, tmp_path_factory):
     """With cluster_centroids.npy in the artifact directory the facade labels spikes by the nearest
     centroid (OnlineSpikesV2::loadKilosortClusteringData / closestCluster) and deduplicates per
-    label."""
+    label; the socket is handed channelMap.npy's channels, and a channel map of the wrong length
+    is refused."""
     from livespikesorter_amd.export import write_oss_input
     from livespikesorter_amd.output import load_cluster_centroids
     from livespikesorter_amd.sorter import Sorter
@@ -131,7 +133,9 @@ def test_facade_labels_from_directory_centroids(synth1, tmp_path_factory):
     rng = np.random.default_rng(21)
     xc, yc = np.asarray(synth1.art["xc"]), np.asarray(synth1.art["yc"])
     xy = np.stack([rng.uniform(xc.min(), xc.max(), 12), rng.uniform(yc.min(), yc.max(), 12)], 1)
-    art_dir = write_oss_input(synth1.art, d / "oss_input", cluster_xy=xy)
+    C = np.asarray(synth1.art["U"]).shape[2]
+    cmap = np.arange(C)[::-1] + 100      # the channel list handed to the socket
+    art_dir = write_oss_input(synth1.art, d / "oss_input", cluster_xy=xy, channel_map=cmap)
     rec = np.ascontiguousarray(synth1.rec[:30000], np.int16)
     rec.tofile(d / "rec.i16")
     out = d / "out"
@@ -139,6 +143,8 @@ def test_facade_labels_from_directory_centroids(synth1, tmp_path_factory):
     r = subprocess.run([str(EXE), str(art_dir), str(d / "rec.i16"), str(rec.shape[1]), "3000", "3000", str(out)],
                        capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
+    f = r.stdout.split()
+    assert (int(f[9]), int(f[11]), int(f[13])) == (C, int(cmap[0]), int(cmap[-1]))
     sched = _schedule(rec.shape[0], 0, 3000, 3000, 61, 100000, 0, 0)
     cents = load_cluster_centroids(art_dir)
     with Sorter.from_dir(art_dir, max_window=3000 + 2 * 127, max_passes=100, max_spikes=1 << 16) as s:
@@ -147,3 +153,8 @@ def test_facade_labels_from_directory_centroids(synth1, tmp_path_factory):
     assert len(got) == len(exp) > 0
     np.testing.assert_array_equal(got, exp)
     assert got[:, 1].max() < len(xy)
+    # a channel map of the wrong length is refused, as OnlineSpikesV2::loadChannelMap refuses it
+    np.save(art_dir / "channelMap.npy", np.arange(C - 1, dtype=np.int32))
+    r = subprocess.run([str(EXE), str(art_dir), str(d / "rec.i16"), str(rec.shape[1]), "3000", "3000", str(out)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "channelMap.npy" in r.stderr, (r.returncode, r.stderr)

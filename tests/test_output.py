@@ -89,3 +89,34 @@ def test_load_cluster_centroids(tmp_path, small_synth):
     buf = np.zeros(2, np.float32)
     st = lib().lss_load_cluster_centroids(str(d).encode(), buf.ctypes.data, buf.ctypes.data, 2, ct.byref(n))
     assert st == 4 and n.value == 3
+
+
+def test_load_channel_map(tmp_path, small_synth):
+    """channelMap.npy (the channels OnlineSpikesV2 hands its DataSocket): written by the exporter
+    (the channels 0..C-1 by default, or the given list), read back from int32, int64 and float
+    files; missing files, negative or fractional entries and too-small outputs are reported."""
+    import ctypes as ct
+    import pytest
+    from livespikesorter_amd._lib import LssError, lib
+    from livespikesorter_amd.export import write_oss_input
+    from livespikesorter_amd.output import load_channel_map
+    C = np.asarray(small_synth.art["U"]).shape[2]
+    d = write_oss_input(small_synth.art, tmp_path / "a")
+    np.testing.assert_array_equal(load_channel_map(d), np.arange(C))
+    cm = (np.arange(C)[::-1] * 3 + 7).astype(np.int64)
+    d = write_oss_input(small_synth.art, tmp_path / "b", channel_map=cm)
+    np.testing.assert_array_equal(load_channel_map(d), cm)
+    np.save(d / "channelMap.npy", cm.astype(np.float64))
+    np.testing.assert_array_equal(load_channel_map(d), cm)
+    for bad in (np.array([1, -2, 3]), np.array([0.5, 1.0])):
+        np.save(d / "channelMap.npy", bad)
+        with pytest.raises(LssError, match="IO"):
+            load_channel_map(d)
+    with pytest.raises(LssError, match="IO"):
+        load_channel_map(tmp_path / "missing")
+    with pytest.raises(ValueError):
+        write_oss_input(small_synth.art, tmp_path / "c", channel_map=np.arange(C + 1))
+    np.save(d / "channelMap.npy", cm.astype(np.int32))
+    n = ct.c_int32()
+    buf = np.zeros(2, np.int32)
+    assert lib().lss_load_channel_map(str(d).encode(), buf.ctypes.data, 2, ct.byref(n)) == 4 and n.value == C
