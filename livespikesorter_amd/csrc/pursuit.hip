@@ -115,15 +115,29 @@ __global__ __launch_bounds__(256) void score_all_kernel(PursuitArgs a) {
 constexpr int kFlagCols = 1024;
 constexpr int kPeakHalo = 128;                       // >= nt
 
-// p points at the column's Cfmax in a buffer holding its +-nt neighbours (-inf outside the window)
+// p points at the column's Cfmax in a buffer holding its +-nt neighbours (-inf outside the window).
+// The neighbours are read 8 distances at a time (all 16 loads in flight together); distances past
+// nt read the distance-nt values again, which changes neither the test nor the maximum.
+constexpr int kPeakStep = 8;
 __device__ __forceinline__ bool peak_test(const float* p, float th2, int nt) {
     const float cf = p[0];
     if (!((double)cf > (double)th2 - 2e-9) || p[-1] - cf >= 1e-9f || p[1] - cf >= 1e-9f) return false;
     float cm = fmaxf(cf, fmaxf(p[-1], p[1]));
-    for (int d = 2; d <= nt; ++d) {
-        const float vl = p[-d], vr = p[d];
-        if (vl - cf >= 1e-9f || vr - cf >= 1e-9f) return false;
-        cm = fmaxf(cm, fmaxf(vl, vr));
+    for (int d0 = 2; d0 <= nt; d0 += kPeakStep) {
+        float vl[kPeakStep], vr[kPeakStep];
+#pragma unroll
+        for (int j = 0; j < kPeakStep; ++j) {
+            const int d = min(d0 + j, nt);
+            vl[j] = p[-d];
+            vr[j] = p[d];
+        }
+        bool bad = false;
+#pragma unroll
+        for (int j = 0; j < kPeakStep; ++j) {
+            bad |= (vl[j] - cf >= 1e-9f) | (vr[j] - cf >= 1e-9f);
+            cm = fmaxf(cm, fmaxf(vl[j], vr[j]));
+        }
+        if (bad) return false;
     }
     return cm > th2 && fabsf(cm - cf) < 1e-9f;
 }
