@@ -1,0 +1,99 @@
+"""The template-length limits of lss_params.nt (odd, 3 <= nt <= 127): at nt = 127 the peak test
+reads its full 128-column halo, the projection its widest band and the pursuit 255 ctc lags.
+
+* nt = 127, the pursuit given identical FP32 scores, residual and tables: bit for bit against the
+  oracle's run_matching (ks/template_matching.py:157-198) on an 8254-sample window (per-pass
+  kernels) and a 4000-sample window (persistent kernel, and two per-pass rounds before it);
+* nt = 127 and nt = 3, the whole window: FP32 against the FP64 oracle (spike F1 >= 0.999 within
+  +-1 sample and same template, matched amplitudes within 1e-3 relative) and bf16x3 against FP32
+  (the same bar)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import ks_oracle as ko
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+NT = 8000
+LIMITS = {127: (42, 6), 3: (1, 3)}   # nt: (nt0min, PCs)
+
+
+def _synth(nt):
+    from livespikesorter_amd import synth
+    nt0, K = LIMITS[nt]
+    return synth.generate(synth.SynthConfig(n_chan=64, n_units=20, n_samples=2 * NT, NT=NT, nt=nt, nt0min=nt0,
+                                            n_pcs=K, rate_lo=20.0, rate_hi=60.0, seed=11))
+
+
+@pytest.fixture(scope="module")
+def s127():
+    return _synth(127)
+
+
+def _amp_err(ref_s, ref_t, ref_a, got):
+    ref = {(int(s), int(t)): a for s, t, a in zip(ref_s, ref_t, ref_a)}
+    err = [abs(a - ref[(int(s), int(t))]) / abs(ref[(int(s), int(t))])
+           for s, t, a in zip(got.sample, got.template, got.amplitude) if (int(s), int(t)) in ref]
+    return (max(err) if err else 0.0), len(err)
+
+
+@pytest.mark.parametrize("W,grid,first", [(8254, 0, 0), (4000, 128, 0), (4000, 64, 2)])
+def test_pursuit_bit_exact_nt127(s127, W, grid, first):
+    from livespikesorter_amd.sorter import Sorter
+    print(f"case: nt=127 pursuit bit-exact W={W} persistent grid {grid} after {first}", flush=True)
+    a = s127.art
+    nt = 127
+    X = ko.padded_batch(s127.rec, 1, NT, nt, torch.float64)
+    ws = ko.window_start(1, NT, nt)
+    Wm = torch.as_tensor(a["drift"]).double() @ torch.as_tensor(a["Wrot"]).double()
+    x32 = ko.filter_batch(X, torch.from_numpy(a["hp_filter"]).double(), Wm).float()
+    if W < x32.shape[1]:
+        x32 = x32[:, 2000:2000 + W].contiguous()
+        ws = ws + 2000
+    U32, w32 = torch.from_numpy(a["U"]), torch.from_numpy(a["wPCA"])
+    B32 = ko.project(x32, U32, w32)
+    with Sorter(a, max_window=X.shape[1], max_passes=100, max_spikes=1 << 15) as s:
+        s.set_persistent(grid, first)
+        tb = s.tables()
+        ctc = torch.from_numpy(tb["ctc"]).permute(2, 0, 1).contiguous()
+        ref = ko.run_matching(x32, U32, w32, ctc, Th=a["Th"], max_passes=100, nm=torch.from_numpy(tb["nm"]),
+                              waves=torch.from_numpy(tb["waves"]), B=B32)
+        Bd = B32.T.contiguous().to(DEV)
+        Xd = x32.T.contiguous().to(DEV)
+        got = s.stage_pursuit(Bd, Xd, ws)
+        assert got.passes == ref.passes
+        keep = ko.spike_samples(ref.st, ws, nt, a["nt0min"]).numpy() >= 0
+        st = ref.st.numpy()
+        assert keep.sum() > 20
+        np.testing.assert_array_equal(got.column, st[keep, 0])
+        np.testing.assert_array_equal(got.template, st[keep, 1])
+        np.testing.assert_array_equal(got.amplitude, ref.amps[keep, 0].numpy())
+        assert np.array_equal(Bd.cpu().numpy().T, ref.B.numpy())
+        assert np.array_equal(Xd.cpu().numpy().T, ref.Xres.numpy())
+
+
+@pytest.mark.parametrize("nt", [127, 3])
+def test_window_at_nt_limit(s127, nt):
+    from livespikesorter_amd.sorter import Sorter
+    print(f"case: nt={nt} whole window fp32 vs fp64 oracle, bf16x3 vs fp32", flush=True)
+    s = s127 if nt == 127 else _synth(nt)
+    a = s.art
+    X = ko.padded_batch(s.rec, 1, NT, nt, torch.float64)
+    win = np.ascontiguousarray(X.numpy().T.astype(np.int16))
+    ws = ko.window_start(1, NT, nt)
+    ref = ko.sort_window(X, a, ws, dtype=torch.float64, max_passes=100)
+    assert len(ref.samples) > 50
+    with Sorter(a, max_window=win.shape[0], max_passes=100) as so:
+        got = so.sort_window(win, ws)
+    f1 = ko.spike_f1(ref.samples, ref.templates, got.sample, got.template, dt=1)
+    assert f1 >= 0.999, f1
+    err, n = _amp_err(ref.samples, ref.templates, ref.amps, got)
+    assert n >= 0.99 * len(ref.samples) and err <= 1e-3, (n, err)
+    with Sorter(a, max_window=win.shape[0], max_passes=100, precision="bf16x3") as so:
+        x3 = so.sort_window(win, ws)
+    f1 = ko.spike_f1(got.sample, got.template, x3.sample, x3.template, dt=1)
+    assert f1 >= 0.999, f1
+    err, n = _amp_err(got.sample, got.template, got.amplitude, x3)
+    assert n >= 0.99 * len(got) and err <= 1e-3, (n, err)
