@@ -149,34 +149,50 @@ __global__ __launch_bounds__(512) void car_kernel(const int16_t* __restrict__ wi
         mean[i] = (c < C) ? (float)((double)sums[c] * invW) : 0.f;
     }
     const int k = (C - 1) / 2;   // lower median rank (torch.median)
-    for (int si = wave; si < kCarSamples; si += 8) {
-        int64_t w = w0 + si;
-        if (w >= W) break;
-        float v[VPL];
-        uint32_t key[VPL];
+    // two samples per step (si and si + 8), their loads and searches interleaved
+    for (int si = wave; si < kCarSamples; si += 16) {
+        const int64_t wa = w0 + si, wb = wa + 8;
+        if (wa >= W) break;
+        const bool hb = wb < W;
+        float va[VPL], vb[VPL];
+        uint32_t ka[VPL], kb[VPL];
 #pragma unroll
         for (int i = 0; i < VPL; ++i) {
-            int c = lane + 64 * i;
-            v[i] = (c < C) ? ((float)win[w * C + c] - mean[i]) : 0.f;
-            key[i] = (c < C) ? f2key(v[i]) : 0xffffffffu;
+            const int c = lane + 64 * i;
+            va[i] = (c < C) ? ((float)win[wa * C + c] - mean[i]) : 0.f;
+            vb[i] = (c < C && hb) ? ((float)win[wb * C + c] - mean[i]) : 0.f;
+            ka[i] = (c < C) ? f2key(va[i]) : 0xffffffffu;
+            kb[i] = (c < C) ? f2key(vb[i]) : 0xffffffffu;
         }
-        float med = 0.f;
+        float meda = 0.f, medb = 0.f;
         if (do_car) {
-            // smallest key K with #{key <= K} >= k+1
-            uint32_t lo = 0u, hi = 0xffffffffu;
-            while (lo < hi) {
-                uint32_t mid = lo + ((hi - lo) >> 1);
-                int cnt = 0;
+            // the smallest key K with #{key <= K} >= k+1, one bit per step from the top: bit b
+            // stays 0 when the largest key with the bits above fixed and bit b clear already
+            // counts k+1
+            uint32_t Ka = 0u, Kb = 0u;
+#pragma unroll 2
+            for (int b = 31; b >= 0; --b) {
+                const uint32_t m = (1u << b) - 1u;
+                const uint32_t ta = Ka | m, tb = Kb | m;
+                int ca = 0, cb = 0;
 #pragma unroll
-                for (int i = 0; i < VPL; ++i) cnt += __popcll(__ballot(key[i] <= mid));
-                if (cnt >= k + 1) hi = mid; else lo = mid + 1;
+                for (int i = 0; i < VPL; ++i) {
+                    ca += __popcll(__ballot(ka[i] <= ta));
+                    cb += __popcll(__ballot(kb[i] <= tb));
+                }
+                if (ca < k + 1) Ka |= 1u << b;
+                if (cb < k + 1) Kb |= 1u << b;
             }
-            med = key2f(lo);
+            meda = key2f(Ka);
+            medb = key2f(Kb);
         }
 #pragma unroll
         for (int i = 0; i < VPL; ++i) {
-            int c = lane + 64 * i;
-            if (c < C) tile[c * (kCarSamples + 1) + si] = v[i] - med;
+            const int c = lane + 64 * i;
+            if (c < C) {
+                tile[c * (kCarSamples + 1) + si] = va[i] - meda;
+                if (hb) tile[c * (kCarSamples + 1) + si + 8] = vb[i] - medb;
+            }
         }
     }
     __syncthreads();
