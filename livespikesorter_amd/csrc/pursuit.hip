@@ -885,14 +885,18 @@ hipError_t launch_residual(const PursuitArgs& a, hipStream_t s) {
 // once, then lane (j, k) forms sum_m X[w - nt/2 + m][iCC[j, iU[t]]] wPCA[k][m] + amp Ucc[j][t][k].
 // ------------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void features_kernel(FeatureArgs a) {
-    extern __shared__ float fsm[];   // wpca [K*nt] | per wavefront: window [nt][nC], features [nC*K]
+    // wpca [K*nt] | per wavefront: window [nt][nC], features [nC*K], channel mass, x, y [nC]
+    extern __shared__ float fsm[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int n = a.counters ? a.counters[0] + a.counters[1] : a.n;
     if (n > a.cap) n = a.cap;
     const int h = a.nt / 2, K = a.K, nC = a.nC, nt = a.nt;
     float* wp = fsm;
-    float* xwin = fsm + K * nt + wave * (nt * nC + nC * K);
+    float* xwin = fsm + K * nt + wave * (nt * nC + nC * K + 3 * nC);
     float* sf = xwin + nt * nC;
+    float* sm = sf + nC * K;
+    float* sx = sm + nC;
+    float* sy = sx + nC;
     for (int e = threadIdx.x; e < K * nt; e += 256) wp[e] = a.wpca[e];
     __syncthreads();
     for (int s0 = blockIdx.x * 4; s0 < n; s0 += gridDim.x * 4) {
@@ -922,21 +926,26 @@ __global__ __launch_bounds__(256) void features_kernel(FeatureArgs a) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            // lane j: channel j's mass sum_k F^2 and site; lane 0 then sums in channel order
+            if (lane < nC) {
+                float m = 0.f;
+                for (int k = 0; k < K; ++k) m += sf[lane * K + k] * sf[lane * K + k];
+                const int ch = a.iCC[lane * a.C + iu];
+                sm[lane] = m;
+                sx[lane] = a.xc[ch];
+                sy[lane] = a.yc[ch];
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
             if (lane == 0) {
                 float tot = 0.f;
-                for (int j = 0; j < nC; ++j) {
-                    float m = 0.f;
-                    for (int k = 0; k < K; ++k) m += sf[j * K + k] * sf[j * K + k];
-                    tot += m;
-                }
+                for (int j = 0; j < nC; ++j) tot += sm[j];
                 float xs = 0.f, ys = 0.f;
                 for (int j = 0; j < nC; ++j) {
-                    float m = 0.f;
-                    for (int k = 0; k < K; ++k) m += sf[j * K + k] * sf[j * K + k];
-                    const int ch = a.iCC[j * a.C + iu];
-                    const float wgt = m / tot;
-                    xs += a.xc[ch] * wgt;
-                    ys += a.yc[ch] * wgt;
+                    const float wgt = sm[j] / tot;
+                    xs += sx[j] * wgt;
+                    ys += sy[j] * wgt;
                 }
                 a.xs[s] = xs;
                 a.ys[s] = ys;
@@ -950,7 +959,7 @@ __global__ __launch_bounds__(256) void features_kernel(FeatureArgs a) {
 
 hipError_t launch_features(const FeatureArgs& a, int grid, hipStream_t s) {
     if (a.nC * a.K > 32 * 8 || a.nC > 32) return hipErrorInvalidValue;
-    const size_t lds = sizeof(float) * ((size_t)a.K * a.nt + 4 * ((size_t)a.nt * a.nC + a.nC * a.K));
+    const size_t lds = sizeof(float) * ((size_t)a.K * a.nt + 4 * ((size_t)a.nt * a.nC + a.nC * a.K + 3 * a.nC));
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void*)features_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)lds);
