@@ -21,7 +21,15 @@ are in ``tests/test_oracle.py``):
 * ``prepare_extract``     - brute force on a 4-channel probe.
 * ``run_matching``        - the reference's own integer KAT (``src/Python/conv1d_test.py``:14-43)
                             for projection/contraction/score/max-pool; brute-force single-spike
-                            recovery (exact amplitude, exact residual); explicit tie cases.
+                            recovery (exact amplitude, exact residual); explicit tie cases; the
+                            multi-spike update against a pure-Python spike-by-spike pursuit that
+                            rescores the projection of the residual every pass (a tiny integer
+                            case with an exactly tied plateau and two peaks of one pass
+                            nt < d <= 2nt apart: spike list, amplitudes, residual and final
+                            scores); the linearity invariant B = project(Xres) after the last
+                            pass of a crowded FP64 window; ``_sub_windows`` against torch's own
+                            in-place indexed subtract with repeated columns.  Both pursuit checks
+                            fail when the even/odd split is dropped (``UPDATE_GROUPS = 1``).
 * ``extract_features``    - brute force on tiny inputs.
 * ``spike_positions``     - closed form (all mass on one channel gives that channel's position).
 * ``nmatch``              - hand-checked lists.
@@ -218,6 +226,13 @@ def _sub_windows(A: torch.Tensor, cols: torch.Tensor, vals: torch.Tensor):
         A[:, cols[s]] = snap[:, s] - vals[:, s]
 
 
+# Peaks of one pass are subtracted in this many interleaved groups (ks/template_matching.py:194:
+# n = 2, even-indexed peaks first).  Consecutive peaks of a pass can sit nt < d <= 2nt apart, so
+# their +-nt score windows overlap; members of one group are two peaks apart (> 2nt) and never
+# overlap unless peaks tie exactly.
+UPDATE_GROUPS = 2
+
+
 @dataclass
 class MatchResult:
     st: torch.Tensor        # [nsp][2] int64: (padded column iX, template)
@@ -261,7 +276,7 @@ def run_matching(X: torch.Tensor, U: torch.Tensor, wPCA: torch.Tensor, ctc: torc
         st_l.append(torch.stack((iX, iY), 1))
         amp_l.append(amp)
         per_pass.append(int(iX.numel()))
-        n = 2
+        n = UPDATE_GROUPS
         for j in range(n):
             ix, iy, a = iX[j::n], iY[j::n], amp[j::n]
             if ix.numel() == 0:

@@ -234,6 +234,157 @@ def test_sub_windows_duplicates_last_wins():
     assert A[0].tolist() == [0, -1, -10, -10, -10, 0]
 
 
+# ---- multi-spike update order (ks/template_matching.py:192-198) ------------------------------
+def _project_bruteforce(X, U, W):
+    """B[t][w] = sum_{k,c} U[t,k,c] sum_m X[c, w+m-nt//2] W[k,m], zero outside [0, n)."""
+    C, n = X.shape
+    T, K, _ = U.shape
+    nt = W.shape[1]
+    h = nt // 2
+    P = [[[sum(X[c][w + m - h] * W[k][m] for m in range(nt) if 0 <= w + m - h < n)
+           for w in range(n)] for k in range(K)] for c in range(C)]
+    return np.array([[sum(U[t][k][c] * P[c][k][w] for k in range(K) for c in range(C))
+                      for w in range(n)] for t in range(T)])
+
+
+def _pursuit_bruteforce(X, U, W, Th, max_passes):
+    """Greedy pursuit by definition: each pass scores the projection of the current residual,
+    takes every column whose score is the maximum of its +-nt neighbourhood and above Th^2
+    (ascending), amplitudes from the pass's scores, then subtracts each spike's whole waveform
+    from the residual one spike at a time.  No ctc table and no update groups."""
+    X = np.array(X, np.float64)
+    C, n = X.shape
+    T, K, _ = U.shape
+    nt = W.shape[1]
+    h = nt // 2
+    nm = [float(sum(U[t][k][c] ** 2 for k in range(K) for c in range(C))) for t in range(T)]
+    wave = [[[sum(U[t][k][c] * W[k][m] for k in range(K)) for c in range(C)] for m in range(nt)]
+            for t in range(T)]
+    spikes, amps, per_pass = [], [], []
+    for _ in range(max_passes):
+        B = _project_bruteforce(X, U, W)
+        Cfmax, imax = [], []
+        for w in range(n):
+            cf = [0.0 if (w < nt or w >= n - nt) else max(B[t][w], 0.0) ** 2 / nm[t] for t in range(T)]
+            m = max(cf)
+            Cfmax.append(m)
+            imax.append(cf.index(m))
+        peaks = []
+        for w in range(n):
+            pooled = max(Cfmax[max(0, w - nt):w + nt + 1])
+            if pooled > Th ** 2 and abs(pooled - Cfmax[w]) < 1e-9:
+                peaks.append(w)
+        if not peaks:
+            break
+        pass_amps = [B[imax[w]][w] / nm[imax[w]] for w in peaks]
+        for w, a in zip(peaks, pass_amps):
+            for m in range(nt):
+                for c in range(C):
+                    X[c][w - h + m] -= a * wave[imax[w]][m][c]
+            spikes.append((w, imax[w]))
+            amps.append(a)
+        per_pass.append(len(peaks))
+    return spikes, np.array(amps), X, per_pass, _project_bruteforce(X, U, W)
+
+
+def _crowded_tiny_case(seed=5, C=4, T=3, K=2, nt=9, n=96):
+    """Integer data (exact FP64 sums).  A block of nt+1 equal columns at 20.. gives the scores a
+    two-column plateau (columns 24, 25: an exact tie, both peaks), a planted template nt < d <= 2nt
+    after it puts two peaks of the first pass in overlapping +-nt score windows."""
+    rng = np.random.default_rng(seed)
+    W = rng.integers(-3, 4, (K, nt)).astype(np.float64)
+    U = rng.integers(-3, 4, (T, K, C)).astype(np.float64)
+    X = rng.integers(-2, 3, (C, n)).astype(np.float64)
+    a, h = 20, nt // 2
+    X[:, a:a + nt + 1] = rng.integers(-9, 10, C)[:, None]
+    wave = np.einsum("tkc,km->tmc", U, W)
+    for w, t, amp in [(a + h + 1 + int(rng.integers(nt + 1, 2 * nt + 1)), int(rng.integers(T)), float(rng.integers(3, 6))),
+                      (70 + int(rng.integers(0, 8)), int(rng.integers(T)), float(rng.integers(3, 6)))]:
+        if w + h < n - nt:
+            X[:, w - h:w + h + 1] += amp * wave[t].T
+    return X, U, W
+
+
+def _match_tiny(X, U, W, Th, passes):
+    Xt, Ut, Wt = map(torch.from_numpy, (X, U, W))
+    return ko.run_matching(Xt, Ut, Wt, ko.prepare_matching(Ut, Wt), Th=Th, max_passes=passes)
+
+
+def _check_against_bruteforce(X, U, W, Th=2.0, passes=4):
+    sp, amps, Xres, per_pass, B = _pursuit_bruteforce(X, U, W, Th, passes)
+    r = _match_tiny(X, U, W, Th, passes)
+    assert r.per_pass == per_pass
+    assert [tuple(s) for s in r.st.tolist()] == sp
+    np.testing.assert_allclose(r.amps[:, 0].numpy(), amps, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(r.Xres.numpy(), Xres, rtol=0, atol=1e-12 * np.abs(Xres).max())
+    np.testing.assert_allclose(r.B.numpy(), B, rtol=0, atol=1e-12 * np.abs(B).max())
+
+
+def test_run_matching_equals_spike_by_spike_pursuit():
+    """Two peaks of one pass whose +-nt score windows overlap (nt < d <= 2nt) and an exactly tied
+    plateau: run_matching's grouped update equals subtracting every spike's whole waveform."""
+    X, U, W = _crowded_tiny_case()
+    nt = W.shape[1]
+    r = _match_tiny(X, U, W, 2.0, 1)
+    first = r.st[:, 0].tolist()
+    d = np.diff(first)
+    assert 1 in d.tolist()                                      # the tied plateau: both columns
+    assert any(nt < x <= 2 * nt for x in d)                     # overlapping score windows
+    _check_against_bruteforce(X, U, W)
+
+
+def test_torch_indexed_subtract_keeps_last_duplicate():
+    """The reference's in-place ``A[:, idx] -= v`` (template_matching.py:197-198) with repeated
+    columns in idx: on CPU torch the last occurrence's write stays, which ``_sub_windows``
+    emulates for exactly tied peaks."""
+    rng = np.random.default_rng(21)
+    for _ in range(20):
+        n, L = int(rng.integers(2, 6)), int(rng.integers(2, 8))
+        starts = np.sort(rng.integers(0, 10, n))
+        cols = torch.from_numpy(starts[:, None] + np.arange(L))
+        A = torch.from_numpy(rng.normal(size=(3, 20)))
+        vals = torch.from_numpy(rng.normal(size=(3, n, L)))
+        lit = A.clone()
+        lit[:, cols] -= vals
+        ko._sub_windows(A, cols, vals)
+        np.testing.assert_array_equal(A.numpy(), lit.numpy())
+
+
+def test_pursuit_scores_stay_projection_of_residual():
+    """Linearity: after every pass the scores equal the projection of the residual (B = U.Xproj
+    of Xres), on a crowded FP64 window (many pairs of one pass nt < d <= 2nt apart)."""
+    from livespikesorter_amd import synth
+    s = synth.generate(synth.SynthConfig(n_chan=32, n_units=12, n_samples=20000, NT=10000,
+                                         nrange=16, nC=4, rate_lo=40.0, rate_hi=80.0, seed=4))
+    a = s.art
+    X = ko.padded_batch(s.rec, 1, a["NT"], a["nt"], D)
+    x = ko.filter_batch(X, torch.from_numpy(a["hp_filter"]), torch.from_numpy(a["Wrot"]).double())
+    U, W = torch.from_numpy(a["U"]).double(), torch.from_numpy(a["wPCA"]).double()
+    nt = W.shape[1]
+    r = ko.run_matching(x, U, W, ko.prepare_matching(U, W), Th=a["Th"], max_passes=100)
+    close = 0
+    off = 0
+    for n in r.per_pass:
+        cols = r.st[off:off + n, 0].numpy()
+        d = np.diff(cols)
+        close += int(((d > nt) & (d <= 2 * nt)).sum())
+        off += n
+    assert close >= 20, close
+    Bp = ko.project(r.Xres, U, W)
+    assert float((r.B - Bp).abs().max()) <= 1e-9 * float(Bp.abs().max())
+
+
+def test_update_checks_catch_a_single_group(monkeypatch):
+    """The two checks above fail when the even/odd split is dropped (one group: overlapping
+    windows keep only the later spike's write)."""
+    monkeypatch.setattr(ko, "UPDATE_GROUPS", 1)
+    X, U, W = _crowded_tiny_case()
+    with pytest.raises(AssertionError):
+        _check_against_bruteforce(X, U, W)
+    with pytest.raises(AssertionError):
+        test_pursuit_scores_stay_projection_of_residual()
+
+
 def test_extract_features_and_positions_bruteforce():
     rng = np.random.default_rng(6)
     C, T, K, nt, Wn, nC = 6, 3, 2, 5, 40, 3
