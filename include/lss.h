@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define LSS_ABI_VERSION 1
+#define LSS_ABI_VERSION 2
 
 typedef enum lss_status {
     LSS_OK = 0,
@@ -66,6 +66,8 @@ typedef struct lss_params {
     double th_learned;     /* Th (Kilosort's Th_learned, a double): a peak needs relu(B)^2/nm >
                               (float)(Th*Th), the reference's FP32 comparison with Th**2 */
     float fs;              /* sampling rate (Hz) */
+    int32_t dup_bins;      /* duplicate window of the output step in samples (misc.txt
+                              duplicate_spike_bins, OnlineSpikesV2.cu:456; default 7) */
 } lss_params;
 
 /* Kilosort4 outputs as host float32 row-major arrays (the oss_input files,
@@ -79,6 +81,13 @@ typedef struct lss_artifacts {
     int32_t hp_len;
     const float* xc;         /* [C] channel x (um) */
     const float* yc;         /* [C] channel y (um) */
+    /* Optional feature-extraction tables of prepare_extract (kilosort/template_matching.py:20-26),
+     * as the exporter writes them (run_online_spikes.py:533, 572-574) and OnlineSpikesV2 loads them
+     * (OnlineSpikesV2.cu:561-563); NULL = built from U and the channel sites (iCC with ties broken
+     * by channel index).  A template refinement step rebuilds iU and Ucc from the refined U. */
+    const int32_t* iCC;      /* [nC][C] nearest channels of each channel, or NULL */
+    const int32_t* iU;       /* [T] channel of largest template energy, or NULL */
+    const float* Ucc;        /* [nC][T][K] U at the nearest channels of iU, or NULL */
 } lss_artifacts;
 
 /* Caller-owned host output buffers of one window. */
@@ -107,10 +116,25 @@ lss_status lss_default_params(lss_params* p);
  * (replaces OnlineSpikesV2::initializeSorter / initializeStaticMemory, OnlineSpikesV2.cu:321-630). */
 lss_status lss_create(const lss_params* p, const lss_artifacts* a, void** handle);
 
-/* Same, reading an oss_input directory (Wall3.npy, wPCA.npy, whiteningMat.npy or
- * whitening_mat.npy, drift_matrix.npy, hp_filter.npy, xc.npy, yc.npy, misc.txt);
- * p supplies everything misc.txt and the array shapes do not. */
+/* Same, reading an oss_input directory as OnlineSpikesV2::loadKilosortTrainingData does
+ * (OnlineSpikesV2.cu:435-457, 546-610): Wall3.npy, wPCA.npy, whiteningMat.npy or whitening_mat.npy,
+ * drift_matrix.npy, hp_filter.npy, xc.npy, yc.npy, misc.txt (nt0min, numNearestChans, Th_learned,
+ * duplicate_spike_bins), and when present iCC.npy [nC][C], iU.npy [T] (any integer dtype) and
+ * Ucc.npy [nC][T][K], which are used as loaded.  A present ctc.npy [T][T][2nt+1] is compared with
+ * the table rebuilt in FP64 (lss_get_load_info); the rebuilt one is used.  p supplies everything
+ * misc.txt and the array shapes do not. */
 lss_status lss_create_from_dir(const char* dir, const lss_params* p, void** handle);
+
+/* What lss_create / lss_create_from_dir took from the caller or the directory. */
+typedef struct lss_load_info {
+    int32_t icc_given;         /* iCC came from the caller / iCC.npy (else built) */
+    int32_t iu_given;          /* iU likewise */
+    int32_t ucc_given;         /* Ucc likewise */
+    int32_t ctc_in_dir;        /* ctc.npy was present and compared */
+    double ctc_max_abs_diff;   /* max |ctc.npy - FP64 rebuild| (0 without ctc.npy) */
+    double ctc_max_abs;        /* max |FP64 rebuild| */
+} lss_load_info;
+lss_status lss_get_load_info(void* handle, lss_load_info* info);
 
 lss_status lss_destroy(void* handle);
 /* Makes `device` the calling thread's current HIP device (OnlineSpikesV2::initializeSorter's
@@ -251,8 +275,9 @@ lss_status lss_stream_stats(void* stream, float* last_ms, int64_t* batches);
  * [5] total. */
 lss_status lss_stream_breakdown(void* stream, float* ms);
 /* Template refinement while streaming: every `every_batches` steps (0 = never) a refinement round
- * starts in which probe i's handle takes one lss_refine_templates step at the end of the round's
- * i-th step, on that step's window (drift tracking); the refining step's wall time includes it.
+ * starts (unless one is still under way, which carries on) in which probe i's handle takes one
+ * lss_refine_templates step at the end of the round's i-th step, on that step's window (drift
+ * tracking), so every probe is refined once per round; the refining step's wall time includes it.
  * lss_stream_last_refine gives a probe's last info (4 doubles). */
 lss_status lss_stream_set_refine(void* stream, int32_t every_batches);
 lss_status lss_stream_last_refine(void* stream, int32_t probe, double* info);

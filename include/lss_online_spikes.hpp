@@ -86,7 +86,8 @@ struct InputParameters {
     int iMaxPasses = 50;                 // pursuit pass cap (OnlineSpikesV2.cu:903)
     int iPrecision = LSS_PREC_FP32;
     int iMaxSpikes = 1 << 16;            // spikes one batch may produce
-    int iDt = 7;                         // duplicate window in samples
+    int iDt = -1;                        // duplicate window in samples; < 0: misc.txt's
+                                         // duplicate_spike_bins (OnlineSpikesV2.cu:456)
     std::vector<float> vCentroidX, vCentroidY;   // cluster centroids (um); empty: those of
                                                  // sInputFolder/cluster_centroids.npy, or template ids
 };
@@ -150,6 +151,7 @@ private:
         C_ = got.n_chan;
         nt_ = got.nt;
         T_ = got.n_templates;
+        if (prm_.iDt < 0) prm_.iDt = got.dup_bins;
         os_.lNChans = C_;
         if (prm_.vChannelSubset.empty() && !prm_.sInputFolder.empty()) {
             // the artifact directory's channel list (channelMap.npy), when there is one

@@ -29,7 +29,8 @@ EXPORTS = ["lss_last_error", "lss_abi_version", "lss_default_params", "lss_creat
            "lss_refine_templates", "lss_stream_set_refine", "lss_stream_last_refine", "lss_stream_push_device",
            "lss_set_drift", "lss_covariance_add", "lss_whitening_from_covariance", "lss_whitening_local",
            "lss_drift_inverse", "lss_drift_matrix", "lss_extract_snippets", "lss_wpca", "lss_train_release",
-           "lss_select_device", "lss_sort_windows", "lss_cluster_centroids_pca", "lss_assign_clusters", "lss_load_cluster_centroids", "lss_load_channel_map"]
+           "lss_select_device", "lss_sort_windows", "lss_cluster_centroids_pca", "lss_assign_clusters", "lss_load_cluster_centroids", "lss_load_channel_map",
+           "lss_get_load_info"]
 
 
 class Params(ct.Structure):
@@ -37,13 +38,19 @@ class Params(ct.Structure):
                 ("nt", ct.c_int32), ("nt0min", ct.c_int32), ("n_nearest", ct.c_int32),
                 ("max_window", ct.c_int32), ("max_passes", ct.c_int32), ("max_spikes", ct.c_int32),
                 ("do_car", ct.c_int32), ("precision", ct.c_int32), ("drop_negative", ct.c_int32),
-                ("th_learned", ct.c_double), ("fs", ct.c_float)]
+                ("th_learned", ct.c_double), ("fs", ct.c_float), ("dup_bins", ct.c_int32)]
 
 
 class Artifacts(ct.Structure):
     _fields_ = [("U", ct.c_void_p), ("wPCA", ct.c_void_p), ("Wrot", ct.c_void_p),
                 ("drift", ct.c_void_p), ("hp_filter", ct.c_void_p), ("hp_len", ct.c_int32),
-                ("xc", ct.c_void_p), ("yc", ct.c_void_p)]
+                ("xc", ct.c_void_p), ("yc", ct.c_void_p), ("iCC", ct.c_void_p), ("iU", ct.c_void_p),
+                ("Ucc", ct.c_void_p)]
+
+
+class LoadInfo(ct.Structure):
+    _fields_ = [("icc_given", ct.c_int32), ("iu_given", ct.c_int32), ("ucc_given", ct.c_int32),
+                ("ctc_in_dir", ct.c_int32), ("ctc_max_abs_diff", ct.c_double), ("ctc_max_abs", ct.c_double)]
 
 
 class Spikes(ct.Structure):
@@ -133,6 +140,7 @@ def lib() -> ct.CDLL:
             "lss_assign_clusters": ([vp, i64, i32, vp, i32, vp, vp, vp, vp, f64, vp, vp], i32),
             "lss_load_cluster_centroids": ([ct.c_char_p, vp, vp, i32, ct.POINTER(i32)], i32),
             "lss_load_channel_map": ([ct.c_char_p, vp, i32, ct.POINTER(i32)], i32),
+            "lss_get_load_info": ([vp, ct.POINTER(LoadInfo)], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

@@ -74,10 +74,19 @@ struct Handle {
     DevBuf<double> wrot64;      // [C][C] Wrot in FP64 (per-batch drift: wmat = M @ Wrot)
     DevBuf<double> drift_ws;    // split-K partials and the FP64 product of lss_set_drift
     DevBuf<int> iCC, iU;
-    DevBuf<float2> spec, tw;
-    std::vector<float> hp_taps;
-    int64_t spec_W = -1;
-    int spec_lo = 0, spec_hi = 0;
+    DevBuf<float2> tw;
+    DevBuf<float> hp_dev;        // highpass taps (hp_filter) on the device
+    DevBuf<double2> tw64;        // exp(-2 pi i m / N) in FP64, for the spectra
+    // overlap-save spectra per window length: built on the device on first use, kept; entries a
+    // streaming batcher captured in its graphs (pins > 0) are never evicted
+    struct Spec {
+        float2* p = nullptr;
+        int lo = 0, hi = 0;
+        int pins = 0;
+        uint64_t used = 0;
+    };
+    std::map<int64_t, Spec> specs;
+    uint64_t spec_clock = 0;
     std::vector<float> h_xc, h_yc;
     int64_t last_W = 0;          // window length of the last pursuit (template refinement reads it)
 
@@ -117,6 +126,7 @@ struct Handle {
     float upd_ms = 0.f;
     int upd_launches = 0;
     float last_ms[8] = {};   // car, highpass, whiten, project, scores GEMM, pursuit, features, total
+    lss_load_info load{};    // which extraction tables were given, the ctc.npy comparison
     bool alive = false;
 
     ~Handle() {
@@ -130,6 +140,8 @@ struct Handle {
         for (void* q : {(void*)hs_counters, (void*)hs_col, (void*)hs_tmp, (void*)hs_amp, (void*)hs_tF,
                         (void*)hs_xs, (void*)hs_ys})
             if (q) (void)hipHostFree(q);
+        for (auto& kv : specs)
+            if (kv.second.p) (void)hipFree(kv.second.p);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -146,6 +158,7 @@ lss_status check_params(const lss_params& p) {
     if (p.max_passes < 1 || p.max_passes > 256) return fail(LSS_ERR_ARG, "max_passes must be in [1, 256]");
     if (p.max_spikes < 1) return fail(LSS_ERR_ARG, "max_spikes must be >= 1");
     if (p.precision < LSS_PREC_FP32 || p.precision > LSS_PREC_FP16X3) return fail(LSS_ERR_ARG, "bad precision");
+    if (p.dup_bins < 0) return fail(LSS_ERR_ARG, "dup_bins must be >= 0");
     return LSS_OK;
 }
 
@@ -167,60 +180,40 @@ void nearest_channels(const std::vector<float>& xc, const std::vector<float>& yc
     }
 }
 
-// Spectrum of the overlap-save kernel for window length W (fft_highpass pad/crop rule,
-// kilosort/preprocessing.py:138-154, then fftshift): g[j] = f[(W/2 + j) mod W].
-lss_status prepare_spectrum(Handle& h, int64_t W) {
-    if (h.spec_W == W) return LSS_OK;
-    const int64_t ft = (int64_t)h.hp_taps.size();
-    auto f_at = [&](int64_t m) -> double {   // padded/cropped filter value at index m in [0, W)
-        if (ft < W) {
-            int64_t pad = (W - ft) / 2;
-            int64_t q = m - pad;
-            return (q >= 0 && q < ft) ? (double)h.hp_taps[q] : 0.0;
-        } else if (ft > W) {
-            int64_t crop = (ft - W) / 2;
-            return (double)h.hp_taps[crop + m];
-        }
-        return (double)h.hp_taps[m];
-    };
-    const int Llo = (int)std::min<int64_t>(lss::kHpHalf, W / 2);
-    const int Lhi = (int)std::min<int64_t>(lss::kHpHalf, W - 1 - W / 2);
-    const int N = lss::kFftN;
-    std::vector<double> hr(N, 0.0);
-    for (int j = -Llo; j <= Lhi; ++j) {
-        int64_t m = ((W / 2 + j) % W + W) % W;
-        // circular convolution kernel h[p] = g[-p]
-        hr[((-j) % N + N) % N] += f_at(m);
-    }
-    // forward DFT of hr (iterative radix-2, FP64), scaled by 1/N
-    std::vector<double> re(hr), im(N, 0.0);
-    for (int i = 1, j = 0; i < N; ++i) {
-        int bit = N >> 1;
-        for (; j & bit; bit >>= 1) j ^= bit;
-        j ^= bit;
-        if (i < j) { std::swap(re[i], re[j]); std::swap(im[i], im[j]); }
-    }
-    const double two_pi = 6.283185307179586476925286766559;
-    for (int len = 2; len <= N; len <<= 1) {
-        for (int i = 0; i < N; i += len) {
-            for (int k = 0; k < len / 2; ++k) {
-                const double ang = -two_pi * k / len;
-                const double wr = std::cos(ang), wi = std::sin(ang);
-                const int a = i + k, b = i + k + len / 2;
-                const double xr = re[b] * wr - im[b] * wi, xi = re[b] * wi + im[b] * wr;
-                re[b] = re[a] - xr; im[b] = im[a] - xi;
-                re[a] += xr; im[a] += xi;
+// Spectrum of the overlap-save kernel for window length W (the fft_highpass pad/crop rule,
+// kilosort/preprocessing.py:138-154, then fftshift), from the handle's cache or built on the
+// device in FP64 on the handle's stream (ordered before any highpass that reads it).  pin = a
+// streaming batcher captures the buffer in its graphs: the entry then stays until unpinned.
+constexpr size_t kSpecCache = 16;
+
+lss_status get_spectrum(Handle& h, int64_t W, bool pin, const Handle::Spec** out) {
+    auto it = h.specs.find(W);
+    if (it == h.specs.end()) {
+        if (h.specs.size() >= kSpecCache) {
+            auto victim = h.specs.end();
+            for (auto j = h.specs.begin(); j != h.specs.end(); ++j)
+                if (j->second.pins == 0 && (victim == h.specs.end() || j->second.used < victim->second.used))
+                    victim = j;
+            if (victim != h.specs.end()) {
+                HIP_TRY(hipStreamSynchronize(h.stream));   // no queued highpass still reads it
+                HIP_TRY(hipFree(victim->second.p));
+                h.specs.erase(victim);
             }
         }
+        Handle::Spec sp;
+        sp.lo = (int)std::min<int64_t>(lss::kHpHalf, W / 2);
+        sp.hi = (int)std::min<int64_t>(lss::kHpHalf, W - 1 - W / 2);
+        HIP_TRY(hipMalloc((void**)&sp.p, sizeof(float2) * lss::kFftN));
+        hipError_t e = lss::launch_hp_spectrum(h.hp_dev.p, (int)h.hp_dev.n, W, sp.lo, sp.hi, h.tw64.p, sp.p, h.stream);
+        if (e != hipSuccess) {
+            (void)hipFree(sp.p);
+            return fail(LSS_ERR_HIP, std::string("launch_hp_spectrum: ") + hipGetErrorString(e));
+        }
+        it = h.specs.emplace(W, sp).first;
     }
-    std::vector<float2> spec(N);
-    for (int n = 0; n < N; ++n) spec[n] = make_float2((float)(re[n] / N), (float)(im[n] / N));
-    // the previous window's highpass may still read the old spectrum on the handle's stream
-    HIP_TRY(hipMemcpyAsync(h.spec.p, spec.data(), sizeof(float2) * N, hipMemcpyHostToDevice, h.stream));
-    HIP_TRY(hipStreamSynchronize(h.stream));
-    h.spec_W = W;
-    h.spec_lo = Llo;
-    h.spec_hi = Lhi;
+    it->second.used = ++h.spec_clock;
+    if (pin) ++it->second.pins;
+    *out = &it->second;
     return LSS_OK;
 }
 
@@ -269,8 +262,9 @@ lss_status create_impl(const lss_params* p, const lss_artifacts* a, Handle** out
     HIP_TRY(h->yc.alloc(C));
     HIP_TRY(h->iCC.alloc((size_t)nC * C));
     HIP_TRY(h->iU.alloc(T));
-    HIP_TRY(h->spec.alloc(lss::kFftN));
     HIP_TRY(h->tw.alloc(lss::kFftN));
+    HIP_TRY(h->tw64.alloc(lss::kFftN));
+    HIP_TRY(h->hp_dev.alloc(a->hp_len));
     HIP_TRY(hipMemset(h->U.p, 0, sizeof(float) * T * h->ldq));
     HIP_TRY(hipMemcpy2D(h->U.p, sizeof(float) * h->ldq, a->U, sizeof(float) * K * C, sizeof(float) * K * C, T,
                         hipMemcpyHostToDevice));
@@ -279,7 +273,7 @@ lss_status create_impl(const lss_params* p, const lss_artifacts* a, Handle** out
     HIP_TRY(hipMemcpy(h->yc.p, a->yc, sizeof(float) * C, hipMemcpyHostToDevice));
     h->h_xc.assign(a->xc, a->xc + C);
     h->h_yc.assign(a->yc, a->yc + C);
-    h->hp_taps.assign(a->hp_filter, a->hp_filter + a->hp_len);
+    HIP_TRY(hipMemcpy(h->hp_dev.p, a->hp_filter, sizeof(float) * a->hp_len, hipMemcpyHostToDevice));
 
     // whitening operator (M @ Wrot) in FP64 (kilosort/io.py:787)
     {
@@ -315,14 +309,47 @@ lss_status create_impl(const lss_params* p, const lss_artifacts* a, Handle** out
             tw[m] = make_float2((float)std::cos(ang), (float)std::sin(ang));
         }
         HIP_TRY(hipMemcpy(h->tw.p, tw.data(), sizeof(float2) * lss::kFftN, hipMemcpyHostToDevice));
+        std::vector<double2> tw64(lss::kFftN);
+        for (int m = 0; m < lss::kFftN; ++m) {
+            double ang = -two_pi * m / lss::kFftN;
+            tw64[m] = make_double2(std::cos(ang), std::sin(ang));
+        }
+        HIP_TRY(hipMemcpy(h->tw64.p, tw64.data(), sizeof(double2) * lss::kFftN, hipMemcpyHostToDevice));
     }
-    // iCC / iU / Ucc (prepare_extract, template_matching.py:20-26)
+    // iCC / iU / Ucc (prepare_extract, template_matching.py:20-26): the caller's tables where given
+    // (the exporter's files, OnlineSpikesV2.cu:561-563), else built from the sites and U
     {
         std::vector<int> iCC;
-        nearest_channels(h->h_xc, h->h_yc, nC, iCC);
+        if (a->iCC) {
+            iCC.assign(a->iCC, a->iCC + (size_t)nC * C);
+            for (int v : iCC)
+                if (v < 0 || v >= C) return fail(LSS_ERR_ARG, "iCC entry outside [0, n_chan)");
+        } else {
+            nearest_channels(h->h_xc, h->h_yc, nC, iCC);
+        }
         HIP_TRY(hipMemcpy(h->iCC.p, iCC.data(), sizeof(int) * nC * C, hipMemcpyHostToDevice));
         if (lss_status st = refresh_extract(*h)) return st;
         HIP_TRY(hipStreamSynchronize(h->stream));
+        if (a->iU) {
+            for (int t = 0; t < T; ++t)
+                if (a->iU[t] < 0 || a->iU[t] >= C) return fail(LSS_ERR_ARG, "iU entry outside [0, n_chan)");
+            HIP_TRY(hipMemcpy(h->iU.p, a->iU, sizeof(int) * T, hipMemcpyHostToDevice));
+        }
+        if (a->Ucc) {
+            HIP_TRY(hipMemcpy(h->Ucc.p, a->Ucc, sizeof(float) * nC * T * K, hipMemcpyHostToDevice));
+        } else if (a->iU) {
+            // Ucc[j][t][k] = U[t][k][iCC[j][iU[t]]] for the given iU
+            std::vector<float> ucc((size_t)nC * T * K);
+            for (int j = 0; j < nC; ++j)
+                for (int t = 0; t < T; ++t)
+                    for (int k = 0; k < K; ++k)
+                        ucc[((size_t)j * T + t) * K + k] =
+                            a->U[((size_t)t * K + k) * C + iCC[(size_t)j * C + a->iU[t]]];
+            HIP_TRY(hipMemcpy(h->Ucc.p, ucc.data(), sizeof(float) * ucc.size(), hipMemcpyHostToDevice));
+        }
+        h->load.icc_given = a->iCC != nullptr;
+        h->load.iu_given = a->iU != nullptr;
+        h->load.ucc_given = a->Ucc != nullptr;
     }
     HIP_TRY(h->tables_ws.alloc(lss::tables_workspace_doubles(T, K, C, nt)));
     HIP_TRY(lss::launch_tables(h->U.p, h->ldq, h->wpca.p, T, K, C, nt, h->nm.p, h->waves.p, h->ldC, h->ctc.p, h->ldT,
@@ -382,8 +409,9 @@ lss_status run_car(Handle& h, const int16_t* d_win, int64_t W) {
 }
 
 lss_status run_highpass(Handle& h, const float* in, float* out, int64_t W) {
-    if (lss_status st = prepare_spectrum(h, W)) return st;
-    HIP_TRY(lss::launch_highpass(in, h.ldw, out, h.ldw, h.C, W, h.spec_lo, h.spec_hi, h.spec.p, h.tw.p, h.stream));
+    const Handle::Spec* sp = nullptr;
+    if (lss_status st = get_spectrum(h, W, false, &sp)) return st;
+    HIP_TRY(lss::launch_highpass(in, h.ldw, out, h.ldw, h.C, W, sp->lo, sp->hi, sp->p, h.tw.p, h.stream));
     return LSS_OK;
 }
 
@@ -490,10 +518,9 @@ lss_status run_features(Handle& h, const float* X, int64_t W) {
 }
 
 // Device part of one window up to the scores (no events, no host synchronisation: capturable).
-lss_status enqueue_pre(Handle& h, const int16_t* d_win, int64_t W) {
+lss_status enqueue_pre(Handle& h, const Handle::Spec& sp, const int16_t* d_win, int64_t W) {
     if (lss_status st = run_car(h, d_win, W)) return st;
-    HIP_TRY(lss::launch_highpass(h.xcw.p, h.ldw, h.xhp.p, h.ldw, h.C, W, h.spec_lo, h.spec_hi, h.spec.p, h.tw.p,
-                                 h.stream));
+    HIP_TRY(lss::launch_highpass(h.xcw.p, h.ldw, h.xhp.p, h.ldw, h.C, W, sp.lo, sp.hi, sp.p, h.tw.p, h.stream));
     if (lss_status st = run_whiten(h, h.xhp.p, h.xw.p, W)) return st;
     return run_scores(h, h.xw.p, h.B.p, W);
 }
@@ -634,6 +661,7 @@ struct Probe {
     int64_t W = 0;
     int P1 = 0;
     int max_grid = 0;             // persistent-pursuit workgroups, so that all probes' fit at once
+    const Handle::Spec* spec = nullptr;   // the window's highpass spectrum (pinned in the handle)
     hipEvent_t done = nullptr;
 };
 
@@ -649,7 +677,8 @@ struct Batcher {
     int64_t batches = 0;
     float last_ms = 0.f;
     bool single = false;     // every probe runs all its passes in segment A: one graph per step
-    int refine_every = 0;    // template refinement step every this many batches (0 = never)
+    int refine_every = 0;    // template refinement round every this many batches (0 = never)
+    int refine_next = -1;    // probe the current refinement round refines next (-1: no round)
     std::vector<std::array<double, 4>> refine_info;   // per probe, last refinement (lss_refine_templates)
     float part_ms[6] = {};   // assemble, segment A, extra passes, segment C, emit, total
     ~Batcher() {
@@ -660,6 +689,7 @@ struct Batcher {
         for (auto& p : pr) {
             if (p.pin) (void)hipHostFree(p.pin);
             if (p.done) (void)hipEventDestroy(p.done);
+            if (p.spec) --const_cast<Handle::Spec*>(p.spec)->pins;
         }
         if (fork) (void)hipEventDestroy(fork);
         if (main) (void)hipStreamDestroy(main);
@@ -672,7 +702,7 @@ lss_status probe_A(Probe& p) {
     Handle& h = *p.h;
     if (p.mode == 1)
         HIP_TRY(hipMemcpyAsync(h.win.p, p.pin, sizeof(int16_t) * p.W * h.C, hipMemcpyHostToDevice, h.stream));
-    if (lss_status st = enqueue_pre(h, h.win.p, p.W)) return st;
+    if (lss_status st = enqueue_pre(h, *p.spec, h.win.p, p.W)) return st;
     const lss::PursuitArgs a = pursuit_args(h, h.B.p, h.xw.p, p.W);
     if (lss_status st = enqueue_pursuit(h, a, p.P1, p.max_grid)) return st;
     if (p.P1 < h.prm.max_passes)
@@ -805,6 +835,7 @@ lss_status lss_default_params(lss_params* p) {
     p->drop_negative = 1;
     p->th_learned = 8.0;
     p->fs = 30000.f;
+    p->dup_bins = 7;
     return LSS_OK;
 }
 
@@ -836,6 +867,7 @@ lss_status lss_create_from_dir(const char* dir, const lss_params* p0, void** han
             if (kv.count("nt0min")) p.nt0min = std::stoi(kv["nt0min"]);
             if (kv.count("numNearestChans")) p.n_nearest = std::stoi(kv["numNearestChans"]);
             if (kv.count("Th_learned")) p.th_learned = std::stod(kv["Th_learned"]);
+            if (kv.count("duplicate_spike_bins")) p.dup_bins = std::stoi(kv["duplicate_spike_bins"]);
         } catch (...) {
             return fail(LSS_ERR_IO, "malformed misc.txt");
         }
@@ -852,6 +884,8 @@ lss_status lss_create_from_dir(const char* dir, const lss_params* p0, void** han
     if (!lss::npy_load_f32(d + "xc.npy", xc, err)) return fail(LSS_ERR_IO, err);
     if (!lss::npy_load_f32(d + "yc.npy", yc, err)) return fail(LSS_ERR_IO, err);
     if (U.shape.size() != 3 || wpca.shape.size() != 2) return fail(LSS_ERR_IO, "Wall3/wPCA rank");
+    if (U.shape[0] > INT32_MAX || U.shape[1] > 64 || U.shape[2] > INT32_MAX || wpca.shape[1] > 1024)
+        return fail(LSS_ERR_IO, "Wall3/wPCA shape out of range");
     p.n_templates = (int)U.shape[0];
     p.n_pcs = (int)U.shape[1];
     p.n_chan = (int)U.shape[2];
@@ -868,6 +902,34 @@ lss_status lss_create_from_dir(const char* dir, const lss_params* p0, void** han
         for (int i = 0; i < p.n_chan; ++i)
             for (int j = 0; j < p.n_chan; ++j) dm[(size_t)i * p.n_chan + j] = drift.data[(size_t)j * p.n_chan + i];
     }
+    const int64_t T = p.n_templates, K = p.n_pcs, C = p.n_chan, nC = p.n_nearest;
+    // prepare_extract's tables as exported (run_online_spikes.py:533, 572-574), when present
+    std::vector<int32_t> icc, iu;
+    lss::NpyArray ucc;
+    {
+        lss::NpyArrayI64 a64;
+        if (lss::npy_load_i64(d + "iCC.npy", a64, err)) {
+            if (a64.shape.size() != 2 || a64.shape[0] != nC || a64.shape[1] != C)
+                return fail(LSS_ERR_IO, "iCC.npy must be [numNearestChans][C]");
+            icc.resize(a64.data.size());
+            for (size_t i = 0; i < icc.size(); ++i) {
+                if (a64.data[i] < 0 || a64.data[i] >= C) return fail(LSS_ERR_IO, "iCC.npy: channel out of range");
+                icc[i] = (int32_t)a64.data[i];
+            }
+        }
+        if (lss::npy_load_i64(d + "iU.npy", a64, err)) {
+            if (a64.shape.size() != 1 || a64.shape[0] != T) return fail(LSS_ERR_IO, "iU.npy must be [T]");
+            iu.resize(a64.data.size());
+            for (size_t i = 0; i < iu.size(); ++i) {
+                if (a64.data[i] < 0 || a64.data[i] >= C) return fail(LSS_ERR_IO, "iU.npy: channel out of range");
+                iu[i] = (int32_t)a64.data[i];
+            }
+        }
+        if (lss::npy_load_f32(d + "Ucc.npy", ucc, err)) {
+            if (ucc.shape.size() != 3 || ucc.shape[0] != nC || ucc.shape[1] != T || ucc.shape[2] != K)
+                return fail(LSS_ERR_IO, "Ucc.npy must be [numNearestChans][T][K]");
+        }
+    }
     lss_artifacts a{};
     a.U = U.data.data();
     a.wPCA = wpca.data.data();
@@ -877,7 +939,47 @@ lss_status lss_create_from_dir(const char* dir, const lss_params* p0, void** han
     a.hp_len = (int32_t)hp.data.size();
     a.xc = xc.data.data();
     a.yc = yc.data.data();
-    return lss_create(&p, &a, handle);
+    a.iCC = icc.empty() ? nullptr : icc.data();
+    a.iU = iu.empty() ? nullptr : iu.data();
+    a.Ucc = ucc.data.empty() ? nullptr : ucc.data.data();
+    void* hv = nullptr;
+    if (lss_status st = lss_create(&p, &a, &hv)) return st;
+    // ctc.npy (the reference loads it, OnlineSpikesV2.cu:558): compared with the FP64 rebuild
+    lss::NpyArray ctc;
+    if (lss::npy_load_f32(d + "ctc.npy", ctc, err)) {
+        Handle& h = *as_handle(hv);
+        const int64_t L = 2 * (int64_t)p.nt + 1;
+        if (ctc.shape.size() != 3 || ctc.shape[0] != T || ctc.shape[1] != T || ctc.shape[2] != L) {
+            lss_destroy(hv);
+            return fail(LSS_ERR_IO, "ctc.npy must be [T][T][2nt+1]");
+        }
+        std::vector<float> dev((size_t)T * L * T);   // device layout ctc[j][l][i]
+        lss_status st = lss_get_tables(hv, nullptr, nullptr, dev.data(), nullptr);
+        if (st != LSS_OK) {
+            lss_destroy(hv);
+            return st;
+        }
+        double md = 0.0, mx = 0.0;
+        for (int64_t i = 0; i < T; ++i)
+            for (int64_t j = 0; j < T; ++j)
+                for (int64_t l = 0; l < L; ++l) {
+                    const double r = dev[((size_t)j * L + l) * T + i];
+                    const double f = ctc.data[((size_t)i * T + j) * L + l];
+                    md = std::max(md, std::fabs(f - r));
+                    mx = std::max(mx, std::fabs(r));
+                }
+        h.load.ctc_in_dir = 1;
+        h.load.ctc_max_abs_diff = md;
+        h.load.ctc_max_abs = mx;
+    }
+    *handle = hv;
+    return LSS_OK;
+}
+
+lss_status lss_get_load_info(void* handle, lss_load_info* info) {
+    if (!handle || !info) return fail(LSS_ERR_ARG, "null argument");
+    *info = as_handle(handle)->load;
+    return LSS_OK;
 }
 
 lss_status lss_destroy(void* handle) {
@@ -1454,7 +1556,8 @@ lss_status lss_stream_create(void* const* handles, int32_t n_probes, int32_t bat
         if (p.W > h.Wmax) return fail(LSS_ERR_ARG, "batch + 2*nt exceeds a handle's max_window");
         if (h.prof_update) return fail(LSS_ERR_STATE, "turn update profiling off before streaming");
         if (lss_status st = ensure_stage(h)) return st;
-        if (lss_status st = prepare_spectrum(h, p.W)) return st;
+        if (lss_status st = get_spectrum(h, p.W, true, &p.spec)) return st;
+        HIP_TRY(hipStreamSynchronize(h.stream));   // in place before any replay on other streams
         HIP_TRY(hipHostMalloc((void**)&p.pin, sizeof(int16_t) * p.W * h.C, hipHostMallocDefault));
         HIP_TRY(hipEventCreateWithFlags(&p.done, hipEventDisableTiming));
         p.P1 = (h.tuned_passes > 0) ? std::min(h.tuned_passes, h.prm.max_passes) : h.prm.max_passes;
@@ -1586,12 +1689,14 @@ lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready) {
     b.part_ms[4] = ms_since(t1);
     ++b.batches;
     if (b.refine_every > 0) {
-        // one probe per step, round-robin, starting every refine_every steps: the step that refines
-        // pays for one probe's refinement only
-        const int64_t k = b.batches % b.refine_every;
-        if (b.batches >= b.refine_every && k < (int64_t)b.pr.size()) {
+        // a round starts every refine_every steps (one still under way carries on); in a round,
+        // step i refines probe i, so the step that refines pays for one probe's refinement only
+        if (b.batches % b.refine_every == 0 && b.refine_next < 0) b.refine_next = 0;
+        if (b.refine_next >= 0) {
+            const int k = b.refine_next;
             b.refine_info.resize(b.pr.size());
             if (lss_status st = lss_refine_templates(b.pr[k].h, b.refine_info[k].data())) return st;
+            b.refine_next = (k + 1 < (int)b.pr.size()) ? k + 1 : -1;
         }
     }
     b.part_ms[5] = ms_since(t_start);

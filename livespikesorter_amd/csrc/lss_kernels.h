@@ -26,6 +26,10 @@ constexpr int kHpHalf = 800;         // taps kept on each side of the filter cen
 // tw: [kFftN] float2 twiddles exp(-2 pi i m / N).
 hipError_t launch_highpass(const float* x, int64_t ldx, float* y, int64_t ldy, int C, int64_t W,
                            int Llo, int Lhi, const float2* spec, const float2* tw, hipStream_t s);
+// spec for window length W from the device taps [ft] (FP64 sums, FP32 result); tw64: [kFftN]
+// exp(-2 pi i m / N) in FP64.  Llo, Lhi = min(kHpHalf, W/2), min(kHpHalf, W-1-W/2).
+hipError_t launch_hp_spectrum(const float* taps, int ft, int64_t W, int Llo, int Lhi, const double2* tw64,
+                              float2* spec, hipStream_t s);
 // Plain batched forward FFT of [nb][kFftN] complex rows (test hook of the LDS FFT).
 hipError_t launch_fft_rows(float2* data, int nb, const float2* tw, hipStream_t s);
 

@@ -421,4 +421,44 @@ hipError_t launch_highpass(const float* x, int64_t ldx, float* y, int64_t ldy, i
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------------------------------------
+// Spectrum of the overlap-save kernel for window length W, on the device in FP64.  The circular
+// correlation kernel g[j] = f[(W/2 + j) mod W] (f = the taps padded / cropped to W as
+// fft_highpass does, kilosort/preprocessing.py:138-154) enters the convolution as h[(-j) mod N],
+// so spec[n] = (1/N) sum_{j=-Llo..Lhi} g[j] exp(+2 pi i n j / N), rounded to FP32 once.
+// One thread per frequency; tw64[m] = exp(-2 pi i m / N) in FP64 (the exponent's index reduced
+// mod N in integers, so every term uses an exactly rounded table value).
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void hp_spectrum_kernel(const float* __restrict__ taps, int ft, int64_t W,
+                                                          int Llo, int Lhi, const double2* __restrict__ tw64,
+                                                          float2* __restrict__ spec) {
+    const int n = blockIdx.x * 256 + threadIdx.x;
+    if (n >= kFftN) return;
+    double re = 0.0, im = 0.0;
+    for (int j = -Llo; j <= Lhi; ++j) {
+        const int64_t m = ((W / 2 + j) % W + W) % W;
+        double g;
+        if (ft < W) {
+            const int64_t q = m - (W - ft) / 2;
+            g = (q >= 0 && q < ft) ? (double)taps[q] : 0.0;
+        } else if (ft > W) {
+            g = (double)taps[(ft - W) / 2 + m];
+        } else {
+            g = (double)taps[m];
+        }
+        const int e = (int)(((int64_t)n * j % kFftN + kFftN) % kFftN);
+        const double2 w = tw64[e];   // exp(-2 pi i e / N); the term needs its conjugate
+        re += g * w.x;
+        im -= g * w.y;
+    }
+    spec[n] = make_float2((float)(re / kFftN), (float)(im / kFftN));
+}
+
+hipError_t launch_hp_spectrum(const float* taps, int ft, int64_t W, int Llo, int Lhi, const double2* tw64,
+                              float2* spec, hipStream_t s) {
+    if (ft < 1 || W < 1 || Llo < 0 || Lhi < 0 || Llo + Lhi + 1 > kFftN) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(hp_spectrum_kernel, dim3(kFftN / 256), dim3(256), 0, s, taps, ft, W, Llo, Lhi, tw64, spec);
+    return hipGetLastError();
+}
+
 }  // namespace lss

@@ -105,16 +105,22 @@ class Sorter:
         U = _f32(art["U"])
         T, K, C = U.shape
         wpca = _f32(art["wPCA"])
+        opt = lambda key, dt: None if art.get(key) is None else np.ascontiguousarray(np.asarray(art[key]), dtype=dt)
         self._keep = dict(U=U, wPCA=wpca, Wrot=_f32(art["Wrot"]),
                           drift=None if art.get("drift") is None else _f32(art["drift"]),
-                          hp=_f32(art["hp_filter"]), xc=_f32(art["xc"]), yc=_f32(art["yc"]))
+                          hp=_f32(art["hp_filter"]), xc=_f32(art["xc"]), yc=_f32(art["yc"]),
+                          iCC=opt("iCC", np.int32), iU=opt("iU", np.int32), Ucc=opt("Ucc", np.float32))
         k = self._keep
         self.params = default_params(
             n_chan=C, n_templates=T, n_pcs=K, nt=wpca.shape[1], nt0min=int(art.get("nt0min", 20)),
             n_nearest=int(art.get("nC", 10)), max_window=int(max_window), max_passes=int(max_passes),
             max_spikes=int(max_spikes), do_car=int(do_car), precision=precision,
             drop_negative=int(drop_negative), th_learned=float(art.get("Th", 8.0)),
-            fs=float(art.get("fs", 30000.0)))
+            fs=float(art.get("fs", 30000.0)), dup_bins=int(art.get("dup_bins", 7)))
+        nC = self.params.n_nearest
+        for key, shape in (("iCC", (nC, C)), ("iU", (T,)), ("Ucc", (nC, T, K))):
+            if k[key] is not None and k[key].shape != shape:
+                raise ValueError(f"{key} must have shape {shape}, not {k[key].shape}")
         a = Artifacts()
         a.U = k["U"].ctypes.data
         a.wPCA = k["wPCA"].ctypes.data
@@ -124,6 +130,8 @@ class Sorter:
         a.hp_len = len(k["hp"])
         a.xc = k["xc"].ctypes.data
         a.yc = k["yc"].ctypes.data
+        for key in ("iCC", "iU", "Ucc"):
+            setattr(a, key, None if k[key] is None else k[key].ctypes.data)
         h = ct.c_void_p()
         check(lib().lss_create(ct.byref(self.params), ct.byref(a), ct.byref(h)), "lss_create")
         self._h = h
@@ -208,6 +216,13 @@ class Sorter:
                                       ct.byref(ws)), "lss_pad_batch")
             parts.append(self.sort_window(win, ws.value))
         return SpikeList.concat(parts).sorted()
+
+    def load_info(self) -> dict:
+        """Which extraction tables (iCC, iU, Ucc) were given rather than built, and the comparison of
+        an oss_input directory's ctc.npy with the FP64 rebuild (max |diff|, max |ctc|)."""
+        info = _lib.LoadInfo()
+        check(lib().lss_get_load_info(self._h, ct.byref(info)), "lss_get_load_info")
+        return {f: getattr(info, f) for f, _ in info._fields_}
 
     def timings(self):
         ms = np.zeros(8, np.float32)
@@ -397,6 +412,7 @@ def sort_windows_native(sorters, windows, starts, capacity: int = 1 << 15, chunk
     for g0 in range(0, len(windows), group):
         ws = windows[g0:g0 + group]
         if on_device:
+            _ready()
             ptrs = [_ptr(w) for w in ws]
         else:
             ws = [np.ascontiguousarray(w, dtype=np.int16) for w in ws]

@@ -23,7 +23,11 @@ class StreamingSorter:
               "lss_stream_create")
         self._h = h
         self.batch = int(batch)
+        # device tensors referenced by the batcher, per probe: (tensor, first sample, end sample);
+        # released once the batcher has trimmed them (end <= t0 - nt after a step)
         self._held = {}
+        self._recv = [0] * len(self.sorters)
+        self._t0 = 0
         if refine_every:
             check(lib().lss_stream_set_refine(h, int(refine_every)), "lss_stream_set_refine")
         arr_out = (Spikes * len(self.sorters))()
@@ -41,14 +45,18 @@ class StreamingSorter:
         if hasattr(samples, "data_ptr"):
             if not samples.is_contiguous():
                 raise ValueError("device samples must be contiguous")
-            self._held.setdefault(probe, []).append(samples)
             import torch
             torch.cuda.current_stream().synchronize()   # the samples are complete before a step copies them
-            check(lib().lss_stream_push_device(self._h, int(probe), samples.data_ptr(), samples.shape[0]),
+            n = int(samples.shape[0])
+            check(lib().lss_stream_push_device(self._h, int(probe), samples.data_ptr(), n),
                   "lss_stream_push_device")
+            if n:
+                self._held.setdefault(probe, []).append((samples, self._recv[probe], self._recv[probe] + n))
+            self._recv[probe] += n
             return
         x = np.ascontiguousarray(samples, np.int16)
         check(lib().lss_stream_push(self._h, int(probe), x.ctypes.data, x.shape[0]), "lss_stream_push")
+        self._recv[probe] += int(x.shape[0])
 
     def step(self):
         """Spike lists of the next batch for every probe, or None if samples are missing."""
@@ -56,7 +64,18 @@ class StreamingSorter:
         check(lib().lss_stream_step(self._h, self._arr_out, ct.byref(ready)), "lss_stream_step")
         if not ready.value:
             return None
-        return [Sorter._list(self._arr_out[i], self._bufs[i]) for i in range(len(self.sorters))]
+        out = [Sorter._list(self._arr_out[i], self._bufs[i]) for i in range(len(self.sorters))]
+        # the batcher keeps samples from t0 - nt on (lss_stream_step): drop the tensors before that
+        self._t0 += self.batch
+        for probe, held in self._held.items():
+            keep_from = max(0, self._t0 - self.sorters[probe].nt)
+            while held and held[0][2] <= keep_from:
+                held.pop(0)
+        return out
+
+    def held(self, probe: int) -> int:
+        """Device tensors of a probe still referenced for the batcher."""
+        return len(self._held.get(probe, []))
 
     def last_ms(self) -> float:
         ms, n = ct.c_float(), ct.c_int64()

@@ -121,7 +121,7 @@ def test_facade_matches_python_path(facade_inputs, mn, mx, tb, first, lag, small
 def test_facade_labels_from_directory_centroids(synth1, tmp_path_factory):
     """With cluster_centroids.npy in the artifact directory the facade labels spikes by the nearest
     centroid (OnlineSpikesV2::loadKilosortClusteringData / closestCluster) and deduplicates per
-    label; the socket is handed channelMap.npy's channels, and a channel map of the wrong length
+    label with misc.txt's duplicate_spike_bins (OnlineSpikesV2.cu:456); the socket is handed channelMap.npy's channels, and a channel map of the wrong length
     is refused."""
     from livespikesorter_amd.export import write_oss_input
     from livespikesorter_amd.output import load_cluster_centroids
@@ -135,7 +135,8 @@ def test_facade_labels_from_directory_centroids(synth1, tmp_path_factory):
     xy = np.stack([rng.uniform(xc.min(), xc.max(), 12), rng.uniform(yc.min(), yc.max(), 12)], 1)
     C = np.asarray(synth1.art["U"]).shape[2]
     cmap = np.arange(C)[::-1] + 100      # the channel list handed to the socket
-    art_dir = write_oss_input(synth1.art, d / "oss_input", cluster_xy=xy, channel_map=cmap)
+    # a duplicate window other than the default: the facade takes it from misc.txt
+    art_dir = write_oss_input(dict(synth1.art, dup_bins=40), d / "oss_input", cluster_xy=xy, channel_map=cmap)
     rec = np.ascontiguousarray(synth1.rec[:30000], np.int16)
     rec.tofile(d / "rec.i16")
     out = d / "out"
@@ -148,7 +149,8 @@ def test_facade_labels_from_directory_centroids(synth1, tmp_path_factory):
     sched = _schedule(rec.shape[0], 0, 3000, 3000, 61, 100000, 0, 0)
     cents = load_cluster_centroids(art_dir)
     with Sorter.from_dir(art_dir, max_window=3000 + 2 * 127, max_passes=100, max_spikes=1 << 16) as s:
-        exp = _expected(s, rec, sched, 61, len(xy), centroids=cents)
+        assert s.params.dup_bins == 40
+        exp = _expected(s, rec, sched, 61, len(xy), dt=40, centroids=cents)
     got = _read(out / "spikeOutput.txt")
     assert len(got) == len(exp) > 0
     np.testing.assert_array_equal(got, exp)

@@ -179,3 +179,76 @@ def test_stream_device_samples_match_host_samples(probes):
             np.testing.assert_array_equal(a.features, b.features)
     for s in sorters:
         s.close()
+
+
+def test_stream_releases_device_samples(probes):
+    """Device sample buffers are referenced only while the batcher still reads them: over many
+    steps the number held per probe stays bounded (the batcher keeps samples from t0 - nt on)."""
+    print("[lss] test_stream_releases_device_samples", flush=True)
+    from livespikesorter_amd.streaming import StreamingSorter
+    sorters = _make_sorters(probes)
+    drec = [torch.from_numpy(p.rec).to("cuda:0") for p in probes]
+    most = 0
+    steps = 0
+    with StreamingSorter(sorters, batch=BATCH, use_graph=True) as st:
+        n = probes[0].rec.shape[0]
+        for pos in range(0, n, 500):                     # 6 pushes per batch
+            for i in range(len(probes)):
+                st.push(i, drec[i][pos:pos + 500])
+            while st.step() is not None:
+                steps += 1
+                most = max(most, max(st.held(i) for i in range(len(probes))))
+    print(f"[lss]   {steps} steps, at most {most} buffers held", flush=True)
+    assert steps >= 4 and most <= BATCH // 500 + 2
+    for s in sorters:
+        s.close()
+
+
+def test_stream_unaffected_by_windows_of_other_lengths(probes):
+    """A handle that sorts windows of other lengths between stream steps (each length has its
+    own highpass spectrum) leaves the stream's results unchanged."""
+    print("[lss] test_stream_unaffected_by_windows_of_other_lengths", flush=True)
+    from livespikesorter_amd.streaming import StreamingSorter
+    plain = _stream(probes, _make_sorters(probes), use_graph=True, chunk_seed=7)
+    sorters = _make_sorters(probes)
+    out = [[] for _ in probes]
+    with StreamingSorter(sorters, batch=BATCH, use_graph=True) as st:
+        for i, p in enumerate(probes):
+            st.push(i, p.rec)
+        k = 0
+        while True:
+            r = st.step()
+            if r is None:
+                break
+            for i, p in enumerate(probes):
+                out[i].append(r[i])
+                W = 1000 + (700 * k + 131 * i) % 2000            # a different length every time
+                sorters[i].sort_window(np.ascontiguousarray(p.rec[:W]), 0)
+            k += 1
+    for i in range(len(probes)):
+        assert len(out[i]) == len(plain[i])
+        for a, b in zip(out[i], plain[i]):
+            np.testing.assert_array_equal(a.sample, b.sample)
+            np.testing.assert_array_equal(a.amplitude, b.amplitude)
+            np.testing.assert_array_equal(a.features, b.features)
+    for s in sorters:
+        s.close()
+
+
+def test_refinement_rounds_shorter_than_probe_count(probes):
+    """refine_every = 1 with two probes: a round still under way carries on, so every probe is
+    refined (not only probe 0)."""
+    print("[lss] test_refinement_rounds_shorter_than_probe_count", flush=True)
+    from livespikesorter_amd.streaming import StreamingSorter
+    sorters = _make_sorters(probes)
+    with StreamingSorter(sorters, batch=BATCH, use_graph=True, refine_every=1) as st:
+        for i, p in enumerate(probes):
+            st.push(i, p.rec[:4 * BATCH])
+        while st.step() is not None:
+            pass
+        infos = [st.last_refine(i) for i in range(len(probes))]
+    print(f"[lss]   {infos}", flush=True)
+    for inf in infos:
+        assert inf["loss_before"] > 0 and inf["loss_after"] < inf["loss_before"]
+    for s in sorters:
+        s.close()
