@@ -159,11 +159,15 @@ def bench_stream(args):
     refine_every = int(round(args.refine_seconds * fs / batch)) if args.refine_seconds > 0 else 0
     tuning = []
     shares = batcher_grid_shares(sorters, batch + 2 * probes[0].art["nt"])
-    for s, p, cap in zip(sorters, probes, shares):
+    for s, s_plain, p, cap in zip(sorters, plain_sorters, probes, shares):
         w = torch.from_numpy(window(p.rec, 3, p.art["nt"])).to("cuda:0")
+        tiles, tile_ms = s.autotune_tiles(w)
+        plain_tiles, _ = s_plain.autotune_tiles(w)
         g, p1, _ = autotune(s, w)
         pg, pf, pms = autotune_persistent(s, w, max_grid=cap)
-        tuning.append({"update_grid": g, "first_passes": p1, "grid_share": cap, "persistent_grid": pg, "persistent_after": pf,
+        tuning.append({"tiles": tiles, "tile_ms": tile_ms, "plain_tiles": plain_tiles,
+                       "update_grid": g, "first_passes": p1, "grid_share": cap, "persistent_grid": pg,
+                       "persistent_after": pf,
                        "persistent_ms": {f"{k[0]}/{k[1]}": round(v, 3) for k, v in pms.items()}})
     nbat = (n - 2 * batch) // batch
     dev = torch.device("cuda:0")
@@ -235,6 +239,7 @@ def bench_stream(args):
     host_sorters = [Sorter(p.art, max_window=batch + 2 * p.art["nt"], max_passes=100, precision=args.precision,
                            max_spikes=1 << 15) for p in probes]
     for s2, tu in zip(host_sorters, tuning):
+        s2.set_tiles(*tu["tiles"])
         s2.set_persistent(tu["persistent_grid"], tu["persistent_after"])
         from livespikesorter_amd._lib import lib as _l
         _l().lss_set_tuning(s2._h, tu["update_grid"], tu["first_passes"])

@@ -324,6 +324,24 @@ lss_status lss_autotune(void* handle, const int16_t* d_window, int64_t W, int32_
 lss_status lss_set_tuning(void* handle, int32_t update_grid, int32_t first_passes);
 lss_status lss_get_tuning(void* handle, int32_t* update_grid, int32_t* first_passes);
 
+/* Tile shapes of the window's three contractions: the whitening GEMM, the wPCA projection and the
+ * U.Xproj score GEMM.  GEMM output tiles: 0 = 128 x 128 (default), 1 = 128 x 64, 2 = 64 x 128,
+ * 3 = 64 x 64 samples x outputs (small tiles fill the device on short windows); projection
+ * workgroups: FP32 path 0 = by window length (default), 1/2/3 = 32/64/128 samples; split-precision
+ * path 0 and 2 = 64 channels (default), 1 = 32, 3 = 128.  Every shape gives bit-identical results
+ * (each output's accumulation order is the same), so tuning never changes a spike list.  A
+ * streaming batcher captures the shapes set when it first runs a step. */
+#define LSS_TILE_SHAPES 4
+lss_status lss_set_tiles(void* handle, int32_t whiten, int32_t project, int32_t scores);
+/* tiles[3] = whitening, projection, scores */
+lss_status lss_get_tiles(void* handle, int32_t* tiles);
+/* Times each stage of a real window (device int16 [W][C]) with every shape (HIP events on the
+ * handle's stream, best of reps runs after a warm-up run) and keeps the fastest per stage; best[3]
+ * receives the choice, ms (12 floats, may be NULL) the times: [4*stage + shape], stage 0 =
+ * whitening, 1 = projection, 2 = scores. */
+lss_status lss_autotune_tiles(void* handle, const int16_t* d_window, int64_t W, int32_t reps, float* ms,
+                              int32_t* best);
+
 /* Persistent pursuit: windows of at most LSS_PERSIST_MAX_WINDOW samples (streaming batches) run
  * their first `first_passes` passes (the ones with many spikes, whose wide updates want the whole
  * chip) as per-pass kernels and the rest in one launch of `grid` co-resident workgroups with two

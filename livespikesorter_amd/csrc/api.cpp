@@ -97,6 +97,8 @@ struct Handle {
     DevBuf<int> imax;
     DevBuf<unsigned char> flags;
     DevBuf<int> sp_col, sp_tmp, counters, pass_off;
+    DevBuf<float> prow_cf, prow_amp;   // persistent pursuit rows [2][kPersistMaxW] (pass parity)
+    DevBuf<int> prow_im;
     DevBuf<float> sp_amp, tF, xs, ys;
     int* h_counters = nullptr;   // pinned
     // pinned readback staging of the batched/streaming path (first stage_cap spikes)
@@ -122,6 +124,9 @@ struct Handle {
     int tuned_passes = 0;       // passes in the first graph segment of the batcher (0 = all)
     int pers_grid = 128;        // workgroups of the persistent pursuit (windows <= kPersistMaxW; 0 = off)
     int pers_first = 0;         // leading passes on the per-pass kernels before the persistent kernel
+    int tile_whiten = 0;        // GEMM output tile shapes (launch_gemm_*: 0 = 128x128 .. 3 = 64x64), autotuned
+    int tile_scores = 0;
+    int proj_tile = 0;          // wPCA projection workgroup shape (launch_project*), autotuned
     std::vector<hipEvent_t> upd_ev;
     float upd_ms = 0.f;
     int upd_launches = 0;
@@ -375,6 +380,9 @@ lss_status create_impl(const lss_params* p, const lss_artifacts* a, Handle** out
     HIP_TRY(h->sp_amp.alloc(p->max_spikes));
     HIP_TRY(h->counters.alloc(lss::kCounters));
     HIP_TRY(h->pass_off.alloc(p->max_passes + 2));
+    HIP_TRY(h->prow_cf.alloc(2 * (size_t)lss::kPersistMaxW));
+    HIP_TRY(h->prow_im.alloc(2 * (size_t)lss::kPersistMaxW));
+    HIP_TRY(h->prow_amp.alloc(2 * (size_t)lss::kPersistMaxW));
     HIP_TRY(h->tF.alloc((size_t)p->max_spikes * nC * K));
     HIP_TRY(h->xs.alloc(p->max_spikes));
     HIP_TRY(h->ys.alloc(p->max_spikes));
@@ -418,10 +426,11 @@ lss_status run_highpass(Handle& h, const float* in, float* out, int64_t W) {
 lss_status run_whiten(Handle& h, const float* in_cw, float* out_wc, int64_t W) {
     // Xw[w][c'] = sum_c X[c][w] Wm[c'][c]
     if (h.prm.precision == LSS_PREC_FP32)
-        HIP_TRY(lss::launch_gemm_f32(in_cw, h.ldw, 1, h.wmat.p, h.ldm, out_wc, h.ldC, (int)W, h.C, h.C, h.stream));
+        HIP_TRY(lss::launch_gemm_f32(in_cw, h.ldw, 1, h.wmat.p, h.ldm, out_wc, h.ldC, (int)W, h.C, h.C, h.stream,
+                                     h.tile_whiten));
     else
         HIP_TRY(lss::launch_gemm_lp(h.prm.precision, in_cw, h.ldw, 1, h.wmat.p, h.ldm, out_wc, h.ldC, (int)W, h.C, h.C,
-                                    h.stream));
+                                    h.stream, h.tile_whiten));
     return LSS_OK;
 }
 
@@ -429,16 +438,17 @@ lss_status run_scores(Handle& h, const float* x_wc, float* b_wt, int64_t W, hipE
     // the split-precision paths project on the matrix cores with the same split as their score GEMM
     if (h.prm.precision == LSS_PREC_BF16X3 || h.prm.precision == LSS_PREC_FP16X3)
         HIP_TRY(lss::launch_project_x3(h.prm.precision, x_wc, h.ldC, W, h.C, h.wpca.p, h.K, h.nt, h.xp.p, h.ldq,
-                                       h.stream));
+                                       h.stream, h.proj_tile));
     else
-        HIP_TRY(lss::launch_project(x_wc, h.ldC, W, h.C, h.wpca.p, h.K, h.nt, h.xp.p, h.ldq, h.stream));
+        HIP_TRY(lss::launch_project(x_wc, h.ldC, W, h.C, h.wpca.p, h.K, h.nt, h.xp.p, h.ldq, h.stream, h.proj_tile));
     if (mid) HIP_TRY(hipEventRecord(mid, h.stream));
     const int KC = h.K * h.C;
     if (h.prm.precision == LSS_PREC_FP32)
-        HIP_TRY(lss::launch_gemm_f32(h.xp.p, h.ldq, 0, h.U.p, h.ldq, b_wt, h.ldT, (int)W, h.T, KC, h.stream));
+        HIP_TRY(lss::launch_gemm_f32(h.xp.p, h.ldq, 0, h.U.p, h.ldq, b_wt, h.ldT, (int)W, h.T, KC, h.stream,
+                                     h.tile_scores));
     else
         HIP_TRY(lss::launch_gemm_lp(h.prm.precision, h.xp.p, h.ldq, 0, h.U.p, h.ldq, b_wt, h.ldT, (int)W, h.T, KC,
-                                    h.stream));
+                                    h.stream, h.tile_scores));
     return LSS_OK;
 }
 
@@ -451,6 +461,7 @@ lss::PursuitArgs pursuit_args(Handle& h, float* B, float* X, int64_t W) {
     lss::PursuitArgs a{};
     a.B = B; a.X = X; a.nm = h.nm.p; a.ctc = h.ctc.p; a.waves = h.waves.p;
     a.pass_off = h.pass_off.p;
+    a.prow_cf = h.prow_cf.p; a.prow_im = h.prow_im.p; a.prow_amp = h.prow_amp.p;
     a.cfmax = h.cfmax.p; a.imax = h.imax.p; a.flags = h.flags.p;
     a.sp_col = h.sp_col.p; a.sp_tmp = h.sp_tmp.p; a.sp_amp = h.sp_amp.p; a.counters = h.counters.p;
     a.W = W; a.ldT = h.ldT; a.ldC = h.ldC; a.T = h.T; a.C = h.C; a.K = h.K; a.nt = h.nt; a.cap = h.prm.max_spikes;
@@ -1837,6 +1848,91 @@ lss_status lss_autotune_persistent(void* handle, const int16_t* d_window, int64_
     h.pers_first = bf;
     *best_grid = bg;
     *best_first = bf;
+    return LSS_OK;
+}
+
+lss_status lss_set_tiles(void* handle, int32_t whiten, int32_t project, int32_t scores) {
+    if (!handle) return fail(LSS_ERR_ARG, "null handle");
+    for (int t : {whiten, project, scores})
+        if (t < 0 || t >= lss::kGemmTiles) return fail(LSS_ERR_ARG, "tile shape out of range");
+    Handle& h = *as_handle(handle);
+    h.tile_whiten = whiten;
+    h.proj_tile = project;
+    h.tile_scores = scores;
+    return LSS_OK;
+}
+
+lss_status lss_get_tiles(void* handle, int32_t* tiles) {
+    if (!handle || !tiles) return fail(LSS_ERR_ARG, "null argument");
+    Handle& h = *as_handle(handle);
+    tiles[0] = h.tile_whiten;
+    tiles[1] = h.proj_tile;
+    tiles[2] = h.tile_scores;
+    return LSS_OK;
+}
+
+lss_status lss_autotune_tiles(void* handle, const int16_t* d_window, int64_t W, int32_t reps, float* ms,
+                              int32_t* best) {
+    if (!handle || !d_window || reps < 1 || !best) return fail(LSS_ERR_ARG, "bad argument");
+    Handle& h = *as_handle(handle);
+    if (!h.alive) return fail(LSS_ERR_STATE, "handle not usable");
+    if (lss_status st = check_window(h, W)) return st;
+    // the window's filtered data as the stages below see it
+    if (lss_status st = run_car(h, d_window, W)) return st;
+    if (lss_status st = run_highpass(h, h.xcw.p, h.xhp.p, W)) return st;
+    const bool x3 = h.prm.precision == LSS_PREC_BF16X3 || h.prm.precision == LSS_PREC_FP16X3;
+    int* knob[3] = {&h.tile_whiten, &h.proj_tile, &h.tile_scores};
+    // stage 0: whitening GEMM, 1: projection, 2: score GEMM; each on the previous stage's output
+    auto run_stage = [&](int stage) -> lss_status {
+        if (stage == 0) return run_whiten(h, h.xhp.p, h.xw.p, W);
+        const int KC = h.K * h.C;
+        if (stage == 1) {
+            if (x3)
+                HIP_TRY(lss::launch_project_x3(h.prm.precision, h.xw.p, h.ldC, W, h.C, h.wpca.p, h.K, h.nt, h.xp.p,
+                                               h.ldq, h.stream, h.proj_tile));
+            else
+                HIP_TRY(lss::launch_project(h.xw.p, h.ldC, W, h.C, h.wpca.p, h.K, h.nt, h.xp.p, h.ldq, h.stream,
+                                            h.proj_tile));
+            return LSS_OK;
+        }
+        if (h.prm.precision == LSS_PREC_FP32)
+            HIP_TRY(lss::launch_gemm_f32(h.xp.p, h.ldq, 0, h.U.p, h.ldq, h.B.p, h.ldT, (int)W, h.T, KC, h.stream,
+                                         h.tile_scores));
+        else
+            HIP_TRY(lss::launch_gemm_lp(h.prm.precision, h.xp.p, h.ldq, 0, h.U.p, h.ldq, h.B.p, h.ldT, (int)W, h.T, KC,
+                                        h.stream, h.tile_scores));
+        return LSS_OK;
+    };
+    for (int stage = 0; stage < 3; ++stage) {
+        const int keep = *knob[stage];
+        float bt = 1e30f;
+        int bi = keep;
+        for (int t = 0; t < lss::kGemmTiles; ++t) {
+            *knob[stage] = t;
+            float tm = 1e30f;
+            for (int r = 0; r <= reps; ++r) {   // the first run warms up
+                HIP_TRY(hipEventRecord(h.ev[0], h.stream));
+                if (lss_status st = run_stage(stage)) {
+                    *knob[stage] = keep;
+                    return st;
+                }
+                HIP_TRY(hipEventRecord(h.ev[1], h.stream));
+                HIP_TRY(hipEventSynchronize(h.ev[1]));
+                float e = 0.f;
+                HIP_TRY(hipEventElapsedTime(&e, h.ev[0], h.ev[1]));
+                if (r > 0) tm = std::min(tm, e);
+            }
+            if (ms) ms[stage * lss::kGemmTiles + t] = tm;
+            if (tm < bt) {
+                bt = tm;
+                bi = t;
+            }
+        }
+        *knob[stage] = bi;
+        best[stage] = bi;
+        if (lss_status st = run_stage(stage)) return st;   // the next stage's input
+    }
+    HIP_TRY(hipStreamSynchronize(h.stream));
     return LSS_OK;
 }
 

@@ -3,8 +3,11 @@
 //   whitening : Xw[w][c'] = sum_c X[c][w] * Wm[c'][c]            (M = W, N = C, K = C)
 //   scores    : B[w][t]   = sum_q Xp[w][q] * U[t][q], q = k*C+c   (M = W, N = T, K = K*C)
 //
-// both as C[m][n] = sum_k A(m,k) B(n,k).  A 128x128 output tile per 256-thread workgroup; each of
-// the 4 wavefronts owns a 64x64 quarter as 2x2 blocks of 32x32 MFMA accumulators.  The K loop
+// both as C[m][n] = sum_k A(m,k) B(n,k).  A TM x TN output tile per 256-thread workgroup (TM, TN in
+// {64, 128}: four shapes, picked per handle by the tile autotuner); each of the 4 wavefronts owns a
+// (TM/2) x (TN/2) quarter as blocks of 32x32 MFMA accumulators.  Every output element belongs to
+// one 32x32 block whose K loop runs in the same order for every shape, so all shapes give
+// bit-identical results; small tiles fill the device on short windows.  The K loop
 // advances 32 at a time with register prefetch: the global loads of step k+1 are issued before
 // the MFMAs of step k, so load latency hides behind matrix work.  Workgroups are renumbered so
 // that each XCD (blockIdx % 8) walks a contiguous run of tiles with the N tiles of one M
@@ -25,7 +28,8 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
 
-constexpr int BM = 128, BN = 128, BK = 32;
+constexpr int BK = 32;
+constexpr int kMaxTile = 128;
 
 __device__ __forceinline__ void xcd_tile(int nM, int nN, int& mt, int& nt) {
     const int total = nM * nN;
@@ -38,21 +42,23 @@ __device__ __forceinline__ void xcd_tile(int nM, int nN, int& mt, int& nt) {
     nt = v % nN;
 }
 
-// ---- global tile fetch: 128 rows x 32 k of an MK (row = m, k contiguous) or KM operand -------
-// MK: thread t loads rows t/2 (+64 for the second half), k quarter (t&1)*16 .. +16 as 4 float4.
-// KM: thread t loads k row t/8, m quarter (t&7)*16 .. +16 as 4 float4.
+// ---- global tile fetch: ROWS rows (64 or 128) x 32 k of an MK (row = m, k contiguous) or KM operand
+// MK: thread t loads row t/(256/ROWS), k range (t % (256/ROWS)) * (ROWS/8) .. + ROWS/8 as float4s.
+// KM: thread t loads k row t/8, m range (t&7) * (ROWS/8) .. + ROWS/8 as float4s.
 struct Frag {
     floatx4 v[4];
 };
 
-template <bool KM>
+template <bool KM, int ROWS>
 __device__ __forceinline__ void fetch(const float* __restrict__ P, int64_t ld, int r0, int k0, int R, int K,
                                       Frag& f) {
+    constexpr int H = ROWS / 32;             // float4 per thread
     const int tid = threadIdx.x;
     if (!KM) {
-        const int row = tid >> 1, kq = (tid & 1) * 16;
+        constexpr int TPR = 256 / ROWS;      // threads per row
+        const int row = tid / TPR, kq = (tid % TPR) * (4 * H);
 #pragma unroll
-        for (int h = 0; h < 4; ++h) {
+        for (int h = 0; h < H; ++h) {
             const int r = r0 + row;
             const int k = k0 + kq + 4 * h;
             floatx4 v = {0.f, 0.f, 0.f, 0.f};
@@ -64,12 +70,11 @@ __device__ __forceinline__ void fetch(const float* __restrict__ P, int64_t ld, i
             }
             f.v[h] = v;
         }
-        (void)R;
     } else {
-        const int kr = tid >> 3, mq = (tid & 7) * 16;
+        const int kr = tid >> 3, mq = (tid & 7) * (4 * H);
         const int k = k0 + kr;
 #pragma unroll
-        for (int h = 0; h < 4; ++h) {
+        for (int h = 0; h < H; ++h) {
             const int r = r0 + mq + 4 * h;
             floatx4 v = {0.f, 0.f, 0.f, 0.f};
             if (k < K) {
@@ -83,82 +88,87 @@ __device__ __forceinline__ void fetch(const float* __restrict__ P, int64_t ld, i
     }
 }
 
-// FP32: LDS tile [BK][128 + 4] (k-major)
-template <bool KM>
-__device__ __forceinline__ void stash_f32(float (*S)[BM + 4], const Frag& f) {
+// FP32: LDS tile [BK][ROWS + 4] (k-major)
+template <bool KM, int ROWS>
+__device__ __forceinline__ void stash_f32(float (*S)[kMaxTile + 4], const Frag& f) {
+    constexpr int H = ROWS / 32;
     const int tid = threadIdx.x;
     if (!KM) {
-        const int row = tid >> 1, kq = (tid & 1) * 16;
+        constexpr int TPR = 256 / ROWS;
+        const int row = tid / TPR, kq = (tid % TPR) * (4 * H);
 #pragma unroll
-        for (int h = 0; h < 4; ++h)
+        for (int h = 0; h < H; ++h)
 #pragma unroll
             for (int e = 0; e < 4; ++e) S[kq + 4 * h + e][row] = f.v[h][e];
     } else {
-        const int kr = tid >> 3, mq = (tid & 7) * 16;
+        const int kr = tid >> 3, mq = (tid & 7) * (4 * H);
 #pragma unroll
-        for (int h = 0; h < 4; ++h) *(floatx4*)&S[kr][mq + 4 * h] = f.v[h];
+        for (int h = 0; h < H; ++h) *(floatx4*)&S[kr][mq + 4 * h] = f.v[h];
     }
 }
 
-template <bool A_KM>
+// TM x TN tile: wavefront (wm, wn) owns rows wm*TM/2 .. as I = TM/64 blocks of 32 and columns
+// wn*TN/2 .. as J = TN/64 blocks
+template <bool A_KM, int TM, int TN>
 __global__ __launch_bounds__(256) void gemm_f32_kernel(const float* __restrict__ A, int64_t lda,
                                                        const float* __restrict__ B, int64_t ldb,
                                                        float* __restrict__ C, int64_t ldc,
                                                        int M, int N, int K, int nM, int nN) {
-    __shared__ float As[BK][BM + 4];
-    __shared__ float Bs[BK][BN + 4];
+    constexpr int I = TM / 64, J = TN / 64;
+    __shared__ float As[BK][kMaxTile + 4];
+    __shared__ float Bs[BK][kMaxTile + 4];
     int mt, ntile;
     xcd_tile(nM, nN, mt, ntile);
     if (mt >= nM) return;
-    const int m0 = mt * BM, n0 = ntile * BN;
+    const int m0 = mt * TM, n0 = ntile * TN;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave >> 1, wn = wave & 1;
 
-    floatx16 acc[2][2];
+    floatx16 acc[I][J];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < I; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < J; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
 
     Frag fa, fb;
-    fetch<A_KM>(A, lda, m0, 0, M, K, fa);
-    fetch<false>(B, ldb, n0, 0, N, K, fb);
+    fetch<A_KM, TM>(A, lda, m0, 0, M, K, fa);
+    fetch<false, TN>(B, ldb, n0, 0, N, K, fb);
     for (int k0 = 0; k0 < K; k0 += BK) {
-        stash_f32<A_KM>(As, fa);
-        stash_f32<false>(Bs, fb);
+        stash_f32<A_KM, TM>(As, fa);
+        stash_f32<false, TN>(Bs, fb);
         __syncthreads();
         if (k0 + BK < K) {
-            fetch<A_KM>(A, lda, m0, k0 + BK, M, K, fa);
-            fetch<false>(B, ldb, n0, k0 + BK, N, K, fb);
+            fetch<A_KM, TM>(A, lda, m0, k0 + BK, M, K, fa);
+            fetch<false, TN>(B, ldb, n0, k0 + BK, N, K, fb);
         }
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 2) {
             const int kr = kk + (lane >> 5);
-            float a[2], b[2];
+            float a[I], b[J];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) a[i] = As[kr][wm * 64 + i * 32 + (lane & 31)];
+            for (int i = 0; i < I; ++i) a[i] = As[kr][wm * (TM / 2) + i * 32 + (lane & 31)];
 #pragma unroll
-            for (int j = 0; j < 2; ++j) b[j] = Bs[kr][wn * 64 + j * 32 + (lane & 31)];
+            for (int j = 0; j < J; ++j) b[j] = Bs[kr][wn * (TN / 2) + j * 32 + (lane & 31)];
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < I; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j)
+                for (int j = 0; j < J; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i], b[j], acc[i][j], 0, 0, 0);
         }
         __syncthreads();
     }
     // acc[r] of lane l is row 8*(r/4) + 4*(l/32) + r%4, column l%32 of its 32x32 block
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < I; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int n = n0 + wn * 64 + j * 32 + (lane & 31);
+        for (int j = 0; j < J; ++j) {
+            const int n = n0 + wn * (TN / 2) + j * 32 + (lane & 31);
             if (n >= N) continue;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const int m = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int m = m0 + wm * (TM / 2) + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
                 if (m < M) C[(int64_t)m * ldc + n] = acc[i][j][r];
             }
         }
@@ -187,17 +197,17 @@ constexpr bool is_split() { return PREC == LSS_PREC_BF16X3 || PREC == LSS_PREC_F
 // (bf16x3 scores 0.68 -> 0.61 ms, plain BF16 0.56 -> 0.42 ms).  The k-major operand kept its
 // layout (32 lanes per k row made the whitening GEMM slower, 0.11 -> 0.17 ms), and so did the
 // FP32 kernel (eight lanes per row: 1.71 vs 1.67 ms; its MFMA work is 16x slower per element).
-template <bool KM>
+template <bool KM, int ROWS>
 __device__ __forceinline__ void fetch_lp(const float* __restrict__ P, int64_t ld, int r0, int k0, int R, int K,
                                          Frag& f) {
     if (KM) {
-        fetch<true>(P, ld, r0, k0, R, K, f);
+        fetch<true, ROWS>(P, ld, r0, k0, R, K, f);
         return;
     }
     const int tid = threadIdx.x;
     const int row = tid >> 3, k = k0 + (tid & 7) * 4;
 #pragma unroll
-    for (int h = 0; h < 4; ++h) {
+    for (int h = 0; h < ROWS / 32; ++h) {
         const int r = r0 + row + 32 * h;
         floatx4 v = {0.f, 0.f, 0.f, 0.f};
         if (r < R) {
@@ -211,28 +221,28 @@ __device__ __forceinline__ void fetch_lp(const float* __restrict__ P, int64_t ld
 }
 
 // element (row, k) of the tile held by this thread's Frag (as fetch_lp loads it)
-template <bool KM>
+template <bool KM, int ROWS>
 __device__ __forceinline__ void frag_coord(int h, int e, int& row, int& k) {
     const int tid = threadIdx.x;
     if (!KM) {
         row = (tid >> 3) + 32 * h;
         k = (tid & 7) * 4 + e;
     } else {
-        row = (tid & 7) * 16 + 4 * h + e;
+        row = (tid & 7) * (ROWS / 8) + 4 * h + e;
         k = tid >> 3;
     }
 }
 
-template <int PREC, bool KM>
+template <int PREC, bool KM, int ROWS>
 __device__ __forceinline__ void stash_lp(typename LpType<PREC>::T (*Hi)[kLdsK], typename LpType<PREC>::T (*Lo)[kLdsK],
                                          const Frag& f) {
     typedef typename LpType<PREC>::T T;
 #pragma unroll
-    for (int h = 0; h < 4; ++h)
+    for (int h = 0; h < ROWS / 32; ++h)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             int row, k;
-            frag_coord<KM>(h, e, row, k);
+            frag_coord<KM, ROWS>(h, e, row, k);
             const float x = f.v[h][e];
             const T hi = (T)x;
             Hi[row][k] = hi;
@@ -249,7 +259,7 @@ __device__ __forceinline__ floatx16 mfma_lp(typename LpType<PREC>::V a, typename
 
 // Three workgroups per CU (registers capped at 168; the 40 KB of LDS fits three times): the
 // extra wavefront per SIMD hides more of the tile loads (bf16x3 scores 0.71 -> 0.68 ms).
-template <int PREC, bool A_KM>
+template <int PREC, bool A_KM, int TM, int TN>
 __global__ __launch_bounds__(256, 3) void gemm_lp_kernel(const float* __restrict__ A, int64_t lda,
                                                       const float* __restrict__ B, int64_t ldb,
                                                       float* __restrict__ C, int64_t ldc,
@@ -257,56 +267,57 @@ __global__ __launch_bounds__(256, 3) void gemm_lp_kernel(const float* __restrict
     typedef typename LpType<PREC>::T T;
     typedef typename LpType<PREC>::V V;
     constexpr bool X3 = is_split<PREC>();
-    __shared__ __attribute__((aligned(16))) T Ah[BM][kLdsK];
-    __shared__ __attribute__((aligned(16))) T Bh[BN][kLdsK];
-    __shared__ __attribute__((aligned(16))) T Al[X3 ? BM : 1][kLdsK];
-    __shared__ __attribute__((aligned(16))) T Bl[X3 ? BN : 1][kLdsK];
+    constexpr int I = TM / 64, J = TN / 64;
+    __shared__ __attribute__((aligned(16))) T Ah[TM][kLdsK];
+    __shared__ __attribute__((aligned(16))) T Bh[TN][kLdsK];
+    __shared__ __attribute__((aligned(16))) T Al[X3 ? TM : 1][kLdsK];
+    __shared__ __attribute__((aligned(16))) T Bl[X3 ? TN : 1][kLdsK];
     int mt, ntile;
     xcd_tile(nM, nN, mt, ntile);
     if (mt >= nM) return;
-    const int m0 = mt * BM, n0 = ntile * BN;
+    const int m0 = mt * TM, n0 = ntile * TN;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wm = wave >> 1, wn = wave & 1;
 
-    floatx16 acc[2][2];
+    floatx16 acc[I][J];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < I; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < J; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
 
     Frag fa, fb;
-    fetch_lp<A_KM>(A, lda, m0, 0, M, K, fa);
-    fetch_lp<false>(B, ldb, n0, 0, N, K, fb);
+    fetch_lp<A_KM, TM>(A, lda, m0, 0, M, K, fa);
+    fetch_lp<false, TN>(B, ldb, n0, 0, N, K, fb);
     for (int k0 = 0; k0 < K; k0 += BK) {
-        stash_lp<PREC, A_KM>(Ah, Al, fa);
-        stash_lp<PREC, false>(Bh, Bl, fb);
+        stash_lp<PREC, A_KM, TM>(Ah, Al, fa);
+        stash_lp<PREC, false, TN>(Bh, Bl, fb);
         __syncthreads();
         if (k0 + BK < K) {
-            fetch_lp<A_KM>(A, lda, m0, k0 + BK, M, K, fa);
-            fetch_lp<false>(B, ldb, n0, k0 + BK, N, K, fb);
+            fetch_lp<A_KM, TM>(A, lda, m0, k0 + BK, M, K, fa);
+            fetch_lp<false, TN>(B, ldb, n0, k0 + BK, N, K, fb);
         }
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 16) {
             const int kq = kk + 8 * (lane >> 5);
-            V ah[2], bh[2], al[2], bl[2];
+            V ah[I], bh[J], al[I], bl[J];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                const int r = wm * 64 + i * 32 + (lane & 31);
+            for (int i = 0; i < I; ++i) {
+                const int r = wm * (TM / 2) + i * 32 + (lane & 31);
                 ah[i] = *(const V*)&Ah[r][kq];
                 if constexpr (X3) al[i] = *(const V*)&Al[r][kq];
             }
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                const int r = wn * 64 + j * 32 + (lane & 31);
+            for (int j = 0; j < J; ++j) {
+                const int r = wn * (TN / 2) + j * 32 + (lane & 31);
                 bh[j] = *(const V*)&Bh[r][kq];
                 if constexpr (X3) bl[j] = *(const V*)&Bl[r][kq];
             }
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < I; ++i)
 #pragma unroll
-                for (int j = 0; j < 2; ++j) {
+                for (int j = 0; j < J; ++j) {
                     if constexpr (X3) {
                         acc[i][j] = mfma_lp<PREC>(al[i], bh[j], acc[i][j]);
                         acc[i][j] = mfma_lp<PREC>(ah[i], bl[j], acc[i][j]);
@@ -317,51 +328,80 @@ __global__ __launch_bounds__(256, 3) void gemm_lp_kernel(const float* __restrict
         __syncthreads();
     }
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < I; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int n = n0 + wn * 64 + j * 32 + (lane & 31);
+        for (int j = 0; j < J; ++j) {
+            const int n = n0 + wn * (TN / 2) + j * 32 + (lane & 31);
             if (n >= N) continue;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const int m = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int m = m0 + wm * (TM / 2) + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
                 if (m < M) C[(int64_t)m * ldc + n] = acc[i][j][r];
             }
         }
 }
 
-hipError_t launch_gemm_f32(const float* A, int64_t lda, int a_km, const float* B, int64_t ldb,
-                           float* C, int64_t ldc, int M, int N, int K, hipStream_t s) {
-    if ((lda & 3) || (ldb & 3)) return hipErrorInvalidValue;
-    const int nM = (M + BM - 1) / BM, nN = (N + BN - 1) / BN;
+// tile shape t: 0 = 128 x 128, 1 = 128 x 64, 2 = 64 x 128, 3 = 64 x 64 (TM x TN)
+template <int TM, int TN>
+hipError_t f32_launch(const float* A, int64_t lda, int a_km, const float* B, int64_t ldb, float* C, int64_t ldc,
+                      int M, int N, int K, hipStream_t s) {
+    const int nM = (M + TM - 1) / TM, nN = (N + TN - 1) / TN;
     dim3 grid(8 * ((nM * nN + 7) / 8));
     if (a_km)
-        hipLaunchKernelGGL(gemm_f32_kernel<true>, grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K, nM, nN);
+        hipLaunchKernelGGL((gemm_f32_kernel<true, TM, TN>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K,
+                           nM, nN);
     else
-        hipLaunchKernelGGL(gemm_f32_kernel<false>, grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K, nM, nN);
+        hipLaunchKernelGGL((gemm_f32_kernel<false, TM, TN>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K,
+                           nM, nN);
+    return hipGetLastError();
+}
+
+hipError_t launch_gemm_f32(const float* A, int64_t lda, int a_km, const float* B, int64_t ldb,
+                           float* C, int64_t ldc, int M, int N, int K, hipStream_t s, int tile) {
+    if ((lda & 3) || (ldb & 3)) return hipErrorInvalidValue;
+    switch (tile) {
+        case 0: return f32_launch<128, 128>(A, lda, a_km, B, ldb, C, ldc, M, N, K, s);
+        case 1: return f32_launch<128, 64>(A, lda, a_km, B, ldb, C, ldc, M, N, K, s);
+        case 2: return f32_launch<64, 128>(A, lda, a_km, B, ldb, C, ldc, M, N, K, s);
+        case 3: return f32_launch<64, 64>(A, lda, a_km, B, ldb, C, ldc, M, N, K, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <int PREC, int TM, int TN>
+hipError_t lp_launch(const float* A, int64_t lda, int a_km, const float* B, int64_t ldb, float* C, int64_t ldc,
+                     int M, int N, int K, hipStream_t s) {
+    const int nM = (M + TM - 1) / TM, nN = (N + TN - 1) / TN;
+    dim3 grid(8 * ((nM * nN + 7) / 8));
+    if (a_km)
+        hipLaunchKernelGGL((gemm_lp_kernel<PREC, true, TM, TN>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K,
+                           nM, nN);
+    else
+        hipLaunchKernelGGL((gemm_lp_kernel<PREC, false, TM, TN>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N,
+                           K, nM, nN);
     return hipGetLastError();
 }
 
 template <int PREC>
-hipError_t lp_launch(const float* A, int64_t lda, int a_km, const float* B, int64_t ldb, float* C, int64_t ldc,
-                     int M, int N, int K, hipStream_t s) {
-    const int nM = (M + BM - 1) / BM, nN = (N + BN - 1) / BN;
-    dim3 grid(8 * ((nM * nN + 7) / 8));
-    if (a_km)
-        hipLaunchKernelGGL((gemm_lp_kernel<PREC, true>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K, nM, nN);
-    else
-        hipLaunchKernelGGL((gemm_lp_kernel<PREC, false>), grid, dim3(256), 0, s, A, lda, B, ldb, C, ldc, M, N, K, nM, nN);
-    return hipGetLastError();
+hipError_t lp_tile(const float* A, int64_t lda, int a_km, const float* B, int64_t ldb, float* C, int64_t ldc,
+                   int M, int N, int K, hipStream_t s, int tile) {
+    switch (tile) {
+        case 0: return lp_launch<PREC, 128, 128>(A, lda, a_km, B, ldb, C, ldc, M, N, K, s);
+        case 1: return lp_launch<PREC, 128, 64>(A, lda, a_km, B, ldb, C, ldc, M, N, K, s);
+        case 2: return lp_launch<PREC, 64, 128>(A, lda, a_km, B, ldb, C, ldc, M, N, K, s);
+        case 3: return lp_launch<PREC, 64, 64>(A, lda, a_km, B, ldb, C, ldc, M, N, K, s);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_gemm_lp(int prec, const float* A, int64_t lda, int a_km, const float* B, int64_t ldb,
-                          float* C, int64_t ldc, int M, int N, int K, hipStream_t s) {
+                          float* C, int64_t ldc, int M, int N, int K, hipStream_t s, int tile) {
     if ((lda & 3) || (ldb & 3)) return hipErrorInvalidValue;
     switch (prec) {
-        case LSS_PREC_BF16: return lp_launch<LSS_PREC_BF16>(A, lda, a_km, B, ldb, C, ldc, M, N, K, s);
-        case LSS_PREC_FP16: return lp_launch<LSS_PREC_FP16>(A, lda, a_km, B, ldb, C, ldc, M, N, K, s);
-        case LSS_PREC_BF16X3: return lp_launch<LSS_PREC_BF16X3>(A, lda, a_km, B, ldb, C, ldc, M, N, K, s);
-        case LSS_PREC_FP16X3: return lp_launch<LSS_PREC_FP16X3>(A, lda, a_km, B, ldb, C, ldc, M, N, K, s);
+        case LSS_PREC_BF16: return lp_tile<LSS_PREC_BF16>(A, lda, a_km, B, ldb, C, ldc, M, N, K, s, tile);
+        case LSS_PREC_FP16: return lp_tile<LSS_PREC_FP16>(A, lda, a_km, B, ldb, C, ldc, M, N, K, s, tile);
+        case LSS_PREC_BF16X3: return lp_tile<LSS_PREC_BF16X3>(A, lda, a_km, B, ldb, C, ldc, M, N, K, s, tile);
+        case LSS_PREC_FP16X3: return lp_tile<LSS_PREC_FP16X3>(A, lda, a_km, B, ldb, C, ldc, M, N, K, s, tile);
         default: return hipErrorInvalidValue;
     }
 }

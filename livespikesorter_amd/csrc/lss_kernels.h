@@ -36,18 +36,21 @@ hipError_t launch_fft_rows(float2* data, int nb, const float2* tw, hipStream_t s
 // ---- contractions (gemm.hip, project.hip) ---------------------------------------------------
 // C[m][n] = sum_k A(m,k) B(n,k);  A(m,k) = A[m*lda+k] (a_km=0) or A[k*lda+m] (a_km=1);
 // B(n,k) = B[n*ldb+k].  FP32 MFMA, FP32 accumulation.  lda, ldb multiples of 4; 16-byte aligned.
+// tile: output tile shape 0 = 128x128, 1 = 128x64, 2 = 64x128, 3 = 64x64 (results identical).
+constexpr int kGemmTiles = 4;
 hipError_t launch_gemm_f32(const float* A, int64_t lda, int a_km, const float* B, int64_t ldb,
-                           float* C, int64_t ldc, int M, int N, int K, hipStream_t s);
+                           float* C, int64_t ldc, int M, int N, int K, hipStream_t s, int tile = 0);
 // Precision variants with operands converted on the fly (prec = lss_precision).
 hipError_t launch_gemm_lp(int prec, const float* A, int64_t lda, int a_km, const float* B,
-                          int64_t ldb, float* C, int64_t ldc, int M, int N, int K, hipStream_t s);
+                          int64_t ldb, float* C, int64_t ldc, int M, int N, int K, hipStream_t s, int tile = 0);
 // Xp[w][k*C+c] = sum_m X[w+m-nt/2][c] * wpca[k][m] (zero outside [0,W)); X rows have stride ldx,
 // Xp rows ldp.  K <= 8, nt <= 127.
+// tile: workgroup shape 0..3 (0 = by window length; results identical for every shape).
 hipError_t launch_project(const float* x_wc, int64_t ldx, int64_t W, int C, const float* wpca, int K,
-                          int nt, float* xp, int64_t ldp, hipStream_t s);
+                          int nt, float* xp, int64_t ldp, hipStream_t s, int tile = 0);
 // The same product on the matrix cores with split operands (prec = LSS_PREC_BF16X3 / FP16X3).
 hipError_t launch_project_x3(int prec, const float* x_wc, int64_t ldx, int64_t W, int C, const float* wpca, int K,
-                             int nt, float* xp, int64_t ldp, hipStream_t s);
+                             int nt, float* xp, int64_t ldp, hipStream_t s, int tile = 0);
 
 // ---- pursuit (pursuit.hip) ------------------------------------------------------------------
 // Row strides ldT (B, ctc, nm) and ldC (X, waves) are multiples of 4 (float4 access); padding
@@ -65,6 +68,11 @@ struct PursuitArgs {
     int* sp_tmp;           // [cap] template
     float* sp_amp;         // [cap]
     int* pass_off;         // [max_passes + 1] first spike of each pass that found spikes
+    float* amprow;         // persistent pursuit: the Cfmax/imax row's amplitudes (set by the launcher)
+    // persistent pursuit: Cfmax, imax and amplitude rows [2][kPersistMaxW], one per pass parity
+    float* prow_cf;
+    int* prow_im;
+    float* prow_amp;
     // [0] spikes of committed passes, [1] spikes of the last pass, [2] capacity overflow,
     // [3] converged (a pass found nothing), [4] passes that found spikes, [5] residual list overflow,
     // [6] grid-barrier arrivals and [7] grid-barrier timeout of the persistent pursuit, [8] the

@@ -87,15 +87,18 @@ hipError_t project_launch(hipStream_t s, const float* x_wc, int64_t ldx, int64_t
 // Windows of fewer than this many samples use 32-sample workgroups.
 constexpr int64_t kProjShortW = 16384;
 
+// tile: 0 = by window length (32 samples below kProjShortW, else 128), 1 = 32, 2 = 64, 3 = 128
+// samples per workgroup (every output is the same FMA chain over m whatever the shape)
 hipError_t launch_project(const float* x_wc, int64_t ldx, int64_t W, int C, const float* wpca, int K,
-                          int nt, float* xp, int64_t ldp, hipStream_t s) {
-    if (K < 1 || K > kMaxK || nt < 1 || nt > kMaxNt) return hipErrorInvalidValue;
-    const bool shortw = W < kProjShortW;
+                          int nt, float* xp, int64_t ldp, hipStream_t s, int tile) {
+    if (K < 1 || K > kMaxK || nt < 1 || nt > kMaxNt || tile < 0 || tile > 3) return hipErrorInvalidValue;
+    const int pw = tile == 1 ? 32 : tile == 2 ? 64 : tile == 3 ? 128 : (W < kProjShortW ? 32 : 128);
     switch (K) {
 #define LSS_PK(KV)                                                                                   \
     case KV:                                                                                         \
-        return shortw ? project_launch<KV, 32>(s, x_wc, ldx, W, C, wpca, nt, xp, ldp)              \
-                      : project_launch<KV, 128>(s, x_wc, ldx, W, C, wpca, nt, xp, ldp);
+        return pw == 32 ? project_launch<KV, 32>(s, x_wc, ldx, W, C, wpca, nt, xp, ldp)            \
+             : pw == 64 ? project_launch<KV, 64>(s, x_wc, ldx, W, C, wpca, nt, xp, ldp)            \
+                        : project_launch<KV, 128>(s, x_wc, ldx, W, C, wpca, nt, xp, ldp);
         LSS_PK(1) LSS_PK(2) LSS_PK(3) LSS_PK(4) LSS_PK(5) LSS_PK(6) LSS_PK(7) LSS_PK(8)
 #undef LSS_PK
     }
@@ -116,9 +119,8 @@ typedef __bf16 pbf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 phalfx8 __attribute__((ext_vector_type(8)));
 
 constexpr int kPxS = 32;      // output samples per workgroup
-constexpr int kPxNB = 2;      // 32-channel blocks per workgroup (64 channels: 28 KB of LDS, so
-                              // more workgroups stay resident than with 128; 0.32 -> 0.24 ms)
-constexpr int kPxC = 32 * kPxNB;
+// NB 32-channel blocks per workgroup: 2 by default (64 channels, 28 KB of LDS, so more workgroups
+// stay resident than with 128 on 2-s windows: 0.32 -> 0.24 ms); 1 and 4 for the tile autotuner
 
 template <int PREC>
 struct PxType;
@@ -133,12 +135,13 @@ __device__ __forceinline__ floatx16 px_mfma(typename PxType<PREC>::V a, typename
     else return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
 }
 
-template <int PREC>
+template <int PREC, int NB>
 __global__ __launch_bounds__(512) void project_x3_kernel(const float* __restrict__ x, int64_t ldx, int64_t W, int C,
                                                          const float* __restrict__ wpca, int K, int nt,
                                                          float* __restrict__ xp, int64_t ldp, int J) {
     typedef typename PxType<PREC>::T T;
     typedef typename PxType<PREC>::V V;
+    constexpr int kPxNB = NB, kPxC = 32 * NB;
     extern __shared__ __attribute__((aligned(16))) unsigned char px_smem[];
     const int ldj = J + 8;                             // 16-byte aligned rows
     T* Xh = (T*)px_smem;                               // [kPxC][ldj]
@@ -201,29 +204,43 @@ __global__ __launch_bounds__(512) void project_x3_kernel(const float* __restrict
     }
 }
 
-template <int PREC>
+template <int PREC, int NB>
 static hipError_t project_x3_launch(const float* x_wc, int64_t ldx, int64_t W, int C, const float* wpca, int K, int nt,
                                     float* xp, int64_t ldp, hipStream_t s) {
+    constexpr int kPxC = 32 * NB;
     const int J = (kPxS + nt - 1 + 15) / 16 * 16;
     const size_t lds = 2 * sizeof(typename PxType<PREC>::T) * (size_t)kPxC * (J + 8) + sizeof(float) * K * nt;
     if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void*)project_x3_kernel<PREC>,
+        hipError_t e = hipFuncSetAttribute((const void*)project_x3_kernel<PREC, NB>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     const dim3 grid((unsigned)((W + kPxS - 1) / kPxS), (C + kPxC - 1) / kPxC);
-    hipLaunchKernelGGL((project_x3_kernel<PREC>), grid, dim3(64 * K), lds, s, x_wc, ldx, W, C, wpca, K, nt, xp, ldp,
-                       J);
+    hipLaunchKernelGGL((project_x3_kernel<PREC, NB>), grid, dim3(64 * K), lds, s, x_wc, ldx, W, C, wpca, K, nt, xp,
+                       ldp, J);
     return hipGetLastError();
 }
 
+template <int PREC>
+static hipError_t project_x3_tile(const float* x_wc, int64_t ldx, int64_t W, int C, const float* wpca, int K, int nt,
+                                  float* xp, int64_t ldp, hipStream_t s, int tile) {
+    switch (tile) {
+        case 0:
+        case 2: return project_x3_launch<PREC, 2>(x_wc, ldx, W, C, wpca, K, nt, xp, ldp, s);
+        case 1: return project_x3_launch<PREC, 1>(x_wc, ldx, W, C, wpca, K, nt, xp, ldp, s);
+        case 3: return project_x3_launch<PREC, 4>(x_wc, ldx, W, C, wpca, K, nt, xp, ldp, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+// tile: 0 or 2 = 64 channels per workgroup, 1 = 32, 3 = 128 (per-element sums identical)
 hipError_t launch_project_x3(int prec, const float* x_wc, int64_t ldx, int64_t W, int C, const float* wpca, int K,
-                             int nt, float* xp, int64_t ldp, hipStream_t s) {
+                             int nt, float* xp, int64_t ldp, hipStream_t s, int tile) {
     if (K < 1 || K > kMaxK || nt < 1 || nt > kMaxNt) return hipErrorInvalidValue;
     if (prec == LSS_PREC_BF16X3)
-        return project_x3_launch<LSS_PREC_BF16X3>(x_wc, ldx, W, C, wpca, K, nt, xp, ldp, s);
+        return project_x3_tile<LSS_PREC_BF16X3>(x_wc, ldx, W, C, wpca, K, nt, xp, ldp, s, tile);
     if (prec == LSS_PREC_FP16X3)
-        return project_x3_launch<LSS_PREC_FP16X3>(x_wc, ldx, W, C, wpca, K, nt, xp, ldp, s);
+        return project_x3_tile<LSS_PREC_FP16X3>(x_wc, ldx, W, C, wpca, K, nt, xp, ldp, s, tile);
     return hipErrorInvalidValue;
 }
 

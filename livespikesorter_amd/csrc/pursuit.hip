@@ -40,6 +40,25 @@ __device__ __forceinline__ void wave_argmax(float& v, int& i) {
     }
 }
 
+// The same with the winner's raw score b carried along (the persistent pursuit publishes each
+// rescored column's amplitude b / nm[i] beside its Cfmax/imax).
+__device__ __forceinline__ void argmax_merge_b(float& v, int& i, float& b, float& m, float v2, int i2, float b2,
+                                               float m2) {
+    if (v2 > v || (v2 == v && i2 < i)) { v = v2; i = i2; b = b2; m = m2; }
+}
+
+// b = the winner's raw score B[w][i], m = its nm[i]
+__device__ __forceinline__ void wave_argmax_b(float& v, int& i, float& b, float& m) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        float v2 = __shfl_xor(v, off, 64);
+        int i2 = __shfl_xor(i, off, 64);
+        float b2 = __shfl_xor(b, off, 64);
+        float m2 = __shfl_xor(m, off, 64);
+        argmax_merge_b(v, i, b, m, v2, i2, b2, m2);
+    }
+}
+
 __device__ __forceinline__ float score_of(float b, float nmv) {
     float r = fmaxf(b, 0.f);
     return (r * r) / nmv;
@@ -52,13 +71,16 @@ __device__ __forceinline__ V ld_dev(const V* p) { return __hip_atomic_load(p, __
 template <typename V>
 __device__ __forceinline__ void st_dev(V* p, V v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
-// Cfmax/imax of column w: plain stores between kernels, device-scope stores inside the persistent
-// pursuit (DEV)
+// Cfmax/imax of column w: plain stores between kernels; inside the persistent pursuit (DEV)
+// device-scope stores of Cfmax, imax and the amplitude b / m the column would have as a peak
+// (b = B[w][i], m = nm[i]: the expression the per-pass compaction evaluates)
 template <bool DEV>
-__device__ __forceinline__ void store_score(const PursuitArgs& a, int64_t w, float v, int i) {
+__device__ __forceinline__ void store_score(const PursuitArgs& a, int64_t w, float v, int i, float b = 0.f,
+                                            float m = 1.f) {
     if constexpr (DEV) {
         st_dev(a.cfmax + w, v);
         st_dev(a.imax + w, i);
+        st_dev(a.amprow + w, b / m);
     } else {
         a.cfmax[w] = v;
         a.imax[w] = i;
@@ -70,7 +92,7 @@ __device__ __forceinline__ void store_score(const PursuitArgs& a, int64_t w, flo
 // ------------------------------------------------------------------------------------------------
 template <int NV, bool DEV = false>   // float4 vectors per lane: NV*256 >= T
 __device__ __forceinline__ void score_column(const PursuitArgs& a, int64_t w, int lane) {
-    float best = 0.f;
+    float best = 0.f, bb = 0.f, bm = 1.f;
     int bi = 0;
     if (w >= a.nt && w < a.W - a.nt) {
         best = -1.f;
@@ -90,11 +112,16 @@ __device__ __forceinline__ void score_column(const PursuitArgs& a, int64_t w, in
             const int t = 4 * lane + 256 * i;
 #pragma unroll
             for (int e = 0; e < 4; ++e)
-                if (t + e < a.T) argmax_merge(best, bi, score_of(b[i][e], n[i][e]), t + e);
+                if (t + e < a.T) {
+                    if constexpr (DEV)
+                        argmax_merge_b(best, bi, bb, bm, score_of(b[i][e], n[i][e]), t + e, b[i][e], n[i][e]);
+                    else argmax_merge(best, bi, score_of(b[i][e], n[i][e]), t + e);
+                }
         }
-        wave_argmax(best, bi);
+        if constexpr (DEV) wave_argmax_b(best, bi, bb, bm);
+        else wave_argmax(best, bi);
     }
-    if (lane == 0) store_score<DEV>(a, w, best, bi);
+    if (lane == 0) store_score<DEV>(a, w, best, bi, bb, bm);
 }
 
 template <int NV>
@@ -142,27 +169,10 @@ __device__ __forceinline__ bool peak_test(const float* p, float th2, int nt) {
     return cm > th2 && fabsf(cm - cf) < 1e-9f;
 }
 
-// Branch-free form of the same test for a whole row held in LDS: mx[i] = max(src[i .. i+S-1]) by
-// log2(S) doubling steps (S = the largest power of two <= 2nt+1, so two such windows cover
-// [w-nt, w+nt]); pool is then exact and the reference's literal test applies.  NT threads, each
-// owning MAXR entries; mx holds NT*MAXR + S/2 values whose tail past the row is -inf, so every
-// read is in bounds and none is conditional.
-template <int NT, int MAXR>
-__device__ __forceinline__ void window_max_pow2(float* mx, int S) {
-    for (int s = 1; s < S; s <<= 1) {
-        float t[MAXR];
-#pragma unroll
-        for (int r = 0; r < MAXR; ++r) {
-            const int i = threadIdx.x + NT * r;
-            t[r] = fmaxf(mx[i], mx[i + s]);
-        }
-        __syncthreads();
-#pragma unroll
-        for (int r = 0; r < MAXR; ++r) mx[threadIdx.x + NT * r] = t[r];
-        __syncthreads();
-    }
-}
-
+// Branch-free form of the same test for a whole row held in LDS (the persistent pursuit):
+// mx[i] = max(row[i .. i+S-1]) by log2(S) doubling steps (S = the largest power of two <= 2nt+1,
+// so two such windows cover [w-nt, w+nt]); pool is then exact and the reference's literal test
+// applies.
 __device__ __forceinline__ int pow2_window(int nt) {
     int S = 1;
     while (2 * S <= 2 * nt + 1) S *= 2;
@@ -338,17 +348,19 @@ __device__ __forceinline__ void covering(const L& l, int s, int n, int c1, int c
     }
 }
 
+// Columns wa and wb of this wavefront (va / vb: updated at all; sa / sb: the first spike of the
+// pass whose window covers the column, with ca1, ca2 / cb1, cb2 the columns of the next two
+// spikes): B -= amp ctc for the last even and last odd covering spike, then the column rescored.
 template <int NV, typename List, bool DEV>   // float4 vectors per lane over templates: NV*256 >= T
-__device__ __forceinline__ void update_score_columns(const PursuitArgs& a, const List& l, int s, int n, int c1, int c2,
-                                                     int c0, int lo, int w1, const floatx4* nmv) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+__device__ __forceinline__ void update_two(const PursuitArgs& a, const List& l, int n, int wa, int sa, int ca1,
+                                           int ca2, bool va, int wb, int sb, int cb1, int cb2, bool vb,
+                                           const floatx4* nmv) {
+    const int lane = threadIdx.x & 63;
     const int nt = a.nt, L = 2 * nt + 1;
     // two columns per wavefront, loads of both issued before any use
-    const int wa = c0 + 2 * wave, wb = wa + 1;
-    const bool va = wa >= lo && wa <= w1, vb = wb >= lo && wb <= w1;
-    int ea, oa, eb, ob;
-    covering(l, s, n, c1, c2, wa, nt, ea, oa);
-    covering(l, s, n, c1, c2, wb, nt, eb, ob);
+    int ea = -1, oa = -1, eb = -1, ob = -1;
+    if (va) covering(l, sa, n, ca1, ca2, wa, nt, ea, oa);
+    if (vb) covering(l, sb, n, cb1, cb2, wb, nt, eb, ob);
     floatx4 ba[NV], bb[NV], cea[NV], coa[NV], ceb[NV], cob[NV];
     float* pa = a.B + (int64_t)wa * a.ldT;
     float* pb = a.B + (int64_t)wb * a.ldT;
@@ -376,7 +388,7 @@ __device__ __forceinline__ void update_score_columns(const PursuitArgs& a, const
     const float aeb = (eb >= 0) ? l.a(eb) : 0.f, aob = (ob >= 0) ? l.a(ob) : 0.f;
     const bool ra = va && wa >= nt && wa < a.W - nt;
     const bool rb = vb && wb >= nt && wb < a.W - nt;
-    float besta = -1.f, bestb = -1.f;
+    float besta = -1.f, bestb = -1.f, ba_ = 0.f, bb_ = 0.f, ma_ = 1.f, mb_ = 1.f;
     int ia = 0x7fffffff, ib = 0x7fffffff;
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
@@ -390,7 +402,11 @@ __device__ __forceinline__ void update_score_columns(const PursuitArgs& a, const
                     if (qea) v = v - aea * cea[i][e];
                     if (qoa) v = v - aoa * coa[i][e];
                     x[e] = v;
-                    if (ra && t + e < a.T) argmax_merge(besta, ia, score_of(v, nmv[i][e]), t + e);
+                    if (ra && t + e < a.T) {
+                        if constexpr (DEV)
+                            argmax_merge_b(besta, ia, ba_, ma_, score_of(v, nmv[i][e]), t + e, v, nmv[i][e]);
+                        else argmax_merge(besta, ia, score_of(v, nmv[i][e]), t + e);
+                    }
                 }
                 *(floatx4*)(pa + t) = x;
             }
@@ -402,20 +418,35 @@ __device__ __forceinline__ void update_score_columns(const PursuitArgs& a, const
                     if (qeb) v = v - aeb * ceb[i][e];
                     if (qob) v = v - aob * cob[i][e];
                     x[e] = v;
-                    if (rb && t + e < a.T) argmax_merge(bestb, ib, score_of(v, nmv[i][e]), t + e);
+                    if (rb && t + e < a.T) {
+                        if constexpr (DEV)
+                            argmax_merge_b(bestb, ib, bb_, mb_, score_of(v, nmv[i][e]), t + e, v, nmv[i][e]);
+                        else argmax_merge(bestb, ib, score_of(v, nmv[i][e]), t + e);
+                    }
                 }
                 *(floatx4*)(pb + t) = x;
             }
         }
     }
     if (ra) {
-        wave_argmax(besta, ia);
-        if (lane == 0) store_score<DEV>(a, wa, besta, ia);
+        if constexpr (DEV) wave_argmax_b(besta, ia, ba_, ma_);
+        else wave_argmax(besta, ia);
+        if (lane == 0) store_score<DEV>(a, wa, besta, ia, ba_, ma_);
     }
     if (rb) {
-        wave_argmax(bestb, ib);
-        if (lane == 0) store_score<DEV>(a, wb, bestb, ib);
+        if constexpr (DEV) wave_argmax_b(bestb, ib, bb_, mb_);
+        else wave_argmax(bestb, ib);
+        if (lane == 0) store_score<DEV>(a, wb, bestb, ib, bb_, mb_);
     }
+}
+
+// Per-pass update: the chunk [c0, c0 + kColsB) of the columns [lo, w1] whose first covering spike is s
+// (c1, c2 = the next two spikes' columns), two columns per wavefront.
+template <int NV, typename List, bool DEV>
+__device__ __forceinline__ void update_score_columns(const PursuitArgs& a, const List& l, int s, int n, int c1, int c2,
+                                                     int c0, int lo, int w1, const floatx4* nmv) {
+    const int wa = c0 + 2 * (int)(threadIdx.x >> 6), wb = wa + 1;
+    update_two<NV, List, DEV>(a, l, n, wa, s, c1, c2, wa >= lo && wa <= w1, wb, s, c1, c2, wb >= lo && wb <= w1, nmv);
 }
 
 template <int NV>
@@ -452,34 +483,6 @@ __device__ __forceinline__ void update_items(const PursuitArgs& a, const ListPla
     }
 }
 
-// The columns [a0, a1) a workgroup owns: for each spike whose columns (those its window covers and
-// the previous spike's does not) meet them, that intersection in chunks of kColsB.  The same
-// columns are written as by update_items, by the same arithmetic.
-template <int NV, typename List>
-__device__ __forceinline__ void update_owned(const PursuitArgs& a, const List& l, int n, int a0, int a1,
-                                             const floatx4* nmv) {
-    const int nt = a.nt;
-    int lo_s = 0, hi_s = n;   // first spike with col + nt >= a0
-    while (lo_s < hi_s) {
-        const int mid = (lo_s + hi_s) >> 1;
-        if (l.c(mid) + nt < a0) lo_s = mid + 1;
-        else hi_s = mid;
-    }
-    for (int s = lo_s; s < n; ++s) {
-        const int ws = l.c(s);
-        if (ws - nt >= a1) break;
-        const int cp = (s > 0) ? l.c(s - 1) : INT_MIN / 2;
-        const int c1 = (s + 1 < n) ? l.c(s + 1) : INT_MAX / 2;
-        const int c2 = (s + 2 < n) ? l.c(s + 2) : INT_MAX / 2;
-        int lo = ws - nt;
-        if (s > 0) lo = max(lo, cp + nt + 1);
-        lo = max(max(lo, 0), a0);
-        const int hi = min(min(ws + nt, (int)a.W - 1), a1 - 1);
-        for (int c0 = lo; c0 <= hi; c0 += kColsB)
-            update_score_columns<NV, List, true>(a, l, s, n, c1, c2, c0, c0, min(c0 + kColsB - 1, hi), nmv);
-    }
-}
-
 template <int NV>
 __global__ __launch_bounds__(256) void update_kernel(PursuitArgs a) {
     const int base = a.counters[0];
@@ -493,30 +496,34 @@ __global__ __launch_bounds__(256) void update_kernel(PursuitArgs a) {
 
 // ------------------------------------------------------------------------------------------------
 // persistent pursuit for short windows (W <= kPersistMaxW, the streaming batches): the first
-// scoring and every pass in one launch of G co-resident workgroups, two grid barriers per pass.
+// scoring and every pass in one launch of G co-resident 512-thread workgroups, ONE grid barrier
+// per pass.
 //
 // Workgroup g owns the columns [a0, a1) (a fixed partition): it alone scores them, reads their B
 // rows and updates them, so B stays in the caches of one XCD and needs no coherence traffic.  What
-// workgroups exchange - Cfmax/imax, the pass's amplitudes and stored spike list - goes through
-// device-scope loads and stores only, so the grid barrier is a plain arrival count with no cache
-// write-back or invalidate.
+// workgroups exchange goes through device-scope loads and stores only (no cache write-back or
+// invalidate at the barrier): per column its Cfmax, imax and the amplitude B[w][imax]/nm[imax] it
+// would have as a peak, in one of two rows chosen by pass parity.
 //
-// After barrier A every workgroup loads the whole Cfmax/imax row into LDS, forms the exact window
-// max by doubling (window_max_pow2), applies the reference's test and compacts in ascending order
-// as peaks_flags/peaks_emit do, so each holds the pass's columns and templates without exchanging
-// them; the owner of a peak's column publishes its amplitude, workgroup 0 stores the list.  Barrier
-// B keeps every read of the pass ahead of the update (the update of a spike zeroes its own score)
-// and makes the amplitudes visible.  Every workgroup sees the same scores, so all take the same
-// branches and leave the loop on the same pass.
+// Pass p: every workgroup loads row p&1 (Cfmax, imax, amplitude of every column) into LDS, forms
+// the exact window max by doubling (window_max_pow2), applies the reference's test and compacts
+// the peaks in ascending order, so each holds the pass's list (columns, templates, amplitudes)
+// without exchanging it; workgroup 0 stores the list.  Each then updates its own columns' B rows
+// and publishes the rescored columns into row (p+1)&1, copying the values of its columns the pass
+// leaves unchanged; the barrier ends the pass.  A workgroup writes row (p+1)&1 while others may
+// still read row p&1, never the row they read, so the one barrier per pass orders every read of a
+// pass before the writes that overwrite it.  Every workgroup sees the same row, so all take the
+// same branches and leave the loop on the same pass.
 //
 // The barrier waits a bounded time: on a timeout (workgroups not co-resident) the kernel sets
 // counters[7], every workgroup leaves, and the host reports the error; the launcher caps G at the
 // occupancy limit so that this does not happen in normal operation.
 // ------------------------------------------------------------------------------------------------
-constexpr int kPersistList = 512;
-constexpr int kBarrierSpin = 1 << 20;   // ~1 s of polling
-constexpr int kPersistLoads = (kPersistMaxW + 2 * kPeakHalo + 255) / 256;   // row values per thread
-constexpr int kPersistMx = 256 * kPersistLoads + kPeakHalo;                   // window-max buffer
+constexpr int kPT = 512;                                // threads per persistent workgroup
+constexpr int kPW = kPT / 64;                           // its wavefronts
+constexpr int kBarrierSpin = 1 << 20;                   // ~1 s of polling
+constexpr int kPersistLoads = (kPersistMaxW + 2 * kPeakHalo + kPT - 1) / kPT;   // row values per thread
+constexpr int kPersistMx = kPT * kPersistLoads + kPeakHalo;                      // window-max buffer
 
 // Grid barrier number `gen` (1, 2, ...) over counters[6] (zero at launch).  Data shared between
 // workgroups is only accessed with device-scope atomics, so waiting for this wavefront's memory
@@ -545,68 +552,155 @@ __device__ __forceinline__ bool grid_barrier(int* counters, int gen, int* s_ok) 
     return *s_ok != 0;
 }
 
+// the row of pass parity `par`
+__device__ __forceinline__ void use_row(PursuitArgs& b, const PursuitArgs& a, int par) {
+    b.cfmax = a.prow_cf + (int64_t)par * kPersistMaxW;
+    b.imax = a.prow_im + (int64_t)par * kPersistMaxW;
+    b.amprow = a.prow_amp + (int64_t)par * kPersistMaxW;
+}
+
+// First spike of an ascending list whose +-nt window reaches column w (n if none does).
+__device__ __forceinline__ int first_cover(const ListPlain& l, int n, int w, int nt) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (l.col[mid] + nt < w) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// The persistent pursuit's update of its own columns [a0, a1): rounds of 2 * kPW columns, two per
+// wavefront, every load of a round in flight together.  A column a stored spike's window covers
+// gets the update update_items gives it (the last even and last odd covering spike) and, inside
+// [nt, W - nt), is rescored into the next row; every other column carries its values (cf, im, am:
+// the pass's row in LDS, cf indexed by column) into the next row.
 template <int NV>
-__global__ __launch_bounds__(256) void pursuit_persistent_kernel(PursuitArgs a, int first_pass, int max_passes) {
-    extern __shared__ float psm[];   // cf [W + 2H] | mx [kPersistMx] | im [W] | list col, tmp, amp [kPersistList]
-    __shared__ int wsum[4];
+__device__ __forceinline__ void update_own_columns(const PursuitArgs& a, const ListPlain& l, int n, int a0, int a1,
+                                                   const float* cf, const int* im, const float* am,
+                                                   const floatx4* nmv) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nt = a.nt, W = (int)a.W;
+    for (int c0 = a0; c0 < a1; c0 += 2 * kPW) {
+        const int wa = c0 + 2 * wave, wb = wa + 1;
+        const int sa = first_cover(l, n, wa, nt), sb = first_cover(l, n, wb, nt);
+        const bool va = wa < a1 && sa < n && l.col[sa] - nt <= wa;
+        const bool vb = wb < a1 && sb < n && l.col[sb] - nt <= wb;
+        const int ca1 = (sa + 1 < n) ? l.col[sa + 1] : INT_MAX / 2, ca2 = (sa + 2 < n) ? l.col[sa + 2] : INT_MAX / 2;
+        const int cb1 = (sb + 1 < n) ? l.col[sb + 1] : INT_MAX / 2, cb2 = (sb + 2 < n) ? l.col[sb + 2] : INT_MAX / 2;
+        update_two<NV, ListPlain, true>(a, l, n, wa, sa, ca1, ca2, va, wb, sb, cb1, cb2, vb, nmv);
+        if (lane == 0) {
+            if (wa < a1 && !(va && wa >= nt && wa < W - nt)) {
+                st_dev(a.cfmax + wa, cf[wa]);
+                st_dev(a.imax + wa, im[wa]);
+                st_dev(a.amprow + wa, am[wa]);
+            }
+            if (wb < a1 && !(vb && wb >= nt && wb < W - nt)) {
+                st_dev(a.cfmax + wb, cf[wb]);
+                st_dev(a.imax + wb, im[wb]);
+                st_dev(a.amprow + wb, am[wb]);
+            }
+        }
+    }
+}
+
+// window_max_pow2 with two buffers (one barrier per step): src holds the row with a -inf tail of
+// kPeakHalo values past NT * MAXR, tmp the same tail; returns the buffer holding the result.
+template <int NT, int MAXR>
+__device__ __forceinline__ float* window_max_pow2_pp(float* src, float* tmp, int S) {
+    float* dst = tmp;
+    for (int s = 1; s < S; s <<= 1) {
+#pragma unroll
+        for (int r = 0; r < MAXR; ++r) {
+            const int i = threadIdx.x + NT * r;
+            dst[i] = fmaxf(src[i], src[i + s]);
+        }
+        __syncthreads();
+        float* t = src;
+        src = dst;
+        dst = t;
+    }
+    return src;
+}
+
+template <int NV>
+__global__ __launch_bounds__(kPT) void pursuit_persistent_kernel(PursuitArgs a, int first_pass, int max_passes) {
+    // cf [W + 2H] | mx, mx2 [kPersistMx] | im [W] | am [W] | list col, tmp, amp [W]
+    extern __shared__ float psm[];
+    __shared__ int wsum[kPW];
     __shared__ int s_ok;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int W = (int)a.W, H = kPeakHalo;
     const int N = W + 2 * H;
     float* cf = psm;
     float* mx = cf + N;
-    int* im = (int*)(mx + kPersistMx);
-    int* lcol = im + W;
-    int* ltmp = lcol + kPersistList;
-    float* lamp = (float*)(ltmp + kPersistList);
+    float* mx2 = mx + kPersistMx;
+    int* im = (int*)(mx2 + kPersistMx);
+    float* am = (float*)(im + W);
+    int* lcol = (int*)(am + W);
+    int* ltmp = lcol + W;
+    float* lamp = (float*)(ltmp + W);
     const bool lead = blockIdx.x == 0 && tid == 0;
     // this workgroup's columns: kColsB-aligned runs, G of them cover [0, W)
     const int Wc = (W + (int)gridDim.x * kColsB - 1) / ((int)gridDim.x * kColsB) * kColsB;
     const int a0 = min(W, (int)blockIdx.x * Wc), a1 = min(W, a0 + Wc);
     floatx4 nmv[NV];
     load_norms<NV>(a, nmv);
+    int par = first_pass & 1;
+    PursuitArgs cur = a;
+    use_row(cur, a, par);
     int base = 0, np = 0;
     bool done = false;
     if (first_pass == 0) {
         // first pass: own columns scored, one wavefront per column
-        for (int w = a0 + wave; w < a1; w += 4) score_column<NV, true>(a, w, lane);
+        for (int w = a0 + wave; w < a1; w += kPW) score_column<NV, true>(cur, w, lane);
     } else {
-        // after first_pass passes of the per-pass kernels: commit the last one as peaks_emit does
+        // after first_pass passes of the per-pass kernels (their last pass committed as the peak
+        // kernel commits it): own columns' Cfmax/imax as they left them, with their amplitudes
         base = a.counters[0] + a.counters[1];
         np = a.counters[4];
         done = a.counters[3] != 0;
+        for (int w = a0 + tid; w < a1; w += kPT) {
+            const int i = a.imax[w];
+            store_score<true>(cur, w, a.cfmax[w], i, a.B[(int64_t)w * a.ldT + i], a.nm[i]);
+        }
     }
+    if (tid < kPeakHalo) mx2[kPT * kPersistLoads + tid] = -INFINITY;   // tail of the second buffer
     int gen = 1;
     if (!grid_barrier(a.counters, gen++, &s_ok)) return;
-    const int R = ((W + 255) / 256) | 1;   // columns per thread, odd: conflict-free LDS strides
+    const int R = ((W + kPT - 1) / kPT) | 1;   // columns per thread, odd: conflict-free LDS strides
     const int S = pow2_window(a.nt);
     for (int pass = first_pass; pass < max_passes && !done; ++pass) {
         {
-            // the Cfmax/imax row (with -inf halos) into LDS, every load in flight at once
-            float v[kPersistLoads];
+            // the row (with -inf halos) into LDS, every load in flight at once
+            float v[kPersistLoads], b[kPersistLoads];
             int m[kPersistLoads];
 #pragma unroll
             for (int r = 0; r < kPersistLoads; ++r) {
-                const int w = tid + 256 * r - H;
+                const int w = tid + kPT * r - H;
                 const bool in = w >= 0 && w < W;
-                v[r] = in ? ld_dev(a.cfmax + w) : -INFINITY;
-                m[r] = in ? ld_dev(a.imax + w) : 0;
+                v[r] = in ? ld_dev(cur.cfmax + w) : -INFINITY;
+                m[r] = in ? ld_dev(cur.imax + w) : 0;
+                b[r] = in ? ld_dev(cur.amprow + w) : 0.f;
             }
 #pragma unroll
             for (int r = 0; r < kPersistLoads; ++r) {
-                const int i = tid + 256 * r, w = i - H;
+                const int i = tid + kPT * r, w = i - H;
                 if (i < N) cf[i] = v[r];
                 mx[i] = v[r];   // -inf past the row
-                if (w >= 0 && w < W) im[w] = m[r];
+                if (w >= 0 && w < W) {
+                    im[w] = m[r];
+                    am[w] = b[r];
+                }
             }
-            if (tid < kPeakHalo) mx[256 * kPersistLoads + tid] = -INFINITY;
+            if (tid < kPeakHalo) mx[kPT * kPersistLoads + tid] = -INFINITY;
         }
         __syncthreads();
-        window_max_pow2<256, kPersistLoads>(mx, S);
+        const float* wmx = window_max_pow2_pp<kPT, kPersistLoads>(mx, mx2, S);
         uint32_t bits = 0;
         for (int q = 0; q < R; ++q) {
             const int w = tid * R + q;
-            const bool pk = peak_pooled(cf, mx, H + min(w, W - 1), a.nt, S, a.th2);
+            const bool pk = peak_pooled(cf, wmx, H + min(w, W - 1), a.nt, S, a.th2);
             if (pk && w < W) bits |= 1u << q;
         }
         const int cnt = __popc(bits);
@@ -620,7 +714,7 @@ __global__ __launch_bounds__(256) void pursuit_persistent_kernel(PursuitArgs a, 
         __syncthreads();
         int woff = 0, n = 0;
 #pragma unroll
-        for (int k = 0; k < 4; ++k) {
+        for (int k = 0; k < kPW; ++k) {
             const int vv = wsum[k];
             if (k < wave) woff += vv;
             n += vv;
@@ -631,13 +725,14 @@ __global__ __launch_bounds__(256) void pursuit_persistent_kernel(PursuitArgs a, 
             bits &= bits - 1;
             const int w = tid * R + q;
             const int t = im[w];
-            if (pos < kPersistList) { lcol[pos] = w; ltmp[pos] = t; }
-            if (base + pos < a.cap) {
-                if (w >= a0 && w < a1) st_dev(a.sp_amp + base + pos, a.B[(int64_t)w * a.ldT + t] / a.nm[t]);
-                if (blockIdx.x == 0) {
-                    st_dev(a.sp_col + base + pos, w);
-                    st_dev(a.sp_tmp + base + pos, t);
-                }
+            const float amp = am[w];
+            lcol[pos] = w;
+            ltmp[pos] = t;
+            lamp[pos] = amp;
+            if (blockIdx.x == 0 && base + pos < a.cap) {
+                st_dev(a.sp_col + base + pos, w);
+                st_dev(a.sp_tmp + base + pos, t);
+                st_dev(a.sp_amp + base + pos, amp);
             }
             ++pos;
         }
@@ -651,18 +746,14 @@ __global__ __launch_bounds__(256) void pursuit_persistent_kernel(PursuitArgs a, 
             a.pass_off[np + 1] = min(base + n, a.cap);
         }
         ++np;
-        // every workgroup has read this pass's Cfmax/imax before any update writes them, and the
-        // amplitudes and the stored list are complete
-        if (!grid_barrier(a.counters, gen++, &s_ok)) return;
-        const int nu = min(n, a.cap - base);   // spikes stored, as update_kernel applies them
-        if (nu > kPersistList) {
-            update_owned<NV>(a, ListDev{a.sp_col + base, a.sp_tmp + base, a.sp_amp + base}, nu, a0, a1, nmv);
-        } else {
-            for (int i = tid; i < nu; i += 256) lamp[i] = ld_dev(a.sp_amp + base + i);
-            __syncthreads();
-            update_owned<NV>(a, ListPlain{lcol, ltmp, lamp}, nu, a0, a1, nmv);
-        }
+        __syncthreads();   // the pass's list is complete in LDS
+        const int nu = max(0, min(n, a.cap - base));   // spikes stored, as update_kernel applies them
+        PursuitArgs nxt = a;
+        use_row(nxt, a, par ^ 1);
+        update_own_columns<NV>(nxt, ListPlain{lcol, ltmp, lamp}, nu, a0, a1, cf + H, im, am, nmv);
         base += n;
+        par ^= 1;
+        cur = nxt;
         if (pass + 1 < max_passes && !grid_barrier(a.counters, gen++, &s_ok)) return;
     }
     if (lead) {
@@ -829,8 +920,8 @@ hipError_t launch_update(const PursuitArgs& a, int grid, hipStream_t s) {
 }
 
 size_t persistent_lds(int64_t W) {
-    return sizeof(float) * (W + 2 * kPeakHalo + kPersistMx) + sizeof(int) * W +
-           (2 * sizeof(int) + sizeof(float)) * kPersistList;
+    return sizeof(float) * (W + 2 * kPeakHalo + 2 * kPersistMx) + (sizeof(int) + sizeof(float)) * W +
+           (2 * sizeof(int) + sizeof(float)) * W;
 }
 
 typedef void (*PersistentFn)(PursuitArgs, int, int);
@@ -850,7 +941,9 @@ hipError_t persistent_max_grid(int T, int64_t W, int* grid) {
     int dev = 0, cus = 0, per = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)fn, 256, persistent_lds(W));
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)persistent_lds(W));
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)fn, kPT, persistent_lds(W));
     if (e != hipSuccess) return e;
     *grid = per * cus;
     return hipSuccess;
@@ -858,13 +951,14 @@ hipError_t persistent_max_grid(int T, int64_t W, int* grid) {
 
 hipError_t launch_pursuit_persistent(const PursuitArgs& a, int grid, int first_pass, int max_passes, hipStream_t s) {
     const PersistentFn fn = persistent_fn(a.T);
-    if (!fn || a.W > kPersistMaxW || a.nt > kPeakHalo || grid < 1 || first_pass < 0 || first_pass >= max_passes)
+    if (!fn || a.W > kPersistMaxW || a.nt > kPeakHalo || grid < 1 || first_pass < 0 || first_pass >= max_passes ||
+        !a.prow_cf || !a.prow_im || !a.prow_amp)
         return hipErrorInvalidValue;
     int cap = 0;
     hipError_t e = persistent_max_grid(a.T, a.W, &cap);
     if (e != hipSuccess) return e;
     if (cap < 1) return hipErrorInvalidConfiguration;
-    hipLaunchKernelGGL(fn, dim3(std::min(grid, cap)), dim3(256), persistent_lds(a.W), s, a, first_pass, max_passes);
+    hipLaunchKernelGGL(fn, dim3(std::min(grid, cap)), dim3(kPT), persistent_lds(a.W), s, a, first_pass, max_passes);
     return hipGetLastError();
 }
 

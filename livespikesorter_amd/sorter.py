@@ -229,6 +229,27 @@ class Sorter:
         check(lib().lss_last_timings(self._h, ms.ctypes.data, 8), "lss_last_timings")
         return dict(zip(TIMING_KEYS, ms.tolist()))
 
+    def set_tiles(self, whiten: int, project: int, scores: int):
+        """Tile shapes of the whitening GEMM, wPCA projection and score GEMM (0..3; identical results)."""
+        check(lib().lss_set_tiles(self._h, int(whiten), int(project), int(scores)), "lss_set_tiles")
+
+    def tiles(self):
+        t = np.zeros(3, np.int32)
+        check(lib().lss_get_tiles(self._h, t.ctypes.data), "lss_get_tiles")
+        return tuple(int(x) for x in t)
+
+    def autotune_tiles(self, window_dev, reps: int = 3):
+        """Times every tile shape of the three contractions on a device window and keeps the fastest;
+        returns ((whiten, project, scores), {stage: [ms per shape]})."""
+        ms = np.zeros(12, np.float32)
+        best = np.zeros(3, np.int32)
+        _ready()
+        check(lib().lss_autotune_tiles(self._h, _ptr(window_dev), window_dev.shape[0], int(reps), ms.ctypes.data,
+                                       best.ctypes.data), "lss_autotune_tiles")
+        names = ["whiten", "project", "scores"]
+        return tuple(int(x) for x in best), {names[i]: [round(float(v), 4) for v in ms[4 * i:4 * i + 4]]
+                                             for i in range(3)}
+
     def set_persistent(self, grid: int, first_passes: int = 0):
         """Pursuit of windows <= 4096 samples: the first `first_passes` passes on the per-pass kernels,
         the rest in the persistent kernel of `grid` workgroups (grid 0 = per-pass kernels only)."""
