@@ -1,23 +1,19 @@
 """Benchmark of the online Kilosort4 template-matching path on one MI355X.
 
-Workload (the drop-in configuration at full probe size): 384 channels, 600 templates, 6 PCs,
-nt = 61, 10 s of seeded synthetic int16 at 30 kHz cut into Kilosort batches of NT = 60000 samples
-(padded windows of 60122 samples), at most 50 pursuit passes per window.  One step = one padded
-window through the whole device path (CAR, highpass, whitening, projection, U.Xproj, pursuit,
-residual, PC features, positions, spike-list readback into host arrays) with the int16 windows
-already resident in HBM; the benchmark keeps each window's sample/template/amplitude for its
-checks and drops the features and positions after the call, as a consumer writing them out would.
+Default workload (config 5, the metric's batch-latency case): four probes of uneven size (384 ch /
+600 templates, 384/250, 128/120, 64/40) at 30 kHz, 60 s each, cut into 3000-sample (100 ms)
+batches with 61-sample halos; one step = one graph-replayed batch step over all four probes
+(upload-free device samples, CAR, highpass, whitening, projection, U.Xproj, pursuit, residual, PC
+features, positions, spike-list readback), bf16x3 GEMMs, tiles and pursuit schedules autotuned
+beforehand, a template-refinement round every 10 s.  `value` is the aggregate x-realtime of the
+`--steps` timed steps after `--warmup` untimed ones (all remaining batches of the 60 s by
+default); p50/p99 are per 100-ms batch step.  Beside it: the same 60 s stream to its end with
+the refinement rounds, the plain FP32 path on the same windows (one probe at a time), the same
+stream with host samples (PCIe in every step), the pursuit update kernel's roofline (HIP events
+around each launch on the library's stream), the FP32 CPU oracle on a sample of the same windows,
+and the config-2 figures (384 ch / 600 templates, 2-s Kilosort windows) as `window_2s`.
 
-`value` is the bf16x3 path (whitening and U.Xproj on BF16 MFMA with the split correction, the
-pursuit in FP32) over the recording with --in-flight windows (default 3) sorted concurrently by as
-many handles, one host thread each (one window's GEMMs overlap another's pursuit); the
-one-at-a-time rate and its p50/p99 per-window latency (the live per-batch case) are reported beside
-it.  The plain FP32 path runs the same windows (`compare`) with the spike F1 between the two.
-
-Prints one JSON line: x-realtime (seconds of recording per wall second), p50/p99 per-window
-latency, the roofline figure of the dominant kernel (the pursuit update kernel, timed per launch
-with HIP events on the library's stream in a separate one-at-a-time run) and the CPU oracle timed
-on a bounded sample on the host.
+Prints one JSON line.
 """
 from __future__ import annotations
 
@@ -58,6 +54,7 @@ def run_path(sorter, dwin, starts, steps, warmup, profile=False):
         sorter.profile_update(True)
     lat, stage, outs = [], [], []
     upd_ms, upd_n = 0.0, 0
+    upd_each, upd_gpu, upd_spikes = [], [], []
     t_start = time.perf_counter()
     for i in range(steps):
         t1 = time.perf_counter()
@@ -69,21 +66,37 @@ def run_path(sorter, dwin, starts, steps, warmup, profile=False):
             ms, n = sorter.update_stats()
             upd_ms += ms
             upd_n += n
+            e, g, sp = sorter.update_times()
+            upd_each.append(e)
+            upd_gpu.append(g)
+            upd_spikes.append(sp)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
     if profile:
         sorter.profile_update(False)
     return dict(elapsed=elapsed, lat_ms=np.array(lat) * 1e3,
                 stage_ms={k: float(np.mean([d[k] for d in stage])) for k in stage[0]},
-                outs=outs, upd_ms=upd_ms, upd_n=upd_n)
+                outs=outs, upd_ms=upd_ms, upd_n=upd_n, upd_each=upd_each, upd_gpu=upd_gpu, upd_spikes=upd_spikes)
 
 
-def run_in_flight(a, sorter, dwin, starts, steps, warmup, k, precision, Wp, seq_outs):
-    """The same windows with k in flight (the sorter plus k-1 more handles, one host thread each);
-    checks the spike lists against the one-at-a-time run."""
+def update_columns_of(sorter):
+    from livespikesorter_amd.streaming import update_columns
+    return update_columns(sorter)
+
+
+def run_in_flight(a, sorter, dwin, starts, steps, warmup, k, precision, Wp, seq_outs, tiles):
+    """The same windows with k in flight (the sorter plus k-1 more handles with its tuning: tile
+    shapes, update grid and layout; one host thread each); checks the spike lists against the
+    one-at-a-time run."""
     import torch
     from livespikesorter_amd.sorter import Sorter, sort_windows
+    from livespikesorter_amd._lib import lib
     extra = [Sorter(a, max_window=Wp, max_passes=50, precision=precision, max_spikes=1 << 16) for _ in range(k - 1)]
+    shapes, grid, cols = tiles
+    for so in extra:
+        so.set_tiles(*shapes)
+        lib().lss_set_tuning(so._h, grid, 0)
+        lib().lss_set_update_columns(so._h, cols)
     group = [sorter] + extra
     nb = len(starts)
     wl = [dwin[i % nb] for i in range(steps)]
@@ -116,28 +129,58 @@ def host_cpu() -> dict:
     return {"cpu_model": model, "host_cpus": os.cpu_count()}
 
 
-def read_traffic():
-    """HBM bytes per update launch measured with rocprofv3 --pmc (committed summary), or None."""
+def read_traffic(path=TRAFFIC_FILE):
+    """HBM bytes per launch measured with rocprofv3 --pmc (committed summary), or None."""
     try:
-        txt = open(TRAFFIC_FILE).read()
+        txt = open(path).read()
     except OSError:
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt)
     return float(m.group(1)) if m else None
 
 
+METRIC = "x-realtime & p99 batch latency, 384ch@30kHz; bf16x3 vs fp32 at spike F1≥0.999"
 STREAM_PROBES = [(384, 600), (384, 250), (128, 120), (64, 40)]   # (channels, templates)
+STREAM_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "update_kernel_traffic_stream_384ch_600t.md")
+
+
+def _pct(x, q):
+    return round(float(np.percentile(np.asarray(x) * 1e3, q)), 3)
+
+
+def _update_roofline(kernel, ev_ms, gpu_ms, spikes, bytes_per_spike, traffic, where):
+    """Roofline of the pursuit update kernel over the launches that applied spikes: algorithmic
+    bytes per launch (bytes_per_spike x spikes) over the average launch duration from the
+    workgroups' GPU-clock stamps (the execution time a kernel trace reports); the HIP-event time
+    of the same launches, which also holds the dispatch latency, is stated beside it."""
+    d_ms = float(gpu_ms.mean())
+    per_launch = float(spikes.mean()) * bytes_per_spike
+    achieved = per_launch / (d_ms * 1e-3) / 1e9
+    return {"kernel": kernel, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "launch_us": round(d_ms * 1e3, 2), "launch_us_hip_events": round(float(ev_ms.mean()) * 1e3, 2),
+            "per_launch": f"{bytes_per_spike / 1e3:.1f} KB per accepted spike x {spikes.mean():.1f} spikes per "
+                          f"launch = {per_launch / 1e6:.2f} MB; avg {d_ms * 1e3:.2f} us per launch from the workgroups' "
+                          f"GPU-clock stamps (HIP events around each launch: {float(ev_ms.mean()) * 1e3:.2f} us, "
+                          f"which includes the dispatch latency) over {len(gpu_ms)} launches that applied spikes "
+                          f"({where}, measurement mode)"}
 
 
 def bench_stream(args):
-    """Four probes of uneven size on one clock, 3000-sample (100 ms) batches with nt-sample halos,
-    one graph-replayed batch step for all probes; per-batch latency and aggregate x-realtime,
-    beside the same windows sorted probe by probe with the plain single-window path."""
+    """Config 5: four probes of uneven size on one clock, 3000-sample (100 ms) batches with
+    nt-sample halos, 60 s of each, one graph-replayed step per batch for all probes, bf16x3 GEMMs,
+    a template-refinement round every refine_seconds.  After `warmup` untimed steps exactly `steps`
+    steps are timed (the value); the stream then runs to the end of the 60 s (stream_60s, which
+    holds the refinement steps).  Beside it: the plain FP32 path (one probe at a time,
+    sort_window), the same stream with host samples (PCIe in every step), the update kernel's
+    roofline and the FP32 CPU oracle on a sample of the same windows."""
+    import gc
     import torch
     from livespikesorter_amd import synth
     from livespikesorter_amd.metrics import f1_same_template
     from livespikesorter_amd.sorter import Sorter
-    from livespikesorter_amd.streaming import StreamingSorter, autotune, autotune_persistent, batcher_grid_shares
+    from livespikesorter_amd.streaming import (StreamingSorter, autotune, autotune_persistent, batcher_grid_shares,
+                                               update_columns)
 
     batch, fs = 3000, 30000.0
     n = int(args.seconds * fs) + 2 * batch
@@ -146,30 +189,31 @@ def bench_stream(args):
                                                nrange=min(32, C), seed=100 + i))
               for i, (C, T) in enumerate(STREAM_PROBES)]
     print(f"[bench] stream workload ready in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    nbat = (n - 2 * batch) // batch
+    warm = max(0, min(args.warmup, nbat - 1))
+    K = nbat - warm if args.steps is None else max(1, min(args.steps, nbat - warm))
 
     def window(rec, k, nt):
         idx = np.clip(np.arange(k * batch - nt, (k + 1) * batch + nt), 0, None)
         return np.ascontiguousarray(rec[idx])
 
-    sorters = [Sorter(p.art, max_window=batch + 2 * p.art["nt"], max_passes=100, precision=args.precision,
-                      max_spikes=1 << 15) for p in probes]
-    # the plain per-probe baseline: FP32, one window at a time, no refinement
-    plain_sorters = [Sorter(p.art, max_window=batch + 2 * p.art["nt"], max_passes=100, precision="fp32",
-                            max_spikes=1 << 15) for p in probes]
+    mk = lambda p, prec: Sorter(p.art, max_window=batch + 2 * p.art["nt"], max_passes=100, precision=prec,
+                                max_spikes=1 << 15)
+    sorters = [mk(p, args.precision) for p in probes]
+    plain_sorters = [mk(p, "fp32") for p in probes]      # the plain per-probe path
     refine_every = int(round(args.refine_seconds * fs / batch)) if args.refine_seconds > 0 else 0
     tuning = []
     shares = batcher_grid_shares(sorters, batch + 2 * probes[0].art["nt"])
-    for s, s_plain, p, cap in zip(sorters, plain_sorters, probes, shares):
+    for s_, s_plain, p, cap in zip(sorters, plain_sorters, probes, shares):
         w = torch.from_numpy(window(p.rec, 3, p.art["nt"])).to("cuda:0")
-        tiles, tile_ms = s.autotune_tiles(w)
+        tiles, tile_ms = s_.autotune_tiles(w)
         plain_tiles, _ = s_plain.autotune_tiles(w)
-        g, p1, _ = autotune(s, w)
-        pg, pf, pms = autotune_persistent(s, w, max_grid=cap)
-        tuning.append({"tiles": tiles, "tile_ms": tile_ms, "plain_tiles": plain_tiles,
-                       "update_grid": g, "first_passes": p1, "grid_share": cap, "persistent_grid": pg,
-                       "persistent_after": pf,
+        g, p1, _ = autotune(s_, w)
+        pg, pf, pms = autotune_persistent(s_, w, max_grid=cap)
+        tuning.append({"tiles": tiles, "tile_ms": tile_ms, "plain_tiles": plain_tiles, "update_grid": g,
+                       "update_columns": update_columns(s_),
+                       "first_passes": p1, "grid_share": cap, "persistent_grid": pg, "persistent_after": pf,
                        "persistent_ms": {f"{k[0]}/{k[1]}": round(v, 3) for k, v in pms.items()}})
-    nbat = (n - 2 * batch) // batch
     dev = torch.device("cuda:0")
     drecs = [torch.from_numpy(p.rec).to(dev) for p in probes]     # recordings resident in HBM
     w0 = [torch.from_numpy(window(p.rec, 0, p.art["nt"])).to(dev) for p in probes]
@@ -178,74 +222,70 @@ def bench_stream(args):
         nt = probes[i].art["nt"]
         return w0[i] if k == 0 else drecs[i][k * batch - nt:(k + 1) * batch + nt]
 
-    # plain per-probe path: the same windows one probe at a time (device-resident windows)
-    plain = []
-    for i, (s_, p) in enumerate(zip(plain_sorters, probes)):     # warm (first-window setup) untimed
+    # plain per-probe path: every batch's windows one probe at a time (device-resident windows)
+    for i, (s_, p) in enumerate(zip(plain_sorters, probes)):     # first-window setup, untimed
         s_.sort_window(dwindow(i, 0), -p.art["nt"])
     torch.cuda.synchronize()
-    t_plain = time.perf_counter()
+    plain, plain_lat = [], []
     for k in range(nbat):
-        plain.append([s.sort_window(dwindow(i, k), k * batch - p.art["nt"])
-                      for i, (s, p) in enumerate(zip(plain_sorters, probes))])
-    t_plain = time.perf_counter() - t_plain
-
-    warm = max(0, min(args.warmup, nbat - 1))
+        t1 = time.perf_counter()
+        plain.append([s_.sort_window(dwindow(i, k), k * batch - p.art["nt"])
+                      for i, (s_, p) in enumerate(zip(plain_sorters, probes))])
+        plain_lat.append(time.perf_counter() - t1)
 
     def run_stream(srt, device_input):
-        """One pass over the recordings: samples pushed 100 ms at a time, one step per batch; the
-        first `warm` batches (graph capture included) are untimed."""
-        import gc
-        lat, outs, parts, refines = [], [], [], []
+        """One pass over the recordings, samples pushed 100 ms at a time, one step per batch.
+        Returns per-step latencies, outputs, and the wall times of steps [warm, warm+K) and
+        [warm, nbat)."""
+        lat, outs, refines = [], [], []
         src = drecs if device_input else [p.rec for p in probes]
         gc.collect()
         gc.disable()      # no collector pauses of the Python harness inside the timed steps
         try:
-            return _run(srt, src, lat, outs, parts, refines)
+            with StreamingSorter(srt, batch=batch, use_graph=True, refine_every=refine_every) as st:
+                for i in range(len(probes)):
+                    st.push(i, src[i][:batch + 64])
+                pos = batch + 64
+                k = 0
+                t_a = t_b = None
+                t_k = 0.0
+                while k < nbat:
+                    if k == warm:
+                        torch.cuda.synchronize()
+                        t_a = time.perf_counter()
+                    for i in range(len(probes)):
+                        st.push(i, src[i][pos:pos + batch])
+                    pos += batch
+                    t1 = time.perf_counter()
+                    r = st.step()
+                    if r is None:
+                        continue
+                    lat.append(time.perf_counter() - t1)
+                    outs.append(r)
+                    k += 1
+                    if k == warm + K:
+                        torch.cuda.synchronize()
+                        t_k = time.perf_counter() - t_a
+                    if refine_every and k % refine_every == 0:
+                        refines.append([st.last_refine(i) for i in range(len(probes))])
+                torch.cuda.synchronize()
+                t_b = time.perf_counter() - t_a
         finally:
             gc.enable()
-
-    def _run(srt, src, lat, outs, parts, refines):
-        with StreamingSorter(srt, batch=batch, use_graph=True, refine_every=refine_every) as st:
-            for i in range(len(probes)):
-                st.push(i, src[i][:batch + 64])
-            pos = batch + 64
-            torch.cuda.synchronize()
-            t_s = time.perf_counter()
-            k = 0
-            while k < nbat:
-                if k == warm:
-                    torch.cuda.synchronize()
-                    t_s = time.perf_counter()
-                for i in range(len(probes)):
-                    st.push(i, src[i][pos:pos + batch])
-                pos += batch
-                t1 = time.perf_counter()
-                r = st.step()
-                if r is None:
-                    continue
-                if k >= warm:
-                    lat.append(time.perf_counter() - t1)
-                    parts.append(st.breakdown())
-                outs.append(r)
-                k += 1
-                if refine_every and k % refine_every == 0:
-                    refines.append([st.last_refine(i) for i in range(len(probes))])
-            t_s = time.perf_counter() - t_s
-        return lat, outs, parts, refines, t_s
+        return lat, outs, refines, t_k, t_b
 
     # the measured run: recordings resident in HBM, windows assembled device to device
-    lat, outs, parts, refines, t_s = run_stream(sorters, True)
+    lat, outs, refines, t_k, t_all = run_stream(sorters, True)
     # the same stream with host samples (each batch crosses PCIe): fresh sorters, same tuning
-    host_sorters = [Sorter(p.art, max_window=batch + 2 * p.art["nt"], max_passes=100, precision=args.precision,
-                           max_spikes=1 << 15) for p in probes]
+    host_sorters = [mk(p, args.precision) for p in probes]
+    from livespikesorter_amd._lib import lib as _l
     for s2, tu in zip(host_sorters, tuning):
         s2.set_tiles(*tu["tiles"])
         s2.set_persistent(tu["persistent_grid"], tu["persistent_after"])
-        from livespikesorter_amd._lib import lib as _l
         _l().lss_set_tuning(s2._h, tu["update_grid"], tu["first_passes"])
-    lat_h, _, _, _, t_h = run_stream(host_sorters, False)
-    lat_ms = np.array(lat) * 1e3
-    # spike F1 per probe, pooled over batches (samples are recording samples, disjoint per batch)
+        _l().lss_set_update_columns(s2._h, tu["update_columns"])
+    lat_h, _, _, t_kh, t_allh = run_stream(host_sorters, False)
+
     def pooled_f1(k0, k1):
         out = []
         for i in range(len(probes)):
@@ -256,38 +296,272 @@ def bench_stream(args):
     first = min(refine_every, len(outs)) if refine_every else len(outs)
     f1 = min(pooled_f1(0, first))
     f1_all = min(pooled_f1(0, len(outs)))
-    timed = nbat - warm
-    data_s = timed * batch / fs * len(probes)
-    data_plain = nbat * batch / fs * len(probes)
+    P = len(probes)
+    sec = lambda nb: nb * batch / fs * P                       # seconds of data in nb steps
+    value = sec(K) / t_k
+    plain_k = sec(K) / sum(plain_lat[warm:warm + K])
+    timed_lat, all_lat = lat[warm:warm + K], lat[warm:]
+    n_ref_timed = sum(1 for k in range(warm, warm + K) if refine_every and (k + 1) >= refine_every
+                      and ((k + 1) % refine_every) < P)
+
+    # roofline: the update kernel of the 384-ch / 600-template probe (the step's critical path),
+    # per launch with HIP events on the sorter's stream over windows of the timed region
+    so = sorters[0]
+    T0, nt0 = so.T, so.nt
+    bytes_per_spike = 3 * 4 * T0 * (2 * nt0 + 1)      # ctc rows read + B rows read and written
+    so.profile_update(True)
+    ev_all, gpu_all, sp_all = [], [], []
+    for k in range(warm, warm + min(K, 50)):
+        so.sort_window(dwindow(0, k), k * batch - nt0)
+        ev_, gpu_, sp_ = so.update_times()
+        ev_all.append(ev_[sp_ > 0])
+        gpu_all.append(gpu_[sp_ > 0])
+        sp_all.append(sp_[sp_ > 0])
+    so.profile_update(False)
+    ev_all, gpu_all, sp_all = np.concatenate(ev_all), np.concatenate(gpu_all), np.concatenate(sp_all)
+    roofline = _update_roofline("update_kernel<3, C> (pursuit score update of the 384-ch / 600-template probe, "
+                                f"{update_columns(so)} score column(s) per wavefront)", ev_all, gpu_all, sp_all,
+                                bytes_per_spike, read_traffic(STREAM_TRAFFIC_FILE),
+                                f"{min(K, 50)} windows of the timed region, one at a time")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import ks_oracle as ko
+        torch.set_num_threads(args.cpu_threads)
+        t_art = time.perf_counter()
+        ctcs = []
+        for p in probes:
+            U32, W32 = torch.from_numpy(p.art["U"]), torch.from_numpy(p.art["wPCA"])
+            ctcs.append(ko.prepare_matching(U32, W32))
+        t_art = time.perf_counter() - t_art
+        nb_cpu = min(args.cpu_batches, K)
+        per_batch, f1s = [], []
+        for k in range(warm, warm + nb_cpu):
+            t1 = time.perf_counter()
+            refs = []
+            for i, p in enumerate(probes):
+                nt = p.art["nt"]
+                X = torch.from_numpy(window(p.rec, k, nt).T.astype(np.float32))
+                refs.append(ko.sort_window(X, p.art, k * batch - nt, dtype=torch.float32, max_passes=100,
+                                           ctc=ctcs[i]))
+            per_batch.append(time.perf_counter() - t1)
+            for i, r in enumerate(refs):
+                f1s.append(ko.spike_f1(r.samples, r.templates, plain[k][i].sample, plain[k][i].template, dt=1))
+        cpu = {"value": round(sec(nb_cpu) / sum(per_batch), 4), "unit": "x realtime (aggregate over probes)",
+               "cores": torch.get_num_threads(), **host_cpu(), "kind": "port",
+               "p50_ms": _pct(per_batch, 50), "p99_ms": _pct(per_batch, 99),
+               "sample": f"{nb_cpu} batches x {P} probes (100 ms windows of the timed region) through the FP32 torch "
+                         f"CPU oracle, {sum(per_batch):.1f} s; ctc of the four probes built once beforehand "
+                         f"({t_art:.1f} s, not in the time); spike F1 of the plain FP32 GPU path against it: min "
+                         f"{min(f1s):.5f}"}
+
     res = {
-        "metric": "x-realtime & p99 batch latency, 4 probes (384/600, 384/250, 128/120, 64/40) @30kHz, 100 ms batches",
-        "value": round(data_s / t_s, 2), "unit": "x realtime (aggregate over probes)", "n_gpus": 1,
-        "steps": timed, "warmup": warm, "ms_per_step": round(t_s / timed * 1e3, 3), "higher_is_better": True,
+        "metric": METRIC,
+        "value": round(value, 2), "unit": "x realtime (aggregate over 4 probes)", "n_gpus": 1,
+        "steps": K, "warmup": warm, "ms_per_step": round(t_k / K * 1e3, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": args.precision,
         "data": "synthetic (seeded probes, templates and spikes; no recordings)",
-        "config": {"workload": "streaming, 4 probes, batch 3000 + 61-sample halos, one graph-replayed step per batch, "
-                               "template refinement every refine_seconds; recordings resident in HBM (device samples)",
-                   "probes": STREAM_PROBES, "batch": batch, "seconds_per_probe": round(nbat * batch / fs, 2)},
-        "p50_ms": round(float(np.percentile(lat_ms, 50)), 3),
-        "p99_ms": round(float(np.percentile(lat_ms, 99)), 3),
-        "max_ms": round(float(lat_ms.max()), 3),
-        "slowest_steps": [[int(i) + warm, round(float(lat_ms[i]), 3)] for i in np.argsort(-lat_ms)[:6]],
-        "plain_per_probe_x_realtime": round(data_plain / t_plain, 2),
-        "speedup_vs_plain": round((data_s / t_s) / (data_plain / t_plain), 3),
-        "host_input": {"x_realtime": round(data_s / t_h, 2),
-                       "p50_ms": round(float(np.percentile(np.array(lat_h) * 1e3, 50)), 3),
-                       "p99_ms": round(float(np.percentile(np.array(lat_h) * 1e3, 99)), 3),
-                       "note": "samples pushed from host memory, each window uploaded over PCIe in the step"},
+        "config": {"workload": "config 5: 4 probes (384 ch/600 templates, 384/250, 128/120, 64/40) at 30 kHz, "
+                               "3000-sample (100 ms) batches with 61-sample halos, one graph-replayed step per "
+                               "batch for all probes, bf16x3 GEMMs, autotuned tiles and pursuit schedules, a "
+                               "template-refinement round every refine_seconds; recordings resident in HBM",
+                   "probes": STREAM_PROBES, "batch": batch, "seconds_per_probe": round(nbat * batch / fs, 2),
+                   "refine_seconds": args.refine_seconds},
+        "p50_ms": _pct(timed_lat, 50), "p99_ms": _pct(timed_lat, 99), "max_ms": _pct(timed_lat, 100),
+        "latency_note": "per 100-ms batch step (all four probes), over the timed steps",
+        "refinement_steps_in_timed_region": n_ref_timed,
+        "speedup_vs_plain": round(value / plain_k, 3),
+        "plain_fp32": {"x_realtime": round(plain_k, 2), "p50_ms": _pct(plain_lat[warm:warm + K], 50),
+                       "p99_ms": _pct(plain_lat[warm:warm + K], 99),
+                       "note": "FP32 GEMMs, the same windows one probe at a time through lss_sort_window "
+                               "(tiles tuned, default pursuit schedule), the timed steps' batches"},
+        "stream_60s": {"steps": nbat - warm, "x_realtime": round(sec(nbat - warm) / t_all, 2),
+                       "p50_ms": _pct(all_lat, 50), "p99_ms": _pct(all_lat, 99), "max_ms": _pct(all_lat, 100),
+                       "slowest_steps": [[int(i) + warm, round(float(all_lat[i]) * 1e3, 3)]
+                                         for i in np.argsort(-np.asarray(all_lat))[:6]],
+                       "speedup_vs_plain": round((sec(nbat - warm) / t_all) / (sec(nbat - warm) /
+                                                                               sum(plain_lat[warm:])), 3),
+                       "refinement_rounds": len(refines)},
+        "host_input": {"x_realtime": round(sec(K) / t_kh, 2), "p50_ms": _pct(lat_h[warm:warm + K], 50),
+                       "p99_ms": _pct(lat_h[warm:warm + K], 99),
+                       "stream_60s_x_realtime": round(sec(nbat - warm) / t_allh, 2),
+                       "note": "the same stream with samples pushed from host memory: each step uploads its "
+                               "windows over PCIe (the ingest-inclusive rate)"},
         "spike_f1_vs_plain_fp32_before_refinement": round(float(f1), 6),
         "spike_f1_vs_plain_fp32_all_batches": round(float(f1_all), 6),
         "f1_note": "per probe, pooled over batches, +-1 sample and same template; the minimum over probes",
-        "refinement": {"every_batches": refine_every, "steps": len(refines),
-                       "loss_ratio_after_before": [[round(x["loss_after"] / x["loss_before"], 6) if x["loss_before"] else None
-                                                    for x in rr] for rr in refines]},
+        "roofline": roofline,
+        "refinement": {"every_batches": refine_every, "rounds": len(refines),
+                       "loss_ratio_after_before": [[round(x["loss_after"] / x["loss_before"], 6) if x["loss_before"]
+                                                    else None for x in rr] for rr in refines]},
         "tuning": tuning,
-        "step_breakdown_ms_p50": {k: round(float(np.percentile([p[k] for p in parts], 50)), 3) for k in parts[0]},
+        "cpu_baseline": cpu,
     }
-    print(json.dumps(res))
+    for s_ in sorters + plain_sorters + host_sorters:
+        s_.close()
+    return res
+
+
+def bench_window(args, steps, warmup, config=2, in_flight=3, compare="fp32", host_input=True):
+    """Config 2 (or 1): 384 ch, 600 templates, 10 s of int16 in NT = 60000 Kilosort windows
+    (W' = 60122), <= 50 passes.  One step = one padded window through the whole device path incl.
+    the spike-list readback.  Returns the result dict (value: windows in flight on separate
+    handles)."""
+    import torch
+    from livespikesorter_amd import synth
+    from livespikesorter_amd.sorter import Sorter, pad_batch
+    from livespikesorter_amd.metrics import f1_same_template
+
+    t0 = time.time()
+    s = synth.make(config, seed=args.seed)
+    a = s.art
+    NT, nt, C, T = a["NT"], a["nt"], s.rec.shape[1], a["U"].shape[0]
+    K = a["U"].shape[1]
+    nb = (s.rec.shape[0] + NT - 1) // NT
+    wins, starts = [], []
+    for ib in range(nb):
+        w, ws = pad_batch(s.rec, ib, NT, nt)
+        wins.append(w)
+        starts.append(ws)
+    dev = torch.device("cuda:0")
+    dwin = torch.from_numpy(np.stack(wins)).to(dev)          # [nb][W'][C] int16 resident in HBM
+    Wp = dwin.shape[1]
+    print(f"[bench] window workload ready in {time.time() - t0:.1f}s: C={C} T={T} windows={nb} W'={Wp}",
+          file=sys.stderr, flush=True)
+
+    from livespikesorter_amd.streaming import autotune
+    sorter = Sorter(a, max_window=Wp, max_passes=50, precision=args.precision, max_spikes=1 << 16)
+    tiles, tile_ms = sorter.autotune_tiles(dwin[1 % nb])
+    upd_grid, _, _ = autotune(sorter, dwin[1 % nb])
+    tiles = (tiles, upd_grid, update_columns_of(sorter))
+    r = run_path(sorter, dwin, starts, steps, warmup)
+    fly = run_in_flight(a, sorter, dwin, starts, steps, warmup, in_flight, args.precision, Wp,
+                        r["outs"], tiles) if in_flight > 1 else None
+    # the roofline's per-launch durations come from a second timed run with an event pair around
+    # every update launch (kept out of the run above, whose wall clock is the reported value)
+    rp = run_path(sorter, dwin, starts, min(steps, 2 * nb), 0, profile=True)
+    print(f"[bench] {args.precision}: {steps} windows in {r['elapsed']:.3f}s", file=sys.stderr, flush=True)
+    fs = a["fs"]
+    xrt_seq = steps * NT / fs / r["elapsed"]
+    xrt = steps * NT / fs / fly["elapsed"] if fly else xrt_seq
+    spikes = sum(len(o) for o in r["outs"])
+    passes = float(np.mean([o.passes for o in r["outs"]]))
+
+    # dominant kernel: the pursuit update (per accepted spike it reads and writes the B rows of the
+    # spike's 2nt+1 columns and reads as many ctc rows: 3 * 4 * T * (2nt+1) bytes), averaged over
+    # the launches that applied spikes
+    bytes_per_spike = 3 * 4 * T * (2 * nt + 1)
+    ev_w = np.concatenate(rp["upd_each"])
+    gpu_w = np.concatenate(rp["upd_gpu"])
+    sp_w = np.concatenate(rp["upd_spikes"])
+    work = sp_w > 0
+    traffic = read_traffic() if (C, T) == (384, 600) else None
+    from livespikesorter_amd.streaming import update_columns
+    roofline = _update_roofline(f"update_kernel<{(T + 255) // 256}, C> (pursuit score update, {update_columns(sorter)} "
+                                "score column(s) per wavefront)", ev_w[work], gpu_w[work], sp_w[work],
+                                bytes_per_spike, traffic, f"{len(rp['outs'])} windows one at a time")
+    gemm_flops = 2.0 * T * K * C * Wp
+    gemm_ms = r["stage_ms"]["scores_gemm"]
+    peak_gemm = FP32_MFMA_PEAK_TFLOPS if args.precision == "fp32" else BF16_MFMA_PEAK_TFLOPS
+    gemm = {"kernel": "U.Xproj GEMM", "bound": "mfma",
+            "achieved": round(gemm_flops / (gemm_ms * 1e-3) / 1e12, 2), "peak": peak_gemm, "unit": "TFLOP/s",
+            "frac": round(gemm_flops / (gemm_ms * 1e-3) / 1e12 / peak_gemm, 4)}
+    if args.precision in ("bf16x3", "fp16x3"):
+        gemm["issued_frac"] = round(3 * gemm["frac"], 4)
+        gemm["note"] = "achieved/frac count the algorithmic 2TKCW' flops; issued_frac counts the three MFMA products"
+
+    host = None
+    if host_input:
+        # ingest-inclusive: host int16 windows, uploaded by lss_sort_window in every call
+        hw = [np.ascontiguousarray(w) for w in wins]
+        for i in range(warmup):
+            sorter.sort_window(hw[i % nb], starts[i % nb])
+        lat_h = []
+        t1 = time.perf_counter()
+        for i in range(steps):
+            t2 = time.perf_counter()
+            sorter.sort_window(hw[i % nb], starts[i % nb])
+            lat_h.append(time.perf_counter() - t2)
+        el = time.perf_counter() - t1
+        host = {"x_realtime": round(steps * NT / fs / el, 2), "p50_ms": _pct(lat_h, 50), "p99_ms": _pct(lat_h, 99),
+                "note": "host windows one at a time (the upload of each 46 MB window in the step)"}
+
+    cmp = None
+    if compare and compare not in ("none", args.precision):
+        sorter2 = Sorter(a, max_window=Wp, max_passes=50, precision=compare, max_spikes=1 << 16)
+        tiles2, _ = sorter2.autotune_tiles(dwin[1 % nb])
+        g2, _, _ = autotune(sorter2, dwin[1 % nb])
+        tiles2 = (tiles2, g2, update_columns_of(sorter2))
+        r2 = run_path(sorter2, dwin, starts, steps, warmup)
+        fly2 = run_in_flight(a, sorter2, dwin, starts, steps, warmup, in_flight, compare, Wp,
+                             r2["outs"], tiles2) if in_flight > 1 else None
+        f1s = [f1_same_template(o1.sample, o1.template, o2.sample, o2.template, dt=1)
+               for o1, o2 in zip(r["outs"][:nb], r2["outs"][:nb])]
+        cmp = {"precision": compare,
+               "value": round(steps * NT / fs / (fly2 or r2)["elapsed"], 2),
+               "sequential_value": round(steps * NT / fs / r2["elapsed"], 2),
+               "p50_ms": round(float(np.percentile(r2["lat_ms"], 50)), 3),
+               "p99_ms": round(float(np.percentile(r2["lat_ms"], 99)), 3),
+               "scores_gemm_ms": round(r2["stage_ms"]["scores_gemm"], 4),
+               "whiten_ms": round(r2["stage_ms"]["whiten"], 4),
+               "spike_f1_min": round(float(min(f1s)), 6),
+               "f1_note": "spike F1 (+-1 sample, same template) of these spike lists against the "
+                          "primary precision's, min over the recording's windows",
+               "value_ratio": round(xrt / (steps * NT / fs / (fly2 or r2)["elapsed"]), 3)}
+        sorter2.close()
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import ks_oracle as ko
+        torch.set_num_threads(args.cpu_threads)
+        ib = 1 % nb
+        X = ko.padded_batch(s.rec, ib, NT, nt, torch.float32)
+        c0 = time.perf_counter()
+        ctc = ko.prepare_matching(torch.from_numpy(a["U"]), torch.from_numpy(a["wPCA"]))
+        t_ctc = time.perf_counter() - c0
+        c0 = time.perf_counter()
+        ref = ko.sort_window(X, a, ko.window_start(ib, NT, nt), torch.float32, max_passes=50, ctc=ctc)
+        csec = time.perf_counter() - c0
+        got = sorter.sort_window(dwin[ib], starts[ib])
+        f1 = ko.spike_f1(ref.samples, ref.templates, got.sample, got.template, dt=1)
+        cpu = {"value": round(NT / fs / csec, 4), "unit": "x realtime", "cores": torch.get_num_threads(), **host_cpu(),
+               "kind": "port",
+               "sample": f"one padded window (batch {ib}, {NT} samples = {NT / fs:.1f} s) through the "
+                         f"FP32 torch CPU oracle, {csec:.1f} s (ctc built beforehand, {t_ctc:.1f} s); "
+                         f"GPU-vs-CPU spike F1 {f1:.5f}"}
+    sorter.close()
+    return {
+        "metric": METRIC,
+        "value": round(xrt, 2), "unit": "x realtime", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": round((fly or r)["elapsed"] / steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": args.precision,
+        "data": "synthetic (seeded probe, templates and spikes; no recordings)",
+        "config": {"workload": f"config {config}: {C} ch, {T} templates, {K} PCs, nt={nt}, NT={NT}-sample Kilosort "
+                               f"windows (W'={int(Wp)}), {s.rec.shape[0] / a['fs']:.0f} s recording cycled, "
+                               f"<=50 pursuit passes, windows resident in HBM",
+                   "n_chan": C, "n_templates": T, "window": int(Wp), "NT": NT, "max_passes": 50,
+                   "precision": args.precision, "tiles": tiles[0], "tile_ms": tile_ms, "update_grid": tiles[1],
+                   "update_columns": tiles[2]},
+        "p50_ms": round(float(np.percentile(r["lat_ms"], 50)), 3),
+        "p99_ms": round(float(np.percentile(r["lat_ms"], 99)), 3),
+        "latency_note": "p50/p99: one window at a time (the live per-batch case)",
+        "windows_in_flight": in_flight,
+        "sequential": {"value": round(xrt_seq, 2), "ms_per_window": round(r["elapsed"] / steps * 1e3, 3)},
+        "in_flight": None if fly is None else {
+            "windows": in_flight, "value": round(xrt, 2),
+            "ms_per_window": round(fly["elapsed"] / steps * 1e3, 3),
+            "spikes_equal_to_sequential": fly["equal"],
+            "note": "value: windows spread over this many handles, one host thread each, so one "
+                    "window's GEMMs overlap another's pursuit"},
+        "host_input": host,
+        "spikes_per_window": round(spikes / steps, 1),
+        "passes_per_window": round(passes, 2),
+        "stage_ms": {k: round(v, 4) for k, v in r["stage_ms"].items()},
+        "roofline": roofline,
+        "gemm_roofline": gemm,
+        "compare": cmp,
+        "cpu_baseline": cpu,
+    }
 
 
 def bench_adjoint(args):
@@ -594,21 +868,26 @@ def bench_train(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (stream: 100-ms batches, default all after the warmup; window: windows, "
+                         "default 50)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--precision", default="bf16x3", choices=["fp32", "bf16", "fp16", "bf16x3", "fp16x3"])
-    ap.add_argument("--compare", default="fp32", help="second precision reported beside the first ('none' = none)")
+    ap.add_argument("--compare", default="fp32", help="window workload: second precision reported beside the first")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--workload", default="window", choices=["window", "stream", "adjoint", "precision", "train"],
-                    help="window: 384-ch drop-in windows (default); stream: 4 uneven probes, 100 ms batches; "
+    ap.add_argument("--cpu-batches", type=int, default=80, help="stream workload: batches the CPU oracle sorts")
+    ap.add_argument("--workload", default="stream", choices=["stream", "window", "adjoint", "precision", "train"],
+                    help="stream: config 5, 4 uneven probes, 100 ms batches (default; the config-2 windows are "
+                         "reported beside it unless --no-window); window: 384-ch drop-in 2-s windows; "
                          "adjoint: template-refinement backward on a 2 s drift window; "
                          "precision: config-3 precision report (1000 templates, 60 s); "
                          "train: training-side FP64 builders (whitening, drift, wPCA)")
-    ap.add_argument("--seconds", type=float, default=20.0, help="stream workload: seconds per probe")
+    ap.add_argument("--seconds", type=float, default=60.0, help="stream workload: seconds per probe")
     ap.add_argument("--refine-seconds", type=float, default=10.0,
-                    help="stream workload: one template-refinement step per probe every this many seconds (0 = off)")
+                    help="stream workload: one template-refinement round every this many seconds (0 = off)")
+    ap.add_argument("--no-window", action="store_true", help="stream workload: skip the nested 2-s window figures")
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3],
                     help="window workload: synthetic configuration (1: 64 ch / 32 units, the reference's CPU-runnable "
                          "case; 2: 384 ch / 600 units, default; 3: 384 ch / 1000 units)")
@@ -634,146 +913,22 @@ def main():
         return
 
     if args.workload == "stream":
-        return bench_stream(args)
+        res = bench_stream(args)
+        if not args.no_window:
+            res["window_2s"] = bench_window(args, steps=20, warmup=5)
+        print(json.dumps(res))
+        return
+    if args.steps is None:
+        args.steps = 50
     if args.workload == "adjoint":
         return bench_adjoint(args)
     if args.workload == "precision":
         return bench_precision(args)
     if args.workload == "train":
         return bench_train(args)
-
-    from livespikesorter_amd import synth
-    from livespikesorter_amd.sorter import Sorter, pad_batch
-    from livespikesorter_amd.metrics import f1_same_template
-
-    t0 = time.time()
-    s = synth.make(args.config, seed=args.seed)
-    a = s.art
-    NT, nt, C, T = a["NT"], a["nt"], s.rec.shape[1], a["U"].shape[0]
-    K = a["U"].shape[1]
-    nb = (s.rec.shape[0] + NT - 1) // NT
-    wins, starts = [], []
-    for ib in range(nb):
-        w, ws = pad_batch(s.rec, ib, NT, nt)
-        wins.append(w)
-        starts.append(ws)
-    dev = torch.device("cuda:0")
-    dwin = torch.from_numpy(np.stack(wins)).to(dev)          # [nb][W'][C] int16 resident in HBM
-    Wp = dwin.shape[1]
-    print(f"[bench] workload ready in {time.time() - t0:.1f}s: C={C} T={T} windows={nb} W'={Wp}",
-          file=sys.stderr, flush=True)
-
-    sorter = Sorter(a, max_window=Wp, max_passes=50, precision=args.precision, max_spikes=1 << 16)
-    r = run_path(sorter, dwin, starts, args.steps, args.warmup)
-    fly = run_in_flight(a, sorter, dwin, starts, args.steps, args.warmup, args.in_flight, args.precision, Wp,
-                        r["outs"]) if args.in_flight > 1 else None
-    # the roofline's per-launch durations come from a second timed run with an event pair around
-    # every update launch (kept out of the run above, whose wall clock is the reported value)
-    rp = run_path(sorter, dwin, starts, min(args.steps, 2 * nb), 0, profile=True)
-    print(f"[bench] {args.precision}: {args.steps} windows in {r['elapsed']:.3f}s", file=sys.stderr, flush=True)
-    fs = a["fs"]
-    xrt_seq = args.steps * NT / fs / r["elapsed"]
-    xrt = args.steps * NT / fs / fly["elapsed"] if fly else xrt_seq
-    spikes = sum(len(o) for o in r["outs"])
-    passes = float(np.mean([o.passes for o in r["outs"]]))
-
-    # dominant kernel: the pursuit update (per accepted spike it reads and writes the B rows of the
-    # spike's 2nt+1 columns and reads as many ctc rows: 3 * 4 * T * (2nt+1) bytes)
-    bytes_per_spike = 3 * 4 * T * (2 * nt + 1)
-    upd_launches = rp["upd_n"]
-    spikes_p = sum(len(o) for o in rp["outs"])
-    per_launch_bytes = spikes_p * bytes_per_spike / max(upd_launches, 1)
-    avg_launch_ms = rp["upd_ms"] / max(upd_launches, 1)
-    achieved = per_launch_bytes / (avg_launch_ms * 1e-3) / 1e9
-    # the committed counter measurement covers the 384-ch / 600-template workload only
-    traffic = read_traffic() if (C, T) == (384, 600) else None
-    roofline = {"kernel": "update_kernel (pursuit score update)", "bound": "hbm",
-                "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic,
-                "per_launch": f"{bytes_per_spike / 1e3:.1f} KB per accepted spike x "
-                              f"{spikes_p / max(upd_launches, 1):.1f} spikes per launch (avg over "
-                              f"{upd_launches // len(rp['outs'])} launches per window incl. converged ones), "
-                              f"avg {avg_launch_ms * 1e3:.1f} us per launch"}
-    gemm_flops = 2.0 * T * K * C * Wp
-    gemm_ms = r["stage_ms"]["scores_gemm"]
-    peak_gemm = FP32_MFMA_PEAK_TFLOPS if args.precision == "fp32" else BF16_MFMA_PEAK_TFLOPS
-    gemm = {"kernel": "U.Xproj GEMM", "bound": "mfma",
-            "achieved": round(gemm_flops / (gemm_ms * 1e-3) / 1e12, 2), "peak": peak_gemm, "unit": "TFLOP/s",
-            "frac": round(gemm_flops / (gemm_ms * 1e-3) / 1e12 / peak_gemm, 4)}
-    if args.precision in ("bf16x3", "fp16x3"):
-        # the split product issues three MFMA products (lo*hi, hi*lo, hi*hi) per algorithmic one
-        gemm["issued_frac"] = round(3 * gemm["frac"], 4)
-        gemm["note"] = "achieved/frac count the algorithmic 2TKCW' flops; issued_frac counts the three MFMA products"
-
-    compare = None
-    if args.compare and args.compare not in ("none", args.precision):
-        sorter2 = Sorter(a, max_window=Wp, max_passes=50, precision=args.compare, max_spikes=1 << 16)
-        r2 = run_path(sorter2, dwin, starts, args.steps, args.warmup)
-        fly2 = run_in_flight(a, sorter2, dwin, starts, args.steps, args.warmup, args.in_flight, args.compare, Wp,
-                             r2["outs"]) if args.in_flight > 1 else None
-        f1s = [f1_same_template(o1.sample, o1.template, o2.sample, o2.template, dt=1)
-               for o1, o2 in zip(r["outs"][:nb], r2["outs"][:nb])]
-        compare = {"precision": args.compare,
-                   "value": round(args.steps * NT / fs / (fly2 or r2)["elapsed"], 2),
-                   "sequential_value": round(args.steps * NT / fs / r2["elapsed"], 2),
-                   "p50_ms": round(float(np.percentile(r2["lat_ms"], 50)), 3),
-                   "p99_ms": round(float(np.percentile(r2["lat_ms"], 99)), 3),
-                   "scores_gemm_ms": round(r2["stage_ms"]["scores_gemm"], 4),
-                   "whiten_ms": round(r2["stage_ms"]["whiten"], 4),
-                   "spike_f1_min": round(float(min(f1s)), 6),
-                   "f1_note": "spike F1 (+-1 sample, same template) of these spike lists against the "
-                              "primary precision's, min over the recording's windows",
-                   "value_ratio": round(xrt / (args.steps * NT / fs / (fly2 or r2)["elapsed"]), 3)}
-        sorter2.close()
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        from oracle import ks_oracle as ko
-        torch.set_num_threads(args.cpu_threads)
-        ib = 1 % nb
-        X = ko.padded_batch(s.rec, ib, NT, nt, torch.float32)
-        c0 = time.perf_counter()
-        ref = ko.sort_window(X, a, ko.window_start(ib, NT, nt), torch.float32, max_passes=50)
-        csec = time.perf_counter() - c0
-        got = sorter.sort_window(dwin[ib], starts[ib])
-        f1 = ko.spike_f1(ref.samples, ref.templates, got.sample, got.template, dt=1)
-        cpu = {"value": round(NT / fs / csec, 4), "unit": "x realtime", "cores": torch.get_num_threads(), **host_cpu(),
-               "kind": "port",
-               "sample": f"one padded window (batch {ib}, {NT} samples = {NT / fs:.1f} s) through the "
-                         f"FP32 torch CPU oracle, {csec:.1f} s; GPU-vs-CPU spike F1 {f1:.5f}"}
-
-    res = {
-        "metric": "x-realtime & p99 batch latency, 384ch@30kHz; bf16x3 vs fp32 at spike F1>=0.999",
-        "value": round(xrt, 2), "unit": "x realtime", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round((fly or r)["elapsed"] / args.steps * 1e3, 3),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": args.precision,
-        "data": "synthetic (seeded probe, templates and spikes; no recordings)",
-        "config": {"workload": f"{C} ch, {T} templates, {K} PCs, nt={nt}, NT={NT}-sample Kilosort windows "
-                               f"(W'={int(Wp)}), {s.rec.shape[0] / a['fs']:.0f} s recording cycled, <=50 pursuit passes",
-                   "synthetic_config": args.config,
-                   "n_chan": C, "n_templates": T, "window": int(Wp), "NT": NT, "max_passes": 50,
-                   "precision": args.precision},
-        "p50_ms": round(float(np.percentile(r["lat_ms"], 50)), 3),
-        "p99_ms": round(float(np.percentile(r["lat_ms"], 99)), 3),
-        "latency_note": "p50/p99: one window at a time (the live per-batch case)",
-        "windows_in_flight": args.in_flight,
-        "sequential": {"value": round(xrt_seq, 2), "ms_per_window": round(r["elapsed"] / args.steps * 1e3, 3)},
-        "in_flight": None if fly is None else {
-            "windows": args.in_flight, "value": round(xrt, 2),
-            "ms_per_window": round(fly["elapsed"] / args.steps * 1e3, 3),
-            "spikes_equal_to_sequential": fly["equal"],
-            "note": "value: windows spread over this many handles, one host thread each, so one "
-                    "window's GEMMs overlap another's pursuit"},
-        "spikes_per_window": round(spikes / args.steps, 1),
-        "passes_per_window": round(passes, 2),
-        "stage_ms": {k: round(v, 4) for k, v in r["stage_ms"].items()},
-        "roofline": roofline,
-        "gemm_roofline": gemm,
-        "compare": compare,
-        "cpu_baseline": cpu,
-    }
-    print(json.dumps(res))
+    steps = 50 if args.steps is None else args.steps
+    print(json.dumps(bench_window(args, steps, args.warmup, config=args.config, in_flight=args.in_flight,
+                                  compare=args.compare)))
 
 
 if __name__ == "__main__":

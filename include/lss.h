@@ -315,14 +315,19 @@ lss_status lss_refine_templates(void* handle, double* info);
 
 /* ---- tuning ------------------------------------------------------------------------------------- */
 
-/* Per-pass kernels: times the window (device memory) with each candidate update grid (256 .. 8192
- * workgroups) and keeps the fastest; sets the batcher's first-segment pass count to the passes
- * seen + 4 (capped at max_passes).  grid_ms (6 floats, may be NULL) receives the best time per
- * candidate.  Results do not depend on the tuning. */
+/* Per-pass kernels: times the window (device memory) with each candidate update layout (2 or 1
+ * score columns per wavefront: fewer or lighter work items) and grid (256 .. 8192 workgroups) and
+ * keeps the fastest; sets the batcher's first-segment pass count to the passes seen + 4 (capped at
+ * max_passes).  grid_ms (12 floats, may be NULL) receives the best time per candidate,
+ * [6 * layout + grid] with layout 0 = two columns, 1 = one column per wavefront.  Results do not
+ * depend on the tuning. */
 lss_status lss_autotune(void* handle, const int16_t* d_window, int64_t W, int32_t reps, float* grid_ms,
                         int32_t* best_grid, int32_t* first_passes);
 lss_status lss_set_tuning(void* handle, int32_t update_grid, int32_t first_passes);
 lss_status lss_get_tuning(void* handle, int32_t* update_grid, int32_t* first_passes);
+/* The update layout lss_autotune chose: score columns per wavefront, 1 or 2 (default 2). */
+lss_status lss_set_update_columns(void* handle, int32_t per_wave);
+lss_status lss_get_update_columns(void* handle, int32_t* per_wave);
 
 /* Tile shapes of the window's three contractions: the whitening GEMM, the wPCA projection and the
  * U.Xproj score GEMM.  GEMM output tiles: 0 = 128 x 128 (default), 1 = 128 x 64, 2 = 64 x 128,
@@ -368,6 +373,14 @@ lss_status lss_autotune_persistent(void* handle, const int16_t* d_window, int64_
  * of update launches (max_passes per window). */
 lss_status lss_profile_update(void* handle, int32_t enable);
 lss_status lss_last_update_stats(void* handle, float* total_ms, int32_t* launches);
+/* Per launch of the last window's update kernel (measurement mode): event_ms[i] between HIP events
+ * recorded around it (includes the dispatch latency, ~9 us here, so short launches read long),
+ * gpu_ms[i] from the workgroups' own GPU-clock stamps (last end - first start:
+ * the execution time a kernel trace reports), spikes[i] the spikes it applied (0 for launches after
+ * the pursuit converged), for i < min(n, *launches).  lss_profile_update must be enabled after the
+ * update grid is set. */
+lss_status lss_last_update_times(void* handle, float* event_ms, float* gpu_ms, int32_t* spikes, int32_t n,
+                                 int32_t* launches);
 
 /* ---- per-batch drift correction ----------------------------------------------------------------
  * Replaces the handle's whitening operator with (M @ Wrot), M a device [C][C] FP64 matrix (row

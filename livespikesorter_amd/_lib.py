@@ -30,7 +30,8 @@ EXPORTS = ["lss_last_error", "lss_abi_version", "lss_default_params", "lss_creat
            "lss_set_drift", "lss_covariance_add", "lss_whitening_from_covariance", "lss_whitening_local",
            "lss_drift_inverse", "lss_drift_matrix", "lss_extract_snippets", "lss_wpca", "lss_train_release",
            "lss_select_device", "lss_sort_windows", "lss_cluster_centroids_pca", "lss_assign_clusters", "lss_load_cluster_centroids", "lss_load_channel_map",
-           "lss_get_load_info", "lss_set_tiles", "lss_get_tiles", "lss_autotune_tiles"]
+           "lss_get_load_info", "lss_set_tiles", "lss_get_tiles", "lss_autotune_tiles",
+           "lss_last_update_times", "lss_set_update_columns", "lss_get_update_columns"]
 
 
 class Params(ct.Structure):
@@ -144,6 +145,9 @@ def lib() -> ct.CDLL:
             "lss_set_tiles": ([vp, i32, i32, i32], i32),
             "lss_get_tiles": ([vp, vp], i32),
             "lss_autotune_tiles": ([vp, vp, i64, i32, vp, vp], i32),
+            "lss_last_update_times": ([vp, vp, vp, vp, i32, ct.POINTER(i32)], i32),
+            "lss_set_update_columns": ([vp, i32], i32),
+            "lss_get_update_columns": ([vp, ct.POINTER(i32)], i32),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)

@@ -121,6 +121,7 @@ struct Handle {
     // per-launch timing of the pursuit update kernel (measurement mode)
     bool prof_update = false;
     int upd_grid = 8192;        // workgroups of the update kernel (autotuned)
+    int upd_cpw = 2;            // score columns per wavefront of the update kernel (autotuned)
     int tuned_passes = 0;       // passes in the first graph segment of the batcher (0 = all)
     int pers_grid = 128;        // workgroups of the persistent pursuit (windows <= kPersistMaxW; 0 = off)
     int pers_first = 0;         // leading passes on the per-pass kernels before the persistent kernel
@@ -130,6 +131,11 @@ struct Handle {
     std::vector<hipEvent_t> upd_ev;
     float upd_ms = 0.f;
     int upd_launches = 0;
+    std::vector<float> upd_each;   // per-launch ms of the last window (measurement mode): HIP events
+    std::vector<float> upd_gpu;    // the same from the workgroups' GPU-clock stamps
+    std::vector<int> upd_spikes;   // spikes each of those launches applied
+    DevBuf<unsigned long long> upd_ts;   // [max_passes][upd_grid][2] GPU-clock stamps
+    DevBuf<unsigned long long> upd_span; // [max_passes][2] first start, last end per launch
     float last_ms[8] = {};   // car, highpass, whiten, project, scores GEMM, pursuit, features, total
     lss_load_info load{};    // which extraction tables were given, the ctc.npy comparison
     bool alive = false;
@@ -479,9 +485,13 @@ lss_status enqueue_passes(Handle& h, const lss::PursuitArgs& a, int p0, int p1) 
     }
     for (int p = p0; p < p1; ++p) {
         HIP_TRY(lss::launch_peaks(a, h.stream));
-        if (h.prof_update) HIP_TRY(hipEventRecord(h.upd_ev[2 * p], h.stream));
-        HIP_TRY(lss::launch_update(a, h.upd_grid, h.stream));
-        if (h.prof_update) HIP_TRY(hipEventRecord(h.upd_ev[2 * p + 1], h.stream));
+        if (h.prof_update) {
+            lss::PursuitArgs b = a;
+            // stamps only while the buffer matches the grid (lss_profile_update sizes it)
+            if (h.upd_ts.n == (size_t)h.prm.max_passes * 2 * h.upd_grid) b.ts = h.upd_ts.p + (size_t)p * 2 * h.upd_grid;
+            HIP_TRY(lss::launch_update(b, h.upd_grid, h.stream, h.upd_ev[2 * p], h.upd_ev[2 * p + 1], h.upd_cpw));
+        } else
+            HIP_TRY(lss::launch_update(a, h.upd_grid, h.stream, nullptr, nullptr, h.upd_cpw));
     }
     return LSS_OK;
 }
@@ -593,11 +603,34 @@ lss_status check_counts(Handle& h, const int* c, int* n, int* passes) {
     if (h.prof_update) {
         h.upd_ms = 0.f;
         h.upd_launches = h.prm.max_passes;
+        h.upd_each.assign(h.prm.max_passes, 0.f);
+        h.upd_spikes.assign(h.prm.max_passes, 0);
         for (int p = 0; p < h.prm.max_passes; ++p) {
             float ms = 0.f;
             HIP_TRY(hipEventElapsedTime(&ms, h.upd_ev[2 * p], h.upd_ev[2 * p + 1]));
             h.upd_ms += ms;
+            h.upd_each[p] = ms;
         }
+        // GPU-clock duration of launch p: last workgroup end - first workgroup start (10 ns ticks)
+        const size_t nts = (size_t)h.prm.max_passes * 2 * h.upd_grid;
+        h.upd_gpu.assign(h.prm.max_passes, 0.f);
+        if (h.upd_ts.n == nts) {
+            std::vector<unsigned long long> span(2 * (size_t)h.prm.max_passes);
+            HIP_TRY(lss::launch_stamp_span(h.upd_ts.p, h.prm.max_passes, h.upd_grid, h.upd_span.p, h.stream));
+            HIP_TRY(hipMemcpyAsync(span.data(), h.upd_span.p, sizeof(unsigned long long) * span.size(),
+                                   hipMemcpyDeviceToHost, h.stream));
+            HIP_TRY(hipStreamSynchronize(h.stream));
+            for (int p = 0; p < h.prm.max_passes; ++p) {
+                const unsigned long long t0 = span[2 * p], t1 = span[2 * p + 1];
+                h.upd_gpu[p] = t1 > t0 ? (float)((double)(t1 - t0) * 1e-5) : 0.f;
+            }
+        }
+        // launch p applied the spikes of pass p (passes that found spikes come first)
+        std::vector<int> off(h.prm.max_passes + 2, 0);
+        HIP_TRY(hipMemcpyAsync(off.data(), h.pass_off.p, sizeof(int) * (h.prm.max_passes + 2), hipMemcpyDeviceToHost,
+                               h.stream));
+        HIP_TRY(hipStreamSynchronize(h.stream));
+        for (int p = 0; p < *passes && p < h.prm.max_passes; ++p) h.upd_spikes[p] = off[p + 1] - off[p];
     }
     return LSS_OK;
 }
@@ -1520,6 +1553,14 @@ lss_status lss_profile_update(void* handle, int32_t enable) {
         h.upd_ev.assign(2 * (size_t)h.prm.max_passes, nullptr);
         for (auto& e : h.upd_ev) HIP_TRY(hipEventCreate(&e));
     }
+    if (enable && h.upd_ts.n != (size_t)h.prm.max_passes * 2 * h.upd_grid) {
+        // stamps for the current update grid (re-sized if the grid changed since)
+        HIP_TRY(hipStreamSynchronize(h.stream));
+        if (h.upd_ts.p) HIP_TRY(hipFree(h.upd_ts.p));
+        h.upd_ts.p = nullptr;
+        HIP_TRY(h.upd_ts.alloc((size_t)h.prm.max_passes * 2 * h.upd_grid));
+        if (!h.upd_span.p) HIP_TRY(h.upd_span.alloc(2 * (size_t)h.prm.max_passes));
+    }
     h.prof_update = enable != 0;
     return LSS_OK;
 }
@@ -1530,6 +1571,21 @@ lss_status lss_last_update_stats(void* handle, float* total_ms, int32_t* launche
     if (!h.prof_update) return fail(LSS_ERR_STATE, "update profiling is off (lss_profile_update)");
     *total_ms = h.upd_ms;
     *launches = h.upd_launches;
+    return LSS_OK;
+}
+
+lss_status lss_last_update_times(void* handle, float* event_ms, float* gpu_ms, int32_t* spikes, int32_t n,
+                                 int32_t* launches) {
+    if (!handle || !launches || n < 0 || (n > 0 && (!event_ms || !gpu_ms || !spikes)))
+        return fail(LSS_ERR_ARG, "bad argument");
+    Handle& h = *as_handle(handle);
+    if (!h.prof_update) return fail(LSS_ERR_STATE, "update profiling is off (lss_profile_update)");
+    *launches = (int32_t)h.upd_each.size();
+    for (int i = 0; i < n && i < (int)h.upd_each.size(); ++i) {
+        event_ms[i] = h.upd_each[i];
+        gpu_ms[i] = h.upd_gpu[i];
+        spikes[i] = h.upd_spikes[i];
+    }
     return LSS_OK;
 }
 
@@ -1944,32 +2000,49 @@ lss_status lss_autotune(void* handle, const int16_t* d_window, int64_t W, int32_
     lss_spikes out{};
     out.capacity = h.prm.max_spikes;   // counts only: every output array is NULL
     float best = 1e30f;
-    int bg = h.upd_grid, max_passes_seen = 0;
+    int bg = h.upd_grid, bc = h.upd_cpw, max_passes_seen = 0;
     const int pers = h.pers_grid;   // the sweep times the per-pass kernels
     h.pers_grid = 0;
-    for (int gi = 0; gi < 6; ++gi) {
-        h.upd_grid = kGrids[gi];
-        float t = 1e30f;
-        for (int r = 0; r <= reps; ++r) {   // first run warms up
-            if (lss_status st = lss_sort_window(handle, d_window, W, 1, 0, &out)) {
-                h.pers_grid = pers;
-                return st;
+    for (int li = 0; li < 2; ++li) {
+        h.upd_cpw = 2 - li;          // two, then one score column per wavefront
+        for (int gi = 0; gi < 6; ++gi) {
+            h.upd_grid = kGrids[gi];
+            float t = 1e30f;
+            for (int r = 0; r <= reps; ++r) {   // first run warms up
+                if (lss_status st = lss_sort_window(handle, d_window, W, 1, 0, &out)) {
+                    h.pers_grid = pers;
+                    return st;
+                }
+                if (r > 0) t = std::min(t, h.last_ms[7]);
+                max_passes_seen = std::max(max_passes_seen, out.passes);
             }
-            if (r > 0) t = std::min(t, h.last_ms[7]);
-            max_passes_seen = std::max(max_passes_seen, out.passes);
-        }
-        if (grid_ms) grid_ms[gi] = t;
-        if (t < best) {
-            best = t;
-            bg = kGrids[gi];
+            if (grid_ms) grid_ms[6 * li + gi] = t;
+            if (t < best) {
+                best = t;
+                bg = kGrids[gi];
+                bc = h.upd_cpw;
+            }
         }
     }
     h.pers_grid = pers;
     h.upd_grid = bg;
+    h.upd_cpw = bc;
     // the first graph segment covers the passes seen plus a margin; the rest run only when needed
     h.tuned_passes = std::min(h.prm.max_passes, max_passes_seen + 4);
     *best_grid = bg;
     *first_passes = h.tuned_passes;
+    return LSS_OK;
+}
+
+lss_status lss_set_update_columns(void* handle, int32_t per_wave) {
+    if (!handle || (per_wave != 1 && per_wave != 2)) return fail(LSS_ERR_ARG, "bad argument");
+    as_handle(handle)->upd_cpw = per_wave;
+    return LSS_OK;
+}
+
+lss_status lss_get_update_columns(void* handle, int32_t* per_wave) {
+    if (!handle || !per_wave) return fail(LSS_ERR_ARG, "null argument");
+    *per_wave = as_handle(handle)->upd_cpw;
     return LSS_OK;
 }
 

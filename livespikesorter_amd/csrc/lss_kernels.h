@@ -69,6 +69,9 @@ struct PursuitArgs {
     float* sp_amp;         // [cap]
     int* pass_off;         // [max_passes + 1] first spike of each pass that found spikes
     float* amprow;         // persistent pursuit: the Cfmax/imax row's amplitudes (set by the launcher)
+    // measurement mode: the update kernel's workgroup b writes its GPU-clock (100 MHz) start and end
+    // to ts[2b], ts[2b + 1] (NULL otherwise)
+    unsigned long long* ts;
     // persistent pursuit: Cfmax, imax and amplitude rows [2][kPersistMaxW], one per pass parity
     float* prow_cf;
     int* prow_im;
@@ -88,8 +91,14 @@ constexpr int kCounters = 16;
 hipError_t launch_reset_counters(int* counters, hipStream_t s);        // counters[0..kCounters) = 0
 hipError_t launch_score_all(const PursuitArgs& a, hipStream_t s);     // T <= 1024
 hipError_t launch_peaks(const PursuitArgs& a, hipStream_t s);
-// grid = workgroups (grid-stride over the pass's work items; any grid gives the same result)
-hipError_t launch_update(const PursuitArgs& a, int grid, hipStream_t s);
+// grid = workgroups (grid-stride over the pass's work items; any grid gives the same result);
+// start/stop (measurement mode): events recorded around the launch; cpw = score columns per
+// wavefront, 1 or 2 (same results)
+hipError_t launch_update(const PursuitArgs& a, int grid, hipStream_t s, hipEvent_t start = nullptr,
+                         hipEvent_t stop = nullptr, int cpw = 2);
+// measurement mode: out[2p], out[2p + 1] = min start, max end of the update stamps ts[p][grid][2]
+hipError_t launch_stamp_span(const unsigned long long* ts, int launches, int grid, unsigned long long* out,
+                             hipStream_t s);
 // Persistent pursuit of windows W <= kPersistMaxW: passes [first_pass, max_passes) in one launch of
 // `grid` co-resident workgroups (clamped at persistent_max_grid); with first_pass = 0 it also does
 // the first scoring, otherwise it continues from the state that first_pass per-pass (peaks,

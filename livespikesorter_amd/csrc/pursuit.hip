@@ -300,9 +300,11 @@ __global__ __launch_bounds__(kFlagCols) void peaks_kernel(PursuitArgs a) {
 }
 
 // ------------------------------------------------------------------------------------------------
-// update: work items (spike s, chunk of kColsB of its window's score columns)
+// update: work items (spike s, chunk of 4 * CPW of its window's score columns, CPW columns per
+// wavefront: 2 (fewer items, for passes with many spikes) or 1 (half the work per lane, for
+// passes with few spikes); every column gets the same arithmetic either way)
 // ------------------------------------------------------------------------------------------------
-constexpr int kColsB = 8;     // score columns per work item (2 per wavefront)
+constexpr int kColsB = 8;     // column alignment of the persistent pursuit's ownership runs
 constexpr int kUpdateGrid = 8192;   // workgroups per update launch (grid-stride over work items)
 
 // A pass's spike list (ascending columns): plain loads (a list stored by an earlier kernel, or in
@@ -440,13 +442,14 @@ __device__ __forceinline__ void update_two(const PursuitArgs& a, const List& l, 
     }
 }
 
-// Per-pass update: the chunk [c0, c0 + kColsB) of the columns [lo, w1] whose first covering spike is s
-// (c1, c2 = the next two spikes' columns), two columns per wavefront.
-template <int NV, typename List, bool DEV>
+// Per-pass update: the chunk [c0, c0 + 4 CPW) of the columns [lo, w1] whose first covering spike is
+// s (c1, c2 = the next two spikes' columns), CPW columns per wavefront.
+template <int NV, typename List, bool DEV, int CPW>
 __device__ __forceinline__ void update_score_columns(const PursuitArgs& a, const List& l, int s, int n, int c1, int c2,
                                                      int c0, int lo, int w1, const floatx4* nmv) {
-    const int wa = c0 + 2 * (int)(threadIdx.x >> 6), wb = wa + 1;
-    update_two<NV, List, DEV>(a, l, n, wa, s, c1, c2, wa >= lo && wa <= w1, wb, s, c1, c2, wb >= lo && wb <= w1, nmv);
+    const int wa = c0 + CPW * (int)(threadIdx.x >> 6), wb = wa + 1;
+    update_two<NV, List, DEV>(a, l, n, wa, s, c1, c2, wa >= lo && wa <= w1, wb, s, c1, c2,
+                              CPW == 2 && wb >= lo && wb <= w1, nmv);
 }
 
 template <int NV>
@@ -461,11 +464,12 @@ __device__ __forceinline__ void load_norms(const PursuitArgs& a, floatx4* nmv) {
 
 // Work items q = q0, q0 + stride, ... of a pass's n spikes (col/tmp/amp, ascending columns; global
 // or LDS memory).  A column is written only by the item of the first spike whose window covers it.
-template <int NV>
+template <int NV, int CPW>
 __device__ __forceinline__ void update_items(const PursuitArgs& a, const ListPlain& l, int n, int64_t q0,
                                              int64_t stride, const floatx4* nmv) {
+    constexpr int kCols = 4 * CPW;
     const int nt = a.nt;
-    const int nB = (2 * nt + 1 + kColsB - 1) / kColsB;
+    const int nB = (2 * nt + 1 + kCols - 1) / kCols;
     for (int64_t q = q0; q < (int64_t)n * nB; q += stride) {
         const int s = (int)(q / nB), sub = (int)(q % nB);
         // the spike's column and its neighbours' (independent loads, issued together)
@@ -477,21 +481,29 @@ __device__ __forceinline__ void update_items(const PursuitArgs& a, const ListPla
         if (s > 0) lo = max(lo, cp + nt + 1);
         lo = max(lo, 0);
         const int hi = (int)min((int64_t)ws + nt, a.W - 1);
-        const int c0 = ws - nt + sub * kColsB;
-        const int w1 = min(c0 + kColsB - 1, hi);
-        if (max(c0, lo) <= w1) update_score_columns<NV, ListPlain, false>(a, l, s, n, c1, c2, c0, lo, w1, nmv);
+        const int c0 = ws - nt + sub * kCols;
+        const int w1 = min(c0 + kCols - 1, hi);
+        if (max(c0, lo) <= w1) update_score_columns<NV, ListPlain, false, CPW>(a, l, s, n, c1, c2, c0, lo, w1, nmv);
     }
 }
 
-template <int NV>
+template <int NV, int CPW>
 __global__ __launch_bounds__(256) void update_kernel(PursuitArgs a) {
+    if (a.ts && threadIdx.x == 0) a.ts[2 * blockIdx.x] = __builtin_amdgcn_s_memrealtime();
     const int base = a.counters[0];
     int n = a.counters[1];
-    if (n == 0) return;
     if (base + n > a.cap) n = a.cap - base;
-    floatx4 nmv[NV];
-    load_norms<NV>(a, nmv);
-    update_items<NV>(a, ListPlain{a.sp_col + base, a.sp_tmp + base, a.sp_amp + base}, n, blockIdx.x, gridDim.x, nmv);
+    if (n > 0) {
+        floatx4 nmv[NV];
+        load_norms<NV>(a, nmv);
+        update_items<NV, CPW>(a, ListPlain{a.sp_col + base, a.sp_tmp + base, a.sp_amp + base}, n, blockIdx.x,
+                              gridDim.x, nmv);
+    }
+    if (a.ts) {
+        __builtin_amdgcn_s_waitcnt(0);   // this wavefront's stores issued and done
+        __syncthreads();
+        if (threadIdx.x == 0) a.ts[2 * blockIdx.x + 1] = __builtin_amdgcn_s_memrealtime();
+    }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -907,11 +919,57 @@ hipError_t launch_peaks(const PursuitArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_update(const PursuitArgs& a, int grid, hipStream_t s) {
+// Measurement mode: per launch p, the first workgroup start and last workgroup end of the stamps
+// ts[p][b][2] (b < grid) into out[2p], out[2p + 1]; one workgroup per launch.
+__global__ __launch_bounds__(256) void stamp_span_kernel(const unsigned long long* __restrict__ ts, int grid,
+                                                         unsigned long long* __restrict__ out) {
+    __shared__ unsigned long long s0[256], s1[256];
+    const unsigned long long* t = ts + (size_t)blockIdx.x * 2 * grid;
+    unsigned long long a = ~0ull, b = 0;
+    for (int i = threadIdx.x; i < grid; i += 256) {
+        a = min(a, t[2 * i]);
+        b = max(b, t[2 * i + 1]);
+    }
+    s0[threadIdx.x] = a;
+    s1[threadIdx.x] = b;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (threadIdx.x < h) {
+            s0[threadIdx.x] = min(s0[threadIdx.x], s0[threadIdx.x + h]);
+            s1[threadIdx.x] = max(s1[threadIdx.x], s1[threadIdx.x + h]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * blockIdx.x] = s0[0];
+        out[2 * blockIdx.x + 1] = s1[0];
+    }
+}
+
+hipError_t launch_stamp_span(const unsigned long long* ts, int launches, int grid, unsigned long long* out,
+                             hipStream_t s) {
+    if (launches < 1 || grid < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(stamp_span_kernel, dim3(launches), dim3(256), 0, s, ts, grid, out);
+    return hipGetLastError();
+}
+
+template <int NV, int CPW>
+static void update_launch(const PursuitArgs& a, int grid, hipStream_t s, hipEvent_t start, hipEvent_t stop) {
+    // measurement mode: events recorded on the stream around the launch (their gap also holds the
+    // dispatch latency; the workgroups' GPU-clock stamps give the execution time)
+    if (start) (void)hipEventRecord(start, s);
+    hipLaunchKernelGGL((update_kernel<NV, CPW>), dim3(grid), dim3(256), 0, s, a);
+    if (stop) (void)hipEventRecord(stop, s);
+}
+
+hipError_t launch_update(const PursuitArgs& a, int grid, hipStream_t s, hipEvent_t start, hipEvent_t stop, int cpw) {
     const int nv = (a.T + 255) / 256;
     if (grid < 1) grid = kUpdateGrid;
-    switch (nv) {
-#define LSS_UP(V) case V: hipLaunchKernelGGL(update_kernel<V>, dim3(grid), dim3(256), 0, s, a); break;
+    if (cpw != 1 && cpw != 2) return hipErrorInvalidValue;
+    switch (nv * 2 + cpw - 1) {
+#define LSS_UP(V)                                                            \
+    case 2 * V: update_launch<V, 1>(a, grid, s, start, stop); break;         \
+    case 2 * V + 1: update_launch<V, 2>(a, grid, s, start, stop); break;
         LSS_UP(1) LSS_UP(2) LSS_UP(3) LSS_UP(4)
 #undef LSS_UP
         default: return hipErrorInvalidValue;

@@ -224,6 +224,17 @@ class Sorter:
         check(lib().lss_get_load_info(self._h, ct.byref(info)), "lss_get_load_info")
         return {f: getattr(info, f) for f, _ in info._fields_}
 
+    def update_times(self):
+        """Measurement mode: (event ms, GPU-clock ms, spikes) per update launch of the last window."""
+        n = ct.c_int32()
+        check(lib().lss_last_update_times(self._h, None, None, None, 0, ct.byref(n)), "lss_last_update_times")
+        ev = np.zeros(n.value, np.float32)
+        gpu = np.zeros(n.value, np.float32)
+        sp = np.zeros(n.value, np.int32)
+        check(lib().lss_last_update_times(self._h, ev.ctypes.data, gpu.ctypes.data, sp.ctypes.data, n.value,
+                                          ct.byref(n)), "lss_last_update_times")
+        return ev, gpu, sp
+
     def timings(self):
         ms = np.zeros(8, np.float32)
         check(lib().lss_last_timings(self._h, ms.ctypes.data, 8), "lss_last_timings")

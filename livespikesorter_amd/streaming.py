@@ -112,14 +112,25 @@ class StreamingSorter:
         self.close()
 
 
+UPDATE_GRIDS = [256, 512, 1024, 2048, 4096, 8192]
+
+
 def autotune(sorter: Sorter, window_dev, reps: int = 2):
-    """Picks the update grid (fastest of 256..8192 workgroups) and the first-segment pass count
-    for `sorter` on a device window; returns (grid, first_passes, {grid: ms})."""
-    ms = np.zeros(6, np.float32)
+    """Picks the update layout (2 or 1 score columns per wavefront) and grid (fastest of
+    256..8192 workgroups) and the first-segment pass count for `sorter` on a device window;
+    returns (grid, first_passes, {(columns per wavefront, grid): ms})."""
+    ms = np.zeros(12, np.float32)
     g, p = ct.c_int32(), ct.c_int32()
     check(lib().lss_autotune(sorter._h, window_dev.data_ptr(), window_dev.shape[0], int(reps), ms.ctypes.data,
                              ct.byref(g), ct.byref(p)), "lss_autotune")
-    return g.value, p.value, dict(zip([256, 512, 1024, 2048, 4096, 8192], ms.tolist()))
+    return g.value, p.value, {(2 - i // 6, UPDATE_GRIDS[i % 6]): float(v) for i, v in enumerate(ms)}
+
+
+def update_columns(sorter: Sorter) -> int:
+    """Score columns per wavefront of the sorter's update kernel (1 or 2)."""
+    c = ct.c_int32()
+    check(lib().lss_get_update_columns(sorter._h, ct.byref(c)), "lss_get_update_columns")
+    return c.value
 
 
 PERSIST_GRIDS = [0, 64, 128, 256, 384]

@@ -417,11 +417,12 @@ class WindowResult:
 
 
 def sort_window(Xraw: torch.Tensor, art: dict, wstart: int, dtype=torch.float64,
-                max_passes: int = 100, drop_negative: bool = True) -> WindowResult:
+                max_passes: int = 100, drop_negative: bool = True, ctc: torch.Tensor | None = None) -> WindowResult:
     """One padded window through the whole path in ``dtype``:
     filter (ks/io.py:757-790) -> run_matching (template_matching.py:128-202) -> features
     (:50-51) -> negative-time drop (:53-59) -> positions (postprocessing.py:33-46) -> samples (:68).
 
+    ``ctc`` (from ``prepare_matching``) may be passed to keep the load-time table out of a timing.
     ``art`` holds numpy arrays: U [T][K][C], wPCA [K][nt], Wrot [C][C], drift [C][C] (or None),
     hp_filter, xc, yc and scalars Th, nt0min, nC.
     """
@@ -432,7 +433,9 @@ def sort_window(Xraw: torch.Tensor, art: dict, wstart: int, dtype=torch.float64,
     hp = cast(art["hp_filter"])
     nt = wPCA.shape[1]
     X = filter_batch(Xraw.to(dtype), hp, Wm)
-    ctc = prepare_matching(U, wPCA)
+    if ctc is None:                       # load-time table (template_matching.py:38); may be given
+        ctc = prepare_matching(U, wPCA)
+    ctc = ctc.to(dtype)
     m = run_matching(X, U, wPCA, ctc, Th=float(art["Th"]), max_passes=max_passes)
     iCC, iU, Ucc = prepare_extract(np.asarray(art["xc"], np.float64), np.asarray(art["yc"], np.float64),
                                    U, int(art["nC"]))

@@ -129,6 +129,21 @@ def test_pursuit_bit_exact(sorter, synth1, W, grid, first):
         sorter.set_persistent(128)
 
 
+@pytest.mark.parametrize("W", [60122, 3122])
+def test_pursuit_bit_exact_one_column_per_wavefront(sorter, synth1, W):
+    """The update kernel's other layout (one score column per wavefront, twice the work items)
+    reproduces the reference pursuit bit for bit as well."""
+    from livespikesorter_amd._lib import lib
+    print(f"[lss] test_pursuit_bit_exact_one_column_per_wavefront W={W}", flush=True)
+    sorter.set_persistent(0)
+    assert lib().lss_set_update_columns(sorter._h, 1) == 0
+    try:
+        _pursuit_bit_exact(sorter, synth1, W)
+    finally:
+        lib().lss_set_update_columns(sorter._h, 2)
+        sorter.set_persistent(128)
+
+
 def _pursuit_bit_exact(sorter, synth1, W):
     x32, B32, tb, ctc_ks, ws, U32, w32 = _pursuit_case(sorter, synth1, 0)
     if W < x32.shape[1]:
