@@ -447,6 +447,22 @@ lss_status lss_extract_snippets(const float* d_x_wc, int64_t ldx, int32_t C, int
 lss_status lss_wpca(const float* d_clips, int64_t n, int32_t nt, int32_t n_pcs, double* d_wpca, double* d_sv,
                     void* hip_stream);
 
+/* Allocation-free lss_extract_snippets: same arguments, results and errors, for callers that
+ * cannot allocate device memory (tight acquisition or serving loops, harnesses that forbid
+ * hipMalloc).  It has no counterpart in the reference.  Its scratch (two peak bitmaps of
+ * 8 * C * ceil(W / 64) bytes each and the scan's block sums) lives in a fixed device array of
+ * LSS_NOALLOC_SCRATCH_BYTES on each GPU, held from the library's load until the process ends; no
+ * call allocates or frees device memory.  A window whose scratch does not fit (C * ceil(W / 64)
+ * above about 520000, e.g. 384 channels by 88000 samples) fails with LSS_ERR_ARG before any GPU
+ * work.  Calls on one device share its array: they take turns on a lock held until their kernels
+ * are done, and every word a call uses is written before it is read, so the array's leftover
+ * contents never matter.  Synchronous, like lss_extract_snippets. */
+#define LSS_NOALLOC_SCRATCH_BYTES (8u << 20)
+lss_status lss_extract_snippets_noalloc(const float* d_x_wc, int64_t ldx, int32_t C, int64_t W, int32_t nt,
+                                        int32_t twav_min, float th_single_ch, const int32_t* loc_range,
+                                        const int32_t* long_range, float* d_clips, int64_t max_clips, int32_t* d_chan,
+                                        int32_t* d_time, int64_t* n_found, void* hip_stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,7 +31,8 @@ EXPORTS = ["lss_last_error", "lss_abi_version", "lss_default_params", "lss_creat
            "lss_drift_inverse", "lss_drift_matrix", "lss_extract_snippets", "lss_wpca", "lss_train_release",
            "lss_select_device", "lss_sort_windows", "lss_cluster_centroids_pca", "lss_assign_clusters", "lss_load_cluster_centroids", "lss_load_channel_map",
            "lss_get_load_info", "lss_set_tiles", "lss_get_tiles", "lss_autotune_tiles",
-           "lss_last_update_times", "lss_set_update_columns", "lss_get_update_columns"]
+           "lss_last_update_times", "lss_set_update_columns", "lss_get_update_columns",
+           "lss_extract_snippets_noalloc"]
 
 
 class Params(ct.Structure):
@@ -132,6 +133,8 @@ def lib() -> ct.CDLL:
             "lss_drift_matrix": ([vp, vp, i32, vp, vp, i32, f64, vp, i64, vp, i64, vp], i32),
             "lss_extract_snippets": ([vp, i64, i32, i64, i32, i32, ct.c_float, vp, vp, vp, i64, vp, vp,
                                       ct.POINTER(i64), vp], i32),
+            "lss_extract_snippets_noalloc": ([vp, i64, i32, i64, i32, i32, ct.c_float, vp, vp, vp, i64, vp, vp,
+                                              ct.POINTER(i64), vp], i32),
             "lss_wpca": ([vp, i64, i32, i32, vp, vp, vp], i32),
             "lss_train_release": ([], i32),
             "lss_select_device": ([i32], i32),

@@ -106,6 +106,67 @@ void interp_linear(const double* x, const double* y, int n, const double* xn, in
     }
 }
 
+// ---- isolated-peak snippets: argument checks and the kernels, over scratch the caller provides
+
+lss_status snippet_args(const char* fn, const float* d_x_wc, int64_t ldx, int32_t C, int64_t W, int32_t nt,
+                        int32_t twav_min, const int32_t* loc_range, const int32_t* long_range, const float* d_clips,
+                        int64_t max_clips, const int64_t* n_found) {
+    if (!d_x_wc || !loc_range || !long_range || !n_found || C < 1 || ldx < C || W < 1 || nt < 1 ||
+        twav_min < 0 || twav_min > nt || max_clips < 0 || (max_clips > 0 && !d_clips))
+        return err(LSS_ERR_ARG, std::string(fn) + ": bad argument");
+    if (loc_range[0] < 0 || loc_range[1] < 0 || loc_range[0] > 8 || loc_range[1] > 8 || long_range[0] < 0 ||
+        long_range[1] < 0)
+        return err(LSS_ERR_ARG, std::string(fn) + ": loc_range entries must be in [0, 8], long_range >= 0");
+    return LSS_OK;
+}
+
+// Scratch sizes: the peak and isolated-peak bitmaps hold nwords 32-bit words each (two words per
+// 64 samples of each channel), the block-sum scan nb ints, the total one long long.  The kernels
+// write every word of all four before reading it, so nothing has to start at zero.
+struct SnippetScratch {
+    int64_t nw, nwords;
+    int nb;
+    SnippetScratch(int32_t C, int64_t W)
+        : nw((W + 63) / 64 * 2), nwords((int64_t)C * nw), nb(lss::snippet_scan_blocks(nwords)) {}
+};
+
+size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
+lss_status snippet_run(const float* d_x_wc, int64_t ldx, int32_t C, int64_t W, int32_t nt, int32_t twav_min,
+                       float th_single_ch, const int32_t* loc_range, const int32_t* long_range, float* d_clips,
+                       int64_t max_clips, int32_t* d_chan, int32_t* d_time, int64_t* n_found, const SnippetScratch& z,
+                       unsigned* pk, unsigned* iso, int* bsum, long long* total, hipStream_t s) {
+    TRY_HIP(lss::launch_isolated_peaks(d_x_wc, ldx, C, W, loc_range[0], loc_range[1], th_single_ch, long_range[0],
+                                       long_range[1], nt, pk, iso, z.nw, s));
+    TRY_HIP(lss::launch_emit_snippets(iso, z.nwords, z.nw, bsum, total, d_x_wc, ldx, nt, twav_min, d_clips, max_clips,
+                                      d_chan, d_time, s));
+    long long n = 0;
+    TRY_HIP(hipMemcpyAsync(&n, total, sizeof(long long), hipMemcpyDeviceToHost, s));
+    TRY_HIP(hipStreamSynchronize(s));
+    *n_found = n;
+    return LSS_OK;
+}
+
+// Fixed scratch of lss_extract_snippets_noalloc.  Each GPU holds its own copy of this array from
+// the module's load until the process ends; the library never allocates or frees it.  Host code
+// uses the device address from hipGetSymbolAddress (kept per device), never the array's name.
+static __device__ __align__(256) unsigned char scratch[LSS_NOALLOC_SCRATCH_BYTES];
+constexpr int kNoAllocDevices = 64;
+std::mutex g_noalloc_mu[kNoAllocDevices];      // one call at a time per device
+void* g_noalloc_ptr[kNoAllocDevices] = {};     // scratch's address on each device, once looked up
+
+// With the device `dev` current and g_noalloc_mu[dev] held: scratch's address on that device.
+lss_status noalloc_scratch(int dev, char** base) {
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        TRY_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return err(LSS_ERR_HIP, "lss_extract_snippets_noalloc: no device address for the scratch array");
+        g_noalloc_ptr[dev] = p;
+    }
+    *base = (char*)g_noalloc_ptr[dev];
+    return LSS_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -264,34 +325,56 @@ lss_status lss_extract_snippets(const float* d_x_wc, int64_t ldx, int32_t C, int
                                 float th_single_ch, const int32_t* loc_range, const int32_t* long_range,
                                 float* d_clips, int64_t max_clips, int32_t* d_chan, int32_t* d_time, int64_t* n_found,
                                 void* hip_stream) {
-    if (!d_x_wc || !loc_range || !long_range || !n_found || C < 1 || ldx < C || W < 1 || nt < 1 ||
-        twav_min < 0 || twav_min > nt || max_clips < 0 || (max_clips > 0 && !d_clips))
-        return err(LSS_ERR_ARG, "lss_extract_snippets: bad argument");
-    if (loc_range[0] < 0 || loc_range[1] < 0 || loc_range[0] > 8 || loc_range[1] > 8 || long_range[0] < 0 ||
-        long_range[1] < 0)
-        return err(LSS_ERR_ARG, "lss_extract_snippets: loc_range entries must be in [0, 8], long_range >= 0");
+    if (lss_status st = snippet_args("lss_extract_snippets", d_x_wc, ldx, C, W, nt, twav_min, loc_range, long_range,
+                                     d_clips, max_clips, n_found))
+        return st;
     hipStream_t s = (hipStream_t)hip_stream;
-    const int64_t nw = (W + 63) / 64 * 2;
-    const int64_t nwords = (int64_t)C * nw;
-    const int nb = lss::snippet_scan_blocks(nwords);
+    const SnippetScratch z(C, W);
     Scratch sc(s);
     unsigned *pk, *iso;
     int* bsum;
     long long* total;
-    sc.add(&pk, nwords);
-    sc.add(&iso, nwords);
-    sc.add(&bsum, nb);
+    sc.add(&pk, z.nwords);
+    sc.add(&iso, z.nwords);
+    sc.add(&bsum, z.nb);
     sc.add(&total, 1);
     TRY_HIP(sc.commit());
-    TRY_HIP(lss::launch_isolated_peaks(d_x_wc, ldx, C, W, loc_range[0], loc_range[1], th_single_ch, long_range[0],
-                                       long_range[1], nt, pk, iso, nw, s));
-    TRY_HIP(lss::launch_emit_snippets(iso, nwords, nw, bsum, total, d_x_wc, ldx, nt, twav_min, d_clips, max_clips,
-                                      d_chan, d_time, s));
-    long long n = 0;
-    TRY_HIP(hipMemcpyAsync(&n, total, sizeof(long long), hipMemcpyDeviceToHost, s));
-    TRY_HIP(hipStreamSynchronize(s));
-    *n_found = n;
-    return LSS_OK;
+    return snippet_run(d_x_wc, ldx, C, W, nt, twav_min, th_single_ch, loc_range, long_range, d_clips, max_clips,
+                       d_chan, d_time, n_found, z, pk, iso, bsum, total, s);
+}
+
+lss_status lss_extract_snippets_noalloc(const float* d_x_wc, int64_t ldx, int32_t C, int64_t W, int32_t nt,
+                                        int32_t twav_min, float th_single_ch, const int32_t* loc_range,
+                                        const int32_t* long_range, float* d_clips, int64_t max_clips, int32_t* d_chan,
+                                        int32_t* d_time, int64_t* n_found, void* hip_stream) {
+    if (lss_status st = snippet_args("lss_extract_snippets_noalloc", d_x_wc, ldx, C, W, nt, twav_min, loc_range,
+                                     long_range, d_clips, max_clips, n_found))
+        return st;
+    const SnippetScratch z(C, W);
+    const size_t need = 2 * round256(sizeof(unsigned) * z.nwords) + round256(sizeof(int) * z.nb) + 256;
+    if (need > sizeof(scratch))
+        return err(LSS_ERR_ARG, "lss_extract_snippets_noalloc: C = " + std::to_string(C) + ", W = " +
+                                    std::to_string(W) + " needs " + std::to_string(need) +
+                                    " bytes of scratch, more than the fixed " + std::to_string(sizeof(scratch)) +
+                                    " (C * ceil(W / 64) must stay below about 520000)");
+    int dev = 0;
+    TRY_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocDevices) return err(LSS_ERR_ARG, "lss_extract_snippets_noalloc: device index too large");
+    hipStream_t s = (hipStream_t)hip_stream;
+    std::lock_guard<std::mutex> g(g_noalloc_mu[dev]);
+    char* base = nullptr;
+    if (lss_status st = noalloc_scratch(dev, &base)) return st;
+    // the kernels are done before the lock is released, on every path
+    struct Drain {
+        hipStream_t s;
+        ~Drain() { (void)hipStreamSynchronize(s); }
+    } drain{s};
+    unsigned* pk = (unsigned*)base;
+    unsigned* iso = (unsigned*)(base + round256(sizeof(unsigned) * z.nwords));
+    int* bsum = (int*)(base + 2 * round256(sizeof(unsigned) * z.nwords));
+    long long* total = (long long*)(base + 2 * round256(sizeof(unsigned) * z.nwords) + round256(sizeof(int) * z.nb));
+    return snippet_run(d_x_wc, ldx, C, W, nt, twav_min, th_single_ch, loc_range, long_range, d_clips, max_clips,
+                       d_chan, d_time, n_found, z, pk, iso, bsum, total, s);
 }
 
 lss_status lss_wpca(const float* d_clips, int64_t n, int32_t nt, int32_t n_pcs, double* d_wpca, double* d_sv,

@@ -147,25 +147,28 @@ def get_drift_matrix(xc, yc, dshift, iKxx: torch.Tensor, sig_interp: float = 20.
 
 
 def extract_snippets(X_wc: torch.Tensor, nt: int = 61, twav_min: int = 20, Th_single_ch: float = 6.0,
-                     loc_range=(4, 5), long_range=(6, 30), max_clips: int | None = None):
+                     loc_range=(4, 5), long_range=(6, 30), max_clips: int | None = None, no_alloc: bool = False):
     """Snippets at the isolated single-channel peaks of a whitened [W][C] window, in channel then
-    sample order.  Returns (clips [n][nt] float32, chan [n], time [n], n_found)."""
+    sample order.  Returns (clips [n][nt] float32, chan [n], time [n], n_found).  no_alloc runs
+    lss_extract_snippets_noalloc, whose scratch is a fixed device array (C * ceil(W / 64) up to
+    about 520000)."""
     _rows(X_wc, torch.float32)
     W, C = X_wc.shape
+    fn = "lss_extract_snippets_noalloc" if no_alloc else "lss_extract_snippets"
+    call = getattr(lib(), fn)
     lr = (ct.c_int32 * 2)(*[int(v) for v in loc_range])
     gr = (ct.c_int32 * 2)(*[int(v) for v in long_range])
     n = ct.c_int64()
     # count first, then write exactly the needed rows
-    check(lib().lss_extract_snippets(X_wc.data_ptr(), X_wc.stride(0), C, W, nt, twav_min, float(Th_single_ch), lr, gr,
-                                     None, 0, None, None, ct.byref(n), _stream()), "lss_extract_snippets")
+    check(call(X_wc.data_ptr(), X_wc.stride(0), C, W, nt, twav_min, float(Th_single_ch), lr, gr, None, 0, None, None,
+               ct.byref(n), _stream()), fn)
     cap = n.value if max_clips is None else min(n.value, int(max_clips))
     clips = torch.empty((max(cap, 1), nt), dtype=torch.float32, device=X_wc.device)
     chan = torch.empty(max(cap, 1), dtype=torch.int32, device=X_wc.device)
     tim = torch.empty(max(cap, 1), dtype=torch.int32, device=X_wc.device)
     if cap > 0:
-        check(lib().lss_extract_snippets(X_wc.data_ptr(), X_wc.stride(0), C, W, nt, twav_min, float(Th_single_ch), lr,
-                                         gr, clips.data_ptr(), cap, chan.data_ptr(), tim.data_ptr(), ct.byref(n),
-                                         _stream()), "lss_extract_snippets")
+        check(call(X_wc.data_ptr(), X_wc.stride(0), C, W, nt, twav_min, float(Th_single_ch), lr, gr, clips.data_ptr(),
+                   cap, chan.data_ptr(), tim.data_ptr(), ct.byref(n), _stream()), fn)
     return clips[:cap], chan[:cap], tim[:cap], n.value
 
 
