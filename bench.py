@@ -177,6 +177,7 @@ def bench_stream(args):
     import gc
     import torch
     from livespikesorter_amd import synth
+    from livespikesorter_amd._lib import lib as _l
     from livespikesorter_amd.metrics import f1_same_template
     from livespikesorter_amd.sorter import Sorter
     from livespikesorter_amd.streaming import (StreamingSorter, autotune, autotune_persistent, batcher_grid_shares,
@@ -204,7 +205,23 @@ def bench_stream(args):
     refine_every = int(round(args.refine_seconds * fs / batch)) if args.refine_seconds > 0 else 0
     tuning = []
     shares = batcher_grid_shares(sorters, batch + 2 * probes[0].art["nt"])
-    for s_, s_plain, p, cap in zip(sorters, plain_sorters, probes, shares):
+    pinned = json.loads(args.pin_tuning) if args.pin_tuning else None
+    if pinned is not None and len(pinned) != len(probes):
+        raise SystemExit(f"--pin-tuning: {len(pinned)} entries for {len(probes)} probes")
+    for i, (s_, s_plain, p, cap) in enumerate(zip(sorters, plain_sorters, probes, shares)):
+        if pinned is not None:
+            # a schedule printed by an earlier run, replayed as is (profiling runs, whose tracer
+            # overhead would steer the autotuner elsewhere)
+            tiles, plain_tiles, g, cols, p1, pg, pf = pinned[i]
+            s_.set_tiles(*tiles)
+            s_plain.set_tiles(*plain_tiles)
+            _l().lss_set_tuning(s_._h, g, p1)
+            _l().lss_set_update_columns(s_._h, cols)
+            s_.set_persistent(pg, pf)
+            tuning.append({"tiles": list(tiles), "plain_tiles": list(plain_tiles), "update_grid": g,
+                           "update_columns": cols, "first_passes": p1, "grid_share": cap, "persistent_grid": pg,
+                           "persistent_after": pf, "pinned": True})
+            continue
         w = torch.from_numpy(window(p.rec, 3, p.art["nt"])).to("cuda:0")
         tiles, tile_ms = s_.autotune_tiles(w)
         plain_tiles, _ = s_plain.autotune_tiles(w)
@@ -278,7 +295,6 @@ def bench_stream(args):
     lat, outs, refines, t_k, t_all = run_stream(sorters, True)
     # the same stream with host samples (each batch crosses PCIe): fresh sorters, same tuning
     host_sorters = [mk(p, args.precision) for p in probes]
-    from livespikesorter_amd._lib import lib as _l
     for s2, tu in zip(host_sorters, tuning):
         s2.set_tiles(*tu["tiles"])
         s2.set_persistent(tu["persistent_grid"], tu["persistent_after"])
@@ -888,6 +904,9 @@ def main():
     ap.add_argument("--refine-seconds", type=float, default=10.0,
                     help="stream workload: one template-refinement round every this many seconds (0 = off)")
     ap.add_argument("--no-window", action="store_true", help="stream workload: skip the nested 2-s window figures")
+    ap.add_argument("--pin-tuning", default=None,
+                    help="stream workload: JSON list, per probe [tiles, plain_tiles, update_grid, update_columns, "
+                         "first_passes, persistent_grid, persistent_after], used instead of the autotuner")
     ap.add_argument("--config", type=int, default=2, choices=[1, 2, 3],
                     help="window workload: synthetic configuration (1: 64 ch / 32 units, the reference's CPU-runnable "
                          "case; 2: 384 ch / 600 units, default; 3: 384 ch / 1000 units)")

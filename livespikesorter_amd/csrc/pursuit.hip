@@ -26,6 +26,7 @@
 namespace lss {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef unsigned uint32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ void argmax_merge(float& v, int& i, float v2, int i2) {
     if (v2 > v || (v2 == v && i2 < i)) { v = v2; i = i2; }
@@ -59,9 +60,87 @@ __device__ __forceinline__ void wave_argmax_b(float& v, int& i, float& b, float&
     }
 }
 
+// Wavefront maximum (uniform result): DPP steps within rows of 16 lanes, then the row broadcasts.
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ float dpp_f(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, v), __builtin_bit_cast(int, v),
+                                                                 CTRL, ROWS, 0xF, false));
+}
+__device__ __forceinline__ float readlane_f(float v, int l) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ __forceinline__ float wave_max(float v) {
+    v = fmaxf(v, dpp_f<0xB1>(v));         // quad_perm [1, 0, 3, 2]
+    v = fmaxf(v, dpp_f<0x4E>(v));         // quad_perm [2, 3, 0, 1]
+    v = fmaxf(v, dpp_f<0x141>(v));        // row_half_mirror
+    v = fmaxf(v, dpp_f<0x140>(v));        // row_mirror
+    v = fmaxf(v, dpp_f<0x142, 0xA>(v));   // row_bcast:15 into rows 1 and 3
+    v = fmaxf(v, dpp_f<0x143, 0xC>(v));   // row_bcast:31 into rows 2 and 3
+    return readlane_f(v, 63);
+}
+
 __device__ __forceinline__ float score_of(float b, float nmv) {
     float r = fmaxf(b, 0.f);
     return (r * r) / nmv;
+}
+
+// Column rescoring, exact: Cfmax = max_t q_t with q_t = fl(fl(r_t r_t) / nm_t), r_t = max(B[w][t], 0)
+// (the reference's relu(B)**2 / nm), and its first argmax, over t < T; x[i][e] holds the column's
+// B[w][4 lane + 256 i + e].  The division is only evaluated where it can decide the result: with
+// s_t = fl(fl(r_t r_t) rn_t), rn_t = fl(1 / nm_t), |s_t / q_t - 1| < 2^-22 in the normal range, so
+// every t with q_t = max q has s_t >= M (1 - 2^-20), M = max s.  Those candidates (nearly always a
+// single one) get the exact quotient and the first-maximum merge; M outside [1e-30, 1e30] makes
+// every template a candidate.  With DEV, b and m receive the winner's B[w][t] and nm_t.
+constexpr float kCandidate = 0.99999905f;   // 1 - 2^-20
+template <int NV, bool DEV>
+__device__ __forceinline__ void rescore(const floatx4* x, const floatx4* nmv, const floatx4* rnv, int T, float& best,
+                                        int& bi, float& bb, float& bm) {
+    const int lane = threadIdx.x & 63;
+    float sa[NV][4];
+    float ml = -1.f;
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const float r = fmaxf(x[i][e], 0.f);
+            const float sv = (r * r) * rnv[i][e];
+            sa[i][e] = (4 * lane + 256 * i + e < T) ? sv : -1.f;
+            ml = fmaxf(ml, sa[i][e]);
+        }
+    const float M = wave_max(ml);
+    const float thr = (M >= 1e-30f && M <= 1e30f) ? M * kCandidate : -0.5f;
+    best = -1.f;
+    bi = 0x7fffffff;
+    bb = 0.f;
+    bm = 1.f;
+#pragma unroll
+    for (int i = 0; i < NV; ++i)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            if (sa[i][e] >= thr) {
+                const float q = score_of(x[i][e], nmv[i][e]);
+                if (q > best) {   // ascending t within the lane: the first maximum stays
+                    best = q;
+                    bi = 4 * lane + 256 * i + e;
+                    if constexpr (DEV) {
+                        bb = x[i][e];
+                        bm = nmv[i][e];
+                    }
+                }
+            }
+    const unsigned long long cand = __ballot(bi != 0x7fffffff);
+    if (__popcll(cand) == 1) {
+        const int l = __ffsll((long long)cand) - 1;
+        best = readlane_f(best, l);
+        bi = __builtin_amdgcn_readlane(bi, l);
+        if constexpr (DEV) {
+            bb = readlane_f(bb, l);
+            bm = readlane_f(bm, l);
+        }
+    } else if (cand != 0) {
+        if constexpr (DEV) wave_argmax_b(best, bi, bb, bm);
+        else wave_argmax(best, bi);
+    }
 }
 
 // Device-scope (all XCDs) relaxed loads and stores: the only way workgroups of one launch of the
@@ -70,6 +149,31 @@ template <typename V>
 __device__ __forceinline__ V ld_dev(const V* p) { return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 template <typename V>
 __device__ __forceinline__ void st_dev(V* p, V v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+
+// Score rows B[w][t .. t+3]: plain 16-byte accesses between kernels; inside the persistent pursuit
+// (DEV), whose workgroups may update any column in any pass, device-coherent ones (the same cache
+// policy as the device-scope loads and stores above) through a buffer resource over B, whose bounds
+// the hardware checks (offsets past the end read 0 and drop the store).
+constexpr int kCoherent = 16;          // buffer cache policy: device scope
+constexpr int kRawBufferWord3 = 0x00020000;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t b_rsrc(const PursuitArgs& a) {
+    const int64_t bytes = a.W * a.ldT * 4;
+    return __builtin_amdgcn_make_buffer_rsrc(a.B, (short)0, (int)min(bytes, (int64_t)INT_MAX), kRawBufferWord3);
+}
+template <bool DEV>
+__device__ __forceinline__ floatx4 ld_b(const PursuitArgs& a, __amdgpu_buffer_rsrc_t r, int64_t w, int t) {
+    if constexpr (DEV)
+        return __builtin_bit_cast(floatx4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)((w * a.ldT + t) * 4), 0,
+                                                                                 kCoherent));
+    else return *(const floatx4*)(a.B + w * a.ldT + t);
+}
+template <bool DEV>
+__device__ __forceinline__ void st_b(const PursuitArgs& a, __amdgpu_buffer_rsrc_t r, int64_t w, int t, floatx4 v) {
+    if constexpr (DEV)
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(uint32x4, v), r, (int)((w * a.ldT + t) * 4), 0,
+                                               kCoherent);
+    else *(floatx4*)(a.B + w * a.ldT + t) = v;
+}
 
 // Cfmax/imax of column w: plain stores between kernels; inside the persistent pursuit (DEV)
 // device-scope stores of Cfmax, imax and the amplitude b / m the column would have as a peak
@@ -91,43 +195,46 @@ __device__ __forceinline__ void store_score(const PursuitArgs& a, int64_t w, flo
 // full scan (first pass): one wavefront per column, float4 loads (ldT % 4 == 0)
 // ------------------------------------------------------------------------------------------------
 template <int NV, bool DEV = false>   // float4 vectors per lane: NV*256 >= T
-__device__ __forceinline__ void score_column(const PursuitArgs& a, int64_t w, int lane) {
+__device__ __forceinline__ void score_column(const PursuitArgs& a, int64_t w, int lane, const floatx4* nmv,
+                                             const floatx4* rnv) {
     float best = 0.f, bb = 0.f, bm = 1.f;
     int bi = 0;
     if (w >= a.nt && w < a.W - a.nt) {
-        best = -1.f;
-        bi = 0x7fffffff;
-        const float* col = a.B + w * a.ldT;
-        floatx4 b[NV], n[NV];
+        const __amdgpu_buffer_rsrc_t r = b_rsrc(a);
+        floatx4 b[NV];
 #pragma unroll
         for (int i = 0; i < NV; ++i) {
             const int t = 4 * lane + 256 * i;
-            if (t < a.T) {
-                b[i] = *(const floatx4*)(col + t);
-                n[i] = *(const floatx4*)(a.nm + t);
-            }
+            b[i] = (t < a.T) ? ld_b<DEV>(a, r, w, t) : floatx4{0.f, 0.f, 0.f, 0.f};
         }
-#pragma unroll
-        for (int i = 0; i < NV; ++i) {
-            const int t = 4 * lane + 256 * i;
-#pragma unroll
-            for (int e = 0; e < 4; ++e)
-                if (t + e < a.T) {
-                    if constexpr (DEV)
-                        argmax_merge_b(best, bi, bb, bm, score_of(b[i][e], n[i][e]), t + e, b[i][e], n[i][e]);
-                    else argmax_merge(best, bi, score_of(b[i][e], n[i][e]), t + e);
-                }
-        }
-        if constexpr (DEV) wave_argmax_b(best, bi, bb, bm);
-        else wave_argmax(best, bi);
+        rescore<NV, DEV>(b, nmv, rnv, a.T, best, bi, bb, bm);
     }
     if (lane == 0) store_score<DEV>(a, w, best, bi, bb, bm);
 }
 
+// template norms nm and their reciprocals rn in the lanes' float4 layout (1 past T)
+template <int NV>
+__device__ __forceinline__ void load_norms(const PursuitArgs& a, floatx4* nmv, floatx4* rnv) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int i = 0; i < NV; ++i) {
+        const int t = 4 * lane + 256 * i;
+        nmv[i] = (t < a.T) ? *(const floatx4*)(a.nm + t) : floatx4{1.f, 1.f, 1.f, 1.f};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            if (t + e >= a.T) nmv[i][e] = 1.f;
+            rnv[i][e] = 1.f / nmv[i][e];
+        }
+    }
+}
+
+// one wavefront per column, grid-stride over the columns
 template <int NV>
 __global__ __launch_bounds__(256) void score_all_kernel(PursuitArgs a) {
-    const int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (w < a.W) score_column<NV>(a, w, threadIdx.x & 63);
+    floatx4 nmv[NV], rnv[NV];
+    load_norms<NV>(a, nmv, rnv);
+    for (int64_t w = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6); w < a.W; w += (int64_t)gridDim.x * 4)
+        score_column<NV>(a, w, threadIdx.x & 63, nmv, rnv);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -167,22 +274,6 @@ __device__ __forceinline__ bool peak_test(const float* p, float th2, int nt) {
         if (bad) return false;
     }
     return cm > th2 && fabsf(cm - cf) < 1e-9f;
-}
-
-// Branch-free form of the same test for a whole row held in LDS (the persistent pursuit):
-// mx[i] = max(row[i .. i+S-1]) by log2(S) doubling steps (S = the largest power of two <= 2nt+1,
-// so two such windows cover [w-nt, w+nt]); pool is then exact and the reference's literal test
-// applies.
-__device__ __forceinline__ int pow2_window(int nt) {
-    int S = 1;
-    while (2 * S <= 2 * nt + 1) S *= 2;
-    return S;
-}
-
-// j indexes the column in cf/mx (window max of S-runs of cf)
-__device__ __forceinline__ bool peak_pooled(const float* cf, const float* mx, int j, int nt, int S, float th2) {
-    const float pool = fmaxf(mx[j - nt], mx[j + nt - S + 1]);
-    return pool > th2 && fabsf(pool - cf[j]) < 1e-9f;
 }
 
 // peaks_kernel: one thread per column over the whole chip; each wavefront publishes its 64 peak
@@ -356,7 +447,7 @@ __device__ __forceinline__ void covering(const L& l, int s, int n, int c1, int c
 template <int NV, typename List, bool DEV>   // float4 vectors per lane over templates: NV*256 >= T
 __device__ __forceinline__ void update_two(const PursuitArgs& a, const List& l, int n, int wa, int sa, int ca1,
                                            int ca2, bool va, int wb, int sb, int cb1, int cb2, bool vb,
-                                           const floatx4* nmv) {
+                                           const floatx4* nmv, const floatx4* rnv) {
     const int lane = threadIdx.x & 63;
     const int nt = a.nt, L = 2 * nt + 1;
     // two columns per wavefront, loads of both issued before any use
@@ -364,8 +455,9 @@ __device__ __forceinline__ void update_two(const PursuitArgs& a, const List& l, 
     if (va) covering(l, sa, n, ca1, ca2, wa, nt, ea, oa);
     if (vb) covering(l, sb, n, cb1, cb2, wb, nt, eb, ob);
     floatx4 ba[NV], bb[NV], cea[NV], coa[NV], ceb[NV], cob[NV];
-    float* pa = a.B + (int64_t)wa * a.ldT;
-    float* pb = a.B + (int64_t)wb * a.ldT;
+    const __amdgpu_buffer_rsrc_t r = b_rsrc(a);
+#pragma unroll
+    for (int i = 0; i < NV; ++i) ba[i] = bb[i] = floatx4{0.f, 0.f, 0.f, 0.f};
     const float* qea = (ea >= 0) ? a.ctc + ((int64_t)l.t(ea) * L + (wa - l.c(ea) + nt)) * a.ldT : nullptr;
     const float* qoa = (oa >= 0) ? a.ctc + ((int64_t)l.t(oa) * L + (wa - l.c(oa) + nt)) * a.ldT : nullptr;
     const float* qeb = (eb >= 0) ? a.ctc + ((int64_t)l.t(eb) * L + (wb - l.c(eb) + nt)) * a.ldT : nullptr;
@@ -375,12 +467,12 @@ __device__ __forceinline__ void update_two(const PursuitArgs& a, const List& l, 
         const int t = 4 * lane + 256 * i;
         if (t < a.T) {
             if (va) {
-                ba[i] = *(const floatx4*)(pa + t);
+                ba[i] = ld_b<DEV>(a, r, wa, t);
                 if (qea) cea[i] = *(const floatx4*)(qea + t);
                 if (qoa) coa[i] = *(const floatx4*)(qoa + t);
             }
             if (vb) {
-                bb[i] = *(const floatx4*)(pb + t);
+                bb[i] = ld_b<DEV>(a, r, wb, t);
                 if (qeb) ceb[i] = *(const floatx4*)(qeb + t);
                 if (qob) cob[i] = *(const floatx4*)(qob + t);
             }
@@ -388,57 +480,42 @@ __device__ __forceinline__ void update_two(const PursuitArgs& a, const List& l, 
     }
     const float aea = (ea >= 0) ? l.a(ea) : 0.f, aoa = (oa >= 0) ? l.a(oa) : 0.f;
     const float aeb = (eb >= 0) ? l.a(eb) : 0.f, aob = (ob >= 0) ? l.a(ob) : 0.f;
-    const bool ra = va && wa >= nt && wa < a.W - nt;
-    const bool rb = vb && wb >= nt && wb < a.W - nt;
-    float besta = -1.f, bestb = -1.f, ba_ = 0.f, bb_ = 0.f, ma_ = 1.f, mb_ = 1.f;
-    int ia = 0x7fffffff, ib = 0x7fffffff;
 #pragma unroll
     for (int i = 0; i < NV; ++i) {
         const int t = 4 * lane + 256 * i;
-        if (t < a.T) {
-            if (va) {
-                floatx4 x = ba[i];
+        if (va) {
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    float v = x[e];
-                    if (qea) v = v - aea * cea[i][e];
-                    if (qoa) v = v - aoa * coa[i][e];
-                    x[e] = v;
-                    if (ra && t + e < a.T) {
-                        if constexpr (DEV)
-                            argmax_merge_b(besta, ia, ba_, ma_, score_of(v, nmv[i][e]), t + e, v, nmv[i][e]);
-                        else argmax_merge(besta, ia, score_of(v, nmv[i][e]), t + e);
-                    }
-                }
-                *(floatx4*)(pa + t) = x;
+            for (int e = 0; e < 4; ++e) {
+                float v = ba[i][e];
+                if (qea) v = v - aea * cea[i][e];
+                if (qoa) v = v - aoa * coa[i][e];
+                ba[i][e] = v;
             }
-            if (vb) {
-                floatx4 x = bb[i];
+            if (t < a.T) st_b<DEV>(a, r, wa, t, ba[i]);
+        }
+        if (vb) {
 #pragma unroll
-                for (int e = 0; e < 4; ++e) {
-                    float v = x[e];
-                    if (qeb) v = v - aeb * ceb[i][e];
-                    if (qob) v = v - aob * cob[i][e];
-                    x[e] = v;
-                    if (rb && t + e < a.T) {
-                        if constexpr (DEV)
-                            argmax_merge_b(bestb, ib, bb_, mb_, score_of(v, nmv[i][e]), t + e, v, nmv[i][e]);
-                        else argmax_merge(bestb, ib, score_of(v, nmv[i][e]), t + e);
-                    }
-                }
-                *(floatx4*)(pb + t) = x;
+            for (int e = 0; e < 4; ++e) {
+                float v = bb[i][e];
+                if (qeb) v = v - aeb * ceb[i][e];
+                if (qob) v = v - aob * cob[i][e];
+                bb[i][e] = v;
             }
+            if (t < a.T) st_b<DEV>(a, r, wb, t, bb[i]);
         }
     }
-    if (ra) {
-        if constexpr (DEV) wave_argmax_b(besta, ia, ba_, ma_);
-        else wave_argmax(besta, ia);
-        if (lane == 0) store_score<DEV>(a, wa, besta, ia, ba_, ma_);
+    // rescored inside [nt, W - nt) only (the reference's Cf there; the edges never hold a peak)
+    if (va && wa >= nt && wa < a.W - nt) {
+        float best, b_, m_;
+        int bi;
+        rescore<NV, DEV>(ba, nmv, rnv, a.T, best, bi, b_, m_);
+        if (lane == 0) store_score<DEV>(a, wa, best, bi, b_, m_);
     }
-    if (rb) {
-        if constexpr (DEV) wave_argmax_b(bestb, ib, bb_, mb_);
-        else wave_argmax(bestb, ib);
-        if (lane == 0) store_score<DEV>(a, wb, bestb, ib, bb_, mb_);
+    if (vb && wb >= nt && wb < a.W - nt) {
+        float best, b_, m_;
+        int bi;
+        rescore<NV, DEV>(bb, nmv, rnv, a.T, best, bi, b_, m_);
+        if (lane == 0) store_score<DEV>(a, wb, best, bi, b_, m_);
     }
 }
 
@@ -446,27 +523,19 @@ __device__ __forceinline__ void update_two(const PursuitArgs& a, const List& l, 
 // s (c1, c2 = the next two spikes' columns), CPW columns per wavefront.
 template <int NV, typename List, bool DEV, int CPW>
 __device__ __forceinline__ void update_score_columns(const PursuitArgs& a, const List& l, int s, int n, int c1, int c2,
-                                                     int c0, int lo, int w1, const floatx4* nmv) {
+                                                     int c0, int lo, int w1, const floatx4* nmv,
+                                                     const floatx4* rnv) {
     const int wa = c0 + CPW * (int)(threadIdx.x >> 6), wb = wa + 1;
     update_two<NV, List, DEV>(a, l, n, wa, s, c1, c2, wa >= lo && wa <= w1, wb, s, c1, c2,
-                              CPW == 2 && wb >= lo && wb <= w1, nmv);
+                              CPW == 2 && wb >= lo && wb <= w1, nmv, rnv);
 }
 
-template <int NV>
-__device__ __forceinline__ void load_norms(const PursuitArgs& a, floatx4* nmv) {
-    const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-        const int t = 4 * lane + 256 * i;
-        if (t < a.T) nmv[i] = *(const floatx4*)(a.nm + t);
-    }
-}
 
 // Work items q = q0, q0 + stride, ... of a pass's n spikes (col/tmp/amp, ascending columns; global
 // or LDS memory).  A column is written only by the item of the first spike whose window covers it.
 template <int NV, int CPW>
 __device__ __forceinline__ void update_items(const PursuitArgs& a, const ListPlain& l, int n, int64_t q0,
-                                             int64_t stride, const floatx4* nmv) {
+                                             int64_t stride, const floatx4* nmv, const floatx4* rnv) {
     constexpr int kCols = 4 * CPW;
     const int nt = a.nt;
     const int nB = (2 * nt + 1 + kCols - 1) / kCols;
@@ -483,7 +552,7 @@ __device__ __forceinline__ void update_items(const PursuitArgs& a, const ListPla
         const int hi = (int)min((int64_t)ws + nt, a.W - 1);
         const int c0 = ws - nt + sub * kCols;
         const int w1 = min(c0 + kCols - 1, hi);
-        if (max(c0, lo) <= w1) update_score_columns<NV, ListPlain, false, CPW>(a, l, s, n, c1, c2, c0, lo, w1, nmv);
+        if (max(c0, lo) <= w1) update_score_columns<NV, ListPlain, false, CPW>(a, l, s, n, c1, c2, c0, lo, w1, nmv, rnv);
     }
 }
 
@@ -494,10 +563,10 @@ __global__ __launch_bounds__(256) void update_kernel(PursuitArgs a) {
     int n = a.counters[1];
     if (base + n > a.cap) n = a.cap - base;
     if (n > 0) {
-        floatx4 nmv[NV];
-        load_norms<NV>(a, nmv);
+        floatx4 nmv[NV], rnv[NV];
+        load_norms<NV>(a, nmv, rnv);
         update_items<NV, CPW>(a, ListPlain{a.sp_col + base, a.sp_tmp + base, a.sp_amp + base}, n, blockIdx.x,
-                              gridDim.x, nmv);
+                              gridDim.x, nmv, rnv);
     }
     if (a.ts) {
         __builtin_amdgcn_s_waitcnt(0);   // this wavefront's stores issued and done
@@ -533,9 +602,11 @@ __global__ __launch_bounds__(256) void update_kernel(PursuitArgs a) {
 // ------------------------------------------------------------------------------------------------
 constexpr int kPT = 512;                                // threads per persistent workgroup
 constexpr int kPW = kPT / 64;                           // its wavefronts
+// LDS index of row entry i with one pad word per 32 (stride-8 reads conflict-free); pad_len(n) covers [0, n)
+__device__ __forceinline__ int pad32(int i) { return i + (i >> 5); }
+__host__ __device__ constexpr int pad_len(int n) { return n + (n >> 5) + 1; }
 constexpr int kBarrierSpin = 1 << 20;                   // ~1 s of polling
 constexpr int kPersistLoads = (kPersistMaxW + 2 * kPeakHalo + kPT - 1) / kPT;   // row values per thread
-constexpr int kPersistMx = kPT * kPersistLoads + kPeakHalo;                      // window-max buffer
 
 // Grid barrier number `gen` (1, 2, ...) over counters[6] (zero at launch).  Data shared between
 // workgroups is only accessed with device-scope atomics, so waiting for this wavefront's memory
@@ -582,72 +653,176 @@ __device__ __forceinline__ int first_cover(const ListPlain& l, int n, int w, int
     return lo;
 }
 
-// The persistent pursuit's update of its own columns [a0, a1): rounds of 2 * kPW columns, two per
-// wavefront, every load of a round in flight together.  A column a stored spike's window covers
-// gets the update update_items gives it (the last even and last odd covering spike) and, inside
-// [nt, W - nt), is rescored into the next row; every other column carries its values (cf, im, am:
-// the pass's row in LDS, cf indexed by column) into the next row.
+// The persistent pursuit's update, spread evenly over every wavefront of the launch.  The columns
+// the pass changes are the union of its spikes' windows [col - nt, col + nt] within [0, W);
+// spike s adds the run [st_s, en_s], st_s = max(col_s - nt, col_{s-1} + nt + 1, 0),
+// en_s = min(col_s + nt, W - 1), and pfx[s] (exclusive prefix of the run lengths, pfx[n] = K) numbers
+// them in ascending order.  Covered column k goes to wavefront k / 2 mod (G * kPW) of the launch, two
+// per wavefront and round; its first covering spike is the s whose run holds it (no earlier
+// spike's window reaches past en_{s-1}).  Each column gets the update update_items gives it and,
+// inside [nt, W - nt), is rescored into the next row.
+__device__ __forceinline__ int run_start(const ListPlain& l, int s, int nt) {
+    return max(max(l.col[s] - nt, s > 0 ? l.col[s - 1] + nt + 1 : 0), 0);
+}
+
+// pfx[0 .. n] for the first n spikes of the list (n <= kPT * kPersistLoads); every thread returns K.
+__device__ __forceinline__ int covered_prefix(const ListPlain& l, int n, int nt, int W, int* pfx, int* wsum) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int per = (n + kPT - 1) / kPT;
+    const int s0 = tid * per;
+    int loc = 0;
+    for (int q = 0; q < per; ++q) {
+        const int sp = s0 + q;
+        if (sp < n) loc += max(0, min(l.col[sp] + nt, W - 1) - run_start(l, sp, nt) + 1);
+    }
+    int inc = loc;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(inc, off, 64);
+        if (lane >= off) inc += v;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    int woff = 0, K = 0;
+#pragma unroll
+    for (int k = 0; k < kPW; ++k) {
+        const int v = wsum[k];
+        if (k < wave) woff += v;
+        K += v;
+    }
+    int run = woff + inc - loc;
+    for (int q = 0; q < per; ++q) {
+        const int sp = s0 + q;
+        if (sp < n) {
+            pfx[sp] = run;
+            run += max(0, min(l.col[sp] + nt, W - 1) - run_start(l, sp, nt) + 1);
+        }
+    }
+    if (tid == 0) pfx[n] = K;
+    __syncthreads();
+    return K;
+}
+
+// the covered column numbered k: its column and first covering spike (pfx[s] <= k < pfx[s + 1])
+__device__ __forceinline__ int covered_column(const ListPlain& l, const int* pfx, int n, int nt, int k, int* s_out) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {   // last s with pfx[s] <= k
+        const int mid = (lo + hi + 1) >> 1;
+        if (pfx[mid] <= k) lo = mid;
+        else hi = mid - 1;
+    }
+    *s_out = lo;
+    return run_start(l, lo, nt) + (k - pfx[lo]);
+}
+
 template <int NV>
-__device__ __forceinline__ void update_own_columns(const PursuitArgs& a, const ListPlain& l, int n, int a0, int a1,
-                                                   const float* cf, const int* im, const float* am,
-                                                   const floatx4* nmv) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int nt = a.nt, W = (int)a.W;
-    for (int c0 = a0; c0 < a1; c0 += 2 * kPW) {
-        const int wa = c0 + 2 * wave, wb = wa + 1;
-        const int sa = first_cover(l, n, wa, nt), sb = first_cover(l, n, wb, nt);
-        const bool va = wa < a1 && sa < n && l.col[sa] - nt <= wa;
-        const bool vb = wb < a1 && sb < n && l.col[sb] - nt <= wb;
+__device__ __forceinline__ void update_covered(const PursuitArgs& a, const ListPlain& l, int n, const int* pfx, int K,
+                                               const floatx4* nmv, const floatx4* rnv) {
+    const int nt = a.nt;
+    const int gw = (int)blockIdx.x * kPW + (int)(threadIdx.x >> 6), nw = (int)gridDim.x * kPW;
+    for (int k0 = 2 * gw; k0 < K; k0 += 2 * nw) {
+        int sa = 0, sb = 0;
+        const int wa = covered_column(l, pfx, n, nt, k0, &sa);
+        const bool vb = k0 + 1 < K;
+        const int wb = vb ? covered_column(l, pfx, n, nt, k0 + 1, &sb) : wa;
         const int ca1 = (sa + 1 < n) ? l.col[sa + 1] : INT_MAX / 2, ca2 = (sa + 2 < n) ? l.col[sa + 2] : INT_MAX / 2;
         const int cb1 = (sb + 1 < n) ? l.col[sb + 1] : INT_MAX / 2, cb2 = (sb + 2 < n) ? l.col[sb + 2] : INT_MAX / 2;
-        update_two<NV, ListPlain, true>(a, l, n, wa, sa, ca1, ca2, va, wb, sb, cb1, cb2, vb, nmv);
-        if (lane == 0) {
-            if (wa < a1 && !(va && wa >= nt && wa < W - nt)) {
-                st_dev(a.cfmax + wa, cf[wa]);
-                st_dev(a.imax + wa, im[wa]);
-                st_dev(a.amprow + wa, am[wa]);
-            }
-            if (wb < a1 && !(vb && wb >= nt && wb < W - nt)) {
-                st_dev(a.cfmax + wb, cf[wb]);
-                st_dev(a.imax + wb, im[wb]);
-                st_dev(a.amprow + wb, am[wb]);
-            }
+        update_two<NV, ListPlain, true>(a, l, n, wa, sa, ca1, ca2, true, wb, sb, cb1, cb2, vb, nmv, rnv);
+    }
+}
+
+// The columns [a0, a1) the pass leaves unrescored (uncovered, or covered within nt of an edge) carry
+// their values (cf, im, am: the pass's row in LDS; cf padded, with its halo) into the next row.
+__device__ __forceinline__ void carry_own_columns(const PursuitArgs& a, const ListPlain& l, int n, int a0, int a1,
+                                                  const float* cf, const int* im, const float* am) {
+    const int nt = a.nt, W = (int)a.W;
+    for (int w = a0 + (int)threadIdx.x; w < a1; w += kPT) {
+        const int s = first_cover(l, n, w, nt);
+        const bool cov = s < n && l.col[s] - nt <= w;
+        if (!(cov && w >= nt && w < W - nt)) {
+            st_dev(a.cfmax + w, cf[pad32(kPeakHalo + w)]);
+            st_dev(a.imax + w, im[w]);
+            st_dev(a.amprow + w, am[w]);
         }
     }
 }
 
-// window_max_pow2 with two buffers (one barrier per step): src holds the row with a -inf tail of
-// kPeakHalo values past NT * MAXR, tmp the same tail; returns the buffer holding the result.
-template <int NT, int MAXR>
-__device__ __forceinline__ float* window_max_pow2_pp(float* src, float* tmp, int S) {
-    float* dst = tmp;
-    for (int s = 1; s < S; s <<= 1) {
+// The persistent pursuit's peak test of one thread's 8 columns w = 8t + q (q < 8, w < W): the
+// window maximum pool_q over [w - nt, w + nt] (-inf outside the row) and the reference's test
+// pool > Th^2, |pool - cf| < 1e-9 (max-pool equality), as bits q of the result.  The row cf is
+// in LDS at pad(H + column) (one pad word per 32: the stride-8 reads below are conflict-free) and
+// bm[b + kBmHalo] holds the maximum of block b (columns 8b .. 8b + 7).  For nt >= 7 every window of
+// the 8 columns contains C = [8t + 7 - nt, 8t + nt]: its maximum comes from the whole blocks inside
+// it and the two partial runs at its ends; column q adds head[q .. 6] (head[k] = column
+// 8t - nt + k) and tail[0 .. q - 1] (tail[k] = column 8t + nt + 1 + k).  Smaller nt scans directly.
+constexpr int kBmHalo = 17;   // blocks of -inf kept on each side of the row (nt <= 127)
+__device__ __forceinline__ uint32_t peak_bits8(const float* cf, const float* bm, int t, int W, int nt, float th2) {
+    const int H = kPeakHalo, c0 = 8 * t;
+    float pool[8];
+    if (nt >= 7) {
+        const int bl = -((nt - 7) >> 3), bh = ((nt + 1) >> 3) - 1;
+        const int ph = (nt - 7) & 7, pt = (nt + 1) & 7;
+        float cm = -INFINITY;
+        for (int b = t + bl; b <= t + bh; ++b) cm = fmaxf(cm, bm[b + kBmHalo]);
+        const int s0 = c0 + 7 - nt, e1 = 8 * (t + bh + 1);
+        for (int k = 0; k < ph; ++k) cm = fmaxf(cm, cf[pad32(H + s0 + k)]);
+        for (int k = 0; k < pt; ++k) cm = fmaxf(cm, cf[pad32(H + e1 + k)]);
+        float hv[7], tv[7];
 #pragma unroll
-        for (int r = 0; r < MAXR; ++r) {
-            const int i = threadIdx.x + NT * r;
-            dst[i] = fmaxf(src[i], src[i + s]);
+        for (int k = 0; k < 7; ++k) {
+            hv[k] = cf[pad32(H + c0 - nt + k)];
+            tv[k] = cf[pad32(H + c0 + nt + 1 + k)];
         }
-        __syncthreads();
-        float* t = src;
-        src = dst;
-        dst = t;
+        float suf = -INFINITY, pre = -INFINITY;
+        float hs[8];
+        hs[7] = -INFINITY;
+#pragma unroll
+        for (int k = 6; k >= 0; --k) {
+            suf = fmaxf(suf, hv[k]);
+            hs[k] = suf;
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            pool[q] = fmaxf(cm, fmaxf(hs[q], pre));
+            if (q < 7) pre = fmaxf(pre, tv[q]);
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            float m = -INFINITY;
+            for (int d = -nt; d <= nt; ++d) m = fmaxf(m, cf[pad32(H + c0 + q + d)]);
+            pool[q] = m;
+        }
     }
-    return src;
+    uint32_t bits = 0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const float c = cf[pad32(H + c0 + q)];
+        if (c0 + q < W && pool[q] > th2 && fabsf(pool[q] - c) < 1e-9f) bits |= 1u << q;
+    }
+    return bits;
 }
+
+// LDS of the persistent pursuit for windows of W columns (floats/ints): the padded row with its
+// halos and a -inf tail of 16, the block maxima with their halos, the covered-run prefix, imax,
+// amplitude and the pass's list (column, template, amplitude)
+__host__ __device__ constexpr int persist_cf_len(int W) { return pad_len(W + 2 * kPeakHalo + 16); }
+__host__ __device__ constexpr int persist_bm_len(int W) { return (W + 7) / 8 + 2 * kBmHalo; }
 
 template <int NV>
 __global__ __launch_bounds__(kPT) void pursuit_persistent_kernel(PursuitArgs a, int first_pass, int max_passes) {
-    // cf [W + 2H] | mx, mx2 [kPersistMx] | im [W] | am [W] | list col, tmp, amp [W]
+    // cf [persist_cf_len] | bm [persist_bm_len] | pfx [W + 1] | im [W] | am [W] | list col, tmp, amp [W]
     extern __shared__ float psm[];
     __shared__ int wsum[kPW];
     __shared__ int s_ok;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int W = (int)a.W, H = kPeakHalo;
     const int N = W + 2 * H;
+    const int nbk = (W + 7) / 8;   // blocks of 8 columns, one per thread
     float* cf = psm;
-    float* mx = cf + N;
-    float* mx2 = mx + kPersistMx;
-    int* im = (int*)(mx2 + kPersistMx);
+    float* bm = cf + persist_cf_len(W);
+    int* pfx = (int*)(bm + persist_bm_len(W));
+    int* im = pfx + W + 1;
     float* am = (float*)(im + W);
     int* lcol = (int*)(am + W);
     int* ltmp = lcol + W;
@@ -656,16 +831,16 @@ __global__ __launch_bounds__(kPT) void pursuit_persistent_kernel(PursuitArgs a, 
     // this workgroup's columns: kColsB-aligned runs, G of them cover [0, W)
     const int Wc = (W + (int)gridDim.x * kColsB - 1) / ((int)gridDim.x * kColsB) * kColsB;
     const int a0 = min(W, (int)blockIdx.x * Wc), a1 = min(W, a0 + Wc);
-    floatx4 nmv[NV];
-    load_norms<NV>(a, nmv);
+    floatx4 nmv[NV], rnv[NV];
+    load_norms<NV>(a, nmv, rnv);
     int par = first_pass & 1;
     PursuitArgs cur = a;
     use_row(cur, a, par);
     int base = 0, np = 0;
     bool done = false;
     if (first_pass == 0) {
-        // first pass: own columns scored, one wavefront per column
-        for (int w = a0 + wave; w < a1; w += kPW) score_column<NV, true>(cur, w, lane);
+        // first pass: every column scored, one wavefront per column over the whole launch
+        for (int w = (int)blockIdx.x * kPW + wave; w < W; w += (int)gridDim.x * kPW) score_column<NV, true>(cur, w, lane, nmv, rnv);
     } else {
         // after first_pass passes of the per-pass kernels (their last pass committed as the peak
         // kernel commits it): own columns' Cfmax/imax as they left them, with their amplitudes
@@ -674,14 +849,17 @@ __global__ __launch_bounds__(kPT) void pursuit_persistent_kernel(PursuitArgs a, 
         done = a.counters[3] != 0;
         for (int w = a0 + tid; w < a1; w += kPT) {
             const int i = a.imax[w];
-            store_score<true>(cur, w, a.cfmax[w], i, a.B[(int64_t)w * a.ldT + i], a.nm[i]);
+            store_score<true>(cur, w, a.cfmax[w], i, ld_dev(a.B + (int64_t)w * a.ldT + i), a.nm[i]);
         }
     }
-    if (tid < kPeakHalo) mx2[kPT * kPersistLoads + tid] = -INFINITY;   // tail of the second buffer
+    // constant -inf parts: the row's tail past N and the block halos
+    if (tid < 16) cf[pad32(N + tid)] = -INFINITY;
+    if (tid < kBmHalo) {
+        bm[tid] = -INFINITY;
+        bm[kBmHalo + nbk + tid] = -INFINITY;
+    }
     int gen = 1;
     if (!grid_barrier(a.counters, gen++, &s_ok)) return;
-    const int R = ((W + kPT - 1) / kPT) | 1;   // columns per thread, odd: conflict-free LDS strides
-    const int S = pow2_window(a.nt);
     for (int pass = first_pass; pass < max_passes && !done; ++pass) {
         {
             // the row (with -inf halos) into LDS, every load in flight at once
@@ -698,23 +876,22 @@ __global__ __launch_bounds__(kPT) void pursuit_persistent_kernel(PursuitArgs a, 
 #pragma unroll
             for (int r = 0; r < kPersistLoads; ++r) {
                 const int i = tid + kPT * r, w = i - H;
-                if (i < N) cf[i] = v[r];
-                mx[i] = v[r];   // -inf past the row
+                if (i < N) cf[pad32(i)] = v[r];
                 if (w >= 0 && w < W) {
                     im[w] = m[r];
                     am[w] = b[r];
                 }
             }
-            if (tid < kPeakHalo) mx[kPT * kPersistLoads + tid] = -INFINITY;
         }
         __syncthreads();
-        const float* wmx = window_max_pow2_pp<kPT, kPersistLoads>(mx, mx2, S);
-        uint32_t bits = 0;
-        for (int q = 0; q < R; ++q) {
-            const int w = tid * R + q;
-            const bool pk = peak_pooled(cf, wmx, H + min(w, W - 1), a.nt, S, a.th2);
-            if (pk && w < W) bits |= 1u << q;
+        if (tid < nbk) {
+            float m = cf[pad32(H + 8 * tid)];
+#pragma unroll
+            for (int k = 1; k < 8; ++k) m = fmaxf(m, cf[pad32(H + 8 * tid + k)]);
+            bm[kBmHalo + tid] = m;
         }
+        __syncthreads();
+        uint32_t bits = tid < nbk ? peak_bits8(cf, bm, tid, W, a.nt, a.th2) : 0u;
         const int cnt = __popc(bits);
         int inc = cnt;
 #pragma unroll
@@ -735,7 +912,7 @@ __global__ __launch_bounds__(kPT) void pursuit_persistent_kernel(PursuitArgs a, 
         while (bits) {
             const int q = __ffs(bits) - 1;
             bits &= bits - 1;
-            const int w = tid * R + q;
+            const int w = tid * 8 + q;
             const int t = im[w];
             const float amp = am[w];
             lcol[pos] = w;
@@ -762,7 +939,10 @@ __global__ __launch_bounds__(kPT) void pursuit_persistent_kernel(PursuitArgs a, 
         const int nu = max(0, min(n, a.cap - base));   // spikes stored, as update_kernel applies them
         PursuitArgs nxt = a;
         use_row(nxt, a, par ^ 1);
-        update_own_columns<NV>(nxt, ListPlain{lcol, ltmp, lamp}, nu, a0, a1, cf + H, im, am, nmv);
+        const ListPlain lst{lcol, ltmp, lamp};
+        const int K = covered_prefix(lst, nu, a.nt, W, pfx, wsum);
+        carry_own_columns(nxt, lst, nu, a0, a1, cf, im, am);
+        update_covered<NV>(nxt, lst, nu, pfx, K, nmv, rnv);
         base += n;
         par ^= 1;
         cur = nxt;
@@ -901,7 +1081,7 @@ __global__ __launch_bounds__(256) void residual_kernel(PursuitArgs a) {
 
 // ------------------------------------------------------------------------------------------------
 hipError_t launch_score_all(const PursuitArgs& a, hipStream_t s) {
-    unsigned nb = (unsigned)((a.W + 3) / 4);
+    const unsigned nb = (unsigned)std::min<int64_t>((a.W + 3) / 4, 4096);   // 4 columns per workgroup and round
     const int nv = (a.T + 255) / 256;
     switch (nv) {
 #define LSS_SA(V) case V: hipLaunchKernelGGL(score_all_kernel<V>, dim3(nb), dim3(256), 0, s, a); break;
@@ -978,8 +1158,9 @@ hipError_t launch_update(const PursuitArgs& a, int grid, hipStream_t s, hipEvent
 }
 
 size_t persistent_lds(int64_t W) {
-    return sizeof(float) * (W + 2 * kPeakHalo + 2 * kPersistMx) + (sizeof(int) + sizeof(float)) * W +
-           (2 * sizeof(int) + sizeof(float)) * W;
+    const int w = (int)W;
+    return sizeof(float) * (persist_cf_len(w) + persist_bm_len(w)) + sizeof(int) * (w + 1) +
+           (sizeof(int) + sizeof(float)) * w + (2 * sizeof(int) + sizeof(float)) * w;
 }
 
 typedef void (*PersistentFn)(PursuitArgs, int, int);

@@ -3,7 +3,9 @@ reads its full 128-column halo, the projection its widest band and the pursuit 2
 
 * nt = 127, the pursuit given identical FP32 scores, residual and tables: bit for bit against the
   oracle's run_matching (ks/template_matching.py:157-198) on an 8254-sample window (per-pass
-  kernels) and a 4000-sample window (persistent kernel, and two per-pass rounds before it);
+  kernels) and a 4000-sample window (persistent kernel, and two per-pass rounds before it); the
+  same on 4000-sample windows at nt = 3 (the persistent kernel's direct window scan), 7 and 9 (its
+  block-maximum scan with no and with partial runs at the ends);
 * nt = 127 and nt = 3, the whole window: FP32 against the FP64 oracle (spike F1 >= 0.999 within
   +-1 sample and same template, matched amplitudes within 1e-3 relative) and bf16x3 against FP32
   (the same bar)."""
@@ -17,7 +19,8 @@ pytestmark = pytest.mark.gpu
 
 DEV = "cuda:0"
 NT = 8000
-LIMITS = {127: (42, 6), 3: (1, 3)}   # nt: (nt0min, PCs)
+LIMITS = {127: (42, 6), 3: (1, 3), 7: (2, 3), 9: (3, 3)}   # nt: (nt0min, PCs)
+_SYNTH = {}
 
 
 def _synth(nt):
@@ -27,9 +30,15 @@ def _synth(nt):
                                             n_pcs=K, rate_lo=20.0, rate_hi=60.0, seed=11))
 
 
+def _cached(nt):
+    if nt not in _SYNTH:
+        _SYNTH[nt] = _synth(nt)
+    return _SYNTH[nt]
+
+
 @pytest.fixture(scope="module")
 def s127():
-    return _synth(127)
+    return _cached(127)
 
 
 def _amp_err(ref_s, ref_t, ref_a, got):
@@ -39,13 +48,14 @@ def _amp_err(ref_s, ref_t, ref_a, got):
     return (max(err) if err else 0.0), len(err)
 
 
-@pytest.mark.parametrize("W,grid,first", [(8254, 0, 0), (4000, 128, 0), (4000, 64, 2)])
-def test_pursuit_bit_exact_nt127(s127, W, grid, first):
+@pytest.mark.parametrize("nt,W,grid,first", [(127, 8254, 0, 0), (127, 4000, 128, 0), (127, 4000, 64, 2),
+                                             (3, 4000, 128, 0), (7, 4000, 128, 0), (9, 4000, 64, 2)])
+def test_pursuit_bit_exact_nt_limits(nt, W, grid, first):
     from livespikesorter_amd.sorter import Sorter
-    print(f"case: nt=127 pursuit bit-exact W={W} persistent grid {grid} after {first}", flush=True)
-    a = s127.art
-    nt = 127
-    X = ko.padded_batch(s127.rec, 1, NT, nt, torch.float64)
+    print(f"case: nt={nt} pursuit bit-exact W={W} persistent grid {grid} after {first}", flush=True)
+    sy = _cached(nt)
+    a = sy.art
+    X = ko.padded_batch(sy.rec, 1, NT, nt, torch.float64)
     ws = ko.window_start(1, NT, nt)
     Wm = torch.as_tensor(a["drift"]).double() @ torch.as_tensor(a["Wrot"]).double()
     x32 = ko.filter_batch(X, torch.from_numpy(a["hp_filter"]).double(), Wm).float()
@@ -78,7 +88,7 @@ def test_pursuit_bit_exact_nt127(s127, W, grid, first):
 def test_window_at_nt_limit(s127, nt):
     from livespikesorter_amd.sorter import Sorter
     print(f"case: nt={nt} whole window fp32 vs fp64 oracle, bf16x3 vs fp32", flush=True)
-    s = s127 if nt == 127 else _synth(nt)
+    s = s127 if nt == 127 else _cached(nt)
     a = s.art
     X = ko.padded_batch(s.rec, 1, NT, nt, torch.float64)
     win = np.ascontiguousarray(X.numpy().T.astype(np.int16))
