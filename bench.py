@@ -180,8 +180,8 @@ def bench_stream(args):
     from livespikesorter_amd._lib import lib as _l
     from livespikesorter_amd.metrics import f1_same_template
     from livespikesorter_amd.sorter import Sorter
-    from livespikesorter_amd.streaming import (StreamingSorter, autotune, autotune_persistent, batcher_grid_shares,
-                                               update_columns)
+    from livespikesorter_amd.streaming import (StreamingSorter, autotune, autotune_persistent, autotune_stream,
+                                               batcher_grid_shares, update_columns)
 
     batch, fs = 3000, 30000.0
     n = int(args.seconds * fs) + 2 * batch
@@ -233,6 +233,14 @@ def bench_stream(args):
                        "persistent_ms": {f"{k[0]}/{k[1]}": round(v, 3) for k, v in pms.items()}})
     dev = torch.device("cuda:0")
     drecs = [torch.from_numpy(p.rec).to(dev) for p in probes]     # recordings resident in HBM
+    stream_tuning = None
+    if pinned is None:
+        # the probes' schedules chosen together, on whole streaming steps
+        joint, joint_ms = autotune_stream(sorters, drecs, batch)
+        for tu, (g, f) in zip(tuning, joint):
+            tu["persistent_grid"], tu["persistent_after"] = g, f
+        stream_tuning = {"chosen": [list(c) for c in joint],
+                         "step_ms": {"/".join(f"{g}:{f}" for g, f in k): round(v, 4) for k, v in joint_ms.items()}}
     w0 = [torch.from_numpy(window(p.rec, 0, p.art["nt"])).to(dev) for p in probes]
 
     def dwindow(i, k):
@@ -411,6 +419,7 @@ def bench_stream(args):
                        "loss_ratio_after_before": [[round(x["loss_after"] / x["loss_before"], 6) if x["loss_before"]
                                                     else None for x in rr] for rr in refines]},
         "tuning": tuning,
+        "stream_tuning": stream_tuning,
         "cpu_baseline": cpu,
     }
     for s_ in sorters + plain_sorters + host_sorters:

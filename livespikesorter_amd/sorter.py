@@ -266,6 +266,12 @@ class Sorter:
         the rest in the persistent kernel of `grid` workgroups (grid 0 = per-pass kernels only)."""
         check(lib().lss_set_persistent(self._h, int(grid), int(first_passes)), "lss_set_persistent")
 
+    def tuning(self):
+        """(update grid, first-segment pass count) of the per-pass pursuit (lss_autotune / lss_set_tuning)."""
+        g, f = ct.c_int32(), ct.c_int32()
+        check(lib().lss_get_tuning(self._h, ct.byref(g), ct.byref(f)), "lss_get_tuning")
+        return g.value, f.value
+
     def persistent(self, W: int):
         """(grid, first_passes, device co-resident limit for windows of W samples; 0 if W is too long)."""
         g, f, m = ct.c_int32(), ct.c_int32(), ct.c_int32()

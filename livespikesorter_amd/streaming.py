@@ -159,3 +159,56 @@ def batcher_grid_shares(sorters: list[Sorter], W: int) -> list[int]:
     w = [float(np.sqrt(s.T * s.C)) for s in sorters]
     tot = sum(w)
     return [max(1, int(np.floor(s.persistent(W)[2] * wi / tot))) for s, wi in zip(sorters, w)]
+
+
+def autotune_stream(sorters: list[Sorter], recordings, batch: int = 3000, steps: int = 12, warmup: int = 3,
+                    options=None):
+    """Picks every probe's pursuit schedule jointly, by timing whole streaming steps: probes share
+    the device inside a step, so the schedule each is fastest with alone (autotune_persistent) is
+    not always the fastest together (a persistent pursuit keeps its workgroups for the whole
+    pursuit, per-pass kernels take whatever the other probes leave free).  options[i] lists probe
+    i's candidate (persistent grid, first per-pass rounds) pairs; by default the probe's current
+    setting and (its grid, the first-segment pass count autotune chose).  Every combination (at most
+    64) runs `warmup` + `steps` graph-replayed steps of a fresh batcher over the device recordings
+    (torch int16 [n][C], one per probe, each at least (warmup + steps + 1) * batch + nt samples); the
+    combination with the lowest median step time is set on the sorters.  Results do not depend on
+    the schedule.  Returns (chosen [(grid, first)], {combination: median ms})."""
+    import itertools
+    import statistics
+    import torch
+
+    if options is None:
+        options = []
+        for s in sorters:
+            g, f, _ = s.persistent(batch + 2 * s.nt)
+            _, p1 = s.tuning()
+            opts = [(g, f)]
+            if p1 > 0 and (g, p1) not in opts:
+                opts.append((g, p1))
+            options.append(opts)
+    combos = list(itertools.product(*options))
+    if len(combos) > 64:
+        raise ValueError(f"{len(combos)} combinations: pass options with at most 64")
+    need = (warmup + steps + 1) * batch
+    for s, r in zip(sorters, recordings):
+        if r.shape[0] < need + 2 * s.nt:
+            raise ValueError(f"a recording of {r.shape[0]} samples is shorter than the {need + 2 * s.nt} needed")
+    table = {}
+    for combo in combos:
+        for s, (g, f) in zip(sorters, combo):
+            s.set_persistent(g, f)
+        times = []
+        with StreamingSorter(sorters, batch=batch, use_graph=True) as st:
+            for i, r in enumerate(recordings):
+                st.push(i, r[:need + 2 * sorters[i].nt])
+            for k in range(warmup + steps):
+                torch.cuda.synchronize()
+                if st.step() is None:
+                    raise RuntimeError("autotune_stream: a step found samples missing")
+                if k >= warmup:
+                    times.append(st.last_ms())
+        table[combo] = statistics.median(times)
+    best = min(table, key=table.get)
+    for s, (g, f) in zip(sorters, best):
+        s.set_persistent(g, f)
+    return list(best), table

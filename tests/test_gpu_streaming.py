@@ -252,3 +252,29 @@ def test_refinement_rounds_shorter_than_probe_count(probes):
         assert inf["loss_before"] > 0 and inf["loss_after"] < inf["loss_before"]
     for s in sorters:
         s.close()
+
+
+def test_joint_schedule_tuning_is_result_neutral(probes):
+    """autotune_stream times every combination of the probes' candidate schedules on whole steps
+    and keeps the fastest; a stream on the chosen schedules returns the spikes of the default one."""
+    print("[lss] test_joint_schedule_tuning_is_result_neutral", flush=True)
+    from livespikesorter_amd.streaming import autotune, autotune_stream
+    sorters = _make_sorters(probes, precision="bf16x3")
+    ref = _stream(probes, sorters, True)
+    drecs = [torch.from_numpy(p.rec).to("cuda:0") for p in probes]
+    for s, p in zip(sorters, probes):
+        autotune(s, torch.from_numpy(_window(p.rec, BATCH, p.art["nt"])).to("cuda:0"))
+    options = [[(0, 0), (64, 0), (64, s.tuning()[1])] for s in sorters]
+    best, table = autotune_stream(sorters, drecs, batch=BATCH, steps=2, warmup=1, options=options)
+    print(f"[lss]   chosen {best}; {table}", flush=True)
+    assert len(table) == 9 and all(v > 0 for v in table.values())
+    assert all(tuple(b) in o for b, o in zip(best, options))
+    assert [s.persistent(BATCH + 2 * s.nt)[:2] for s in sorters] == [tuple(b) for b in best]
+    got = _stream(probes, sorters, True)
+    for i in range(len(probes)):
+        for g, r in zip(got[i], ref[i]):
+            np.testing.assert_array_equal(g.sample, r.sample)
+            np.testing.assert_array_equal(g.template, r.template)
+            np.testing.assert_array_equal(g.amplitude, r.amplitude)
+    for s in sorters:
+        s.close()
