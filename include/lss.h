@@ -52,7 +52,7 @@ typedef enum lss_precision {
  * dt: src/SpikeSorter/OnlineSpikesV2.h:84-99; misc.txt keys OnlineSpikesV2.cu:435-457). */
 typedef struct lss_params {
     int32_t n_chan;        /* C */
-    int32_t n_templates;   /* T */
+    int32_t n_templates;   /* T (<= 2048; above 1024 the pursuit always runs on per-pass kernels) */
     int32_t n_pcs;         /* K (<= 8) */
     int32_t nt;            /* template length M (odd, <= 127) */
     int32_t nt0min;        /* trough sample of the template window */

@@ -159,7 +159,7 @@ struct Handle {
 
 lss_status check_params(const lss_params& p) {
     if (p.n_chan < 1 || p.n_chan > 1024) return fail(LSS_ERR_ARG, "n_chan must be in [1, 1024]");
-    if (p.n_templates < 1 || p.n_templates > 1024) return fail(LSS_ERR_ARG, "n_templates must be in [1, 1024]");
+    if (p.n_templates < 1 || p.n_templates > 2048) return fail(LSS_ERR_ARG, "n_templates must be in [1, 2048]");
     if (p.n_pcs < 1 || p.n_pcs > 8) return fail(LSS_ERR_ARG, "n_pcs must be in [1, 8]");
     if (p.nt < 3 || p.nt > 127 || (p.nt % 2) == 0) return fail(LSS_ERR_ARG, "nt must be odd in [3, 127]");
     if (p.nt0min < 0 || p.nt0min >= p.nt) return fail(LSS_ERR_ARG, "nt0min must be in [0, nt)");
@@ -497,7 +497,7 @@ lss_status enqueue_passes(Handle& h, const lss::PursuitArgs& a, int p0, int p1) 
 }
 
 bool use_persistent(const Handle& h, int64_t W) {
-    return h.pers_grid > 0 && W <= lss::kPersistMaxW && !h.prof_update;
+    return h.pers_grid > 0 && W <= lss::kPersistMaxW && !h.prof_update && lss::persistent_supports(h.T);
 }
 
 // The pursuit of one window: short windows run their first pers_first passes (the wide ones) as

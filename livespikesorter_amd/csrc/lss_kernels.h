@@ -89,7 +89,7 @@ struct PursuitArgs {
 };
 constexpr int kCounters = 16;
 hipError_t launch_reset_counters(int* counters, hipStream_t s);        // counters[0..kCounters) = 0
-hipError_t launch_score_all(const PursuitArgs& a, hipStream_t s);     // T <= 1024
+hipError_t launch_score_all(const PursuitArgs& a, hipStream_t s);     // T <= 2048
 hipError_t launch_peaks(const PursuitArgs& a, hipStream_t s);
 // grid = workgroups (grid-stride over the pass's work items; any grid gives the same result);
 // start/stop (measurement mode): events recorded around the launch; cpw = score columns per
@@ -107,8 +107,10 @@ hipError_t launch_stamp_span(const unsigned long long* ts, int launches, int gri
 constexpr int kPersistMaxW = 4096;
 hipError_t launch_pursuit_persistent(const PursuitArgs& a, int grid, int first_pass, int max_passes,
                                      hipStream_t s);
-// Workgroups of the persistent pursuit that the device holds at once for T templates, window W.
+// Workgroups of the persistent pursuit that the device holds at once for T templates, window W (0
+// when T > 1024: those pursuits run on the per-pass kernels)
 hipError_t persistent_max_grid(int T, int64_t W, int* grid);
+bool persistent_supports(int T);
 // residual of all passes in pass order (even, then odd spikes of a pass), after the last pass
 hipError_t launch_residual(const PursuitArgs& a, hipStream_t s);
 

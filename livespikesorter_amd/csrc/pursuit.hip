@@ -1085,7 +1085,7 @@ hipError_t launch_score_all(const PursuitArgs& a, hipStream_t s) {
     const int nv = (a.T + 255) / 256;
     switch (nv) {
 #define LSS_SA(V) case V: hipLaunchKernelGGL(score_all_kernel<V>, dim3(nb), dim3(256), 0, s, a); break;
-        LSS_SA(1) LSS_SA(2) LSS_SA(3) LSS_SA(4)
+        LSS_SA(1) LSS_SA(2) LSS_SA(3) LSS_SA(4) LSS_SA(5) LSS_SA(6) LSS_SA(7) LSS_SA(8)
 #undef LSS_SA
         default: return hipErrorInvalidValue;
     }
@@ -1146,12 +1146,17 @@ hipError_t launch_update(const PursuitArgs& a, int grid, hipStream_t s, hipEvent
     const int nv = (a.T + 255) / 256;
     if (grid < 1) grid = kUpdateGrid;
     if (cpw != 1 && cpw != 2) return hipErrorInvalidValue;
+    if (nv > 4) cpw = 1;   // more than 1024 templates: one score column per wavefront (registers)
     switch (nv * 2 + cpw - 1) {
 #define LSS_UP(V)                                                            \
     case 2 * V: update_launch<V, 1>(a, grid, s, start, stop); break;         \
     case 2 * V + 1: update_launch<V, 2>(a, grid, s, start, stop); break;
         LSS_UP(1) LSS_UP(2) LSS_UP(3) LSS_UP(4)
 #undef LSS_UP
+        case 10: update_launch<5, 1>(a, grid, s, start, stop); break;
+        case 12: update_launch<6, 1>(a, grid, s, start, stop); break;
+        case 14: update_launch<7, 1>(a, grid, s, start, stop); break;
+        case 16: update_launch<8, 1>(a, grid, s, start, stop); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1174,9 +1179,13 @@ PersistentFn persistent_fn(int T) {
     }
 }
 
+bool persistent_supports(int T) { return persistent_fn(T) != nullptr; }
+
 hipError_t persistent_max_grid(int T, int64_t W, int* grid) {
     const PersistentFn fn = persistent_fn(T);
-    if (!fn || W < 1 || W > kPersistMaxW) return hipErrorInvalidValue;
+    *grid = 0;
+    if (!fn) return hipSuccess;   // more than 1024 templates: per-pass kernels only
+    if (W < 1 || W > kPersistMaxW) return hipErrorInvalidValue;
     int dev = 0, cus = 0, per = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
