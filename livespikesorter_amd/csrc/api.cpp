@@ -596,8 +596,8 @@ lss_status check_counts(Handle& h, const int* c, int* n, int* passes) {
     if (c[7]) return fail(LSS_ERR_STATE, kBarrierTimeout);
     if (c[2]) return fail(LSS_ERR_CAPACITY, "spike capacity (max_spikes) exceeded");
     if (c[5])
-        return fail(LSS_ERR_CAPACITY, "more than 1024 spikes of all passes reach one 64-sample block "
-                                      "(residual list); the residual is incomplete");
+        return fail(LSS_ERR_CAPACITY, "more than 1024 spikes of a single pass (tied peaks) reach one 16-column "
+                                      "block of the residual; the residual is incomplete");
     *n = c[0] + c[1];
     *passes = c[4];
     if (h.prof_update) {
@@ -1745,7 +1745,9 @@ lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready) {
         const int* c = h.hs_counters;
         if (c[7]) return fail(LSS_ERR_STATE, kBarrierTimeout);
         if (c[2]) return fail(LSS_ERR_CAPACITY, "spike capacity (max_spikes) exceeded");
-        if (c[5]) return fail(LSS_ERR_CAPACITY, "residual list overflow");
+        if (c[5])
+            return fail(LSS_ERR_CAPACITY, "more than 1024 spikes of a single pass (tied peaks) reach one 16-column "
+                                          "block of the residual; the residual is incomplete");
         const int n = c[0] + c[1];
         const int64_t ws = b.t0 - h.nt;
         lss_status st = (n <= h.stage_cap)

@@ -77,7 +77,7 @@ struct PursuitArgs {
     int* prow_im;
     float* prow_amp;
     // [0] spikes of committed passes, [1] spikes of the last pass, [2] capacity overflow,
-    // [3] converged (a pass found nothing), [4] passes that found spikes, [5] residual list overflow,
+    // [3] converged (a pass found nothing), [4] passes that found spikes, [5] residual list overflow (one pass),
     // [6] grid-barrier arrivals and [7] grid-barrier timeout of the persistent pursuit, [8] the
     // peak kernel's workgroup ticket (kCounters ints)
     int* counters;

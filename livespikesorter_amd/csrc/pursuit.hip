@@ -956,15 +956,18 @@ __global__ __launch_bounds__(kPT) void pursuit_persistent_kernel(PursuitArgs a, 
 }
 
 // ------------------------------------------------------------------------------------------------
-// residual, once after the last pass.  A workgroup owns 64 columns; it finds, per pass, the spikes
+// residual, once after the last pass.  A workgroup owns 16 columns; it finds, per pass, the spikes
 // whose +-nt/2 window reaches its columns (binary search in the pass's column-sorted list), lists
 // them in pursuit order in LDS, and each wavefront walks that list for its columns applying, pass
 // by pass, the last even then the last odd covering spike:
 //   X[w][c] -= amp * waves[t][w - col + nt/2][c]
-// which is the residual the reference's per-pass updates leave (template_matching.py:197).
+// which is the residual the reference's per-pass updates leave (template_matching.py:197).  When
+// the passes' spikes reaching the block exceed the list (kResCap; short templates over many passes),
+// the passes are taken in consecutive runs that fit, the columns updated run after run in pass
+// order, so the result is the same.
 // ------------------------------------------------------------------------------------------------
 constexpr int kResCols = 16;
-constexpr int kResCap = 1024;       // listed spikes per workgroup
+constexpr int kResCap = 1024;       // listed spikes per workgroup and run of passes
 constexpr int kResMaxPasses = 256;
 constexpr int kResBatch = 8;        // wave-row loads in flight per column
 
@@ -974,6 +977,7 @@ __global__ __launch_bounds__(256) void residual_kernel(PursuitArgs a) {
     __shared__ int l_col[kResCap], l_tmp[kResCap], l_pp[kResCap];   // l_pp = pass * 2 + parity
     __shared__ float l_amp[kResCap];
     __shared__ int l_apply[4][2 * kResMaxPasses];                    // per wavefront
+    __shared__ int s_run[2];                                         // passes [s_run[0], s_run[1])
     if (a.counters[2]) return;                  // spike list overflowed: the call reports an error
     const int P = a.counters[4];                // passes that found spikes
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -996,86 +1000,102 @@ __global__ __launch_bounds__(256) void residual_kernel(PursuitArgs a) {
         s_cnt[p] = l2 - l;
     }
     __syncthreads();
-    if (tid == 0) {
-        int acc = 0;
-        for (int p = 0; p < P; ++p) { s_off[p] = acc; acc += s_cnt[p]; }
-        s_off[P] = acc;
-        if (acc > kResCap) a.counters[5] = 1;
-    }
-    __syncthreads();
-    const int total = min(s_off[P], kResCap);
-    for (int p = tid; p < P; p += 256) {
-        for (int k = 0; k < s_cnt[p]; ++k) {
-            const int d = s_off[p] + k;
-            if (d >= kResCap) break;
-            const int j = s_first[p] + k;
-            l_col[d] = a.sp_col[j];
-            l_tmp[d] = a.sp_tmp[j];
-            l_amp[d] = a.sp_amp[j];
-            l_pp[d] = p * 2 + ((j - a.pass_off[p]) & 1);
-        }
-    }
-    __syncthreads();
-    if (s_off[P] > kResCap) return;
     constexpr int nvc = NVC;
     int* apply = l_apply[wave];
-    for (int cw = wave; cw < kResCols; cw += 4) {
-        const int64_t w = w0 + cw;
-        if (w >= a.W) break;
-        // 1) the ordered list of subtractions for this column: per pass, last even then last odd
-        int na = 0, cur = -1, e = -1, o = -1;
-        for (int d = 0; d <= total; ++d) {
-            const bool end = d == total;
-            const int pp = end ? -2 : l_pp[d];
-            const bool cov = !end && (int)(w - l_col[d]) <= h && (int)(l_col[d] - w) <= h;
-            if (end || (cov && (pp >> 1) != cur)) {
-                if (e >= 0) apply[na++] = e;
-                if (o >= 0) apply[na++] = o;
-                e = o = -1;
-                if (!end) cur = pp >> 1;
+    // every thread keeps the run's end in a register: s_run is rewritten only after the barriers
+    // that follow every thread's read of it
+    for (int next = 0; next < P;) {
+        if (tid == 0) {
+            // the next run of passes whose spikes fit the list (at least one pass; a single pass
+            // beyond the list, possible only with tied peaks, is cut and reported)
+            const int p0 = next;
+            int acc = 0, p = p0;
+            while (p < P && (p == p0 || acc + s_cnt[p] <= kResCap)) {
+                s_off[p] = acc;
+                acc += s_cnt[p];
+                ++p;
             }
-            if (cov) {
-                if (pp & 1) o = d; else e = d;
+            s_off[p] = acc;
+            if (acc > kResCap) a.counters[5] = 1;
+            s_run[0] = p0;
+            s_run[1] = p;
+        }
+        __syncthreads();
+        const int p0 = s_run[0], p1 = s_run[1];
+        const int total = min(s_off[p1], kResCap);
+        for (int p = p0 + tid; p < p1; p += 256) {
+            for (int k = 0; k < s_cnt[p]; ++k) {
+                const int d = s_off[p] + k;
+                if (d >= kResCap) break;
+                const int j = s_first[p] + k;
+                l_col[d] = a.sp_col[j];
+                l_tmp[d] = a.sp_tmp[j];
+                l_amp[d] = a.sp_amp[j];
+                l_pp[d] = p * 2 + ((j - a.pass_off[p]) & 1);
             }
         }
-        // 2) apply them in order, with the wave rows of 8 subtractions loaded ahead
-        float* xr = a.X + w * a.ldC;
-        floatx4 x[NVC];
+        __syncthreads();
+        for (int cw = wave; cw < kResCols; cw += 4) {
+            const int64_t w = w0 + cw;
+            if (w >= a.W) break;
+            // 1) the ordered list of subtractions for this column: per pass, last even then last odd
+            int na = 0, cur = -1, e = -1, o = -1;
+            for (int d = 0; d <= total; ++d) {
+                const bool end = d == total;
+                const int pp = end ? -2 : l_pp[d];
+                const bool cov = !end && (int)(w - l_col[d]) <= h && (int)(l_col[d] - w) <= h;
+                if (end || (cov && (pp >> 1) != cur)) {
+                    if (e >= 0) apply[na++] = e;
+                    if (o >= 0) apply[na++] = o;
+                    e = o = -1;
+                    if (!end) cur = pp >> 1;
+                }
+                if (cov) {
+                    if (pp & 1) o = d; else e = d;
+                }
+            }
+            if (na == 0) continue;
+            // 2) apply them in order, with the wave rows of 8 subtractions loaded ahead
+            float* xr = a.X + w * a.ldC;
+            floatx4 x[NVC];
 #pragma unroll
-        for (int v = 0; v < nvc; ++v) {
-            const int c = 4 * lane + 256 * v;
-            if (c < a.C) x[v] = *(const floatx4*)(xr + c);
-        }
-        for (int b0 = 0; b0 < na; b0 += kResBatch) {
-            floatx4 wv[kResBatch][NVC];
+            for (int v = 0; v < nvc; ++v) {
+                const int c = 4 * lane + 256 * v;
+                if (c < a.C) x[v] = *(const floatx4*)(xr + c);
+            }
+            for (int b0 = 0; b0 < na; b0 += kResBatch) {
+                floatx4 wv[kResBatch][NVC];
 #pragma unroll
-            for (int q = 0; q < kResBatch; ++q) {
-                if (b0 + q < na) {
-                    const int d = apply[b0 + q];
-                    const float* wr = a.waves + ((int64_t)l_tmp[d] * a.nt + (w - l_col[d] + h)) * a.ldC;
+                for (int q = 0; q < kResBatch; ++q) {
+                    if (b0 + q < na) {
+                        const int d = apply[b0 + q];
+                        const float* wr = a.waves + ((int64_t)l_tmp[d] * a.nt + (w - l_col[d] + h)) * a.ldC;
 #pragma unroll
-                    for (int v = 0; v < nvc; ++v) {
-                        const int c = 4 * lane + 256 * v;
-                        if (c < a.C) wv[q][v] = *(const floatx4*)(wr + c);
+                        for (int v = 0; v < nvc; ++v) {
+                            const int c = 4 * lane + 256 * v;
+                            if (c < a.C) wv[q][v] = *(const floatx4*)(wr + c);
+                        }
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < kResBatch; ++q) {
+                    if (b0 + q < na) {
+                        const float am = l_amp[apply[b0 + q]];
+#pragma unroll
+                        for (int v = 0; v < nvc; ++v)
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) x[v][r] = x[v][r] - am * wv[q][v][r];
                     }
                 }
             }
 #pragma unroll
-            for (int q = 0; q < kResBatch; ++q) {
-                if (b0 + q < na) {
-                    const float am = l_amp[apply[b0 + q]];
-#pragma unroll
-                    for (int v = 0; v < nvc; ++v)
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) x[v][r] = x[v][r] - am * wv[q][v][r];
-                }
+            for (int v = 0; v < nvc; ++v) {
+                const int c = 4 * lane + 256 * v;
+                if (c < a.C) *(floatx4*)(xr + c) = x[v];
             }
         }
-#pragma unroll
-        for (int v = 0; v < nvc; ++v) {
-            const int c = 4 * lane + 256 * v;
-            if (c < a.C) *(floatx4*)(xr + c) = x[v];
-        }
+        __syncthreads();   // the list is rewritten by the next run
+        next = p1;
     }
 }
 

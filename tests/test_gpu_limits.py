@@ -107,3 +107,51 @@ def test_window_at_nt_limit(s127, nt):
     assert f1 >= 0.999, f1
     err, n = _amp_err(got.sample, got.template, got.amplitude, x3)
     assert n >= 0.99 * len(got) and err <= 1e-3, (n, err)
+
+
+def test_residual_runs_of_passes_bit_exact():
+    """nt = 3, 256 templates on disjoint channels (ctc zero between templates) and scores that put a
+    peak every 4 columns in each of 256 passes: 4-5 spikes of every pass reach each 16-column block
+    of the residual kernel, over 1024 in all, so its list is filled run after run of passes.  Spikes,
+    B and the residual bit for bit against the oracle's run_matching."""
+    from livespikesorter_amd import synth
+    from livespikesorter_amd.sorter import Sorter
+    print("case: residual runs of passes nt=3 T=256 C=256 W=400, 256 passes", flush=True)
+    T = C = 256
+    nt, W = 3, 400
+    sy = synth.generate(synth.SynthConfig(n_chan=C, n_units=T, n_samples=4000, NT=2000, nt=nt, nt0min=1, n_pcs=2,
+                                          nC=4, nrange=16, seed=5))
+    a = dict(sy.art)
+    U = np.zeros((T, 2, C), np.float32)
+    U[np.arange(T), 0, np.arange(T)] = 1.0
+    U[np.arange(T), 1, np.arange(T)] = 0.5
+    a["U"] = U
+    rng = np.random.default_rng(5)
+    B = np.ones((T, W), np.float32)
+    cols = np.arange(4, W - nt, 4)
+    tt, ww = np.meshgrid(np.arange(T), cols, indexing="ij")
+    B[:, cols] = (100.0 + 5.0 * ((7 * tt + ww) % 50)).astype(np.float32)
+    x32 = torch.from_numpy(rng.standard_normal((C, W)).astype(np.float32))
+    B32 = torch.from_numpy(B)
+    U32, w32 = torch.from_numpy(U), torch.from_numpy(a["wPCA"])
+    ws = 1000
+    with Sorter(a, max_window=W, max_passes=256, max_spikes=1 << 15) as s:
+        s.set_persistent(0)
+        tb = s.tables()
+        ctc = torch.from_numpy(tb["ctc"]).permute(2, 0, 1).contiguous()
+        ref = ko.run_matching(x32, U32, w32, ctc, Th=a["Th"], max_passes=256, nm=torch.from_numpy(tb["nm"]),
+                              waves=torch.from_numpy(tb["waves"]), B=B32)
+        Bd = B32.T.contiguous().to(DEV)
+        Xd = x32.T.contiguous().to(DEV)
+        got = s.stage_pursuit(Bd, Xd, ws)
+        # spikes per 16-column block (+-nt/2 reach) over all passes: the residual list is exceeded
+        st = ref.st.numpy()
+        reach = max(int(np.sum((st[:, 0] >= b - nt // 2) & (st[:, 0] <= b + 15 + nt // 2))) for b in range(0, W, 16))
+        print(f"case:   {len(st)} spikes, {ref.passes} passes, at most {reach} reach one block", flush=True)
+        assert ref.passes == 256 and reach > 1024
+        assert got.passes == ref.passes
+        np.testing.assert_array_equal(got.column, st[:, 0])
+        np.testing.assert_array_equal(got.template, st[:, 1])
+        np.testing.assert_array_equal(got.amplitude, ref.amps[:, 0].numpy())
+        assert np.array_equal(Bd.cpu().numpy().T, ref.B.numpy())
+        assert np.array_equal(Xd.cpu().numpy().T, ref.Xres.numpy())
