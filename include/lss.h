@@ -22,6 +22,7 @@
 #ifndef LSS_H_
 #define LSS_H_
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -140,6 +141,12 @@ lss_status lss_destroy(void* handle);
 /* Makes `device` the calling thread's current HIP device (OnlineSpikesV2::initializeSorter's
  * cudaSetDevice, OnlineSpikesV2.cu:326-327); handles are created on the current device. */
 lss_status lss_select_device(int32_t device);
+/* Page-locked host memory for windows and sample chunks: lss_sort_window and lss_stream_push read
+ * it by DMA at the link's full rate, where pageable memory is staged through the runtime's own
+ * buffers.  The reference keeps its fetch buffer page-locked the same way (its "Pinned" memory
+ * variables, cudaMallocHost, src/SpikeSorter/OnlineSpikesV2.cu:399-405). */
+lss_status lss_host_alloc(size_t bytes, void** p);
+lss_status lss_host_free(void* p);
 lss_status lss_get_params(void* handle, lss_params* p);
 lss_status lss_set_stream(void* handle, void* hip_stream);
 

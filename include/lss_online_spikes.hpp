@@ -39,6 +39,7 @@
 #include <climits>
 #include <cstdint>
 #include <functional>
+#include <memory>
 #include <numeric>
 #include <stdexcept>
 #include <string>
@@ -188,7 +189,11 @@ private:
         amp_.resize(cap);
         x_.resize(cap);
         y_.resize(cap);
-        win_.resize((size_t)(prm_.iMaxScanWindow + 2 * nt_) * C_);
+        // the window buffer is page-locked, as the reference's fetch buffer is (OnlineSpikesV2.cu:399-405):
+        // lss_sort_window's upload is then a DMA from it
+        void* wp = nullptr;
+        call(lss_host_alloc(sizeof(int16_t) * (size_t)(prm_.iMaxScanWindow + 2 * nt_) * C_, &wp));
+        win_.reset((int16_t*)wp);
         if (!prm_.sOSSOutputFolder.empty()) {
             out_path_ = prm_.sOSSOutputFolder;
             if (out_path_.back() != '/') out_path_ += '/';
@@ -232,7 +237,7 @@ public:
         float p2p = 0.f;
         {
             int16_t lo = INT16_MAX, hi = INT16_MIN;
-            const int16_t* q = win_.data() + (size_t)nt * C_;
+            const int16_t* q = win_.get() + (size_t)nt * C_;
             for (size_t i = 0; i < (size_t)n * C_; ++i) {
                 lo = std::min(lo, q[i]);
                 hi = std::max(hi, q[i]);
@@ -246,7 +251,7 @@ public:
         out.amplitude = amp_.data();
         out.x = x_.data();
         out.y = y_.data();
-        call(lss_sort_window(h_, win_.data(), W, 0, start - nt, &out));
+        call(lss_sort_window(h_, win_.get(), W, 0, start - nt, &out));
         const int64_t ns = out.n;
         // labels, time order, duplicates
         std::vector<int32_t> lab((size_t)ns);
@@ -315,17 +320,17 @@ private:
 
     // samples [first, first + W) into win_; rows before sample 0 repeat sample 0
     void fetch_window(long first, long W) {
+        int16_t* w = win_.get();
         long row = 0;
         if (first < 0) {
             const long pad = std::min(-first, W);
-            sock_->fetchFromPlace(win_.data() + (size_t)pad * C_, os_, 0, 1);
+            sock_->fetchFromPlace(w + (size_t)pad * C_, os_, 0, 1);
             for (long r = 0; r < pad; ++r)
-                std::copy(win_.data() + (size_t)pad * C_, win_.data() + (size_t)(pad + 1) * C_,
-                          win_.data() + (size_t)r * C_);
+                std::copy(w + (size_t)pad * C_, w + (size_t)(pad + 1) * C_, w + (size_t)r * C_);
             row = pad;
         }
         while (row < W) {
-            const t_ull got = sock_->fetchFromPlace(win_.data() + (size_t)row * C_, os_, (t_ull)(first + row), W - row);
+            const t_ull got = sock_->fetchFromPlace(w + (size_t)row * C_, os_, (t_ull)(first + row), W - row);
             const long k = (long)got - (first + row);
             if (k <= 0) throw std::runtime_error("OnlineSpikesV2: DataSocket returned no samples");
             row += k;
@@ -346,7 +351,10 @@ private:
     std::vector<int64_t> sample_;
     std::vector<int32_t> tmpl_;
     std::vector<float> amp_, x_, y_;
-    std::vector<int16_t> win_;
+    struct HostFree {
+        void operator()(int16_t* p) const { (void)lss_host_free(p); }
+    };
+    std::unique_ptr<int16_t, HostFree> win_;   // page-locked window [iMaxScanWindow + 2 nt][C]
 };
 
 }  // namespace lss

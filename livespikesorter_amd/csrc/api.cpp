@@ -862,6 +862,18 @@ lss_status lss_select_device(int32_t device) {
 }
 int32_t lss_abi_version(void) { return LSS_ABI_VERSION; }
 
+lss_status lss_host_alloc(size_t bytes, void** p) {
+    if (!p || bytes == 0) return fail(LSS_ERR_ARG, "lss_host_alloc: bad argument");
+    *p = nullptr;
+    HIP_TRY(hipHostMalloc(p, bytes, hipHostMallocDefault));
+    return LSS_OK;
+}
+
+lss_status lss_host_free(void* p) {
+    if (p) HIP_TRY(hipHostFree(p));
+    return LSS_OK;
+}
+
 lss_status lss_default_params(lss_params* p) {
     if (!p) return fail(LSS_ERR_ARG, "null params");
     std::memset(p, 0, sizeof(*p));

@@ -28,6 +28,30 @@ def test_library_exports_every_declared_symbol():
     assert set(_lib.EXPORTS) == set(names)
 
 
+def test_header_compiles_as_c_and_cpp(tmp_path):
+    """include/lss.h is a plain C header (C99, no C++ types) that C++ can include as well."""
+    import shutil
+    import subprocess
+    src = tmp_path / "t.c"
+    src.write_text('#include "lss.h"\nint main(void) { return lss_abi_version() > 0 ? 0 : 1; }\n')
+    for cc, std in (("gcc", "-std=c99"), ("g++", "-std=c++17")):
+        exe = shutil.which(cc)
+        if exe is None:
+            pytest.skip(f"{cc} not found")
+        r = subprocess.run([exe, std, "-Wall", "-Werror", "-fsyntax-only", "-x", "c" if cc == "gcc" else "c++",
+                            "-I", str(ROOT / "include"), str(src)], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+
+
+def test_pinned_host_buffers():
+    from livespikesorter_amd import _lib
+    L = _lib.lib()
+    p = ct.c_void_p()
+    if L.lss_host_alloc(0, ct.byref(p)) == 0:
+        raise AssertionError("a zero-byte request must be rejected")
+    assert L.lss_host_free(None) == 0
+
+
 def test_default_params():
     from livespikesorter_amd.sorter import default_params
     p = default_params()

@@ -367,3 +367,27 @@ def test_sort_windows_in_flight_equals_one_at_a_time(sorter, synth1):
         assert got == [len(r) for r in ref]
         with pytest.raises(LssError):
             sort_windows_native([sorter, sorter], wins, starts)
+
+
+def test_sort_window_from_page_locked_host_memory(sorter, synth1):
+    """A window in lss_host_alloc memory (the facade's page-locked window buffer) sorts exactly as
+    the same window from device memory and from pageable host memory."""
+    import ctypes as ct
+    print("[lss] test_sort_window_from_page_locked_host_memory", flush=True)
+    from livespikesorter_amd._lib import check, lib
+    win, _, ws = _window(synth1, 1)
+    ref = sorter.sort_window(torch.from_numpy(win).to(DEV), ws)
+    p = ct.c_void_p()
+    check(lib().lss_host_alloc(win.nbytes, ct.byref(p)), "lss_host_alloc")
+    try:
+        buf = np.ctypeslib.as_array(ct.cast(p, ct.POINTER(ct.c_int16)), shape=win.shape)
+        buf[...] = win
+        for src in (buf, np.ascontiguousarray(win)):
+            got = sorter.sort_window(src, ws)
+            assert len(got) > 0
+            np.testing.assert_array_equal(got.sample, ref.sample)
+            np.testing.assert_array_equal(got.template, ref.template)
+            np.testing.assert_array_equal(got.amplitude, ref.amplitude)
+            np.testing.assert_array_equal(got.features, ref.features)
+    finally:
+        check(lib().lss_host_free(p), "lss_host_free")
