@@ -142,6 +142,7 @@ def read_traffic(path=TRAFFIC_FILE):
 METRIC = "x-realtime & p99 batch latency, 384ch@30kHz; bf16x3 vs fp32 at spike F1≥0.999"
 STREAM_PROBES = [(384, 600), (384, 250), (128, 120), (64, 40)]   # (channels, templates)
 STREAM_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "update_kernel_traffic_stream_384ch_600t.md")
+STREAM_PERSIST_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "persistent_kernel_traffic_stream_384ch_600t.md")
 
 
 def _pct(x, q):
@@ -247,6 +248,56 @@ def bench_stream(args):
         nt = probes[i].art["nt"]
         return w0[i] if k == 0 else drecs[i][k * batch - nt:(k + 1) * batch + nt]
 
+    def update_roofline(k0, nwin):
+        # the pursuit of the 384-ch / 600-template probe (the step's critical path) in measurement
+        # mode over windows k0 .. k0 + nwin - 1, one at a time, on the schedule the step uses: the
+        # persistent kernel (the passes after its first per-pass rounds) when the schedule takes
+        # it, the update kernel of the per-pass rounds beside it
+        so = sorters[0]
+        T0, nt0 = so.T, so.nt
+        bytes_per_spike = 3 * 4 * T0 * (2 * nt0 + 1)      # ctc rows read + B rows read and written
+        so.profile_update(True)
+        ev_all, gpu_all, sp_all, pers = [], [], [], []
+        for k in range(k0, k0 + nwin):
+            so.sort_window(dwindow(0, k), k * batch - nt0)
+            ev_, gpu_, sp_ = so.update_times()
+            ev_all.append(ev_[sp_ > 0])
+            gpu_all.append(gpu_[sp_ > 0])
+            sp_all.append(sp_[sp_ > 0])
+            pers.append(so.persistent_times())
+        so.profile_update(False)
+        ev_all, gpu_all, sp_all = np.concatenate(ev_all), np.concatenate(gpu_all), np.concatenate(sp_all)
+        upd = None
+        if len(sp_all):
+            upd = _update_roofline("update_kernel<3, C> (pursuit score update of the 384-ch / 600-template probe, "
+                                   f"{update_columns(so)} score column(s) per wavefront)", ev_all, gpu_all, sp_all,
+                                   bytes_per_spike, read_traffic(STREAM_TRAFFIC_FILE),
+                                   f"{nwin} windows of the timed region, one at a time")
+        pers = np.array([p_ for p_ in pers if p_[1] > 0], np.float64)
+        if not len(pers):
+            return upd, None
+        ms, spk, ps, grid = pers[:, 0], pers[:, 1], pers[:, 2], pers[:, 3]
+        per_launch = float(spk.mean()) * bytes_per_spike
+        achieved = per_launch / (float(ms.mean()) * 1e-3) / 1e9
+        roof = {"kernel": f"pursuit_persistent_kernel<3> (the 384-ch / 600-template probe's passes after its first "
+                          f"{so.persistent(batch + 2 * nt0)[1]}, {int(grid.mean())} workgroups)",
+                "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(STREAM_PERSIST_TRAFFIC_FILE),
+                "launch_us": round(float(ms.mean()) * 1e3, 2),
+                "us_per_pass": round(float(ms.sum()) * 1e3 / float(ps.sum()), 2),
+                "per_launch": f"{bytes_per_spike / 1e3:.1f} KB per spike it applies x {spk.mean():.1f} spikes per "
+                              f"launch = {per_launch / 1e6:.2f} MB over {ps.mean():.1f} passes; avg "
+                              f"{ms.mean() * 1e3:.1f} us per launch from the workgroups' GPU-clock stamps over "
+                              f"{len(pers)} windows of the timed region, one at a time (measurement mode); each "
+                              f"pass is a dependent step ending in a grid barrier, so the kernel is bound by "
+                              f"the pass latency, not by HBM"}
+        return roof, upd
+
+    if args.roofline_only:
+        # counter runs: only the measured pursuit launches (with the schedule pinned or tuned)
+        r, u = update_roofline(args.warmup, 50)
+        return {"roofline": r, "roofline_update_kernel": u, "tuning": tuning}
+
     # plain per-probe path: every batch's windows one probe at a time (device-resident windows)
     for i, (s_, p) in enumerate(zip(plain_sorters, probes)):     # first-window setup, untimed
         s_.sort_window(dwindow(i, 0), -p.art["nt"])
@@ -328,25 +379,7 @@ def bench_stream(args):
     n_ref_timed = sum(1 for k in range(warm, warm + K) if refine_every and (k + 1) >= refine_every
                       and ((k + 1) % refine_every) < P)
 
-    # roofline: the update kernel of the 384-ch / 600-template probe (the step's critical path),
-    # per launch with HIP events on the sorter's stream over windows of the timed region
-    so = sorters[0]
-    T0, nt0 = so.T, so.nt
-    bytes_per_spike = 3 * 4 * T0 * (2 * nt0 + 1)      # ctc rows read + B rows read and written
-    so.profile_update(True)
-    ev_all, gpu_all, sp_all = [], [], []
-    for k in range(warm, warm + min(K, 50)):
-        so.sort_window(dwindow(0, k), k * batch - nt0)
-        ev_, gpu_, sp_ = so.update_times()
-        ev_all.append(ev_[sp_ > 0])
-        gpu_all.append(gpu_[sp_ > 0])
-        sp_all.append(sp_[sp_ > 0])
-    so.profile_update(False)
-    ev_all, gpu_all, sp_all = np.concatenate(ev_all), np.concatenate(gpu_all), np.concatenate(sp_all)
-    roofline = _update_roofline("update_kernel<3, C> (pursuit score update of the 384-ch / 600-template probe, "
-                                f"{update_columns(so)} score column(s) per wavefront)", ev_all, gpu_all, sp_all,
-                                bytes_per_spike, read_traffic(STREAM_TRAFFIC_FILE),
-                                f"{min(K, 50)} windows of the timed region, one at a time")
+    roofline, roofline_update = update_roofline(warm, min(K, 50))
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -415,6 +448,7 @@ def bench_stream(args):
         "spike_f1_vs_plain_fp32_all_batches": round(float(f1_all), 6),
         "f1_note": "per probe, pooled over batches, +-1 sample and same template; the minimum over probes",
         "roofline": roofline,
+        "roofline_update_kernel": roofline_update,
         "refinement": {"every_batches": refine_every, "rounds": len(refines),
                        "loss_ratio_after_before": [[round(x["loss_after"] / x["loss_before"], 6) if x["loss_before"]
                                                     else None for x in rr] for rr in refines]},
@@ -920,6 +954,8 @@ def main():
     ap.add_argument("--refine-seconds", type=float, default=10.0,
                     help="stream workload: one template-refinement round every this many seconds (0 = off)")
     ap.add_argument("--no-window", action="store_true", help="stream workload: skip the nested 2-s window figures")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="stream workload: only the update-kernel roofline windows (counter runs)")
     ap.add_argument("--pin-tuning", default=None,
                     help="stream workload: JSON list, per probe [tiles, plain_tiles, update_grid, update_columns, "
                          "first_passes, persistent_grid, persistent_after], used instead of the autotuner")
@@ -949,7 +985,7 @@ def main():
 
     if args.workload == "stream":
         res = bench_stream(args)
-        if not args.no_window:
+        if not args.no_window and not args.roofline_only:
             res["window_2s"] = bench_window(args, steps=20, warmup=5)
         print(json.dumps(res))
         return

@@ -376,8 +376,9 @@ lss_status lss_autotune_persistent(void* handle, const int16_t* d_window, int64_
                                    int32_t max_grid, float* ms, int32_t* best_grid, int32_t* best_first);
 
 /* Measurement mode: time every launch of the pursuit update kernel with HIP events on the
- * handle's stream; lss_last_update_stats gives the last window's summed update time and the number
- * of update launches (max_passes per window). */
+ * handle's stream, and the persistent pursuit (when the window takes it) with its workgroups'
+ * GPU-clock stamps; lss_last_update_stats gives the last window's summed update time and the
+ * number of update launches (max_passes per window).  Results do not change. */
 lss_status lss_profile_update(void* handle, int32_t enable);
 lss_status lss_last_update_stats(void* handle, float* total_ms, int32_t* launches);
 /* Per launch of the last window's update kernel (measurement mode): event_ms[i] between HIP events
@@ -388,6 +389,11 @@ lss_status lss_last_update_stats(void* handle, float* total_ms, int32_t* launche
  * update grid is set. */
 lss_status lss_last_update_times(void* handle, float* event_ms, float* gpu_ms, int32_t* spikes, int32_t n,
                                  int32_t* launches);
+/* Measurement mode, persistent pursuit (lss_set_persistent): for the last window, its GPU-clock
+ * span (last workgroup end - first workgroup start), the spikes it applied, the passes it ran and
+ * the workgroups it launched; all 0 when the window took the per-pass kernels only.  Its passes are
+ * then absent from lss_last_update_times (their spikes read 0 there). */
+lss_status lss_last_persistent_stats(void* handle, float* gpu_ms, int32_t* spikes, int32_t* passes, int32_t* grid);
 
 /* ---- per-batch drift correction ----------------------------------------------------------------
  * Replaces the handle's whitening operator with (M @ Wrot), M a device [C][C] FP64 matrix (row

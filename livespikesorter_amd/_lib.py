@@ -32,7 +32,7 @@ EXPORTS = ["lss_last_error", "lss_abi_version", "lss_default_params", "lss_creat
            "lss_select_device", "lss_sort_windows", "lss_cluster_centroids_pca", "lss_assign_clusters", "lss_load_cluster_centroids", "lss_load_channel_map",
            "lss_get_load_info", "lss_set_tiles", "lss_get_tiles", "lss_autotune_tiles",
            "lss_last_update_times", "lss_set_update_columns", "lss_get_update_columns",
-           "lss_extract_snippets_noalloc", "lss_host_alloc", "lss_host_free"]
+           "lss_extract_snippets_noalloc", "lss_host_alloc", "lss_host_free", "lss_last_persistent_stats"]
 
 
 class Params(ct.Structure):
@@ -139,6 +139,8 @@ def lib() -> ct.CDLL:
             "lss_train_release": ([], i32),
             "lss_select_device": ([i32], i32),
             "lss_host_alloc": ([ct.c_size_t, ct.POINTER(vp)], i32),
+            "lss_last_persistent_stats": ([vp, ct.POINTER(ct.c_float), ct.POINTER(i32), ct.POINTER(i32),
+                                           ct.POINTER(i32)], i32),
             "lss_host_free": ([vp], i32),
             "lss_sort_windows": ([ct.POINTER(vp), i32, ct.POINTER(vp), ct.POINTER(i64), ct.POINTER(i64), i32, i32,
                                   ct.POINTER(Spikes)], i32),

@@ -136,6 +136,12 @@ struct Handle {
     std::vector<int> upd_spikes;   // spikes each of those launches applied
     DevBuf<unsigned long long> upd_ts;   // [max_passes][upd_grid][2] GPU-clock stamps
     DevBuf<unsigned long long> upd_span; // [max_passes][2] first start, last end per launch
+    // measurement mode, persistent pursuit: its workgroups' stamps, the span, its first pass and
+    // grid in the last window (pers_p0 < 0: it did not run) and what it did
+    DevBuf<unsigned long long> pers_ts, pers_span;
+    int pers_p0 = -1, pers_used = 0;
+    float pers_ms = 0.f;
+    int pers_spikes = 0, pers_passes = 0;
     float last_ms[8] = {};   // car, highpass, whiten, project, scores GEMM, pursuit, features, total
     lss_load_info load{};    // which extraction tables were given, the ctc.npy comparison
     bool alive = false;
@@ -497,7 +503,7 @@ lss_status enqueue_passes(Handle& h, const lss::PursuitArgs& a, int p0, int p1) 
 }
 
 bool use_persistent(const Handle& h, int64_t W) {
-    return h.pers_grid > 0 && W <= lss::kPersistMaxW && !h.prof_update && lss::persistent_supports(h.T);
+    return h.pers_grid > 0 && W <= lss::kPersistMaxW && lss::persistent_supports(h.T);
 }
 
 // The pursuit of one window: short windows run their first pers_first passes (the wide ones) as
@@ -512,9 +518,13 @@ lss_status enqueue_pursuit(Handle& h, const lss::PursuitArgs& a, int p1, int max
         } else {
             HIP_TRY(lss::launch_reset_counters(h.counters.p, h.stream));
         }
-        HIP_TRY(lss::launch_pursuit_persistent(a, g, p0, h.prm.max_passes, h.stream));
+        lss::PursuitArgs b = a;
+        if (h.prof_update && h.pers_ts.p) b.ts = h.pers_ts.p;
+        HIP_TRY(lss::launch_pursuit_persistent(b, g, p0, h.prm.max_passes, h.stream, &h.pers_used));
+        h.pers_p0 = p0;
         return LSS_OK;
     }
+    h.pers_p0 = -1;
     return enqueue_passes(h, a, 0, p1);
 }
 
@@ -587,6 +597,7 @@ lss_status enqueue_readback(Handle& h) {
     return LSS_OK;
 }
 
+constexpr int kPersStamps = 2048;   // persistent-pursuit workgroups measurement mode can stamp
 const char* const kBarrierTimeout =
     "persistent pursuit: grid barrier timed out (workgroups not co-resident); results are invalid";
 
@@ -601,11 +612,14 @@ lss_status check_counts(Handle& h, const int* c, int* n, int* passes) {
     *n = c[0] + c[1];
     *passes = c[4];
     if (h.prof_update) {
+        // per-pass update launches of this window: all max_passes, or the first pers_p0 when the
+        // persistent pursuit took the rest (the other events were not recorded)
+        const int ran = h.pers_p0 >= 0 ? std::min(h.pers_p0, h.prm.max_passes) : h.prm.max_passes;
         h.upd_ms = 0.f;
         h.upd_launches = h.prm.max_passes;
         h.upd_each.assign(h.prm.max_passes, 0.f);
         h.upd_spikes.assign(h.prm.max_passes, 0);
-        for (int p = 0; p < h.prm.max_passes; ++p) {
+        for (int p = 0; p < ran; ++p) {
             float ms = 0.f;
             HIP_TRY(hipEventElapsedTime(&ms, h.upd_ev[2 * p], h.upd_ev[2 * p + 1]));
             h.upd_ms += ms;
@@ -620,7 +634,7 @@ lss_status check_counts(Handle& h, const int* c, int* n, int* passes) {
             HIP_TRY(hipMemcpyAsync(span.data(), h.upd_span.p, sizeof(unsigned long long) * span.size(),
                                    hipMemcpyDeviceToHost, h.stream));
             HIP_TRY(hipStreamSynchronize(h.stream));
-            for (int p = 0; p < h.prm.max_passes; ++p) {
+            for (int p = 0; p < ran; ++p) {
                 const unsigned long long t0 = span[2 * p], t1 = span[2 * p + 1];
                 h.upd_gpu[p] = t1 > t0 ? (float)((double)(t1 - t0) * 1e-5) : 0.f;
             }
@@ -630,7 +644,20 @@ lss_status check_counts(Handle& h, const int* c, int* n, int* passes) {
         HIP_TRY(hipMemcpyAsync(off.data(), h.pass_off.p, sizeof(int) * (h.prm.max_passes + 2), hipMemcpyDeviceToHost,
                                h.stream));
         HIP_TRY(hipStreamSynchronize(h.stream));
-        for (int p = 0; p < *passes && p < h.prm.max_passes; ++p) h.upd_spikes[p] = off[p + 1] - off[p];
+        // per-pass launches ran passes [0, pers_p0) when the persistent pursuit took the rest
+        const int pp = h.pers_p0 >= 0 ? std::min(h.pers_p0, *passes) : *passes;
+        for (int p = 0; p < pp && p < h.prm.max_passes; ++p) h.upd_spikes[p] = off[p + 1] - off[p];
+        h.pers_ms = 0.f;
+        h.pers_spikes = h.pers_passes = 0;
+        if (h.pers_p0 >= 0 && h.pers_used > 0 && h.pers_used <= kPersStamps) {
+            unsigned long long span[2] = {0, 0};
+            HIP_TRY(lss::launch_stamp_span(h.pers_ts.p, 1, h.pers_used, h.pers_span.p, h.stream));
+            HIP_TRY(hipMemcpyAsync(span, h.pers_span.p, sizeof(span), hipMemcpyDeviceToHost, h.stream));
+            HIP_TRY(hipStreamSynchronize(h.stream));
+            h.pers_ms = span[1] > span[0] ? (float)((double)(span[1] - span[0]) * 1e-5) : 0.f;
+            h.pers_spikes = (*passes > h.pers_p0) ? *n - off[h.pers_p0] : 0;
+            h.pers_passes = std::max(0, *passes - h.pers_p0);
+        }
     }
     return LSS_OK;
 }
@@ -1573,7 +1600,23 @@ lss_status lss_profile_update(void* handle, int32_t enable) {
         HIP_TRY(h.upd_ts.alloc((size_t)h.prm.max_passes * 2 * h.upd_grid));
         if (!h.upd_span.p) HIP_TRY(h.upd_span.alloc(2 * (size_t)h.prm.max_passes));
     }
+    if (enable && !h.pers_ts.p) {
+        HIP_TRY(h.pers_ts.alloc(2 * (size_t)kPersStamps));
+        HIP_TRY(h.pers_span.alloc(2));
+    }
     h.prof_update = enable != 0;
+    return LSS_OK;
+}
+
+lss_status lss_last_persistent_stats(void* handle, float* gpu_ms, int32_t* spikes, int32_t* passes, int32_t* grid) {
+    if (!handle || !gpu_ms || !spikes || !passes || !grid) return fail(LSS_ERR_ARG, "null argument");
+    Handle& h = *as_handle(handle);
+    if (!h.prof_update) return fail(LSS_ERR_STATE, "update profiling is off (lss_profile_update)");
+    const bool ran = h.pers_p0 >= 0;
+    *gpu_ms = ran ? h.pers_ms : 0.f;
+    *spikes = ran ? h.pers_spikes : 0;
+    *passes = ran ? h.pers_passes : 0;
+    *grid = ran ? h.pers_used : 0;
     return LSS_OK;
 }
 

@@ -105,8 +105,10 @@ hipError_t launch_stamp_span(const unsigned long long* ts, int launches, int gri
 // update) rounds left.  Results equal those of score_all + max_passes x (peaks, update).
 // counters[7] = 1 reports a barrier timeout.
 constexpr int kPersistMaxW = 4096;
+// used (may be NULL): the workgroups launched.  With a.ts set (measurement mode) workgroup b writes
+// its GPU-clock start and end to ts[2b], ts[2b + 1].
 hipError_t launch_pursuit_persistent(const PursuitArgs& a, int grid, int first_pass, int max_passes,
-                                     hipStream_t s);
+                                     hipStream_t s, int* used = nullptr);
 // Workgroups of the persistent pursuit that the device holds at once for T templates, window W (0
 // when T > 1024: those pursuits run on the per-pass kernels)
 hipError_t persistent_max_grid(int T, int64_t W, int* grid);

@@ -828,6 +828,7 @@ __global__ __launch_bounds__(kPT) void pursuit_persistent_kernel(PursuitArgs a, 
     int* ltmp = lcol + W;
     float* lamp = (float*)(ltmp + W);
     const bool lead = blockIdx.x == 0 && tid == 0;
+    if (a.ts && tid == 0) a.ts[2 * blockIdx.x] = __builtin_amdgcn_s_memrealtime();   // measurement mode
     // this workgroup's columns: kColsB-aligned runs, G of them cover [0, W)
     const int Wc = (W + (int)gridDim.x * kColsB - 1) / ((int)gridDim.x * kColsB) * kColsB;
     const int a0 = min(W, (int)blockIdx.x * Wc), a1 = min(W, a0 + Wc);
@@ -952,6 +953,11 @@ __global__ __launch_bounds__(kPT) void pursuit_persistent_kernel(PursuitArgs a, 
         a.counters[0] = base;
         a.counters[1] = 0;
         a.counters[4] = np;
+    }
+    if (a.ts) {
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        if (tid == 0) a.ts[2 * blockIdx.x + 1] = __builtin_amdgcn_s_memrealtime();
     }
 }
 
@@ -1217,7 +1223,8 @@ hipError_t persistent_max_grid(int T, int64_t W, int* grid) {
     return hipSuccess;
 }
 
-hipError_t launch_pursuit_persistent(const PursuitArgs& a, int grid, int first_pass, int max_passes, hipStream_t s) {
+hipError_t launch_pursuit_persistent(const PursuitArgs& a, int grid, int first_pass, int max_passes, hipStream_t s,
+                                     int* used) {
     const PersistentFn fn = persistent_fn(a.T);
     if (!fn || a.W > kPersistMaxW || a.nt > kPeakHalo || grid < 1 || first_pass < 0 || first_pass >= max_passes ||
         !a.prow_cf || !a.prow_im || !a.prow_amp)
@@ -1226,6 +1233,7 @@ hipError_t launch_pursuit_persistent(const PursuitArgs& a, int grid, int first_p
     hipError_t e = persistent_max_grid(a.T, a.W, &cap);
     if (e != hipSuccess) return e;
     if (cap < 1) return hipErrorInvalidConfiguration;
+    if (used) *used = std::min(grid, cap);
     hipLaunchKernelGGL(fn, dim3(std::min(grid, cap)), dim3(kPT), persistent_lds(a.W), s, a, first_pass, max_passes);
     return hipGetLastError();
 }

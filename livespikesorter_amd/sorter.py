@@ -235,6 +235,14 @@ class Sorter:
                                           ct.byref(n)), "lss_last_update_times")
         return ev, gpu, sp
 
+    def persistent_times(self):
+        """Measurement mode: (GPU-clock ms, spikes applied, passes, workgroups) of the last window's
+        persistent pursuit; zeros when the window took the per-pass kernels only."""
+        ms, sp, ps, g = ct.c_float(), ct.c_int32(), ct.c_int32(), ct.c_int32()
+        check(lib().lss_last_persistent_stats(self._h, ct.byref(ms), ct.byref(sp), ct.byref(ps), ct.byref(g)),
+              "lss_last_persistent_stats")
+        return ms.value, sp.value, ps.value, g.value
+
     def timings(self):
         ms = np.zeros(8, np.float32)
         check(lib().lss_last_timings(self._h, ms.ctypes.data, 8), "lss_last_timings")

@@ -391,3 +391,35 @@ def test_sort_window_from_page_locked_host_memory(sorter, synth1):
             np.testing.assert_array_equal(got.features, ref.features)
     finally:
         check(lib().lss_host_free(p), "lss_host_free")
+
+
+@pytest.mark.parametrize("grid,first", [(0, 0), (64, 0), (64, 2)])
+def test_measurement_mode_accounts_every_spike(sorter, synth1, grid, first):
+    """Measurement mode (lss_profile_update) on a 3122-sample window, from a handle's first window
+    on: the spikes the per-pass update
+    launches report plus those the persistent pursuit reports are the window's spikes, its passes
+    split the same way, and the spike list is the one sorting without measurement gives."""
+    print(f"[lss] test_measurement_mode_accounts_every_spike grid {grid} first {first}", flush=True)
+    from livespikesorter_amd.sorter import Sorter
+    win, _, ws = _window(synth1, 1)
+    w = torch.from_numpy(np.ascontiguousarray(win[:3122])).to(DEV)
+    ref = sorter.sort_window(w, ws)
+    # a fresh handle: measurement mode's first window (no update launch timed before it)
+    with Sorter(synth1.art, max_window=3122, max_passes=100) as s2:
+        s2.set_persistent(grid, first)
+        s2.profile_update(True)
+        got = s2.sort_window(w, ws)
+        ev, gpu, sp = s2.update_times()
+        pms, psp, pps, pg = s2.persistent_times()
+        s2.profile_update(False)
+    np.testing.assert_array_equal(got.sample, ref.sample)
+    np.testing.assert_array_equal(got.amplitude, ref.amplitude)
+    n_pass = int(np.count_nonzero(sp))
+    print(f"[lss]   per-pass spikes {int(sp.sum())} over {n_pass} launches; persistent {psp} spikes, {pps} passes, "
+          f"{pms:.3f} ms, {pg} workgroups", flush=True)
+    assert len(got) > 0 and int(sp.sum()) + psp == len(got)
+    if grid == 0:
+        assert pg == 0 and psp == 0 and pms == 0.0 and n_pass == got.passes
+    else:
+        assert pg == grid and pms > 0 and n_pass == min(first, got.passes) and n_pass + pps == got.passes
+        assert np.all(gpu[:n_pass] > 0)
