@@ -99,6 +99,7 @@ struct Handle {
     DevBuf<int> sp_col, sp_tmp, counters, pass_off;
     DevBuf<float> prow_cf, prow_amp;   // persistent pursuit rows [2][kPersistMaxW] (pass parity)
     DevBuf<int> prow_im;
+    DevBuf<int> pbar;                   // persistent pursuit barrier flags [kPersistMaxGrid]
     DevBuf<float> sp_amp, tF, xs, ys;
     int* h_counters = nullptr;   // pinned
     // pinned readback staging of the batched/streaming path (first stage_cap spikes)
@@ -391,10 +392,13 @@ lss_status create_impl(const lss_params* p, const lss_artifacts* a, Handle** out
     HIP_TRY(h->sp_tmp.alloc(p->max_spikes));
     HIP_TRY(h->sp_amp.alloc(p->max_spikes));
     HIP_TRY(h->counters.alloc(lss::kCounters));
+    HIP_TRY(hipMemset(h->counters.p, 0, sizeof(int) * lss::kCounters));
     HIP_TRY(h->pass_off.alloc(p->max_passes + 2));
     HIP_TRY(h->prow_cf.alloc(2 * (size_t)lss::kPersistMaxW));
     HIP_TRY(h->prow_im.alloc(2 * (size_t)lss::kPersistMaxW));
     HIP_TRY(h->prow_amp.alloc(2 * (size_t)lss::kPersistMaxW));
+    HIP_TRY(h->pbar.alloc((size_t)lss::kPersistMaxGrid));
+    HIP_TRY(hipMemset(h->pbar.p, 0, sizeof(int) * lss::kPersistMaxGrid));
     HIP_TRY(h->tF.alloc((size_t)p->max_spikes * nC * K));
     HIP_TRY(h->xs.alloc(p->max_spikes));
     HIP_TRY(h->ys.alloc(p->max_spikes));
@@ -473,7 +477,7 @@ lss::PursuitArgs pursuit_args(Handle& h, float* B, float* X, int64_t W) {
     lss::PursuitArgs a{};
     a.B = B; a.X = X; a.nm = h.nm.p; a.ctc = h.ctc.p; a.waves = h.waves.p;
     a.pass_off = h.pass_off.p;
-    a.prow_cf = h.prow_cf.p; a.prow_im = h.prow_im.p; a.prow_amp = h.prow_amp.p;
+    a.prow_cf = h.prow_cf.p; a.prow_im = h.prow_im.p; a.prow_amp = h.prow_amp.p; a.bar = h.pbar.p;
     a.cfmax = h.cfmax.p; a.imax = h.imax.p; a.flags = h.flags.p;
     a.sp_col = h.sp_col.p; a.sp_tmp = h.sp_tmp.p; a.sp_amp = h.sp_amp.p; a.counters = h.counters.p;
     a.W = W; a.ldT = h.ldT; a.ldC = h.ldC; a.T = h.T; a.C = h.C; a.K = h.K; a.nt = h.nt; a.cap = h.prm.max_spikes;
@@ -486,7 +490,7 @@ lss::PursuitArgs pursuit_args(Handle& h, float* B, float* X, int64_t W) {
 // device); pass 0 also resets the counters and scores every column.
 lss_status enqueue_passes(Handle& h, const lss::PursuitArgs& a, int p0, int p1) {
     if (p0 == 0) {
-        HIP_TRY(lss::launch_reset_counters(h.counters.p, h.stream));
+        HIP_TRY(lss::launch_reset_counters(h.counters.p, h.stream, h.pbar.p));
         HIP_TRY(lss::launch_score_all(a, h.stream));
     }
     for (int p = p0; p < p1; ++p) {
@@ -508,15 +512,18 @@ bool use_persistent(const Handle& h, int64_t W) {
 
 // The pursuit of one window: short windows run their first pers_first passes (the wide ones) as
 // per-pass kernels and the rest in the persistent kernel (grid capped at max_grid when > 0);
-// others get passes [0, p1) as per-pass kernels.
+// others, and short windows whose score rows do not fit the LDS of that grid, get passes [0, p1)
+// as per-pass kernels.
 lss_status enqueue_pursuit(Handle& h, const lss::PursuitArgs& a, int p1, int max_grid) {
-    if (use_persistent(h, a.W)) {
-        const int g = max_grid > 0 ? std::min(h.pers_grid, max_grid) : h.pers_grid;
+    int g = 0;
+    if (use_persistent(h, a.W))
+        HIP_TRY(lss::persistent_grid(a, max_grid > 0 ? std::min(h.pers_grid, max_grid) : h.pers_grid, &g));
+    if (g > 0) {
         const int p0 = std::min(h.pers_first, h.prm.max_passes - 1);
         if (p0 > 0) {
             if (lss_status st = enqueue_passes(h, a, 0, p0)) return st;
         } else {
-            HIP_TRY(lss::launch_reset_counters(h.counters.p, h.stream));
+            HIP_TRY(lss::launch_reset_counters(h.counters.p, h.stream, h.pbar.p));
         }
         lss::PursuitArgs b = a;
         if (h.prof_update && h.pers_ts.p) b.ts = h.pers_ts.p;

@@ -72,14 +72,17 @@ struct PursuitArgs {
     // measurement mode: the update kernel's workgroup b writes its GPU-clock (100 MHz) start and end
     // to ts[2b], ts[2b + 1] (NULL otherwise)
     unsigned long long* ts;
-    // persistent pursuit: Cfmax, imax and amplitude rows [2][kPersistMaxW], one per pass parity
+    // persistent pursuit: Cfmax, imax and amplitude rows [2][kPersistMaxW], one per pass parity,
+    // and the workgroups' barrier flags [kPersistMaxGrid]
     float* prow_cf;
     int* prow_im;
     float* prow_amp;
+    int* bar;
     // [0] spikes of committed passes, [1] spikes of the last pass, [2] capacity overflow,
     // [3] converged (a pass found nothing), [4] passes that found spikes, [5] residual list overflow (one pass),
     // [6] grid-barrier arrivals and [7] grid-barrier timeout of the persistent pursuit, [8] the
-    // peak kernel's workgroup ticket (kCounters ints)
+    // peak kernel's workgroup ticket, [kEpoch] the persistent pursuit's barrier epoch (advanced, not
+    // zeroed, by launch_reset_counters) (kCounters ints)
     int* counters;
     int64_t W;
     int64_t ldT, ldC;
@@ -88,7 +91,10 @@ struct PursuitArgs {
     float th2;
 };
 constexpr int kCounters = 16;
-hipError_t launch_reset_counters(int* counters, hipStream_t s);        // counters[0..kCounters) = 0
+constexpr int kEpoch = 15;
+// counters[0..kCounters) = 0 except counters[kEpoch], incremented (bar: the persistent pursuit's
+// barrier flags [kPersistMaxGrid], raised to the epoch's base every 2^16 epochs; may be null)
+hipError_t launch_reset_counters(int* counters, hipStream_t s, int* bar = nullptr);
 hipError_t launch_score_all(const PursuitArgs& a, hipStream_t s);     // T <= 2048
 hipError_t launch_peaks(const PursuitArgs& a, hipStream_t s);
 // grid = workgroups (grid-stride over the pass's work items; any grid gives the same result);
@@ -100,18 +106,25 @@ hipError_t launch_update(const PursuitArgs& a, int grid, hipStream_t s, hipEvent
 hipError_t launch_stamp_span(const unsigned long long* ts, int launches, int grid, unsigned long long* out,
                              hipStream_t s);
 // Persistent pursuit of windows W <= kPersistMaxW: passes [first_pass, max_passes) in one launch of
-// `grid` co-resident workgroups (clamped at persistent_max_grid); with first_pass = 0 it also does
+// co-resident workgroups (persistent_grid of `grid`: hipErrorInvalidConfiguration when that is 0;
+// the pursuit then belongs on the per-pass kernels); with first_pass = 0 it also does
 // the first scoring, otherwise it continues from the state that first_pass per-pass (peaks,
 // update) rounds left.  Results equal those of score_all + max_passes x (peaks, update).
 // counters[7] = 1 reports a barrier timeout.
 constexpr int kPersistMaxW = 4096;
+constexpr int kPersistMaxGrid = 2048;   // workgroups of one persistent launch (barrier flags)
 // used (may be NULL): the workgroups launched.  With a.ts set (measurement mode) workgroup b writes
 // its GPU-clock start and end to ts[2b], ts[2b + 1].
 hipError_t launch_pursuit_persistent(const PursuitArgs& a, int grid, int first_pass, int max_passes,
                                      hipStream_t s, int* used = nullptr);
 // Workgroups of the persistent pursuit that the device holds at once for T templates, window W (0
-// when T > 1024: those pursuits run on the per-pass kernels)
+// when T > 1024: those pursuits run on the per-pass kernels).  Each workgroup keeps the score rows
+// of ceil(W / grid) columns in its LDS, so a grid also has a minimum (persistent_grid).
 hipError_t persistent_max_grid(int T, int64_t W, int* grid);
+// The grid a launch of `a` at `want` workgroups runs: the largest one <= want whose workgroups, with
+// the LDS it needs, the device holds at once; 0 when the window's score rows do not fit the LDS of
+// any such grid (the pursuit then runs on the per-pass kernels).
+hipError_t persistent_grid(const PursuitArgs& a, int want, int* grid);
 bool persistent_supports(int T);
 // residual of all passes in pass order (even, then odd spikes of a pass), after the last pass
 hipError_t launch_residual(const PursuitArgs& a, hipStream_t s);

@@ -79,6 +79,20 @@ __device__ __forceinline__ float wave_max(float v) {
     return readlane_f(v, 63);
 }
 
+// Wavefront inclusive prefix sum: shifts by 1, 2, 4, 8 within rows of 16 lanes (zero from outside
+// the row), then the last lane of each row added into the rows after it.
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ int dpp_i0(int v) { return __builtin_amdgcn_update_dpp(0, v, CTRL, ROWS, 0xF, false); }
+__device__ __forceinline__ int wave_incl_scan(int v) {
+    v += dpp_i0<0x111>(v);         // row_shr:1
+    v += dpp_i0<0x112>(v);         // row_shr:2
+    v += dpp_i0<0x114>(v);         // row_shr:4
+    v += dpp_i0<0x118>(v);         // row_shr:8
+    v += dpp_i0<0x142, 0xA>(v);    // row_bcast:15 into rows 1 and 3
+    v += dpp_i0<0x143, 0xC>(v);    // row_bcast:31 into rows 2 and 3
+    return v;
+}
+
 __device__ __forceinline__ float score_of(float b, float nmv) {
     float r = fmaxf(b, 0.f);
     return (r * r) / nmv;
@@ -90,44 +104,87 @@ __device__ __forceinline__ float score_of(float b, float nmv) {
 // s_t = fl(fl(r_t r_t) rn_t), rn_t = fl(1 / nm_t), |s_t / q_t - 1| < 2^-22 in the normal range, so
 // every t with q_t = max q has s_t >= M (1 - 2^-20), M = max s.  Those candidates (nearly always a
 // single one) get the exact quotient and the first-maximum merge; M outside [1e-30, 1e30] makes
-// every template a candidate.  With DEV, b and m receive the winner's B[w][t] and nm_t.
+// every template a candidate.  When every lane holds at most one candidate, each takes one
+// quotient, its element selected without branches.  M = 0 with every fl(r_t r_t) = 0 (no positive
+// score) gives q_t = 0 for all t: the first maximum is t = 0.  With DEV, b and m receive the
+// winner's B[w][t] and nm_t.
 constexpr float kCandidate = 0.99999905f;   // 1 - 2^-20
 template <int NV, bool DEV>
 __device__ __forceinline__ void rescore(const floatx4* x, const floatx4* nmv, const floatx4* rnv, int T, float& best,
                                         int& bi, float& bb, float& bm) {
     const int lane = threadIdx.x & 63;
     float sa[NV][4];
-    float ml = -1.f;
+    float ml = -1.f, pl = 0.f;
 #pragma unroll
     for (int i = 0; i < NV; ++i)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const float r = fmaxf(x[i][e], 0.f);
-            const float sv = (r * r) * rnv[i][e];
-            sa[i][e] = (4 * lane + 256 * i + e < T) ? sv : -1.f;
+            const float p = r * r;
+            const float sv = p * rnv[i][e];
+            const bool in = 4 * lane + 256 * i + e < T;
+            sa[i][e] = in ? sv : -1.f;
             ml = fmaxf(ml, sa[i][e]);
+            pl = fmaxf(pl, in ? p : 0.f);
         }
     const float M = wave_max(ml);
-    const float thr = (M >= 1e-30f && M <= 1e30f) ? M * kCandidate : -0.5f;
     best = -1.f;
     bi = 0x7fffffff;
     bb = 0.f;
     bm = 1.f;
+    if (M == 0.f && wave_max(pl) == 0.f) {
+        // no positive score: every q_t = 0
+        best = 0.f;
+        bi = 0;
+        if constexpr (DEV) {
+            bb = readlane_f(x[0][0], 0);
+            bm = readlane_f(nmv[0][0], 0);
+        }
+        return;
+    }
+    const float thr = (M >= 1e-30f && M <= 1e30f) ? M * kCandidate : -0.5f;
+    uint32_t cm = 0;
 #pragma unroll
     for (int i = 0; i < NV; ++i)
 #pragma unroll
-        for (int e = 0; e < 4; ++e)
-            if (sa[i][e] >= thr) {
-                const float q = score_of(x[i][e], nmv[i][e]);
-                if (q > best) {   // ascending t within the lane: the first maximum stays
-                    best = q;
-                    bi = 4 * lane + 256 * i + e;
-                    if constexpr (DEV) {
-                        bb = x[i][e];
-                        bm = nmv[i][e];
+        for (int e = 0; e < 4; ++e) cm |= (sa[i][e] >= thr ? 1u : 0u) << (4 * i + e);
+    if (__ballot(__popc(cm) > 1) == 0) {
+        // at most one candidate per lane
+        if (cm) {
+            const int k = __ffs(cm) - 1;
+            float xv = 0.f, nv = 1.f;
+#pragma unroll
+            for (int i = 0; i < NV; ++i)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (k == 4 * i + e) {
+                        xv = x[i][e];
+                        nv = nmv[i][e];
+                    }
+            best = score_of(xv, nv);
+            bi = 4 * lane + 256 * (k >> 2) + (k & 3);
+            if constexpr (DEV) {
+                bb = xv;
+                bm = nv;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                if (sa[i][e] >= thr) {
+                    const float q = score_of(x[i][e], nmv[i][e]);
+                    if (q > best) {   // ascending t within the lane: the first maximum stays
+                        best = q;
+                        bi = 4 * lane + 256 * i + e;
+                        if constexpr (DEV) {
+                            bb = x[i][e];
+                            bm = nmv[i][e];
+                        }
                     }
                 }
-            }
+    }
     const unsigned long long cand = __ballot(bi != 0x7fffffff);
     if (__popcll(cand) == 1) {
         const int l = __ffsll((long long)cand) - 1;
@@ -150,29 +207,14 @@ __device__ __forceinline__ V ld_dev(const V* p) { return __hip_atomic_load(p, __
 template <typename V>
 __device__ __forceinline__ void st_dev(V* p, V v) { __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
 
-// Score rows B[w][t .. t+3]: plain 16-byte accesses between kernels; inside the persistent pursuit
-// (DEV), whose workgroups may update any column in any pass, device-coherent ones (the same cache
-// policy as the device-scope loads and stores above) through a buffer resource over B, whose bounds
-// the hardware checks (offsets past the end read 0 and drop the store).
-constexpr int kCoherent = 16;          // buffer cache policy: device scope
-constexpr int kRawBufferWord3 = 0x00020000;
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t b_rsrc(const PursuitArgs& a) {
-    const int64_t bytes = a.W * a.ldT * 4;
-    return __builtin_amdgcn_make_buffer_rsrc(a.B, (short)0, (int)min(bytes, (int64_t)INT_MAX), kRawBufferWord3);
-}
-template <bool DEV>
-__device__ __forceinline__ floatx4 ld_b(const PursuitArgs& a, __amdgpu_buffer_rsrc_t r, int64_t w, int t) {
-    if constexpr (DEV)
-        return __builtin_bit_cast(floatx4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)((w * a.ldT + t) * 4), 0,
-                                                                                 kCoherent));
-    else return *(const floatx4*)(a.B + w * a.ldT + t);
-}
-template <bool DEV>
-__device__ __forceinline__ void st_b(const PursuitArgs& a, __amdgpu_buffer_rsrc_t r, int64_t w, int t, floatx4 v) {
-    if constexpr (DEV)
-        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(uint32x4, v), r, (int)((w * a.ldT + t) * 4), 0,
-                                               kCoherent);
-    else *(floatx4*)(a.B + w * a.ldT + t) = v;
+// 16-byte loads of values other workgroups of the same launch stored with st_dev: buffer loads with
+// the device-scope cache policy (the same one as the device-scope loads above), through a resource
+// over `bytes` bytes at p (the hardware returns 0 past the end).
+constexpr int kCoherent = 16;                   // buffer cache policy: device scope
+constexpr int kRawBufferWord3 = 0x00020000;     // raw (untyped) buffer
+__device__ __forceinline__ uint32x4 ld_dev16(const void* p, int bytes, int off) {
+    const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, bytes, kRawBufferWord3);
+    return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, kCoherent);
 }
 
 // Cfmax/imax of column w: plain stores between kernels; inside the persistent pursuit (DEV)
@@ -194,22 +236,21 @@ __device__ __forceinline__ void store_score(const PursuitArgs& a, int64_t w, flo
 // ------------------------------------------------------------------------------------------------
 // full scan (first pass): one wavefront per column, float4 loads (ldT % 4 == 0)
 // ------------------------------------------------------------------------------------------------
-template <int NV, bool DEV = false>   // float4 vectors per lane: NV*256 >= T
+template <int NV>   // float4 vectors per lane: NV*256 >= T
 __device__ __forceinline__ void score_column(const PursuitArgs& a, int64_t w, int lane, const floatx4* nmv,
                                              const floatx4* rnv) {
     float best = 0.f, bb = 0.f, bm = 1.f;
     int bi = 0;
     if (w >= a.nt && w < a.W - a.nt) {
-        const __amdgpu_buffer_rsrc_t r = b_rsrc(a);
         floatx4 b[NV];
 #pragma unroll
         for (int i = 0; i < NV; ++i) {
             const int t = 4 * lane + 256 * i;
-            b[i] = (t < a.T) ? ld_b<DEV>(a, r, w, t) : floatx4{0.f, 0.f, 0.f, 0.f};
+            b[i] = (t < a.T) ? *(const floatx4*)(a.B + w * a.ldT + t) : floatx4{0.f, 0.f, 0.f, 0.f};
         }
-        rescore<NV, DEV>(b, nmv, rnv, a.T, best, bi, bb, bm);
+        rescore<NV, false>(b, nmv, rnv, a.T, best, bi, bb, bm);
     }
-    if (lane == 0) store_score<DEV>(a, w, best, bi, bb, bm);
+    if (lane == 0) store_score<false>(a, w, best, bi, bb, bm);
 }
 
 // template norms nm and their reciprocals rn in the lanes' float4 layout (1 past T)
@@ -395,11 +436,9 @@ __global__ __launch_bounds__(kFlagCols) void peaks_kernel(PursuitArgs a) {
 // wavefront: 2 (fewer items, for passes with many spikes) or 1 (half the work per lane, for
 // passes with few spikes); every column gets the same arithmetic either way)
 // ------------------------------------------------------------------------------------------------
-constexpr int kColsB = 8;     // column alignment of the persistent pursuit's ownership runs
 constexpr int kUpdateGrid = 8192;   // workgroups per update launch (grid-stride over work items)
 
-// A pass's spike list (ascending columns): plain loads (a list stored by an earlier kernel, or in
-// LDS) or device-scope loads (a list another workgroup of the same launch stored).
+// A pass's spike list (ascending columns), stored by the peak kernel.
 struct ListPlain {
     const int* col;
     const int* tmp;
@@ -408,20 +447,11 @@ struct ListPlain {
     __device__ int t(int i) const { return tmp[i]; }
     __device__ float a(int i) const { return amp[i]; }
 };
-struct ListDev {
-    const int* col;
-    const int* tmp;
-    const float* amp;
-    __device__ int c(int i) const { return ld_dev(col + i); }
-    __device__ int t(int i) const { return ld_dev(tmp + i); }
-    __device__ float a(int i) const { return ld_dev(amp + i); }
-};
 
 // Spikes of the pass covering column w within +-reach, starting at spike s (whose window covers
 // w): the last even-indexed (e) and odd-indexed (o) ones.  Spikes are in ascending column order,
 // so the covering ones are s, s+1, ...; c1, c2 are col[s+1], col[s+2] (INT_MAX past the end).
-template <typename L>
-__device__ __forceinline__ void covering(const L& l, int s, int n, int c1, int c2, int w, int reach,
+__device__ __forceinline__ void covering(const ListPlain& l, int s, int n, int c1, int c2, int w, int reach,
                                          int& e, int& o) {
     int last = s;
     if (c1 - reach <= w) {
@@ -444,8 +474,8 @@ __device__ __forceinline__ void covering(const L& l, int s, int n, int c1, int c
 // Columns wa and wb of this wavefront (va / vb: updated at all; sa / sb: the first spike of the
 // pass whose window covers the column, with ca1, ca2 / cb1, cb2 the columns of the next two
 // spikes): B -= amp ctc for the last even and last odd covering spike, then the column rescored.
-template <int NV, typename List, bool DEV>   // float4 vectors per lane over templates: NV*256 >= T
-__device__ __forceinline__ void update_two(const PursuitArgs& a, const List& l, int n, int wa, int sa, int ca1,
+template <int NV>   // float4 vectors per lane over templates: NV*256 >= T
+__device__ __forceinline__ void update_two(const PursuitArgs& a, const ListPlain& l, int n, int wa, int sa, int ca1,
                                            int ca2, bool va, int wb, int sb, int cb1, int cb2, bool vb,
                                            const floatx4* nmv, const floatx4* rnv) {
     const int lane = threadIdx.x & 63;
@@ -455,7 +485,6 @@ __device__ __forceinline__ void update_two(const PursuitArgs& a, const List& l, 
     if (va) covering(l, sa, n, ca1, ca2, wa, nt, ea, oa);
     if (vb) covering(l, sb, n, cb1, cb2, wb, nt, eb, ob);
     floatx4 ba[NV], bb[NV], cea[NV], coa[NV], ceb[NV], cob[NV];
-    const __amdgpu_buffer_rsrc_t r = b_rsrc(a);
 #pragma unroll
     for (int i = 0; i < NV; ++i) ba[i] = bb[i] = floatx4{0.f, 0.f, 0.f, 0.f};
     const float* qea = (ea >= 0) ? a.ctc + ((int64_t)l.t(ea) * L + (wa - l.c(ea) + nt)) * a.ldT : nullptr;
@@ -467,12 +496,12 @@ __device__ __forceinline__ void update_two(const PursuitArgs& a, const List& l, 
         const int t = 4 * lane + 256 * i;
         if (t < a.T) {
             if (va) {
-                ba[i] = ld_b<DEV>(a, r, wa, t);
+                ba[i] = *(const floatx4*)(a.B + (int64_t)wa * a.ldT + t);
                 if (qea) cea[i] = *(const floatx4*)(qea + t);
                 if (qoa) coa[i] = *(const floatx4*)(qoa + t);
             }
             if (vb) {
-                bb[i] = ld_b<DEV>(a, r, wb, t);
+                bb[i] = *(const floatx4*)(a.B + (int64_t)wb * a.ldT + t);
                 if (qeb) ceb[i] = *(const floatx4*)(qeb + t);
                 if (qob) cob[i] = *(const floatx4*)(qob + t);
             }
@@ -491,7 +520,7 @@ __device__ __forceinline__ void update_two(const PursuitArgs& a, const List& l, 
                 if (qoa) v = v - aoa * coa[i][e];
                 ba[i][e] = v;
             }
-            if (t < a.T) st_b<DEV>(a, r, wa, t, ba[i]);
+            if (t < a.T) *(floatx4*)(a.B + (int64_t)wa * a.ldT + t) = ba[i];
         }
         if (vb) {
 #pragma unroll
@@ -501,32 +530,32 @@ __device__ __forceinline__ void update_two(const PursuitArgs& a, const List& l, 
                 if (qob) v = v - aob * cob[i][e];
                 bb[i][e] = v;
             }
-            if (t < a.T) st_b<DEV>(a, r, wb, t, bb[i]);
+            if (t < a.T) *(floatx4*)(a.B + (int64_t)wb * a.ldT + t) = bb[i];
         }
     }
     // rescored inside [nt, W - nt) only (the reference's Cf there; the edges never hold a peak)
     if (va && wa >= nt && wa < a.W - nt) {
         float best, b_, m_;
         int bi;
-        rescore<NV, DEV>(ba, nmv, rnv, a.T, best, bi, b_, m_);
-        if (lane == 0) store_score<DEV>(a, wa, best, bi, b_, m_);
+        rescore<NV, false>(ba, nmv, rnv, a.T, best, bi, b_, m_);
+        if (lane == 0) store_score<false>(a, wa, best, bi, b_, m_);
     }
     if (vb && wb >= nt && wb < a.W - nt) {
         float best, b_, m_;
         int bi;
-        rescore<NV, DEV>(bb, nmv, rnv, a.T, best, bi, b_, m_);
-        if (lane == 0) store_score<DEV>(a, wb, best, bi, b_, m_);
+        rescore<NV, false>(bb, nmv, rnv, a.T, best, bi, b_, m_);
+        if (lane == 0) store_score<false>(a, wb, best, bi, b_, m_);
     }
 }
 
 // Per-pass update: the chunk [c0, c0 + 4 CPW) of the columns [lo, w1] whose first covering spike is
 // s (c1, c2 = the next two spikes' columns), CPW columns per wavefront.
-template <int NV, typename List, bool DEV, int CPW>
-__device__ __forceinline__ void update_score_columns(const PursuitArgs& a, const List& l, int s, int n, int c1, int c2,
+template <int NV, int CPW>
+__device__ __forceinline__ void update_score_columns(const PursuitArgs& a, const ListPlain& l, int s, int n, int c1, int c2,
                                                      int c0, int lo, int w1, const floatx4* nmv,
                                                      const floatx4* rnv) {
     const int wa = c0 + CPW * (int)(threadIdx.x >> 6), wb = wa + 1;
-    update_two<NV, List, DEV>(a, l, n, wa, s, c1, c2, wa >= lo && wa <= w1, wb, s, c1, c2,
+    update_two<NV>(a, l, n, wa, s, c1, c2, wa >= lo && wa <= w1, wb, s, c1, c2,
                               CPW == 2 && wb >= lo && wb <= w1, nmv, rnv);
 }
 
@@ -552,7 +581,7 @@ __device__ __forceinline__ void update_items(const PursuitArgs& a, const ListPla
         const int hi = (int)min((int64_t)ws + nt, a.W - 1);
         const int c0 = ws - nt + sub * kCols;
         const int w1 = min(c0 + kCols - 1, hi);
-        if (max(c0, lo) <= w1) update_score_columns<NV, ListPlain, false, CPW>(a, l, s, n, c1, c2, c0, lo, w1, nmv, rnv);
+        if (max(c0, lo) <= w1) update_score_columns<NV, CPW>(a, l, s, n, c1, c2, c0, lo, w1, nmv, rnv);
     }
 }
 
@@ -578,27 +607,31 @@ __global__ __launch_bounds__(256) void update_kernel(PursuitArgs a) {
 // ------------------------------------------------------------------------------------------------
 // persistent pursuit for short windows (W <= kPersistMaxW, the streaming batches): the first
 // scoring and every pass in one launch of G co-resident 512-thread workgroups, ONE grid barrier
-// per pass.
+// per pass, each workgroup's score rows resident in its LDS.
 //
-// Workgroup g owns the columns [a0, a1) (a fixed partition): it alone scores them, reads their B
-// rows and updates them, so B stays in the caches of one XCD and needs no coherence traffic.  What
-// workgroups exchange goes through device-scope loads and stores only (no cache write-back or
+// Workgroup g owns the columns [a0, a1), a0 = gW/G (a fixed partition).  Their B rows are
+// contiguous in B: they are copied into LDS at the start, updated there in every pass and copied
+// back at the end, so the pass loop reads no score row from HBM; the workgroup alone scores them.
+// What workgroups exchange goes through device-scope loads and stores only (no cache write-back or
 // invalidate at the barrier): per column its Cfmax, imax and the amplitude B[w][imax]/nm[imax] it
 // would have as a peak, in one of two rows chosen by pass parity.
 //
-// Pass p: every workgroup loads row p&1 (Cfmax, imax, amplitude of every column) into LDS, forms
-// the exact window max by doubling (window_max_pow2), applies the reference's test and compacts
-// the peaks in ascending order, so each holds the pass's list (columns, templates, amplitudes)
-// without exchanging it; workgroup 0 stores the list.  Each then updates its own columns' B rows
-// and publishes the rescored columns into row (p+1)&1, copying the values of its columns the pass
-// leaves unchanged; the barrier ends the pass.  A workgroup writes row (p+1)&1 while others may
-// still read row p&1, never the row they read, so the one barrier per pass orders every read of a
-// pass before the writes that overwrite it.  Every workgroup sees the same row, so all take the
-// same branches and leave the loop on the same pass.
+// Pass p: every workgroup loads the Cfmax row p&1 of the whole window into LDS (with -inf halos),
+// and imax/amplitude of its local range [a0 - nt, a1 + nt); it tests every column (peak_bits8) and
+// counts the peaks before each 8-column block, so it knows the index of every peak in the pass's
+// ascending list without exchanging it.  Its local list holds the peaks of the local range — the
+// spikes whose window [col - nt, col + nt] can reach its columns — and their indices' parity: a
+// column takes the last even-indexed, then the last odd-indexed covering spike
+// (template_matching.py:195-198).  The owner of a peak's column stores it in the spike list.  Each
+// workgroup then subtracts amp * ctc rows (streamed from HBM) from its covered columns' LDS rows,
+// rescores them and publishes every own column into row (p+1)&1; the barrier ends the pass.  A
+// workgroup writes row (p+1)&1 while others may still read row p&1, never the row they read, so the
+// one barrier per pass orders every read of a pass before the writes that overwrite it.  Every
+// workgroup sees the same row, so all take the same branches and leave the loop on the same pass.
 //
 // The barrier waits a bounded time: on a timeout (workgroups not co-resident) the kernel sets
-// counters[7], every workgroup leaves, and the host reports the error; the launcher caps G at the
-// occupancy limit so that this does not happen in normal operation.
+// counters[7], every workgroup leaves, and the host reports the error; the launcher runs only grids
+// the device holds at once with the LDS they need, so that this does not happen in normal operation.
 // ------------------------------------------------------------------------------------------------
 constexpr int kPT = 512;                                // threads per persistent workgroup
 constexpr int kPW = kPT / 64;                           // its wavefronts
@@ -608,27 +641,36 @@ __host__ __device__ constexpr int pad_len(int n) { return n + (n >> 5) + 1; }
 constexpr int kBarrierSpin = 1 << 20;                   // ~1 s of polling
 constexpr int kPersistLoads = (kPersistMaxW + 2 * kPeakHalo + kPT - 1) / kPT;   // row values per thread
 
-// Grid barrier number `gen` (1, 2, ...) over counters[6] (zero at launch).  Data shared between
-// workgroups is only accessed with device-scope atomics, so waiting for this wavefront's memory
-// operations before the workgroup meets orders them before the arrival; no cache maintenance.
-// Returns false after a timeout or when another workgroup has timed out.
-__device__ __forceinline__ bool grid_barrier(int* counters, int gen, int* s_ok) {
+// Grid barrier number `gen` (1, 2, ...) of this launch: each workgroup, once its memory operations
+// have completed, stores base + gen into its flag bar[g]; wavefront 0 then polls every workgroup's
+// flag (device-scope loads, 64 per instruction) until all have reached base + gen.  No atomics, so
+// arrivals do not queue on one address.  base = the launch's epoch (counters[kEpoch], advanced by
+// the counter reset before every pursuit) times 4096, so flags of earlier launches stay below it
+// (differences taken modulo 2^32).  Data shared between workgroups is only accessed with
+// device-scope operations, so no cache maintenance is needed.  Returns false after a timeout or
+// when another workgroup has timed out.
+__device__ __forceinline__ bool grid_barrier(const PursuitArgs& a, unsigned base, int gen, int* s_ok) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    if (threadIdx.x == 0) {
-        atomicAdd(counters + 6, 1);
-        const int target = gen * (int)gridDim.x;
+    const unsigned target = base + (unsigned)gen;
+    unsigned* const bar = (unsigned*)a.bar;
+    if (threadIdx.x == 0) st_dev(bar + blockIdx.x, target);
+    if (threadIdx.x < 64) {
         int ok = 1;
-        for (int it = 0; ld_dev(counters + 6) < target; ++it) {
-            if (it >= kBarrierSpin || ld_dev(counters + 7)) {
-                atomicExch(counters + 7, 1);
+        for (int it = 0;; ++it) {
+            bool behind = false;
+            for (int i = (int)threadIdx.x; i < (int)gridDim.x; i += 64)
+                behind |= (int)(ld_dev(bar + i) - target) < 0;
+            if (__ballot(behind) == 0) break;
+            if (it >= kBarrierSpin || __ballot(ld_dev(a.counters + 7) != 0) != 0) {
+                if (threadIdx.x == 0) atomicExch(a.counters + 7, 1);
                 ok = 0;
                 break;
             }
-            __builtin_amdgcn_s_sleep(2);
+            __builtin_amdgcn_s_sleep(1);
         }
-        *s_ok = ok;
+        if (threadIdx.x == 0) *s_ok = ok;
     }
     __syncthreads();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
@@ -642,111 +684,6 @@ __device__ __forceinline__ void use_row(PursuitArgs& b, const PursuitArgs& a, in
     b.amprow = a.prow_amp + (int64_t)par * kPersistMaxW;
 }
 
-// First spike of an ascending list whose +-nt window reaches column w (n if none does).
-__device__ __forceinline__ int first_cover(const ListPlain& l, int n, int w, int nt) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (l.col[mid] + nt < w) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
-// The persistent pursuit's update, spread evenly over every wavefront of the launch.  The columns
-// the pass changes are the union of its spikes' windows [col - nt, col + nt] within [0, W);
-// spike s adds the run [st_s, en_s], st_s = max(col_s - nt, col_{s-1} + nt + 1, 0),
-// en_s = min(col_s + nt, W - 1), and pfx[s] (exclusive prefix of the run lengths, pfx[n] = K) numbers
-// them in ascending order.  Covered column k goes to wavefront k / 2 mod (G * kPW) of the launch, two
-// per wavefront and round; its first covering spike is the s whose run holds it (no earlier
-// spike's window reaches past en_{s-1}).  Each column gets the update update_items gives it and,
-// inside [nt, W - nt), is rescored into the next row.
-__device__ __forceinline__ int run_start(const ListPlain& l, int s, int nt) {
-    return max(max(l.col[s] - nt, s > 0 ? l.col[s - 1] + nt + 1 : 0), 0);
-}
-
-// pfx[0 .. n] for the first n spikes of the list (n <= kPT * kPersistLoads); every thread returns K.
-__device__ __forceinline__ int covered_prefix(const ListPlain& l, int n, int nt, int W, int* pfx, int* wsum) {
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int per = (n + kPT - 1) / kPT;
-    const int s0 = tid * per;
-    int loc = 0;
-    for (int q = 0; q < per; ++q) {
-        const int sp = s0 + q;
-        if (sp < n) loc += max(0, min(l.col[sp] + nt, W - 1) - run_start(l, sp, nt) + 1);
-    }
-    int inc = loc;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int v = __shfl_up(inc, off, 64);
-        if (lane >= off) inc += v;
-    }
-    if (lane == 63) wsum[wave] = inc;
-    __syncthreads();
-    int woff = 0, K = 0;
-#pragma unroll
-    for (int k = 0; k < kPW; ++k) {
-        const int v = wsum[k];
-        if (k < wave) woff += v;
-        K += v;
-    }
-    int run = woff + inc - loc;
-    for (int q = 0; q < per; ++q) {
-        const int sp = s0 + q;
-        if (sp < n) {
-            pfx[sp] = run;
-            run += max(0, min(l.col[sp] + nt, W - 1) - run_start(l, sp, nt) + 1);
-        }
-    }
-    if (tid == 0) pfx[n] = K;
-    __syncthreads();
-    return K;
-}
-
-// the covered column numbered k: its column and first covering spike (pfx[s] <= k < pfx[s + 1])
-__device__ __forceinline__ int covered_column(const ListPlain& l, const int* pfx, int n, int nt, int k, int* s_out) {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {   // last s with pfx[s] <= k
-        const int mid = (lo + hi + 1) >> 1;
-        if (pfx[mid] <= k) lo = mid;
-        else hi = mid - 1;
-    }
-    *s_out = lo;
-    return run_start(l, lo, nt) + (k - pfx[lo]);
-}
-
-template <int NV>
-__device__ __forceinline__ void update_covered(const PursuitArgs& a, const ListPlain& l, int n, const int* pfx, int K,
-                                               const floatx4* nmv, const floatx4* rnv) {
-    const int nt = a.nt;
-    const int gw = (int)blockIdx.x * kPW + (int)(threadIdx.x >> 6), nw = (int)gridDim.x * kPW;
-    for (int k0 = 2 * gw; k0 < K; k0 += 2 * nw) {
-        int sa = 0, sb = 0;
-        const int wa = covered_column(l, pfx, n, nt, k0, &sa);
-        const bool vb = k0 + 1 < K;
-        const int wb = vb ? covered_column(l, pfx, n, nt, k0 + 1, &sb) : wa;
-        const int ca1 = (sa + 1 < n) ? l.col[sa + 1] : INT_MAX / 2, ca2 = (sa + 2 < n) ? l.col[sa + 2] : INT_MAX / 2;
-        const int cb1 = (sb + 1 < n) ? l.col[sb + 1] : INT_MAX / 2, cb2 = (sb + 2 < n) ? l.col[sb + 2] : INT_MAX / 2;
-        update_two<NV, ListPlain, true>(a, l, n, wa, sa, ca1, ca2, true, wb, sb, cb1, cb2, vb, nmv, rnv);
-    }
-}
-
-// The columns [a0, a1) the pass leaves unrescored (uncovered, or covered within nt of an edge) carry
-// their values (cf, im, am: the pass's row in LDS; cf padded, with its halo) into the next row.
-__device__ __forceinline__ void carry_own_columns(const PursuitArgs& a, const ListPlain& l, int n, int a0, int a1,
-                                                  const float* cf, const int* im, const float* am) {
-    const int nt = a.nt, W = (int)a.W;
-    for (int w = a0 + (int)threadIdx.x; w < a1; w += kPT) {
-        const int s = first_cover(l, n, w, nt);
-        const bool cov = s < n && l.col[s] - nt <= w;
-        if (!(cov && w >= nt && w < W - nt)) {
-            st_dev(a.cfmax + w, cf[pad32(kPeakHalo + w)]);
-            st_dev(a.imax + w, im[w]);
-            st_dev(a.amprow + w, am[w]);
-        }
-    }
-}
-
 // The persistent pursuit's peak test of one thread's 8 columns w = 8t + q (q < 8, w < W): the
 // window maximum pool_q over [w - nt, w + nt] (-inf outside the row) and the reference's test
 // pool > Th^2, |pool - cf| < 1e-9 (max-pool equality), as bits q of the result.  The row cf is
@@ -755,6 +692,9 @@ __device__ __forceinline__ void carry_own_columns(const PursuitArgs& a, const Li
 // the 8 columns contains C = [8t + 7 - nt, 8t + nt]: its maximum comes from the whole blocks inside
 // it and the two partial runs at its ends; column q adds head[q .. 6] (head[k] = column
 // 8t - nt + k) and tail[0 .. q - 1] (tail[k] = column 8t + nt + 1 + k).  Smaller nt scans directly.
+// Every load is unconditional (so all are in flight together): indices past a run's end read the
+// run's last entry again, and an empty partial run reads a column of the nearest whole block, both
+// inside C, which changes no maximum.
 constexpr int kBmHalo = 17;   // blocks of -inf kept on each side of the row (nt <= 127)
 __device__ __forceinline__ uint32_t peak_bits8(const float* cf, const float* bm, int t, int W, int nt, float th2) {
     const int H = kPeakHalo, c0 = 8 * t;
@@ -763,10 +703,15 @@ __device__ __forceinline__ uint32_t peak_bits8(const float* cf, const float* bm,
         const int bl = -((nt - 7) >> 3), bh = ((nt + 1) >> 3) - 1;
         const int ph = (nt - 7) & 7, pt = (nt + 1) & 7;
         float cm = -INFINITY;
-        for (int b = t + bl; b <= t + bh; ++b) cm = fmaxf(cm, bm[b + kBmHalo]);
+#pragma unroll
+        for (int j = 0; j < 16; ++j) cm = fmaxf(cm, bm[min(t + bl + j, t + bh) + kBmHalo]);
+        for (int b = t + bl + 16; b <= t + bh; ++b) cm = fmaxf(cm, bm[b + kBmHalo]);   // nt > 63
         const int s0 = c0 + 7 - nt, e1 = 8 * (t + bh + 1);
-        for (int k = 0; k < ph; ++k) cm = fmaxf(cm, cf[pad32(H + s0 + k)]);
-        for (int k = 0; k < pt; ++k) cm = fmaxf(cm, cf[pad32(H + e1 + k)]);
+#pragma unroll
+        for (int k = 0; k < 7; ++k) {
+            cm = fmaxf(cm, cf[pad32(H + s0 + min(k, max(ph - 1, 0)))]);
+            cm = fmaxf(cm, cf[pad32(H + e1 + min(k, pt - 1))]);
+        }
         float hv[7], tv[7];
 #pragma unroll
         for (int k = 0; k < 7; ++k) {
@@ -803,103 +748,249 @@ __device__ __forceinline__ uint32_t peak_bits8(const float* cf, const float* bm,
     return bits;
 }
 
-// LDS of the persistent pursuit for windows of W columns (floats/ints): the padded row with its
-// halos and a -inf tail of 16, the block maxima with their halos, the covered-run prefix, imax,
-// amplitude and the pass's list (column, template, amplitude)
+// LDS of the persistent pursuit (4-byte words) for W columns, row stride ldT and G workgroups: own
+// B rows [ceil(W/G)][ldT] | template norms and their reciprocals [ldT] each | the padded Cfmax row
+// with its halos and a -inf tail of 16 | block maxima with their halos | the pass's list: column |
+// template << 16, amplitude [W] | the own columns' Cfmax, imax, amplitude of the pass [ceil(W/G)]
+// each
 __host__ __device__ constexpr int persist_cf_len(int W) { return pad_len(W + 2 * kPeakHalo + 16); }
 __host__ __device__ constexpr int persist_bm_len(int W) { return (W + 7) / 8 + 2 * kBmHalo; }
+struct PersistLayout {
+    int own, nm, rn, cf, bm, lct, lamp, ocf, oim, oam, words;
+    __host__ __device__ PersistLayout(int W, int ldT, int G) {
+        own = (W + G - 1) / G;
+        nm = own * ldT;
+        rn = nm + ldT;
+        cf = rn + ldT;
+        bm = cf + persist_cf_len(W);
+        lct = bm + persist_bm_len(W);
+        lamp = lct + W;
+        ocf = lamp + W;
+        oim = ocf + own;
+        oam = oim + own;
+        words = oam + own;
+    }
+};
+
+// First spike of the pass's list (column in the low 16 bits, ascending) whose +-nt window reaches
+// column w (n if none does).
+__device__ __forceinline__ int first_cover_packed(const int* lct, int n, int w, int nt) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((lct[mid] & 0xFFFF) + nt < w) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Up to two covered own columns of one wavefront (n2 = 1 or 2): own index j, column w, last and
+// second-to-last covering spikes of the pass's list (pr = -1 when one spike covers it; list index =
+// index in the pass, so the covering spikes are its last even- and odd-indexed ones).  The LDS row
+// gets B -= amp ctc for the even-indexed spike, then for the odd-indexed one (ctc rows loaded
+// straight into registers, every load of both columns in flight at once), and is rescored into row
+// nx inside [nt, W - nt) (columns nearer an edge are carried by the caller).
+template <int NV, int NC>
+__device__ __forceinline__ void persist_update_pair(const PursuitArgs& a, const PursuitArgs& nx, floatx4* Bs4, int ld4,
+                                                    const int* lct, const float* lamp, int n2, const int* j,
+                                                    const int* w, const int* la, const int* pr, const floatx4* nm4,
+                                                    const floatx4* rn4) {
+    const int lane = threadIdx.x & 63;
+    const int nt = a.nt, L = 2 * nt + 1;
+    const floatx4 z = {0.f, 0.f, 0.f, 0.f};
+    const float* r1[NC];
+    const float* r2[NC];
+    float a1[NC], a2[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const int cc = c < n2 ? c : 0;
+        const int sl = la[cc], sp = pr[cc];
+        const float* ql = a.ctc + ((int64_t)(lct[sl] >> 16) * L + (w[cc] - (lct[sl] & 0xFFFF) + nt)) * a.ldT;
+        const float* qp = sp >= 0 ? a.ctc + ((int64_t)(lct[sp] >> 16) * L + (w[cc] - (lct[sp] & 0xFFFF) + nt)) * a.ldT
+                                  : nullptr;
+        const float al = lamp[sl], ap = sp >= 0 ? lamp[sp] : 0.f;
+        const bool swap = qp && (sl & 1);   // the even-indexed spike first
+        r1[c] = swap ? qp : ql;
+        r2[c] = swap ? ql : qp;
+        a1[c] = swap ? ap : al;
+        a2[c] = swap ? al : ap;
+    }
+    floatx4 x[NC][NV], c1[NC][NV], c2[NC][NV];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            const int t = 4 * lane + 256 * i;
+            x[c][i] = c1[c][i] = c2[c][i] = z;
+            if (c < n2 && t < a.T) {
+                c1[c][i] = *(const floatx4*)(r1[c] + t);
+                if (r2[c]) c2[c][i] = *(const floatx4*)(r2[c] + t);
+                x[c][i] = Bs4[j[c] * ld4 + lane + 64 * i];
+            }
+        }
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int i = 0; i < NV; ++i) {
+            floatx4 v = x[c][i];
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[e] = v[e] - a1[c] * c1[c][i][e];
+            if (r2[c]) {
+#pragma unroll
+                for (int e = 0; e < 4; ++e) v[e] = v[e] - a2[c] * c2[c][i][e];
+            }
+            x[c][i] = v;
+            if (c < n2 && 4 * lane + 256 * i < a.T) Bs4[j[c] * ld4 + lane + 64 * i] = v;
+        }
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        if (c >= n2) break;
+        if (w[c] >= nt && w[c] < a.W - nt) {
+            floatx4 nmv[NV], rnv[NV];
+#pragma unroll
+            for (int i = 0; i < NV; ++i) {
+                const bool in = lane + 64 * i < ld4;
+                nmv[i] = in ? nm4[lane + 64 * i] : floatx4{1.f, 1.f, 1.f, 1.f};
+                rnv[i] = in ? rn4[lane + 64 * i] : floatx4{1.f, 1.f, 1.f, 1.f};
+            }
+            float best, b_, m_;
+            int bi;
+            rescore<NV, true>(x[c], nmv, rnv, a.T, best, bi, b_, m_);
+            if (lane == 0) store_score<true>(nx, w[c], best, bi, b_, m_);
+        }
+    }
+}
 
 template <int NV>
 __global__ __launch_bounds__(kPT) void pursuit_persistent_kernel(PursuitArgs a, int first_pass, int max_passes) {
-    // cf [persist_cf_len] | bm [persist_bm_len] | pfx [W + 1] | im [W] | am [W] | list col, tmp, amp [W]
-    extern __shared__ float psm[];
+    extern __shared__ floatx4 psm4[];
+    float* const psm = (float*)psm4;
     __shared__ int wsum[kPW];
     __shared__ int s_ok;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int W = (int)a.W, H = kPeakHalo;
+    const int W = (int)a.W, H = kPeakHalo, nt = a.nt, L = 2 * nt + 1;
     const int N = W + 2 * H;
+    const int G = (int)gridDim.x, g = (int)blockIdx.x;
     const int nbk = (W + 7) / 8;   // blocks of 8 columns, one per thread
-    float* cf = psm;
-    float* bm = cf + persist_cf_len(W);
-    int* pfx = (int*)(bm + persist_bm_len(W));
-    int* im = pfx + W + 1;
-    float* am = (float*)(im + W);
-    int* lcol = (int*)(am + W);
-    int* ltmp = lcol + W;
-    float* lamp = (float*)(ltmp + W);
+    const PersistLayout ly(W, (int)a.ldT, G);
+    float* cf = psm + ly.cf;
+    float* bm = psm + ly.bm;
+    int* lct = (int*)(psm + ly.lct);
+    float* lamp = psm + ly.lamp;
+    float* ocf = psm + ly.ocf;
+    int* oim = (int*)(psm + ly.oim);
+    float* oam = psm + ly.oam;
     const bool lead = blockIdx.x == 0 && tid == 0;
     if (a.ts && tid == 0) a.ts[2 * blockIdx.x] = __builtin_amdgcn_s_memrealtime();   // measurement mode
-    // this workgroup's columns: kColsB-aligned runs, G of them cover [0, W)
-    const int Wc = (W + (int)gridDim.x * kColsB - 1) / ((int)gridDim.x * kColsB) * kColsB;
-    const int a0 = min(W, (int)blockIdx.x * Wc), a1 = min(W, a0 + Wc);
-    floatx4 nmv[NV], rnv[NV];
-    load_norms<NV>(a, nmv, rnv);
+    // own columns g + G j (j < nown): strided, so that every workgroup holds a share of the columns
+    // a pass updates wherever its spikes are; their B rows into LDS
+    const int nown = g < W ? (W - 1 - g) / G + 1 : 0;
+    const int ld4 = (int)(a.ldT / 4);
+    for (int j = wave; j < nown; j += kPW) {
+        const floatx4* src = (const floatx4*)(a.B + (int64_t)(g + G * j) * a.ldT);
+        for (int i = lane; i < ld4; i += 64) psm4[j * ld4 + i] = src[i];
+    }
+    // template norms and reciprocals in the lanes' float4 layout (1 past T)
+    floatx4* const nm4 = psm4 + ly.nm / 4;
+    floatx4* const rn4 = psm4 + ly.rn / 4;
+    if (wave == 0) {
+        floatx4 nmv[NV], rnv[NV];
+        load_norms<NV>(a, nmv, rnv);
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+            if (lane + 64 * i < ld4) {
+                nm4[lane + 64 * i] = nmv[i];
+                rn4[lane + 64 * i] = rnv[i];
+            }
+    }
     int par = first_pass & 1;
     PursuitArgs cur = a;
     use_row(cur, a, par);
     int base = 0, np = 0;
     bool done = false;
+    // constant -inf parts: the row's halos and tail, the block halos
+    for (int i = tid; i < H; i += kPT) cf[pad32(i)] = -INFINITY;
+    for (int i = H + 8 * nbk + tid; i < N + 16; i += kPT) cf[pad32(i)] = -INFINITY;
+    if (tid < kBmHalo) {
+        bm[tid] = -INFINITY;
+        bm[kBmHalo + nbk + tid] = -INFINITY;
+    }
+    __syncthreads();
     if (first_pass == 0) {
-        // first pass: every column scored, one wavefront per column over the whole launch
-        for (int w = (int)blockIdx.x * kPW + wave; w < W; w += (int)gridDim.x * kPW) score_column<NV, true>(cur, w, lane, nmv, rnv);
+        // first pass: the own columns scored from LDS, one wavefront per column
+        for (int j = wave; j < nown; j += kPW) {
+            const int w = g + G * j;
+            float best = 0.f, bb = 0.f, bmv = 1.f;
+            int bi = 0;
+            if (w >= nt && w < W - nt) {
+                const floatx4* r = psm4 + j * ld4;
+                floatx4 x[NV], nmv[NV], rnv[NV];
+#pragma unroll
+                for (int i = 0; i < NV; ++i) {
+                    const bool in = lane + 64 * i < ld4;
+                    x[i] = (4 * lane + 256 * i < a.T) ? r[lane + 64 * i] : floatx4{0.f, 0.f, 0.f, 0.f};
+                    nmv[i] = in ? nm4[lane + 64 * i] : floatx4{1.f, 1.f, 1.f, 1.f};
+                    rnv[i] = in ? rn4[lane + 64 * i] : floatx4{1.f, 1.f, 1.f, 1.f};
+                }
+                rescore<NV, true>(x, nmv, rnv, a.T, best, bi, bb, bmv);
+            }
+            if (lane == 0) store_score<true>(cur, w, best, bi, bb, bmv);
+        }
     } else {
         // after first_pass passes of the per-pass kernels (their last pass committed as the peak
         // kernel commits it): own columns' Cfmax/imax as they left them, with their amplitudes
         base = a.counters[0] + a.counters[1];
         np = a.counters[4];
         done = a.counters[3] != 0;
-        for (int w = a0 + tid; w < a1; w += kPT) {
-            const int i = a.imax[w];
-            store_score<true>(cur, w, a.cfmax[w], i, ld_dev(a.B + (int64_t)w * a.ldT + i), a.nm[i]);
+        for (int j = tid; j < nown; j += kPT) {
+            const int w = g + G * j, i = a.imax[w];
+            store_score<true>(cur, w, a.cfmax[w], i, psm[(int64_t)j * a.ldT + i], a.nm[i]);
         }
     }
-    // constant -inf parts: the row's tail past N and the block halos
-    if (tid < 16) cf[pad32(N + tid)] = -INFINITY;
-    if (tid < kBmHalo) {
-        bm[tid] = -INFINITY;
-        bm[kBmHalo + nbk + tid] = -INFINITY;
-    }
+    const unsigned bbase = (unsigned)a.counters[kEpoch] * 4096u;
     int gen = 1;
-    if (!grid_barrier(a.counters, gen++, &s_ok)) return;
+    if (!grid_barrier(a, bbase, gen++, &s_ok)) return;
+    // this thread's block: columns c0 .. c0 + 7, r0 = c0 mod G (the first is owned when r0 == g)
+    const int c0 = 8 * tid, r0 = c0 % G;
     for (int pass = first_pass; pass < max_passes && !done; ++pass) {
-        {
-            // the row (with -inf halos) into LDS, every load in flight at once
-            float v[kPersistLoads], b[kPersistLoads];
-            int m[kPersistLoads];
-#pragma unroll
-            for (int r = 0; r < kPersistLoads; ++r) {
-                const int w = tid + kPT * r - H;
-                const bool in = w >= 0 && w < W;
-                v[r] = in ? ld_dev(cur.cfmax + w) : -INFINITY;
-                m[r] = in ? ld_dev(cur.imax + w) : 0;
-                b[r] = in ? ld_dev(cur.amprow + w) : 0.f;
-            }
-#pragma unroll
-            for (int r = 0; r < kPersistLoads; ++r) {
-                const int i = tid + kPT * r, w = i - H;
-                if (i < N) cf[pad32(i)] = v[r];
-                if (w >= 0 && w < W) {
-                    im[w] = m[r];
-                    am[w] = b[r];
-                }
-            }
-        }
-        __syncthreads();
+        // the row: each thread its block's Cfmax (into LDS, with the block maximum), imax and
+        // amplitudes (kept in registers), every load in flight at once
+        float cv[8], av[8];
+        int iv[8];
         if (tid < nbk) {
-            float m = cf[pad32(H + 8 * tid)];
+            floatx4 fc[2], fa[2];
+            uint32x4 ui[2];
 #pragma unroll
-            for (int k = 1; k < 8; ++k) m = fmaxf(m, cf[pad32(H + 8 * tid + k)]);
+            for (int h = 0; h < 2; ++h) {
+                // whole blocks stay inside the row's kPersistMaxW entries (8 ceil(W/8) <= kPersistMaxW)
+                fc[h] = __builtin_bit_cast(floatx4, ld_dev16(cur.cfmax, 4 * kPersistMaxW, 4 * (c0 + 4 * h)));
+                ui[h] = ld_dev16(cur.imax, 4 * kPersistMaxW, 4 * (c0 + 4 * h));
+                fa[h] = __builtin_bit_cast(floatx4, ld_dev16(cur.amprow, 4 * kPersistMaxW, 4 * (c0 + 4 * h)));
+            }
+            float m = -INFINITY;
+            int r = r0;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const bool in = c0 + q < W;
+                cv[q] = in ? fc[q >> 2][q & 3] : -INFINITY;
+                iv[q] = (int)ui[q >> 2][q & 3];
+                av[q] = fa[q >> 2][q & 3];
+                cf[pad32(H + c0 + q)] = cv[q];
+                m = fmaxf(m, cv[q]);
+                if (r == g && in) {
+                    const int j = (c0 + q) / G;
+                    ocf[j] = cv[q];
+                    oim[j] = iv[q];
+                    oam[j] = av[q];
+                }
+                if (++r == G) r = 0;
+            }
             bm[kBmHalo + tid] = m;
         }
         __syncthreads();
-        uint32_t bits = tid < nbk ? peak_bits8(cf, bm, tid, W, a.nt, a.th2) : 0u;
+        const uint32_t bits = tid < nbk ? peak_bits8(cf, bm, tid, W, nt, a.th2) : 0u;
         const int cnt = __popc(bits);
-        int inc = cnt;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const int vv = __shfl_up(inc, off, 64);
-            if (lane >= off) inc += vv;
-        }
+        const int inc = wave_incl_scan(cnt);
         if (lane == 63) wsum[wave] = inc;
         __syncthreads();
         int woff = 0, n = 0;
@@ -909,26 +1000,33 @@ __global__ __launch_bounds__(kPT) void pursuit_persistent_kernel(PursuitArgs a, 
             if (k < wave) woff += vv;
             n += vv;
         }
-        int pos = woff + inc - cnt;
-        while (bits) {
-            const int q = __ffs(bits) - 1;
-            bits &= bits - 1;
-            const int w = tid * 8 + q;
-            const int t = im[w];
-            const float amp = am[w];
-            lcol[pos] = w;
-            ltmp[pos] = t;
-            lamp[pos] = amp;
-            if (blockIdx.x == 0 && base + pos < a.cap) {
-                st_dev(a.sp_col + base + pos, w);
-                st_dev(a.sp_tmp + base + pos, t);
-                st_dev(a.sp_amp + base + pos, amp);
-            }
-            ++pos;
-        }
         if (n == 0) {
             if (lead) a.counters[3] = 1;
             break;
+        }
+        // the pass's list (ascending columns: list index = index in the pass) of the spikes applied,
+        // and the owners' entries of the stored spike list
+        const int nu = max(0, min(n, a.cap - base));   // spikes stored, as update_kernel applies them
+        {
+            int gi = woff + inc - cnt;   // peaks before this thread's block
+            int r = r0;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                if ((bits >> q) & 1u) {
+                    const int c = c0 + q;
+                    if (gi < nu) {
+                        lct[gi] = c | (iv[q] << 16);
+                        lamp[gi] = av[q];
+                    }
+                    if (r == g && base + gi < a.cap) {
+                        a.sp_col[base + gi] = c;
+                        a.sp_tmp[base + gi] = iv[q];
+                        a.sp_amp[base + gi] = av[q];
+                    }
+                    ++gi;
+                }
+                if (++r == G) r = 0;
+            }
         }
         if (lead) {
             if (base + n > a.cap) a.counters[2] = 1;
@@ -936,23 +1034,63 @@ __global__ __launch_bounds__(kPT) void pursuit_persistent_kernel(PursuitArgs a, 
             a.pass_off[np + 1] = min(base + n, a.cap);
         }
         ++np;
-        __syncthreads();   // the pass's list is complete in LDS
-        const int nu = max(0, min(n, a.cap - base));   // spikes stored, as update_kernel applies them
+        __syncthreads();   // the list is complete
         PursuitArgs nxt = a;
         use_row(nxt, a, par ^ 1);
-        const ListPlain lst{lcol, ltmp, lamp};
-        const int K = covered_prefix(lst, nu, a.nt, W, pfx, wsum);
-        carry_own_columns(nxt, lst, nu, a0, a1, cf, im, am);
-        update_covered<NV>(nxt, lst, nu, pfx, K, nmv, rnv);
+        // own columns j = wave + kPW l (lane l of the wavefront): the pass's coverage of each (its last
+        // and second-to-last covering spikes: the last even- and odd-indexed ones, as covering spikes
+        // are consecutive), carried into the next row when not rescored (uncovered, or covered within
+        // nt of an edge); the covered ones updated two at a time by the wavefront
+        for (int j0 = wave; j0 < nown; j0 += 64 * kPW) {
+            const int jl = j0 + kPW * lane, wl = g + G * jl;
+            int last = -1, prev = -1;
+            if (jl < nown) {
+                const int s0 = first_cover_packed(lct, nu, wl, nt);
+                if (s0 < nu && (lct[s0] & 0xFFFF) - nt <= wl) {
+                    last = s0;
+                    while (last + 1 < nu && (lct[last + 1] & 0xFFFF) - nt <= wl) ++last;
+                    prev = last > s0 ? last - 1 : -1;
+                }
+                if (!(last >= 0 && wl >= nt && wl < W - nt)) {
+                    st_dev(nxt.cfmax + wl, ocf[jl]);
+                    st_dev(nxt.imax + wl, oim[jl]);
+                    st_dev(nxt.amprow + wl, oam[jl]);
+                }
+            }
+            constexpr int NC = NV <= 3 ? 2 : 1;   // columns per step (registers)
+            for (uint64_t m = __ballot(last >= 0); m;) {
+                int jj[NC], ww[NC], ll[NC], pp[NC], n2 = 0;
+#pragma unroll
+                for (int c = 0; c < NC; ++c) {
+                    jj[c] = ww[c] = ll[c] = 0;
+                    pp[c] = -1;
+                    if (m) {
+                        const int l = __ffsll((long long)m) - 1;
+                        m &= m - 1;
+                        jj[c] = __builtin_amdgcn_readlane(jl, l);
+                        ww[c] = __builtin_amdgcn_readlane(wl, l);
+                        ll[c] = __builtin_amdgcn_readlane(last, l);
+                        pp[c] = __builtin_amdgcn_readlane(prev, l);
+                        ++n2;
+                    }
+                }
+                persist_update_pair<NV, NC>(a, nxt, psm4, ld4, lct, lamp, n2, jj, ww, ll, pp, nm4, rn4);
+            }
+        }
         base += n;
         par ^= 1;
         cur = nxt;
-        if (pass + 1 < max_passes && !grid_barrier(a.counters, gen++, &s_ok)) return;
+        if (pass + 1 < max_passes && !grid_barrier(a, bbase, gen++, &s_ok)) return;
     }
     if (lead) {
         a.counters[0] = base;
         a.counters[1] = 0;
         a.counters[4] = np;
+    }
+    __syncthreads();   // every wavefront's rows final
+    for (int j = wave; j < nown; j += kPW) {
+        floatx4* dst = (floatx4*)(a.B + (int64_t)(g + G * j) * a.ldT);
+        for (int i = lane; i < ld4; i += 64) dst[i] = psm4[j * ld4 + i];
     }
     if (a.ts) {
         __builtin_amdgcn_s_waitcnt(0);
@@ -1188,10 +1326,11 @@ hipError_t launch_update(const PursuitArgs& a, int grid, hipStream_t s, hipEvent
     return hipGetLastError();
 }
 
-size_t persistent_lds(int64_t W) {
-    const int w = (int)W;
-    return sizeof(float) * (persist_cf_len(w) + persist_bm_len(w)) + sizeof(int) * (w + 1) +
-           (sizeof(int) + sizeof(float)) * w + (2 * sizeof(int) + sizeof(float)) * w;
+// LDS of the persistent pursuit at grid G (bytes) and the most a workgroup may take (gfx950: 160 KB,
+// less the kernel's static arrays)
+constexpr size_t kPersistLdsMax = 160 * 1024 - 256;
+size_t persistent_lds(int64_t W, int64_t ldT, int G) {
+    return sizeof(float) * (size_t)PersistLayout((int)W, (int)ldT, G).words;
 }
 
 typedef void (*PersistentFn)(PursuitArgs, int, int);
@@ -1207,34 +1346,70 @@ PersistentFn persistent_fn(int T) {
 
 bool persistent_supports(int T) { return persistent_fn(T) != nullptr; }
 
+// workgroups of fn with `lds` bytes each that the device holds at once
+static hipError_t resident_blocks(PersistentFn fn, size_t lds, int* n) {
+    int dev = 0, cus = 0, per = 0;
+    *n = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPersistLdsMax);
+    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)fn, kPT, lds);
+    if (e == hipSuccess) *n = per * cus;
+    return e;
+}
+
+// The largest grid g <= want whose workgroups, with the LDS that grid needs (fewer workgroups hold
+// more score rows each), the device holds at once; 0 if none does.
+static hipError_t coresident_grid(PersistentFn fn, int64_t W, int64_t ldT, int want, int* grid) {
+    *grid = 0;
+    int g = want;
+    for (int it = 0; it < 16 && g >= 1; ++it) {
+        const size_t lds = persistent_lds(W, ldT, g);
+        if (lds > kPersistLdsMax) return hipSuccess;   // a smaller grid needs more LDS still
+        int cap = 0;
+        const hipError_t e = resident_blocks(fn, lds, &cap);
+        if (e != hipSuccess) return e;
+        if (cap >= g) {
+            *grid = g;
+            return hipSuccess;
+        }
+        g = cap;
+    }
+    return hipSuccess;
+}
+
 hipError_t persistent_max_grid(int T, int64_t W, int* grid) {
     const PersistentFn fn = persistent_fn(T);
     *grid = 0;
     if (!fn) return hipSuccess;   // more than 1024 templates: per-pass kernels only
     if (W < 1 || W > kPersistMaxW) return hipErrorInvalidValue;
-    int dev = 0, cus = 0, per = 0;
+    int dev = 0, cus = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e == hipSuccess)
-        e = hipFuncSetAttribute((const void*)fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)persistent_lds(W));
-    if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)fn, kPT, persistent_lds(W));
     if (e != hipSuccess) return e;
-    *grid = per * cus;
-    return hipSuccess;
+    return coresident_grid(fn, W, (T + 3) / 4 * 4, 4 * cus, grid);
+}
+
+hipError_t persistent_grid(const PursuitArgs& a, int want, int* grid) {
+    const PersistentFn fn = persistent_fn(a.T);
+    *grid = 0;
+    if (!fn || a.W < 1 || a.W > kPersistMaxW || want < 1) return hipSuccess;
+    return coresident_grid(fn, a.W, a.ldT, want, grid);
 }
 
 hipError_t launch_pursuit_persistent(const PursuitArgs& a, int grid, int first_pass, int max_passes, hipStream_t s,
                                      int* used) {
     const PersistentFn fn = persistent_fn(a.T);
     if (!fn || a.W > kPersistMaxW || a.nt > kPeakHalo || grid < 1 || first_pass < 0 || first_pass >= max_passes ||
-        !a.prow_cf || !a.prow_im || !a.prow_amp)
+        !a.prow_cf || !a.prow_im || !a.prow_amp || !a.bar || a.ldT % 4 != 0)
         return hipErrorInvalidValue;
-    int cap = 0;
-    hipError_t e = persistent_max_grid(a.T, a.W, &cap);
+    int g = 0;
+    const hipError_t e = coresident_grid(fn, a.W, a.ldT, std::min(grid, kPersistMaxGrid), &g);
     if (e != hipSuccess) return e;
-    if (cap < 1) return hipErrorInvalidConfiguration;
-    if (used) *used = std::min(grid, cap);
-    hipLaunchKernelGGL(fn, dim3(std::min(grid, cap)), dim3(kPT), persistent_lds(a.W), s, a, first_pass, max_passes);
+    if (g < 1) return hipErrorInvalidConfiguration;   // the score rows do not fit the LDS of that grid
+    if (used) *used = g;
+    hipLaunchKernelGGL(fn, dim3(g), dim3(kPT), persistent_lds(a.W, a.ldT, g), s, a, first_pass, max_passes);
     return hipGetLastError();
 }
 
@@ -1365,12 +1540,18 @@ hipError_t launch_stage_spikes(const StageArgs& a, hipStream_t s) {
 }
 
 // counters[0..7] = 0 (a fill kernel, so that the sequence can be captured in a graph)
-__global__ void reset_counters_kernel(int* c) {
-    if (threadIdx.x < kCounters) c[threadIdx.x] = 0;
+// Counters zeroed and the persistent pursuit's barrier epoch advanced; every 2^16 epochs the barrier
+// flags are raised to the new epoch's base, so that no flag falls 2^31 behind a barrier's target.
+__global__ void reset_counters_kernel(int* c, int* bar) {
+    const int e = c[kEpoch] + 1;
+    __syncthreads();
+    if (threadIdx.x < kCounters) c[threadIdx.x] = threadIdx.x == kEpoch ? e : 0;
+    if (bar && (e & 0xFFFF) == 0)
+        for (int i = threadIdx.x; i < kPersistMaxGrid; i += blockDim.x) bar[i] = (int)((unsigned)e * 4096u);
 }
 
-hipError_t launch_reset_counters(int* counters, hipStream_t s) {
-    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, counters);
+hipError_t launch_reset_counters(int* counters, hipStream_t s, int* bar) {
+    hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, counters, bar);
     return hipGetLastError();
 }
 
