@@ -99,7 +99,7 @@ struct Handle {
     DevBuf<int> sp_col, sp_tmp, counters, pass_off;
     DevBuf<float> prow_cf, prow_amp;   // persistent pursuit rows [2][kPersistMaxW] (pass parity)
     DevBuf<int> prow_im;
-    DevBuf<int> pbar;                   // persistent pursuit barrier flags [kPersistMaxGrid]
+    DevBuf<int> pbar;                   // persistent pursuit barrier flags [kPersistMaxGrid][kBarStride]
     DevBuf<float> sp_amp, tF, xs, ys;
     int* h_counters = nullptr;   // pinned
     // pinned readback staging of the batched/streaming path (first stage_cap spikes)
@@ -397,8 +397,8 @@ lss_status create_impl(const lss_params* p, const lss_artifacts* a, Handle** out
     HIP_TRY(h->prow_cf.alloc(2 * (size_t)lss::kPersistMaxW));
     HIP_TRY(h->prow_im.alloc(2 * (size_t)lss::kPersistMaxW));
     HIP_TRY(h->prow_amp.alloc(2 * (size_t)lss::kPersistMaxW));
-    HIP_TRY(h->pbar.alloc((size_t)lss::kPersistMaxGrid));
-    HIP_TRY(hipMemset(h->pbar.p, 0, sizeof(int) * lss::kPersistMaxGrid));
+    HIP_TRY(h->pbar.alloc((size_t)lss::kPersistMaxGrid * lss::kBarStride));
+    HIP_TRY(hipMemset(h->pbar.p, 0, sizeof(int) * lss::kPersistMaxGrid * lss::kBarStride));
     HIP_TRY(h->tF.alloc((size_t)p->max_spikes * nC * K));
     HIP_TRY(h->xs.alloc(p->max_spikes));
     HIP_TRY(h->ys.alloc(p->max_spikes));
@@ -842,24 +842,22 @@ lss_status capture(Batcher& b, lss_status (*seg)(Batcher&), hipGraph_t* g, hipGr
 
 // Assembles probe p's window for batch start t0 into its pinned staging buffer: one contiguous
 // copy, after the rows that repeat sample 0 when the window starts before the recording.
-// Device mode: the window's rows (samples t0 - nt .. t0 + batch + nt - 1, sample 0 repeated
-// before the recording starts) copied device to device into the handle's window buffer, on the
-// stream the step's graph is launched on.
-lss_status assemble_dev(Probe& p, int64_t t0, hipStream_t s) {
+// Device mode: the segments that place the window's rows (samples t0 - nt .. t0 + batch + nt - 1,
+// sample 0 repeated before the recording starts) into the handle's window buffer, appended to
+// segs; one gather launch then copies every probe's segments.
+lss_status assemble_dev(Probe& p, int64_t t0, std::vector<lss::GatherSeg>& segs) {
     Handle& h = *p.h;
     const int C = h.C, nt = h.nt;
-    const size_t row = sizeof(int16_t) * C;
     int64_t r = 0;
-    for (; r < p.W && t0 - nt + r < 0; ++r)
-        HIP_TRY(hipMemcpyAsync(h.win.p + r * C, p.dchunks.front().first, row, hipMemcpyDeviceToDevice, s));
+    while (r < p.W && t0 - nt + r < 0) ++r;
+    if (r > 0) segs.push_back({h.win.p, p.dchunks.front().first, r * C, (int64_t)C, 0});
     for (size_t k = 0; k < p.dchunks.size() && r < p.W; ++k) {
         const int64_t c0 = p.dchunks[k].second, c1 = c0 + p.dlen[k];
         const int64_t need = t0 - nt + r;   // next sample to place
         if (c1 <= need) continue;
         const int64_t from = std::max(need, c0);
         const int64_t rows = std::min(c1 - from, p.W - r);
-        HIP_TRY(hipMemcpyAsync(h.win.p + r * C, p.dchunks[k].first + (from - c0) * C, row * rows,
-                               hipMemcpyDeviceToDevice, s));
+        segs.push_back({h.win.p + r * C, p.dchunks[k].first + (from - c0) * C, rows * C, 0, 0});
         r += rows;
     }
     if (r != p.W) return fail(LSS_ERR_STATE, "device window incomplete");
@@ -1755,13 +1753,15 @@ lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready) {
     using clk = std::chrono::steady_clock;
     auto ms_since = [](clk::time_point t) { return std::chrono::duration<float, std::milli>(clk::now() - t).count(); };
     auto t_start = clk::now();
+    std::vector<lss::GatherSeg> segs;
     for (auto& p : b.pr) {
         if (p.mode == 1) {
             assemble(p, b.t0);
-        } else if (lss_status st = assemble_dev(p, b.t0, b.main)) {
+        } else if (lss_status st = assemble_dev(p, b.t0, segs)) {
             return st;
         }
     }
+    if (!segs.empty()) HIP_TRY(lss::launch_gather(segs.data(), (int)segs.size(), b.main));
     b.part_ms[0] = ms_since(t_start);
     auto t1 = clk::now();
     // segment A

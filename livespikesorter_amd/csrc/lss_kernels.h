@@ -18,6 +18,24 @@ hipError_t launch_channel_sums(const int16_t* win, int64_t W, int C, long long* 
 hipError_t launch_car_transpose(const int16_t* win, int64_t W, int C, const long long* sums,
                                 int do_car, float* out, int64_t ldo, hipStream_t s);
 
+// Window assembly from device samples: segment k writes n int16 at dst, from src (period 0:
+// contiguous) or repeating the period-element row src; up to kGatherMaxSeg segments per launch
+// (launch_gather splits longer lists).  chunk0 is set by the launcher.
+struct GatherSeg {
+    int16_t* dst;
+    const int16_t* src;
+    int64_t n;
+    int64_t period;
+    int64_t chunk0;
+};
+constexpr int kGatherMaxSeg = 32;
+struct GatherArgs {
+    GatherSeg seg[kGatherMaxSeg];
+    int n;
+    int64_t chunks;
+};
+hipError_t launch_gather(const GatherSeg* segs, int n, hipStream_t s);
+
 // Overlap-save highpass: y[c][i] = sum_{j=-Llo..Lhi} g[j] x[c][(i+j) mod W].
 constexpr int kFftN = 8192;          // FFT length of one segment
 constexpr int kFftLog2 = 13;
@@ -73,7 +91,7 @@ struct PursuitArgs {
     // to ts[2b], ts[2b + 1] (NULL otherwise)
     unsigned long long* ts;
     // persistent pursuit: Cfmax, imax and amplitude rows [2][kPersistMaxW], one per pass parity,
-    // and the workgroups' barrier flags [kPersistMaxGrid]
+    // and the workgroups' barrier flags [kPersistMaxGrid][kBarStride]
     float* prow_cf;
     int* prow_im;
     float* prow_amp;
@@ -93,7 +111,8 @@ struct PursuitArgs {
 constexpr int kCounters = 16;
 constexpr int kEpoch = 15;
 // counters[0..kCounters) = 0 except counters[kEpoch], incremented (bar: the persistent pursuit's
-// barrier flags [kPersistMaxGrid], raised to the epoch's base every 2^16 epochs; may be null)
+// barrier flags [kPersistMaxGrid][kBarStride], raised to the epoch's base every 2^16 epochs; may be
+// null)
 hipError_t launch_reset_counters(int* counters, hipStream_t s, int* bar = nullptr);
 hipError_t launch_score_all(const PursuitArgs& a, hipStream_t s);     // T <= 2048
 hipError_t launch_peaks(const PursuitArgs& a, hipStream_t s);
@@ -113,6 +132,7 @@ hipError_t launch_stamp_span(const unsigned long long* ts, int launches, int gri
 // counters[7] = 1 reports a barrier timeout.
 constexpr int kPersistMaxW = 4096;
 constexpr int kPersistMaxGrid = 2048;   // workgroups of one persistent launch (barrier flags)
+constexpr int kBarStride = 16;          // ints between two barrier flags (one per 64 bytes)
 // used (may be NULL): the workgroups launched.  With a.ts set (measurement mode) workgroup b writes
 // its GPU-clock start and end to ts[2b], ts[2b + 1].
 hipError_t launch_pursuit_persistent(const PursuitArgs& a, int grid, int first_pass, int max_passes,

@@ -461,4 +461,46 @@ hipError_t launch_hp_spectrum(const float* taps, int ft, int64_t W, int Llo, int
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------------------------------------
+// Window assembly from device samples: segments of int16 copied into window rows, all probes' in
+// one launch.  Segment k writes n elements at dst from src (period 0: contiguous) or repeats the
+// period-element row src (the rows before the recording starts).  Threads take 8-element chunks
+// of the concatenated segments (16-byte accesses where both ends are 16-byte aligned).
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void gather_kernel(GatherArgs g) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; q < g.chunks; q += stride) {
+        int k = 0;
+        while (k + 1 < g.n && g.seg[k + 1].chunk0 <= q) ++k;
+        const GatherSeg& sg = g.seg[k];
+        const int64_t i0 = (q - sg.chunk0) * 8;
+        const int64_t m = sg.n - i0 < 8 ? sg.n - i0 : 8;
+        if (sg.period == 0 && m == 8 && (((uintptr_t)(sg.dst + i0) | (uintptr_t)(sg.src + i0)) & 15) == 0) {
+            *(int4*)(sg.dst + i0) = *(const int4*)(sg.src + i0);
+        } else {
+            for (int64_t i = i0; i < i0 + m; ++i) sg.dst[i] = sg.src[sg.period ? i % sg.period : i];
+        }
+    }
+}
+
+hipError_t launch_gather(const GatherSeg* segs, int n, hipStream_t s) {
+    for (int k0 = 0; k0 < n; k0 += kGatherMaxSeg) {
+        GatherArgs g{};
+        g.n = std::min(kGatherMaxSeg, n - k0);
+        int64_t c = 0;
+        for (int k = 0; k < g.n; ++k) {
+            g.seg[k] = segs[k0 + k];
+            g.seg[k].chunk0 = c;
+            c += (segs[k0 + k].n + 7) / 8;
+        }
+        g.chunks = c;
+        if (c == 0) continue;
+        const unsigned nb = (unsigned)std::min<int64_t>((c + 255) / 256, 2048);
+        hipLaunchKernelGGL(gather_kernel, dim3(nb), dim3(256), 0, s, g);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 }  // namespace lss

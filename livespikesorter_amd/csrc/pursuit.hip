@@ -200,6 +200,126 @@ __device__ __forceinline__ void rescore(const floatx4* x, const floatx4* nmv, co
     }
 }
 
+// Wavefront maxima of NC values at once (uniform results), their DPP steps interleaved.  The values
+// are non-negative or -1, whose bit patterns order as signed integers the way the floats do.
+template <int CTRL, int ROWS = 0xF>
+__device__ __forceinline__ int dpp_i(int v) { return __builtin_amdgcn_update_dpp(v, v, CTRL, ROWS, 0xF, false); }
+template <int NC>
+__device__ __forceinline__ void wave_max_n(float* v) {
+    int x[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) x[c] = __builtin_bit_cast(int, v[c]);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) x[c] = max(x[c], dpp_i<0xB1>(x[c]));         // quad_perm [1, 0, 3, 2]
+#pragma unroll
+    for (int c = 0; c < NC; ++c) x[c] = max(x[c], dpp_i<0x4E>(x[c]));         // quad_perm [2, 3, 0, 1]
+#pragma unroll
+    for (int c = 0; c < NC; ++c) x[c] = max(x[c], dpp_i<0x141>(x[c]));        // row_half_mirror
+#pragma unroll
+    for (int c = 0; c < NC; ++c) x[c] = max(x[c], dpp_i<0x140>(x[c]));        // row_mirror
+#pragma unroll
+    for (int c = 0; c < NC; ++c) x[c] = max(x[c], dpp_i<0x142, 0xA>(x[c]));   // row_bcast:15 into rows 1, 3
+#pragma unroll
+    for (int c = 0; c < NC; ++c) x[c] = max(x[c], dpp_i<0x143, 0xC>(x[c]));   // row_bcast:31 into rows 2, 3
+#pragma unroll
+    for (int c = 0; c < NC; ++c) v[c] = __builtin_bit_cast(float, __builtin_amdgcn_readlane(x[c], 63));
+}
+
+// rescore<NV, true> of NC columns x[c] (c < nc valid) with their steps interleaved: the same
+// results column by column.
+template <int NV, int NC>
+__device__ __forceinline__ void rescore_n(const floatx4 (*x)[NV], const floatx4* nmv, const floatx4* rnv, int T, int nc,
+                                          float* best, int* bi, float* bb, float* bm) {
+    const int lane = threadIdx.x & 63;
+    float sa[NC][NV][4], ml[NC], pl[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        ml[c] = -1.f;
+        pl[c] = 0.f;
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const float r = fmaxf(x[c][i][e], 0.f);
+                const float p = r * r;
+                const float sv = p * rnv[i][e];
+                const bool in = 4 * lane + 256 * i + e < T;
+                sa[c][i][e] = in ? sv : -1.f;
+                ml[c] = fmaxf(ml[c], sa[c][i][e]);
+                pl[c] = fmaxf(pl[c], in ? p : 0.f);
+            }
+    }
+    wave_max_n<NC>(ml);
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        best[c] = -1.f;
+        bi[c] = 0x7fffffff;
+        bb[c] = 0.f;
+        bm[c] = 1.f;
+        if (c >= nc) continue;
+        const float M = ml[c];
+        if (M == 0.f && wave_max(pl[c]) == 0.f) {
+            // no positive score: every q_t = 0
+            best[c] = 0.f;
+            bi[c] = 0;
+            bb[c] = readlane_f(x[c][0][0], 0);
+            bm[c] = readlane_f(nmv[0][0], 0);
+            continue;
+        }
+        const float thr = (M >= 1e-30f && M <= 1e30f) ? M * kCandidate : -0.5f;
+        uint32_t cm = 0;
+#pragma unroll
+        for (int i = 0; i < NV; ++i)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) cm |= (sa[c][i][e] >= thr ? 1u : 0u) << (4 * i + e);
+        float b = -1.f, xb = 0.f, nb = 1.f;
+        int k_ = 0x7fffffff;
+        if (__ballot(__popc(cm) > 1) == 0) {
+            if (cm) {
+                const int k = __ffs(cm) - 1;
+#pragma unroll
+                for (int i = 0; i < NV; ++i)
+#pragma unroll
+                    for (int e = 0; e < 4; ++e)
+                        if (k == 4 * i + e) {
+                            xb = x[c][i][e];
+                            nb = nmv[i][e];
+                        }
+                b = score_of(xb, nb);
+                k_ = 4 * lane + 256 * (k >> 2) + (k & 3);
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < NV; ++i)
+#pragma unroll
+                for (int e = 0; e < 4; ++e)
+                    if (sa[c][i][e] >= thr) {
+                        const float q = score_of(x[c][i][e], nmv[i][e]);
+                        if (q > b) {   // ascending t within the lane: the first maximum stays
+                            b = q;
+                            k_ = 4 * lane + 256 * i + e;
+                            xb = x[c][i][e];
+                            nb = nmv[i][e];
+                        }
+                    }
+        }
+        const unsigned long long cand = __ballot(k_ != 0x7fffffff);
+        if (__popcll(cand) == 1) {
+            const int l = __ffsll((long long)cand) - 1;
+            b = readlane_f(b, l);
+            k_ = __builtin_amdgcn_readlane(k_, l);
+            xb = readlane_f(xb, l);
+            nb = readlane_f(nb, l);
+        } else if (cand != 0) {
+            wave_argmax_b(b, k_, xb, nb);
+        }
+        best[c] = b;
+        bi[c] = k_;
+        bb[c] = xb;
+        bm[c] = nb;
+    }
+}
+
 // Device-scope (all XCDs) relaxed loads and stores: the only way workgroups of one launch of the
 // persistent pursuit exchange data, so that its grid barrier needs no cache write-back/invalidate.
 template <typename V>
@@ -642,9 +762,10 @@ constexpr int kBarrierSpin = 1 << 20;                   // ~1 s of polling
 constexpr int kPersistLoads = (kPersistMaxW + 2 * kPeakHalo + kPT - 1) / kPT;   // row values per thread
 
 // Grid barrier number `gen` (1, 2, ...) of this launch: each workgroup, once its memory operations
-// have completed, stores base + gen into its flag bar[g]; wavefront 0 then polls every workgroup's
-// flag (device-scope loads, 64 per instruction) until all have reached base + gen.  No atomics, so
-// arrivals do not queue on one address.  base = the launch's epoch (counters[kEpoch], advanced by
+// have completed, stores base + gen into its flag bar[kBarStride g]; wavefront 0 then polls every
+// workgroup's flag (device-scope loads, 64 per instruction) until all have reached base + gen.  No
+// atomics, so arrivals do not queue on one address, and one flag per 64 bytes, so the pollers'
+// loads spread over as many cache lines as there are workgroups.  base = the launch's epoch (counters[kEpoch], advanced by
 // the counter reset before every pursuit) times 4096, so flags of earlier launches stay below it
 // (differences taken modulo 2^32).  Data shared between workgroups is only accessed with
 // device-scope operations, so no cache maintenance is needed.  Returns false after a timeout or
@@ -655,13 +776,13 @@ __device__ __forceinline__ bool grid_barrier(const PursuitArgs& a, unsigned base
     __syncthreads();
     const unsigned target = base + (unsigned)gen;
     unsigned* const bar = (unsigned*)a.bar;
-    if (threadIdx.x == 0) st_dev(bar + blockIdx.x, target);
+    if (threadIdx.x == 0) st_dev(bar + kBarStride * blockIdx.x, target);
     if (threadIdx.x < 64) {
         int ok = 1;
         for (int it = 0;; ++it) {
             bool behind = false;
             for (int i = (int)threadIdx.x; i < (int)gridDim.x; i += 64)
-                behind |= (int)(ld_dev(bar + i) - target) < 0;
+                behind |= (int)(ld_dev(bar + kBarStride * i) - target) < 0;
             if (__ballot(behind) == 0) break;
             if (it >= kBarrierSpin || __ballot(ld_dev(a.counters + 7) != 0) != 0) {
                 if (threadIdx.x == 0) atomicExch(a.counters + 7, 1);
@@ -750,50 +871,51 @@ __device__ __forceinline__ uint32_t peak_bits8(const float* cf, const float* bm,
 
 // LDS of the persistent pursuit (4-byte words) for W columns, row stride ldT and G workgroups: own
 // B rows [ceil(W/G)][ldT] | template norms and their reciprocals [ldT] each | the padded Cfmax row
-// with its halos and a -inf tail of 16 | block maxima with their halos | the pass's list: column |
-// template << 16, amplitude [W] | the own columns' Cfmax, imax, amplitude of the pass [ceil(W/G)]
-// each
+// with its halos and a -inf tail of 16 | block maxima with their halos | the row's imax and
+// amplitudes [W] | the pass's list: column | template << 16, amplitude [W] | per own column the last
+// even- and odd-indexed spike covering it | the covered own columns: own index, those two spikes
+// [ceil(W/G)] each
 __host__ __device__ constexpr int persist_cf_len(int W) { return pad_len(W + 2 * kPeakHalo + 16); }
 __host__ __device__ constexpr int persist_bm_len(int W) { return (W + 7) / 8 + 2 * kBmHalo; }
 struct PersistLayout {
-    int own, nm, rn, cf, bm, lct, lamp, ocf, oim, oam, words;
+    int own, nm, rn, cf, bm, rim, ram, lct, lamp, cove, covo, cvj, cvl, cvp, words;
     __host__ __device__ PersistLayout(int W, int ldT, int G) {
         own = (W + G - 1) / G;
         nm = own * ldT;
         rn = nm + ldT;
         cf = rn + ldT;
         bm = cf + persist_cf_len(W);
-        lct = bm + persist_bm_len(W);
+        rim = bm + persist_bm_len(W);
+        ram = rim + W;
+        lct = ram + W;
         lamp = lct + W;
-        ocf = lamp + W;
-        oim = ocf + own;
-        oam = oim + own;
-        words = oam + own;
+        cove = lamp + W;
+        covo = cove + own;
+        cvj = covo + own;
+        cvl = cvj + own;
+        cvp = cvl + own;
+        words = cvp + own;
     }
 };
 
-// First spike of the pass's list (column in the low 16 bits, ascending) whose +-nt window reaches
-// column w (n if none does).
-__device__ __forceinline__ int first_cover_packed(const int* lct, int n, int w, int nt) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if ((lct[mid] & 0xFFFF) + nt < w) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
+// floor(x / d) for 0 <= x < 2^24, 1 <= d, r = 1/d in FP32: the float quotient is off by at most one
+__device__ __forceinline__ int floor_div(int x, int d, float r) {
+    int q = (int)((float)x * r);
+    if ((q + 1) * d <= x) ++q;
+    if (q * d > x) --q;
+    return q;
 }
 
-// Up to two covered own columns of one wavefront (n2 = 1 or 2): own index j, column w, last and
-// second-to-last covering spikes of the pass's list (pr = -1 when one spike covers it; list index =
-// index in the pass, so the covering spikes are its last even- and odd-indexed ones).  The LDS row
-// gets B -= amp ctc for the even-indexed spike, then for the odd-indexed one (ctc rows loaded
-// straight into registers, every load of both columns in flight at once), and is rescored into row
-// nx inside [nt, W - nt) (columns nearer an edge are carried by the caller).
+// Up to NC covered own columns of one wavefront (n2 <= NC of them): own index j, column w, and the
+// last even- and last odd-indexed spikes of the pass whose window covers it (se / so, list index =
+// index in the pass; -1 where none).  The LDS row gets B -= amp ctc for the even-indexed spike, then
+// for the odd-indexed one (ctc rows loaded straight into registers, every load of the columns in
+// flight at once), and is rescored into row nx inside [nt, W - nt) (columns nearer an edge are
+// carried by the caller).
 template <int NV, int NC>
 __device__ __forceinline__ void persist_update_pair(const PursuitArgs& a, const PursuitArgs& nx, floatx4* Bs4, int ld4,
                                                     const int* lct, const float* lamp, int n2, const int* j,
-                                                    const int* w, const int* la, const int* pr, const floatx4* nm4,
+                                                    const int* w, const int* se, const int* so, const floatx4* nm4,
                                                     const floatx4* rn4) {
     const int lane = threadIdx.x & 63;
     const int nt = a.nt, L = 2 * nt + 1;
@@ -804,16 +926,11 @@ __device__ __forceinline__ void persist_update_pair(const PursuitArgs& a, const 
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
         const int cc = c < n2 ? c : 0;
-        const int sl = la[cc], sp = pr[cc];
-        const float* ql = a.ctc + ((int64_t)(lct[sl] >> 16) * L + (w[cc] - (lct[sl] & 0xFFFF) + nt)) * a.ldT;
-        const float* qp = sp >= 0 ? a.ctc + ((int64_t)(lct[sp] >> 16) * L + (w[cc] - (lct[sp] & 0xFFFF) + nt)) * a.ldT
-                                  : nullptr;
-        const float al = lamp[sl], ap = sp >= 0 ? lamp[sp] : 0.f;
-        const bool swap = qp && (sl & 1);   // the even-indexed spike first
-        r1[c] = swap ? qp : ql;
-        r2[c] = swap ? ql : qp;
-        a1[c] = swap ? ap : al;
-        a2[c] = swap ? al : ap;
+        const int e = se[cc], o = so[cc];
+        r1[c] = e >= 0 ? a.ctc + ((int64_t)(lct[e] >> 16) * L + (w[cc] - (lct[e] & 0xFFFF) + nt)) * a.ldT : nullptr;
+        r2[c] = o >= 0 ? a.ctc + ((int64_t)(lct[o] >> 16) * L + (w[cc] - (lct[o] & 0xFFFF) + nt)) * a.ldT : nullptr;
+        a1[c] = e >= 0 ? lamp[e] : 0.f;
+        a2[c] = o >= 0 ? lamp[o] : 0.f;
     }
     floatx4 x[NC][NV], c1[NC][NV], c2[NC][NV];
 #pragma unroll
@@ -823,7 +940,7 @@ __device__ __forceinline__ void persist_update_pair(const PursuitArgs& a, const 
             const int t = 4 * lane + 256 * i;
             x[c][i] = c1[c][i] = c2[c][i] = z;
             if (c < n2 && t < a.T) {
-                c1[c][i] = *(const floatx4*)(r1[c] + t);
+                if (r1[c]) c1[c][i] = *(const floatx4*)(r1[c] + t);
                 if (r2[c]) c2[c][i] = *(const floatx4*)(r2[c] + t);
                 x[c][i] = Bs4[j[c] * ld4 + lane + 64 * i];
             }
@@ -833,8 +950,10 @@ __device__ __forceinline__ void persist_update_pair(const PursuitArgs& a, const 
 #pragma unroll
         for (int i = 0; i < NV; ++i) {
             floatx4 v = x[c][i];
+            if (r1[c]) {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) v[e] = v[e] - a1[c] * c1[c][i][e];
+                for (int e = 0; e < 4; ++e) v[e] = v[e] - a1[c] * c1[c][i][e];
+            }
             if (r2[c]) {
 #pragma unroll
                 for (int e = 0; e < 4; ++e) v[e] = v[e] - a2[c] * c2[c][i][e];
@@ -842,23 +961,21 @@ __device__ __forceinline__ void persist_update_pair(const PursuitArgs& a, const 
             x[c][i] = v;
             if (c < n2 && 4 * lane + 256 * i < a.T) Bs4[j[c] * ld4 + lane + 64 * i] = v;
         }
+    // rescored (every column; published inside [nt, W - nt))
+    floatx4 nmv[NV], rnv[NV];
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-        if (c >= n2) break;
-        if (w[c] >= nt && w[c] < a.W - nt) {
-            floatx4 nmv[NV], rnv[NV];
-#pragma unroll
-            for (int i = 0; i < NV; ++i) {
-                const bool in = lane + 64 * i < ld4;
-                nmv[i] = in ? nm4[lane + 64 * i] : floatx4{1.f, 1.f, 1.f, 1.f};
-                rnv[i] = in ? rn4[lane + 64 * i] : floatx4{1.f, 1.f, 1.f, 1.f};
-            }
-            float best, b_, m_;
-            int bi;
-            rescore<NV, true>(x[c], nmv, rnv, a.T, best, bi, b_, m_);
-            if (lane == 0) store_score<true>(nx, w[c], best, bi, b_, m_);
-        }
+    for (int i = 0; i < NV; ++i) {
+        const bool in = lane + 64 * i < ld4;
+        nmv[i] = in ? nm4[lane + 64 * i] : floatx4{1.f, 1.f, 1.f, 1.f};
+        rnv[i] = in ? rn4[lane + 64 * i] : floatx4{1.f, 1.f, 1.f, 1.f};
     }
+    float best[NC], b_[NC], m_[NC];
+    int bi[NC];
+    rescore_n<NV, NC>(x, nmv, rnv, a.T, n2, best, bi, b_, m_);
+    if (lane == 0)
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+            if (c < n2 && w[c] >= nt && w[c] < a.W - nt) store_score<true>(nx, w[c], best[c], bi[c], b_[c], m_[c]);
 }
 
 template <int NV>
@@ -877,14 +994,19 @@ __global__ __launch_bounds__(kPT) void pursuit_persistent_kernel(PursuitArgs a, 
     float* bm = psm + ly.bm;
     int* lct = (int*)(psm + ly.lct);
     float* lamp = psm + ly.lamp;
-    float* ocf = psm + ly.ocf;
-    int* oim = (int*)(psm + ly.oim);
-    float* oam = psm + ly.oam;
+    int* rim = (int*)(psm + ly.rim);
+    float* ram = psm + ly.ram;
+    int* cove = (int*)(psm + ly.cove);
+    int* covo = (int*)(psm + ly.covo);
+    int* cvj = (int*)(psm + ly.cvj);
+    int* cvl = (int*)(psm + ly.cvl);
+    int* cvp = (int*)(psm + ly.cvp);
     const bool lead = blockIdx.x == 0 && tid == 0;
     if (a.ts && tid == 0) a.ts[2 * blockIdx.x] = __builtin_amdgcn_s_memrealtime();   // measurement mode
     // own columns g + G j (j < nown): strided, so that every workgroup holds a share of the columns
     // a pass updates wherever its spikes are; their B rows into LDS
     const int nown = g < W ? (W - 1 - g) / G + 1 : 0;
+    const float rG = 1.f / (float)G;
     const int ld4 = (int)(a.ldT / 4);
     for (int j = wave; j < nown; j += kPW) {
         const floatx4* src = (const floatx4*)(a.B + (int64_t)(g + G * j) * a.ldT);
@@ -953,11 +1075,11 @@ __global__ __launch_bounds__(kPT) void pursuit_persistent_kernel(PursuitArgs a, 
     // this thread's block: columns c0 .. c0 + 7, r0 = c0 mod G (the first is owned when r0 == g)
     const int c0 = 8 * tid, r0 = c0 % G;
     for (int pass = first_pass; pass < max_passes && !done; ++pass) {
-        // the row: each thread its block's Cfmax (into LDS, with the block maximum), imax and
-        // amplitudes (kept in registers), every load in flight at once
-        float cv[8], av[8];
-        int iv[8];
+        // the row: each thread its block's Cfmax (with the block maximum), imax and amplitudes into
+        // LDS, every load in flight at once
         if (tid < nbk) {
+            float cv[8], av[8];
+            int iv[8];
             floatx4 fc[2], fa[2];
             uint32x4 ui[2];
 #pragma unroll
@@ -968,7 +1090,6 @@ __global__ __launch_bounds__(kPT) void pursuit_persistent_kernel(PursuitArgs a, 
                 fa[h] = __builtin_bit_cast(floatx4, ld_dev16(cur.amprow, 4 * kPersistMaxW, 4 * (c0 + 4 * h)));
             }
             float m = -INFINITY;
-            int r = r0;
 #pragma unroll
             for (int q = 0; q < 8; ++q) {
                 const bool in = c0 + q < W;
@@ -976,17 +1097,15 @@ __global__ __launch_bounds__(kPT) void pursuit_persistent_kernel(PursuitArgs a, 
                 iv[q] = (int)ui[q >> 2][q & 3];
                 av[q] = fa[q >> 2][q & 3];
                 cf[pad32(H + c0 + q)] = cv[q];
-                m = fmaxf(m, cv[q]);
-                if (r == g && in) {
-                    const int j = (c0 + q) / G;
-                    ocf[j] = cv[q];
-                    oim[j] = iv[q];
-                    oam[j] = av[q];
+                if (in) {
+                    rim[c0 + q] = iv[q];
+                    ram[c0 + q] = av[q];
                 }
-                if (++r == G) r = 0;
+                m = fmaxf(m, cv[q]);
             }
             bm[kBmHalo + tid] = m;
         }
+        for (int j = tid; j < nown; j += kPT) cove[j] = covo[j] = -1;
         __syncthreads();
         const uint32_t bits = tid < nbk ? peak_bits8(cf, bm, tid, W, nt, a.th2) : 0u;
         const int cnt = __popc(bits);
@@ -1014,14 +1133,21 @@ __global__ __launch_bounds__(kPT) void pursuit_persistent_kernel(PursuitArgs a, 
             for (int q = 0; q < 8; ++q) {
                 if ((bits >> q) & 1u) {
                     const int c = c0 + q;
+                    const int ic = rim[c];
+                    const float ac = ram[c];
                     if (gi < nu) {
-                        lct[gi] = c | (iv[q] << 16);
-                        lamp[gi] = av[q];
+                        lct[gi] = c | (ic << 16);
+                        lamp[gi] = ac;
+                        // own columns g + G j within [c - nt, c + nt]: the spike covers them
+                        const int w0 = max(c - nt, 0), w1 = min(c + nt, W - 1);
+                        const int jlo = w0 <= g ? 0 : floor_div(w0 - g + G - 1, G, rG);
+                        const int jhi = min(w1 < g ? -1 : floor_div(w1 - g, G, rG), nown - 1);
+                        for (int j = jlo; j <= jhi; ++j) atomicMax((gi & 1) ? covo + j : cove + j, gi);
                     }
                     if (r == g && base + gi < a.cap) {
                         a.sp_col[base + gi] = c;
-                        a.sp_tmp[base + gi] = iv[q];
-                        a.sp_amp[base + gi] = av[q];
+                        a.sp_tmp[base + gi] = ic;
+                        a.sp_amp[base + gi] = ac;
                     }
                     ++gi;
                 }
@@ -1037,45 +1163,59 @@ __global__ __launch_bounds__(kPT) void pursuit_persistent_kernel(PursuitArgs a, 
         __syncthreads();   // the list is complete
         PursuitArgs nxt = a;
         use_row(nxt, a, par ^ 1);
-        // own columns j = wave + kPW l (lane l of the wavefront): the pass's coverage of each (its last
-        // and second-to-last covering spikes: the last even- and odd-indexed ones, as covering spikes
-        // are consecutive), carried into the next row when not rescored (uncovered, or covered within
-        // nt of an edge); the covered ones updated two at a time by the wavefront
-        for (int j0 = wave; j0 < nown; j0 += 64 * kPW) {
-            const int jl = j0 + kPW * lane, wl = g + G * jl;
-            int last = -1, prev = -1;
-            if (jl < nown) {
-                const int s0 = first_cover_packed(lct, nu, wl, nt);
-                if (s0 < nu && (lct[s0] & 0xFFFF) - nt <= wl) {
-                    last = s0;
-                    while (last + 1 < nu && (lct[last + 1] & 0xFFFF) - nt <= wl) ++last;
-                    prev = last > s0 ? last - 1 : -1;
-                }
-                if (!(last >= 0 && wl >= nt && wl < W - nt)) {
-                    st_dev(nxt.cfmax + wl, ocf[jl]);
-                    st_dev(nxt.imax + wl, oim[jl]);
-                    st_dev(nxt.amprow + wl, oam[jl]);
+        // own columns, one thread each: carried into the next row when the pass does not rescore
+        // them (uncovered, or covered within nt of an edge); the covered ones, with their last even-
+        // and odd-indexed covering spikes, listed in ascending order and spread over the wavefronts,
+        // NC at a time
+        int ncv = 0;
+        for (int j0 = 0; j0 < nown; j0 += kPT) {
+            const int j = j0 + tid, wj = g + G * j;
+            int e = -1, o = -1;
+            if (j < nown) {
+                e = cove[j];
+                o = covo[j];
+                if (!((e >= 0 || o >= 0) && wj >= nt && wj < W - nt)) {
+                    st_dev(nxt.cfmax + wj, cf[pad32(H + wj)]);
+                    st_dev(nxt.imax + wj, rim[wj]);
+                    st_dev(nxt.amprow + wj, ram[wj]);
                 }
             }
-            constexpr int NC = NV <= 3 ? 2 : 1;   // columns per step (registers)
-            for (uint64_t m = __ballot(last >= 0); m;) {
-                int jj[NC], ww[NC], ll[NC], pp[NC], n2 = 0;
+            const bool cvr = e >= 0 || o >= 0;
+            const uint64_t m = __ballot(cvr);
+            if (lane == 0) wsum[wave] = __popcll(m);
+            __syncthreads();
+            int off = ncv, tot = 0;
 #pragma unroll
-                for (int c = 0; c < NC; ++c) {
-                    jj[c] = ww[c] = ll[c] = 0;
-                    pp[c] = -1;
-                    if (m) {
-                        const int l = __ffsll((long long)m) - 1;
-                        m &= m - 1;
-                        jj[c] = __builtin_amdgcn_readlane(jl, l);
-                        ww[c] = __builtin_amdgcn_readlane(wl, l);
-                        ll[c] = __builtin_amdgcn_readlane(last, l);
-                        pp[c] = __builtin_amdgcn_readlane(prev, l);
-                        ++n2;
-                    }
-                }
-                persist_update_pair<NV, NC>(a, nxt, psm4, ld4, lct, lamp, n2, jj, ww, ll, pp, nm4, rn4);
+            for (int k = 0; k < kPW; ++k) {
+                const int vv = wsum[k];
+                if (k < wave) off += vv;
+                tot += vv;
             }
+            if (cvr) {
+                const int k = off + __popcll(m & ((1ull << lane) - 1ull));
+                cvj[k] = j;
+                cvl[k] = e;
+                cvp[k] = o;
+            }
+            ncv += tot;
+            __syncthreads();   // the covered list (and wsum free again)
+        }
+        constexpr int NC = NV <= 3 ? 2 : 1;   // columns per step (registers)
+        for (int k0 = NC * wave; k0 < ncv; k0 += NC * kPW) {
+            int jj[NC], ww[NC], ee[NC], oo[NC], n2 = 0;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                jj[c] = ww[c] = 0;
+                ee[c] = oo[c] = -1;
+                if (k0 + c < ncv) {
+                    jj[c] = cvj[k0 + c];
+                    ww[c] = g + G * jj[c];
+                    ee[c] = cvl[k0 + c];
+                    oo[c] = cvp[k0 + c];
+                    ++n2;
+                }
+            }
+            persist_update_pair<NV, NC>(a, nxt, psm4, ld4, lct, lamp, n2, jj, ww, ee, oo, nm4, rn4);
         }
         base += n;
         par ^= 1;
@@ -1547,7 +1687,7 @@ __global__ void reset_counters_kernel(int* c, int* bar) {
     __syncthreads();
     if (threadIdx.x < kCounters) c[threadIdx.x] = threadIdx.x == kEpoch ? e : 0;
     if (bar && (e & 0xFFFF) == 0)
-        for (int i = threadIdx.x; i < kPersistMaxGrid; i += blockDim.x) bar[i] = (int)((unsigned)e * 4096u);
+        for (int i = threadIdx.x; i < kPersistMaxGrid; i += blockDim.x) bar[kBarStride * i] = (int)((unsigned)e * 4096u);
 }
 
 hipError_t launch_reset_counters(int* counters, hipStream_t s, int* bar) {
