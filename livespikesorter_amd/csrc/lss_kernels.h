@@ -98,7 +98,7 @@ struct PursuitArgs {
     int* bar;
     // [0] spikes of committed passes, [1] spikes of the last pass, [2] capacity overflow,
     // [3] converged (a pass found nothing), [4] passes that found spikes, [5] residual list overflow (one pass),
-    // [6] grid-barrier arrivals and [7] grid-barrier timeout of the persistent pursuit, [8] the
+    // [7] grid-barrier timeout of the persistent pursuit, [8] the
     // peak kernel's workgroup ticket, [kEpoch] the persistent pursuit's barrier epoch (advanced, not
     // zeroed, by launch_reset_counters) (kCounters ints)
     int* counters;
