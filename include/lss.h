@@ -288,6 +288,17 @@ lss_status lss_stream_breakdown(void* stream, float* ms);
  * lss_stream_last_refine gives a probe's last info (4 doubles). */
 lss_status lss_stream_set_refine(void* stream, int32_t every_batches);
 lss_status lss_stream_last_refine(void* stream, int32_t probe, double* info);
+/* Running ahead (on by default; one-graph steps only): a step whose next batch's samples are
+ * already pushed also preprocesses that batch (window, whitening, scores) into a second window set
+ * per handle, as a second graph beside its own, so that the next step starts with the pursuit; a
+ * backlog is then sorted faster.  Results are the same either way.  A step
+ * that ends with a refinement does not run ahead, and a batch preprocessed ahead is preprocessed
+ * again when lss_set_drift or lss_refine_templates changed its handle in between.  on = 0 turns it
+ * off (every step preprocesses its own batch; the per-batch latency of a live stream).  No
+ * counterpart in the reference (its loop sorts one batch at a time, V2:632-820). */
+lss_status lss_stream_set_run_ahead(void* stream, int32_t on);
+/* Steps so far that preprocessed their next batch (running ahead). */
+lss_status lss_stream_ran_ahead(void* stream, int64_t* steps);
 
 /* ---- template refinement (no reference counterpart) ---------------------------------------------- */
 

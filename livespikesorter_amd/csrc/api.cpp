@@ -94,6 +94,12 @@ struct Handle {
     DevBuf<int16_t> win;
     DevBuf<long long> sums, sum_part;   // channel sums and their per-block partials
     DevBuf<float> xcw, xhp, xw, xp, B, cfmax;
+    // the streaming batcher's second set (allocated by its first step that runs ahead): the next
+    // batch's window, whitened data and scores, preprocessed while the current batch's pursuit runs,
+    // then swapped with win / xw / B when that batch's step starts
+    DevBuf<int16_t> win_alt;
+    DevBuf<float> xw_alt, B_alt;
+    uint64_t pre_version = 0;    // advanced when an input of the preprocessing changes (drift, templates)
     DevBuf<int> imax;
     DevBuf<unsigned char> flags;
     DevBuf<int> sp_col, sp_tmp, counters, pass_off;
@@ -556,11 +562,12 @@ lss_status run_features(Handle& h, const float* X, int64_t W) {
 }
 
 // Device part of one window up to the scores (no events, no host synchronisation: capturable).
-lss_status enqueue_pre(Handle& h, const Handle::Spec& sp, const int16_t* d_win, int64_t W) {
+// Preprocessing and scores of the window d_win into xw (whitened data) and B (scores).
+lss_status enqueue_pre(Handle& h, const Handle::Spec& sp, const int16_t* d_win, int64_t W, float* xw, float* B) {
     if (lss_status st = run_car(h, d_win, W)) return st;
     HIP_TRY(lss::launch_highpass(h.xcw.p, h.ldw, h.xhp.p, h.ldw, h.C, W, sp.lo, sp.hi, sp.p, h.tw.p, h.stream));
-    if (lss_status st = run_whiten(h, h.xhp.p, h.xw.p, W)) return st;
-    return run_scores(h, h.xw.p, h.B.p, W);
+    if (lss_status st = run_whiten(h, h.xhp.p, xw, W)) return st;
+    return run_scores(h, xw, B, W);
 }
 
 // Pinned, device-mapped, coherent host buffers that the staging kernel writes directly.  Windows
@@ -726,6 +733,15 @@ lss_status emit_spikes(Handle& h, int n, int passes, int64_t window_start, lss_s
 // extra passes (non-graph) for probes that need them; segment C = residual, features and the
 // readback of every probe.  With use_graph, A and C are captured once and replayed.  When every
 // probe runs the persistent pursuit (all passes in A), C joins A and a step is one graph.
+//
+// Running ahead (one-graph steps only): when the next batch's samples are already here and no
+// refinement follows the step, the step also preprocesses the next batch (window, whitening,
+// scores) into the handles' second window sets: segment N, a second graph on its own stream (one
+// branch per probe) beside segment A, so that the next step starts with the pursuit.  Each step still reports only
+// its own batch, and the results are the same as without running ahead (the preprocessing of a
+// batch reads only its samples and the templates, which do not change between the two steps).  A
+// step that ran ahead is discarded for a probe whose preprocessing inputs changed in between
+// (lss_set_drift, lss_refine_templates: Handle::pre_version); the batch is then preprocessed again.
 // ------------------------------------------------------------------------------------------------
 struct Probe {
     Handle* h = nullptr;
@@ -736,11 +752,17 @@ struct Probe {
     int64_t base = 0;
     int64_t recv = 0;
     int16_t* pin = nullptr;       // pinned window [W][C]
+    int16_t* pin_alt = nullptr;   // the next batch's (host mode, running ahead)
     int64_t W = 0;
     int P1 = 0;
     int max_grid = 0;             // persistent-pursuit workgroups, so that all probes' fit at once
     const Handle::Spec* spec = nullptr;   // the window's highpass spectrum (pinned in the handle)
     hipEvent_t done = nullptr;
+    // running ahead: the next batch's preprocessing stream and its end, the handle's pre_version
+    // when it ran
+    hipStream_t s2 = nullptr;
+    hipEvent_t done2 = nullptr;
+    uint64_t ahead_version = 0;
 };
 
 struct Batcher {
@@ -749,12 +771,26 @@ struct Batcher {
     bool use_graph = false;
     hipStream_t main = nullptr;
     hipEvent_t fork = nullptr;
+    // running ahead: the next batch's preprocessing (segment N) on its own stream, as a second graph
+    // beside segment A; by swapped
+    hipStream_t main2 = nullptr;
+    hipEvent_t fork2 = nullptr, joinA = nullptr;
+    hipGraph_t graphsN[2] = {};
+    hipGraphExec_t execsN[2] = {};
     hipGraphExec_t gA = nullptr, gC = nullptr;
     hipGraph_t graphA = nullptr, graphC = nullptr;
     int64_t t0 = 0;
     int64_t batches = 0;
     float last_ms = 0.f;
     bool single = false;     // every probe runs all its passes in segment A: one graph per step
+    bool run_ahead = true;   // preprocess the next batch during a step when its samples are here
+    bool ahead = false;      // the last step preprocessed this step's batch into the second sets
+    int64_t ran_ahead = 0;   // steps that preprocessed their next batch
+    int swapped = 0;         // the window sets have been swapped an odd number of times
+    bool cur_pre = true, next_pre = false;   // the step preprocesses this batch / the next batch
+    // one-graph steps, by (cur_pre, swapped): cur_pre | swapped << 1
+    hipGraph_t graphs[4] = {};
+    hipGraphExec_t execs[4] = {};
     int refine_every = 0;    // template refinement round every this many batches (0 = never)
     int refine_next = -1;    // probe the current refinement round refines next (-1: no round)
     std::vector<std::array<double, 4>> refine_info;   // per probe, last refinement (lss_refine_templates)
@@ -764,9 +800,23 @@ struct Batcher {
         if (gC) (void)hipGraphExecDestroy(gC);
         if (graphA) (void)hipGraphDestroy(graphA);
         if (graphC) (void)hipGraphDestroy(graphC);
+        for (int i = 0; i < 4; ++i) {
+            if (execs[i]) (void)hipGraphExecDestroy(execs[i]);
+            if (graphs[i]) (void)hipGraphDestroy(graphs[i]);
+        }
+        for (int i = 0; i < 2; ++i) {
+            if (execsN[i]) (void)hipGraphExecDestroy(execsN[i]);
+            if (graphsN[i]) (void)hipGraphDestroy(graphsN[i]);
+        }
+        if (fork2) (void)hipEventDestroy(fork2);
+        if (joinA) (void)hipEventDestroy(joinA);
+        if (main2) (void)hipStreamDestroy(main2);
         for (auto& p : pr) {
             if (p.pin) (void)hipHostFree(p.pin);
+            if (p.pin_alt) (void)hipHostFree(p.pin_alt);
             if (p.done) (void)hipEventDestroy(p.done);
+            if (p.done2) (void)hipEventDestroy(p.done2);
+            if (p.s2) (void)hipStreamDestroy(p.s2);
             if (p.spec) --const_cast<Handle::Spec*>(p.spec)->pins;
         }
         if (fork) (void)hipEventDestroy(fork);
@@ -774,13 +824,23 @@ struct Batcher {
     }
 };
 
-// Probe part of segment A: upload, preprocessing, scores and the first P1 passes (all passes for
-// a persistent pursuit); the counters go to the host only when extra passes may follow.
-lss_status probe_A(Probe& p) {
+// Upload (host mode) and preprocessing of one of probe p's window sets.
+lss_status probe_pre(Probe& p, bool alt) {
     Handle& h = *p.h;
+    int16_t* win = alt ? h.win_alt.p : h.win.p;
     if (p.mode == 1)
-        HIP_TRY(hipMemcpyAsync(h.win.p, p.pin, sizeof(int16_t) * p.W * h.C, hipMemcpyHostToDevice, h.stream));
-    if (lss_status st = enqueue_pre(h, *p.spec, h.win.p, p.W)) return st;
+        HIP_TRY(hipMemcpyAsync(win, alt ? p.pin_alt : p.pin, sizeof(int16_t) * p.W * h.C, hipMemcpyHostToDevice,
+                               h.stream));
+    return enqueue_pre(h, *p.spec, win, p.W, alt ? h.xw_alt.p : h.xw.p, alt ? h.B_alt.p : h.B.p);
+}
+
+// Probe part of segment A: upload, preprocessing (unless the last step ran it ahead), scores and
+// the first P1 passes (all passes for a persistent pursuit); the counters go to the host only when
+// extra passes may follow.
+lss_status probe_A(Probe& p, bool pre = true) {
+    Handle& h = *p.h;
+    if (pre)
+        if (lss_status st = probe_pre(p, false)) return st;
     const lss::PursuitArgs a = pursuit_args(h, h.B.p, h.xw.p, p.W);
     if (lss_status st = enqueue_pursuit(h, a, p.P1, p.max_grid)) return st;
     if (p.P1 < h.prm.max_passes)
@@ -804,11 +864,28 @@ lss_status segment_A(Batcher& b) {
     for (auto& p : b.pr) {
         Handle& h = *p.h;
         HIP_TRY(hipStreamWaitEvent(h.stream, b.fork, 0));
-        if (lss_status st = probe_A(p)) return st;
+        if (lss_status st = probe_A(p, b.cur_pre)) return st;
         if (b.single)
             if (lss_status st = probe_C(p)) return st;
         HIP_TRY(hipEventRecord(p.done, h.stream));
         HIP_TRY(hipStreamWaitEvent(b.main, p.done, 0));
+    }
+    return LSS_OK;
+}
+
+// Segment N (running ahead): the next batch's preprocessing of every probe into the second window
+// sets, on the probes' second streams.
+lss_status segment_N(Batcher& b) {
+    HIP_TRY(hipEventRecord(b.fork2, b.main2));
+    for (auto& p : b.pr) {
+        Handle& h = *p.h;
+        HIP_TRY(hipStreamWaitEvent(p.s2, b.fork2, 0));
+        std::swap(h.stream, p.s2);   // the probe functions enqueue on h.stream
+        const lss_status st = probe_pre(p, true);
+        std::swap(h.stream, p.s2);
+        if (st != LSS_OK) return st;
+        HIP_TRY(hipEventRecord(p.done2, p.s2));
+        HIP_TRY(hipStreamWaitEvent(b.main2, p.done2, 0));
     }
     return LSS_OK;
 }
@@ -825,11 +902,13 @@ lss_status segment_C(Batcher& b) {
     return LSS_OK;
 }
 
-lss_status capture(Batcher& b, lss_status (*seg)(Batcher&), hipGraph_t* g, hipGraphExec_t* ge) {
-    HIP_TRY(hipStreamBeginCapture(b.main, hipStreamCaptureModeRelaxed));
+lss_status capture(Batcher& b, lss_status (*seg)(Batcher&), hipGraph_t* g, hipGraphExec_t* ge,
+                   hipStream_t s = nullptr) {
+    if (!s) s = b.main;
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
     lss_status st = seg(b);
     hipGraph_t graph = nullptr;
-    hipError_t e = hipStreamEndCapture(b.main, &graph);
+    hipError_t e = hipStreamEndCapture(s, &graph);
     if (st != LSS_OK) {
         if (graph) (void)hipGraphDestroy(graph);
         return st;
@@ -845,32 +924,62 @@ lss_status capture(Batcher& b, lss_status (*seg)(Batcher&), hipGraph_t* g, hipGr
 // Device mode: the segments that place the window's rows (samples t0 - nt .. t0 + batch + nt - 1,
 // sample 0 repeated before the recording starts) into the handle's window buffer, appended to
 // segs; one gather launch then copies every probe's segments.
-lss_status assemble_dev(Probe& p, int64_t t0, std::vector<lss::GatherSeg>& segs) {
+lss_status assemble_dev(Probe& p, int64_t t0, std::vector<lss::GatherSeg>& segs, int16_t* win) {
     Handle& h = *p.h;
     const int C = h.C, nt = h.nt;
     int64_t r = 0;
     while (r < p.W && t0 - nt + r < 0) ++r;
-    if (r > 0) segs.push_back({h.win.p, p.dchunks.front().first, r * C, (int64_t)C, 0});
+    if (r > 0) segs.push_back({win, p.dchunks.front().first, r * C, (int64_t)C, 0});
     for (size_t k = 0; k < p.dchunks.size() && r < p.W; ++k) {
         const int64_t c0 = p.dchunks[k].second, c1 = c0 + p.dlen[k];
         const int64_t need = t0 - nt + r;   // next sample to place
         if (c1 <= need) continue;
         const int64_t from = std::max(need, c0);
         const int64_t rows = std::min(c1 - from, p.W - r);
-        segs.push_back({h.win.p + r * C, p.dchunks[k].first + (from - c0) * C, rows * C, 0, 0});
+        segs.push_back({win + r * C, p.dchunks[k].first + (from - c0) * C, rows * C, 0, 0});
         r += rows;
     }
     if (r != p.W) return fail(LSS_ERR_STATE, "device window incomplete");
     return LSS_OK;
 }
 
-void assemble(Probe& p, int64_t t0) {
+void assemble(Probe& p, int64_t t0, int16_t* pin) {
     Handle& h = *p.h;
     const int C = h.C, nt = h.nt;
     int64_t c = 0;
-    for (; c < p.W && t0 - nt + c < 0; ++c) std::memcpy(p.pin + c * C, p.buf.data() + (0 - p.base) * C, sizeof(int16_t) * C);
+    for (; c < p.W && t0 - nt + c < 0; ++c) std::memcpy(pin + c * C, p.buf.data() + (0 - p.base) * C, sizeof(int16_t) * C);
     const int64_t first = t0 - nt + c;
-    std::memcpy(p.pin + c * C, p.buf.data() + (first - p.base) * C, sizeof(int16_t) * (p.W - c) * C);
+    std::memcpy(pin + c * C, p.buf.data() + (first - p.base) * C, sizeof(int16_t) * (p.W - c) * C);
+}
+
+// The second window sets, streams and events of running ahead (once per batcher and handle).
+lss_status ensure_ahead(Batcher& b) {
+    if (!b.main2) HIP_TRY(hipStreamCreateWithFlags(&b.main2, hipStreamNonBlocking));
+    if (!b.fork2) HIP_TRY(hipEventCreateWithFlags(&b.fork2, hipEventDisableTiming));
+    if (!b.joinA) HIP_TRY(hipEventCreateWithFlags(&b.joinA, hipEventDisableTiming));
+    for (auto& p : b.pr) {
+        Handle& h = *p.h;
+        if (!h.win_alt.p) HIP_TRY(h.win_alt.alloc(h.win.n));
+        if (!h.xw_alt.p) HIP_TRY(h.xw_alt.alloc(h.xw.n));
+        if (!h.B_alt.p) HIP_TRY(h.B_alt.alloc(h.B.n));
+        if (!p.s2) HIP_TRY(hipStreamCreateWithFlags(&p.s2, hipStreamNonBlocking));
+        if (!p.done2) HIP_TRY(hipEventCreateWithFlags(&p.done2, hipEventDisableTiming));
+        if (p.mode == 1 && !p.pin_alt)
+            HIP_TRY(hipHostMalloc((void**)&p.pin_alt, sizeof(int16_t) * p.W * h.C, hipHostMallocDefault));
+    }
+    return LSS_OK;
+}
+
+// whether the step about to run ends with a template-refinement step (lss_stream_step's rule)
+bool refine_follows(const Batcher& b) {
+    return b.refine_every > 0 && (b.refine_next >= 0 || (b.batches + 1) % b.refine_every == 0);
+}
+
+bool graphs_captured(const Batcher& b) {
+    if (b.gA || b.gC) return true;
+    for (auto* e : b.execs)
+        if (e) return true;
+    return false;
 }
 
 std::string trim(const std::string& s) {
@@ -1518,6 +1627,7 @@ lss_status lss_refine_templates(void* handle, double* info) {
     if (lss_status st = refine_loss_at(h, h.rf_U1.p, h.rf_W1.p, h.rf_R.p, n, &L2)) return st;
     if (!(L2 < L0)) return LSS_OK;   // no descent: templates unchanged
     // adopt the step: U, wPCA, then every table derived from them
+    ++h.pre_version;
     HIP_TRY(hipMemcpy2DAsync(h.U.p, sizeof(float) * h.ldq, h.rf_U1.p, sizeof(float) * K * C, sizeof(float) * K * C, T,
                              hipMemcpyDeviceToDevice, h.stream));
     HIP_TRY(hipMemcpyAsync(h.wpca.p, h.rf_W1.p, sizeof(float) * nW, hipMemcpyDeviceToDevice, h.stream));
@@ -1565,6 +1675,7 @@ lss_status lss_set_drift(void* handle, const double* d_m) {
     if (!handle) return fail(LSS_ERR_ARG, "null handle");
     Handle& h = *as_handle(handle);
     const int C = h.C;
+    ++h.pre_version;
     if (!d_m) {
         HIP_TRY(lss::launch_f64_to_f32(h.wrot64.p, C, C, C, h.wmat.p, h.ldm, h.stream));
         HIP_TRY(hipStreamSynchronize(h.stream));
@@ -1735,7 +1846,7 @@ lss_status lss_stream_push_device(void* stream, int32_t probe, const int16_t* d_
     if (probe < 0 || probe >= (int32_t)b.pr.size()) return fail(LSS_ERR_ARG, "probe out of range");
     Probe& p = b.pr[probe];
     if (p.mode == 1) return fail(LSS_ERR_STATE, "this probe receives host samples");
-    if (b.gA && p.mode == 0) return fail(LSS_ERR_STATE, "the step graph was captured for host samples");
+    if (graphs_captured(b) && p.mode == 0) return fail(LSS_ERR_STATE, "the step graph was captured for host samples");
     p.mode = 2;
     if (n == 0) return LSS_OK;
     p.dchunks.emplace_back(d_samples, p.recv);
@@ -1753,24 +1864,73 @@ lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready) {
     using clk = std::chrono::steady_clock;
     auto ms_since = [](clk::time_point t) { return std::chrono::duration<float, std::milli>(clk::now() - t).count(); };
     auto t_start = clk::now();
+    // this batch preprocessed ahead by the last step: its sets become the current ones, unless an
+    // input of some probe's preprocessing changed since (then every probe preprocesses it again)
+    bool have = b.ahead;
+    for (auto& p : b.pr)
+        if (p.h->pre_version != p.ahead_version) have = false;
+    if (have) {
+        for (auto& p : b.pr) {
+            Handle& h = *p.h;
+            std::swap(h.win.p, h.win_alt.p);
+            std::swap(h.xw.p, h.xw_alt.p);
+            std::swap(h.B.p, h.B_alt.p);
+            std::swap(p.pin, p.pin_alt);
+        }
+        b.swapped ^= 1;
+    }
+    b.ahead = false;
+    // the next batch ahead: one-graph steps, its samples here, no refinement after this step (that
+    // would change the templates its scores need)
+    bool next = b.single && b.run_ahead && !refine_follows(b);
+    for (auto& p : b.pr)
+        if (p.recv < b.t0 + 2 * b.batch + p.h->nt) next = false;
+    if (next)
+        if (lss_status st = ensure_ahead(b)) return st;
+    b.cur_pre = !have;
+    b.next_pre = next;
     std::vector<lss::GatherSeg> segs;
     for (auto& p : b.pr) {
-        if (p.mode == 1) {
-            assemble(p, b.t0);
-        } else if (lss_status st = assemble_dev(p, b.t0, segs)) {
-            return st;
+        Handle& h = *p.h;
+        for (int k = 0; k < 2; ++k) {
+            if (!(k == 0 ? b.cur_pre : b.next_pre)) continue;
+            const int64_t t = b.t0 + k * b.batch;
+            if (p.mode == 1) assemble(p, t, k == 0 ? p.pin : p.pin_alt);
+            else if (lss_status st = assemble_dev(p, t, segs, k == 0 ? h.win.p : h.win_alt.p)) return st;
         }
     }
     if (!segs.empty()) HIP_TRY(lss::launch_gather(segs.data(), (int)segs.size(), b.main));
     b.part_ms[0] = ms_since(t_start);
     auto t1 = clk::now();
     // segment A
-    if (b.use_graph) {
+    if (b.use_graph && b.single) {
+        const int key = (b.cur_pre ? 1 : 0) | (b.swapped << 1);
+        if (!b.execs[key])
+            if (lss_status st = capture(b, segment_A, &b.graphs[key], &b.execs[key])) return st;
+        HIP_TRY(hipGraphLaunch(b.execs[key], b.main));
+    } else if (b.use_graph) {
         if (!b.gA)
             if (lss_status st = capture(b, segment_A, &b.graphA, &b.gA)) return st;
         HIP_TRY(hipGraphLaunch(b.gA, b.main));
     } else if (lss_status st = segment_A(b)) {
         return st;
+    }
+    if (b.next_pre) {
+        // segment N beside segment A; after it when this step also preprocesses its own batch
+        // (the two share the handles' intermediate buffers)
+        if (b.cur_pre) {
+            HIP_TRY(hipEventRecord(b.joinA, b.main));
+            HIP_TRY(hipStreamWaitEvent(b.main2, b.joinA, 0));
+        }
+        if (b.use_graph) {
+            if (!b.execsN[b.swapped])
+                if (lss_status st = capture(b, segment_N, &b.graphsN[b.swapped], &b.execsN[b.swapped], b.main2))
+                    return st;
+            HIP_TRY(hipGraphLaunch(b.execsN[b.swapped], b.main2));
+        } else if (lss_status st = segment_N(b)) {
+            return st;
+        }
+        HIP_TRY(hipStreamSynchronize(b.main2));
     }
     HIP_TRY(hipStreamSynchronize(b.main));
     b.part_ms[1] = ms_since(t1);
@@ -1818,6 +1978,11 @@ lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready) {
         if (st != LSS_OK) return st;
     }
     b.part_ms[4] = ms_since(t1);
+    if (next) {
+        b.ahead = true;
+        ++b.ran_ahead;
+        for (auto& p : b.pr) p.ahead_version = p.h->pre_version;
+    }
     ++b.batches;
     if (b.refine_every > 0) {
         // a round starts every refine_every steps (one still under way carries on); in a round,
@@ -1868,6 +2033,18 @@ lss_status lss_stream_set_refine(void* stream, int32_t every_batches) {
         for (auto& p : b.pr)
             if (lss_status st = refine_templates_alloc(*p.h)) return st;
     b.refine_every = every_batches;
+    return LSS_OK;
+}
+
+lss_status lss_stream_set_run_ahead(void* stream, int32_t on) {
+    if (!stream) return fail(LSS_ERR_ARG, "null stream");
+    static_cast<Batcher*>(stream)->run_ahead = on != 0;
+    return LSS_OK;
+}
+
+lss_status lss_stream_ran_ahead(void* stream, int64_t* steps) {
+    if (!stream || !steps) return fail(LSS_ERR_ARG, "null argument");
+    *steps = static_cast<Batcher*>(stream)->ran_ahead;
     return LSS_OK;
 }
 

@@ -13,9 +13,12 @@ from .sorter import Sorter, alloc_spikes
 
 
 class StreamingSorter:
-    def __init__(self, sorters: list[Sorter], batch: int = 3000, use_graph: bool = True, refine_every: int = 0):
+    def __init__(self, sorters: list[Sorter], batch: int = 3000, use_graph: bool = True, refine_every: int = 0,
+                 run_ahead: bool = True):
         """refine_every: one template-refinement step per probe every this many batches (drift
-        tracking, lss_refine_templates on that batch's window); 0 = never."""
+        tracking, lss_refine_templates on that batch's window); 0 = never.  run_ahead: a step whose
+        next batch is already pushed preprocesses it beside its own pursuit (lss_stream_set_run_ahead;
+        same results)."""
         self.sorters = list(sorters)
         arr = (ct.c_void_p * len(self.sorters))(*[s._h.value for s in self.sorters])
         h = ct.c_void_p()
@@ -30,6 +33,7 @@ class StreamingSorter:
         self._t0 = 0
         if refine_every:
             check(lib().lss_stream_set_refine(h, int(refine_every)), "lss_stream_set_refine")
+        check(lib().lss_stream_set_run_ahead(h, int(bool(run_ahead))), "lss_stream_set_run_ahead")
         arr_out = (Spikes * len(self.sorters))()
         self._bufs = []
         for i, s in enumerate(self.sorters):
@@ -72,6 +76,12 @@ class StreamingSorter:
             while held and held[0][2] <= keep_from:
                 held.pop(0)
         return out
+
+    def ran_ahead(self) -> int:
+        """Steps so far that preprocessed their next batch (lss_stream_ran_ahead)."""
+        n = ct.c_int64()
+        check(lib().lss_stream_ran_ahead(self._h, ct.byref(n)), "lss_stream_ran_ahead")
+        return n.value
 
     def held(self, probe: int) -> int:
         """Device tensors of a probe still referenced for the batcher."""

@@ -278,3 +278,81 @@ def test_joint_schedule_tuning_is_result_neutral(probes):
             np.testing.assert_array_equal(g.amplitude, r.amplitude)
     for s in sorters:
         s.close()
+
+
+def _stream_all(probes, sorters, run_ahead, device, refine_every=0, between=None):
+    """Every sample pushed at once (so each step finds its next batch), all batches stepped;
+    between(k) runs after step k.  Returns (outputs per probe, steps that ran ahead)."""
+    from livespikesorter_amd.streaming import StreamingSorter
+    out = [[] for _ in probes]
+    src = [torch.from_numpy(p.rec).cuda() for p in probes] if device else [p.rec for p in probes]
+    with StreamingSorter(sorters, batch=BATCH, use_graph=True, refine_every=refine_every,
+                         run_ahead=run_ahead) as st:
+        for i in range(len(probes)):
+            st.push(i, src[i])
+        k = 0
+        while True:
+            r = st.step()
+            if r is None:
+                break
+            for i in range(len(probes)):
+                out[i].append(r[i])
+            if between is not None:
+                between(k)
+            k += 1
+        ahead = st.ran_ahead()
+    return out, ahead
+
+
+def _assert_same(a, b):
+    for i in range(len(a)):
+        assert len(a[i]) == len(b[i])
+        for x, y in zip(a[i], b[i]):
+            np.testing.assert_array_equal(x.sample, y.sample)
+            np.testing.assert_array_equal(x.template, y.template)
+            np.testing.assert_array_equal(x.amplitude, y.amplitude)
+            np.testing.assert_array_equal(x.features, y.features)
+
+
+@pytest.mark.parametrize("device", [True, False])
+def test_run_ahead_returns_the_step_by_step_results(probes, device):
+    """Steps that preprocess their next batch beside their own pursuit (the second window sets,
+    swapped at the next step) return exactly what one batch at a time returns, with template
+    refinement on (refining steps do not run ahead; the others do)."""
+    print(f"[lss] test_run_ahead_returns_the_step_by_step_results device={device}", flush=True)
+    sa, sb = _make_sorters(probes, precision="bf16x3"), _make_sorters(probes, precision="bf16x3")
+    got, ahead = _stream_all(probes, sa, True, device, refine_every=2)
+    ref, ahead0 = _stream_all(probes, sb, False, device, refine_every=2)
+    nb = len(got[0])
+    print(f"[lss]   {nb} steps, {ahead} ran ahead", flush=True)
+    assert ahead0 == 0 and ahead >= 1
+    _assert_same(got, ref)
+    for s in sa + sb:
+        s.close()
+
+
+def test_run_ahead_redone_after_a_drift_change(probes):
+    """A batch preprocessed ahead with the old whitening is preprocessed again when lss_set_drift
+    changes the handle before its step: the results equal those of a stream that does not run
+    ahead, with the same change at the same point (and differ from a stream without it)."""
+    print("[lss] test_run_ahead_redone_after_a_drift_change", flush=True)
+    C = probes[0].art["Wrot"].shape[0]
+    rng = np.random.default_rng(5)
+    M = torch.from_numpy(np.eye(C) + 0.05 * rng.standard_normal((C, C))).cuda()
+
+    def run(run_ahead, change):
+        srt = _make_sorters(probes)
+        between = (lambda k: srt[0].set_drift(M) if k == 1 else None) if change else None
+        out, ahead = _stream_all(probes, srt, run_ahead, True, between=between)
+        for s in srt:
+            s.close()
+        return out, ahead
+
+    got, ahead = run(True, True)
+    ref, _ = run(False, True)
+    plain, _ = run(False, False)
+    assert ahead >= 2
+    _assert_same(got, ref)
+    # the change reached batch 2 of probe 0 (the batch preprocessed ahead by step 1)
+    assert not (np.array_equal(got[0][2].amplitude, plain[0][2].amplitude)
+                and np.array_equal(got[0][2].sample, plain[0][2].sample))
