@@ -347,7 +347,10 @@ def bench_stream(args):
                     if k == warm + K:
                         torch.cuda.synchronize()
                         t_k = time.perf_counter() - t_a
-                    if refine_every and k % refine_every == 0:
+                    if refine_every and k >= refine_every and k % refine_every == min(refine_every - 1,
+                                                                                         len(probes) + 4):
+                        # the round's refinements (one per probe, each adopted four steps after its
+                        # own) are all in by now
                         refines.append([st.last_refine(i) for i in range(len(probes))])
                 torch.cuda.synchronize()
                 t_b = time.perf_counter() - t_a

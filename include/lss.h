@@ -283,9 +283,13 @@ lss_status lss_stream_stats(void* stream, float* last_ms, int64_t* batches);
 lss_status lss_stream_breakdown(void* stream, float* ms);
 /* Template refinement while streaming: every `every_batches` steps (0 = never) a refinement round
  * starts (unless one is still under way, which carries on) in which probe i's handle takes one
- * lss_refine_templates step at the end of the round's i-th step, on that step's window (drift
- * tracking), so every probe is refined once per round; the refining step's wall time includes it.
- * lss_stream_last_refine gives a probe's last info (4 doubles). */
+ * lss_refine_templates step on the window of the round's i-th step (drift tracking), so every
+ * probe is refined once per round.  The step copies that window's residual and spikes and the
+ * refinement runs in the background (a host thread and a stream of the handle's own, writing a
+ * second template set); the fourth step after it adopts it at its start, waiting for it if needed,
+ * so results do not depend on timing (the steps in between still use the old templates).
+ * lss_stream_last_refine gives a probe's last info (4 doubles; it waits for a refinement still
+ * under way). */
 lss_status lss_stream_set_refine(void* stream, int32_t every_batches);
 lss_status lss_stream_last_refine(void* stream, int32_t probe, double* info);
 /* Running ahead (on by default; one-graph steps only): a step whose next batch's samples are

@@ -123,6 +123,14 @@ struct Handle {
     DevBuf<int> rf_flag;
     hipEvent_t rf_ev[3] = {};
     float rf_ms[3] = {};         // residual (with waves), adjoint, total of the last call
+    // the streaming batcher's refinement in the background: its stream, a copy of the window's
+    // residual and spike lists, and the second template set it writes (copied over the handle's
+    // own when the batcher adopts it)
+    hipStream_t rf_stream = nullptr;
+    DevBuf<float> sn_x, sn_amp;
+    DevBuf<int> sn_col, sn_tmp;
+    DevBuf<float> sh_U, sh_wpca, sh_nm, sh_waves, sh_ctc, sh_Ucc;
+    DevBuf<int> sh_iU;
 
     hipEvent_t ev[8] = {};
     // per-launch timing of the pursuit update kernel (measurement mode)
@@ -166,6 +174,7 @@ struct Handle {
             if (q) (void)hipHostFree(q);
         for (auto& kv : specs)
             if (kv.second.p) (void)hipFree(kv.second.p);
+        if (rf_stream) (void)hipStreamDestroy(rf_stream);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -763,6 +772,17 @@ struct Probe {
     hipStream_t s2 = nullptr;
     hipEvent_t done2 = nullptr;
     uint64_t ahead_version = 0;
+    // refinement in the background: the job, the step count at whose step start it is adopted
+    // (-1: none pending), whether it has been joined, its status, message, result and info; the
+    // event after the copy of its inputs
+    std::thread rjob;
+    int64_t rjob_adopt = -1;
+    bool rjob_joined = false;
+    lss_status rjob_st = LSS_OK;
+    std::string rjob_err;
+    bool rjob_adopted = false;
+    std::array<double, 4> rjob_info{};
+    hipEvent_t rsnap = nullptr;
 };
 
 struct Batcher {
@@ -815,6 +835,8 @@ struct Batcher {
             if (p.pin) (void)hipHostFree(p.pin);
             if (p.pin_alt) (void)hipHostFree(p.pin_alt);
             if (p.done) (void)hipEventDestroy(p.done);
+            if (p.rjob.joinable()) p.rjob.join();
+            if (p.rsnap) (void)hipEventDestroy(p.rsnap);
             if (p.done2) (void)hipEventDestroy(p.done2);
             if (p.s2) (void)hipStreamDestroy(p.s2);
             if (p.spec) --const_cast<Handle::Spec*>(p.spec)->pins;
@@ -968,11 +990,6 @@ lss_status ensure_ahead(Batcher& b) {
             HIP_TRY(hipHostMalloc((void**)&p.pin_alt, sizeof(int16_t) * p.W * h.C, hipHostMallocDefault));
     }
     return LSS_OK;
-}
-
-// whether the step about to run ends with a template-refinement step (lss_stream_step's rule)
-bool refine_follows(const Batcher& b) {
-    return b.refine_every > 0 && (b.refine_next >= 0 || (b.batches + 1) % b.refine_every == 0);
 }
 
 bool graphs_captured(const Batcher& b) {
@@ -1529,29 +1546,39 @@ lss_status lss_residual_energy_grad(void* handle, const float* d_x, int64_t W, c
 
 // Loss of the last window's spike set for templates (U1 [T][K][C], wpca1) given the window's
 // input x (rf_R holds it): sum (x - sum_s amp_s wave1_s)^2.  Spikes must be in rf_col/tmp/amp.
-lss_status refine_loss_at(Handle& h, const float* U1, const float* W1, const float* x, int n, double* loss) {
+lss_status refine_loss_at(Handle& h, hipStream_t s, int64_t W, const float* U1, const float* W1, const float* x,
+                          int n, double* loss) {
     lss::RefineArgs a{};
     a.X = x; a.ldx = h.ldC; a.vec_x = 1;
     a.col = h.rf_col.p; a.tmp = h.rf_tmp.p; a.amp = h.rf_amp.p; a.n = n;
     a.U = U1; a.wpca = W1; a.waves = h.rf_waves.p; a.R = h.rf_R2.p;
     a.part = h.rf_part.p; a.loss = h.rf_loss.p; a.flag = h.rf_flag.p;
-    a.W = h.last_W; a.ldC = h.ldC; a.T = h.T; a.C = h.C; a.K = h.K; a.nt = h.nt;
-    HIP_TRY(lss::launch_waves(U1, (int64_t)h.K * h.C, W1, h.T, h.K, h.C, h.nt, h.rf_waves.p, h.ldC, h.stream));
-    HIP_TRY(lss::launch_refine_residual(a, h.stream));
-    HIP_TRY(hipMemcpyAsync(loss, h.rf_loss.p, sizeof(double), hipMemcpyDeviceToHost, h.stream));
-    HIP_TRY(hipStreamSynchronize(h.stream));
+    a.W = W; a.ldC = h.ldC; a.T = h.T; a.C = h.C; a.K = h.K; a.nt = h.nt;
+    HIP_TRY(lss::launch_waves(U1, (int64_t)h.K * h.C, W1, h.T, h.K, h.C, h.nt, h.rf_waves.p, h.ldC, s));
+    HIP_TRY(lss::launch_refine_residual(a, s));
+    HIP_TRY(hipMemcpyAsync(loss, h.rf_loss.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
     return LSS_OK;
 }
 
-lss_status lss_refine_templates(void* handle, double* info) {
-    if (!handle) return fail(LSS_ERR_ARG, "null handle");
-    Handle& h = *as_handle(handle);
+// Where a refinement step writes the templates and the tables derived from them: the handle's own
+// (lss_refine_templates) or a second set that the streaming batcher copies in later.
+struct TemplateSet {
+    float *U, *wpca, *nm, *waves, *ctc, *Ucc;
+    int* iU;
+};
+TemplateSet live_set(Handle& h) { return {h.U.p, h.wpca.p, h.nm.p, h.waves.p, h.ctc.p, h.Ucc.p, h.iU.p}; }
+
+// One refinement step on stream s from the residual x [W][ldC] that the pursuit left and its n
+// spikes (device col/tmp/amp, pursuit order), starting from the handle's templates; on descent the
+// new U / wPCA and every table derived from them go to `out` (*adopted = true).  info as
+// lss_refine_templates.  Uses the handle's refinement buffers; the handle's templates are only read.
+lss_status refine_core(Handle& h, hipStream_t s, const float* x, int64_t W, const int32_t* d_col,
+                       const int32_t* d_tmp, const float* d_amp, int n, const TemplateSet& out, double* info,
+                       bool* adopted) {
+    *adopted = false;
     if (info)
         for (int i = 0; i < 4; ++i) info[i] = 0.0;
-    if (h.last_W < 1) return fail(LSS_ERR_STATE, "no window has been sorted yet");
-    HIP_TRY(hipMemcpyAsync(h.h_counters, h.counters.p, sizeof(int) * 8, hipMemcpyDeviceToHost, h.stream));
-    HIP_TRY(hipStreamSynchronize(h.stream));
-    const int n = std::min(h.h_counters[0] + h.h_counters[1], h.prm.max_spikes);
     if (n < 1) return LSS_OK;
     if (lss_status st = refine_templates_alloc(h)) return st;
     const int T = h.T, K = h.K, C = h.C, nt = h.nt;
@@ -1559,14 +1586,14 @@ lss_status lss_refine_templates(void* handle, double* info) {
     // the window's spikes sorted by column (the residual kernel's order)
     std::vector<int> col(n), tmp(n), ord(n);
     std::vector<float> amp(n), namp(n);
-    HIP_TRY(hipMemcpyAsync(col.data(), h.sp_col.p, sizeof(int) * n, hipMemcpyDeviceToHost, h.stream));
-    HIP_TRY(hipMemcpyAsync(tmp.data(), h.sp_tmp.p, sizeof(int) * n, hipMemcpyDeviceToHost, h.stream));
-    HIP_TRY(hipMemcpyAsync(amp.data(), h.sp_amp.p, sizeof(float) * n, hipMemcpyDeviceToHost, h.stream));
+    HIP_TRY(hipMemcpyAsync(col.data(), d_col, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(tmp.data(), d_tmp, sizeof(int) * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(amp.data(), d_amp, sizeof(float) * n, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpy2DAsync(h.rf_U.p, sizeof(float) * K * C, h.U.p, sizeof(float) * h.ldq, sizeof(float) * K * C, T,
-                             hipMemcpyDeviceToDevice, h.stream));
-    HIP_TRY(hipStreamSynchronize(h.stream));
+                             hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
     for (int i = 0; i < n; ++i) ord[i] = i;
-    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return col[x] < col[y]; });
+    std::stable_sort(ord.begin(), ord.end(), [&](int x_, int y_) { return col[x_] < col[y_]; });
     {
         std::vector<int> c2(n), t2(n);
         std::vector<float> a2(n);
@@ -1580,69 +1607,87 @@ lss_status lss_refine_templates(void* handle, double* info) {
         tmp.swap(t2);
         amp.swap(a2);
     }
-    HIP_TRY(hipMemcpyAsync(h.rf_col.p, col.data(), sizeof(int) * n, hipMemcpyHostToDevice, h.stream));
-    HIP_TRY(hipMemcpyAsync(h.rf_tmp.p, tmp.data(), sizeof(int) * n, hipMemcpyHostToDevice, h.stream));
+    HIP_TRY(hipMemcpyAsync(h.rf_col.p, col.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(h.rf_tmp.p, tmp.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
     // the window's input x = residual + sum_s amp_s wave_s (residual kernel with negated amplitudes)
-    HIP_TRY(hipMemcpyAsync(h.rf_amp.p, namp.data(), sizeof(float) * n, hipMemcpyHostToDevice, h.stream));
+    HIP_TRY(hipMemcpyAsync(h.rf_amp.p, namp.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
     double* sums = h.rf_sums.p;   // [0..255] partials, [256] |g|^2
     {
         lss::RefineArgs a{};
-        a.X = h.xw.p; a.ldx = h.ldC; a.vec_x = 1;
+        a.X = x; a.ldx = h.ldC; a.vec_x = 1;
         a.col = h.rf_col.p; a.tmp = h.rf_tmp.p; a.amp = h.rf_amp.p; a.n = n;
         a.U = h.rf_U.p; a.wpca = h.wpca.p; a.waves = h.rf_waves.p; a.R = h.rf_R.p;
         a.part = h.rf_part.p; a.loss = h.rf_loss.p; a.flag = h.rf_flag.p;
-        a.W = h.last_W; a.ldC = h.ldC; a.T = T; a.C = C; a.K = K; a.nt = nt;
-        HIP_TRY(lss::launch_waves(h.rf_U.p, (int64_t)K * C, h.wpca.p, T, K, C, nt, h.rf_waves.p, h.ldC, h.stream));
-        HIP_TRY(lss::launch_refine_residual(a, h.stream));
-        // gradient at (U0, W0) from the residual the pursuit left (R = xw); any spike order works here
-        a.R = h.xw.p;
-        a.amp = h.sp_amp.p; a.col = h.sp_col.p; a.tmp = h.sp_tmp.p;
+        a.W = W; a.ldC = h.ldC; a.T = T; a.C = C; a.K = K; a.nt = nt;
+        HIP_TRY(lss::launch_waves(h.rf_U.p, (int64_t)K * C, h.wpca.p, T, K, C, nt, h.rf_waves.p, h.ldC, s));
+        HIP_TRY(lss::launch_refine_residual(a, s));
+        // gradient at (U0, W0) from the residual the pursuit left (R = x, which the adjoint only
+        // reads); any spike order works here
+        a.R = const_cast<float*>(x);
+        a.amp = d_amp; a.col = d_col; a.tmp = d_tmp;
         a.gU = h.rf_gU.p; a.gW = h.rf_gW.p; a.dwpart = h.rf_dwpart.p; a.dupart = h.rf_dupart.p;
-        HIP_TRY(lss::launch_refine_adjoint(a, h.stream));
+        HIP_TRY(lss::launch_refine_adjoint(a, s));
     }
-    HIP_TRY(hipMemcpyAsync(h.rf_amp.p, amp.data(), sizeof(float) * n, hipMemcpyHostToDevice, h.stream));
+    HIP_TRY(hipMemcpyAsync(h.rf_amp.p, amp.data(), sizeof(float) * n, hipMemcpyHostToDevice, s));
     double g2u = 0.0, g2w = 0.0;
-    HIP_TRY(lss::launch_sumsq(h.rf_gU.p, (int64_t)nU, sums, sums + 256, h.stream));
-    HIP_TRY(lss::launch_sumsq(h.rf_gW.p, (int64_t)nW, sums, sums + 257, h.stream));
-    HIP_TRY(hipMemcpyAsync(&g2u, sums + 256, sizeof(double), hipMemcpyDeviceToHost, h.stream));
-    HIP_TRY(hipMemcpyAsync(&g2w, sums + 257, sizeof(double), hipMemcpyDeviceToHost, h.stream));
+    HIP_TRY(lss::launch_sumsq(h.rf_gU.p, (int64_t)nU, sums, sums + 256, s));
+    HIP_TRY(lss::launch_sumsq(h.rf_gW.p, (int64_t)nW, sums, sums + 257, s));
+    HIP_TRY(hipMemcpyAsync(&g2u, sums + 256, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(&g2w, sums + 257, sizeof(double), hipMemcpyDeviceToHost, s));
     double L0 = 0.0;
-    if (lss_status st = refine_loss_at(h, h.rf_U.p, h.wpca.p, h.rf_R.p, n, &L0)) return st;   // synchronises
+    if (lss_status st = refine_loss_at(h, s, W, h.rf_U.p, h.wpca.p, h.rf_R.p, n, &L0)) return st;   // synchronises
     const double g2 = g2u + g2w;
     if (!(g2 > 0.0) || !(L0 > 0.0)) return LSS_OK;
     // quadratic fit of L(alpha) = L(U0 - alpha gU, W0 - alpha gW) through one trial step
     auto step = [&](double al) -> lss_status {
-        HIP_TRY(lss::launch_step(h.rf_U1.p, h.rf_U.p, h.rf_gU.p, al, (int64_t)nU, h.stream));
-        HIP_TRY(lss::launch_step(h.rf_W1.p, h.wpca.p, h.rf_gW.p, al, (int64_t)nW, h.stream));
+        HIP_TRY(lss::launch_step(h.rf_U1.p, h.rf_U.p, h.rf_gU.p, al, (int64_t)nU, s));
+        HIP_TRY(lss::launch_step(h.rf_W1.p, h.wpca.p, h.rf_gW.p, al, (int64_t)nW, s));
         return LSS_OK;
     };
     const double a1 = 1e-4 * L0 / g2;
     if (lss_status st = step(a1)) return st;
     double L1 = 0.0;
-    if (lss_status st = refine_loss_at(h, h.rf_U1.p, h.rf_W1.p, h.rf_R.p, n, &L1)) return st;
+    if (lss_status st = refine_loss_at(h, s, W, h.rf_U1.p, h.rf_W1.p, h.rf_R.p, n, &L1)) return st;
     const double q = (L1 - L0 + a1 * g2) / (a1 * a1);
     const double al = (q > 0.0) ? std::min(g2 / (2.0 * q), 100.0 * a1) : a1;
     if (lss_status st = step(al)) return st;
     double L2 = 0.0;
-    if (lss_status st = refine_loss_at(h, h.rf_U1.p, h.rf_W1.p, h.rf_R.p, n, &L2)) return st;
+    if (lss_status st = refine_loss_at(h, s, W, h.rf_U1.p, h.rf_W1.p, h.rf_R.p, n, &L2)) return st;
     if (!(L2 < L0)) return LSS_OK;   // no descent: templates unchanged
-    // adopt the step: U, wPCA, then every table derived from them
-    ++h.pre_version;
-    HIP_TRY(hipMemcpy2DAsync(h.U.p, sizeof(float) * h.ldq, h.rf_U1.p, sizeof(float) * K * C, sizeof(float) * K * C, T,
-                             hipMemcpyDeviceToDevice, h.stream));
-    HIP_TRY(hipMemcpyAsync(h.wpca.p, h.rf_W1.p, sizeof(float) * nW, hipMemcpyDeviceToDevice, h.stream));
-    if (lss_status st = refresh_extract(h)) return st;
-    HIP_TRY(lss::launch_tables(h.U.p, h.ldq, h.wpca.p, T, K, C, nt, h.nm.p, h.waves.p, h.ldC, h.ctc.p, h.ldT,
-                               h.tables_ws.p, h.stream));
-    // complete before returning: the streaming batcher replays its graphs on another stream, which
-    // must see the new tables
-    HIP_TRY(hipStreamSynchronize(h.stream));
+    // the step: U, wPCA, then every table derived from them
+    HIP_TRY(hipMemcpy2DAsync(out.U, sizeof(float) * h.ldq, h.rf_U1.p, sizeof(float) * K * C, sizeof(float) * K * C, T,
+                             hipMemcpyDeviceToDevice, s));
+    HIP_TRY(hipMemcpyAsync(out.wpca, h.rf_W1.p, sizeof(float) * nW, hipMemcpyDeviceToDevice, s));
+    HIP_TRY(lss::launch_extract(out.U, h.ldq, T, K, C, h.iCC.p, h.nC, out.iU, out.Ucc, s));
+    HIP_TRY(lss::launch_tables(out.U, h.ldq, out.wpca, T, K, C, nt, out.nm, out.waves, h.ldC, out.ctc, h.ldT,
+                               h.tables_ws.p, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    *adopted = true;
     if (info) {
         info[0] = L0;
         info[1] = L2;
         info[2] = al;
         info[3] = std::sqrt(g2);
     }
+    return LSS_OK;
+}
+
+lss_status lss_refine_templates(void* handle, double* info) {
+    if (!handle) return fail(LSS_ERR_ARG, "null handle");
+    Handle& h = *as_handle(handle);
+    if (info)
+        for (int i = 0; i < 4; ++i) info[i] = 0.0;
+    if (h.last_W < 1) return fail(LSS_ERR_STATE, "no window has been sorted yet");
+    HIP_TRY(hipMemcpyAsync(h.h_counters, h.counters.p, sizeof(int) * 8, hipMemcpyDeviceToHost, h.stream));
+    HIP_TRY(hipStreamSynchronize(h.stream));
+    const int n = std::min(h.h_counters[0] + h.h_counters[1], h.prm.max_spikes);
+    bool adopted = false;
+    // the handle's own templates: complete before returning (streaming steps replay graphs on
+    // other streams, which must see the new tables)
+    if (lss_status st = refine_core(h, h.stream, h.xw.p, h.last_W, h.sp_col.p, h.sp_tmp.p, h.sp_amp.p, n,
+                                    live_set(h), info, &adopted))
+        return st;
+    if (adopted) ++h.pre_version;
     return LSS_OK;
 }
 
@@ -1774,6 +1819,109 @@ lss_status lss_last_timings(void* handle, float* ms, int32_t n) {
 // =================================================================================================
 extern "C" {
 
+// ---- refinement in the background ------------------------------------------------------------
+// A refinement round refines probe i at the end of its i-th step (lss_stream_set_refine).  The
+// step copies the window's residual and spike lists (device to device, before the next step may
+// overwrite them) and a host thread runs the refinement step on the handle's own stream for it,
+// writing the new templates and tables to the handle's second set; the step kRefineLag steps later
+// adopts the result at its start (waiting for the thread if it has not finished), so results do
+// not depend on timing, and the steps in between run with the old templates.  kRefineLag leaves the
+// thread the time of a few steps (a refinement step of a 384-channel / 600-template probe takes
+// ~2.5 ms beside the steps), so that the adopting step rarely waits.
+constexpr int kRefineLag = 4;
+
+static lss_status refine_buffers(Handle& h) {
+    if (lss_status st = refine_templates_alloc(h)) return st;
+    if (h.sh_U.p) return LSS_OK;
+    HIP_TRY(hipStreamCreateWithFlags(&h.rf_stream, hipStreamNonBlocking));
+    HIP_TRY(h.sn_x.alloc(h.xw.n));
+    HIP_TRY(h.sn_col.alloc(h.prm.max_spikes));
+    HIP_TRY(h.sn_tmp.alloc(h.prm.max_spikes));
+    HIP_TRY(h.sn_amp.alloc(h.prm.max_spikes));
+    HIP_TRY(h.sh_U.alloc(h.U.n));
+    HIP_TRY(h.sh_wpca.alloc(h.wpca.n));
+    HIP_TRY(h.sh_nm.alloc(h.nm.n));
+    HIP_TRY(h.sh_waves.alloc(h.waves.n));
+    HIP_TRY(h.sh_ctc.alloc(h.ctc.n));
+    HIP_TRY(h.sh_Ucc.alloc(h.Ucc.n));
+    HIP_TRY(h.sh_iU.alloc(h.iU.n));
+    return LSS_OK;
+}
+
+// Waits for probe p's job (if not done yet) and keeps its info.
+static lss_status refine_join(Batcher& b, Probe& p) {
+    if (p.rjob_adopt < 0 || p.rjob_joined) return LSS_OK;
+    p.rjob.join();
+    p.rjob_joined = true;
+    if (p.rjob_st != LSS_OK) return fail(p.rjob_st, "template refinement: " + p.rjob_err);
+    b.refine_info.resize(b.pr.size());
+    b.refine_info[&p - b.pr.data()] = p.rjob_info;
+    return LSS_OK;
+}
+
+// Adopts probe p's refinement: its templates and tables replace the handle's, in stream order
+// before the step's work (the copies run on b.main).
+static lss_status refine_adopt(Batcher& b, Probe& p) {
+    if (p.rjob_adopt < 0) return LSS_OK;
+    if (lss_status st = refine_join(b, p)) return st;
+    p.rjob_adopt = -1;
+    p.rjob_joined = false;
+    if (!p.rjob_adopted) return LSS_OK;
+    Handle& h = *p.h;
+    auto cp = [&](void* dst, const void* src, size_t bytes) {
+        return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, b.main);
+    };
+    HIP_TRY(cp(h.U.p, h.sh_U.p, sizeof(float) * h.U.n));
+    HIP_TRY(cp(h.wpca.p, h.sh_wpca.p, sizeof(float) * h.wpca.n));
+    HIP_TRY(cp(h.nm.p, h.sh_nm.p, sizeof(float) * h.nm.n));
+    HIP_TRY(cp(h.waves.p, h.sh_waves.p, sizeof(float) * h.waves.n));
+    HIP_TRY(cp(h.ctc.p, h.sh_ctc.p, sizeof(float) * h.ctc.n));
+    HIP_TRY(cp(h.Ucc.p, h.sh_Ucc.p, sizeof(float) * h.Ucc.n));
+    HIP_TRY(cp(h.iU.p, h.sh_iU.p, sizeof(int) * h.iU.n));
+    ++h.pre_version;
+    return LSS_OK;
+}
+
+// Starts probe p's refinement on the window of the step that just ended (its residual and spikes).
+static lss_status refine_start(Batcher& b, Probe& p) {
+    if (lss_status st = refine_adopt(b, p)) return st;   // one job per probe at a time
+    Handle& h = *p.h;
+    if (lss_status st = refine_buffers(h)) return st;
+    if (!p.rsnap) HIP_TRY(hipEventCreateWithFlags(&p.rsnap, hipEventDisableTiming));
+    const int n = std::min(h.hs_counters[0] + h.hs_counters[1], h.prm.max_spikes);
+    HIP_TRY(hipMemcpyAsync(h.sn_x.p, h.xw.p, sizeof(float) * p.W * h.ldC, hipMemcpyDeviceToDevice, h.rf_stream));
+    HIP_TRY(hipMemcpyAsync(h.sn_col.p, h.sp_col.p, sizeof(int) * n, hipMemcpyDeviceToDevice, h.rf_stream));
+    HIP_TRY(hipMemcpyAsync(h.sn_tmp.p, h.sp_tmp.p, sizeof(int) * n, hipMemcpyDeviceToDevice, h.rf_stream));
+    HIP_TRY(hipMemcpyAsync(h.sn_amp.p, h.sp_amp.p, sizeof(float) * n, hipMemcpyDeviceToDevice, h.rf_stream));
+    HIP_TRY(hipEventRecord(p.rsnap, h.rf_stream));
+    HIP_TRY(hipStreamWaitEvent(b.main, p.rsnap, 0));   // the next step writes them only after the copies
+    if (b.main2) HIP_TRY(hipStreamWaitEvent(b.main2, p.rsnap, 0));
+    p.rjob_adopt = b.batches + kRefineLag - 1;   // b.batches already counts the step that just ended
+    p.rjob_joined = false;
+    p.rjob_st = LSS_OK;
+    p.rjob_adopted = false;
+    p.rjob_info = {};
+    const int64_t W = p.W;
+    Handle* hp = &h;
+    Probe* pp = &p;
+    p.rjob = std::thread([hp, pp, n, W] {
+        Handle& hh = *hp;
+        if (hipSetDevice(hh.device) != hipSuccess) {
+            pp->rjob_st = LSS_ERR_HIP;
+            pp->rjob_err = "hipSetDevice failed";
+            return;
+        }
+        const TemplateSet out{hh.sh_U.p, hh.sh_wpca.p, hh.sh_nm.p, hh.sh_waves.p, hh.sh_ctc.p, hh.sh_Ucc.p,
+                              hh.sh_iU.p};
+        bool adopted = false;
+        pp->rjob_st = refine_core(hh, hh.rf_stream, hh.sn_x.p, W, hh.sn_col.p, hh.sn_tmp.p, hh.sn_amp.p, n, out,
+                                  pp->rjob_info.data(), &adopted);
+        if (pp->rjob_st != LSS_OK) pp->rjob_err = g_err;
+        pp->rjob_adopted = adopted;
+    });
+    return LSS_OK;
+}
+
 lss_status lss_stream_create(void* const* handles, int32_t n_probes, int32_t batch, int32_t use_graph,
                              void** stream_out) {
     if (!handles || n_probes < 1 || batch < 1 || !stream_out) return fail(LSS_ERR_ARG, "bad argument");
@@ -1864,6 +2012,10 @@ lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready) {
     using clk = std::chrono::steady_clock;
     auto ms_since = [](clk::time_point t) { return std::chrono::duration<float, std::milli>(clk::now() - t).count(); };
     auto t_start = clk::now();
+    // refinements due at this step replace their probes' templates first
+    for (auto& p : b.pr)
+        if (p.rjob_adopt >= 0 && p.rjob_adopt <= b.batches)
+            if (lss_status st = refine_adopt(b, p)) return st;
     // this batch preprocessed ahead by the last step: its sets become the current ones, unless an
     // input of some probe's preprocessing changed since (then every probe preprocesses it again)
     bool have = b.ahead;
@@ -1880,9 +2032,11 @@ lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready) {
         b.swapped ^= 1;
     }
     b.ahead = false;
-    // the next batch ahead: one-graph steps, its samples here, no refinement after this step (that
-    // would change the templates its scores need)
-    bool next = b.single && b.run_ahead && !refine_follows(b);
+    // the next batch ahead: one-graph steps, its samples here, and no refinement adopted at the next
+    // step's start (that changes the templates its scores need)
+    bool next = b.single && b.run_ahead;
+    for (auto& p : b.pr)
+        if (p.rjob_adopt == b.batches + 1) next = false;
     for (auto& p : b.pr)
         if (p.recv < b.t0 + 2 * b.batch + p.h->nt) next = false;
     if (next)
@@ -1991,7 +2145,7 @@ lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready) {
         if (b.refine_next >= 0) {
             const int k = b.refine_next;
             b.refine_info.resize(b.pr.size());
-            if (lss_status st = lss_refine_templates(b.pr[k].h, b.refine_info[k].data())) return st;
+            if (lss_status st = refine_start(b, b.pr[k])) return st;
             b.refine_next = (k + 1 < (int)b.pr.size()) ? k + 1 : -1;
         }
     }
@@ -2031,7 +2185,7 @@ lss_status lss_stream_set_refine(void* stream, int32_t every_batches) {
     // the refinement buffers are allocated now, not in the step that first refines
     if (every_batches > 0)
         for (auto& p : b.pr)
-            if (lss_status st = refine_templates_alloc(*p.h)) return st;
+            if (lss_status st = refine_buffers(*p.h)) return st;
     b.refine_every = every_batches;
     return LSS_OK;
 }
@@ -2052,6 +2206,7 @@ lss_status lss_stream_last_refine(void* stream, int32_t probe, double* info) {
     if (!stream || !info) return fail(LSS_ERR_ARG, "null argument");
     Batcher& b = *static_cast<Batcher*>(stream);
     if (probe < 0 || probe >= (int32_t)b.pr.size()) return fail(LSS_ERR_ARG, "probe out of range");
+    if (lss_status st = refine_join(b, b.pr[probe])) return st;   // a step still under way: its result
     for (int i = 0; i < 4; ++i) info[i] = (probe < (int32_t)b.refine_info.size()) ? b.refine_info[probe][i] : 0.0;
     return LSS_OK;
 }
