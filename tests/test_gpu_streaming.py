@@ -356,3 +356,47 @@ def test_run_ahead_redone_after_a_drift_change(probes):
     # the change reached batch 2 of probe 0 (the batch preprocessed ahead by step 1)
     assert not (np.array_equal(got[0][2].amplitude, plain[0][2].amplitude)
                 and np.array_equal(got[0][2].sample, plain[0][2].sample))
+
+
+@pytest.fixture(scope="module")
+def long_probes():
+    from livespikesorter_amd import synth
+    a = synth.generate(synth.SynthConfig(n_chan=64, n_units=24, n_samples=13 * BATCH + 200, NT=3000, seed=21))
+    b = synth.generate(synth.SynthConfig(n_chan=32, n_units=8, n_samples=13 * BATCH + 200, NT=3000, nrange=16, nC=4,
+                                         seed=22))
+    return [a, b]
+
+
+def test_background_refinement_adopted_four_steps_later(long_probes):
+    """refine_every=6: probe 0 is refined on batch 5's window in the background and adopts the result
+    at the start of step 9.  Its batches then equal: 0-8 those of an unrefined handle, 9-11 those of
+    a handle that ran lss_refine_templates right after sorting batch 5 (the same refinement step,
+    run in the foreground).  A second stream gives the same lists (no dependence on timing)."""
+    print("[lss] test_background_refinement_adopted_four_steps_later", flush=True)
+    probes = long_probes
+    runs = []
+    for _ in range(2):
+        sorters = _make_sorters(probes)
+        out, _ = _stream_all(probes, sorters, True, True, refine_every=6)
+        runs.append(out)
+        for s in sorters:
+            s.close()
+    _assert_same(runs[0], runs[1])
+    got = runs[0][0]
+    p = probes[0]
+    nt = p.art["nt"]
+    plain, refined = _make_sorters([p, p])
+    for k in range(12):
+        t0 = k * BATCH
+        w = _window(p.rec, t0, nt)
+        ref = (refined if k >= 9 or k <= 5 else plain).sort_window(w, t0 - nt)
+        if k == 5:
+            info = refined.refine_templates()
+            print(f"[lss]   refinement on batch 5: {info}", flush=True)
+            assert info["loss_after"] < info["loss_before"]
+        np.testing.assert_array_equal(got[k].sample, ref.sample)
+        np.testing.assert_array_equal(got[k].template, ref.template)
+        np.testing.assert_array_equal(got[k].amplitude, ref.amplitude)
+        np.testing.assert_array_equal(got[k].features, ref.features)
+    plain.close()
+    refined.close()
