@@ -20,6 +20,7 @@
 #include <memory>
 #include <numeric>
 #include <string>
+#include <atomic>
 #include <thread>
 #include <vector>
 
@@ -127,6 +128,7 @@ struct Handle {
     // residual and spike lists, and the second template set it writes (copied over the handle's
     // own when the batcher adopts it)
     hipStream_t rf_stream = nullptr;
+    std::atomic<bool> rf_busy{false};   // a background refinement holds the refinement buffers
     DevBuf<float> sn_x, sn_amp;
     DevBuf<int> sn_col, sn_tmp;
     DevBuf<float> sh_U, sh_wpca, sh_nm, sh_waves, sh_ctc, sh_Ucc;
@@ -835,7 +837,10 @@ struct Batcher {
             if (p.pin) (void)hipHostFree(p.pin);
             if (p.pin_alt) (void)hipHostFree(p.pin_alt);
             if (p.done) (void)hipEventDestroy(p.done);
-            if (p.rjob.joinable()) p.rjob.join();
+            if (p.rjob.joinable()) {
+                p.rjob.join();
+                p.h->rf_busy = false;
+            }
             if (p.rsnap) (void)hipEventDestroy(p.rsnap);
             if (p.done2) (void)hipEventDestroy(p.done2);
             if (p.s2) (void)hipStreamDestroy(p.s2);
@@ -1497,6 +1502,9 @@ lss_status refine_templates_alloc(Handle& h) {
     return LSS_OK;
 }
 
+const char* const kRefineBusy =
+    "a streaming batcher is refining this handle's templates in the background (until the step that adopts it)";
+
 lss_status refine_run(Handle& h, const float* d_x, int64_t W, const int32_t* d_col, const int32_t* d_tmp,
                       const float* d_amp, int64_t n, const float* d_U, const float* d_wpca, double* loss, float* d_gU,
                       float* d_gW) {
@@ -1504,6 +1512,7 @@ lss_status refine_run(Handle& h, const float* d_x, int64_t W, const int32_t* d_c
         return fail(LSS_ERR_ARG, "null argument");
     if (W < 1 || W > h.Wmax) return fail(LSS_ERR_ARG, "W must be in [1, max_window]");
     if (n < 0 || n > INT32_MAX) return fail(LSS_ERR_ARG, "bad spike count");
+    if (h.rf_busy) return fail(LSS_ERR_STATE, kRefineBusy);
     if (lss_status st = refine_alloc(h)) return st;
     lss::RefineArgs a{};
     a.X = d_x; a.ldx = h.C;
@@ -1678,6 +1687,7 @@ lss_status lss_refine_templates(void* handle, double* info) {
     if (info)
         for (int i = 0; i < 4; ++i) info[i] = 0.0;
     if (h.last_W < 1) return fail(LSS_ERR_STATE, "no window has been sorted yet");
+    if (h.rf_busy) return fail(LSS_ERR_STATE, kRefineBusy);
     HIP_TRY(hipMemcpyAsync(h.h_counters, h.counters.p, sizeof(int) * 8, hipMemcpyDeviceToHost, h.stream));
     HIP_TRY(hipStreamSynchronize(h.stream));
     const int n = std::min(h.h_counters[0] + h.h_counters[1], h.prm.max_spikes);
@@ -1853,6 +1863,7 @@ static lss_status refine_join(Batcher& b, Probe& p) {
     if (p.rjob_adopt < 0 || p.rjob_joined) return LSS_OK;
     p.rjob.join();
     p.rjob_joined = true;
+    p.h->rf_busy = false;
     if (p.rjob_st != LSS_OK) return fail(p.rjob_st, "template refinement: " + p.rjob_err);
     b.refine_info.resize(b.pr.size());
     b.refine_info[&p - b.pr.data()] = p.rjob_info;
@@ -1904,6 +1915,7 @@ static lss_status refine_start(Batcher& b, Probe& p) {
     const int64_t W = p.W;
     Handle* hp = &h;
     Probe* pp = &p;
+    h.rf_busy = true;
     p.rjob = std::thread([hp, pp, n, W] {
         Handle& hh = *hp;
         if (hipSetDevice(hh.device) != hipSuccess) {

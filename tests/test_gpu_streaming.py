@@ -400,3 +400,25 @@ def test_background_refinement_adopted_four_steps_later(long_probes):
         np.testing.assert_array_equal(got[k].features, ref.features)
     plain.close()
     refined.close()
+
+
+def test_refinement_calls_refused_while_refining_in_background(long_probes):
+    """While a stream's background refinement holds a handle's refinement buffers (from the step
+    that starts it to the step that adopts it), a direct refinement call on that handle fails with
+    a clear error instead of racing it; once the stream is closed it works."""
+    print("[lss] test_refinement_calls_refused_while_refining_in_background", flush=True)
+    from livespikesorter_amd._lib import LssError
+    from livespikesorter_amd.streaming import StreamingSorter
+    probes = long_probes
+    sorters = _make_sorters(probes)
+    with StreamingSorter(sorters, batch=BATCH, use_graph=True, refine_every=2) as st:
+        for i, p in enumerate(probes):
+            st.push(i, p.rec)
+        for _ in range(2):            # step 1 starts probe 0's refinement (adopted at step 5)
+            assert st.step() is not None
+        with pytest.raises(LssError, match="background"):
+            sorters[0].refine_templates()
+    info = sorters[0].refine_templates()
+    assert info["loss_before"] > 0
+    for s in sorters:
+        s.close()
