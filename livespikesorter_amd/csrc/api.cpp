@@ -797,6 +797,7 @@ struct Batcher {
     // beside segment A; by swapped
     hipStream_t main2 = nullptr;
     hipEvent_t fork2 = nullptr, joinA = nullptr;
+    hipEvent_t staged = nullptr;   // b.main's work before segment A: adopted templates, gathered windows
     hipGraph_t graphsN[2] = {};
     hipGraphExec_t execsN[2] = {};
     hipGraphExec_t gA = nullptr, gC = nullptr;
@@ -832,6 +833,7 @@ struct Batcher {
         }
         if (fork2) (void)hipEventDestroy(fork2);
         if (joinA) (void)hipEventDestroy(joinA);
+        if (staged) (void)hipEventDestroy(staged);
         if (main2) (void)hipStreamDestroy(main2);
         for (auto& p : pr) {
             if (p.pin) (void)hipHostFree(p.pin);
@@ -984,6 +986,7 @@ lss_status ensure_ahead(Batcher& b) {
     if (!b.main2) HIP_TRY(hipStreamCreateWithFlags(&b.main2, hipStreamNonBlocking));
     if (!b.fork2) HIP_TRY(hipEventCreateWithFlags(&b.fork2, hipEventDisableTiming));
     if (!b.joinA) HIP_TRY(hipEventCreateWithFlags(&b.joinA, hipEventDisableTiming));
+    if (!b.staged) HIP_TRY(hipEventCreateWithFlags(&b.staged, hipEventDisableTiming));
     for (auto& p : b.pr) {
         Handle& h = *p.h;
         if (!h.win_alt.p) HIP_TRY(h.win_alt.alloc(h.win.n));
@@ -2066,6 +2069,12 @@ lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready) {
         }
     }
     if (!segs.empty()) HIP_TRY(lss::launch_gather(segs.data(), (int)segs.size(), b.main));
+    if (b.next_pre) {
+        // segment N reads the windows just gathered and the templates a refinement adopted at the
+        // start of this step, both written on b.main
+        HIP_TRY(hipEventRecord(b.staged, b.main));
+        HIP_TRY(hipStreamWaitEvent(b.main2, b.staged, 0));
+    }
     b.part_ms[0] = ms_since(t_start);
     auto t1 = clk::now();
     // segment A
