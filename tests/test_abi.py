@@ -99,6 +99,11 @@ def test_argument_checks_before_any_device_work():
     assert L.lss_cluster_centroids_pca(None, p) == 1
     assert L.lss_sort_windows(None, 0, None, None, None, 1, 1, None) == 1
     assert L.lss_sort_window(None, p, 100, 1, 0, None) == 1
+    # streaming controls: null stream / output
+    assert L.lss_stream_set_run_ahead(None, 1) == 1
+    assert L.lss_stream_ran_ahead(None, None) == 1
+    assert L.lss_stream_set_refine(None, 10) == 1
+    assert L.lss_stream_last_refine(None, 0, None) == 1
 
 
 def _npy_bytes(header: str, data: bytes) -> bytes:
