@@ -315,7 +315,7 @@ def bench_stream(args):
         and each step preprocesses the next batch beside its own pursuit (lss_stream_set_run_ahead);
         otherwise live pacing: each step finds exactly its own batch (and its nt-sample halo).
         Returns per-step latencies, outputs, refinement infos, the wall times of steps [warm, warm+K)
-        and [warm, nbat), and the steps that ran ahead."""
+        and [warm, nbat), the steps that ran ahead and whether the steps ran grouped."""
         lat, outs, refines = [], [], []
         src = drecs if device_input else [p.rec for p in probes]
         first = batch + 64 if run_ahead else max(p.art["nt"] for p in probes)
@@ -355,12 +355,13 @@ def bench_stream(args):
                 torch.cuda.synchronize()
                 t_b = time.perf_counter() - t_a
                 ahead = st.ran_ahead()
+                grouped = st.grouped()
         finally:
             gc.enable()
-        return lat, outs, refines, t_k, t_b, ahead
+        return lat, outs, refines, t_k, t_b, ahead, grouped
 
     # the measured run: recordings resident in HBM, windows assembled device to device
-    lat, outs, refines, t_k, t_all, n_ahead = run_stream(sorters, True)
+    lat, outs, refines, t_k, t_all, n_ahead, grouped = run_stream(sorters, True)
 
     def fresh_sorters():
         # new handles (the measured run refined the templates of its own) with the same tuning
@@ -373,7 +374,7 @@ def bench_stream(args):
         return out
     # live pacing (no backlog, so no running ahead): the per-batch latency of a live stream
     live_sorters = fresh_sorters()
-    lat_live, outs_live, _, t_kl, _, _ = run_stream(live_sorters, True, run_ahead=False)
+    lat_live, outs_live, _, t_kl, _, _, _ = run_stream(live_sorters, True, run_ahead=False)
     for k in range(len(outs)):
         for i in range(len(probes)):
             if not (np.array_equal(outs[k][i].sample, outs_live[k][i].sample)
@@ -381,7 +382,7 @@ def bench_stream(args):
                 raise SystemExit(f"stream results differ with and without running ahead (batch {k}, probe {i})")
     # the same stream with host samples (each batch crosses PCIe): fresh sorters, same tuning
     host_sorters = fresh_sorters()
-    lat_h, _, _, t_kh, t_allh, _ = run_stream(host_sorters, False)
+    lat_h, _, _, t_kh, t_allh, _, _ = run_stream(host_sorters, False)
 
     def pooled_f1(k0, k1):
         out = []
@@ -451,7 +452,7 @@ def bench_stream(args):
         "latency_note": "per 100-ms batch (all four probes) of a live stream: each step finds only its own batch "
                         "(no running ahead), over the timed steps",
         "throughput_run": {"step_p50_ms": _pct(timed_lat, 50), "step_p99_ms": _pct(timed_lat, 99),
-                           "steps_ran_ahead": int(n_ahead), "steps": len(lat),
+                           "steps_ran_ahead": int(n_ahead), "steps": len(lat), "grouped_steps": bool(grouped),
                            "live_x_realtime": round(sec(K) / t_kl, 2),
                            "note": "value's run: the stream one batch ahead of the steps (a backlog of one batch), "
                                    "each step preprocessing the next batch beside its own pursuit; results equal "

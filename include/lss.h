@@ -303,6 +303,15 @@ lss_status lss_stream_last_refine(void* stream, int32_t probe, double* info);
 lss_status lss_stream_set_run_ahead(void* stream, int32_t on);
 /* Steps so far that preprocessed their next batch (running ahead). */
 lss_status lss_stream_ran_ahead(void* stream, int64_t* steps);
+/* Grouped steps (on by default): when every probe's pursuit runs in the persistent kernel from its
+ * first pass (one-graph steps, at most 8 probes, all the probes' persistent workgroups resident at
+ * once), a step runs the pursuit, residual, features and readback of all probes as one launch per
+ * stage on one stream instead of a branch per probe, which cuts the host cost of launching a step.
+ * Results are the same either way.  Set before the first step (LSS_ERR_STATE after it);
+ * lss_stream_grouped reports whether the steps run grouped (decided at the first step).  No
+ * counterpart in the reference (its loop sorts one probe per process, V2:632-820). */
+lss_status lss_stream_set_grouped(void* stream, int32_t on);
+lss_status lss_stream_grouped(void* stream, int32_t* grouped);
 
 /* ---- template refinement (no reference counterpart) ---------------------------------------------- */
 

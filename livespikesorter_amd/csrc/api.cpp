@@ -560,7 +560,7 @@ lss_status run_pursuit(Handle& h, float* B, float* X, int64_t W) {
     return LSS_OK;
 }
 
-lss_status run_features(Handle& h, const float* X, int64_t W) {
+lss::FeatureArgs feature_args(const Handle& h, const float* X, int64_t W) {
     lss::FeatureArgs f{};
     f.X = X; f.wpca = h.wpca.p; f.iCC = h.iCC.p; f.iU = h.iU.p; f.Ucc = h.Ucc.p;
     f.xc = h.xc.p; f.yc = h.yc.p; f.sp_col = h.sp_col.p; f.sp_tmp = h.sp_tmp.p; f.sp_amp = h.sp_amp.p;
@@ -568,7 +568,11 @@ lss_status run_features(Handle& h, const float* X, int64_t W) {
     f.tF = h.tF.p; f.xs = h.xs.p; f.ys = h.ys.p;
     f.n = 0; f.cap = h.prm.max_spikes; f.C = h.C; f.T = h.T; f.K = h.K; f.nt = h.nt; f.nC = h.nC;
     f.W = W; f.ldC = h.ldC;
-    HIP_TRY(lss::launch_features(f, kFeatureGrid, h.stream));
+    return f;
+}
+
+lss_status run_features(Handle& h, const float* X, int64_t W) {
+    HIP_TRY(lss::launch_features(feature_args(h, X, W), kFeatureGrid, h.stream));
     return LSS_OK;
 }
 
@@ -787,6 +791,10 @@ struct Probe {
     hipEvent_t rsnap = nullptr;
 };
 
+// Preprocessing lanes of a grouped step: with the pursuit tail's stream, no more hardware queues
+// than the runtime gives a process by default (4).
+constexpr int kPreLanes = 3;
+
 struct Batcher {
     std::vector<Probe> pr;
     int batch = 0;
@@ -818,7 +826,41 @@ struct Batcher {
     int refine_next = -1;    // probe the current refinement round refines next (-1: no round)
     std::vector<std::array<double, 4>> refine_info;   // per probe, last refinement (lss_refine_templates)
     float part_ms[6] = {};   // assemble, segment A, extra passes, segment C, emit, total
+    // grouped pursuit tail: every probe's counter reset, persistent pursuit, residual, features and
+    // readback in one launch each on the main stream (lss_stream_set_grouped); group: -1 not yet
+    // decided, 0 per-probe branches, 1 grouped.  Member descriptors by swapped (device and host),
+    // each probe's persistent grid and max_passes.
+    bool grouped = true;
+    int group = -1;
+    DevBuf<lss::GroupMember> gdev[2];
+    std::vector<lss::GroupMember> ghost[2];
+    int ggrid[lss::kGroupMax] = {}, gmp[lss::kGroupMax] = {};
+    // grouped steps: the preprocessing lanes (probes of each, in order), each on a stream with a
+    // hardware queue of its own, with its end events (this batch, the next batch); graphs of the
+    // lanes by (next batch << 1 | swapped) and of the pursuit tail by swapped
+    int nlanes = 0;
+    std::vector<int> lanes[kPreLanes];
+    hipStream_t lane_s[kPreLanes] = {};
+    hipEvent_t lane_ev[kPreLanes] = {}, lane_ev2[kPreLanes] = {};
+    hipGraph_t gpre[4][kPreLanes] = {};
+    hipGraphExec_t xpre[4][kPreLanes] = {};
+    hipGraph_t gtail[2] = {};
+    hipGraphExec_t xtail[2] = {};
     ~Batcher() {
+        for (int k = 0; k < 4; ++k)
+            for (int l = 0; l < kPreLanes; ++l) {
+                if (xpre[k][l]) (void)hipGraphExecDestroy(xpre[k][l]);
+                if (gpre[k][l]) (void)hipGraphDestroy(gpre[k][l]);
+            }
+        for (int k = 0; k < 2; ++k) {
+            if (xtail[k]) (void)hipGraphExecDestroy(xtail[k]);
+            if (gtail[k]) (void)hipGraphDestroy(gtail[k]);
+        }
+        for (int l = 0; l < kPreLanes; ++l) {
+            if (lane_ev[l]) (void)hipEventDestroy(lane_ev[l]);
+            if (lane_ev2[l]) (void)hipEventDestroy(lane_ev2[l]);
+            if (lane_s[l]) (void)hipStreamDestroy(lane_s[l]);
+        }
         if (gA) (void)hipGraphExecDestroy(gA);
         if (gC) (void)hipGraphExecDestroy(gC);
         if (graphA) (void)hipGraphDestroy(graphA);
@@ -884,6 +926,182 @@ lss_status probe_C(Probe& p) {
     HIP_TRY(lss::launch_residual(a, h.stream));
     if (lss_status st = run_features(h, h.xw.p, p.W)) return st;
     return enqueue_readback(h);
+}
+
+// The pursuit tail of a grouped step on the main stream, one launch per stage for all probes:
+// counter resets, persistent pursuits, residuals, features, readbacks.
+lss_status segment_tail(Batcher& b) {
+    const int n = (int)b.pr.size();
+    const lss::GroupMember* d = b.gdev[b.swapped].p;
+    const lss::GroupMember* m = b.ghost[b.swapped].data();
+    HIP_TRY(lss::launch_group_reset(d, n, b.main));
+    HIP_TRY(lss::launch_group_persistent(d, m, b.ggrid, b.gmp, n, b.main));
+    HIP_TRY(lss::launch_group_residual(d, m, n, b.main));
+    HIP_TRY(lss::launch_group_features(d, m, n, b.main));
+    HIP_TRY(lss::launch_group_stage(d, n, b.main));
+    for (int i = 0; i < n; ++i) {
+        b.pr[i].h->pers_p0 = 0;
+        b.pr[i].h->pers_used = b.ggrid[i];
+    }
+    return LSS_OK;
+}
+
+// Lane l's preprocessing of its probes' current (alt = false) or second (alt = true) window sets,
+// one probe after another on the lane's stream.
+lss_status lane_pre(Batcher& b, int l, bool alt) {
+    for (int i : b.lanes[l]) {
+        Probe& p = b.pr[i];
+        Handle& h = *p.h;
+        hipStream_t s = b.lane_s[l];
+        std::swap(h.stream, s);   // the probe functions enqueue on h.stream
+        const lss_status st = probe_pre(p, alt);
+        std::swap(h.stream, s);
+        if (st != LSS_OK) return st;
+    }
+    return LSS_OK;
+}
+
+// Captures what f enqueues on stream s into a graph and instantiates it.
+template <class F>
+lss_status capture_on(hipStream_t s, F f, hipGraph_t* g, hipGraphExec_t* ge) {
+    HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+    const lss_status st = f();
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &graph);
+    if (st != LSS_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return st;
+    }
+    if (e != hipSuccess) return fail(LSS_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    *g = graph;
+    HIP_TRY(hipGraphInstantiate(ge, graph, nullptr, nullptr, 0));
+    return LSS_OK;
+}
+
+lss_status run_lane(Batcher& b, int l, bool alt) {
+    if (!b.use_graph) return lane_pre(b, l, alt);
+    const int key = (alt ? 2 : 0) | b.swapped;
+    if (!b.xpre[key][l])
+        if (lss_status st = capture_on(b.lane_s[l], [&] { return lane_pre(b, l, alt); }, &b.gpre[key][l],
+                                       &b.xpre[key][l]))
+            return st;
+    HIP_TRY(hipGraphLaunch(b.xpre[key][l], b.lane_s[l]));
+    return LSS_OK;
+}
+
+// The device work of a grouped step.  The lanes preprocess this batch when the last step did not
+// (the tail waits for them), the tail runs on the main stream, and the lanes preprocess the next
+// batch beside it (running ahead).  Every lane starts after the main stream's work so far (the
+// gathered windows, templates adopted at the step's start); on a lane the next batch's
+// preprocessing follows this batch's, which shares the probes' intermediate buffers.
+lss_status grouped_step(Batcher& b) {
+    HIP_TRY(hipEventRecord(b.staged, b.main));
+    for (int l = 0; l < b.nlanes; ++l)
+        if (b.cur_pre || b.next_pre) HIP_TRY(hipStreamWaitEvent(b.lane_s[l], b.staged, 0));
+    if (b.cur_pre)
+        for (int l = 0; l < b.nlanes; ++l) {
+            if (lss_status st = run_lane(b, l, false)) return st;
+            HIP_TRY(hipEventRecord(b.lane_ev[l], b.lane_s[l]));
+            HIP_TRY(hipStreamWaitEvent(b.main, b.lane_ev[l], 0));
+        }
+    if (b.use_graph) {
+        if (!b.xtail[b.swapped])
+            if (lss_status st = capture_on(b.main, [&] { return segment_tail(b); }, &b.gtail[b.swapped],
+                                           &b.xtail[b.swapped]))
+                return st;
+        HIP_TRY(hipGraphLaunch(b.xtail[b.swapped], b.main));
+    } else if (lss_status st = segment_tail(b)) {
+        return st;
+    }
+    if (b.next_pre)
+        for (int l = 0; l < b.nlanes; ++l) {
+            if (lss_status st = run_lane(b, l, true)) return st;
+            HIP_TRY(hipEventRecord(b.lane_ev2[l], b.lane_s[l]));
+        }
+    if (b.next_pre)
+        for (int l = 0; l < b.nlanes; ++l) HIP_TRY(hipEventSynchronize(b.lane_ev2[l]));
+    HIP_TRY(hipStreamSynchronize(b.main));
+    return LSS_OK;
+}
+
+// A stream with a hardware queue of its own: a stream with a compute-unit mask (here every CU of
+// the device) gets a queue that the runtime's shared pool never hands to another stream.
+lss_status own_queue_stream(hipStream_t* s) {
+    int dev = 0, cus = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    std::vector<uint32_t> mask((cus + 31) / 32, 0xFFFFFFFFu);
+    if (cus % 32) mask.back() = (1u << (cus % 32)) - 1u;
+    HIP_TRY(hipExtStreamCreateWithCUMask(s, (uint32_t)mask.size(), mask.data()));
+    return LSS_OK;
+}
+
+// The lanes of grouped steps: the largest probes (by C*T) one each, the smallest sharing the last.
+lss_status make_lanes(Batcher& b) {
+    const int n = (int)b.pr.size();
+    std::vector<int> ord(n);
+    for (int i = 0; i < n; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) {
+        return (int64_t)b.pr[x].h->C * b.pr[x].h->T > (int64_t)b.pr[y].h->C * b.pr[y].h->T;
+    });
+    if (!b.staged) HIP_TRY(hipEventCreateWithFlags(&b.staged, hipEventDisableTiming));
+    b.nlanes = std::min(n, kPreLanes);
+    for (int r = 0; r < n; ++r) b.lanes[std::min(r, kPreLanes - 1)].push_back(ord[r]);
+    for (int l = 0; l < b.nlanes; ++l) {
+        if (lss_status st = own_queue_stream(&b.lane_s[l])) return st;
+        HIP_TRY(hipEventCreateWithFlags(&b.lane_ev[l], hipEventDisableTiming));
+        HIP_TRY(hipEventCreateWithFlags(&b.lane_ev2[l], hipEventDisableTiming));
+    }
+    return LSS_OK;
+}
+
+// Whether steps run grouped (decided once per batcher): one-graph steps, at most kGroupMax probes,
+// every probe's pursuit persistent from its first pass, and all the probes' persistent workgroups
+// resident at once.
+lss_status group_decide(Batcher& b) {
+    if (b.group >= 0) return LSS_OK;
+    b.group = 0;
+    const int n = (int)b.pr.size();
+    if (!b.grouped || !b.single || n > lss::kGroupMax) return LSS_OK;
+    std::vector<lss::GroupMember> m(n);
+    for (int i = 0; i < n; ++i) {
+        Probe& p = b.pr[i];
+        Handle& h = *p.h;
+        if (!use_persistent(h, p.W) || h.pers_first > 0 || h.prof_update) return LSS_OK;
+        m[i].a = pursuit_args(h, h.B.p, h.xw.p, p.W);
+        int g = 0;
+        HIP_TRY(lss::persistent_grid(m[i].a, p.max_grid > 0 ? std::min(h.pers_grid, p.max_grid) : h.pers_grid, &g));
+        if (g < 1) return LSS_OK;
+        b.ggrid[i] = g;
+        b.gmp[i] = h.prm.max_passes;
+    }
+    bool fits = false;
+    HIP_TRY(lss::group_persistent_fits(m.data(), b.ggrid, n, &fits));
+    if (!fits) return LSS_OK;
+    if (lss_status st = make_lanes(b)) return st;
+    b.group = 1;
+    return LSS_OK;
+}
+
+// The member descriptors of the current window sets (once per set, before the first launch that
+// reads them).
+lss_status group_members(Batcher& b) {
+    std::vector<lss::GroupMember>& m = b.ghost[b.swapped];
+    if (!m.empty()) return LSS_OK;
+    const int n = (int)b.pr.size();
+    std::vector<lss::GroupMember> v(n);
+    for (int i = 0; i < n; ++i) {
+        Probe& p = b.pr[i];
+        Handle& h = *p.h;
+        v[i].a = pursuit_args(h, h.B.p, h.xw.p, p.W);
+        v[i].f = feature_args(h, h.xw.p, p.W);
+        v[i].s = h.stage;
+    }
+    DevBuf<lss::GroupMember>& d = b.gdev[b.swapped];
+    HIP_TRY(d.alloc(n));
+    HIP_TRY(hipMemcpy(d.p, v.data(), sizeof(lss::GroupMember) * n, hipMemcpyHostToDevice));
+    m = std::move(v);
+    return LSS_OK;
 }
 
 // Segment A; when no probe can need extra passes (b.single) it also holds segment C, so that a
@@ -1001,9 +1219,12 @@ lss_status ensure_ahead(Batcher& b) {
 }
 
 bool graphs_captured(const Batcher& b) {
-    if (b.gA || b.gC) return true;
+    if (b.gA || b.gC || b.xtail[0] || b.xtail[1]) return true;
     for (auto* e : b.execs)
         if (e) return true;
+    for (int k = 0; k < 4; ++k)
+        for (auto* e : b.xpre[k])
+            if (e) return true;
     return false;
 }
 
@@ -1943,7 +2164,7 @@ lss_status lss_stream_create(void* const* handles, int32_t n_probes, int32_t bat
     std::unique_ptr<Batcher> b(new Batcher());
     b->batch = batch;
     b->use_graph = use_graph != 0;
-    HIP_TRY(hipStreamCreateWithFlags(&b->main, hipStreamNonBlocking));
+    if (lss_status st = own_queue_stream(&b->main)) return st;   // no other stream's work queued before a step's
     HIP_TRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
     b->pr.resize(n_probes);
     for (int i = 0; i < n_probes; ++i) {
@@ -2069,7 +2290,12 @@ lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready) {
         }
     }
     if (!segs.empty()) HIP_TRY(lss::launch_gather(segs.data(), (int)segs.size(), b.main));
-    if (b.next_pre) {
+    if (b.single) {
+        if (lss_status st = group_decide(b)) return st;
+        if (b.group == 1)
+            if (lss_status st = group_members(b)) return st;
+    }
+    if (b.next_pre && b.group != 1) {
         // segment N reads the windows just gathered and the templates a refinement adopted at the
         // start of this step, both written on b.main
         HIP_TRY(hipEventRecord(b.staged, b.main));
@@ -2078,7 +2304,9 @@ lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready) {
     b.part_ms[0] = ms_since(t_start);
     auto t1 = clk::now();
     // segment A
-    if (b.use_graph && b.single) {
+    if (b.group == 1) {
+        if (lss_status st = grouped_step(b)) return st;
+    } else if (b.use_graph && b.single) {
         const int key = (b.cur_pre ? 1 : 0) | (b.swapped << 1);
         if (!b.execs[key])
             if (lss_status st = capture(b, segment_A, &b.graphs[key], &b.execs[key])) return st;
@@ -2090,7 +2318,7 @@ lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready) {
     } else if (lss_status st = segment_A(b)) {
         return st;
     }
-    if (b.next_pre) {
+    if (b.next_pre && b.group != 1) {
         // segment N beside segment A; after it when this step also preprocesses its own batch
         // (the two share the handles' intermediate buffers)
         if (b.cur_pre) {
@@ -2214,6 +2442,21 @@ lss_status lss_stream_set_refine(void* stream, int32_t every_batches) {
 lss_status lss_stream_set_run_ahead(void* stream, int32_t on) {
     if (!stream) return fail(LSS_ERR_ARG, "null stream");
     static_cast<Batcher*>(stream)->run_ahead = on != 0;
+    return LSS_OK;
+}
+
+lss_status lss_stream_set_grouped(void* stream, int32_t on) {
+    if (!stream) return fail(LSS_ERR_ARG, "null stream");
+    Batcher& b = *static_cast<Batcher*>(stream);
+    if (b.batches > 0 || graphs_captured(b)) return fail(LSS_ERR_STATE, "set grouping before the first step");
+    b.grouped = on != 0;
+    b.group = -1;
+    return LSS_OK;
+}
+
+lss_status lss_stream_grouped(void* stream, int32_t* grouped) {
+    if (!stream || !grouped) return fail(LSS_ERR_ARG, "null argument");
+    *grouped = static_cast<Batcher*>(stream)->group == 1 ? 1 : 0;
     return LSS_OK;
 }
 

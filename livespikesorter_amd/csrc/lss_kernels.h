@@ -190,6 +190,29 @@ struct StageArgs {
 };
 hipError_t launch_stage_spikes(const StageArgs& a, hipStream_t s);
 
+// ---- grouped launches: the pursuit tail of several windows (one per handle) in one launch each ----
+// Member i of a group takes workgroups [first[i], first[i + 1]) of a launch and runs on them what
+// the single-window kernel would run on a grid of its own, so results equal those launches'.
+constexpr int kGroupMax = 8;
+constexpr int kGroupFeatureGrid = 256;   // workgroups per member of a grouped features launch
+struct GroupMember {
+    PursuitArgs a;   // the window's pursuit (counters, bar: its counter reset)
+    FeatureArgs f;   // its features (counters set)
+    StageArgs s;     // its readback
+};
+// d: the members in device memory (read by the kernels), m: the same members on the host (their
+// sizes shape the launch); n <= kGroupMax.
+hipError_t launch_group_reset(const GroupMember* d, int n, hipStream_t s);
+// Persistent pursuits from pass 0: member i on grids[i] workgroups (a grid that persistent_grid
+// gives for it) up to max_passes[i]; every member's workgroups must be co-resident together
+// (group_persistent_fits), which the LDS of the largest member per workgroup decides.
+hipError_t group_persistent_fits(const GroupMember* m, const int* grids, int n, bool* fits);
+hipError_t launch_group_persistent(const GroupMember* d, const GroupMember* m, const int* grids,
+                                   const int* max_passes, int n, hipStream_t s);
+hipError_t launch_group_residual(const GroupMember* d, const GroupMember* m, int n, hipStream_t s);
+hipError_t launch_group_features(const GroupMember* d, const GroupMember* m, int n, hipStream_t s);
+hipError_t launch_group_stage(const GroupMember* d, int n, hipStream_t s);
+
 // ---- template refinement (adjoint.hip) --------------------------------------------------------
 // Residual energy of a fixed spike set (sorted by column) and its gradient in U and wPCA.
 struct RefineArgs {

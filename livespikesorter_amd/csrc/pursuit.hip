@@ -769,19 +769,19 @@ constexpr int kPersistLoads = (kPersistMaxW + 2 * kPeakHalo + kPT - 1) / kPT;   
 // the counter reset before every pursuit) times 4096, so flags of earlier launches stay below it
 // (differences taken modulo 2^32).  Data shared between workgroups is only accessed with
 // device-scope operations, so no cache maintenance is needed.  Returns false after a timeout or
-// when another workgroup has timed out.
-__device__ __forceinline__ bool grid_barrier(const PursuitArgs& a, unsigned base, int gen, int* s_ok) {
+// when another workgroup has timed out.  g, G: this workgroup and the workgroups of its pursuit.
+__device__ __forceinline__ bool grid_barrier(const PursuitArgs& a, unsigned base, int gen, int* s_ok, int g, int G) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     const unsigned target = base + (unsigned)gen;
     unsigned* const bar = (unsigned*)a.bar;
-    if (threadIdx.x == 0) st_dev(bar + kBarStride * blockIdx.x, target);
+    if (threadIdx.x == 0) st_dev(bar + kBarStride * g, target);
     if (threadIdx.x < 64) {
         int ok = 1;
         for (int it = 0;; ++it) {
             bool behind = false;
-            for (int i = (int)threadIdx.x; i < (int)gridDim.x; i += 64)
+            for (int i = (int)threadIdx.x; i < G; i += 64)
                 behind |= (int)(ld_dev(bar + kBarStride * i) - target) < 0;
             if (__ballot(behind) == 0) break;
             if (it >= kBarrierSpin || __ballot(ld_dev(a.counters + 7) != 0) != 0) {
@@ -978,8 +978,10 @@ __device__ __forceinline__ void persist_update_pair(const PursuitArgs& a, const 
             if (c < n2 && w[c] >= nt && w[c] < a.W - nt) store_score<true>(nx, w[c], best[c], bi[c], b_[c], m_[c]);
 }
 
+// The persistent pursuit of one window by workgroup g of the G that run it (a launch of its own, or
+// a share of a grouped launch).
 template <int NV>
-__global__ __launch_bounds__(kPT) void pursuit_persistent_kernel(PursuitArgs a, int first_pass, int max_passes) {
+__device__ __forceinline__ void persistent_body(const PursuitArgs& a, int first_pass, int max_passes, int g, int G) {
     extern __shared__ floatx4 psm4[];
     float* const psm = (float*)psm4;
     __shared__ int wsum[kPW];
@@ -987,7 +989,6 @@ __global__ __launch_bounds__(kPT) void pursuit_persistent_kernel(PursuitArgs a, 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int W = (int)a.W, H = kPeakHalo, nt = a.nt, L = 2 * nt + 1;
     const int N = W + 2 * H;
-    const int G = (int)gridDim.x, g = (int)blockIdx.x;
     const int nbk = (W + 7) / 8;   // blocks of 8 columns, one per thread
     const PersistLayout ly(W, (int)a.ldT, G);
     float* cf = psm + ly.cf;
@@ -1001,8 +1002,8 @@ __global__ __launch_bounds__(kPT) void pursuit_persistent_kernel(PursuitArgs a, 
     int* cvj = (int*)(psm + ly.cvj);
     int* cvl = (int*)(psm + ly.cvl);
     int* cvp = (int*)(psm + ly.cvp);
-    const bool lead = blockIdx.x == 0 && tid == 0;
-    if (a.ts && tid == 0) a.ts[2 * blockIdx.x] = __builtin_amdgcn_s_memrealtime();   // measurement mode
+    const bool lead = g == 0 && tid == 0;
+    if (a.ts && tid == 0) a.ts[2 * g] = __builtin_amdgcn_s_memrealtime();   // measurement mode
     // own columns g + G j (j < nown): strided, so that every workgroup holds a share of the columns
     // a pass updates wherever its spikes are; their B rows into LDS
     const int nown = g < W ? (W - 1 - g) / G + 1 : 0;
@@ -1071,7 +1072,7 @@ __global__ __launch_bounds__(kPT) void pursuit_persistent_kernel(PursuitArgs a, 
     }
     const unsigned bbase = (unsigned)a.counters[kEpoch] * 4096u;
     int gen = 1;
-    if (!grid_barrier(a, bbase, gen++, &s_ok)) return;
+    if (!grid_barrier(a, bbase, gen++, &s_ok, g, G)) return;
     // this thread's block: columns c0 .. c0 + 7, r0 = c0 mod G (the first is owned when r0 == g)
     const int c0 = 8 * tid, r0 = c0 % G;
     for (int pass = first_pass; pass < max_passes && !done; ++pass) {
@@ -1220,7 +1221,7 @@ __global__ __launch_bounds__(kPT) void pursuit_persistent_kernel(PursuitArgs a, 
         base += n;
         par ^= 1;
         cur = nxt;
-        if (pass + 1 < max_passes && !grid_barrier(a, bbase, gen++, &s_ok)) return;
+        if (pass + 1 < max_passes && !grid_barrier(a, bbase, gen++, &s_ok, g, G)) return;
     }
     if (lead) {
         a.counters[0] = base;
@@ -1235,8 +1236,13 @@ __global__ __launch_bounds__(kPT) void pursuit_persistent_kernel(PursuitArgs a, 
     if (a.ts) {
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        if (tid == 0) a.ts[2 * blockIdx.x + 1] = __builtin_amdgcn_s_memrealtime();
+        if (tid == 0) a.ts[2 * g + 1] = __builtin_amdgcn_s_memrealtime();
     }
+}
+
+template <int NV>
+__global__ __launch_bounds__(kPT) void pursuit_persistent_kernel(PursuitArgs a, int first_pass, int max_passes) {
+    persistent_body<NV>(a, first_pass, max_passes, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1255,8 +1261,8 @@ constexpr int kResCap = 1024;       // listed spikes per workgroup and run of pa
 constexpr int kResMaxPasses = 256;
 constexpr int kResBatch = 8;        // wave-row loads in flight per column
 
-template <int NVC>   // float4 channel chunks per lane: NVC*256 >= C
-__global__ __launch_bounds__(256) void residual_kernel(PursuitArgs a) {
+template <int NVC>   // float4 channel chunks per lane: NVC*256 >= C; blk: the workgroup's 16 columns
+__device__ __forceinline__ void residual_body(const PursuitArgs& a, int blk) {
     __shared__ int s_first[kResMaxPasses], s_cnt[kResMaxPasses], s_off[kResMaxPasses + 1];
     __shared__ int l_col[kResCap], l_tmp[kResCap], l_pp[kResCap];   // l_pp = pass * 2 + parity
     __shared__ float l_amp[kResCap];
@@ -1266,7 +1272,7 @@ __global__ __launch_bounds__(256) void residual_kernel(PursuitArgs a) {
     const int P = a.counters[4];                // passes that found spikes
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int h = a.nt / 2;
-    const int64_t w0 = (int64_t)blockIdx.x * kResCols;
+    const int64_t w0 = (int64_t)blk * kResCols;
     const int lo = (int)(w0 - h), hi = (int)(w0 + kResCols - 1 + h);
     for (int p = tid; p < P; p += 256) {
         const int b0 = a.pass_off[p], b1 = a.pass_off[p + 1];
@@ -1381,6 +1387,11 @@ __global__ __launch_bounds__(256) void residual_kernel(PursuitArgs a) {
         __syncthreads();   // the list is rewritten by the next run
         next = p1;
     }
+}
+
+template <int NVC>
+__global__ __launch_bounds__(256) void residual_kernel(PursuitArgs a) {
+    residual_body<NVC>(a, (int)blockIdx.x);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1569,7 +1580,8 @@ hipError_t launch_residual(const PursuitArgs& a, hipStream_t s) {
 // per spike.  The spike's nt x nC residual window is gathered into LDS with all loads in flight at
 // once, then lane (j, k) forms sum_m X[w - nt/2 + m][iCC[j, iU[t]]] wPCA[k][m] + amp Ucc[j][t][k].
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void features_kernel(FeatureArgs a) {
+// Workgroup g of G (grid-stride over the spikes).
+__device__ __forceinline__ void features_body(const FeatureArgs& a, int g, int G) {
     // wpca [K*nt] | per wavefront: window [nt][nC], features [nC*K], channel mass, x, y [nC]
     extern __shared__ float fsm[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1584,7 +1596,7 @@ __global__ __launch_bounds__(256) void features_kernel(FeatureArgs a) {
     float* sy = sx + nC;
     for (int e = threadIdx.x; e < K * nt; e += 256) wp[e] = a.wpca[e];
     __syncthreads();
-    for (int s0 = blockIdx.x * 4; s0 < n; s0 += gridDim.x * 4) {
+    for (int s0 = g * 4; s0 < n; s0 += G * 4) {
         const int s = s0 + wave;
         if (s < n) {
             const int w = a.sp_col[s];
@@ -1642,6 +1654,10 @@ __global__ __launch_bounds__(256) void features_kernel(FeatureArgs a) {
     }
 }
 
+__global__ __launch_bounds__(256) void features_kernel(FeatureArgs a) {
+    features_body(a, (int)blockIdx.x, (int)gridDim.x);
+}
+
 hipError_t launch_features(const FeatureArgs& a, int grid, hipStream_t s) {
     if (a.nC * a.K > 32 * 8 || a.nC > 32) return hipErrorInvalidValue;
     const size_t lds = sizeof(float) * ((size_t)a.K * a.nt + 4 * ((size_t)a.nt * a.nC + a.nC * a.K + 3 * a.nC));
@@ -1658,10 +1674,10 @@ hipError_t launch_features(const FeatureArgs& a, int grid, hipStream_t s) {
 // Readback staging: the counters and the first min(n, cap) spikes (n = counters[0] + counters[1])
 // written straight into pinned host buffers, so that a replayed graph moves only the spikes found.
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void stage_spikes_kernel(StageArgs a) {
+__device__ __forceinline__ void stage_body(const StageArgs& a, int g, int G) {
     int n = a.counters[0] + a.counters[1];
     n = min(n, a.cap);
-    const int gid = blockIdx.x * 256 + threadIdx.x, stride = gridDim.x * 256;
+    const int gid = g * 256 + threadIdx.x, stride = G * 256;
     if (gid < 8) a.h_counters[gid] = a.counters[gid];
     for (int i = gid; i < n; i += stride) {
         a.h_col[i] = a.col[i];
@@ -1674,6 +1690,10 @@ __global__ __launch_bounds__(256) void stage_spikes_kernel(StageArgs a) {
     for (int64_t e = gid; e < nf; e += stride) a.h_tF[e] = a.tF[e];
 }
 
+__global__ __launch_bounds__(256) void stage_spikes_kernel(StageArgs a) {
+    stage_body(a, (int)blockIdx.x, (int)gridDim.x);
+}
+
 hipError_t launch_stage_spikes(const StageArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(stage_spikes_kernel, dim3(64), dim3(256), 0, s, a);
     return hipGetLastError();
@@ -1682,7 +1702,7 @@ hipError_t launch_stage_spikes(const StageArgs& a, hipStream_t s) {
 // counters[0..7] = 0 (a fill kernel, so that the sequence can be captured in a graph)
 // Counters zeroed and the persistent pursuit's barrier epoch advanced; every 2^16 epochs the barrier
 // flags are raised to the new epoch's base, so that no flag falls 2^31 behind a barrier's target.
-__global__ void reset_counters_kernel(int* c, int* bar) {
+__device__ __forceinline__ void reset_body(int* c, int* bar) {
     const int e = c[kEpoch] + 1;
     __syncthreads();
     if (threadIdx.x < kCounters) c[threadIdx.x] = threadIdx.x == kEpoch ? e : 0;
@@ -1690,8 +1710,189 @@ __global__ void reset_counters_kernel(int* c, int* bar) {
         for (int i = threadIdx.x; i < kPersistMaxGrid; i += blockDim.x) bar[kBarStride * i] = (int)((unsigned)e * 4096u);
 }
 
+__global__ void reset_counters_kernel(int* c, int* bar) { reset_body(c, bar); }
+
 hipError_t launch_reset_counters(int* counters, hipStream_t s, int* bar) {
     hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, counters, bar);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------------------
+// Grouped launches: the windows of several handles (members) in one launch per stage, member i on
+// workgroups [first[i], first[i + 1]), each running the single-window kernel's body on its share.
+// The member descriptors are read from device memory; the span (kernel argument) is indexed only
+// with constants.
+// ------------------------------------------------------------------------------------------------
+struct GroupSpan {
+    int n;
+    int first[kGroupMax + 1];
+    int aux[kGroupMax];   // persistent pursuit: the member's max_passes
+};
+
+// The member this workgroup belongs to, its first workgroup g0, its end g1 and its aux entry.
+__device__ __forceinline__ int group_member(const GroupSpan& sp, int* g0, int* g1, int* aux) {
+    int m = 0, a0 = 0, a1 = sp.first[1], x = sp.aux[0];
+#pragma unroll
+    for (int i = 1; i < kGroupMax; ++i)
+        if (i < sp.n && (int)blockIdx.x >= sp.first[i]) {
+            m = i;
+            a0 = sp.first[i];
+            a1 = sp.first[i + 1];
+            x = sp.aux[i];
+        }
+    *g0 = a0;
+    *g1 = a1;
+    *aux = x;
+    return m;
+}
+
+__global__ void reset_group_kernel(const GroupMember* __restrict__ d) {
+    reset_body(d[blockIdx.x].a.counters, d[blockIdx.x].a.bar);
+}
+
+__global__ __launch_bounds__(kPT) void pursuit_persistent_group_kernel(const GroupMember* __restrict__ d, GroupSpan sp) {
+    int g0, g1, mp;
+    const int m = group_member(sp, &g0, &g1, &mp);
+    const PursuitArgs a = d[m].a;
+    const int g = (int)blockIdx.x - g0, G = g1 - g0;
+    switch ((a.T + 255) >> 8) {
+        case 1: persistent_body<1>(a, 0, mp, g, G); break;
+        case 2: persistent_body<2>(a, 0, mp, g, G); break;
+        case 3: persistent_body<3>(a, 0, mp, g, G); break;
+        case 4: persistent_body<4>(a, 0, mp, g, G); break;
+        default: break;   // rejected by the launcher
+    }
+}
+
+template <int NVC>   // for the widest member: NVC*256 >= C of every member
+__global__ __launch_bounds__(256) void residual_group_kernel(const GroupMember* __restrict__ d, GroupSpan sp) {
+    int g0, g1, x;
+    const int m = group_member(sp, &g0, &g1, &x);
+    const PursuitArgs a = d[m].a;
+    residual_body<NVC>(a, (int)blockIdx.x - g0);
+}
+
+__global__ __launch_bounds__(256) void features_group_kernel(const GroupMember* __restrict__ d, GroupSpan sp) {
+    int g0, g1, x;
+    const int m = group_member(sp, &g0, &g1, &x);
+    const FeatureArgs f = d[m].f;
+    features_body(f, (int)blockIdx.x - g0, g1 - g0);
+}
+
+__global__ __launch_bounds__(256) void stage_group_kernel(const GroupMember* __restrict__ d, GroupSpan sp) {
+    int g0, g1, x;
+    const int m = group_member(sp, &g0, &g1, &x);
+    const StageArgs st = d[m].s;
+    stage_body(st, (int)blockIdx.x - g0, g1 - g0);
+}
+
+static GroupSpan even_span(int n, int each) {
+    GroupSpan sp{};
+    sp.n = n;
+    for (int i = 0; i <= n; ++i) sp.first[i] = i * each;
+    return sp;
+}
+
+hipError_t launch_group_reset(const GroupMember* d, int n, hipStream_t s) {
+    if (!d || n < 1 || n > kGroupMax) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(reset_group_kernel, dim3(n), dim3(64), 0, s, d);
+    return hipGetLastError();
+}
+
+// LDS per workgroup of a grouped persistent launch (the largest member's) and its total grid; false
+// when a member cannot run in it.
+static bool group_persistent_shape(const GroupMember* m, const int* grids, int n, size_t* lds, int* total) {
+    *lds = 0;
+    *total = 0;
+    if (!m || !grids || n < 1 || n > kGroupMax) return false;
+    for (int i = 0; i < n; ++i) {
+        const PursuitArgs& a = m[i].a;
+        if (!persistent_fn(a.T) || a.W < 1 || a.W > kPersistMaxW || a.nt > kPeakHalo || grids[i] < 1 ||
+            grids[i] > kPersistMaxGrid || !a.prow_cf || !a.prow_im || !a.prow_amp || !a.bar || a.ldT % 4 != 0)
+            return false;
+        *lds = std::max(*lds, persistent_lds(a.W, a.ldT, grids[i]));
+        *total += grids[i];
+    }
+    return *lds <= kPersistLdsMax;
+}
+
+hipError_t group_persistent_fits(const GroupMember* m, const int* grids, int n, bool* fits) {
+    *fits = false;
+    size_t lds = 0;
+    int total = 0;
+    if (!group_persistent_shape(m, grids, n, &lds, &total)) return hipSuccess;
+    int dev = 0, cus = 0, per = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e == hipSuccess)
+        e = hipFuncSetAttribute((const void*)pursuit_persistent_group_kernel,
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kPersistLdsMax);
+    if (e == hipSuccess)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void*)pursuit_persistent_group_kernel, kPT, lds);
+    if (e == hipSuccess) *fits = per * cus >= total;
+    return e;
+}
+
+hipError_t launch_group_persistent(const GroupMember* d, const GroupMember* m, const int* grids, const int* max_passes,
+                                   int n, hipStream_t s) {
+    size_t lds = 0;
+    int total = 0;
+    if (!d || !max_passes || !group_persistent_shape(m, grids, n, &lds, &total)) return hipErrorInvalidValue;
+    bool fits = false;
+    const hipError_t e = group_persistent_fits(m, grids, n, &fits);
+    if (e != hipSuccess) return e;
+    if (!fits) return hipErrorInvalidConfiguration;   // not every workgroup would be resident at once
+    GroupSpan sp{};
+    sp.n = n;
+    for (int i = 0; i < n; ++i) {
+        if (max_passes[i] < 1) return hipErrorInvalidValue;
+        sp.first[i + 1] = sp.first[i] + grids[i];
+        sp.aux[i] = max_passes[i];
+    }
+    hipLaunchKernelGGL(pursuit_persistent_group_kernel, dim3(total), dim3(kPT), lds, s, d, sp);
+    return hipGetLastError();
+}
+
+hipError_t launch_group_residual(const GroupMember* d, const GroupMember* m, int n, hipStream_t s) {
+    if (!d || !m || n < 1 || n > kGroupMax) return hipErrorInvalidValue;
+    GroupSpan sp{};
+    sp.n = n;
+    int cmax = 0;
+    for (int i = 0; i < n; ++i) {
+        if (m[i].a.C > 1024 || m[i].a.W < 1) return hipErrorInvalidValue;
+        cmax = std::max(cmax, m[i].a.C);
+        sp.first[i + 1] = sp.first[i] + (int)((m[i].a.W + kResCols - 1) / kResCols);
+    }
+    const dim3 grid(sp.first[n]);
+    switch ((cmax + 255) / 256) {
+        case 1: hipLaunchKernelGGL(residual_group_kernel<1>, grid, dim3(256), 0, s, d, sp); break;
+        case 2: hipLaunchKernelGGL(residual_group_kernel<2>, grid, dim3(256), 0, s, d, sp); break;
+        default: hipLaunchKernelGGL(residual_group_kernel<4>, grid, dim3(256), 0, s, d, sp); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_group_features(const GroupMember* d, const GroupMember* m, int n, hipStream_t s) {
+    if (!d || !m || n < 1 || n > kGroupMax) return hipErrorInvalidValue;
+    size_t lds = 0;
+    for (int i = 0; i < n; ++i) {
+        const FeatureArgs& a = m[i].f;
+        if (a.nC * a.K > 32 * 8 || a.nC > 32 || !a.counters) return hipErrorInvalidValue;
+        lds = std::max(lds, sizeof(float) * ((size_t)a.K * a.nt + 4 * ((size_t)a.nt * a.nC + a.nC * a.K + 3 * a.nC)));
+    }
+    if (lds > 65536) {
+        const hipError_t e = hipFuncSetAttribute((const void*)features_group_kernel,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(features_group_kernel, dim3(n * kGroupFeatureGrid), dim3(256), lds, s, d,
+                       even_span(n, kGroupFeatureGrid));
+    return hipGetLastError();
+}
+
+hipError_t launch_group_stage(const GroupMember* d, int n, hipStream_t s) {
+    if (!d || n < 1 || n > kGroupMax) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(stage_group_kernel, dim3(n * 64), dim3(256), 0, s, d, even_span(n, 64));
     return hipGetLastError();
 }
 

@@ -14,11 +14,12 @@ from .sorter import Sorter, alloc_spikes
 
 class StreamingSorter:
     def __init__(self, sorters: list[Sorter], batch: int = 3000, use_graph: bool = True, refine_every: int = 0,
-                 run_ahead: bool = True):
+                 run_ahead: bool = True, grouped: bool = True):
         """refine_every: one template-refinement step per probe every this many batches (drift
         tracking, lss_refine_templates on that batch's window); 0 = never.  run_ahead: a step whose
         next batch is already pushed preprocesses it beside its own pursuit (lss_stream_set_run_ahead;
-        same results)."""
+        same results).  grouped: all probes' pursuit tails as one launch per stage when every probe's
+        pursuit is persistent (lss_stream_set_grouped; same results)."""
         self.sorters = list(sorters)
         arr = (ct.c_void_p * len(self.sorters))(*[s._h.value for s in self.sorters])
         h = ct.c_void_p()
@@ -34,6 +35,7 @@ class StreamingSorter:
         if refine_every:
             check(lib().lss_stream_set_refine(h, int(refine_every)), "lss_stream_set_refine")
         check(lib().lss_stream_set_run_ahead(h, int(bool(run_ahead))), "lss_stream_set_run_ahead")
+        check(lib().lss_stream_set_grouped(h, int(bool(grouped))), "lss_stream_set_grouped")
         arr_out = (Spikes * len(self.sorters))()
         self._bufs = []
         for i, s in enumerate(self.sorters):
@@ -82,6 +84,12 @@ class StreamingSorter:
         n = ct.c_int64()
         check(lib().lss_stream_ran_ahead(self._h, ct.byref(n)), "lss_stream_ran_ahead")
         return n.value
+
+    def grouped(self) -> bool:
+        """Whether the steps run grouped (lss_stream_grouped; decided at the first step)."""
+        g = ct.c_int32()
+        check(lib().lss_stream_grouped(self._h, ct.byref(g)), "lss_stream_grouped")
+        return bool(g.value)
 
     def held(self, probe: int) -> int:
         """Device tensors of a probe still referenced for the batcher."""
@@ -178,7 +186,9 @@ def autotune_stream(sorters: list[Sorter], recordings, batch: int = 3000, steps:
     not always the fastest together (a persistent pursuit keeps its workgroups for the whole
     pursuit, per-pass kernels take whatever the other probes leave free).  options[i] lists probe
     i's candidate (persistent grid, first per-pass rounds) pairs; by default the probe's current
-    setting and (its grid, the first-segment pass count autotune chose).  Every combination (at most
+    setting, (its grid, the first-segment pass count autotune chose) and (its grid, 0): when every
+    probe's pursuit is persistent from its first pass the steps run grouped
+    (lss_stream_set_grouped).  Every combination (at most
     64) runs `warmup` + `steps` graph-replayed steps of a fresh batcher over the device recordings
     (torch int16 [n][C], one per probe, each at least (warmup + steps + 1) * batch + nt samples); the
     combination with the lowest median step time is set on the sorters.  Results do not depend on
@@ -195,6 +205,8 @@ def autotune_stream(sorters: list[Sorter], recordings, batch: int = 3000, steps:
             opts = [(g, f)]
             if p1 > 0 and (g, p1) not in opts:
                 opts.append((g, p1))
+            if g > 0 and (g, 0) not in opts:
+                opts.append((g, 0))
             options.append(opts)
     combos = list(itertools.product(*options))
     if len(combos) > 64:

@@ -102,6 +102,8 @@ def test_argument_checks_before_any_device_work():
     # streaming controls: null stream / output
     assert L.lss_stream_set_run_ahead(None, 1) == 1
     assert L.lss_stream_ran_ahead(None, None) == 1
+    assert L.lss_stream_set_grouped(None, 1) == 1
+    assert L.lss_stream_grouped(None, None) == 1
     assert L.lss_stream_set_refine(None, 10) == 1
     assert L.lss_stream_last_refine(None, 0, None) == 1
 

@@ -280,14 +280,16 @@ def test_joint_schedule_tuning_is_result_neutral(probes):
         s.close()
 
 
-def _stream_all(probes, sorters, run_ahead, device, refine_every=0, between=None):
+def _stream_all(probes, sorters, run_ahead, device, refine_every=0, between=None, grouped=True, flags=None,
+                use_graph=True):
     """Every sample pushed at once (so each step finds its next batch), all batches stepped;
-    between(k) runs after step k.  Returns (outputs per probe, steps that ran ahead)."""
+    between(k) runs after step k.  Returns (outputs per probe, steps that ran ahead); flags, when
+    given, gets whether the steps ran grouped."""
     from livespikesorter_amd.streaming import StreamingSorter
     out = [[] for _ in probes]
     src = [torch.from_numpy(p.rec).cuda() for p in probes] if device else [p.rec for p in probes]
-    with StreamingSorter(sorters, batch=BATCH, use_graph=True, refine_every=refine_every,
-                         run_ahead=run_ahead) as st:
+    with StreamingSorter(sorters, batch=BATCH, use_graph=use_graph, refine_every=refine_every,
+                         run_ahead=run_ahead, grouped=grouped) as st:
         for i in range(len(probes)):
             st.push(i, src[i])
         k = 0
@@ -301,6 +303,8 @@ def _stream_all(probes, sorters, run_ahead, device, refine_every=0, between=None
                 between(k)
             k += 1
         ahead = st.ran_ahead()
+        if flags is not None:
+            flags.append(st.grouped())
     return out, ahead
 
 
@@ -421,4 +425,48 @@ def test_refinement_calls_refused_while_refining_in_background(long_probes):
     info = sorters[0].refine_templates()
     assert info["loss_before"] > 0
     for s in sorters:
+        s.close()
+
+
+@pytest.mark.parametrize("device,run_ahead,use_graph", [(True, True, True), (False, False, True),
+                                                        (True, False, False)])
+def test_grouped_steps_equal_per_probe_branches(probes, device, run_ahead, use_graph):
+    """A step that runs every probe's pursuit tail as one launch per stage (each probe on its own
+    share of the workgroups) returns exactly what a step with a branch per probe returns, with
+    refinement on, graphs or direct launches, running ahead or not."""
+    print(f"[lss] test_grouped_steps_equal_per_probe_branches device={device} run_ahead={run_ahead} "
+          f"graph={use_graph}", flush=True)
+    sa, sb = _make_sorters(probes, precision="bf16x3"), _make_sorters(probes, precision="bf16x3")
+    fa, fb = [], []
+    got, _ = _stream_all(probes, sa, run_ahead, device, refine_every=2, grouped=True, flags=fa,
+                         use_graph=use_graph)
+    ref, _ = _stream_all(probes, sb, run_ahead, device, refine_every=2, grouped=False, flags=fb,
+                         use_graph=use_graph)
+    assert fa == [True] and fb == [False]
+    _assert_same(got, ref)
+    for s in sa + sb:
+        s.close()
+
+
+def test_grouping_not_used_with_per_pass_rounds_and_set_before_the_first_step(probes):
+    """A probe whose pursuit starts on the per-pass kernels keeps the per-probe branches (same
+    results); grouping cannot change once the stream has stepped."""
+    print("[lss] test_grouping_not_used_with_per_pass_rounds_and_set_before_the_first_step", flush=True)
+    from livespikesorter_amd._lib import LssError, check, lib
+    from livespikesorter_amd.streaming import StreamingSorter
+    sa, sb = _make_sorters(probes), _make_sorters(probes)
+    sa[1].set_persistent(64, 2)
+    fa = []
+    got, _ = _stream_all(probes, sa, True, True, flags=fa)
+    ref, _ = _stream_all(probes, sb, True, True)
+    assert fa == [False]
+    _assert_same(got, ref)
+    with StreamingSorter(sb, batch=BATCH) as st:
+        for i, p in enumerate(probes):
+            st.push(i, p.rec[:2 * BATCH])
+        assert st.step() is not None
+        assert st.grouped()
+        with pytest.raises(LssError, match="before the first step"):
+            check(lib().lss_stream_set_grouped(st._h, 0), "lss_stream_set_grouped")
+    for s in sa + sb:
         s.close()
