@@ -143,6 +143,7 @@ METRIC = "x-realtime & p99 batch latency, 384ch@30kHz; bf16x3 vs fp32 at spike F
 STREAM_PROBES = [(384, 600), (384, 250), (128, 120), (64, 40)]   # (channels, templates)
 STREAM_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "update_kernel_traffic_stream_384ch_600t.md")
 STREAM_PERSIST_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "persistent_kernel_traffic_stream_384ch_600t.md")
+STREAM_GROUP_TRAFFIC_FILE = os.path.join(ROOT, "profiles", "group_persistent_traffic_stream_4probes.md")
 
 
 def _pct(x, q):
@@ -248,6 +249,62 @@ def bench_stream(args):
         nt = probes[i].art["nt"]
         return w0[i] if k == 0 else drecs[i][k * batch - nt:(k + 1) * batch + nt]
 
+    def fresh_sorters():
+        # new handles (a measured run refines the templates of its own) with the same tuning
+        out = [mk(p, args.precision) for p in probes]
+        for s2, tu in zip(out, tuning):
+            s2.set_tiles(*tu["tiles"])
+            s2.set_persistent(tu["persistent_grid"], tu["persistent_after"])
+            _l().lss_set_tuning(s2._h, tu["update_grid"], tu["first_passes"])
+            _l().lss_set_update_columns(s2._h, tu["update_columns"])
+        return out
+
+    def group_roofline(k0, nsteps):
+        # the grouped persistent pursuit (every probe's passes in one launch: the step's critical
+        # path) in measurement mode over steps k0 .. k0 + nsteps - 1 of a stream that runs ahead as
+        # value's run does (no refinement); None when the schedule does not run grouped
+        srt = fresh_sorters()
+        bps = [3 * 4 * s_.T * (2 * s_.nt + 1) for s_ in srt]   # ctc rows read + B rows read and written
+        rows = []
+        need = (k0 + nsteps + 2) * batch + 64
+        with StreamingSorter(srt, batch=batch, use_graph=True, run_ahead=True) as st:
+            st.profile(True)
+            for i in range(len(probes)):
+                st.push(i, drecs[i][:need])
+            k = 0
+            while k < k0 + nsteps:
+                if st.step() is None:
+                    break
+                if k >= k0:
+                    rows.append(st.last_profile())
+                k += 1
+            grouped_ = st.grouped()
+        for s_ in srt:
+            s_.close()
+        if not grouped_ or not rows:
+            return None
+        us = np.array([r[0] for r in rows])
+        spk = np.array([r[1] for r in rows], np.float64)           # [steps][probes]
+        ps = np.array([r[2] for r in rows], np.float64)
+        grid = rows[0][3]
+        per_launch = float((spk * np.array(bps)).sum(1).mean())
+        achieved = per_launch / (float(us.mean()) * 1e-6) / 1e9
+        parts = ", ".join(f"{C}/{T}: {b_ / 1e3:.1f} KB x {m_:.1f}" for (C, T), b_, m_ in
+                          zip(STREAM_PROBES, bps, spk.mean(0)))
+        return {"kernel": f"pursuit_persistent_group_kernel (the four probes' persistent pursuits in one launch, "
+                          f"{sum(grid)} workgroups: {'/'.join(str(g_) for g_ in grid)})",
+                "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": read_traffic(STREAM_GROUP_TRAFFIC_FILE),
+                "launch_us": round(float(us.mean()), 2),
+                "us_per_pass": round(float(us.mean()) / float(ps[:, 0].mean()), 2),
+                "per_launch": f"per probe (C/T) bytes per spike applied (12 T (2 nt + 1): ctc rows read, score rows "
+                              f"read and written) x spikes per launch: {parts}; {per_launch / 1e6:.2f} MB per launch, "
+                              f"the 384/600 probe over {ps[:, 0].mean():.1f} passes; avg {us.mean():.1f} us per launch "
+                              f"from the workgroups' GPU-clock stamps over {len(rows)} steps of the timed region "
+                              f"(measurement mode: the pursuit tail launched without the graph); each pass is a "
+                              f"dependent step ending in a grid barrier, so the kernel is bound by the pass "
+                              f"latency, not by HBM"}
+
     def update_roofline(k0, nwin):
         # the pursuit of the 384-ch / 600-template probe (the step's critical path) in measurement
         # mode over windows k0 .. k0 + nwin - 1, one at a time, on the schedule the step uses: the
@@ -295,8 +352,9 @@ def bench_stream(args):
 
     if args.roofline_only:
         # counter runs: only the measured pursuit launches (with the schedule pinned or tuned)
-        r, u = update_roofline(args.warmup, 50)
-        return {"roofline": r, "roofline_update_kernel": u, "tuning": tuning}
+        g = group_roofline(args.warmup, 50)
+        r, u = update_roofline(args.warmup, 50) if g is None else (None, None)
+        return {"roofline": g or r, "roofline_update_kernel": u, "tuning": tuning}
 
     # plain per-probe path: every batch's windows one probe at a time (device-resident windows)
     for i, (s_, p) in enumerate(zip(plain_sorters, probes)):     # first-window setup, untimed
@@ -363,15 +421,6 @@ def bench_stream(args):
     # the measured run: recordings resident in HBM, windows assembled device to device
     lat, outs, refines, t_k, t_all, n_ahead, grouped = run_stream(sorters, True)
 
-    def fresh_sorters():
-        # new handles (the measured run refined the templates of its own) with the same tuning
-        out = [mk(p, args.precision) for p in probes]
-        for s2, tu in zip(out, tuning):
-            s2.set_tiles(*tu["tiles"])
-            s2.set_persistent(tu["persistent_grid"], tu["persistent_after"])
-            _l().lss_set_tuning(s2._h, tu["update_grid"], tu["first_passes"])
-            _l().lss_set_update_columns(s2._h, tu["update_columns"])
-        return out
     # live pacing (no backlog, so no running ahead): the per-batch latency of a live stream
     live_sorters = fresh_sorters()
     lat_live, outs_live, _, t_kl, _, _, _ = run_stream(live_sorters, True, run_ahead=False)
@@ -402,7 +451,9 @@ def bench_stream(args):
     n_ref_timed = sum(1 for k in range(warm, warm + K) if refine_every and (k + 1) >= refine_every
                       and ((k + 1) % refine_every) < P)
 
-    roofline, roofline_update = update_roofline(warm, min(K, 50))
+    roofline, roofline_update = group_roofline(warm, min(K, 50)), None
+    if roofline is None:
+        roofline, roofline_update = update_roofline(warm, min(K, 50))
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -312,6 +312,14 @@ lss_status lss_stream_ran_ahead(void* stream, int64_t* steps);
  * counterpart in the reference (its loop sorts one probe per process, V2:632-820). */
 lss_status lss_stream_set_grouped(void* stream, int32_t on);
 lss_status lss_stream_grouped(void* stream, int32_t* grouped);
+/* Measurement mode of grouped steps (off by default): each step launches its pursuit tail without
+ * the graph, the grouped persistent pursuit's workgroups stamp their GPU-clock start and end, and
+ * lss_stream_last_profile gives, for the last step, that launch's span in microseconds (last
+ * workgroup end - first workgroup start) and per probe (arrays of n_probes) the spikes found, the
+ * passes run and the workgroups the probe took.  Results are the same.  LSS_ERR_STATE when
+ * measurement is off or the steps do not run grouped. */
+lss_status lss_stream_profile(void* stream, int32_t on);
+lss_status lss_stream_last_profile(void* stream, double* gpu_us, int32_t* spikes, int32_t* passes, int32_t* grid);
 
 /* ---- template refinement (no reference counterpart) ---------------------------------------------- */
 

@@ -846,6 +846,13 @@ struct Batcher {
     hipGraphExec_t xpre[4][kPreLanes] = {};
     hipGraph_t gtail[2] = {};
     hipGraphExec_t xtail[2] = {};
+    // measurement mode of grouped steps (lss_stream_profile): the tail launched directly with
+    // descriptors whose members stamp their workgroups' GPU-clock start and end into one buffer
+    // (member i from workgroup first[i] on); the last step's span of the grouped persistent launch
+    bool profile = false;
+    DevBuf<lss::GroupMember> gprof[2];
+    DevBuf<unsigned long long> pts, pspan;
+    double prof_us = 0.0;
     ~Batcher() {
         for (int k = 0; k < 4; ++k)
             for (int l = 0; l < kPreLanes; ++l) {
@@ -932,7 +939,7 @@ lss_status probe_C(Probe& p) {
 // counter resets, persistent pursuits, residuals, features, readbacks.
 lss_status segment_tail(Batcher& b) {
     const int n = (int)b.pr.size();
-    const lss::GroupMember* d = b.gdev[b.swapped].p;
+    const lss::GroupMember* d = b.profile ? b.gprof[b.swapped].p : b.gdev[b.swapped].p;
     const lss::GroupMember* m = b.ghost[b.swapped].data();
     HIP_TRY(lss::launch_group_reset(d, n, b.main));
     HIP_TRY(lss::launch_group_persistent(d, m, b.ggrid, b.gmp, n, b.main));
@@ -1004,7 +1011,7 @@ lss_status grouped_step(Batcher& b) {
             HIP_TRY(hipEventRecord(b.lane_ev[l], b.lane_s[l]));
             HIP_TRY(hipStreamWaitEvent(b.main, b.lane_ev[l], 0));
         }
-    if (b.use_graph) {
+    if (b.use_graph && !b.profile) {
         if (!b.xtail[b.swapped])
             if (lss_status st = capture_on(b.main, [&] { return segment_tail(b); }, &b.gtail[b.swapped],
                                            &b.xtail[b.swapped]))
@@ -1020,6 +1027,15 @@ lss_status grouped_step(Batcher& b) {
         }
     if (b.next_pre)
         for (int l = 0; l < b.nlanes; ++l) HIP_TRY(hipEventSynchronize(b.lane_ev2[l]));
+    if (b.profile) {
+        int total = 0;
+        for (size_t i = 0; i < b.pr.size(); ++i) total += b.ggrid[i];
+        unsigned long long span[2] = {0, 0};
+        HIP_TRY(lss::launch_stamp_span(b.pts.p, 1, total, b.pspan.p, b.main));
+        HIP_TRY(hipMemcpyAsync(span, b.pspan.p, sizeof(span), hipMemcpyDeviceToHost, b.main));
+        HIP_TRY(hipStreamSynchronize(b.main));
+        b.prof_us = span[1] > span[0] ? (double)(span[1] - span[0]) * 1e-2 : 0.0;   // 100 MHz clock
+    }
     HIP_TRY(hipStreamSynchronize(b.main));
     return LSS_OK;
 }
@@ -1100,6 +1116,18 @@ lss_status group_members(Batcher& b) {
     DevBuf<lss::GroupMember>& d = b.gdev[b.swapped];
     HIP_TRY(d.alloc(n));
     HIP_TRY(hipMemcpy(d.p, v.data(), sizeof(lss::GroupMember) * n, hipMemcpyHostToDevice));
+    // the same members stamping their workgroups (measurement mode)
+    int total = 0;
+    for (int i = 0; i < n; ++i) total += b.ggrid[i];
+    if (!b.pts.p) {
+        HIP_TRY(b.pts.alloc(2 * (size_t)total));
+        HIP_TRY(b.pspan.alloc(2));
+    }
+    std::vector<lss::GroupMember> w = v;
+    for (int i = 0, g0 = 0; i < n; g0 += b.ggrid[i], ++i) w[i].a.ts = b.pts.p + 2 * (size_t)g0;
+    DevBuf<lss::GroupMember>& dp = b.gprof[b.swapped];
+    HIP_TRY(dp.alloc(n));
+    HIP_TRY(hipMemcpy(dp.p, w.data(), sizeof(lss::GroupMember) * n, hipMemcpyHostToDevice));
     m = std::move(v);
     return LSS_OK;
 }
@@ -2457,6 +2485,27 @@ lss_status lss_stream_set_grouped(void* stream, int32_t on) {
 lss_status lss_stream_grouped(void* stream, int32_t* grouped) {
     if (!stream || !grouped) return fail(LSS_ERR_ARG, "null argument");
     *grouped = static_cast<Batcher*>(stream)->group == 1 ? 1 : 0;
+    return LSS_OK;
+}
+
+lss_status lss_stream_profile(void* stream, int32_t on) {
+    if (!stream) return fail(LSS_ERR_ARG, "null stream");
+    static_cast<Batcher*>(stream)->profile = on != 0;
+    return LSS_OK;
+}
+
+lss_status lss_stream_last_profile(void* stream, double* gpu_us, int32_t* spikes, int32_t* passes, int32_t* grid) {
+    if (!stream || !gpu_us || !spikes || !passes || !grid) return fail(LSS_ERR_ARG, "null argument");
+    Batcher& b = *static_cast<Batcher*>(stream);
+    if (!b.profile) return fail(LSS_ERR_STATE, "stream measurement is off (lss_stream_profile)");
+    if (b.group != 1) return fail(LSS_ERR_STATE, "the stream's steps do not run grouped");
+    *gpu_us = b.prof_us;
+    for (size_t i = 0; i < b.pr.size(); ++i) {
+        const int* c = b.pr[i].h->hs_counters;
+        spikes[i] = c[0] + c[1];
+        passes[i] = c[4];
+        grid[i] = b.ggrid[i];
+    }
     return LSS_OK;
 }
 

@@ -91,6 +91,19 @@ class StreamingSorter:
         check(lib().lss_stream_grouped(self._h, ct.byref(g)), "lss_stream_grouped")
         return bool(g.value)
 
+    def profile(self, on: bool = True):
+        """Measurement mode of grouped steps (lss_stream_profile)."""
+        check(lib().lss_stream_profile(self._h, int(bool(on))), "lss_stream_profile")
+
+    def last_profile(self):
+        """(GPU-clock span of the last step's grouped persistent pursuit in us, [spikes], [passes],
+        [workgroups] per probe) (lss_stream_last_profile)."""
+        n = len(self.sorters)
+        us = ct.c_double()
+        sp, ps, gr = (ct.c_int32 * n)(), (ct.c_int32 * n)(), (ct.c_int32 * n)()
+        check(lib().lss_stream_last_profile(self._h, ct.byref(us), sp, ps, gr), "lss_stream_last_profile")
+        return us.value, list(sp), list(ps), list(gr)
+
     def held(self, probe: int) -> int:
         """Device tensors of a probe still referenced for the batcher."""
         return len(self._held.get(probe, []))

@@ -470,3 +470,40 @@ def test_grouping_not_used_with_per_pass_rounds_and_set_before_the_first_step(pr
             check(lib().lss_stream_set_grouped(st._h, 0), "lss_stream_set_grouped")
     for s in sa + sb:
         s.close()
+
+
+def test_grouped_measurement_mode(probes):
+    """Measurement mode of grouped steps (tail launched without the graph, workgroups stamping the
+    GPU clock) returns the same spikes, and reports per step a positive span of the grouped
+    persistent pursuit, each probe's spike count as returned, its passes and its workgroups."""
+    print("[lss] test_grouped_measurement_mode", flush=True)
+    from livespikesorter_amd._lib import LssError
+    from livespikesorter_amd.streaming import StreamingSorter
+    sa, sb = _make_sorters(probes, precision="bf16x3"), _make_sorters(probes, precision="bf16x3")
+    ref, _ = _stream_all(probes, sb, True, True)
+    src = [torch.from_numpy(p.rec).cuda() for p in probes]
+    got = [[] for _ in probes]
+    with StreamingSorter(sa, batch=BATCH) as st:
+        with pytest.raises(LssError, match="measurement is off"):
+            st.last_profile()
+        st.profile(True)
+        for i in range(len(probes)):
+            st.push(i, src[i])
+        k = 0
+        while True:
+            r = st.step()
+            if r is None:
+                break
+            us, spikes, passes, grid = st.last_profile()
+            assert us > 0.0
+            n_out = [len(x.sample) for x in r]
+            # the first batch's window starts before the recording: spikes there may be dropped
+            assert spikes == n_out if k > 0 else all(a >= b for a, b in zip(spikes, n_out))
+            k += 1
+            assert all(p_ >= 1 for p_ in passes) and all(g > 0 for g in grid)
+            for i in range(len(probes)):
+                got[i].append(r[i])
+        assert st.grouped()
+    _assert_same(got, ref)
+    for s in sa + sb:
+        s.close()
