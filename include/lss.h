@@ -272,6 +272,9 @@ lss_status lss_stream_push(void* stream, int32_t probe, const int16_t* samples, 
  * buffer valid until every step that reads its samples has returned.  A probe takes either host
  * or device samples. */
 lss_status lss_stream_push_device(void* stream, int32_t probe, const int16_t* d_samples, int64_t n);
+/* The batcher's streams are blocking streams: a step's copies of device samples follow work
+ * submitted earlier to the legacy default stream; samples written on any other stream must be
+ * complete (that stream synchronised) before the step that reads them. */
 /* Runs one batch when every probe holds enough samples (*ready = 1) and fills outs[0..n_probes);
  * otherwise *ready = 0 and nothing runs. */
 lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready);
@@ -319,6 +322,15 @@ lss_status lss_stream_grouped(void* stream, int32_t* grouped);
  * passes run and the workgroups the probe took.  Results are the same.  LSS_ERR_STATE when
  * measurement is off or the steps do not run grouped. */
 lss_status lss_stream_profile(void* stream, int32_t on);
+/* Pipelined steps (on by default; grouped steps running ahead, not in measurement mode): a step that
+ * preprocessed its next batch also launches that batch's pursuit before it returns, so the device
+ * works on it while the caller handles this step's spikes; the next step then only waits for it.
+ * The next step sorts its batch again instead (same results) when, in between, any entry point took
+ * one of the stream's handles (each such call first waits for the launch) or an input of its
+ * preprocessing changed; a step that starts a refinement or precedes an adoption launches nothing
+ * early.  lss_stream_pipelined counts the early launches.  No counterpart in the reference. */
+lss_status lss_stream_set_pipelined(void* stream, int32_t on);
+lss_status lss_stream_pipelined(void* stream, int64_t* steps);
 lss_status lss_stream_last_profile(void* stream, double* gpu_us, int32_t* spikes, int32_t* passes, int32_t* grid);
 
 /* ---- template refinement (no reference counterpart) ---------------------------------------------- */

@@ -28,6 +28,7 @@ EXPORTS = ["lss_last_error", "lss_abi_version", "lss_default_params", "lss_creat
            "lss_residual_energy", "lss_residual_energy_grad", "lss_last_refine_timings",
            "lss_refine_templates", "lss_stream_set_refine", "lss_stream_set_run_ahead", "lss_stream_ran_ahead",
            "lss_stream_set_grouped", "lss_stream_grouped", "lss_stream_profile", "lss_stream_last_profile",
+           "lss_stream_set_pipelined", "lss_stream_pipelined",
            "lss_stream_last_refine",
            "lss_stream_push_device",
            "lss_set_drift", "lss_covariance_add", "lss_whitening_from_covariance", "lss_whitening_local",
@@ -132,6 +133,8 @@ def lib() -> ct.CDLL:
     "lss_stream_set_grouped": ([vp, i32], i32),
     "lss_stream_grouped": ([vp, ct.POINTER(i32)], i32),
     "lss_stream_profile": ([vp, i32], i32),
+    "lss_stream_set_pipelined": ([vp, i32], i32),
+    "lss_stream_pipelined": ([vp, ct.POINTER(i64)], i32),
     "lss_stream_last_profile": ([vp, ct.POINTER(ct.c_double), ct.POINTER(i32), ct.POINTER(i32), ct.POINTER(i32)], i32),
             "lss_stream_last_refine": ([vp, i32, vp], i32),
             "lss_set_drift": ([vp, vp], i32),

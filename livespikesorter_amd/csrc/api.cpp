@@ -101,6 +101,11 @@ struct Handle {
     DevBuf<int16_t> win_alt;
     DevBuf<float> xw_alt, B_alt;
     uint64_t pre_version = 0;    // advanced when an input of the preprocessing changes (drift, templates)
+    // a stream's pursuit launched ahead of its step (pipelined steps) that uses this handle: every
+    // entry point that takes the handle first waits for it and marks the handle touched, which makes
+    // that step discard the launch and sort its batch again
+    hipEvent_t pending = nullptr;
+    bool touched = false;
     DevBuf<int> imax;
     DevBuf<unsigned char> flags;
     DevBuf<int> sp_col, sp_tmp, counters, pass_off;
@@ -427,7 +432,15 @@ lss_status create_impl(const lss_params* p, const lss_artifacts* a, Handle** out
     return LSS_OK;
 }
 
-Handle* as_handle(void* p) { return static_cast<Handle*>(p); }
+Handle* as_handle(void* p) {
+    Handle* h = static_cast<Handle*>(p);
+    if (h && h->pending) {
+        (void)hipEventSynchronize(h->pending);
+        h->pending = nullptr;
+        h->touched = true;
+    }
+    return h;
+}
 
 }  // namespace
 
@@ -850,10 +863,20 @@ struct Batcher {
     // descriptors whose members stamp their workgroups' GPU-clock start and end into one buffer
     // (member i from workgroup first[i] on); the last step's span of the grouped persistent launch
     bool profile = false;
+    // pipelined steps (lss_stream_set_pipelined): a step whose next batch it has preprocessed also
+    // launches that batch's pursuit tail before it returns (early); the next step uses it unless a
+    // handle was touched or its preprocessing inputs changed in between (tail_inflight: it does)
+    bool pipelined = true;
+    bool early = false, tail_inflight = false;
+    hipEvent_t early_ev = nullptr;
+    int64_t pipelined_steps = 0;
     DevBuf<lss::GroupMember> gprof[2];
     DevBuf<unsigned long long> pts, pspan;
     double prof_us = 0.0;
     ~Batcher() {
+        for (auto& p : pr)
+            if (p.h && early_ev && p.h->pending == early_ev) p.h->pending = nullptr;
+        if (early_ev) (void)hipEventDestroy(early_ev);
         for (int k = 0; k < 4; ++k)
             for (int l = 0; l < kPreLanes; ++l) {
                 if (xpre[k][l]) (void)hipGraphExecDestroy(xpre[k][l]);
@@ -996,13 +1019,28 @@ lss_status run_lane(Batcher& b, int l, bool alt) {
     return LSS_OK;
 }
 
+// The pursuit tail of a grouped step on the main stream (its graph, or direct launches).
+lss_status launch_tail(Batcher& b) {
+    if (b.use_graph && !b.profile) {
+        if (!b.xtail[b.swapped])
+            if (lss_status st = capture_on(b.main, [&] { return segment_tail(b); }, &b.gtail[b.swapped],
+                                           &b.xtail[b.swapped]))
+                return st;
+        HIP_TRY(hipGraphLaunch(b.xtail[b.swapped], b.main));
+        return LSS_OK;
+    }
+    return segment_tail(b);
+}
+
 // The device work of a grouped step.  The lanes preprocess this batch when the last step did not
 // (the tail waits for them), the tail runs on the main stream, and the lanes preprocess the next
 // batch beside it (running ahead).  Every lane starts after the main stream's work so far (the
 // gathered windows, templates adopted at the step's start); on a lane the next batch's
 // preprocessing follows this batch's, which shares the probes' intermediate buffers.
 lss_status grouped_step(Batcher& b) {
-    HIP_TRY(hipEventRecord(b.staged, b.main));
+    // with this batch's tail already in flight on the main stream the windows were gathered on lane
+    // 0 (nothing else on the main stream precedes this step's preprocessing then)
+    HIP_TRY(hipEventRecord(b.staged, b.tail_inflight ? b.lane_s[0] : b.main));
     for (int l = 0; l < b.nlanes; ++l)
         if (b.cur_pre || b.next_pre) HIP_TRY(hipStreamWaitEvent(b.lane_s[l], b.staged, 0));
     if (b.cur_pre)
@@ -1011,15 +1049,8 @@ lss_status grouped_step(Batcher& b) {
             HIP_TRY(hipEventRecord(b.lane_ev[l], b.lane_s[l]));
             HIP_TRY(hipStreamWaitEvent(b.main, b.lane_ev[l], 0));
         }
-    if (b.use_graph && !b.profile) {
-        if (!b.xtail[b.swapped])
-            if (lss_status st = capture_on(b.main, [&] { return segment_tail(b); }, &b.gtail[b.swapped],
-                                           &b.xtail[b.swapped]))
-                return st;
-        HIP_TRY(hipGraphLaunch(b.xtail[b.swapped], b.main));
-    } else if (lss_status st = segment_tail(b)) {
-        return st;
-    }
+    if (!b.tail_inflight)
+        if (lss_status st = launch_tail(b)) return st;
     if (b.next_pre)
         for (int l = 0; l < b.nlanes; ++l) {
             if (lss_status st = run_lane(b, l, true)) return st;
@@ -1027,7 +1058,7 @@ lss_status grouped_step(Batcher& b) {
         }
     if (b.next_pre)
         for (int l = 0; l < b.nlanes; ++l) HIP_TRY(hipEventSynchronize(b.lane_ev2[l]));
-    if (b.profile) {
+    if (b.profile && !b.tail_inflight) {
         int total = 0;
         for (size_t i = 0; i < b.pr.size(); ++i) total += b.ggrid[i];
         unsigned long long span[2] = {0, 0};
@@ -1041,7 +1072,10 @@ lss_status grouped_step(Batcher& b) {
 }
 
 // A stream with a hardware queue of its own: a stream with a compute-unit mask (here every CU of
-// the device) gets a queue that the runtime's shared pool never hands to another stream.
+// the device) gets a queue that the runtime's shared pool never hands to another stream.  The
+// batcher's streams are blocking streams (ordered after earlier work of the legacy default stream,
+// which device samples pushed from torch's default stream rely on); should the runtime make this
+// one non-blocking, an ordinary blocking stream on the shared pool replaces it.
 lss_status own_queue_stream(hipStream_t* s) {
     int dev = 0, cus = 0;
     HIP_TRY(hipGetDevice(&dev));
@@ -1049,6 +1083,12 @@ lss_status own_queue_stream(hipStream_t* s) {
     std::vector<uint32_t> mask((cus + 31) / 32, 0xFFFFFFFFu);
     if (cus % 32) mask.back() = (1u << (cus % 32)) - 1u;
     HIP_TRY(hipExtStreamCreateWithCUMask(s, (uint32_t)mask.size(), mask.data()));
+    unsigned flags = 0;
+    HIP_TRY(hipStreamGetFlags(*s, &flags));
+    if (flags & hipStreamNonBlocking) {
+        HIP_TRY(hipStreamDestroy(*s));
+        HIP_TRY(hipStreamCreate(s));
+    }
     return LSS_OK;
 }
 
@@ -2267,6 +2307,18 @@ lss_status lss_stream_push_device(void* stream, int32_t probe, const int16_t* d_
     return LSS_OK;
 }
 
+// The window sets preprocessed ahead become the current ones (and the current ones the second).
+void swap_sets(Batcher& b) {
+    for (auto& p : b.pr) {
+        Handle& h = *p.h;
+        std::swap(h.win.p, h.win_alt.p);
+        std::swap(h.xw.p, h.xw_alt.p);
+        std::swap(h.B.p, h.B_alt.p);
+        std::swap(p.pin, p.pin_alt);
+    }
+    b.swapped ^= 1;
+}
+
 lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready) {
     if (!stream || !outs || !ready) return fail(LSS_ERR_ARG, "null argument");
     Batcher& b = *static_cast<Batcher*>(stream);
@@ -2285,16 +2337,21 @@ lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready) {
     bool have = b.ahead;
     for (auto& p : b.pr)
         if (p.h->pre_version != p.ahead_version) have = false;
-    if (have) {
-        for (auto& p : b.pr) {
-            Handle& h = *p.h;
-            std::swap(h.win.p, h.win_alt.p);
-            std::swap(h.xw.p, h.xw_alt.p);
-            std::swap(h.B.p, h.B_alt.p);
-            std::swap(p.pin, p.pin_alt);
-        }
-        b.swapped ^= 1;
+    b.tail_inflight = false;
+    if (b.early) {
+        // the last step launched this batch's tail (sets already swapped); it stands unless a handle
+        // was used or an input of the preprocessing changed since
+        bool ok = true;
+        for (auto& p : b.pr)
+            if (p.h->touched || p.h->pre_version != p.ahead_version) ok = false;
+        if (!ok) HIP_TRY(hipStreamSynchronize(b.main));
+        b.tail_inflight = ok;
+        have = ok;
+        b.early = false;
+    } else if (have) {
+        swap_sets(b);
     }
+    for (auto& p : b.pr) p.h->touched = false;
     b.ahead = false;
     // the next batch ahead: one-graph steps, its samples here, and no refinement adopted at the next
     // step's start (that changes the templates its scores need)
@@ -2317,7 +2374,8 @@ lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready) {
             else if (lss_status st = assemble_dev(p, t, segs, k == 0 ? h.win.p : h.win_alt.p)) return st;
         }
     }
-    if (!segs.empty()) HIP_TRY(lss::launch_gather(segs.data(), (int)segs.size(), b.main));
+    if (!segs.empty())
+        HIP_TRY(lss::launch_gather(segs.data(), (int)segs.size(), b.tail_inflight ? b.lane_s[0] : b.main));
     if (b.single) {
         if (lss_status st = group_decide(b)) return st;
         if (b.group == 1)
@@ -2334,6 +2392,8 @@ lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready) {
     // segment A
     if (b.group == 1) {
         if (lss_status st = grouped_step(b)) return st;
+        for (auto& p : b.pr)
+            if (b.early_ev && p.h->pending == b.early_ev) p.h->pending = nullptr;
     } else if (b.use_graph && b.single) {
         const int key = (b.cur_pre ? 1 : 0) | (b.swapped << 1);
         if (!b.execs[key])
@@ -2415,6 +2475,7 @@ lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready) {
         for (auto& p : b.pr) p.ahead_version = p.h->pre_version;
     }
     ++b.batches;
+    bool refined = false;
     if (b.refine_every > 0) {
         // a round starts every refine_every steps (one still under way carries on); in a round,
         // step i refines probe i, so the step that refines pays for one probe's refinement only
@@ -2423,6 +2484,7 @@ lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready) {
             const int k = b.refine_next;
             b.refine_info.resize(b.pr.size());
             if (lss_status st = refine_start(b, b.pr[k])) return st;
+            refined = true;
             b.refine_next = (k + 1 < (int)b.pr.size()) ? k + 1 : -1;
         }
     }
@@ -2442,6 +2504,25 @@ lss_status lss_stream_step(void* stream, lss_spikes* outs, int32_t* ready) {
         } else if (drop > 0) {
             p.buf.erase(p.buf.begin(), p.buf.begin() + drop * p.h->C);
             p.base = keep_from;
+        }
+    }
+    // pipelined: the next batch, preprocessed by this step, starts its pursuit now (not after a step
+    // that started a refinement, whose snapshot the next preprocessing must follow, nor before a step
+    // that adopts one)
+    if (b.pipelined && b.group == 1 && b.ahead && !b.profile && !refined) {
+        bool ok = true;
+        for (auto& p : b.pr)
+            if (p.rjob_adopt >= 0 && p.rjob_adopt <= b.batches) ok = false;
+        if (ok) {
+            swap_sets(b);
+            b.ahead = false;
+            if (lss_status st = group_members(b)) return st;
+            if (lss_status st = launch_tail(b)) return st;
+            if (!b.early_ev) HIP_TRY(hipEventCreateWithFlags(&b.early_ev, hipEventDisableTiming));
+            HIP_TRY(hipEventRecord(b.early_ev, b.main));
+            for (auto& p : b.pr) p.h->pending = b.early_ev;
+            b.early = true;
+            ++b.pipelined_steps;
         }
     }
     *ready = 1;
@@ -2506,6 +2587,18 @@ lss_status lss_stream_last_profile(void* stream, double* gpu_us, int32_t* spikes
         passes[i] = c[4];
         grid[i] = b.ggrid[i];
     }
+    return LSS_OK;
+}
+
+lss_status lss_stream_set_pipelined(void* stream, int32_t on) {
+    if (!stream) return fail(LSS_ERR_ARG, "null stream");
+    static_cast<Batcher*>(stream)->pipelined = on != 0;
+    return LSS_OK;
+}
+
+lss_status lss_stream_pipelined(void* stream, int64_t* steps) {
+    if (!stream || !steps) return fail(LSS_ERR_ARG, "null argument");
+    *steps = static_cast<Batcher*>(stream)->pipelined_steps;
     return LSS_OK;
 }
 

@@ -14,12 +14,14 @@ from .sorter import Sorter, alloc_spikes
 
 class StreamingSorter:
     def __init__(self, sorters: list[Sorter], batch: int = 3000, use_graph: bool = True, refine_every: int = 0,
-                 run_ahead: bool = True, grouped: bool = True):
+                 run_ahead: bool = True, grouped: bool = True, pipelined: bool = True):
         """refine_every: one template-refinement step per probe every this many batches (drift
         tracking, lss_refine_templates on that batch's window); 0 = never.  run_ahead: a step whose
         next batch is already pushed preprocesses it beside its own pursuit (lss_stream_set_run_ahead;
         same results).  grouped: all probes' pursuit tails as one launch per stage when every probe's
-        pursuit is persistent (lss_stream_set_grouped; same results)."""
+        pursuit is persistent (lss_stream_set_grouped; same results).  pipelined: a grouped step
+        that preprocessed its next batch launches that batch's pursuit before it returns
+        (lss_stream_set_pipelined; same results)."""
         self.sorters = list(sorters)
         arr = (ct.c_void_p * len(self.sorters))(*[s._h.value for s in self.sorters])
         h = ct.c_void_p()
@@ -36,6 +38,7 @@ class StreamingSorter:
             check(lib().lss_stream_set_refine(h, int(refine_every)), "lss_stream_set_refine")
         check(lib().lss_stream_set_run_ahead(h, int(bool(run_ahead))), "lss_stream_set_run_ahead")
         check(lib().lss_stream_set_grouped(h, int(bool(grouped))), "lss_stream_set_grouped")
+        check(lib().lss_stream_set_pipelined(h, int(bool(pipelined))), "lss_stream_set_pipelined")
         arr_out = (Spikes * len(self.sorters))()
         self._bufs = []
         for i, s in enumerate(self.sorters):
@@ -52,7 +55,12 @@ class StreamingSorter:
             if not samples.is_contiguous():
                 raise ValueError("device samples must be contiguous")
             import torch
-            torch.cuda.current_stream().synchronize()   # the samples are complete before a step copies them
+            cs = torch.cuda.current_stream()
+            # the samples are complete before a step copies them: the batcher's streams are blocking
+            # streams, ordered after the work of torch's default (legacy) stream; a stream of the
+            # caller's is waited for here
+            if cs.cuda_stream != 0:
+                cs.synchronize()
             n = int(samples.shape[0])
             check(lib().lss_stream_push_device(self._h, int(probe), samples.data_ptr(), n),
                   "lss_stream_push_device")
@@ -83,6 +91,12 @@ class StreamingSorter:
         """Steps so far that preprocessed their next batch (lss_stream_ran_ahead)."""
         n = ct.c_int64()
         check(lib().lss_stream_ran_ahead(self._h, ct.byref(n)), "lss_stream_ran_ahead")
+        return n.value
+
+    def pipelined(self) -> int:
+        """Steps so far that launched their next batch's pursuit early (lss_stream_pipelined)."""
+        n = ct.c_int64()
+        check(lib().lss_stream_pipelined(self._h, ct.byref(n)), "lss_stream_pipelined")
         return n.value
 
     def grouped(self) -> bool:

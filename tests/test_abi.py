@@ -105,6 +105,8 @@ def test_argument_checks_before_any_device_work():
     assert L.lss_stream_set_grouped(None, 1) == 1
     assert L.lss_stream_grouped(None, None) == 1
     assert L.lss_stream_profile(None, 1) == 1
+    assert L.lss_stream_set_pipelined(None, 1) == 1
+    assert L.lss_stream_pipelined(None, None) == 1
     assert L.lss_stream_last_profile(None, None, None, None, None) == 1
     assert L.lss_stream_set_refine(None, 10) == 1
     assert L.lss_stream_last_refine(None, 0, None) == 1

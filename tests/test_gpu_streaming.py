@@ -281,7 +281,7 @@ def test_joint_schedule_tuning_is_result_neutral(probes):
 
 
 def _stream_all(probes, sorters, run_ahead, device, refine_every=0, between=None, grouped=True, flags=None,
-                use_graph=True):
+                use_graph=True, pipelined=True, counts=None):
     """Every sample pushed at once (so each step finds its next batch), all batches stepped;
     between(k) runs after step k.  Returns (outputs per probe, steps that ran ahead); flags, when
     given, gets whether the steps ran grouped."""
@@ -289,7 +289,7 @@ def _stream_all(probes, sorters, run_ahead, device, refine_every=0, between=None
     out = [[] for _ in probes]
     src = [torch.from_numpy(p.rec).cuda() for p in probes] if device else [p.rec for p in probes]
     with StreamingSorter(sorters, batch=BATCH, use_graph=use_graph, refine_every=refine_every,
-                         run_ahead=run_ahead, grouped=grouped) as st:
+                         run_ahead=run_ahead, grouped=grouped, pipelined=pipelined) as st:
         for i in range(len(probes)):
             st.push(i, src[i])
         k = 0
@@ -305,6 +305,8 @@ def _stream_all(probes, sorters, run_ahead, device, refine_every=0, between=None
         ahead = st.ran_ahead()
         if flags is not None:
             flags.append(st.grouped())
+        if counts is not None:
+            counts.append(st.pipelined())
     return out, ahead
 
 
@@ -504,6 +506,47 @@ def test_grouped_measurement_mode(probes):
             for i in range(len(probes)):
                 got[i].append(r[i])
         assert st.grouped()
+    _assert_same(got, ref)
+    for s in sa + sb:
+        s.close()
+
+
+@pytest.mark.parametrize("device", [True, False])
+def test_pipelined_steps_equal_step_by_step(probes, device):
+    """Steps that launch their next batch's pursuit before returning (pipelined) return exactly what
+    steps without it return, with background refinement on (no early launch around a refinement)."""
+    print(f"[lss] test_pipelined_steps_equal_step_by_step device={device}", flush=True)
+    sa, sb = _make_sorters(probes, precision="bf16x3"), _make_sorters(probes, precision="bf16x3")
+    ca, cb = [], []
+    got, _ = _stream_all(probes, sa, True, device, refine_every=3, pipelined=True, counts=ca)
+    ref, _ = _stream_all(probes, sb, True, device, refine_every=3, pipelined=False, counts=cb)
+    print(f"[lss]   {len(got[0])} steps, {ca[0]} launched early", flush=True)
+    assert ca[0] >= 1 and cb[0] == 0
+    _assert_same(got, ref)
+    for s in sa + sb:
+        s.close()
+
+
+def test_pipelined_step_redone_after_a_handle_call(probes):
+    """A call that takes one of the stream's handles between two steps (here sorting other windows
+    on it, which overwrites the handle's window, scores and spike lists) waits for the pursuit
+    launched early and makes the next step sort its batch again: the stream returns what it returns
+    without the calls."""
+    print("[lss] test_pipelined_step_redone_after_a_handle_call", flush=True)
+    sa, sb = _make_sorters(probes), _make_sorters(probes)
+    rng = np.random.default_rng(5)
+
+    def between(k):
+        i = k % len(probes)
+        p = probes[i]
+        t = int(rng.integers(0, p.rec.shape[0] - BATCH - 200))
+        w = np.ascontiguousarray(p.rec[t:t + BATCH + 2 * p.art["nt"]])
+        sa[i].sort_window(w, t)
+
+    ca = []
+    got, _ = _stream_all(probes, sa, True, True, between=between, counts=ca)
+    ref, _ = _stream_all(probes, sb, True, True, pipelined=False)
+    assert ca[0] >= 1
     _assert_same(got, ref)
     for s in sa + sb:
         s.close()
