@@ -1260,14 +1260,19 @@ constexpr int kResCols = 16;
 constexpr int kResCap = 1024;       // listed spikes per workgroup and run of passes
 constexpr int kResMaxPasses = 256;
 constexpr int kResBatch = 8;        // wave-row loads in flight per column
-constexpr int kResThreads = 512;    // threads per workgroup (its wavefronts share the columns)
+// threads per workgroup (its wavefronts share the 16 columns): 512 for short windows (two columns per
+// wavefront; a 3122-sample window's residual is then latency bound on few workgroups), 256 for long
+// ones (more workgroups of their own list building)
+constexpr int kResThreadsShort = 512, kResThreadsLong = 256;
 
-template <int NVC>   // float4 channel chunks per lane: NVC*256 >= C; blk: the workgroup's 16 columns
+// NVC: float4 channel chunks per lane (NVC*256 >= C); NT: threads per workgroup; blk: the workgroup's
+// 16 columns
+template <int NVC, int NT>
 __device__ __forceinline__ void residual_body(const PursuitArgs& a, int blk) {
     __shared__ int s_first[kResMaxPasses], s_cnt[kResMaxPasses], s_off[kResMaxPasses + 1];
     __shared__ int l_col[kResCap], l_tmp[kResCap], l_pp[kResCap];   // l_pp = pass * 2 + parity
     __shared__ float l_amp[kResCap];
-    __shared__ int l_apply[kResThreads / 64][2 * kResMaxPasses];     // per wavefront
+    __shared__ int l_apply[NT / 64][2 * kResMaxPasses];              // per wavefront
     __shared__ int s_run[2];                                         // passes [s_run[0], s_run[1])
     if (a.counters[2]) return;                  // spike list overflowed: the call reports an error
     const int P = a.counters[4];                // passes that found spikes
@@ -1275,7 +1280,7 @@ __device__ __forceinline__ void residual_body(const PursuitArgs& a, int blk) {
     const int h = a.nt / 2;
     const int64_t w0 = (int64_t)blk * kResCols;
     const int lo = (int)(w0 - h), hi = (int)(w0 + kResCols - 1 + h);
-    for (int p = tid; p < P; p += kResThreads) {
+    for (int p = tid; p < P; p += NT) {
         const int b0 = a.pass_off[p], b1 = a.pass_off[p + 1];
         int l = b0, r = b1;                     // first index with col >= lo
         while (l < r) {
@@ -1314,7 +1319,7 @@ __device__ __forceinline__ void residual_body(const PursuitArgs& a, int blk) {
         __syncthreads();
         const int p0 = s_run[0], p1 = s_run[1];
         const int total = min(s_off[p1], kResCap);
-        for (int p = p0 + tid; p < p1; p += kResThreads) {
+        for (int p = p0 + tid; p < p1; p += NT) {
             for (int k = 0; k < s_cnt[p]; ++k) {
                 const int d = s_off[p] + k;
                 if (d >= kResCap) break;
@@ -1326,7 +1331,7 @@ __device__ __forceinline__ void residual_body(const PursuitArgs& a, int blk) {
             }
         }
         __syncthreads();
-        for (int cw = wave; cw < kResCols; cw += kResThreads / 64) {
+        for (int cw = wave; cw < kResCols; cw += NT / 64) {
             const int64_t w = w0 + cw;
             if (w >= a.W) break;
             // 1) the ordered list of subtractions for this column: per pass, last even then last odd
@@ -1390,9 +1395,9 @@ __device__ __forceinline__ void residual_body(const PursuitArgs& a, int blk) {
     }
 }
 
-template <int NVC>
-__global__ __launch_bounds__(kResThreads) void residual_kernel(PursuitArgs a) {
-    residual_body<NVC>(a, (int)blockIdx.x);
+template <int NVC, int NT>
+__global__ __launch_bounds__(NT) void residual_kernel(PursuitArgs a) {
+    residual_body<NVC, NT>(a, (int)blockIdx.x);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1568,11 +1573,19 @@ hipError_t launch_pursuit_persistent(const PursuitArgs& a, int grid, int first_p
 hipError_t launch_residual(const PursuitArgs& a, hipStream_t s) {
     if (a.C > 1024) return hipErrorInvalidValue;
     const unsigned nb = (unsigned)((a.W + kResCols - 1) / kResCols);
+    const bool shrt = a.W <= kPersistMaxW;
+    const unsigned nt = shrt ? kResThreadsShort : kResThreadsLong;
+#define LSS_RES(V)                                                                          \
+    if (shrt)                                                                               \
+        hipLaunchKernelGGL((residual_kernel<V, kResThreadsShort>), dim3(nb), dim3(nt), 0, s, a); \
+    else                                                                                    \
+        hipLaunchKernelGGL((residual_kernel<V, kResThreadsLong>), dim3(nb), dim3(nt), 0, s, a)
     switch ((a.C + 255) / 256) {
-        case 1: hipLaunchKernelGGL(residual_kernel<1>, dim3(nb), dim3(kResThreads), 0, s, a); break;
-        case 2: hipLaunchKernelGGL(residual_kernel<2>, dim3(nb), dim3(kResThreads), 0, s, a); break;
-        default: hipLaunchKernelGGL(residual_kernel<4>, dim3(nb), dim3(kResThreads), 0, s, a); break;
+        case 1: LSS_RES(1); break;
+        case 2: LSS_RES(2); break;
+        default: LSS_RES(4); break;
     }
+#undef LSS_RES
     return hipGetLastError();
 }
 
@@ -1765,12 +1778,13 @@ __global__ __launch_bounds__(kPT) void pursuit_persistent_group_kernel(const Gro
     }
 }
 
-template <int NVC>   // for the widest member: NVC*256 >= C of every member
-__global__ __launch_bounds__(kResThreads) void residual_group_kernel(const GroupMember* __restrict__ d, GroupSpan sp) {
+template <int NVC>   // for the widest member: NVC*256 >= C of every member (short windows)
+__global__ __launch_bounds__(kResThreadsShort) void residual_group_kernel(const GroupMember* __restrict__ d,
+                                                                          GroupSpan sp) {
     int g0, g1, x;
     const int m = group_member(sp, &g0, &g1, &x);
     const PursuitArgs a = d[m].a;
-    residual_body<NVC>(a, (int)blockIdx.x - g0);
+    residual_body<NVC, kResThreadsShort>(a, (int)blockIdx.x - g0);
 }
 
 __global__ __launch_bounds__(256) void features_group_kernel(const GroupMember* __restrict__ d, GroupSpan sp) {
@@ -1866,9 +1880,9 @@ hipError_t launch_group_residual(const GroupMember* d, const GroupMember* m, int
     }
     const dim3 grid(sp.first[n]);
     switch ((cmax + 255) / 256) {
-        case 1: hipLaunchKernelGGL(residual_group_kernel<1>, grid, dim3(kResThreads), 0, s, d, sp); break;
-        case 2: hipLaunchKernelGGL(residual_group_kernel<2>, grid, dim3(kResThreads), 0, s, d, sp); break;
-        default: hipLaunchKernelGGL(residual_group_kernel<4>, grid, dim3(kResThreads), 0, s, d, sp); break;
+        case 1: hipLaunchKernelGGL(residual_group_kernel<1>, grid, dim3(kResThreadsShort), 0, s, d, sp); break;
+        case 2: hipLaunchKernelGGL(residual_group_kernel<2>, grid, dim3(kResThreadsShort), 0, s, d, sp); break;
+        default: hipLaunchKernelGGL(residual_group_kernel<4>, grid, dim3(kResThreadsShort), 0, s, d, sp); break;
     }
     return hipGetLastError();
 }
