@@ -1617,11 +1617,17 @@ __device__ __forceinline__ void features_body(const FeatureArgs& a, int g, int G
             const int t = a.sp_tmp[s];
             const float am = a.sp_amp[s];
             const int iu = a.iU[t];
-            for (int e = lane; e < nt * nC; e += 64) {
-                const int m = e / nC, j = e % nC;
-                const int64_t ww = (int64_t)w - h + m;
-                const int ch = a.iCC[j * a.C + iu];
-                xwin[e] = (ww >= 0 && ww < a.W) ? a.X[ww * a.ldC + ch] : 0.f;
+            // lane (r, j): channel j of rows r, r + rpi, ... (rpi = 64 / nC rows per round); its
+            // channel index is loaded once, every row load is independent of the others
+            {
+                const int rpi = 64 / nC, r = lane / nC, j = lane - r * nC;
+                if (r < rpi) {
+                    const int ch = a.iCC[j * a.C + iu];
+                    for (int m = r; m < nt; m += rpi) {
+                        const int64_t ww = (int64_t)w - h + m;
+                        xwin[m * nC + j] = (ww >= 0 && ww < a.W) ? a.X[ww * a.ldC + ch] : 0.f;
+                    }
+                }
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
