@@ -959,7 +959,7 @@ lss_status probe_C(Probe& p) {
 }
 
 // The pursuit tail of a grouped step on the main stream, one launch per stage for all probes:
-// counter resets, persistent pursuits, residuals, features, readbacks.
+// counter resets, persistent pursuits, residuals, features with the readbacks.
 lss_status segment_tail(Batcher& b) {
     const int n = (int)b.pr.size();
     const lss::GroupMember* d = b.profile ? b.gprof[b.swapped].p : b.gdev[b.swapped].p;
@@ -967,8 +967,7 @@ lss_status segment_tail(Batcher& b) {
     HIP_TRY(lss::launch_group_reset(d, n, b.main));
     HIP_TRY(lss::launch_group_persistent(d, m, b.ggrid, b.gmp, n, b.main));
     HIP_TRY(lss::launch_group_residual(d, m, n, b.main));
-    HIP_TRY(lss::launch_group_features(d, m, n, b.main));
-    HIP_TRY(lss::launch_group_stage(d, n, b.main));
+    HIP_TRY(lss::launch_group_features(d, m, n, b.main));   // also writes the readback staging
     for (int i = 0; i < n; ++i) {
         b.pr[i].h->pers_p0 = 0;
         b.pr[i].h->pers_used = b.ggrid[i];

@@ -210,8 +210,9 @@ hipError_t group_persistent_fits(const GroupMember* m, const int* grids, int n, 
 hipError_t launch_group_persistent(const GroupMember* d, const GroupMember* m, const int* grids,
                                    const int* max_passes, int n, hipStream_t s);
 hipError_t launch_group_residual(const GroupMember* d, const GroupMember* m, int n, hipStream_t s);
+// Features of every member, each also written with its spikes and counters to the member's
+// readback staging (no separate staging launch).
 hipError_t launch_group_features(const GroupMember* d, const GroupMember* m, int n, hipStream_t s);
-hipError_t launch_group_stage(const GroupMember* d, int n, hipStream_t s);
 
 // ---- template refinement (adjoint.hip) --------------------------------------------------------
 // Residual energy of a fixed spike set (sorted by column) and its gradient in U and wPCA.

@@ -1594,8 +1594,10 @@ hipError_t launch_residual(const PursuitArgs& a, hipStream_t s) {
 // per spike.  The spike's nt x nC residual window is gathered into LDS with all loads in flight at
 // once, then lane (j, k) forms sum_m X[w - nt/2 + m][iCC[j, iU[t]]] wPCA[k][m] + amp Ucc[j][t][k].
 // ------------------------------------------------------------------------------------------------
-// Workgroup g of G (grid-stride over the spikes).
-__device__ __forceinline__ void features_body(const FeatureArgs& a, int g, int G) {
+// Workgroup g of G (grid-stride over the spikes).  stg (may be null): readback staging that the
+// spikes below its capacity (column, template, amplitude, features, position) and the counters are
+// also written to, as the staging kernel would write them.
+__device__ __forceinline__ void features_body(const FeatureArgs& a, int g, int G, const StageArgs* stg = nullptr) {
     // wpca [K*nt] | per wavefront: window [nt][nC], features [nC*K], channel mass, x, y [nC]
     extern __shared__ float fsm[];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1609,6 +1611,7 @@ __device__ __forceinline__ void features_body(const FeatureArgs& a, int g, int G
     float* sx = sm + nC;
     float* sy = sx + nC;
     for (int e = threadIdx.x; e < K * nt; e += 256) wp[e] = a.wpca[e];
+    if (stg && g == 0 && threadIdx.x < 8) stg->h_counters[threadIdx.x] = a.counters[threadIdx.x];
     __syncthreads();
     for (int s0 = g * 4; s0 < n; s0 += G * 4) {
         const int s = s0 + wave;
@@ -1617,6 +1620,12 @@ __device__ __forceinline__ void features_body(const FeatureArgs& a, int g, int G
             const int t = a.sp_tmp[s];
             const float am = a.sp_amp[s];
             const int iu = a.iU[t];
+            const bool staged = stg && s < stg->cap;
+            if (staged && lane == 0) {
+                stg->h_col[s] = w;
+                stg->h_tmp[s] = t;
+                stg->h_amp[s] = am;
+            }
             // lane (r, j): channel j of rows r, r + rpi, ... (rpi = 64 / nC rows per round); its
             // channel index is loaded once, every row load is independent of the others
             {
@@ -1639,6 +1648,7 @@ __device__ __forceinline__ void features_body(const FeatureArgs& a, int g, int G
                 const float f = acc + am * a.Ucc[((int64_t)j * a.T + t) * K + k];
                 sf[p] = f;
                 a.tF[(int64_t)s * nC * K + p] = f;
+                if (staged) stg->h_tF[(int64_t)s * nC * K + p] = f;
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -1666,6 +1676,10 @@ __device__ __forceinline__ void features_body(const FeatureArgs& a, int g, int G
                 }
                 a.xs[s] = xs;
                 a.ys[s] = ys;
+                if (staged) {
+                    stg->h_xs[s] = xs;
+                    stg->h_ys[s] = ys;
+                }
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
@@ -1797,14 +1811,8 @@ __global__ __launch_bounds__(256) void features_group_kernel(const GroupMember* 
     int g0, g1, x;
     const int m = group_member(sp, &g0, &g1, &x);
     const FeatureArgs f = d[m].f;
-    features_body(f, (int)blockIdx.x - g0, g1 - g0);
-}
-
-__global__ __launch_bounds__(256) void stage_group_kernel(const GroupMember* __restrict__ d, GroupSpan sp) {
-    int g0, g1, x;
-    const int m = group_member(sp, &g0, &g1, &x);
-    const StageArgs st = d[m].s;
-    stage_body(st, (int)blockIdx.x - g0, g1 - g0);
+    const StageArgs stg = d[m].s;
+    features_body(f, (int)blockIdx.x - g0, g1 - g0, &stg);
 }
 
 static GroupSpan even_span(int n, int each) {
@@ -1908,12 +1916,6 @@ hipError_t launch_group_features(const GroupMember* d, const GroupMember* m, int
     }
     hipLaunchKernelGGL(features_group_kernel, dim3(n * kGroupFeatureGrid), dim3(256), lds, s, d,
                        even_span(n, kGroupFeatureGrid));
-    return hipGetLastError();
-}
-
-hipError_t launch_group_stage(const GroupMember* d, int n, hipStream_t s) {
-    if (!d || n < 1 || n > kGroupMax) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(stage_group_kernel, dim3(n * 64), dim3(256), 0, s, d, even_span(n, 64));
     return hipGetLastError();
 }
 
