@@ -322,6 +322,7 @@ lss_status lss_stream_grouped(void* stream, int32_t* grouped);
  * passes run and the workgroups the probe took.  Results are the same.  LSS_ERR_STATE when
  * measurement is off or the steps do not run grouped. */
 lss_status lss_stream_profile(void* stream, int32_t on);
+lss_status lss_stream_last_profile(void* stream, double* gpu_us, int32_t* spikes, int32_t* passes, int32_t* grid);
 /* Pipelined steps (on by default; grouped steps running ahead, not in measurement mode): a step that
  * preprocessed its next batch also launches that batch's pursuit before it returns, so the device
  * works on it while the caller handles this step's spikes; the next step then only waits for it.
@@ -331,7 +332,6 @@ lss_status lss_stream_profile(void* stream, int32_t on);
  * early.  lss_stream_pipelined counts the early launches.  No counterpart in the reference. */
 lss_status lss_stream_set_pipelined(void* stream, int32_t on);
 lss_status lss_stream_pipelined(void* stream, int64_t* steps);
-lss_status lss_stream_last_profile(void* stream, double* gpu_us, int32_t* spikes, int32_t* passes, int32_t* grid);
 
 /* ---- template refinement (no reference counterpart) ---------------------------------------------- */
 
