@@ -1715,7 +1715,7 @@ lss_status lss_stage_highpass(void* handle, const float* d_in_cw, int64_t W, flo
 }
 
 // user [W][cols] <-> internal [W][ld] (device to device)
-lss_status copy_rows(Handle& h, float* dst, int64_t ldd, const float* src, int64_t lds, int64_t cols, int64_t W) {
+static lss_status copy_rows(Handle& h, float* dst, int64_t ldd, const float* src, int64_t lds, int64_t cols, int64_t W) {
     HIP_TRY(hipMemcpy2DAsync(dst, sizeof(float) * ldd, src, sizeof(float) * lds, sizeof(float) * cols, W,
                              hipMemcpyDeviceToDevice, h.stream));
     return LSS_OK;
@@ -1761,7 +1761,7 @@ lss_status lss_stage_pursuit(void* handle, float* d_b_wt, float* d_x_wc, int64_t
 }
 
 // ---- template refinement ------------------------------------------------------------------------
-lss_status refine_alloc(Handle& h) {
+static lss_status refine_alloc(Handle& h) {
     if (h.rf_R.p) return LSS_OK;
     HIP_TRY(h.rf_R.alloc((size_t)h.Wmax * h.ldC));
     HIP_TRY(h.rf_waves.alloc((size_t)h.T * h.nt * h.ldC));
@@ -1775,7 +1775,7 @@ lss_status refine_alloc(Handle& h) {
 }
 
 // everything lss_refine_templates needs beyond refine_alloc (done once per handle)
-lss_status refine_templates_alloc(Handle& h) {
+static lss_status refine_templates_alloc(Handle& h) {
     if (lss_status st = refine_alloc(h)) return st;
     if (h.rf_U.p) return LSS_OK;
     const size_t nU = (size_t)h.T * h.K * h.C, nW = (size_t)h.K * h.nt;
@@ -1796,7 +1796,7 @@ lss_status refine_templates_alloc(Handle& h) {
 const char* const kRefineBusy =
     "a streaming batcher is refining this handle's templates in the background (until the step that adopts it)";
 
-lss_status refine_run(Handle& h, const float* d_x, int64_t W, const int32_t* d_col, const int32_t* d_tmp,
+static lss_status refine_run(Handle& h, const float* d_x, int64_t W, const int32_t* d_col, const int32_t* d_tmp,
                       const float* d_amp, int64_t n, const float* d_U, const float* d_wpca, double* loss, float* d_gU,
                       float* d_gW) {
     if (!d_x || !d_U || !d_wpca || !loss || (n > 0 && (!d_col || !d_tmp || !d_amp)))
@@ -1846,7 +1846,7 @@ lss_status lss_residual_energy_grad(void* handle, const float* d_x, int64_t W, c
 
 // Loss of the last window's spike set for templates (U1 [T][K][C], wpca1) given the window's
 // input x (rf_R holds it): sum (x - sum_s amp_s wave1_s)^2.  Spikes must be in rf_col/tmp/amp.
-lss_status refine_loss_at(Handle& h, hipStream_t s, int64_t W, const float* U1, const float* W1, const float* x,
+static lss_status refine_loss_at(Handle& h, hipStream_t s, int64_t W, const float* U1, const float* W1, const float* x,
                           int n, double* loss) {
     lss::RefineArgs a{};
     a.X = x; a.ldx = h.ldC; a.vec_x = 1;
@@ -1867,13 +1867,13 @@ struct TemplateSet {
     float *U, *wpca, *nm, *waves, *ctc, *Ucc;
     int* iU;
 };
-TemplateSet live_set(Handle& h) { return {h.U.p, h.wpca.p, h.nm.p, h.waves.p, h.ctc.p, h.Ucc.p, h.iU.p}; }
+static TemplateSet live_set(Handle& h) { return {h.U.p, h.wpca.p, h.nm.p, h.waves.p, h.ctc.p, h.Ucc.p, h.iU.p}; }
 
 // One refinement step on stream s from the residual x [W][ldC] that the pursuit left and its n
 // spikes (device col/tmp/amp, pursuit order), starting from the handle's templates; on descent the
 // new U / wPCA and every table derived from them go to `out` (*adopted = true).  info as
 // lss_refine_templates.  Uses the handle's refinement buffers; the handle's templates are only read.
-lss_status refine_core(Handle& h, hipStream_t s, const float* x, int64_t W, const int32_t* d_col,
+static lss_status refine_core(Handle& h, hipStream_t s, const float* x, int64_t W, const int32_t* d_col,
                        const int32_t* d_tmp, const float* d_amp, int n, const TemplateSet& out, double* info,
                        bool* adopted) {
     *adopted = false;
@@ -2307,7 +2307,7 @@ lss_status lss_stream_push_device(void* stream, int32_t probe, const int16_t* d_
 }
 
 // The window sets preprocessed ahead become the current ones (and the current ones the second).
-void swap_sets(Batcher& b) {
+static void swap_sets(Batcher& b) {
     for (auto& p : b.pr) {
         Handle& h = *p.h;
         std::swap(h.win.p, h.win_alt.p);
